@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X rendering hot path (one GPU).
+
+Workload: the ~1.0M-triangle scene (48 subdivision-5 icospheres + ground quad
++ 2 x 2 m area light + constant environment, Principled materials),
+1920 x 1080, max depth 16, Russian roulette from depth 3, MIS on; rendered by
+the plain megakernel.  One step = one sample per pixel over the whole image
+(2,073,600 paths).  `value` = Msamples/s over the K timed steps, with the
+scene and film already resident in HBM.
+
+Also reported: Grays/s of the render (extension + shadow + MIS rays counted by
+the kernel), the closest-hit throughput on a 16,777,216-ray seeded batch
+(8M primary + 8M secondary, device resident), the roofline of the dominant
+kernel from its HIP-event-timed launches, and the CPU oracle timed on the
+host cores on a bounded sample (one spp pass of the same image).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+B_RAY_FIXED = 32 + 16      # ray in + hit out
+B_NODE = 64                # one internal visit: both child boxes + child words
+B_TRI = 52                 # 48 B index + vertices, 4 B primitive map
+B_FILM = 16 * 9            # (3 + 1) floats x up to 9 pixels
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--trace-rays", type=int, default=1 << 24)
+    ap.add_argument("--trace-reps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-trace", action="store_true", help="skip the closest-hit batch")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not "
+                                   "part of it (use --gpus 1)"}), flush=True)
+        sys.exit(2)
+
+    import craystal_amd as cr
+    from craystal_amd import _lib as L
+    from craystal_amd import scenes
+
+    cr.load()
+    if cr.device_count() < 1:
+        raise SystemExit("no GPU visible to libcrayhip.so")
+
+    data, camd = scenes.spheres_scene()
+    W, H = camd["size"]
+    t0 = time.perf_counter()
+    scene = cr.Scene(data)
+    build_s = time.perf_counter() - t0
+    bst = scene.bvh_stats()
+    sensor = cr.Sensor((W, H))
+    cam = scenes.make_camera(camd, sensor)
+    scene.setCamera(cam)
+    cfg = cr.Configs(maxDepth=camd["max_depth"], rrThreshold=0.05, useMIS=True,
+                     rrStartDepth=camd["rr_start_depth"])
+    integ = cr.PathTraceIntegrator(cfg)
+
+    # warmup (untimed)
+    if a.warmup > 0:
+        integ.dispatch(scene, a.warmup, first_sample=0, readback=False)
+    # work counters for the roofline, outside the timed region
+    cnt_cfg = cr.Configs(**{**cfg.__dict__})
+    cc = cnt_cfg.to_c()
+    cc.count_work = 1
+    import ctypes as C
+    st = L.crh_stats()
+    L.call("crh_render", scene.handle, C.byref(cam.data), C.byref(cc), 100000, 1,
+           sensor.handle, C.byref(st))
+    px = W * H
+    rays_ps = st.rays / px
+    nodes_ps = st.node_visits / px
+    tris_ps = st.tri_tests / px
+    b_sample = rays_ps * B_RAY_FIXED + nodes_ps * B_NODE + tris_ps * B_TRI + B_FILM
+    sensor.clear()
+
+    # timed region: K steps = K spp passes of the full image
+    t0 = time.perf_counter()
+    stats = integ.dispatch(scene, a.steps, first_sample=a.warmup, readback=False)
+    elapsed = time.perf_counter() - t0
+    samples = px * a.steps
+    msps = samples / elapsed / 1e6
+    main_ms = stats["kernel_ms"]
+    # dominant kernel: the megakernel launches, timed on their own stream
+    st2 = L.crh_stats()
+    cfg_c = cfg.to_c()
+    L.call("crh_render", scene.handle, C.byref(cam.data), C.byref(cfg_c), a.warmup + a.steps,
+           max(1, min(8, a.steps)), sensor.handle, C.byref(st2))
+    launches = max(1, min(8, a.steps))
+    avg_launch_s = st2.main_kernel_ms / launches / 1e3
+    achieved = b_sample * px / avg_launch_s / 1e9
+
+    out = {
+        "metric": "Msamples/s + time-to-relMSE 0.01, guided vs plain, 1 MI355X; Grays/s",
+        "value": round(msps, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(elapsed / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": "spheres-1M: 48 icospheres s5 + ground + area light + env, "
+                               "983,044 tris, 1920x1080, 1 spp per step, max depth 16, "
+                               "RR from depth 3, MIS, plain megakernel",
+                   "triangles": int(scene.triangle_count), "width": W, "height": H,
+                   "spp_timed": a.steps},
+        "grays_per_s": round(stats["rays"] / elapsed / 1e9, 4),
+        "rays_per_sample": round(rays_ps, 4),
+        "gpu_kernel_ms": round(main_ms, 3),
+        "bvh": {"nodes": bst["node_count"], "max_depth": bst["max_depth"],
+                "max_leaf": bst["max_leaf"], "build_s": round(build_s, 3)},
+        "roofline": {"bound": "hbm", "kernel": "kRenderPass",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "bytes_per_sample": round(b_sample, 1),
+                     "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                     "node_visits_per_sample": round(nodes_ps, 3),
+                     "tri_tests_per_sample": round(tris_ps, 3)},
+    }
+
+    rays = None
+    if not a.no_trace:
+        n = a.trace_rays
+        rays = scenes.mixed_batch(data, camd, n)
+        rb = cr.DeviceBuffer(rays.nbytes)
+        hb = cr.DeviceBuffer(n * 16)
+        rb.copyFromHost(rays)
+        cr.trace_closest_device(scene, rb, n, hb)  # warmup
+        cnt = cr.trace_closest_device(scene, rb, n, hb, count_work=True)
+        times = []
+        for _ in range(a.trace_reps):
+            times.append(cr.trace_closest_device(scene, rb, n, hb)["kernel_ms"])
+        kms = float(np.mean(times))
+        b_ray = B_RAY_FIXED + cnt["node_visits"] / n * B_NODE + cnt["tri_tests"] / n * B_TRI
+        ach = b_ray * n / (kms / 1e3) / 1e9
+        out["closest_hit"] = {
+            "rays": n, "grays_per_s": round(n / (kms / 1e3) / 1e9, 4),
+            "kernel_ms": round(kms, 3), "kernel": "kTraceClosest",
+            "node_visits_per_ray": round(cnt["node_visits"] / n, 3),
+            "tri_tests_per_ray": round(cnt["tri_tests"] / n, 3),
+            "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
+                         "bytes_per_ray": round(b_ray, 1)}}
+        del rb, hb
+
+    if not a.no_cpu:
+        import oracle
+        orc_scene = oracle.Scene(data)
+        threads = int(oracle.lib().orc_max_threads())
+        t0 = time.perf_counter()
+        _, _, crays = orc_scene.render_samples(cam, (W, H), cfg, 0)
+        cs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(px / cs / 1e6, 4), "unit": "Msamples/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"one spp pass of the same 1920x1080 image "
+                                         f"({px} paths, {crays} rays) by the C++ oracle, "
+                                         f"OpenMP over rows",
+                               "seconds": round(cs, 3),
+                               "grays_per_s": round(crays / cs / 1e9, 5)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,60 @@
+"""Build libcrayhip.so (gfx950 HIP kernels + C-ABI) in-tree.
+
+Host C++ (api.cpp, bvh_build.cpp) is compiled with g++ against the HIP
+runtime headers; the kernels with hipcc for gfx950.  Everything is built with
+-ffp-contract=off so that traversal arithmetic is bit-identical to the
+host-side restatement of the reference.  Run as ``python -m craystal_amd.build``.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+OUT = PKG / "libcrayhip.so"
+OBJ = PKG / "_build"
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+ARCH = os.environ.get("CRH_OFFLOAD_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
+    "-ffp-contract=off",
+    # Packed FP32 (v_pk_*_f32) generation off.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+HOST_FLAGS = [
+    "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+    "-Wall", "-Wno-unused-function", f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose: bool = True) -> Path:
+    OBJ.mkdir(exist_ok=True)
+    objs = []
+    for src in ("api.cpp", "bvh_build.cpp"):
+        o = OBJ / (src + ".o")
+        _run(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
+        objs.append(str(o))
+    for src in ("kernels.hip",):
+        o = OBJ / (src + ".o")
+        _run([str(ROCM / "bin" / "hipcc"), *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
+        objs.append(str(o))
+    tmp = OUT.with_suffix(".so.tmp")
+    _run([str(ROCM / "bin" / "hipcc"), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
+          *objs, "-lpthread"])
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

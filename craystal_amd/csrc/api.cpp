@@ -1,0 +1,731 @@
+// C-ABI of libcrayhip.so (include/crayhip.h): contexts, device buffers,
+// BVH build, scene upload, ray queries, film and render dispatch.
+//
+// Replaces the reference's host layer: DeviceBuffer (Core/Buffer.cu:7-63),
+// Scene::Scene (Scene/Scene.cu:61-78), TriangleMeshManager
+// (Scene/Triangle.cu:98-165), MaterialManager (Core/Material.cu:15-30),
+// Sensor (Core/Sensor.cu:45-112) and PathTraceIntegrator::dispatch
+// (Core/Integrator.cu:233-262).  Differences by design: size_t buffers, the
+// integrator configuration actually reaches the device, the film is zeroed
+// on creation, every launch is checked, and spp passes are queued back to
+// back on one stream instead of synchronising after each.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/crayhip.h"
+#include "bvh_build.h"
+#include "kernels.h"
+
+static_assert(sizeof(crh_ray) == 32, "crh_ray layout");
+static_assert(sizeof(crh_hit) == 16, "crh_hit layout");
+static_assert(sizeof(crh_bvh_node_ref) == 36, "crh_bvh_node_ref layout");
+static_assert(sizeof(crh_material) == 68, "crh_material layout");
+static_assert(sizeof(crh_camera) == 96, "crh_camera layout");
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+int hipFail(const char* expr, hipError_t e, const char* file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf),
+                  "HIP call %s failed with error %d (%s).\n\n%s:%d", expr,
+                  int(e), hipGetErrorString(e), file, line);
+    g_err = buf;
+    return e == hipErrorOutOfMemory ? CRH_ERR_OOM : CRH_ERR_HIP;
+}
+
+#define HIP_TRY(expr)                                                   \
+    do {                                                                \
+        hipError_t _e = (expr);                                         \
+        if (_e != hipSuccess) return hipFail(#expr, _e, __FILE__, __LINE__); \
+    } while (0)
+
+#define ARG_CHECK(cond, msg)                                            \
+    do {                                                                \
+        if (!(cond)) return fail(CRH_ERR_INVALID_ARG, msg);             \
+    } while (0)
+
+#define GUARD_BEGIN try {
+#define GUARD_END                                                       \
+    }                                                                   \
+    catch (const std::bad_alloc&) {                                     \
+        return fail(CRH_ERR_OOM, "host allocation failed");             \
+    }                                                                   \
+    catch (const std::exception& ex) {                                  \
+        return fail(CRH_ERR_INTERNAL, ex.what());                       \
+    }
+
+// Owned device allocation.
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    ~DevMem() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        reset();
+        bytes = n;
+        if (n == 0) return hipSuccess;
+        return hipMalloc(&p, n);
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+template <class T>
+hipError_t upload(DevMem& m, const std::vector<T>& v, hipStream_t st) {
+    hipError_t e = m.alloc(v.size() * sizeof(T));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(m.p, v.data(), v.size() * sizeof(T),
+                          hipMemcpyHostToDevice, st);
+}
+
+double nowMs() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace
+
+struct crh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct crh_buffer {
+    crh_context* ctx = nullptr;
+    DevMem mem;
+};
+
+struct crh_bvh {
+    crh::BvhResult r;
+};
+
+struct crh_scene {
+    crh_context* ctx = nullptr;
+    crh::BvhResult bvh;
+    DevMem nodes, tris, pos, nrm, uv, idx, primMat, mats, emissive;
+    crh::SceneView view{};
+};
+
+struct crh_film {
+    crh_context* ctx = nullptr;
+    uint32_t w = 0, h = 0;
+    DevMem sum, wsum, L, P;
+};
+
+extern "C" {
+
+const char* crh_last_error(void) { return g_err.c_str(); }
+
+int crh_version(uint32_t* version) {
+    ARG_CHECK(version, "version is NULL");
+    *version = 1u;
+    return CRH_OK;
+}
+
+int crh_device_count(int* count) {
+    ARG_CHECK(count, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return CRH_OK;
+}
+
+// ---- context ---------------------------------------------------------------
+int crh_context_create(int device, crh_context** out) {
+    ARG_CHECK(out, "out is NULL");
+    *out = nullptr;
+    GUARD_BEGIN
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    ARG_CHECK(device >= 0 && device < n, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    auto* c = new crh_context();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        crh_context_destroy(c);
+        return hipFail("context setup", e, __FILE__, __LINE__);
+    }
+    *out = c;
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_context_destroy(crh_context* ctx) {
+    if (!ctx) return CRH_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CRH_OK;
+}
+
+// ---- buffers ---------------------------------------------------------------
+int crh_buffer_create(crh_context* ctx, size_t bytes, crh_buffer** out) {
+    ARG_CHECK(ctx && out, "NULL argument");
+    *out = nullptr;
+    GUARD_BEGIN
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto* b = new crh_buffer();
+    b->ctx = ctx;
+    hipError_t e = b->mem.alloc(bytes);
+    if (e != hipSuccess) {
+        delete b;
+        return hipFail("hipMalloc", e, __FILE__, __LINE__);
+    }
+    *out = b;
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_buffer_upload(crh_buffer* buf, const void* src, size_t bytes) {
+    ARG_CHECK(buf && (src || bytes == 0), "NULL argument");
+    ARG_CHECK(bytes <= buf->mem.bytes, "upload larger than the buffer");
+    if (bytes == 0) return CRH_OK;
+    HIP_TRY(hipSetDevice(buf->ctx->device));
+    HIP_TRY(hipMemcpyAsync(buf->mem.p, src, bytes, hipMemcpyHostToDevice,
+                           buf->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(buf->ctx->stream));
+    return CRH_OK;
+}
+
+int crh_buffer_download(const crh_buffer* buf, void* dst, size_t bytes) {
+    ARG_CHECK(buf && (dst || bytes == 0), "NULL argument");
+    ARG_CHECK(bytes <= buf->mem.bytes, "download larger than the buffer");
+    if (bytes == 0) return CRH_OK;
+    HIP_TRY(hipSetDevice(buf->ctx->device));
+    HIP_TRY(hipMemcpyAsync(dst, buf->mem.p, bytes, hipMemcpyDeviceToHost,
+                           buf->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(buf->ctx->stream));
+    return CRH_OK;
+}
+
+int crh_buffer_memset(crh_buffer* buf, int value) {
+    ARG_CHECK(buf, "NULL argument");
+    if (buf->mem.bytes == 0) return CRH_OK;
+    HIP_TRY(hipSetDevice(buf->ctx->device));
+    HIP_TRY(hipMemsetAsync(buf->mem.p, value, buf->mem.bytes, buf->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(buf->ctx->stream));
+    return CRH_OK;
+}
+
+int crh_buffer_size(const crh_buffer* buf, size_t* bytes) {
+    ARG_CHECK(buf && bytes, "NULL argument");
+    *bytes = buf->mem.bytes;
+    return CRH_OK;
+}
+
+int crh_buffer_destroy(crh_buffer* buf) {
+    if (!buf) return CRH_OK;
+    (void)hipSetDevice(buf->ctx->device);
+    delete buf;
+    return CRH_OK;
+}
+
+// ---- BVH ---------------------------------------------------------------------
+static void fillStats(const crh::BvhResult& r, crh_bvh_stats* st) {
+    std::memset(st, 0, sizeof(*st));
+    st->node_count = uint32_t(r.nodes.size());
+    st->max_depth = r.maxDepth;
+    st->max_leaf = r.maxLeaf;
+    st->root = r.root;
+    st->prim_count = uint32_t(r.primOrder.size());
+    st->internal_count = r.internalCount;
+    for (int k = 0; k < 3; ++k) {
+        st->root_min[k] = r.rootMin[k];
+        st->root_max[k] = r.rootMax[k];
+    }
+    st->build_ms = r.buildMs;
+}
+
+static void exportBvh(const crh::BvhResult& r, crh_bvh_node_ref* nodes,
+                      uint32_t* prims) {
+    if (nodes) std::memcpy(nodes, r.nodes.data(), r.nodes.size() * sizeof(*nodes));
+    if (prims)
+        std::memcpy(prims, r.primOrder.data(), r.primOrder.size() * sizeof(*prims));
+}
+
+int crh_bvh_build(const float* pos, uint32_t nvert, const uint32_t* idx,
+                  uint32_t ntri, const crh_bvh_params* params, crh_bvh** out) {
+    ARG_CHECK(out, "out is NULL");
+    *out = nullptr;
+    ARG_CHECK((pos || nvert == 0) && (idx || ntri == 0), "NULL geometry");
+    GUARD_BEGIN
+    for (size_t i = 0; i < size_t(ntri) * 3; ++i)
+        ARG_CHECK(idx[i] < nvert, "vertex index out of range");
+    crh::BuildParams bp = crh::resolveParams(params);
+    ARG_CHECK(bp.bins >= 2 && bp.bins <= 1024, "bins must be in [2, 1024]");
+    auto* b = new crh_bvh();
+    b->r = crh::buildBvh(pos, nvert, idx, ntri, bp);
+    *out = b;
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_bvh_get_stats(const crh_bvh* bvh, crh_bvh_stats* stats) {
+    ARG_CHECK(bvh && stats, "NULL argument");
+    fillStats(bvh->r, stats);
+    return CRH_OK;
+}
+
+int crh_bvh_export(const crh_bvh* bvh, crh_bvh_node_ref* nodes,
+                   uint32_t* prim_order) {
+    ARG_CHECK(bvh, "NULL argument");
+    exportBvh(bvh->r, nodes, prim_order);
+    return CRH_OK;
+}
+
+int crh_bvh_destroy(crh_bvh* bvh) {
+    delete bvh;
+    return CRH_OK;
+}
+
+// ---- materials ---------------------------------------------------------------
+int crh_material_finalize(crh_material* m) {
+    ARG_CHECK(m, "NULL argument");
+    auto maxOf = [](const float* v) {
+        float r = v[0];
+        for (int i = 1; i < 3; ++i) r = (r < v[i]) ? v[i] : r;
+        return r;
+    };
+    if (maxOf(m->emission) != 0.0f) m->flags |= CRH_MATF_EMISSIVE;
+    if (maxOf(m->specular) == 0.0f) m->flags |= CRH_MATF_DIFFUSE;
+    return CRH_OK;
+}
+
+// ---- scene ---------------------------------------------------------------------
+int crh_scene_create(crh_context* ctx, const crh_scene_desc* d,
+                     crh_scene** out) {
+    ARG_CHECK(ctx && d && out, "NULL argument");
+    *out = nullptr;
+    ARG_CHECK(d->nmesh == 0 || d->meshes, "meshes is NULL");
+    ARG_CHECK(d->nmaterial == 0 || d->materials, "materials is NULL");
+    ARG_CHECK(d->nemissive == 0 || d->emissive_index, "emissive_index is NULL");
+    GUARD_BEGIN
+    // Concatenate meshes (TriangleMeshManager, Scene/Triangle.cu:98-165).
+    std::vector<float> pos, nrm, uv;
+    std::vector<uint32_t> idx, primMat;
+    for (uint32_t m = 0; m < d->nmesh; ++m) {
+        const crh_mesh& mesh = d->meshes[m];
+        ARG_CHECK(mesh.nindex % 3 == 0, "mesh index count is not a multiple of 3");
+        ARG_CHECK(mesh.nvert == 0 || (mesh.position && mesh.normal),
+                  "mesh position/normal is NULL");
+        ARG_CHECK(mesh.nindex == 0 || mesh.index, "mesh index is NULL");
+        ARG_CHECK(mesh.material_id < d->nmaterial, "material_id out of range");
+        uint32_t voff = uint32_t(pos.size() / 3);
+        pos.insert(pos.end(), mesh.position, mesh.position + 3 * size_t(mesh.nvert));
+        nrm.insert(nrm.end(), mesh.normal, mesh.normal + 3 * size_t(mesh.nvert));
+        if (mesh.texcrd)
+            uv.insert(uv.end(), mesh.texcrd, mesh.texcrd + 2 * size_t(mesh.nvert));
+        else
+            uv.insert(uv.end(), 2 * size_t(mesh.nvert), 0.0f);
+        for (uint32_t i = 0; i < mesh.nindex; ++i) {
+            ARG_CHECK(mesh.index[i] < mesh.nvert, "vertex index out of range");
+            idx.push_back(mesh.index[i] + voff);
+        }
+        primMat.insert(primMat.end(), mesh.nindex / 3, mesh.material_id);
+    }
+    const uint32_t ntri = uint32_t(idx.size() / 3);
+    for (uint32_t i = 0; i < d->nemissive; ++i)
+        ARG_CHECK(d->emissive_index[i] < ntri, "emissive index out of range");
+    crh::BuildParams bp = crh::resolveParams(&d->bvh);
+    ARG_CHECK(bp.bins >= 2 && bp.bins <= 1024, "bins must be in [2, 1024]");
+
+    auto* s = new crh_scene();
+    s->ctx = ctx;
+    s->bvh = crh::buildBvh(pos.data(), uint32_t(pos.size() / 3), idx.data(),
+                           ntri, bp);
+    crh::GpuBvh g = crh::toGpuLayout(s->bvh, pos.data(), idx.data());
+    std::vector<crh::GMaterial> gm(d->nmaterial);
+    for (uint32_t i = 0; i < d->nmaterial; ++i) {
+        const crh_material& m = d->materials[i];
+        crh::GMaterial& o = gm[i];
+        for (int k = 0; k < 3; ++k) {
+            o.kd[k] = m.diffuse[k];
+            o.ks[k] = m.specular[k];
+            o.emission[k] = m.emission[k];
+        }
+        o.type = m.type;
+        o.flags = m.flags;
+        o.shininess = m.shininess;
+    }
+    std::vector<uint32_t> em(d->emissive_index, d->emissive_index + d->nemissive);
+
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = upload(s->nodes, g.nodes, st);
+    if (e == hipSuccess) e = upload(s->tris, g.tris, st);
+    if (e == hipSuccess) e = upload(s->pos, pos, st);
+    if (e == hipSuccess) e = upload(s->nrm, nrm, st);
+    if (e == hipSuccess) e = upload(s->uv, uv, st);
+    if (e == hipSuccess) e = upload(s->idx, idx, st);
+    if (e == hipSuccess) e = upload(s->primMat, primMat, st);
+    if (e == hipSuccess) e = upload(s->mats, gm, st);
+    if (e == hipSuccess) e = upload(s->emissive, em, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        delete s;
+        return hipFail("scene upload", e, __FILE__, __LINE__);
+    }
+    crh::SceneView& v = s->view;
+    v.nodes = s->nodes.as<crh::GNode>();
+    v.tris = s->tris.as<crh::GTri>();
+    v.rootRef = g.rootRef;
+    v.hasGeom = g.hasGeom;
+    v.stackCap = s->bvh.levels + 1;
+    v.nEmissive = d->nemissive;
+    for (int k = 0; k < 3; ++k) {
+        v.rootMin[k] = s->bvh.rootMin[k];
+        v.rootMax[k] = s->bvh.rootMax[k];
+        v.env[k] = d->env_radiance[k];
+    }
+    v.pos = s->pos.as<float>();
+    v.nrm = s->nrm.as<float>();
+    v.uv = s->uv.as<float>();
+    v.idx = s->idx.as<uint32_t>();
+    v.primMat = s->primMat.as<uint32_t>();
+    v.mats = s->mats.as<crh::GMaterial>();
+    v.emissive = s->emissive.as<uint32_t>();
+    *out = s;
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_scene_get_bvh_stats(const crh_scene* s, crh_bvh_stats* st) {
+    ARG_CHECK(s && st, "NULL argument");
+    fillStats(s->bvh, st);
+    return CRH_OK;
+}
+
+int crh_scene_export_bvh(const crh_scene* s, crh_bvh_node_ref* nodes,
+                         uint32_t* prims) {
+    ARG_CHECK(s, "NULL argument");
+    exportBvh(s->bvh, nodes, prims);
+    return CRH_OK;
+}
+
+int crh_scene_destroy(crh_scene* s) {
+    if (!s) return CRH_OK;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+    return CRH_OK;
+}
+
+// ---- camera --------------------------------------------------------------------
+int crh_camera_update(crh_camera* c) {
+    ARG_CHECK(c, "NULL argument");
+    auto dot = [](const float* a, const float* b) {
+        float x = a[0] * b[0], y = a[1] * b[1], z = a[2] * b[2];
+        return (x + y) + z;
+    };
+    auto norm = [&](float* v) {
+        float s = 1.0f / std::sqrt(dot(v, v));
+        for (int k = 0; k < 3; ++k) v[k] = v[k] * s;
+    };
+    auto cross = [](const float* a, const float* b, float* o) {
+        o[0] = a[1] * b[2] - b[1] * a[2];
+        o[1] = a[2] * b[0] - b[2] * a[0];
+        o[2] = a[0] * b[1] - b[0] * a[1];
+    };
+    float W[3], U[3], V[3];
+    for (int k = 0; k < 3; ++k) W[k] = c->target[k] - c->pos[k];
+    norm(W);
+    cross(W, c->up, U);
+    norm(U);
+    cross(U, W, V);
+    norm(V);
+    for (int k = 0; k < 3; ++k) {
+        c->w[k] = W[k];
+        c->u[k] = U[k];
+        c->v[k] = V[k];
+    }
+    return CRH_OK;
+}
+
+// ---- ray queries ---------------------------------------------------------------
+static int traceClosestDev(crh_scene* s, const crh_ray* rays, uint64_t n,
+                           crh_hit* hits, uint32_t* work, crh_stats* stats) {
+    crh_context* ctx = s->ctx;
+    double t0 = nowMs();
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(crh::launchTraceClosest(s->view, rays, n, hits, work, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    if (stats) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        stats->kernel_ms = ms;
+        stats->main_kernel_ms = ms;
+        stats->total_ms = nowMs() - t0;
+        stats->rays = n;
+        stats->launches = 1;
+    }
+    return CRH_OK;
+}
+
+int crh_trace_closest_device(crh_scene* s, const crh_buffer* rays, uint64_t n,
+                             crh_buffer* hits, int count_work,
+                             crh_stats* stats) {
+    ARG_CHECK(s && rays && hits, "NULL argument");
+    ARG_CHECK(rays->mem.bytes >= n * sizeof(crh_ray), "ray buffer too small");
+    ARG_CHECK(hits->mem.bytes >= n * sizeof(crh_hit), "hit buffer too small");
+    ARG_CHECK(n < (uint64_t(1) << 40), "ray count too large");
+    GUARD_BEGIN
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (!count_work)
+        return traceClosestDev(s, rays->mem.as<crh_ray>(), n,
+                               hits->mem.as<crh_hit>(), nullptr, stats);
+    DevMem work;
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(work.alloc(n * 2 * sizeof(uint32_t)));
+    int rc = traceClosestDev(s, rays->mem.as<crh_ray>(), n,
+                             hits->mem.as<crh_hit>(), work.as<uint32_t>(), stats);
+    if (rc != CRH_OK) return rc;
+    std::vector<uint32_t> w(n * 2);
+    HIP_TRY(hipMemcpy(w.data(), work.p, w.size() * 4, hipMemcpyDeviceToHost));
+    uint64_t nodes = 0, tris = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        nodes += w[2 * i];
+        tris += w[2 * i + 1];
+    }
+    if (stats) {
+        stats->node_visits = nodes;
+        stats->tri_tests = tris;
+    }
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_trace_any_device(crh_scene* s, const crh_buffer* rays, uint64_t n,
+                         crh_buffer* occ, crh_stats* stats) {
+    ARG_CHECK(s && rays && occ, "NULL argument");
+    ARG_CHECK(rays->mem.bytes >= n * sizeof(crh_ray), "ray buffer too small");
+    ARG_CHECK(occ->mem.bytes >= n, "occlusion buffer too small");
+    GUARD_BEGIN
+    crh_context* ctx = s->ctx;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    double t0 = nowMs();
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(crh::launchTraceAny(s->view, rays->mem.as<crh_ray>(), n,
+                                occ->mem.as<uint8_t>(), ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    if (stats) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        stats->kernel_ms = ms;
+        stats->main_kernel_ms = ms;
+        stats->total_ms = nowMs() - t0;
+        stats->rays = n;
+        stats->launches = 1;
+    }
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_trace_closest(crh_scene* s, const crh_ray* rays, uint64_t n,
+                      crh_hit* hits) {
+    ARG_CHECK(s && (rays || n == 0) && (hits || n == 0), "NULL argument");
+    if (n == 0) return CRH_OK;
+    GUARD_BEGIN
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    DevMem dr, dh;
+    HIP_TRY(dr.alloc(n * sizeof(crh_ray)));
+    HIP_TRY(dh.alloc(n * sizeof(crh_hit)));
+    HIP_TRY(hipMemcpy(dr.p, rays, n * sizeof(crh_ray), hipMemcpyHostToDevice));
+    int rc = traceClosestDev(s, dr.as<crh_ray>(), n, dh.as<crh_hit>(), nullptr,
+                             nullptr);
+    if (rc != CRH_OK) return rc;
+    HIP_TRY(hipMemcpy(hits, dh.p, n * sizeof(crh_hit), hipMemcpyDeviceToHost));
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_trace_any(crh_scene* s, const crh_ray* rays, uint64_t n,
+                  uint8_t* occluded) {
+    ARG_CHECK(s && (rays || n == 0) && (occluded || n == 0), "NULL argument");
+    if (n == 0) return CRH_OK;
+    GUARD_BEGIN
+    crh_context* ctx = s->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevMem dr, dox;
+    HIP_TRY(dr.alloc(n * sizeof(crh_ray)));
+    HIP_TRY(dox.alloc(n));
+    HIP_TRY(hipMemcpy(dr.p, rays, n * sizeof(crh_ray), hipMemcpyHostToDevice));
+    HIP_TRY(crh::launchTraceAny(s->view, dr.as<crh_ray>(), n, dox.as<uint8_t>(),
+                                ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(occluded, dox.p, n, hipMemcpyDeviceToHost));
+    return CRH_OK;
+    GUARD_END
+}
+
+// ---- film ---------------------------------------------------------------------
+int crh_film_create(crh_context* ctx, uint32_t w, uint32_t h, crh_film** out) {
+    ARG_CHECK(ctx && out, "NULL argument");
+    *out = nullptr;
+    ARG_CHECK(w > 0 && h > 0, "film size must be positive");
+    ARG_CHECK(uint64_t(w) * h <= (uint64_t(1) << 31), "film too large");
+    GUARD_BEGIN
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto* f = new crh_film();
+    f->ctx = ctx;
+    f->w = w;
+    f->h = h;
+    size_t n = size_t(w) * h;
+    hipError_t e = f->sum.alloc(n * 3 * sizeof(float));
+    if (e == hipSuccess) e = f->wsum.alloc(n * sizeof(float));
+    if (e == hipSuccess) e = f->L.alloc(n * sizeof(float4));
+    if (e == hipSuccess) e = f->P.alloc(n * sizeof(float2));
+    if (e != hipSuccess) {
+        delete f;
+        return hipFail("film allocation", e, __FILE__, __LINE__);
+    }
+    *out = f;
+    return crh_film_clear(f);
+    GUARD_END
+}
+
+int crh_film_clear(crh_film* f) {
+    ARG_CHECK(f, "NULL argument");
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    HIP_TRY(hipMemsetAsync(f->sum.p, 0, f->sum.bytes, f->ctx->stream));
+    HIP_TRY(hipMemsetAsync(f->wsum.p, 0, f->wsum.bytes, f->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(f->ctx->stream));
+    return CRH_OK;
+}
+
+int crh_film_readback(const crh_film* f, float* sum, float* weight) {
+    ARG_CHECK(f, "NULL argument");
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    if (sum)
+        HIP_TRY(hipMemcpy(sum, f->sum.p, f->sum.bytes, hipMemcpyDeviceToHost));
+    if (weight)
+        HIP_TRY(hipMemcpy(weight, f->wsum.p, f->wsum.bytes, hipMemcpyDeviceToHost));
+    return CRH_OK;
+}
+
+int crh_film_destroy(crh_film* f) {
+    if (!f) return CRH_OK;
+    (void)hipSetDevice(f->ctx->device);
+    (void)hipStreamSynchronize(f->ctx->stream);
+    delete f;
+    return CRH_OK;
+}
+
+// ---- render --------------------------------------------------------------------
+int crh_render(crh_scene* s, const crh_camera* cam,
+               const crh_integrator_cfg* cfg, uint32_t first_sample,
+               uint32_t spp, crh_film* film, crh_stats* stats) {
+    ARG_CHECK(s && cam && cfg && film, "NULL argument");
+    ARG_CHECK(film->ctx == s->ctx, "film and scene belong to different contexts");
+    ARG_CHECK(cfg->mode == CRH_MODE_MEGAKERNEL, "only the megakernel mode is available");
+    ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
+    GUARD_BEGIN
+    crh_context* ctx = s->ctx;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    crh::CameraView cv;
+    for (int k = 0; k < 3; ++k) {
+        cv.pos[k] = cam->pos[k];
+        cv.u[k] = cam->u[k];
+        cv.v[k] = cam->v[k];
+        cv.w[k] = cam->w[k];
+    }
+    // Core/CameraProxy.h:31-34: aspect in float, tan(radians(fovY * 0.5)) in
+    // double (fovY * 0.5 promotes), then narrowed.
+    cv.hh = float(std::tan(double(cam->fov_y) * 0.5 *
+                           0.01745329251994329576923690768489));
+    cv.aspect = float(film->w) / float(film->h);
+    cv.width = film->w;
+    cv.height = film->h;
+    crh::IntegratorView iv{cfg->max_depth, cfg->rr_threshold,
+                           cfg->rr_start_depth, cfg->use_mis ? 1u : 0u};
+    double t0 = nowMs();
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevMem counter;
+    HIP_TRY(counter.alloc(3 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(counter.p, 0, counter.bytes, ctx->stream));
+    // One event pair per pass times the megakernel launches on their stream.
+    std::vector<hipEvent_t> evs;
+    struct EvGuard {
+        std::vector<hipEvent_t>& v;
+        ~EvGuard() {
+            for (hipEvent_t e : v) (void)hipEventDestroy(e);
+        }
+    } evGuard{evs};
+    if (stats) {
+        evs.resize(size_t(spp) * 2, nullptr);
+        for (auto& e : evs) HIP_TRY(hipEventCreate(&e));
+    }
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    for (uint32_t i = 0; i < spp; ++i) {
+        if (stats) HIP_TRY(hipEventRecord(evs[2 * i], ctx->stream));
+        HIP_TRY(crh::launchRenderPass(s->view, cv, iv, first_sample + i,
+                                      film->L.as<float4>(), film->P.as<float2>(),
+                                      counter.as<unsigned long long>(),
+                                      cfg->count_work != 0, ctx->stream));
+        if (stats) HIP_TRY(hipEventRecord(evs[2 * i + 1], ctx->stream));
+        HIP_TRY(crh::launchFilmGather(film->L.as<float4>(), film->P.as<float2>(),
+                                      film->w, film->h, film->sum.as<float>(),
+                                      film->wsum.as<float>(), ctx->stream));
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    if (stats) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        double mainMs = 0.0;
+        for (uint32_t i = 0; i < spp; ++i) {
+            float m = 0.f;
+            HIP_TRY(hipEventElapsedTime(&m, evs[2 * i], evs[2 * i + 1]));
+            mainMs += m;
+        }
+        unsigned long long c[3] = {0, 0, 0};
+        HIP_TRY(hipMemcpy(c, counter.p, sizeof(c), hipMemcpyDeviceToHost));
+        stats->kernel_ms = ms;
+        stats->main_kernel_ms = mainMs;
+        stats->total_ms = nowMs() - t0;
+        stats->rays = c[0];
+        stats->node_visits = c[1];
+        stats->tri_tests = c[2];
+        stats->launches = uint64_t(spp) * 2;
+    }
+    return CRH_OK;
+    GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,368 @@
+// gfx950 kernels: batch closest-hit / any-hit queries, the plain megakernel
+// integrator (one path per thread per sample index) and the deterministic
+// film gather.  Host launchers at the bottom are called by api.cpp.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "shading.h"
+
+namespace crh {
+
+// ---------------------------------------------------------------------------
+// Ray batch queries
+// ---------------------------------------------------------------------------
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kTraceClosest(
+    SceneView s, const crh_ray* __restrict__ rays, uint64_t n,
+    crh_hit* __restrict__ hits, uint32_t* __restrict__ work) {
+    extern __shared__ uint2 lds[];
+    uint64_t i = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4* rp = reinterpret_cast<const float4*>(rays + i);
+    float4 a = rp[0], b = rp[1];
+    Ray ray{mk3(a.x, a.y, a.z), a.w, mk3(b.x, b.y, b.z), b.w};
+    Hit h;
+    WorkCount wc{0, 0};
+    traverse<false, COUNT>(s, ray, h, lds + threadIdx.x, kTraceBlock, wc);
+    reinterpret_cast<uint4*>(hits)[i] =
+        make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.u),
+                   __float_as_uint(h.v));
+    if (COUNT) {
+        work[2 * i] = wc.nodes;
+        work[2 * i + 1] = wc.tris;
+    }
+}
+
+__global__ __launch_bounds__(kTraceBlock) void kTraceAny(
+    SceneView s, const crh_ray* __restrict__ rays, uint64_t n,
+    uint8_t* __restrict__ occ) {
+    extern __shared__ uint2 lds[];
+    uint64_t i = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4* rp = reinterpret_cast<const float4*>(rays + i);
+    float4 a = rp[0], b = rp[1];
+    Ray ray{mk3(a.x, a.y, a.z), a.w, mk3(b.x, b.y, b.z), b.w};
+    Hit h;
+    WorkCount wc{0, 0};
+    occ[i] = traverse<true, false>(s, ray, h, lds + threadIdx.x, kTraceBlock, wc)
+                 ? 1
+                 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Megakernel integrator (Core/Integrator.cu:13-227)
+// ---------------------------------------------------------------------------
+template <bool COUNT>
+struct PathCtx {
+    const SceneView& s;
+    uint2* stack;
+    uint32_t stride;
+    uint32_t rays;
+    WorkCount wc;
+};
+
+template <bool COUNT>
+CRH_D bool closest(PathCtx<COUNT>& c, const Ray& r, Hit& h) {
+    c.rays++;
+    return traverse<false, COUNT>(c.s, r, h, c.stack, c.stride, c.wc);
+}
+template <bool COUNT>
+CRH_D bool occluded(PathCtx<COUNT>& c, const Ray& r) {
+    Hit h;
+    c.rays++;
+    return traverse<true, COUNT>(c.s, r, h, c.stack, c.stride, c.wc);
+}
+
+struct LightSample {
+    Spec weight;
+    F3 dir;
+    float pdf;
+    uint32_t id;
+    float selPdf;
+};
+
+// sampleLight (Core/Integrator.cu:21-72)
+template <bool COUNT>
+CRH_D LightSample sampleLight(PathCtx<COUNT>& c, const Isect& it, const BSDF& bsdf,
+                              Sampler& smp) {
+    LightSample ls;
+    ls.weight = sp(0.f);
+    ls.dir = mk3(0, 0, 0);
+    ls.pdf = 0.f;
+    ls.id = 0xFFFFFFFFu;
+    ls.selPdf = 0.f;
+    const int cnt = int(c.s.nEmissive);
+    if (cnt == 0) return ls;
+    int sel = int(float(cnt) * smp.next1D());
+    sel = sel < cnt - 1 ? sel : cnt - 1;
+    uint32_t eid = c.s.emissive[sel];
+    TriData tri = getTri(c.s, eid);
+    F2 bc = sampleBary(smp.next2D());
+    F3 w = mk3(1.f - bc.x - bc.y, bc.x, bc.y);
+    F3 lp = interp3(tri.p, w);
+    F3 ln = interp3(tri.n, w);
+    F3 target = offsetRayOrigin(lp, faceNormal(tri));
+    F3 toL = target - it.pos;
+    float d2 = dot(toL, toL);
+    float dist = sqrtf(d2);
+    F3 ld = toL / dist;
+    F3 on = oriented(it);
+    if (dot(on, ld) < 0.0f || dot(ln, ld) > 0.0f || dist < kEps) return ls;
+    Ray sr;
+    sr.o = offsetRayOrigin(it.pos, on);
+    sr.tMin = 0.f;
+    sr.d = ld;
+    sr.tMax = length(sr.o - target) - kEps;
+    if (occluded(c, sr)) return ls;
+    const GMaterial& m = c.s.mats[c.s.primMat[eid]];
+    ls.weight = bsdfEval(bsdf, it.view, ld) * sp3(m.emission);
+    ls.dir = ld;
+    ls.pdf = d2 / (triArea(tri) * absDot(ld, ln));
+    ls.id = eid;
+    ls.selPdf = float(1.0 / double(cnt));
+    return ls;
+}
+
+// evalMIS (Core/Integrator.cu:74-140)
+template <bool COUNT>
+CRH_D Spec evalMIS(PathCtx<COUNT>& c, const Isect& it, const BSDF& bsdf,
+                   Sampler& smp) {
+    Spec value = sp(0.f);
+    LightSample ls = sampleLight(c, it, bsdf, smp);
+    if (ls.pdf <= 0.0f) return value;
+    {
+        float lp = ls.pdf;
+        float bp = bsdfPdf(bsdf, it.view, ls.dir);
+        if (bp > 0.0f) value = value + powerHeur(lp, bp) * ls.weight / lp;
+    }
+    {
+        F3 dir;
+        float bp = 0.0f;
+        Spec bw = bsdfSampleEval(bsdf, smp, it.view, dir, bp);
+        if (smaxv(bw) > 0.0f && bp > 0.0f) {
+            Ray r;
+            r.o = offsetRayOrigin(it.pos, oriented(it));
+            r.tMin = 0.f;
+            r.d = dir;
+            r.tMax = __builtin_inff();
+            Hit h;
+            if (closest(c, r, h) && h.prim == ls.id) {
+                Isect lit = makeIsect(c.s, r, h);
+                const GMaterial& m = c.s.mats[c.s.primMat[h.prim]];
+                if (m.flags & 1u) {
+                    F3 toL = lit.pos - it.pos;
+                    float d2 = dot(toL, toL);
+                    F3 lv = toL / sqrtf(d2);
+                    float cl = absDot(lv, lit.faceN);
+                    if (cl != 0.0f) {
+                        float area = triArea(getTri(c.s, h.prim));
+                        float lpdf = d2 / (area * cl);
+                        value = value + powerHeur(bp, lpdf) * sp3(m.emission) *
+                                            bw / bp;
+                    }
+                }
+            }
+        }
+    }
+    return value / ls.selPdf;
+}
+
+CRH_D void tileCoord(uint32_t tilesX, uint32_t& x, uint32_t& y) {
+    // 16x16 pixel tile per block; each wave covers an 8x8 quarter.
+    uint32_t tile = blockIdx.x;
+    uint32_t tx = tile % tilesX, ty = tile / tilesX;
+    uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    x = tx * 16u + (wave & 1u) * 8u + (lane & 7u);
+    y = ty * 16u + (wave >> 1) * 8u + (lane >> 3);
+}
+
+// pathTraceKernel (Core/Integrator.cu:142-227): radiance of sample index
+// `si` for every pixel into L (float4 per pixel, w unused) and the jittered
+// film position into P.
+template <bool MIS, bool COUNT>
+__global__ __launch_bounds__(kRenderBlock) void kRenderPass(
+    SceneView s, CameraView cam, IntegratorView cfg, uint32_t si,
+    uint32_t tilesX, float4* __restrict__ L, float2* __restrict__ P,
+    unsigned long long* __restrict__ counters) {
+    extern __shared__ uint2 lds[];
+    uint32_t x, y;
+    tileCoord(tilesX, x, y);
+    uint32_t rays = 0, nodes = 0, tris = 0;
+    if (x < cam.width && y < cam.height) {
+        PathCtx<COUNT> c{s, lds + threadIdx.x, kRenderBlock, 0, {0, 0}};
+        Sampler smp(x, y, si);
+        F2 j = smp.next2D();
+        F2 pix{float(x) + j.x, float(y) + j.y};
+        // CameraProxy::generateRay (Core/CameraProxy.h:25-40)
+        float u = (2.0f * (pix.x / float(cam.width)) - 1.0f);
+        float v = (1.0f - 2.0f * (pix.y / float(cam.height)));
+        float hw = cam.aspect * cam.hh;
+        F3 dir = normalize(ld3(cam.u) * (u * hw) + ld3(cam.v) * (v * cam.hh) +
+                           ld3(cam.w));
+        Ray ray{ld3(cam.pos), 0.f, dir, __builtin_inff()};
+        Spec rad = sp(0.f), beta = sp(1.f);
+        for (uint32_t depth = 0; depth < cfg.maxDepth; ++depth) {
+            bool terminate = false;
+            Hit h;
+            if (closest(c, ray, h)) {
+                Isect it = makeIsect(s, ray, h);
+                const GMaterial& m = s.mats[s.primMat[h.prim]];
+                if (m.flags & 1u) {
+                    rad = rad + sp3(m.emission) * beta;
+                    break;
+                }
+                BSDF bsdf = makeBSDF(m, it.frame);
+                if (MIS) {
+                    rad = rad + evalMIS(c, it, bsdf, smp) * beta;
+                } else {
+                    LightSample ls = sampleLight(c, it, bsdf, smp);
+                    if (ls.pdf > 0.0f)
+                        rad = rad + ls.weight * beta / ls.pdf / ls.selPdf;
+                }
+                F3 wi;
+                float pdf = 0.0f;
+                Spec f = bsdfSampleEval(bsdf, smp, it.view, wi, pdf);
+                if (pdf <= 0.0f) break;
+                beta = beta * (f / pdf);
+                ray.o = offsetRayOrigin(it.pos, oriented(it));
+                ray.d = wi;
+            } else {
+                rad = rad + sp3(s.env) * beta;
+                terminate = true;
+            }
+            if (smp.next1D() < cfg.rrThreshold) terminate = true;
+            if (terminate) break;
+            if (depth >= cfg.rrStartDepth) {
+                float q = smax(cfg.rrThreshold, 1.0f - smaxv(beta));
+                q = q < 0.0f ? 0.0f : (1.0f < q ? 1.0f : q);
+                if (smp.next1D() < q) break;
+                beta = beta / (1.0f - q);
+            }
+        }
+        size_t id = size_t(y) * cam.width + x;
+        L[id] = make_float4(rad.r, rad.g, rad.b, 0.f);
+        P[id] = make_float2(pix.x, pix.y);
+        rays = c.rays;
+        nodes = c.wc.nodes;
+        tris = c.wc.tris;
+    }
+    // Wave-level reductions, one atomic per wave and counter:
+    // counters = {rays, node visits, triangle tests}.
+    unsigned long long r64 = rays, n64 = nodes, t64 = tris;
+    for (int off = 32; off > 0; off >>= 1) {
+        r64 += __shfl_down(r64, off, 64);
+        if (COUNT) {
+            n64 += __shfl_down(n64, off, 64);
+            t64 += __shfl_down(t64, off, 64);
+        }
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        if (r64) atomicAdd(counters, r64);
+        if (COUNT) {
+            atomicAdd(counters + 1, n64);
+            atomicAdd(counters + 2, t64);
+        }
+    }
+}
+
+// Deterministic film update for one pass.  Each pixel gathers the tent-filter
+// contributions of the 3x3 neighbouring samples in row-major sample order,
+// which is the order a sequential SensorData::addSample loop over the pass
+// (Core/Sensor.cu:19-43) delivers them; samples with NaN/Inf are dropped and
+// zero weights skipped (they add exactly nothing).
+__global__ __launch_bounds__(256) void kFilmGather(
+    const float4* __restrict__ L, const float2* __restrict__ P, uint32_t W,
+    uint32_t H, float* __restrict__ sum, float* __restrict__ wsum) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= W * H) return;
+    int x = int(i % W), y = int(i / W);
+    float s0 = sum[3 * size_t(i)], s1 = sum[3 * size_t(i) + 1],
+          s2 = sum[3 * size_t(i) + 2], ws = wsum[i];
+    for (int dy = -1; dy <= 1; ++dy) {
+        int sy = y + dy;
+        if (sy < 0 || sy >= int(H)) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+            int sx = x + dx;
+            if (sx < 0 || sx >= int(W)) continue;
+            size_t sidx = size_t(sy) * W + sx;
+            float4 l = L[sidx];
+            if (isnan(l.x) || isnan(l.y) || isnan(l.z) || isinf(l.x) ||
+                isinf(l.y) || isinf(l.z))
+                continue;
+            float2 p = P[sidx];
+            float ddx = p.x - (float(x) + 0.5f);
+            float ddy = p.y - (float(y) + 0.5f);
+            float w = smax(0.0f, 1.0f - fabsf(ddx)) * smax(0.0f, 1.0f - fabsf(ddy));
+            if (w == 0.0f) continue;
+            s0 += l.x * w;
+            s1 += l.y * w;
+            s2 += l.z * w;
+            ws += w;
+        }
+    }
+    sum[3 * size_t(i)] = s0;
+    sum[3 * size_t(i) + 1] = s1;
+    sum[3 * size_t(i) + 2] = s2;
+    wsum[i] = ws;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
+                              uint64_t n, crh_hit* hits, uint32_t* work,
+                              hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    size_t lds = size_t(s.stackCap) * kTraceBlock * sizeof(uint2);
+    dim3 grid(uint32_t((n + kTraceBlock - 1) / kTraceBlock));
+    if (work)
+        hipLaunchKernelGGL((kTraceClosest<true>), grid, dim3(kTraceBlock), lds,
+                           st, s, rays, n, hits, work);
+    else
+        hipLaunchKernelGGL((kTraceClosest<false>), grid, dim3(kTraceBlock), lds,
+                           st, s, rays, n, hits, work);
+    return hipGetLastError();
+}
+
+hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
+                          uint8_t* occ, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    size_t lds = size_t(s.stackCap) * kTraceBlock * sizeof(uint2);
+    dim3 grid(uint32_t((n + kTraceBlock - 1) / kTraceBlock));
+    hipLaunchKernelGGL(kTraceAny, grid, dim3(kTraceBlock), lds, st, s, rays, n,
+                       occ);
+    return hipGetLastError();
+}
+
+hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
+                            const IntegratorView& cfg, uint32_t si, float4* L,
+                            float2* P, unsigned long long* counters,
+                            bool countWork, hipStream_t st) {
+    uint32_t tilesX = (cam.width + 15u) / 16u, tilesY = (cam.height + 15u) / 16u;
+    size_t lds = size_t(s.stackCap) * kRenderBlock * sizeof(uint2);
+    dim3 grid(tilesX * tilesY), block(kRenderBlock);
+    if (cfg.useMIS && countWork)
+        hipLaunchKernelGGL((kRenderPass<true, true>), grid, block, lds, st, s, cam,
+                           cfg, si, tilesX, L, P, counters);
+    else if (cfg.useMIS)
+        hipLaunchKernelGGL((kRenderPass<true, false>), grid, block, lds, st, s, cam,
+                           cfg, si, tilesX, L, P, counters);
+    else if (countWork)
+        hipLaunchKernelGGL((kRenderPass<false, true>), grid, block, lds, st, s, cam,
+                           cfg, si, tilesX, L, P, counters);
+    else
+        hipLaunchKernelGGL((kRenderPass<false, false>), grid, block, lds, st, s,
+                           cam, cfg, si, tilesX, L, P, counters);
+    return hipGetLastError();
+}
+
+hipError_t launchFilmGather(const float4* L, const float2* P, uint32_t W,
+                            uint32_t H, float* sum, float* wsum,
+                            hipStream_t st) {
+    uint32_t n = W * H;
+    hipLaunchKernelGGL(kFilmGather, dim3((n + 255u) / 256u), dim3(256), 0, st,
+                       L, P, W, H, sum, wsum);
+    return hipGetLastError();
+}
+
+}  // namespace crh

@@ -1,0 +1,313 @@
+// Device shading: hit reconstruction, shading frame, the three BRDFs and the
+// light sampling / MIS estimator of the reference megakernel.
+//
+// Restates Scene/Scene.cu:31-58 (createIntersection), Core/Frame.cu:6-31,
+// Material/BSDF.cu:6-103, Material/{Lambertian,Phong,Principled}BRDF.cu,
+// Material/Fresnel.cu, Math/Sampling.cu and Core/Integrator.cu:13-140,
+// including the reference's arithmetic quirks (double-precision
+// subexpressions where the reference mixes double literals, mis-normalised
+// Principled lobe probabilities, the Phong chosen-lobe sample pdf).
+// Spectrum = RGB (Core/Spectrum.h:326).
+#pragma once
+#include "traverse.h"
+
+namespace crh {
+
+struct Spec {
+    float r, g, b;
+};
+CRH_D Spec sp(float s) { return {s, s, s}; }
+CRH_D Spec sp3(const float* p) { return {p[0], p[1], p[2]}; }
+CRH_D Spec operator+(Spec a, Spec b) { return {a.r + b.r, a.g + b.g, a.b + b.b}; }
+CRH_D Spec operator-(Spec a, Spec b) { return {a.r - b.r, a.g - b.g, a.b - b.b}; }
+CRH_D Spec operator*(Spec a, Spec b) { return {a.r * b.r, a.g * b.g, a.b * b.b}; }
+CRH_D Spec operator*(Spec a, float s) { return {a.r * s, a.g * s, a.b * s}; }
+CRH_D Spec operator*(float s, Spec a) { return {s * a.r, s * a.g, s * a.b}; }
+CRH_D Spec operator/(Spec a, float s) { return {a.r / s, a.g / s, a.b / s}; }
+CRH_D float smaxv(Spec a) { return smax(smax(a.r, a.g), a.b); }
+CRH_D float savg(Spec a) { return ((0.0f + a.r) + a.g + a.b) / 3.0f; }
+CRH_D bool sbad(Spec a) {
+    return isnan(a.r) || isnan(a.g) || isnan(a.b) || isinf(a.r) ||
+           isinf(a.g) || isinf(a.b);
+}
+
+// --- Triangle data (Scene/Triangle.cu:9-37, 78-92) -----------------------
+struct TriData {
+    F3 p[3], n[3];
+    F2 uv[3];
+};
+CRH_D TriData getTri(const SceneView& s, uint32_t id) {
+    TriData t;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        uint32_t vi = s.idx[3 * id + k];
+        t.p[k] = ld3(s.pos + 3 * size_t(vi));
+        t.n[k] = ld3(s.nrm + 3 * size_t(vi));
+        t.uv[k] = F2{s.uv[2 * size_t(vi)], s.uv[2 * size_t(vi) + 1]};
+    }
+    return t;
+}
+CRH_D F3 faceNormal(const TriData& t) {
+    return normalize(cross(t.p[1] - t.p[0], t.p[2] - t.p[0]));
+}
+CRH_D F3 interp3(const F3* v, F3 w) {
+    return v[0] * w.x + v[1] * w.y + v[2] * w.z;
+}
+CRH_D float triArea(const TriData& t) {
+    return length(cross(t.p[0] - t.p[1], t.p[2] - t.p[1])) / 2.f;
+}
+
+// --- Frame (Core/Frame.cu:6-31) -------------------------------------------
+struct Frame {
+    F3 T, B, N;
+};
+CRH_D Frame makeFrame(F3 n) {
+    F3 t = mk3(0, 1, 0);
+    if (1.0 - double(absDot(n, t)) < double(kEps)) t = mk3(1, 0, 0);
+    F3 bt = normalize(cross(n, t));
+    Frame f;
+    f.N = n;
+    f.T = normalize(cross(bt, n));
+    f.B = cross(f.N, f.T);
+    return f;
+}
+CRH_D F3 toLocal(const Frame& f, F3 w) {
+    return {f.T.x * w.x + f.T.y * w.y + f.T.z * w.z,
+            f.B.x * w.x + f.B.y * w.y + f.B.z * w.z,
+            f.N.x * w.x + f.N.y * w.y + f.N.z * w.z};
+}
+CRH_D F3 toWorld(const Frame& f, F3 l) {
+    return {f.T.x * l.x + f.B.x * l.y + f.N.x * l.z,
+            f.T.y * l.x + f.B.y * l.y + f.N.y * l.z,
+            f.T.z * l.x + f.B.z * l.y + f.N.z * l.z};
+}
+
+// --- Intersection (Scene/Scene.cu:31-58, Core/Intersection.cu:4-10) -------
+struct Isect {
+    F3 pos, faceN, view;
+    Frame frame;
+    bool front;
+};
+CRH_D F3 oriented(const Isect& it) { return it.front ? it.faceN : -it.faceN; }
+
+CRH_D Isect makeIsect(const SceneView& s, const Ray& ray, const Hit& h) {
+    Isect it;
+    it.pos = ray.o + h.t * ray.d;
+    it.view = -normalize(ray.d);
+    F3 w = mk3(1.f - h.u - h.v, h.u, h.v);
+    TriData tri = getTri(s, h.prim);
+    it.faceN = faceNormal(tri);
+    F3 sn = normalize(interp3(tri.n, w));
+    it.front = dot(it.view, it.faceN) > 0.0f;
+    it.frame = makeFrame(it.front ? sn : -sn);
+    return it;
+}
+
+// --- Sampling warps (Math/Sampling.cu) -------------------------------------
+CRH_D F2 sampleBary(F2 u) {
+    bool flip = (u.x + u.y) > 1.0f;
+    return {flip ? 1.0f - u.x : u.x, flip ? 1.0f - u.y : u.y};
+}
+CRH_D F3 cosHemi(F2 u) {
+    float r = sqrtf(u.x);
+    float phi = 2.0f * kPi * u.y;
+    float x = r * cosf(phi);
+    float y = r * sinf(phi);
+    float z = sqrtf(smax(0.0f, 1.0f - x * x - y * y));
+    return {x, y, z};
+}
+CRH_D float cosHemiPdf(F3 d) { return smax(0.0f, d.z) * kInvPi; }
+CRH_D float powerHeur(float f, float g) { return (f * f) / (f * f + g * g); }
+CRH_D float acosSafe(float v) {
+    const float lo = kEps, hi = float(1.0 - double(kEps));
+    float c = v < lo ? lo : (hi < v ? hi : v);
+    return acosf(c);
+}
+
+// --- BSDF (Material/BSDF.cu:6-103) -----------------------------------------
+struct BSDF {
+    Frame frame;
+    Spec kd, ks;
+    float n;
+    float pD, pS;   // lobe probabilities (Phong, Principled)
+    float rough, metal;
+    uint32_t type;
+};
+
+CRH_D BSDF makeBSDF(const GMaterial& m, const Frame& f) {
+    BSDF b;
+    b.frame = f;
+    b.type = m.type;
+    b.kd = sp3(m.kd);
+    b.ks = sp3(m.ks);
+    b.n = m.shininess;
+    b.pD = b.pS = 0.f;
+    b.rough = b.metal = 0.f;
+    if (m.type == 2u) {  // Phong (PhongBRDF.cu:10-19)
+        float d = smax(savg(b.kd), 0.0f), s = smax(savg(b.ks), 0.0f);
+        float sum = smax(d + s, kEps);
+        b.pD = d / sum;
+        b.pS = s / sum;
+    } else if (m.type == 1u) {  // Principled (PrincipledBRDF.cu:47-58)
+        float r = sqrtf(2.0f / (b.n + 1.0f));
+        float mt = savg(b.ks);
+        b.rough = fminf(fmaxf(r, 0.0f), 1.0f);
+        b.metal = fminf(fmaxf(mt, 0.0f), 1.0f);
+        float spr = 0.1f * (1.0f - b.metal) + 1.0f * b.metal;
+        float pd = 1.0f / (1.0f + spr);
+        b.pD = pd;
+        b.pS = spr / (pd + spr);
+    }
+    return b;
+}
+
+// Phong (PhongBRDF.cu:21-68)
+CRH_D Spec phongEval(const BSDF& b, F3 wo, F3 wi) {
+    F3 r = reflect(-wo, mk3(0, 0, 1));
+    return b.kd * kInvPi +
+           b.ks * (b.n + 2) * kInv2Pi * powf(smax(dot(r, wi), 0.0f), b.n);
+}
+CRH_D float phongPdf(const BSDF& b, F3 wo, F3 wi) {
+    float dp = cosHemiPdf(wi);
+    F3 r = reflect(-wo, mk3(0, 0, 1));
+    float ca = smax(dot(r, wi), 0.0f);
+    float spdf = (b.n + 1) * kInv2Pi * powf(ca, b.n);
+    return dp * b.pD + spdf * b.pS;
+}
+CRH_D F3 phongSample(const BSDF& b, F3 wo, F2 u, float& pdf) {
+    F3 wi = mk3(0, 0, 0);
+    pdf = 0.0f;
+    if (b.pD > 0.0f && u.x < b.pD) {
+        F2 u0{u.x / b.pD, u.y};
+        wi = cosHemi(u0);
+        pdf = cosHemiPdf(wi);
+    } else if (b.pS > 0.0f) {
+        float ux = (u.x - b.pD) / b.pS;
+        float alpha = acosSafe(powf(ux, 1 / (b.n + 1)));
+        float th = acosSafe(wo.z) + alpha;
+        const float hiPi = float(double(kPi) / 2.0);
+        th = th < 0.0f ? 0.0f : (hiPi < th ? hiPi : th);
+        float phi = k2Pi * u.y;
+        float st = sinf(th);
+        wi = mk3(st * cosf(phi), st * sinf(phi),
+                 float(sqrt(1.0 - double(st * st))));
+        pdf = (b.n + 1) * kInv2Pi * powf(cosf(alpha), b.n);
+    }
+    return wi;
+}
+
+// Principled (PrincipledBRDF.cu:9-151)
+CRH_D float ggxD(float r, float ch) {
+    float a = r * r;
+    float a2 = a * a;
+    float c2 = ch * ch;
+    float den = float(1.0 + (double(a2) - 1.0) * double(c2));
+    return a2 * kInvPi / (den * den);
+}
+CRH_D float smithG1(float aG, float cv) {
+    float a = aG * aG;
+    float b = cv * cv;
+    return 1 / (cv + sqrtf(a + b - a * b));
+}
+CRH_D float smithG(float r, float ci, float cv) {
+    float a = float(0.5 + double(r) * 0.5);
+    a *= a;
+    return smithG1(a, ci) * smithG1(a, cv);
+}
+CRH_D Spec prEval(const BSDF& b, F3 wo, F3 wi) {
+    if (wo.z <= 0.0f || wi.z <= 0.0f) return sp(0.f);
+    F3 wh = normalize(wo + wi);
+    float cv = wo.z, ci = wi.z, ch = wh.z;
+    float cd = dot(wo, wh);
+    // evaluateDiffuse (:135-141), double subexpressions as in the reference
+    float fd90 = float(0.5 + 2.0 * double(b.rough) * double(cd) * double(cd));
+    float fl = float(1.0 + (double(fd90) - 1.0) * pow(1.0 - double(ci), 5.0));
+    float fv = float(1.0 + (double(fd90) - 1.0) * pow(1.0 - double(cv), 5.0));
+    Spec diff = b.kd * kInvPi * fv * fl * (1.0f - b.metal);
+    // evaluateSpecular (:143-150)
+    float D = ggxD(b.rough, ch);
+    Spec f0 = b.kd * (1.0f - b.metal) + sp(0.04f) * b.metal;
+    float v0 = 1.f - cd;
+    float v1 = v0 * v0;
+    Spec F = f0 + (sp(1.f) - f0) * v1 * v1 * v0;
+    float G = smithG(b.rough, ci, cv);
+    float den = float(4.0 * double(ci) * double(cv));
+    Spec spec = D * F * G / den;
+    return diff + spec;
+}
+CRH_D float prPdf(const BSDF& b, F3 wo, F3 wi) {
+    if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+    F3 wh = normalize(wo + wi);
+    float ch = wh.z;
+    float cd = dot(wo, wh);
+    if (cd <= 0.0f) return 0.0f;
+    float dp = cosHemiPdf(wi);
+    float spdf = 0.0f;
+    if (ch > 0.0f) spdf = ggxD(b.rough, ch) * ch / (4.0f * cd);
+    return dp * b.pD + spdf * b.pS;
+}
+CRH_D F3 prSample(const BSDF& b, F3 wo, F2 u, float& pdf) {
+    if (wo.z <= 0.0f) {
+        pdf = 0.0f;
+        return mk3(0, 0, 0);
+    }
+    float u1 = u.x, u2 = u.y;
+    F3 wi;
+    if (u1 < b.pS) {
+        u1 /= b.pS;
+        float alpha = b.rough * b.rough;
+        float a2 = alpha * alpha;
+        float phi = u2 * k2Pi;
+        float tan2 = a2 * u1 / (1 - u1);
+        float ct = 1 / sqrtf(1 + tan2);
+        float r = sqrtf(smax(1 - ct * ct, 0.0f));
+        F3 wh = mk3(cosf(phi) * r, sinf(phi) * r, ct);
+        wi = reflect(-wo, wh);
+    } else {
+        u1 = (u1 - b.pS) / (1.0f - b.pS);
+        wi = cosHemi(F2{u1, u2});
+    }
+    pdf = prPdf(b, wo, wi);
+    return wi;
+}
+
+// BSDFBase::evaluate multiplies by |cos| (BSDF.h:23-27).
+CRH_D Spec evalLocal(const BSDF& b, F3 wo, F3 wi) {
+    Spec f;
+    if (b.type == 0u) f = b.kd * kInvPi;
+    else if (b.type == 1u) f = prEval(b, wo, wi);
+    else f = phongEval(b, wo, wi);
+    return f * fabsf(wi.z);
+}
+CRH_D float pdfLocal(const BSDF& b, F3 wo, F3 wi) {
+    if (b.type == 0u) return cosHemiPdf(wi);
+    if (b.type == 1u) return prPdf(b, wo, wi);
+    return phongPdf(b, wo, wi);
+}
+CRH_D F3 sampleLocal(const BSDF& b, F3 wo, F2 u, float& pdf) {
+    if (b.type == 0u) {
+        F3 wi = cosHemi(u);
+        pdf = cosHemiPdf(wi);
+        return wi;
+    }
+    if (b.type == 1u) return prSample(b, wo, u, pdf);
+    return phongSample(b, wo, u, pdf);
+}
+CRH_D Spec bsdfEval(const BSDF& b, F3 wo, F3 wi) {
+    return evalLocal(b, toLocal(b.frame, wo), toLocal(b.frame, wi));
+}
+CRH_D float bsdfPdf(const BSDF& b, F3 wo, F3 wi) {
+    return pdfLocal(b, toLocal(b.frame, wo), toLocal(b.frame, wi));
+}
+CRH_D Spec bsdfSampleEval(const BSDF& b, Sampler& smp, F3 wo, F3& wi,
+                          float& pdf) {
+    F3 woL = toLocal(b.frame, wo);
+    pdf = 0.0f;
+    F2 u = smp.next2D();
+    F3 wiL = sampleLocal(b, woL, u, pdf);
+    Spec f = evalLocal(b, woL, wiL);
+    wi = toWorld(b.frame, wiL);
+    return f;
+}
+
+}  // namespace crh

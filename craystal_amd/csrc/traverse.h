@@ -1,0 +1,136 @@
+// Closest-hit and any-hit BVH traversal for gfx950.
+//
+// Semantics are those of BVHData::intersect (Core/BVH.cu:13-84) with
+// intersectShape (Scene/Shape.h:17-37): root-box reject; pop entries whose
+// entry t exceeds the closest hit; at an internal node test both child boxes
+// and visit the nearer first (ties: left first); push a child only if its
+// entry t is below the closest hit; leaf primitives are tested in leaf order
+// and a hit replaces the closest one only if strictly nearer.
+//
+// The reference pushes both children and immediately pops the near one; here
+// the near child stays in registers and only the far child is pushed.  The
+// popped near child would never fail the `entry.tMin > closestT` test (it was
+// pushed with t < closestT and closestT has not changed), so the visit order
+// and every comparison are identical.  With that, the stack never holds more
+// entries than the tree has levels, and it lives in LDS: entry k of a thread
+// is at stack[k * stride], stride = threads per block, so a wave's lanes hit
+// consecutive 8-byte slots.
+#pragma once
+#include "device_math.h"
+
+namespace crh {
+
+struct Hit {
+    uint32_t prim;
+    float t, u, v;
+};
+
+struct WorkCount {
+    uint32_t nodes, tris;
+};
+
+CRH_D void loadNode(const GNode* nodes, uint32_t i, float4& a, float4& b,
+                    float4& c, uint4& d) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    a = p[0];
+    b = p[1];
+    c = p[2];
+    d = reinterpret_cast<const uint4*>(p)[3];
+}
+
+// Returns true if any primitive was hit inside [tMin, tMax] (the reference's
+// return value).  ANY: stop at the first such primitive.
+template <bool ANY, bool COUNT>
+CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, uint2* stack,
+                    uint32_t stride, WorkCount& wc) {
+    hit.prim = 0xFFFFFFFFu;
+    hit.t = __builtin_inff();
+    hit.u = 0.f;
+    hit.v = 0.f;
+    if (!s.hasGeom) return false;
+    RayPre r = prep(ray);
+    float dummy;
+    if (!boxHit(s.rootMin, s.rootMax, r, dummy)) return false;
+
+    bool isHit = false;
+    float closestT = hit.t;
+    uint32_t cur = s.rootRef;
+    int sp = 0;
+    while (true) {
+        if (cur & kLeafBit) {
+            uint32_t i = cur & ~kLeafBit;
+            while (true) {
+                const float4* tp = reinterpret_cast<const float4*>(s.tris + i);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                uint32_t pw = __float_as_uint(a.w);
+                if (COUNT) wc.tris++;
+                float u, v, t;
+                if (triHit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z),
+                           mk3(c.x, c.y, c.z), r, u, v, t)) {
+                    if (!(t < r.tMin || t > r.tMax)) {
+                        isHit = true;
+                        if (t < hit.t) {
+                            hit.t = t;
+                            hit.u = u;
+                            hit.v = v;
+                            hit.prim = pw & ~kLeafBit;
+                        }
+                        closestT = smin(closestT, hit.t);
+                        if (ANY) return true;
+                    }
+                }
+                if (pw & kLeafBit) break;
+                ++i;
+            }
+        } else {
+            if (COUNT) wc.nodes++;
+            float4 a, b, c;
+            uint4 d;
+            loadNode(s.nodes, cur, a, b, c, d);
+            const float lmin[3] = {a.x, a.y, a.z};
+            const float lmax[3] = {a.w, b.x, b.y};
+            const float rmin[3] = {b.z, b.w, c.x};
+            const float rmax[3] = {c.y, c.z, c.w};
+            float lt = __builtin_inff(), rt = __builtin_inff();
+            bool hl = boxHit(lmin, lmax, r, lt);
+            bool hr = boxHit(rmin, rmax, r, rt);
+            if (hl && hr) {
+                // Reference pushes the far child first, then the near one
+                // (Core/BVH.cu:60-72); near = right iff lt > rt.
+                bool rightNear = lt > rt;
+                uint32_t nearRef = rightNear ? d.y : d.x;
+                uint32_t farRef = rightNear ? d.x : d.y;
+                float nearT = rightNear ? rt : lt;
+                float farT = rightNear ? lt : rt;
+                if (nearT < closestT) {
+                    if (farT < closestT) {
+                        stack[sp * stride] = make_uint2(farRef, __float_as_uint(farT));
+                        ++sp;
+                    }
+                    cur = nearRef;
+                    continue;
+                }
+            } else if (hl && lt < closestT) {
+                cur = d.x;
+                continue;
+            } else if (hr && rt < closestT) {
+                cur = d.y;
+                continue;
+            }
+        }
+        // Pop the next entry that is not beyond the closest hit.
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            uint2 e = stack[sp * stride];
+            if (__uint_as_float(e.y) > closestT) continue;
+            cur = e.x;
+            found = true;
+            break;
+        }
+        if (!found) break;
+    }
+    return isHit;
+}
+
+}  // namespace crh

@@ -1,0 +1,331 @@
+"""Seeded procedural scenes and ray batches for the benchmark workloads.
+
+Every generator is deterministic: scene layout draws come from SplitMix64
+seeded 0xC0FFEE, ray batches from SplitMix64 seeded 0x5EED, both evaluated
+as a counter-based stream (value i = mix(seed + (i+1) * golden gamma)).
+The same arrays feed the GPU path and the CPU oracle.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .core import Camera, MaterialData, MaterialType, MeshData, SceneData
+
+SCENE_SEED = 0xC0FFEE
+RAY_SEED = 0x5EED
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M = 0xFFFFFFFFFFFFFFFF
+
+
+def splitmix64(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    """n SplitMix64 outputs of stream `seed`, starting at counter `offset`."""
+    with np.errstate(over="ignore"):
+        i = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * _GAMMA
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    """Doubles in [0, 1) from the top 53 bits."""
+    return (splitmix64(seed, n, offset) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+class Stream:
+    """Sequential scalar draws from one SplitMix64 stream."""
+
+    def __init__(self, seed: int):
+        self.seed, self.i = seed, 0
+
+    def uniform(self, lo=0.0, hi=1.0, n=None):
+        k = 1 if n is None else n
+        u = uniform01(self.seed, k, self.i)
+        self.i += k
+        v = lo + (hi - lo) * u
+        return float(v[0]) if n is None else v
+
+
+# ---------------------------------------------------------------------------
+# Primitive meshes
+# ---------------------------------------------------------------------------
+def quad(corner, e1, e2, material, normal=None) -> MeshData:
+    """Quad corner, corner+e1, corner+e1+e2, corner+e2 as two triangles with
+    counter-clockwise winding around cross(e1, e2)."""
+    c = np.asarray(corner, np.float64)
+    a, b = np.asarray(e1, np.float64), np.asarray(e2, np.float64)
+    P = np.array([c, c + a, c + a + b, c + b], np.float32)
+    n = np.cross(a, b) if normal is None else np.asarray(normal, np.float64)
+    n = (n / np.linalg.norm(n)).astype(np.float32)
+    N = np.tile(n, (4, 1))
+    I = np.array([0, 1, 2, 0, 2, 3], np.uint32)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    return MeshData(P, N, I, material, uv)
+
+
+def box_mesh(center, size, angle_y, material) -> MeshData:
+    """Axis-aligned box of `size` rotated about +y by angle_y (radians),
+    outward normals, 12 triangles."""
+    sx, sy, sz = (np.asarray(size, np.float64) / 2.0)
+    ca, sa = math.cos(angle_y), math.sin(angle_y)
+    R = np.array([[ca, 0, sa], [0, 1, 0], [-sa, 0, ca]])
+    faces = [  # (normal, u-axis, v-axis) in local space, u x v = normal
+        ((1, 0, 0), (0, 0, -1), (0, 1, 0)),
+        ((-1, 0, 0), (0, 0, 1), (0, 1, 0)),
+        ((0, 1, 0), (1, 0, 0), (0, 0, -1)),
+        ((0, -1, 0), (1, 0, 0), (0, 0, 1)),
+        ((0, 0, 1), (1, 0, 0), (0, 1, 0)),
+        ((0, 0, -1), (-1, 0, 0), (0, 1, 0)),
+    ]
+    half = np.array([sx, sy, sz])
+    P, N, I = [], [], []
+    for k, (n, u, v) in enumerate(faces):
+        n, u, v = map(np.asarray, (n, u, v))
+        cen = n * half
+        hu = np.abs(u) @ half
+        hv = np.abs(v) @ half
+        corners = [cen - u * hu - v * hv, cen + u * hu - v * hv, cen + u * hu + v * hv,
+                   cen - u * hu + v * hv]
+        for q in corners:
+            P.append(R @ q + np.asarray(center))
+            N.append(R @ n)
+        b = 4 * k
+        I += [b, b + 1, b + 2, b, b + 2, b + 3]
+    return MeshData(np.array(P, np.float32), np.array(N, np.float32), np.array(I, np.uint32),
+                    material)
+
+
+_ICO_CACHE: dict[int, tuple[np.ndarray, np.ndarray]] = {}
+
+
+def unit_icosphere(subdiv: int):
+    """Unit icosphere: 10*4^s + 2 vertices, 20*4^s triangles (float64)."""
+    if subdiv in _ICO_CACHE:
+        return _ICO_CACHE[subdiv]
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+         (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    V = [np.array(v, np.float64) / np.linalg.norm(v) for v in V]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+         (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+         (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache: dict[tuple[int, int], int] = {}
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = V[a] + V[b]
+                V.append(m / np.linalg.norm(m))
+                cache[key] = len(V) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        F = nf
+    out = (np.array(V, np.float64), np.array(F, np.uint32).reshape(-1))
+    _ICO_CACHE[subdiv] = out
+    return out
+
+
+def icosphere(center, radius, subdiv, material) -> MeshData:
+    V, F = unit_icosphere(subdiv)
+    P = (V * radius + np.asarray(center, np.float64)).astype(np.float32)
+    return MeshData(P, V.astype(np.float32), F, material)
+
+
+def principled(base, roughness, metallic) -> MaterialData:
+    """A Principled material expressed in the reference's (kd, ks, shininess)
+    parameters: roughness = sqrt(2/(n+1)) and metallic = avg(ks)
+    (Material/BSDF.cu:21-24, Material/PrincipledBRDF.cu:51-55)."""
+    n = 2.0 / (roughness * roughness) - 1.0
+    m = MaterialData(diffuseRefl=tuple(base), specularRefl=(metallic,) * 3, shininess=n,
+                     type=MaterialType.Principled)
+    return m.finalize()
+
+
+def diffuse(rgb) -> MaterialData:
+    return MaterialData(diffuseRefl=tuple(rgb), type=MaterialType.FullDiffuse).finalize()
+
+
+def phong(kd, ks, n) -> MaterialData:
+    return MaterialData(diffuseRefl=tuple(kd), specularRefl=tuple(ks), shininess=n,
+                        type=MaterialType.Phong).finalize()
+
+
+def emitter(radiance) -> MaterialData:
+    return MaterialData(emission=tuple(radiance), type=MaterialType.FullDiffuse).finalize()
+
+
+def _emissive_prims(data: SceneData) -> list[int]:
+    """Emissive triangle list (the importer's rule, Utils/Importer.cpp:185-197):
+    every triangle whose mesh material is emissive, in primitive-ID order."""
+    out, base = [], 0
+    for m in data.meshes:
+        nt = len(m.index) // 3
+        if data.materials[m.materialID].isEmissive():
+            out += list(range(base, base + nt))
+        base += nt
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Workloads
+# ---------------------------------------------------------------------------
+def cornell_box(seed: int = SCENE_SEED, light_radiance=(17.0, 12.0, 4.0)):
+    """Cornell-style box: 5 wall quads (10 tris), two boxes rotated about y by
+    U[0, 45 deg] (24 tris), a 0.5 x 0.5 m ceiling light (2 tris).  Returns
+    (SceneData, camera kwargs)."""
+    rng = Stream(seed)
+    white, red, green = diffuse((0.73, 0.73, 0.73)), diffuse((0.63, 0.065, 0.05)), \
+        diffuse((0.14, 0.45, 0.091))
+    light = emitter(light_radiance)
+    mats = [white, red, green, light]
+    W, R, G, Lm = 0, 1, 2, 3
+    meshes = [
+        quad((-1, 0, 1), (2, 0, 0), (0, 0, -2), W),      # floor, +y
+        quad((-1, 2, -1), (2, 0, 0), (0, 0, 2), W),      # ceiling, -y
+        quad((-1, 0, -1), (2, 0, 0), (0, 2, 0), W),      # back wall, +z
+        quad((-1, 0, 1), (0, 0, -2), (0, 2, 0), R),      # left wall, +x
+        quad((1, 0, -1), (0, 0, 2), (0, 2, 0), G),       # right wall, -x
+    ]
+    a1 = math.radians(rng.uniform(0.0, 45.0))
+    a2 = math.radians(rng.uniform(0.0, 45.0))
+    meshes.append(box_mesh((0.35, 0.3, 0.3), (0.6, 0.6, 0.6), a1, W))
+    meshes.append(box_mesh((-0.35, 0.6, -0.35), (0.6, 1.2, 0.6), a2, W))
+    meshes.append(quad((-0.25, 1.98, -0.25), (0.5, 0, 0), (0, 0, 0.5), Lm))  # faces -y
+    data = SceneData(meshes=meshes, materials=mats)
+    data.emissiveIndex = _emissive_prims(data)
+    cam = dict(pos=(0.0, 1.0, 3.9), target=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0), fov=40.0,
+               size=(256, 256), spp=16, max_depth=8, rr_start_depth=3)
+    return data, cam
+
+
+def spheres_scene(n_spheres: int = 48, subdiv: int = 5, seed: int = SCENE_SEED,
+                  env=(0.05, 0.06, 0.08), light_radiance=(60.0, 60.0, 60.0)):
+    """About 1.0M triangles: 48 icospheres (subdivision 5) with radii U[0.2, 1]
+    and non-overlapping centres in a 20 x 20 x 5 m box, a 40 x 40 m ground
+    quad, a 2 x 2 m area light and a constant environment.  Principled
+    materials: baseColor U, roughness U[0.05, 1], metallic Bernoulli(0.3)."""
+    rng = Stream(seed)
+    mats = [diffuse((0.5, 0.5, 0.5)), emitter(light_radiance)]
+    meshes = [quad((-20, 0, 20), (40, 0, 0), (0, 0, -40), 0),
+              quad((-1, 10, -1), (2, 0, 0), (0, 0, 2), 1)]
+    placed: list[tuple[np.ndarray, float]] = []
+    while len(placed) < n_spheres:
+        r = rng.uniform(0.2, 1.0)
+        c = np.array([rng.uniform(-10, 10), rng.uniform(0, 5), rng.uniform(-10, 10)])
+        if any(np.linalg.norm(c - c2) < r + r2 for c2, r2 in placed):
+            continue
+        placed.append((c, r))
+        base = (rng.uniform(), rng.uniform(), rng.uniform())
+        rough = rng.uniform(0.05, 1.0)
+        metal = 1.0 if rng.uniform() < 0.3 else 0.0
+        mats.append(principled(base, rough, metal))
+        meshes.append(icosphere(c, r, subdiv, len(mats) - 1))
+    data = SceneData(meshes=meshes, materials=mats, envRadiance=env)
+    data.emissiveIndex = _emissive_prims(data)
+    cam = dict(pos=(0.0, 7.0, 24.0), target=(0.0, 1.5, 0.0), up=(0.0, 1.0, 0.0), fov=45.0,
+               size=(1920, 1080), spp=64, max_depth=16, rr_start_depth=3)
+    return data, cam
+
+
+def make_camera(cam: dict, sensor=None) -> Camera:
+    c = Camera()
+    c.setWorldPosition(cam["pos"])
+    c.setTarget(cam["target"])
+    c.setUp(cam["up"])
+    c.setFovY(cam["fov"])
+    c.calculateCameraData()
+    if sensor is not None:
+        c.setSensor(sensor)
+    return c
+
+
+# ---------------------------------------------------------------------------
+# Ray batches
+# ---------------------------------------------------------------------------
+def _normalize(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def offset_ray_origin(p: np.ndarray, n: np.ndarray) -> np.ndarray:
+    """offsetRayOrigin (Math/Ray.cu:11-22) on float32 arrays."""
+    p = p.astype(np.float32)
+    n = n.astype(np.float32)
+    of = (np.float32(256.0) * n).astype(np.int32)
+    pi = p.view(np.int32) + np.where(p < 0, -of, of)
+    pI = pi.astype(np.int32).view(np.float32)
+    near = np.abs(p) < np.float32(1.0 / 32.0)
+    return np.where(near, p + np.float32(1.0 / 65535.0) * n, pI).astype(np.float32)
+
+
+def primary_rays(cam: dict, n: int, seed: int = RAY_SEED, offset: int = 0) -> np.ndarray:
+    """n camera rays through uniformly drawn pixels + jitter."""
+    from . import _lib as L
+    W, H = cam["size"]
+    u = uniform01(seed, 2 * n, offset).reshape(n, 2)
+    px = (u[:, 0] * W).astype(np.float32)
+    py = (u[:, 1] * H).astype(np.float32)
+    c = make_camera(cam).getData()
+    U, V, Wv = (np.array(c.u, np.float32), np.array(c.v, np.float32),
+                np.array(c.w, np.float32))
+    hh = np.float32(math.tan(math.radians(cam["fov"] * 0.5)))
+    hw = np.float32(W) / np.float32(H) * hh
+    su = np.float32(2.0) * (px / np.float32(W)) - np.float32(1.0)
+    sv = np.float32(1.0) - np.float32(2.0) * (py / np.float32(H))
+    d = U[None] * (su * hw)[:, None] + V[None] * (sv * hh)[:, None] + Wv[None]
+    d = _normalize(d.astype(np.float64)).astype(np.float32)
+    rays = np.zeros(n, L.RAY_DTYPE)
+    rays["origin"] = np.array(c.pos, np.float32)
+    rays["dir"] = d
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    return rays
+
+
+def secondary_rays(data: SceneData, n: int, seed: int = RAY_SEED, offset: int = 0):
+    """n rays leaving surface points: area-weighted triangle pick, uniform
+    barycentric point (sampleBarycentric fold), origin offset along the face
+    normal, cosine-distributed direction about it."""
+    from . import _lib as L
+    P, I = data.concatenated()
+    tri = I.reshape(-1, 3)
+    v0, v1, v2 = (P[tri[:, k]].astype(np.float64) for k in range(3))
+    fn = np.cross(v1 - v0, v2 - v0)
+    area = 0.5 * np.linalg.norm(fn, axis=1)
+    cdf = np.cumsum(area)
+    cdf /= cdf[-1]
+    u = uniform01(seed, 5 * n, offset).reshape(n, 5)
+    t = np.minimum(np.searchsorted(cdf, u[:, 0], side="right"), len(cdf) - 1)
+    b1, b2 = u[:, 1], u[:, 2]
+    flip = b1 + b2 > 1.0
+    b1 = np.where(flip, 1.0 - b1, b1)
+    b2 = np.where(flip, 1.0 - b2, b2)
+    p = v0[t] * (1 - b1 - b2)[:, None] + v1[t] * b1[:, None] + v2[t] * b2[:, None]
+    nrm = _normalize(fn[t])
+    r = np.sqrt(u[:, 3])
+    phi = 2 * np.pi * u[:, 4]
+    lx, ly = r * np.cos(phi), r * np.sin(phi)
+    lz = np.sqrt(np.maximum(0.0, 1 - lx * lx - ly * ly))
+    a = np.where(np.abs(nrm[:, 1:2]) < 0.9, [[0, 1, 0]], [[1, 0, 0]])
+    tx = _normalize(np.cross(a, nrm))
+    ty = np.cross(nrm, tx)
+    d = _normalize(tx * lx[:, None] + ty * ly[:, None] + nrm * lz[:, None])
+    rays = np.zeros(n, L.RAY_DTYPE)
+    rays["origin"] = offset_ray_origin(p.astype(np.float32), nrm.astype(np.float32))
+    rays["dir"] = d.astype(np.float32)
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    return rays
+
+
+def mixed_batch(data: SceneData, cam: dict, n: int, seed: int = RAY_SEED) -> np.ndarray:
+    """n/2 primary + n/2 secondary rays (the closest-hit benchmark batch)."""
+    h = n // 2
+    return np.concatenate([primary_rays(cam, h, seed, 0),
+                           secondary_rays(data, n - h, seed, 2 * h)])

@@ -1,0 +1,277 @@
+/*
+ * crayhip.h - C-ABI of the MI355X rendering hot path (libcrayhip.so).
+ *
+ * Plain C, plain pointers and sizes.  Every entry point returns an int status
+ * (CRH_OK = 0); on failure crh_last_error() returns a thread-local message in
+ * the reference's shape ("HIP call <expr> failed with error <code> (<name>).").
+ *
+ * Ownership: opaque handles are created and destroyed by the caller through
+ * the *_create / *_destroy pairs.  Host arrays passed in are only read during
+ * the call.  Device memory is owned by the library.  Every call is
+ * synchronous at return.  One context per host thread.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (paths relative to the reference's Source/CRaystal/).
+ */
+#ifndef CRAYHIP_H
+#define CRAYHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define CRH_OK 0
+#define CRH_ERR_INVALID_ARG 1 /* CRAYSTAL_CHECK failures, Core/Error.h:121-125 */
+#define CRH_ERR_HIP 2         /* CRAYSTAL_CUDA_CHECK, Utils/CudaUtils.h:3-11 */
+#define CRH_ERR_OOM 3
+#define CRH_ERR_UNSUPPORTED 4
+#define CRH_ERR_INTERNAL 5
+
+#define CRH_INVALID_PRIM 0xFFFFFFFFu
+
+/* ---- opaque handles --------------------------------------------------- */
+typedef struct crh_context crh_context;
+typedef struct crh_bvh crh_bvh;
+typedef struct crh_scene crh_scene;
+typedef struct crh_film crh_film;
+typedef struct crh_buffer crh_buffer;
+
+/* ---- value types (layouts fixed; static-asserted in the library) ------ */
+
+/* Ray, 32 B.  Same layout as the reference Ray (Math/Ray.h:8-19). */
+typedef struct crh_ray {
+    float origin[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} crh_ray;
+
+/* Closest hit, 16 B.  prim = mesh primitive ID (HitInfo::primitiveIndex,
+ * Core/Hit.h:15-24), t = RayHit::hitT, (u, v) = HitInfo::barycentric.
+ * A miss is prim = CRH_INVALID_PRIM, t = +inf. */
+typedef struct crh_hit {
+    uint32_t prim;
+    float t;
+    float u;
+    float v;
+} crh_hit;
+
+/* BVH build parameters.  Reference constants NUM_BINS = 32,
+ * MIN_TRIS_PER_LEAF = 4 (used as count <= 4), MAX_DEPTH = 20
+ * (Core/BVH.h:104-106).  Zero fields take the reference value. */
+typedef struct crh_bvh_params {
+    uint32_t bins;
+    uint32_t max_leaf;
+    uint32_t max_depth;
+    uint32_t reserved;
+} crh_bvh_params;
+
+/* Canonical export of a reference BVHNode (Core/BVH.h:17-30): tag 0 = leaf
+ * {a = leafOffset, b = leafCount}, tag 1 = internal {a = left, b = right};
+ * nodes are in the reference's post-order, root last. 36 B. */
+typedef struct crh_bvh_node_ref {
+    uint32_t tag;
+    uint32_t a;
+    uint32_t b;
+    float pmin[3];
+    float pmax[3];
+} crh_bvh_node_ref;
+
+/* BVH build statistics (Core/BVH.h:64-69 BuildStats plus root data). */
+typedef struct crh_bvh_stats {
+    uint32_t node_count;
+    uint32_t max_depth;      /* deepest internal node, as the reference logs */
+    uint32_t max_leaf;       /* largest leaf primitive count */
+    uint32_t root;           /* root index in the post-order node array */
+    uint32_t prim_count;
+    uint32_t internal_count;
+    float root_min[3];
+    float root_max[3];
+    double build_ms;
+} crh_bvh_stats;
+
+/* Material types (Core/Material.h:12-16). */
+#define CRH_MAT_DIFFUSE 0u
+#define CRH_MAT_PRINCIPLED 1u
+#define CRH_MAT_PHONG 2u
+/* Material flags (Core/Material.h:26-30). */
+#define CRH_MATF_EMISSIVE 0x1u
+#define CRH_MATF_DIFFUSE 0x2u
+#define CRH_MATF_DIFFUSE_TEXTURE 0x4u
+
+/* Material, 68 B: the reference MaterialData with the RGB Spectrum
+ * (Core/Material.h:34-66).  flags are taken as given; MaterialData::finalize
+ * semantics are provided by crh_material_finalize. */
+typedef struct crh_material {
+    float diffuse[3];
+    uint32_t diffuse_tex;
+    float specular[3];
+    float transmittance[3];
+    float emission[3];
+    float shininess;
+    float ior;
+    uint32_t type;
+    uint32_t flags;
+} crh_material;
+
+/* One mesh (MeshData, Scene/Triangle.h:36-42).  Indices are local to the
+ * mesh; meshes are concatenated in order and the primitive ID is the global
+ * triangle index in that order (TriangleMeshManager, Scene/Triangle.cu:98-165).
+ * normal must hold nvert*3 floats; texcrd may be NULL (zeros). */
+typedef struct crh_mesh {
+    const float* position;
+    const float* normal;
+    const float* texcrd;
+    uint32_t nvert;
+    const uint32_t* index;
+    uint32_t nindex;
+    uint32_t material_id;
+} crh_mesh;
+
+/* SceneData (Scene/Scene.h:40-47) minus spheres/textures, plus the constant
+ * environment radiance added by this build (0 = the reference's black miss). */
+typedef struct crh_scene_desc {
+    const crh_mesh* meshes;
+    uint32_t nmesh;
+    const crh_material* materials;
+    uint32_t nmaterial;
+    const uint32_t* emissive_index;
+    uint32_t nemissive;
+    float env_radiance[3];
+    crh_bvh_params bvh;
+} crh_scene_desc;
+
+/* Camera: the reference CameraProxy layout, 96 B (Core/CameraProxy.h:10-24).
+ * u/v/w are filled by crh_camera_update (Camera::calculateCameraData,
+ * Core/Camera.cu:35-39). */
+typedef struct crh_camera {
+    float pos[3];
+    float fov_y;
+    float target[3];
+    float near_z;
+    float up[3];
+    float far_z;
+    float u[3];
+    uint32_t pad0;
+    float v[3];
+    uint32_t pad1;
+    float w[3];
+    uint32_t pad2;
+} crh_camera;
+
+/* Integrator modes. */
+#define CRH_MODE_MEGAKERNEL 0u
+#define CRH_MODE_WAVEFRONT 1u
+
+/* PathTraceIntegrator::Configs (Core/Integrator.h:14-18), plumbed to the
+ * device.  Russian roulette applies when depth >= rr_start_depth; the
+ * reference's hard-wired behaviour is max_depth 10, rr_threshold 0.05,
+ * rr_start_depth 6 (Core/Integrator.cu:209-223 tests depth > 5). */
+typedef struct crh_integrator_cfg {
+    uint32_t max_depth;
+    float rr_threshold;
+    uint32_t rr_start_depth;
+    uint32_t use_mis;
+    uint32_t mode;
+    uint32_t count_work; /* != 0: run the counting variant (fills
+                            node_visits / tri_tests; slower) */
+    uint32_t reserved[2];
+} crh_integrator_cfg;
+
+/* Timing/counters returned by the render and trace entry points. */
+typedef struct crh_stats {
+    double kernel_ms;      /* GPU time of the whole call (HIP events) */
+    double main_kernel_ms; /* summed GPU time of the dominant kernel's
+                              launches (per-launch HIP events) */
+    double total_ms;       /* host wall time of the call */
+    uint64_t rays;      /* rays traced, if counted (0 otherwise) */
+    uint64_t node_visits;
+    uint64_t tri_tests;
+    uint64_t launches;
+} crh_stats;
+
+/* ---- errors / device ------------------------------------------------- */
+const char* crh_last_error(void);
+int crh_device_count(int* count);
+int crh_version(uint32_t* version);
+
+/* ---- context (replaces the implicit CUDA runtime context) ------------- */
+int crh_context_create(int device, crh_context** out);
+int crh_context_destroy(crh_context* ctx);
+
+/* ---- device buffers (DeviceBuffer, Core/Buffer.h:13-44; size_t sizes) - */
+int crh_buffer_create(crh_context* ctx, size_t bytes, crh_buffer** out);
+int crh_buffer_upload(crh_buffer* buf, const void* src, size_t bytes);
+int crh_buffer_download(const crh_buffer* buf, void* dst, size_t bytes);
+int crh_buffer_memset(crh_buffer* buf, int value);
+int crh_buffer_size(const crh_buffer* buf, size_t* bytes);
+int crh_buffer_destroy(crh_buffer* buf);
+
+/* ---- BVH (BVH::build, Core/BVH.cu:88-338) ------------------------------ */
+/* Host binned-SAH build over one triangle soup; bit-exact with the
+ * reference's node array and primitive order for the same parameters. */
+int crh_bvh_build(const float* pos, uint32_t nvert, const uint32_t* idx,
+                  uint32_t ntri, const crh_bvh_params* params, crh_bvh** out);
+int crh_bvh_get_stats(const crh_bvh* bvh, crh_bvh_stats* stats);
+/* nodes: node_count entries; prim_order: prim_count entries
+ * (bvhToMeshPrimitiveIndex, Core/BVH.cu:322-338). */
+int crh_bvh_export(const crh_bvh* bvh, crh_bvh_node_ref* nodes,
+                   uint32_t* prim_order);
+int crh_bvh_destroy(crh_bvh* bvh);
+
+/* ---- materials --------------------------------------------------------- */
+/* MaterialData::finalize (Core/Material.h:57-65). */
+int crh_material_finalize(crh_material* mat);
+
+/* ---- scene (Scene::Scene, Scene/Scene.cu:61-78) ----------------------- */
+int crh_scene_create(crh_context* ctx, const crh_scene_desc* desc,
+                     crh_scene** out);
+int crh_scene_get_bvh_stats(const crh_scene* scene, crh_bvh_stats* stats);
+int crh_scene_export_bvh(const crh_scene* scene, crh_bvh_node_ref* nodes,
+                         uint32_t* prim_order);
+int crh_scene_destroy(crh_scene* scene);
+
+/* ---- camera ------------------------------------------------------------ */
+/* Camera::calculateCameraData (Core/Camera.cu:35-39). */
+int crh_camera_update(crh_camera* cam);
+
+/* ---- ray queries (SceneView::intersect / intersectOcclusion,
+ *      Scene/Scene.cu:5-29; BVHData::intersect, Core/BVH.cu:13-84) ------ */
+/* Host arrays in and out (includes PCIe copies). */
+int crh_trace_closest(crh_scene* scene, const crh_ray* rays, uint64_t n,
+                      crh_hit* hits);
+int crh_trace_any(crh_scene* scene, const crh_ray* rays, uint64_t n,
+                  uint8_t* occluded);
+/* Device-resident variants: rays/hits are crh_buffers already in HBM.
+ * stats (optional) receives the kernel time; count_work != 0 runs the
+ * counting variant and fills node_visits / tri_tests. */
+int crh_trace_closest_device(crh_scene* scene, const crh_buffer* rays,
+                             uint64_t n, crh_buffer* hits, int count_work,
+                             crh_stats* stats);
+int crh_trace_any_device(crh_scene* scene, const crh_buffer* rays, uint64_t n,
+                         crh_buffer* occluded, crh_stats* stats);
+
+/* ---- film (Sensor / SensorData, Core/Sensor.{h,cu}) -------------------- */
+int crh_film_create(crh_context* ctx, uint32_t width, uint32_t height,
+                    crh_film** out);
+int crh_film_clear(crh_film* film);
+/* sum: width*height*3 floats (Spectrum sums), weight: width*height floats.
+ * Either may be NULL. */
+int crh_film_readback(const crh_film* film, float* sum, float* weight);
+int crh_film_destroy(crh_film* film);
+
+/* ---- render (PathTraceIntegrator::dispatch, Core/Integrator.cu:233-262) */
+/* Renders sample indices [first_sample, first_sample + spp) into film. */
+int crh_render(crh_scene* scene, const crh_camera* cam,
+               const crh_integrator_cfg* cfg, uint32_t first_sample,
+               uint32_t spp, crh_film* film, crh_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRAYHIP_H */

@@ -1,0 +1,1243 @@
+// ORACLE - TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of the reference's rendering hot path (Hyiker/CRaystal),
+// used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+// the checker.  The product (libcrayhip.so) never links, loads or calls it.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the reference's Source/CRaystal/).  The reference's integrator, traversal,
+// shading and film are CUDA device code; there is no CPU render path in the
+// reference, so this file *is* "the reference's CPU path" for timing.
+//
+// Build: g++ -O2 -ffp-contract=off -fno-fast-math -fopenmp (oracle/Makefile).
+//
+// Parity pins (tests/test_oracle_*.py):
+//   Sampler            closed-form integer arithmetic + the reference's
+//                      property tests (Tests/CoreTest.cpp:130-281)
+//   Camera             reference KATs (Tests/CameraTest.cpp:16-50)
+//   offsetRayOrigin    hand KATs of the RTG ch.6 formula
+//   AABB / MT          textbook slab / Moller-Trumbore special cases, brute force
+//   BVH build          invariants + brute-force argmin of the SAH split
+//   Traversal          brute-force closest hit over all triangles
+//   BSDFs              closed forms (Lambert, Schlick, GGX normalisation)
+//   Integrator         furnace-style closed forms (emitter-only, white box)
+//   Film               tent-filter closed form, constant-image KAT
+// Parity unpinned at bit level: glm's exact operation order beyond the
+// camera KAT tolerance and CUDA libdevice transcendental results.
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../include/crayhip.h"
+#include "vecmath.h"
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Sampler (Core/Sampler.cu:4-31)
+// ---------------------------------------------------------------------------
+struct Sampler {
+    uint32_t state, inc;
+    Sampler(uint32_t px, uint32_t py, uint32_t sampleIndex) {
+        state = px * 2654435761u + py * 1597334677u + sampleIndex;  // :6
+        inc = px * 1234567891u + py * 987654321u + 1u;              // :7
+        for (int i = 0; i < 4; ++i) next();                         // :8-10
+    }
+    uint32_t next() {  // :21-27
+        uint32_t old = state;
+        state = old * 747796405u + inc;
+        old = ((old >> ((old >> 28u) + 4u)) ^ old) * 277803737u;
+        return (old >> 22u) ^ old;
+    }
+    float next1D() { return float(next()) * (1.0f / 4294967296.0f); }  // :29-31
+    V2 next2D() {  // :17-19 (x drawn before y)
+        float a = next1D();
+        float b = next1D();
+        return {a, b};
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Ray / offset (Math/Ray.h:8-19, Math/Ray.cu:11-22)
+// ---------------------------------------------------------------------------
+struct Ray {
+    V3 o;
+    float tMin = 0.f;
+    V3 d;
+    float tMax = kFltInf;
+};
+
+V3 offsetRayOrigin(V3 p, V3 n) {
+    const float kOrigin = 1.0f / 32.0f;
+    const float kFltScale = 1.0f / 65535.0f;
+    const float kIntScale = 256.0f;
+    int ox = static_cast<int>(kIntScale * n.x);
+    int oy = static_cast<int>(kIntScale * n.y);
+    int oz = static_cast<int>(kIntScale * n.z);
+    V3 pI(intAsFloat(floatAsInt(p.x) + ((p.x < 0) ? -ox : ox)),
+          intAsFloat(floatAsInt(p.y) + ((p.y < 0) ? -oy : oy)),
+          intAsFloat(floatAsInt(p.z) + ((p.z < 0) ? -oz : oz)));
+    return V3(std::fabs(p.x) < kOrigin ? p.x + kFltScale * n.x : pI.x,
+              std::fabs(p.y) < kOrigin ? p.y + kFltScale * n.y : pI.y,
+              std::fabs(p.z) < kOrigin ? p.z + kFltScale * n.z : pI.z);
+}
+
+// ---------------------------------------------------------------------------
+// AABB (Math/AABB.h:12-138, Math/AABB.cu:6-46)
+// ---------------------------------------------------------------------------
+struct AABB {
+    V3 pMin{kFltInf, kFltInf, kFltInf};
+    V3 pMax{-kFltInf, -kFltInf, -kFltInf};
+    void include(V3 p) {
+        pMin = vmin(pMin, p);
+        pMax = vmax(pMax, p);
+    }
+    void include(const AABB& b) {
+        pMin = vmin(pMin, b.pMin);
+        pMax = vmax(pMax, b.pMax);
+    }
+    V3 diagonal() const { return pMax - pMin; }
+};
+
+bool aabbIntersect(const AABB& box, const Ray& ray, float& hitT) {
+    float tMin = ray.tMin;
+    float tMax = ray.tMax;
+    for (int i = 0; i < 3; ++i) {
+        if (std::fabs(box.pMax[i] - box.pMin[i]) < kFltEps) {  // :12
+            if (std::fabs(ray.d[i]) < kFltEps) {                // :13
+                if (std::fabs(ray.o[i] - box.pMin[i]) > kFltEps) return false;
+                continue;
+            }
+            float t = (box.pMin[i] - ray.o[i]) / ray.d[i];  // :19
+            tMin = fmax_std(tMin, t);
+            tMax = fmin_std(tMax, t);
+        } else {
+            float invD = 1.0f / ray.d[i];  // :23
+            float t0 = (box.pMin[i] - ray.o[i]) * invD;
+            float t1 = (box.pMax[i] - ray.o[i]) * invD;
+            if (invD < 0.0f) std::swap(t0, t1);
+            tMin = fmax_std(tMin, t0);
+            tMax = fmin_std(tMax, t1);
+        }
+        if (tMin > tMax) return false;  // :35
+    }
+    if (tMin > ray.tMax || tMax < ray.tMin) return false;  // :40
+    hitT = tMin;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Scene storage (TriangleMeshManager, Scene/Triangle.cu:98-165;
+// MaterialManager, Core/Material.cu:15-30)
+// ---------------------------------------------------------------------------
+struct NodeRef {  // BVHNode (Core/BVH.h:17-30), canonical export layout
+    uint32_t tag, a, b;
+    AABB bounds;
+};
+
+struct BVHOut {
+    std::vector<NodeRef> nodes;
+    std::vector<uint32_t> primOrder;  // bvhToMeshPrimitiveIndex
+    AABB rootBounds;
+    uint32_t root = 0;
+    uint32_t maxDepth = 0, maxLeaf = 0;
+    double buildMs = 0;
+};
+
+struct Scene {
+    std::vector<V3> pos, nrm;
+    std::vector<V2> uv;
+    std::vector<uint32_t> idx;       // global (offset) vertex indices
+    std::vector<uint32_t> primMesh;  // primitive -> mesh
+    std::vector<uint32_t> meshMat;   // mesh -> material id
+    std::vector<crh_material> mats;
+    std::vector<uint32_t> emissive;
+    Spec env;
+    BVHOut bvh;
+};
+
+// ---------------------------------------------------------------------------
+// BVH build (Core/BVH.cu:88-338)
+// ---------------------------------------------------------------------------
+struct BuildTri {  // Core/BVH.h:73-78
+    V3 centroid;
+    AABB bounds;
+    uint32_t triIndex;
+};
+
+struct Builder {
+    std::vector<BuildTri> tris;
+    std::vector<NodeRef> nodes;
+    uint32_t numBins = 32, maxLeaf = 4, maxDepthCap = 20;
+    uint32_t statMaxDepth = 0, statMaxLeaf = 0;
+
+    // calcAABBSurfaceArea (:284-287): float products and sums, then * 2.0.
+    static float surfaceArea(const AABB& b) {
+        V3 d = b.diagonal();
+        return float((d.x * d.y + d.x * d.z + d.y * d.z) * 2.0);
+    }
+    // evaluateSAH (:289-301)
+    static float sah(const AABB& node, const AABB& l, const AABB& r, int nl,
+                     int nr) {
+        const float traversalCost = 1.0f, intersectionCost = 1.0f;
+        float lsa = surfaceArea(l), rsa = surfaceArea(r),
+              root = surfaceArea(node);
+        return traversalCost +
+               intersectionCost * (nl * lsa + nr * rsa) / root;
+    }
+
+    // findBestSplit (:214-282)
+    void findBestSplit(const std::vector<uint32_t>& ind, uint32_t start,
+                       uint32_t count, const AABB& nb, uint32_t& bestAxis,
+                       float& bestSplit) {
+        float bestCost = kFltInf;
+        bestAxis = 0;
+        bestSplit = 0.0f;
+        const int NB = int(numBins);
+        std::vector<AABB> binB(NB), lb(NB - 1), rb(NB - 1);
+        std::vector<int> binC(NB), lc(NB - 1), rc(NB - 1);
+        for (uint32_t axis = 0; axis < 3; ++axis) {
+            std::fill(binB.begin(), binB.end(), AABB());
+            std::fill(binC.begin(), binC.end(), 0);
+            V3 ext = nb.diagonal();
+            float scale = float(NB) / ext[axis];  // :226
+            for (uint32_t i = start; i < start + count; ++i) {
+                const BuildTri& t = tris[ind[i]];
+                int b = std::min(
+                    NB - 1,
+                    cvt_trunc_x86((t.centroid[axis] - nb.pMin[axis]) * scale));
+                b = std::clamp(b, 0, NB - 1);  // :235
+                binB[b].include(t.bounds);
+                binC[b]++;
+            }
+            AABB cur;
+            int cc = 0;
+            for (int i = 0; i < NB - 1; ++i) {  // :241-246
+                cur.include(binB[i]);
+                cc += binC[i];
+                lb[i] = cur;
+                lc[i] = cc;
+            }
+            AABB curR;
+            int cr = 0;
+            for (int i = NB - 1; i > 0; --i) {  // :250-255
+                curR.include(binB[i]);
+                cr += binC[i];
+                rb[i - 1] = curR;
+                rc[i - 1] = cr;
+            }
+            for (int i = 0; i < NB - 1; ++i) {  // :258-271
+                float splitPos =
+                    nb.pMin[axis] + float(i + 1) * ext[axis] / float(NB);
+                float cost = sah(nb, lb[i], rb[i], lc[i], rc[i]);
+                if (cost < bestCost) {
+                    bestCost = cost;
+                    bestAxis = axis;
+                    bestSplit = splitPos;
+                }
+            }
+        }
+    }
+
+    // buildRecursive (:150-212)
+    uint32_t build(std::vector<uint32_t>& ind, uint32_t start, uint32_t count,
+                   uint32_t depth, AABB& nb) {
+        nb = AABB();
+        for (uint32_t i = start; i < start + count; ++i)
+            nb.include(tris[ind[i]].bounds);
+        NodeRef node;
+        node.bounds = nb;
+        if (count <= maxLeaf || depth >= maxDepthCap) {
+            node.tag = 0;
+            node.a = start;
+            node.b = count;
+            nodes.push_back(node);
+            statMaxLeaf = std::max(statMaxLeaf, count);
+            return uint32_t(nodes.size() - 1);
+        }
+        uint32_t axis;
+        float split;
+        findBestSplit(ind, start, count, nb, axis, split);
+        // libstdc++ std::partition (bidirectional two-pointer algorithm).
+        auto mid = std::partition(
+            ind.begin() + start, ind.begin() + start + count,
+            [&](uint32_t id) { return tris[id].centroid[axis] < split; });
+        uint32_t leftCount = uint32_t(std::distance(ind.begin() + start, mid));
+        if (leftCount == 0 || leftCount == count) {  // :186-195
+            node.tag = 0;
+            node.a = start;
+            node.b = count;
+            nodes.push_back(node);
+            statMaxLeaf = std::max(statMaxLeaf, count);
+            return uint32_t(nodes.size() - 1);
+        }
+        AABB lb, rb;
+        uint32_t l = build(ind, start, leftCount, depth + 1, lb);
+        uint32_t r = build(ind, start + leftCount, count - leftCount,
+                           depth + 1, rb);
+        node.tag = 1;
+        node.a = l;
+        node.b = r;
+        nodes.push_back(node);
+        statMaxDepth = std::max(statMaxDepth, depth);
+        return uint32_t(nodes.size() - 1);
+    }
+};
+
+// BVH::build + gatherTriangles (:88-148) over a global triangle soup.
+BVHOut buildBVH(const std::vector<V3>& pos, const std::vector<uint32_t>& idx,
+                uint32_t bins, uint32_t maxLeaf, uint32_t maxDepth) {
+    auto t0 = std::chrono::steady_clock::now();
+    Builder b;
+    b.numBins = bins ? bins : 32;
+    b.maxLeaf = maxLeaf ? maxLeaf : 4;
+    b.maxDepthCap = maxDepth ? maxDepth : 20;
+    size_t ntri = idx.size() / 3;
+    b.tris.resize(ntri);
+    for (size_t i = 0; i < ntri; ++i) {
+        V3 v0 = pos[idx[3 * i]], v1 = pos[idx[3 * i + 1]],
+           v2 = pos[idx[3 * i + 2]];
+        BuildTri& t = b.tris[i];
+        t.centroid = (v0 + v1 + v2) / 3.0f;  // :138
+        t.bounds = AABB();
+        t.bounds.include(v0);
+        t.bounds.include(v1);
+        t.bounds.include(v2);
+        t.triIndex = uint32_t(i);
+    }
+    std::vector<uint32_t> ind(ntri);
+    std::iota(ind.begin(), ind.end(), 0u);
+    b.nodes.reserve(2 * ntri);
+    BVHOut out;
+    out.root = b.build(ind, 0, uint32_t(ntri), 0, out.rootBounds);
+    out.nodes = std::move(b.nodes);
+    out.primOrder.resize(ntri);
+    for (size_t i = 0; i < ntri; ++i) out.primOrder[i] = b.tris[ind[i]].triIndex;
+    out.maxDepth = b.statMaxDepth;
+    out.maxLeaf = b.statMaxLeaf;
+    out.buildMs = std::chrono::duration<double, std::milli>(
+                      std::chrono::steady_clock::now() - t0)
+                      .count();
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Triangle (Scene/Triangle.cu:9-96), intersectShape (Scene/Shape.h:17-37)
+// ---------------------------------------------------------------------------
+struct Vtx {
+    V3 p, n;
+    V2 uv;
+};
+struct Tri {
+    Vtx v[3];
+    V3 faceNormal() const {  // :9-16
+        V3 e1 = v[1].p - v[0].p;
+        V3 e2 = v[2].p - v[0].p;
+        return normalize(cross(e1, e2));
+    }
+    Vtx interpolate(V3 w) const {  // :18-31
+        Vtx r;
+        r.p = v[0].p * w[0] + v[1].p * w[1] + v[2].p * w[2];
+        r.n = v[0].n * w[0] + v[1].n * w[1] + v[2].n * w[2];
+        r.uv = V2(v[0].uv.x * w[0] + v[1].uv.x * w[1] + v[2].uv.x * w[2],
+                  v[0].uv.y * w[0] + v[1].uv.y * w[1] + v[2].uv.y * w[2]);
+        return r;
+    }
+    float area() const {  // :33-37
+        V3 a = v[0].p - v[1].p;
+        V3 b = v[2].p - v[1].p;
+        return length(cross(a, b)) / 2.f;
+    }
+};
+
+Tri getTriangle(const Scene& s, uint32_t id) {  // :78-92
+    Tri t;
+    for (int i = 0; i < 3; ++i) {
+        uint32_t vi = s.idx[3 * id + i];
+        t.v[i].p = s.pos[vi];
+        t.v[i].n = s.nrm[vi];
+        t.v[i].uv = s.uv[vi];
+    }
+    return t;
+}
+
+// Moller-Trumbore (Scene/Triangle.cu:40-75).
+bool triIntersect(const Scene& s, uint32_t id, const Ray& ray, float& u,
+                  float& v, float& t) {
+    V3 v0 = s.pos[s.idx[3 * id]], v1 = s.pos[s.idx[3 * id + 1]],
+       v2 = s.pos[s.idx[3 * id + 2]];
+    V3 e1 = v1 - v0;
+    V3 e2 = v2 - v0;
+    V3 p = cross(ray.d, e2);
+    float det = dot(e1, p);
+    if (std::fabs(det) < 1e-6f) return false;
+    float inv = 1.0f / det;
+    V3 tv = ray.o - v0;
+    u = dot(tv, p) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    V3 q = cross(tv, e1);
+    v = dot(ray.d, q) * inv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = dot(e2, q) * inv;
+    if (t <= 0.0) return false;
+    return true;
+}
+
+struct RayHit {
+    Ray ray;
+    V2 bary;
+    uint32_t prim = CRH_INVALID_PRIM;
+    float hitT = kFltInf;  // Core/Hit.h:29
+};
+
+// intersectShape (Scene/Shape.h:17-37).
+inline bool intersectShape(const Scene& s, uint32_t id, RayHit& rh) {
+    float u = 0, v = 0, t = kFltInf;
+    bool hit = triIntersect(s, id, rh.ray, u, v, t);
+    if (!hit) return false;  // an unset hitT only ever leads to `false`
+    if (t < rh.ray.tMin || t > rh.ray.tMax) return false;
+    if (t < rh.hitT) {
+        rh.bary = V2(u, v);
+        rh.prim = id;
+        rh.hitT = t;
+    }
+    return true;
+}
+
+struct Counters {
+    uint64_t nodes = 0, tris = 0;
+};
+
+// BVHData::intersect (Core/BVH.cu:13-84).  anyHit returns at the first
+// reported hit; before that it visits exactly the nodes the closest-hit
+// traversal visits, so the boolean equals intersectOcclusion's.
+bool bvhIntersect(const Scene& s, RayHit& rh, Counters* cnt, bool anyHit) {
+    const BVHOut& b = s.bvh;
+    if (b.nodes.empty()) return false;
+    float dummy;
+    if (!aabbIntersect(b.rootBounds, rh.ray, dummy)) return false;
+    struct Entry {
+        uint32_t node;
+        float tMin;
+    };
+    Entry stack[64];
+    int sp = 0;
+    stack[sp++] = {b.root, 0.0f};
+    bool isHit = false;
+    float closestT = rh.hitT;
+    while (sp > 0) {
+        Entry e = stack[--sp];
+        const NodeRef& node = b.nodes[e.node];
+        if (e.tMin > closestT) continue;
+        if (node.tag == 0) {
+            for (uint32_t i = 0; i < node.b; ++i) {
+                uint32_t prim = b.primOrder[node.a + i];
+                if (cnt) cnt->tris++;
+                if (intersectShape(s, prim, rh)) {
+                    isHit = true;
+                    closestT = fmin_std(closestT, rh.hitT);
+                    if (anyHit) return true;
+                }
+            }
+        } else {
+            if (cnt) cnt->nodes++;
+            const NodeRef& L = b.nodes[node.a];
+            const NodeRef& R = b.nodes[node.b];
+            float lt = kFltInf, rt = kFltInf;
+            bool hl = aabbIntersect(L.bounds, rh.ray, lt);
+            bool hr = aabbIntersect(R.bounds, rh.ray, rt);
+            if (hl && hr) {
+                if (lt > rt) {
+                    if (lt < closestT) stack[sp++] = {node.a, lt};
+                    if (rt < closestT) stack[sp++] = {node.b, rt};
+                } else {
+                    if (rt < closestT) stack[sp++] = {node.b, rt};
+                    if (lt < closestT) stack[sp++] = {node.a, lt};
+                }
+            } else if (hl && lt < closestT) {
+                stack[sp++] = {node.a, lt};
+            } else if (hr && rt < closestT) {
+                stack[sp++] = {node.b, rt};
+            }
+        }
+    }
+    return isHit;
+}
+
+// ---------------------------------------------------------------------------
+// Frame / Intersection (Core/Frame.cu:6-31, Core/Intersection.cu:4-10,
+// Scene/Scene.cu:31-58)
+// ---------------------------------------------------------------------------
+struct Frame {
+    V3 T{1, 0, 0}, B{0, 1, 0}, N{0, 0, 1};
+    static V3 tangentSafe(V3 n) {  // :13-20
+        V3 t(0, 1, 0);
+        if (1.0 - double(absDot(n, t)) < double(kEps)) t = V3(1, 0, 0);
+        V3 bt = normalize(cross(n, t));
+        return normalize(cross(bt, n));
+    }
+    explicit Frame(V3 n) {
+        N = n;
+        T = tangentSafe(n);
+        B = cross(N, T);
+    }
+    Frame() = default;
+    V3 toLocal(V3 w) const {  // mat3(T.x,B.x,N.x, T.y,...) * w
+        return V3(T.x * w.x + T.y * w.y + T.z * w.z,
+                  B.x * w.x + B.y * w.y + B.z * w.z,
+                  N.x * w.x + N.y * w.y + N.z * w.z);
+    }
+    V3 toWorld(V3 l) const {  // mat3(T, B, N) * l
+        return V3(T.x * l.x + B.x * l.y + N.x * l.z,
+                  T.y * l.x + B.y * l.y + N.y * l.z,
+                  T.z * l.x + B.z * l.y + N.z * l.z);
+    }
+};
+
+struct Isect {
+    V3 posW, faceN, viewW;
+    V2 uv;
+    bool front;
+    Frame frame;
+    V3 orientedFaceNormal() const { return front ? faceN : -faceN; }
+};
+
+Isect createIntersection(const Scene& s, const RayHit& rh) {
+    Isect it;
+    it.posW = rh.ray.o + rh.hitT * rh.ray.d;
+    it.viewW = -normalize(rh.ray.d);
+    V3 w(1.f - rh.bary.x - rh.bary.y, rh.bary.x, rh.bary.y);  // Hit.h:20-23
+    Tri tri = getTriangle(s, rh.prim);
+    it.faceN = tri.faceNormal();
+    Vtx vd = tri.interpolate(w);
+    V3 sn = normalize(vd.n);
+    it.uv = vd.uv;
+    it.front = dot(it.viewW, it.faceN) > 0.0;
+    it.frame = Frame(it.front ? sn : -sn);
+    return it;
+}
+
+// ---------------------------------------------------------------------------
+// Sampling warps (Math/Sampling.cu:4-58)
+// ---------------------------------------------------------------------------
+V2 sampleBarycentric(V2 u) {
+    float u1 = u.x, u2 = u.y;
+    bool flip = (u1 + u2) > 1.0f;
+    u1 = flip ? (1.0f - u1) : u1;
+    u2 = flip ? (1.0f - u2) : u2;
+    return {u1, u2};
+}
+V3 cosineSampleHemisphere(V2 u) {  // :39-47
+    float r = std::sqrt(u.x);
+    float phi = 2.0f * kPi * u.y;
+    float x = r * std::cos(phi);
+    float y = r * std::sin(phi);
+    float z = std::sqrt(fmax_std(0.0f, 1.0f - x * x - y * y));
+    return {x, y, z};
+}
+float cosineHemispherePdf(V3 d) { return fmax_std(0.0f, d.z) * kInvPi; }
+float powerHeuristic(int nf, float fPdf, int ng, float gPdf) {  // :53-58
+    float f = nf * fPdf, g = ng * gPdf;
+    return (f * f) / (f * f + g * g);
+}
+float acosSafe(float v) {  // Math/CRayMath.cu:11-14
+    float lo = kEps, hi = float(1.0 - kEps);
+    float c = v < lo ? lo : (hi < v ? hi : v);
+    return std::acos(c);
+}
+
+// ---------------------------------------------------------------------------
+// BSDFs (Material/*.cu)
+// ---------------------------------------------------------------------------
+struct BSDF {
+    uint32_t type = CRH_MAT_DIFFUSE;
+    Spec kd, ks;
+    float n = 1.f;
+    // Phong (PhongBRDF.cu:10-19)
+    float pD = 0, pS = 0;
+    // Principled (PrincipledBRDF.cu:47-58)
+    float rough = 0, metal = 0;
+    Frame frame;
+
+    void init(const crh_material& m, const Frame& f) {  // BSDF.cu:6-32
+        frame = f;
+        type = m.type;
+        kd = Spec(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+        ks = Spec(m.specular[0], m.specular[1], m.specular[2]);
+        n = m.shininess;
+        if (type == CRH_MAT_PHONG) {
+            pD = fmax_std(kd.averageValue(), 0.0f);
+            pS = fmax_std(ks.averageValue(), 0.0f);
+            float sum = fmax_std(pD + pS, kEps);
+            pD /= sum;
+            pS /= sum;
+        } else if (type == CRH_MAT_PRINCIPLED) {
+            float r = std::sqrt(2.0f / (n + 1.0f));
+            float mt = ks.averageValue();
+            rough = std::clamp(r, 0.0f, 1.0f);
+            metal = std::clamp(mt, 0.0f, 1.0f);
+            float sp = 0.1f * (1.0f - metal) + 1.0f * metal;  // lerp(0.1,1,m)
+            pD = 1.0f;
+            pS = sp;
+            pD /= (pD + pS);
+            pS /= (pD + pS);  // uses the updated pD (reference behaviour)
+        }
+    }
+
+    // --- Phong (PhongBRDF.cu:21-68)
+    Spec phongEval(V3 wo, V3 wi) const {
+        V3 r = reflect(-wo, V3(0, 0, 1));
+        return kd * kInvPi + ks * (n + 2) * kInv2Pi *
+                                 std::pow(fmax_std(dot(r, wi), 0.0f), n);
+    }
+    float phongPdf(V3 wo, V3 wi) const {
+        float dp = cosineHemispherePdf(wi);
+        V3 r = reflect(-wo, V3(0, 0, 1));
+        float ca = fmax_std(dot(r, wi), 0.0f);
+        float sp = (n + 1) * kInv2Pi * std::pow(ca, n);
+        return dp * pD + sp * pS;
+    }
+    V3 phongSample(V3 wo, V2 u, float& pdf) const {
+        V2 u0 = u;
+        V3 wi(0.f);
+        pdf = 0.0f;
+        bool hasD = pD > 0.0f, hasS = pS > 0.0f;
+        if (hasD && u0.x < pD) {
+            u0.x /= pD;
+            wi = cosineSampleHemisphere(u0);
+            pdf = cosineHemispherePdf(wi);
+        } else if (hasS) {
+            u0.x = (u0.x - pD) / pS;
+            float alpha = acosSafe(std::pow(u0.x, 1 / (n + 1)));
+            float th = acosSafe(wo.z) + alpha;
+            float hiPi = float(kPi / 2.0);
+            th = th < 0.0f ? 0.0f : (hiPi < th ? hiPi : th);
+            float phi = k2Pi * u0.y;
+            float st = std::sin(th);
+            wi = V3(st * std::cos(phi), st * std::sin(phi),
+                    float(std::sqrt(1.0 - double(st * st))));
+            pdf = (n + 1) * kInv2Pi * std::pow(std::cos(alpha), n);
+        }
+        return wi;
+    }
+
+    // --- Principled (PrincipledBRDF.cu:9-151)
+    static float ggxD(float r, float ch) {  // :9-18
+        float a = r * r;
+        float a2 = a * a;
+        float c2 = ch * ch;
+        float den = float(1.0 + (double(a2) - 1.0) * double(c2));
+        return a2 * kInvPi / (den * den);
+    }
+    static float smithG1(float aG, float cv) {  // :20-24
+        float a = aG * aG;
+        float b = cv * cv;
+        return 1 / (cv + std::sqrt(a + b - a * b));
+    }
+    static float smithG(float r, float ci, float cv) {  // :26-33
+        float a = float(0.5 + double(r) * 0.5);
+        a *= a;
+        return smithG1(a, ci) * smithG1(a, cv);
+    }
+    Spec prDiffuse(float cv, float ci, float cd) const {  // :135-141
+        float fd90 = float(0.5 + 2.0 * double(rough) * double(cd) * double(cd));
+        float fl = float(1.0 + (double(fd90) - 1.0) * std::pow(1.0 - double(ci), 5.0));
+        float fv = float(1.0 + (double(fd90) - 1.0) * std::pow(1.0 - double(cv), 5.0));
+        return kd * kInvPi * fv * fl;
+    }
+    Spec prSpecular(float cv, float ci, float ch, float cd) const {  // :143-150
+        float D = ggxD(rough, ch);
+        Spec f0 = kd * (1.0f - metal) + Spec(0.04f) * metal;
+        float v0 = 1.f - cd;
+        float v1 = v0 * v0;
+        Spec F = f0 + (Spec(1.f) - f0) * v1 * v1 * v0;  // Fresnel.cu:3-8
+        float G = smithG(rough, ci, cv);
+        float den = float(4.0 * double(ci) * double(cv));
+        return D * F * G / den;
+    }
+    Spec prEval(V3 wo, V3 wi) const {  // :57-73
+        if (wo.z <= 0.0 || wi.z <= 0.0) return Spec(0.f);
+        V3 wh = normalize(wo + wi);
+        float cv = wo.z, ci = wi.z, ch = wh.z;
+        float cd = dot(wo, wh);
+        Spec d = prDiffuse(cv, ci, cd) * (1.0f - metal);
+        Spec sp = prSpecular(cv, ci, ch, cd);
+        return d + sp;
+    }
+    float prPdf(V3 wo, V3 wi) const {  // :75-98
+        if (wo.z <= 0.0 || wi.z <= 0.0) return 0.0f;
+        V3 wh = normalize(wo + wi);
+        float ch = wh.z;
+        float cd = dot(wo, wh);
+        if (cd <= 0.0f) return 0.0f;
+        float dp = cosineHemispherePdf(wi);
+        float sp = 0.0f;
+        if (ch > 0.0f) {
+            float D = ggxD(rough, ch);
+            sp = D * ch / (4.0f * cd);
+        }
+        return dp * pD + sp * pS;
+    }
+    V3 prSample(V3 wo, V2 u, float& pdf) const {  // :100-133
+        if (wo.z <= 0.0) {
+            pdf = 0.0f;
+            return V3(0.0f);
+        }
+        float u1 = u.x, u2 = u.y;
+        V3 wi;
+        if (u1 < pS) {
+            u1 /= pS;
+            float alpha = rough * rough;
+            float a2 = alpha * alpha;
+            float phi = u2 * k2Pi;
+            float tan2 = a2 * u1 / (1 - u1);
+            float ct = 1 / std::sqrt(1 + tan2);
+            float r = std::sqrt(fmax_std(1 - ct * ct, 0.0f));
+            V3 wh(std::cos(phi) * r, std::sin(phi) * r, ct);
+            wi = reflect(-wo, wh);
+        } else {
+            u1 = (u1 - pS) / (1.0f - pS);
+            wi = cosineSampleHemisphere(V2(u1, u2));
+        }
+        pdf = prPdf(wo, wi);
+        return wi;
+    }
+
+    // --- dispatch in local space (BSDFBase, BSDF.h:15-58)
+    Spec evalLocal(V3 wo, V3 wi) const {
+        Spec f;
+        if (type == CRH_MAT_DIFFUSE) f = kd * kInvPi;
+        else if (type == CRH_MAT_PRINCIPLED) f = prEval(wo, wi);
+        else f = phongEval(wo, wi);
+        return f * std::fabs(wi.z);
+    }
+    float pdfLocal(V3 wo, V3 wi) const {
+        if (type == CRH_MAT_DIFFUSE) return cosineHemispherePdf(wi);
+        if (type == CRH_MAT_PRINCIPLED) return prPdf(wo, wi);
+        return phongPdf(wo, wi);
+    }
+    V3 sampleLocal(V3 wo, V2 u, float& pdf) const {
+        if (type == CRH_MAT_DIFFUSE) {
+            V3 wi = cosineSampleHemisphere(u);
+            pdf = cosineHemispherePdf(wi);
+            return wi;
+        }
+        if (type == CRH_MAT_PRINCIPLED) return prSample(wo, u, pdf);
+        return phongSample(wo, u, pdf);
+    }
+
+    // --- world-space wrappers (BSDF.cu:51-103)
+    Spec evaluate(V3 wo, V3 wi) const {
+        return evalLocal(frame.toLocal(wo), frame.toLocal(wi));
+    }
+    float evaluatePdf(V3 wo, V3 wi) const {
+        return pdfLocal(frame.toLocal(wo), frame.toLocal(wi));
+    }
+    Spec sampleEvaluate(Sampler& smp, V3 wo, V3& wi, float& pdf) const {
+        V3 woL = frame.toLocal(wo);
+        pdf = 0.0f;
+        V2 u = smp.next2D();
+        V3 wiL = sampleLocal(woL, u, pdf);
+        Spec f = evalLocal(woL, wiL);
+        wi = frame.toWorld(wiL);
+        return f;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Camera (Core/Camera.cu:35-39, Core/CameraProxy.h:25-40)
+// ---------------------------------------------------------------------------
+void cameraUpdate(crh_camera& c) {
+    V3 pos(c.pos[0], c.pos[1], c.pos[2]);
+    V3 tgt(c.target[0], c.target[1], c.target[2]);
+    V3 up(c.up[0], c.up[1], c.up[2]);
+    V3 W = normalize(tgt - pos);
+    V3 U = normalize(cross(W, up));
+    V3 V = normalize(cross(U, W));
+    for (int i = 0; i < 3; ++i) {
+        c.w[i] = W[i];
+        c.u[i] = U[i];
+        c.v[i] = V[i];
+    }
+}
+
+Ray generateRay(const crh_camera& c, uint32_t W, uint32_t H, V2 sp) {
+    float u = (2.0f * (sp.x / float(W)) - 1.0f);
+    float v = (1.0f - 2.0f * (sp.y / float(H)));
+    float aspect = float(W) / float(H);
+    // fovY * 0.5 and glm::radians are double; tan in double.
+    float hh = float(std::tan(double(c.fov_y) * 0.5 *
+                              0.01745329251994329576923690768489));
+    float hw = aspect * hh;
+    V3 U(c.u[0], c.u[1], c.u[2]), Vv(c.v[0], c.v[1], c.v[2]),
+        Wv(c.w[0], c.w[1], c.w[2]);
+    V3 dir = normalize(U * (u * hw) + Vv * (v * hh) + Wv);
+    Ray r;
+    r.o = V3(c.pos[0], c.pos[1], c.pos[2]);
+    r.d = dir;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Integrator (Core/Integrator.cu:13-227)
+// ---------------------------------------------------------------------------
+struct LightSample {
+    Spec weight;
+    V3 dirW;
+    float pdf = 0;
+    uint32_t emissiveID = CRH_INVALID_PRIM;
+    float lightSelectPdf = 0.0f;
+};
+
+struct Tally {
+    uint64_t rays = 0;
+};
+
+inline const crh_material& matOf(const Scene& s, uint32_t prim) {
+    return s.mats[s.meshMat[s.primMesh[prim]]];
+}
+inline Spec emissionOf(const crh_material& m) {
+    return Spec(m.emission[0], m.emission[1], m.emission[2]);
+}
+
+bool intersect(const Scene& s, RayHit& rh, Tally& tl) {
+    tl.rays++;
+    return bvhIntersect(s, rh, nullptr, false);
+}
+bool occluded(const Scene& s, const Ray& r, Tally& tl) {
+    tl.rays++;
+    RayHit rh;
+    rh.ray = r;
+    return bvhIntersect(s, rh, nullptr, true);
+}
+
+LightSample sampleLight(const Scene& s, const Isect& it, const BSDF& bsdf,
+                        Sampler& smp, Tally& tl) {  // :21-72
+    int cnt = int(s.emissive.size());
+    if (cnt == 0) return {};
+    int sel = std::min<int>(int(float(cnt) * smp.next1D()), cnt - 1);
+    uint32_t eid = s.emissive[sel];
+    Tri tri = getTriangle(s, eid);
+    V2 b = sampleBarycentric(smp.next2D());
+    Vtx vd = tri.interpolate(V3(1.f - b.x - b.y, b.x, b.y));
+    V3 target = offsetRayOrigin(vd.p, tri.faceNormal());
+    V3 toLight = target - it.posW;
+    float d2 = dot(toLight, toLight);
+    float dist = std::sqrt(d2);
+    V3 ld = toLight / dist;
+    if (dot(it.orientedFaceNormal(), ld) < 0.0 || dot(vd.n, ld) > 0.0 ||
+        dist < kEps)
+        return {};
+    Ray sr;
+    sr.o = offsetRayOrigin(it.posW, it.orientedFaceNormal());
+    sr.d = ld;
+    sr.tMax = length(sr.o - target) - kEps;
+    if (occluded(s, sr, tl)) return {};
+    const crh_material& m = matOf(s, eid);
+    LightSample ls;
+    ls.weight = bsdf.evaluate(it.viewW, ld) * emissionOf(m);
+    ls.dirW = ld;
+    ls.pdf = d2 / (tri.area() * absDot(ld, vd.n));
+    ls.emissiveID = eid;
+    ls.lightSelectPdf = float(1.0 / cnt);
+    return ls;
+}
+
+Spec evalMIS(const Scene& s, const Isect& it, const BSDF& bsdf, Sampler& smp,
+             Tally& tl) {  // :74-140
+    Spec value(0.0f);
+    LightSample ls = sampleLight(s, it, bsdf, smp, tl);
+    if (ls.pdf <= 0.0) return value;
+    {
+        float lightPdf = ls.pdf;
+        if (lightPdf > 0.0) {
+            float bsdfPdf = bsdf.evaluatePdf(it.viewW, ls.dirW);
+            if (bsdfPdf > 0.0)
+                value = value + powerHeuristic(1, lightPdf, 1, bsdfPdf) *
+                                    ls.weight / lightPdf;
+        }
+    }
+    {
+        V3 dir;
+        float bsdfPdf = 0.0f;
+        Spec bw = bsdf.sampleEvaluate(smp, it.viewW, dir, bsdfPdf);
+        if (bw.maxValue() > 0.0 && bsdfPdf > 0.0) {
+            RayHit rh;
+            rh.ray.o = offsetRayOrigin(it.posW, it.orientedFaceNormal());
+            rh.ray.d = dir;
+            if (intersect(s, rh, tl) && rh.prim == ls.emissiveID) {
+                Isect lit = createIntersection(s, rh);
+                const crh_material& m = matOf(s, rh.prim);
+                if (m.flags & CRH_MATF_EMISSIVE) {
+                    Spec em = emissionOf(m);
+                    V3 toLight = lit.posW - it.posW;
+                    float d2 = dot(toLight, toLight);
+                    V3 lv = toLight / std::sqrt(d2);
+                    float cl = absDot(lv, lit.faceN);
+                    if (cl != 0.0) {
+                        float area = getTriangle(s, rh.prim).area();
+                        float lightPdf = d2 / (area * cl);
+                        value = value + powerHeuristic(1, bsdfPdf, 1, lightPdf) *
+                                            em * bw / bsdfPdf;
+                    }
+                }
+            }
+        }
+    }
+    return value / ls.lightSelectPdf;
+}
+
+// pathTraceKernel body (:142-227) for one (pixel, sample index).
+Spec tracePath(const Scene& s, const crh_camera& cam, uint32_t W, uint32_t H,
+               const crh_integrator_cfg& cfg, uint32_t x, uint32_t y,
+               uint32_t sampleIndex, V2& pixelOut, Tally& tl) {
+    Sampler smp(x, y, sampleIndex);
+    V2 j = smp.next2D();
+    V2 pixel(float(x) + j.x, float(y) + j.y);
+    pixelOut = pixel;
+    Ray ray = generateRay(cam, W, H, pixel);
+    Spec radiance(0.f), beta(1.f);
+    const float rr = cfg.rr_threshold;
+    for (uint32_t depth = 0; depth < cfg.max_depth; ++depth) {
+        RayHit rh;
+        rh.ray = ray;
+        bool terminate = false;
+        if (intersect(s, rh, tl)) {
+            Isect it = createIntersection(s, rh);
+            const crh_material& m = matOf(s, rh.prim);
+            if (m.flags & CRH_MATF_EMISSIVE) {
+                radiance = radiance + emissionOf(m) * beta;
+                break;
+            }
+            BSDF bsdf;
+            bsdf.init(m, it.frame);
+            if (cfg.use_mis) {
+                radiance = radiance + evalMIS(s, it, bsdf, smp, tl) * beta;
+            } else {
+                LightSample ls = sampleLight(s, it, bsdf, smp, tl);
+                if (ls.pdf > 0.0)
+                    radiance = radiance +
+                               ls.weight * beta / ls.pdf / ls.lightSelectPdf;
+            }
+            V3 wi;
+            float pdf = 0.0f;
+            Spec f = bsdf.sampleEvaluate(smp, it.viewW, wi, pdf);
+            if (pdf <= 0.0f) break;
+            beta = beta * (f / pdf);
+            ray.o = offsetRayOrigin(it.posW, it.orientedFaceNormal());
+            ray.d = wi;
+        } else {
+            // Constant environment (this build's addition; zero = reference).
+            radiance = radiance + s.env * beta;
+            terminate = true;
+        }
+        if (smp.next1D() < rr) terminate = true;
+        if (terminate) break;
+        if (depth >= cfg.rr_start_depth) {  // reference: depth > 5 (start 6)
+            float q = fmax_std(rr, 1.0f - beta.maxValue());
+            q = std::clamp(q, 0.0f, 1.0f);
+            if (smp.next1D() < q) break;
+            beta = beta / (1.0f - q);
+        }
+    }
+    return radiance;
+}
+
+// SensorData::addSample (Core/Sensor.cu:19-43), one sample, fixed order.
+void splat(float* sum, float* wsum, uint32_t W, uint32_t H, const Spec& L,
+           V2 xy) {
+    if (L.hasNaN() || L.hasInf()) return;
+    int x0 = int(std::floor(xy.x - 1.0f)), y0 = int(std::floor(xy.y - 1.0f));
+    int x1 = int(std::ceil(xy.x + 1.0f)), y1 = int(std::ceil(xy.y + 1.0f));
+    x0 = std::max(x0, 0);
+    y0 = std::max(y0, 0);
+    x1 = std::min(x1, int(W) - 1);
+    y1 = std::min(y1, int(H) - 1);
+    for (int y = y0; y <= y1; ++y)
+        for (int x = x0; x <= x1; ++x) {
+            float dx = xy.x - (float(x) + 0.5f);
+            float dy = xy.y - (float(y) + 0.5f);
+            float w = fmax_std(0.0f, 1.0f - std::fabs(dx)) *
+                      fmax_std(0.0f, 1.0f - std::fabs(dy));
+            size_t id = size_t(W) * y + x;
+            for (int c = 0; c < 3; ++c) sum[3 * id + c] += L.v[c] * w;
+            wsum[id] += w;
+        }
+}
+
+Scene* makeScene(const crh_scene_desc& d) {
+    Scene* s = new Scene();
+    for (uint32_t m = 0; m < d.nmesh; ++m) {
+        const crh_mesh& mesh = d.meshes[m];
+        uint32_t voff = uint32_t(s->pos.size());
+        for (uint32_t i = 0; i < mesh.nvert; ++i) {
+            s->pos.push_back(V3(mesh.position[3 * i], mesh.position[3 * i + 1],
+                                mesh.position[3 * i + 2]));
+            s->nrm.push_back(V3(mesh.normal[3 * i], mesh.normal[3 * i + 1],
+                                mesh.normal[3 * i + 2]));
+            s->uv.push_back(mesh.texcrd ? V2(mesh.texcrd[2 * i], mesh.texcrd[2 * i + 1])
+                                        : V2(0.f, 0.f));
+        }
+        for (uint32_t i = 0; i < mesh.nindex; ++i)
+            s->idx.push_back(mesh.index[i] + voff);
+        for (uint32_t i = 0; i < mesh.nindex / 3; ++i) s->primMesh.push_back(m);
+        s->meshMat.push_back(mesh.material_id);
+    }
+    s->mats.assign(d.materials, d.materials + d.nmaterial);
+    s->emissive.assign(d.emissive_index, d.emissive_index + d.nemissive);
+    s->env = Spec(d.env_radiance[0], d.env_radiance[1], d.env_radiance[2]);
+    s->bvh = buildBVH(s->pos, s->idx, d.bvh.bins, d.bvh.max_leaf,
+                      d.bvh.max_depth);
+    return s;
+}
+
+void fillStats(const BVHOut& b, crh_bvh_stats* st) {
+    std::memset(st, 0, sizeof(*st));
+    st->node_count = uint32_t(b.nodes.size());
+    st->max_depth = b.maxDepth;
+    st->max_leaf = b.maxLeaf;
+    st->root = b.root;
+    st->prim_count = uint32_t(b.primOrder.size());
+    uint32_t ni = 0;
+    for (auto& n : b.nodes) ni += n.tag == 1;
+    st->internal_count = ni;
+    for (int i = 0; i < 3; ++i) {
+        st->root_min[i] = b.rootBounds.pMin[i];
+        st->root_max[i] = b.rootBounds.pMax[i];
+    }
+    st->build_ms = b.buildMs;
+}
+
+void exportNodes(const BVHOut& b, crh_bvh_node_ref* nodes, uint32_t* prims) {
+    if (nodes)
+        for (size_t i = 0; i < b.nodes.size(); ++i) {
+            nodes[i].tag = b.nodes[i].tag;
+            nodes[i].a = b.nodes[i].a;
+            nodes[i].b = b.nodes[i].b;
+            for (int k = 0; k < 3; ++k) {
+                nodes[i].pmin[k] = b.nodes[i].bounds.pMin[k];
+                nodes[i].pmax[k] = b.nodes[i].bounds.pMax[k];
+            }
+        }
+    if (prims) std::copy(b.primOrder.begin(), b.primOrder.end(), prims);
+}
+
+inline Ray toRay(const crh_ray& r) {
+    Ray o;
+    o.o = V3(r.origin[0], r.origin[1], r.origin[2]);
+    o.tMin = r.tmin;
+    o.d = V3(r.dir[0], r.dir[1], r.dir[2]);
+    o.tMax = r.tmax;
+    return o;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+// ---- primitives -----------------------------------------------------------
+void orc_sampler_seq(uint32_t x, uint32_t y, uint32_t idx, uint32_t n,
+                     float* out, uint32_t* raw) {
+    Sampler s(x, y, idx);
+    Sampler s2(x, y, idx);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (out) out[i] = s.next1D();
+        if (raw) raw[i] = s2.next();
+    }
+}
+
+void orc_camera_update(crh_camera* c) { cameraUpdate(*c); }
+
+void orc_camera_ray(const crh_camera* c, uint32_t W, uint32_t H, float px,
+                    float py, crh_ray* out) {
+    Ray r = generateRay(*c, W, H, V2(px, py));
+    for (int i = 0; i < 3; ++i) {
+        out->origin[i] = r.o[i];
+        out->dir[i] = r.d[i];
+    }
+    out->tmin = r.tMin;
+    out->tmax = r.tMax;
+}
+
+void orc_offset_ray_origin(const float* p, const float* n, float* out) {
+    V3 r = offsetRayOrigin(V3(p[0], p[1], p[2]), V3(n[0], n[1], n[2]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+int orc_aabb_intersect(const float* pmin, const float* pmax, const crh_ray* r,
+                       float* t) {
+    AABB b;
+    b.pMin = V3(pmin[0], pmin[1], pmin[2]);
+    b.pMax = V3(pmax[0], pmax[1], pmax[2]);
+    float ht = kFltInf;
+    bool h = aabbIntersect(b, toRay(*r), ht);
+    *t = ht;
+    return h;
+}
+
+// ---- BVH ------------------------------------------------------------------
+void* orc_bvh_build(const float* pos, uint32_t nvert, const uint32_t* idx,
+                    uint32_t ntri, uint32_t bins, uint32_t maxLeaf,
+                    uint32_t maxDepth) {
+    std::vector<V3> p(nvert);
+    for (uint32_t i = 0; i < nvert; ++i)
+        p[i] = V3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    std::vector<uint32_t> id(idx, idx + size_t(ntri) * 3);
+    return new BVHOut(buildBVH(p, id, bins, maxLeaf, maxDepth));
+}
+void orc_bvh_stats(void* h, crh_bvh_stats* st) { fillStats(*(BVHOut*)h, st); }
+void orc_bvh_export(void* h, crh_bvh_node_ref* nodes, uint32_t* prims) {
+    exportNodes(*(BVHOut*)h, nodes, prims);
+}
+void orc_bvh_free(void* h) { delete (BVHOut*)h; }
+
+// ---- scene ----------------------------------------------------------------
+void* orc_scene_create(const crh_scene_desc* d) { return makeScene(*d); }
+void orc_scene_free(void* s) { delete (Scene*)s; }
+void orc_scene_bvh_stats(void* s, crh_bvh_stats* st) {
+    fillStats(((Scene*)s)->bvh, st);
+}
+void orc_scene_bvh_export(void* s, crh_bvh_node_ref* nodes, uint32_t* prims) {
+    exportNodes(((Scene*)s)->bvh, nodes, prims);
+}
+
+// Closest hit per ray; node_visits / tri_tests (optional) per ray.
+void orc_trace_closest(void* sp, const crh_ray* rays, uint64_t n,
+                       crh_hit* hits, uint32_t* node_visits,
+                       uint32_t* tri_tests, int nthreads) {
+    const Scene& s = *(Scene*)sp;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < int64_t(n); ++i) {
+        RayHit rh;
+        rh.ray = toRay(rays[i]);
+        Counters c;
+        bvhIntersect(s, rh, &c, false);
+        hits[i].prim = rh.prim;
+        hits[i].t = rh.hitT;
+        hits[i].u = rh.prim == CRH_INVALID_PRIM ? 0.f : rh.bary.x;
+        hits[i].v = rh.prim == CRH_INVALID_PRIM ? 0.f : rh.bary.y;
+        if (node_visits) node_visits[i] = uint32_t(c.nodes);
+        if (tri_tests) tri_tests[i] = uint32_t(c.tris);
+    }
+}
+
+void orc_trace_any(void* sp, const crh_ray* rays, uint64_t n, uint8_t* occ,
+                   int nthreads) {
+    const Scene& s = *(Scene*)sp;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < int64_t(n); ++i) {
+        RayHit rh;
+        rh.ray = toRay(rays[i]);
+        occ[i] = bvhIntersect(s, rh, nullptr, true) ? 1 : 0;
+    }
+}
+
+// Brute force over every triangle in primitive-ID order (the pin for the
+// traversal: same t; same ID except on exact t ties).
+void orc_trace_brute(void* sp, const crh_ray* rays, uint64_t n, crh_hit* hits) {
+    const Scene& s = *(Scene*)sp;
+    uint32_t ntri = uint32_t(s.idx.size() / 3);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < int64_t(n); ++i) {
+        RayHit rh;
+        rh.ray = toRay(rays[i]);
+        for (uint32_t t = 0; t < ntri; ++t) intersectShape(s, t, rh);
+        hits[i].prim = rh.prim;
+        hits[i].t = rh.hitT;
+        hits[i].u = rh.prim == CRH_INVALID_PRIM ? 0.f : rh.bary.x;
+        hits[i].v = rh.prim == CRH_INVALID_PRIM ? 0.f : rh.bary.y;
+    }
+}
+
+// ---- BSDF in local space (golden vectors) -----------------------------------
+// out_f[3] = evaluate (incl. |cos|), *out_pdf = evaluatePdf.
+void orc_bsdf_eval_local(const crh_material* m, const float* wo,
+                         const float* wi, float* out_f, float* out_pdf) {
+    BSDF b;
+    b.init(*m, Frame());
+    V3 o(wo[0], wo[1], wo[2]), i(wi[0], wi[1], wi[2]);
+    Spec f = b.evalLocal(o, i);
+    for (int k = 0; k < 3; ++k) out_f[k] = f.v[k];
+    *out_pdf = b.pdfLocal(o, i);
+}
+// Sample with explicit u; returns wi, f (evaluate at wi) and the sample pdf.
+void orc_bsdf_sample_local(const crh_material* m, const float* wo,
+                           const float* u, float* out_wi, float* out_f,
+                           float* out_pdf) {
+    BSDF b;
+    b.init(*m, Frame());
+    V3 o(wo[0], wo[1], wo[2]);
+    float pdf = 0.f;
+    V3 wi = b.sampleLocal(o, V2(u[0], u[1]), pdf);
+    Spec f = b.evalLocal(o, wi);
+    for (int k = 0; k < 3; ++k) {
+        out_wi[k] = wi[k];
+        out_f[k] = f.v[k];
+    }
+    *out_pdf = pdf;
+}
+
+void orc_film_splat(float* sum, float* wsum, uint32_t W, uint32_t H,
+                    const float* L, float px, float py) {
+    splat(sum, wsum, W, H, Spec(L[0], L[1], L[2]), V2(px, py));
+}
+
+// Per-sample radiance of one pass (sample index `si`) for every pixel.
+// L: W*H*3 floats.  Returns rays traced.
+uint64_t orc_render_samples(void* sp, const crh_camera* cam, uint32_t W,
+                            uint32_t H, const crh_integrator_cfg* cfg,
+                            uint32_t si, float* L, float* pix, int nthreads) {
+    const Scene& s = *(Scene*)sp;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    uint64_t rays = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : rays)
+    for (int64_t y = 0; y < int64_t(H); ++y) {
+        Tally tl;
+        for (uint32_t x = 0; x < W; ++x) {
+            V2 p;
+            Spec r = tracePath(s, *cam, W, H, *cfg, x, uint32_t(y), si, p, tl);
+            size_t id = size_t(W) * y + x;
+            for (int c = 0; c < 3; ++c) L[3 * id + c] = r.v[c];
+            if (pix) {
+                pix[2 * id] = p.x;
+                pix[2 * id + 1] = p.y;
+            }
+        }
+        rays += tl.rays;
+    }
+    return rays;
+}
+
+// Full render of sample indices [first, first + spp) into (sum, wsum), which
+// the caller zeroes.  Deterministic: per-pass radiance is computed in
+// parallel, then splatted in row-major pixel order.
+uint64_t orc_render(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
+                    const crh_integrator_cfg* cfg, uint32_t first, uint32_t spp,
+                    float* sum, float* wsum, int nthreads) {
+    std::vector<float> L(size_t(W) * H * 3), P(size_t(W) * H * 2);
+    uint64_t rays = 0;
+    for (uint32_t i = 0; i < spp; ++i) {
+        rays += orc_render_samples(sp, cam, W, H, cfg, first + i, L.data(),
+                                   P.data(), nthreads);
+        for (size_t id = 0; id < size_t(W) * H; ++id)
+            splat(sum, wsum, W, H, Spec(L[3 * id], L[3 * id + 1], L[3 * id + 2]),
+                  V2(P[2 * id], P[2 * id + 1]));
+    }
+    return rays;
+}
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"
