@@ -142,6 +142,17 @@ def main():
                      "tri_tests_per_sample": round(tris_ps, 3)},
     }
 
+    # wavefront integrator on the same workload and timed steps
+    wf_cfg = cr.Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
+    wf = cr.PathTraceIntegrator(wf_cfg)
+    wf.dispatch(scene, 1, first_sample=0, readback=False)
+    t0 = time.perf_counter()
+    wst = wf.dispatch(scene, a.steps, first_sample=a.warmup, readback=False)
+    wel = time.perf_counter() - t0
+    out["wavefront"] = {"msamples_per_s": round(samples / wel / 1e6, 3),
+                        "ms_per_step": round(wel / a.steps * 1e3, 3),
+                        "grays_per_s": round(wst["rays"] / wel / 1e9, 4)}
+
     rays = None
     if not a.no_trace:
         n = a.trace_rays

@@ -44,7 +44,7 @@ def build(verbose: bool = True) -> Path:
         o = OBJ / (src + ".o")
         _run(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
         objs.append(str(o))
-    for src in ("kernels.hip",):
+    for src in ("kernels.hip", "wavefront.hip"):
         o = OBJ / (src + ".o")
         _run([str(ROCM / "bin" / "hipcc"), *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
         objs.append(str(o))
@@ -55,6 +55,17 @@ def build(verbose: bool = True) -> Path:
     return OUT
 
 
+def build_cpp_tests() -> Path:
+    """Compile tests/cpp/api_test.cpp against include/ and libcrayhip.so."""
+    root = PKG.parent
+    out = root / "tests" / "cpp" / "_build" / "api_test"
+    out.parent.mkdir(parents=True, exist_ok=True)
+    _run(["g++", "-O1", "-std=c++17", f"-I{root / 'include'}", str(root / "tests" / "cpp" / "api_test.cpp"),
+          f"-L{PKG}", "-lcrayhip", f"-Wl,-rpath,{PKG}", "-o", str(out)])
+    return out
+
+
 if __name__ == "__main__":
     build()
+    build_cpp_tests()
     sys.exit(0)

@@ -23,6 +23,7 @@
 #include "../../include/crayhip.h"
 #include "bvh_build.h"
 #include "kernels.h"
+#include "wavefront.h"
 
 static_assert(sizeof(crh_ray) == 32, "crh_ray layout");
 static_assert(sizeof(crh_hit) == 16, "crh_hit layout");
@@ -136,7 +137,67 @@ struct crh_film {
     crh_context* ctx = nullptr;
     uint32_t w = 0, h = 0;
     DevMem sum, wsum, L, P;
+    // Wavefront path state, allocated on first wavefront render.
+    DevMem wfArena;
+    crh::WfState wf{};
+    bool wfReady = false;
 };
+
+namespace {
+
+// Carves the wavefront structure-of-arrays state out of one allocation.
+hipError_t ensureWavefront(crh_film* f) {
+    if (f->wfReady) return hipSuccess;
+    const size_t n = size_t(f->w) * f->h;
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t v4 = align(n * 16), u1 = align(n * 4);
+    const size_t total = 16 * v4 + 9 * u1 + align(64) + align(64);
+    hipError_t e = f->wfArena.alloc(total);
+    if (e != hipSuccess) return e;
+    char* base = f->wfArena.as<char>();
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base + off;
+        off += bytes;
+        return p;
+    };
+    crh::WfState& w = f->wf;
+    w.width = f->w;
+    w.height = f->h;
+    w.rayO = (float4*)take(v4);
+    w.rayD = (float4*)take(v4);
+    w.hit = (uint4*)take(v4);
+    w.beta = (float4*)take(v4);
+    w.shO = (float4*)take(v4);
+    w.shD = (float4*)take(v4);
+    w.lgt = (float4*)take(v4);
+    w.lgn = (float4*)take(v4);
+    w.misO = (float4*)take(v4);
+    w.misD = (float4*)take(v4);
+    w.misHit = (uint4*)take(v4);
+    w.misVal = (float4*)take(v4);
+    w.misBw = (float4*)take(v4);
+    w.misPos = (float4*)take(v4);
+    w.misBeta = (float4*)take(v4);
+    take(v4);  // spare
+    w.rng = (uint32_t*)take(u1);
+    w.flags = (uint32_t*)take(u1);
+    w.lgId = (uint32_t*)take(u1);
+    w.qExt[0] = (uint32_t*)take(u1);
+    w.qExt[1] = (uint32_t*)take(u1);
+    w.qMis[0] = (uint32_t*)take(u1);
+    w.qMis[1] = (uint32_t*)take(u1);
+    w.qShadow = (uint32_t*)take(u1);
+    w.qCont = (uint32_t*)take(u1);
+    w.counters = (uint32_t*)take(align(64));
+    w.work = (unsigned long long*)take(align(64));
+    w.rad = f->L.as<float4>();
+    w.P = f->P.as<float2>();
+    f->wfReady = true;
+    return hipSuccess;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -653,7 +714,8 @@ int crh_render(crh_scene* s, const crh_camera* cam,
                uint32_t spp, crh_film* film, crh_stats* stats) {
     ARG_CHECK(s && cam && cfg && film, "NULL argument");
     ARG_CHECK(film->ctx == s->ctx, "film and scene belong to different contexts");
-    ARG_CHECK(cfg->mode == CRH_MODE_MEGAKERNEL, "only the megakernel mode is available");
+    ARG_CHECK(cfg->mode == CRH_MODE_MEGAKERNEL || cfg->mode == CRH_MODE_WAVEFRONT,
+              "unknown integrator mode");
     ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
     GUARD_BEGIN
     crh_context* ctx = s->ctx;
@@ -691,13 +753,23 @@ int crh_render(crh_scene* s, const crh_camera* cam,
         evs.resize(size_t(spp) * 2, nullptr);
         for (auto& e : evs) HIP_TRY(hipEventCreate(&e));
     }
+    const bool wavefront = cfg->mode == CRH_MODE_WAVEFRONT;
+    if (wavefront) {
+        HIP_TRY(ensureWavefront(film));
+        HIP_TRY(hipMemsetAsync(film->wf.work, 0, 64, ctx->stream));
+    }
+    crh::WfLaunch wl;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     for (uint32_t i = 0; i < spp; ++i) {
         if (stats) HIP_TRY(hipEventRecord(evs[2 * i], ctx->stream));
-        HIP_TRY(crh::launchRenderPass(s->view, cv, iv, first_sample + i,
-                                      film->L.as<float4>(), film->P.as<float2>(),
-                                      counter.as<unsigned long long>(),
-                                      cfg->count_work != 0, ctx->stream));
+        if (wavefront)
+            HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, film->wf, wl,
+                                       cfg->count_work != 0, ctx->stream));
+        else
+            HIP_TRY(crh::launchRenderPass(s->view, cv, iv, first_sample + i,
+                                          film->L.as<float4>(), film->P.as<float2>(),
+                                          counter.as<unsigned long long>(),
+                                          cfg->count_work != 0, ctx->stream));
         if (stats) HIP_TRY(hipEventRecord(evs[2 * i + 1], ctx->stream));
         HIP_TRY(crh::launchFilmGather(film->L.as<float4>(), film->P.as<float2>(),
                                       film->w, film->h, film->sum.as<float>(),
@@ -715,14 +787,16 @@ int crh_render(crh_scene* s, const crh_camera* cam,
             mainMs += m;
         }
         unsigned long long c[3] = {0, 0, 0};
-        HIP_TRY(hipMemcpy(c, counter.p, sizeof(c), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(c, wavefront ? (void*)film->wf.work : counter.p, sizeof(c),
+                          hipMemcpyDeviceToHost));
         stats->kernel_ms = ms;
         stats->main_kernel_ms = mainMs;
         stats->total_ms = nowMs() - t0;
         stats->rays = c[0];
         stats->node_visits = c[1];
         stats->tri_tests = c[2];
-        stats->launches = uint64_t(spp) * 2;
+        stats->launches = wavefront ? uint64_t(spp) * (6u * cfg->max_depth + 5u)
+                                    : uint64_t(spp) * 2;
     }
     return CRH_OK;
     GUARD_END

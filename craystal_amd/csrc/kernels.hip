@@ -4,7 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
-#include "shading.h"
+#include "integrator.h"
 
 namespace crh {
 
@@ -64,7 +64,7 @@ struct PathCtx {
 template <bool COUNT>
 CRH_D bool closest(PathCtx<COUNT>& c, const Ray& r, Hit& h) {
     c.rays++;
-    return traverse<false, COUNT>(c.s, r, h, c.stack, c.stride, c.wc);
+    return closestHit<COUNT>(c.s, r, h, c.stack, c.stride, c.wc);
 }
 template <bool COUNT>
 CRH_D bool occluded(PathCtx<COUNT>& c, const Ray& r) {
@@ -73,96 +73,30 @@ CRH_D bool occluded(PathCtx<COUNT>& c, const Ray& r) {
     return traverse<true, COUNT>(c.s, r, h, c.stack, c.stride, c.wc);
 }
 
-struct LightSample {
-    Spec weight;
-    F3 dir;
-    float pdf;
-    uint32_t id;
-    float selPdf;
-};
-
 // sampleLight (Core/Integrator.cu:21-72)
 template <bool COUNT>
-CRH_D LightSample sampleLight(PathCtx<COUNT>& c, const Isect& it, const BSDF& bsdf,
-                              Sampler& smp) {
-    LightSample ls;
-    ls.weight = sp(0.f);
-    ls.dir = mk3(0, 0, 0);
-    ls.pdf = 0.f;
-    ls.id = 0xFFFFFFFFu;
-    ls.selPdf = 0.f;
-    const int cnt = int(c.s.nEmissive);
-    if (cnt == 0) return ls;
-    int sel = int(float(cnt) * smp.next1D());
-    sel = sel < cnt - 1 ? sel : cnt - 1;
-    uint32_t eid = c.s.emissive[sel];
-    TriData tri = getTri(c.s, eid);
-    F2 bc = sampleBary(smp.next2D());
-    F3 w = mk3(1.f - bc.x - bc.y, bc.x, bc.y);
-    F3 lp = interp3(tri.p, w);
-    F3 ln = interp3(tri.n, w);
-    F3 target = offsetRayOrigin(lp, faceNormal(tri));
-    F3 toL = target - it.pos;
-    float d2 = dot(toL, toL);
-    float dist = sqrtf(d2);
-    F3 ld = toL / dist;
-    F3 on = oriented(it);
-    if (dot(on, ld) < 0.0f || dot(ln, ld) > 0.0f || dist < kEps) return ls;
-    Ray sr;
-    sr.o = offsetRayOrigin(it.pos, on);
-    sr.tMin = 0.f;
-    sr.d = ld;
-    sr.tMax = length(sr.o - target) - kEps;
-    if (occluded(c, sr)) return ls;
-    const GMaterial& m = c.s.mats[c.s.primMat[eid]];
-    ls.weight = bsdfEval(bsdf, it.view, ld) * sp3(m.emission);
-    ls.dir = ld;
-    ls.pdf = d2 / (triArea(tri) * absDot(ld, ln));
-    ls.id = eid;
-    ls.selPdf = float(1.0 / double(cnt));
-    return ls;
+CRH_D LightSample sampleLight(PathCtx<COUNT>& c, const Isect& it,
+                              const BSDF& bsdf, Sampler& smp) {
+    LightGeom g = lightGeom(c.s, it, smp);
+    if (!g.valid) return noLight();
+    if (occluded(c, g.shadow)) return noLight();
+    return lightFinish(c.s, g, bsdf, it);
 }
 
 // evalMIS (Core/Integrator.cu:74-140)
 template <bool COUNT>
 CRH_D Spec evalMIS(PathCtx<COUNT>& c, const Isect& it, const BSDF& bsdf,
                    Sampler& smp) {
-    Spec value = sp(0.f);
     LightSample ls = sampleLight(c, it, bsdf, smp);
-    if (ls.pdf <= 0.0f) return value;
-    {
-        float lp = ls.pdf;
-        float bp = bsdfPdf(bsdf, it.view, ls.dir);
-        if (bp > 0.0f) value = value + powerHeur(lp, bp) * ls.weight / lp;
-    }
-    {
-        F3 dir;
-        float bp = 0.0f;
-        Spec bw = bsdfSampleEval(bsdf, smp, it.view, dir, bp);
-        if (smaxv(bw) > 0.0f && bp > 0.0f) {
-            Ray r;
-            r.o = offsetRayOrigin(it.pos, oriented(it));
-            r.tMin = 0.f;
-            r.d = dir;
-            r.tMax = __builtin_inff();
-            Hit h;
-            if (closest(c, r, h) && h.prim == ls.id) {
-                Isect lit = makeIsect(c.s, r, h);
-                const GMaterial& m = c.s.mats[c.s.primMat[h.prim]];
-                if (m.flags & 1u) {
-                    F3 toL = lit.pos - it.pos;
-                    float d2 = dot(toL, toL);
-                    F3 lv = toL / sqrtf(d2);
-                    float cl = absDot(lv, lit.faceN);
-                    if (cl != 0.0f) {
-                        float area = triArea(getTri(c.s, h.prim));
-                        float lpdf = d2 / (area * cl);
-                        value = value + powerHeur(bp, lpdf) * sp3(m.emission) *
-                                            bw / bp;
-                    }
-                }
-            }
-        }
+    if (ls.pdf <= 0.0f) return sp(0.f);
+    Spec value = misLightPart(ls, bsdf, it);
+    Ray r;
+    Spec bw;
+    float bp;
+    if (misBsdfRay(bsdf, smp, it, r, bw, bp)) {
+        Hit h;
+        bool isHit = closest(c, r, h);
+        misBsdfPart(c.s, it.pos, r, isHit, h, ls.id, bw, bp, value);
     }
     return value / ls.selPdf;
 }
@@ -193,13 +127,7 @@ __global__ __launch_bounds__(kRenderBlock) void kRenderPass(
         Sampler smp(x, y, si);
         F2 j = smp.next2D();
         F2 pix{float(x) + j.x, float(y) + j.y};
-        // CameraProxy::generateRay (Core/CameraProxy.h:25-40)
-        float u = (2.0f * (pix.x / float(cam.width)) - 1.0f);
-        float v = (1.0f - 2.0f * (pix.y / float(cam.height)));
-        float hw = cam.aspect * cam.hh;
-        F3 dir = normalize(ld3(cam.u) * (u * hw) + ld3(cam.v) * (v * cam.hh) +
-                           ld3(cam.w));
-        Ray ray{ld3(cam.pos), 0.f, dir, __builtin_inff()};
+        Ray ray = cameraRay(cam, pix);
         Spec rad = sp(0.f), beta = sp(1.f);
         for (uint32_t depth = 0; depth < cfg.maxDepth; ++depth) {
             bool terminate = false;
@@ -232,12 +160,7 @@ __global__ __launch_bounds__(kRenderBlock) void kRenderPass(
             }
             if (smp.next1D() < cfg.rrThreshold) terminate = true;
             if (terminate) break;
-            if (depth >= cfg.rrStartDepth) {
-                float q = smax(cfg.rrThreshold, 1.0f - smaxv(beta));
-                q = q < 0.0f ? 0.0f : (1.0f < q ? 1.0f : q);
-                if (smp.next1D() < q) break;
-                beta = beta / (1.0f - q);
-            }
+            if (!roulette(cfg, depth, smp, beta)) break;
         }
         size_t id = size_t(y) * cam.width + x;
         L[id] = make_float4(rad.r, rad.g, rad.b, 0.f);

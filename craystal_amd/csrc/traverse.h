@@ -133,4 +133,15 @@ CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, uint2* stack,
     return isHit;
 }
 
+// SceneView::intersect as the integrator uses it.  A query that reports a
+// hit without recording a primitive (only possible when every comparison
+// sees a NaN, e.g. a NaN ray direction) is treated as a miss: the reference
+// would go on to read an uninitialised primitive index.
+template <bool COUNT>
+CRH_D bool closestHit(const SceneView& s, const Ray& ray, Hit& hit,
+                      uint2* stack, uint32_t stride, WorkCount& wc) {
+    bool h = traverse<false, COUNT>(s, ray, hit, stack, stride, wc);
+    return h && hit.prim != 0xFFFFFFFFu;
+}
+
 }  // namespace crh

@@ -1,0 +1,59 @@
+// Wavefront integrator state (structure of arrays in HBM, one slot per path
+// of a pass) and its host entry point.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "layout.h"
+
+namespace crh {
+
+enum : uint32_t {
+    kCExt0 = 0,  // extension queue, ping
+    kCExt1 = 1,  // extension queue, pong
+    kCMis0 = 2,  // MIS-ray queue, ping
+    kCMis1 = 3,  // MIS-ray queue, pong
+    kCShadow = 4,
+    kCCont = 5,  // paths to shade after the shadow query
+    kWfCounters = 6,
+};
+
+struct WfState {
+    uint32_t width, height;
+    float4* rayO;  // extension ray: origin, tMin
+    float4* rayD;  // direction, tMax
+    uint4* hit;    // closest hit of the extension ray
+    float4* beta;
+    float4* rad;   // radiance (aliases the film's per-pass sample buffer)
+    uint32_t* rng; // Sampler state (the increment is recomputed from the pixel)
+    uint32_t* flags;
+    float2* P;     // jittered film position (aliases the film buffer)
+    float4* shO;   // shadow ray
+    float4* shD;
+    float4* lgt;   // light direction, d^2
+    float4* lgn;   // interpolated light normal, emitter area
+    uint32_t* lgId;
+    float4* misO;  // MIS BSDF ray
+    float4* misD;
+    uint4* misHit;
+    float4* misVal;  // running MIS value, selection pdf
+    float4* misBw;   // BSDF weight, BSDF pdf
+    float4* misPos;  // shading position, emitter id bits
+    float4* misBeta; // path throughput at the MIS vertex
+    uint32_t* qExt[2];
+    uint32_t* qMis[2];
+    uint32_t* qShadow;
+    uint32_t* qCont;
+    uint32_t* counters;
+    unsigned long long* work;  // rays, node visits, triangle tests
+};
+
+struct WfLaunch {
+    uint32_t traceGrid = 2048;  // blocks of 256 for the traversal kernels
+    uint32_t shadeGrid = 2048;  // blocks of 256 for the shading kernels
+};
+
+hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
+                         const IntegratorView& cfg, uint32_t si, WfState& w,
+                         const WfLaunch& L, bool countWork, hipStream_t st);
+
+}  // namespace crh

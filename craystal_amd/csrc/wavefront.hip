@@ -1,0 +1,362 @@
+// Wavefront integrator for gfx950.
+//
+// The megakernel keeps a whole path (traversal stack, BSDF, light sample) in
+// one thread; here the same per-path steps (integrator.h) run as a sequence
+// of small kernels over queues of path ids, one bounce at a time:
+//
+//   generate   camera rays for every pixel of the pass        -> extend queue
+//   extend     closest hit for extension rays + MIS rays      (traversal only)
+//   misResolve finish the MIS estimate of the previous bounce  (MIS queue)
+//   shadeHit   emitter / environment, light selection          -> shadow queue
+//   shadow     any-hit for the shadow rays                     (traversal only)
+//   shadeCont  light weight, MIS BSDF ray, continuation sample -> next queues
+//
+// Because the reference draws the MIS BSDF sample only when the shadow ray
+// is unoccluded (Core/Integrator.cu:80-99), the continuation sample waits for
+// the shadow result; the MIS ray of bounce k is traced together with the
+// extension ray of bounce k+1 and resolved before bounce k+1 is shaded,
+// which is the order in which the megakernel adds those terms.  Every path
+// therefore consumes the same random numbers and performs the same float
+// operations as in the megakernel, and the film is bit-identical.
+//
+// Queues are compacted per wave: a ballot of the lanes that push, one atomic
+// per wave on the queue counter (readfirstlane broadcast), mbcnt for each
+// lane's slot.  All kernels are persistent: a fixed grid loops over the
+// queue length read from device memory, so no host synchronisation is
+// needed between bounces.
+#include <hip/hip_runtime.h>
+
+#include "integrator.h"
+#include "wavefront.h"
+
+namespace crh {
+
+namespace {
+
+enum : uint32_t {
+    F_MIS = 1u,      // MIS ray pending (traced by the next extend)
+    F_LIGHT = 2u,    // light geometry valid, shadow ray traced
+    F_MISHIT = 4u,   // MIS ray query returned a hit (intersect() == true)
+    F_EXTHIT = 8u,   // extension query returned a hit
+};
+
+CRH_D uint32_t laneId() { return __lane_id(); }
+
+// Reserve `pred ? 1 : 0` slots of a queue for each active lane of the wave.
+CRH_D uint32_t wavePush(uint32_t* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0xFFFFFFFFu;
+    uint32_t lo = uint32_t(mask), hi = uint32_t(mask >> 32);
+    uint32_t below = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (int(laneId()) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+    base = __shfl(base, leader, 64);
+    return pred ? base + below : 0xFFFFFFFFu;
+}
+
+CRH_D Ray loadRay(const float4* o, const float4* d, uint32_t p) {
+    float4 a = o[p], b = d[p];
+    return Ray{mk3(a.x, a.y, a.z), a.w, mk3(b.x, b.y, b.z), b.w};
+}
+CRH_D void storeRay(float4* o, float4* d, uint32_t p, const Ray& r) {
+    o[p] = make_float4(r.o.x, r.o.y, r.o.z, r.tMin);
+    d[p] = make_float4(r.d.x, r.d.y, r.d.z, r.tMax);
+}
+CRH_D Hit loadHit(const uint4* h, uint32_t p) {
+    uint4 v = h[p];
+    return Hit{v.x, __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w)};
+}
+CRH_D void storeHit(uint4* h, uint32_t p, const Hit& x) {
+    h[p] = make_uint4(x.prim, __float_as_uint(x.t), __float_as_uint(x.u), __float_as_uint(x.v));
+}
+CRH_D Spec ldS(const float4* a, uint32_t p) {
+    float4 v = a[p];
+    return Spec{v.x, v.y, v.z};
+}
+CRH_D void stS(float4* a, uint32_t p, Spec s, float w = 0.f) {
+    a[p] = make_float4(s.r, s.g, s.b, w);
+}
+CRH_D Sampler loadSampler(const WfState& w, uint32_t p) {
+    uint32_t x = p % w.width, y = p / w.width;
+    Sampler s(0u, 0u, 0u);
+    s.state = w.rng[p];
+    s.inc = x * 1234567891u + y * 987654321u + 1u;
+    return s;
+}
+
+#define PERSISTENT_LOOP(i, count)                                            \
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (count);    \
+         i += gridDim.x * blockDim.x)
+
+}  // namespace
+
+// Counter layout (WfCounters): ext[2], mis[2], shadow, cont.
+__global__ __launch_bounds__(64) void kWfReset(uint32_t* c, uint32_t mask) {
+    uint32_t i = threadIdx.x;
+    if (i < kWfCounters && (mask >> i) & 1u) c[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void kWfGenerate(WfState w, CameraView cam,
+                                                   uint32_t si) {
+    uint32_t n = w.width * w.height;
+    PERSISTENT_LOOP(p, n) {
+        uint32_t x = p % w.width, y = p / w.width;
+        Sampler smp(x, y, si);
+        F2 j = smp.next2D();
+        F2 pix{float(x) + j.x, float(y) + j.y};
+        Ray r = cameraRay(cam, pix);
+        storeRay(w.rayO, w.rayD, p, r);
+        w.rng[p] = smp.state;
+        stS(w.beta, p, sp(1.f));
+        stS(w.rad, p, sp(0.f));
+        w.flags[p] = 0u;
+        w.P[p] = make_float2(pix.x, pix.y);
+        w.qExt[0][p] = p;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) w.counters[kCExt0] = n;
+}
+
+// Closest hit for the extension queue followed by the MIS queue.
+template <bool COUNT>
+__global__ __launch_bounds__(256) void kWfExtend(SceneView s, WfState w,
+                                                 uint32_t cur) {
+    extern __shared__ uint2 lds[];
+    const uint32_t nExt = w.counters[kCExt0 + cur];
+    const uint32_t nMis = w.counters[kCMis0 + cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && nExt + nMis)
+        atomicAdd(w.work, (unsigned long long)(nExt + nMis));
+    WorkCount wc{0, 0};
+    PERSISTENT_LOOP(i, nExt + nMis) {
+        Hit h;
+        if (i < nExt) {
+            uint32_t p = w.qExt[cur][i];
+            Ray r = loadRay(w.rayO, w.rayD, p);
+            bool hit = closestHit<COUNT>(s, r, h, lds + threadIdx.x, 256u, wc);
+            storeHit(w.hit, p, h);
+            if (hit) atomicOr(&w.flags[p], F_EXTHIT);
+        } else {
+            uint32_t p = w.qMis[cur][i - nExt];
+            Ray r = loadRay(w.misO, w.misD, p);
+            bool hit = closestHit<COUNT>(s, r, h, lds + threadIdx.x, 256u, wc);
+            storeHit(w.misHit, p, h);
+            if (hit) atomicOr(&w.flags[p], F_MISHIT);
+        }
+    }
+    if (COUNT) {
+        unsigned long long n64 = wc.nodes, t64 = wc.tris;
+        for (int off = 32; off > 0; off >>= 1) {
+            n64 += __shfl_down(n64, off, 64);
+            t64 += __shfl_down(t64, off, 64);
+        }
+        if (laneId() == 0) {
+            atomicAdd(w.work + 1, n64);
+            atomicAdd(w.work + 2, t64);
+        }
+    }
+}
+
+// Resolve the MIS estimate of the previous bounce:
+// radiance += (value / selPdf) * beta_k  (Core/Integrator.cu:139, :186).
+__global__ __launch_bounds__(256) void kWfMisResolve(SceneView s, WfState w,
+                                                     uint32_t cur) {
+    const uint32_t n = w.counters[kCMis0 + cur];
+    PERSISTENT_LOOP(i, n) {
+        uint32_t p = w.qMis[cur][i];
+        uint32_t fl = w.flags[p];
+        Ray r = loadRay(w.misO, w.misD, p);
+        Hit h = loadHit(w.misHit, p);
+        float4 vs = w.misVal[p];
+        Spec value{vs.x, vs.y, vs.z};
+        float4 bw4 = w.misBw[p];
+        float4 pe = w.misPos[p];
+        misBsdfPart(s, mk3(pe.x, pe.y, pe.z), r, (fl & F_MISHIT) != 0u, h,
+                    __float_as_uint(pe.w), Spec{bw4.x, bw4.y, bw4.z}, bw4.w, value);
+        Spec rad = ldS(w.rad, p);
+        rad = rad + (value / vs.w) * ldS(w.misBeta, p);
+        stS(w.rad, p, rad);
+        w.flags[p] = fl & ~(F_MIS | F_MISHIT);
+    }
+}
+
+// Process the extension hit: environment / emitter terms, or light
+// selection producing a shadow ray (Core/Integrator.cu:165-190, :25-53).
+__global__ __launch_bounds__(256) void kWfShadeHit(SceneView s, WfState w,
+                                                   uint32_t cur) {
+    const uint32_t n = w.counters[kCExt0 + cur];
+    PERSISTENT_LOOP(i, n) {
+        uint32_t p = w.qExt[cur][i];
+        uint32_t fl = w.flags[p];
+        Spec beta = ldS(w.beta, p);
+        bool shade = false, shadow = false;
+        if (!(fl & F_EXTHIT)) {
+            stS(w.rad, p, ldS(w.rad, p) + sp3(s.env) * beta);
+        } else {
+            Ray r = loadRay(w.rayO, w.rayD, p);
+            Hit h = loadHit(w.hit, p);
+            const GMaterial& m = s.mats[s.primMat[h.prim]];
+            if (m.flags & 1u) {
+                stS(w.rad, p, ldS(w.rad, p) + sp3(m.emission) * beta);
+            } else {
+                shade = true;
+                Isect it = makeIsect(s, r, h);
+                Sampler smp = loadSampler(w, p);
+                LightGeom g = lightGeom(s, it, smp);
+                w.rng[p] = smp.state;
+                if (g.valid) {
+                    shadow = true;
+                    storeRay(w.shO, w.shD, p, g.shadow);
+                    w.lgt[p] = make_float4(g.ld.x, g.ld.y, g.ld.z, g.d2);
+                    w.lgn[p] = make_float4(g.ln.x, g.ln.y, g.ln.z, g.area);
+                    w.lgId[p] = g.eid;
+                    fl |= F_LIGHT;
+                } else {
+                    fl &= ~F_LIGHT;
+                }
+            }
+        }
+        w.flags[p] = fl & ~F_EXTHIT;
+        uint32_t slot = wavePush(&w.counters[kCCont], shade);
+        if (shade) w.qCont[slot] = p;
+        slot = wavePush(&w.counters[kCShadow], shadow);
+        if (shadow) w.qShadow[slot] = p;
+    }
+}
+
+__global__ __launch_bounds__(256) void kWfShadow(SceneView s, WfState w) {
+    extern __shared__ uint2 lds[];
+    const uint32_t n = w.counters[kCShadow];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(w.work, (unsigned long long)n);
+    WorkCount wc{0, 0};
+    PERSISTENT_LOOP(i, n) {
+        uint32_t p = w.qShadow[i];
+        Ray r = loadRay(w.shO, w.shD, p);
+        Hit h;
+        bool occ = traverse<true, false>(s, r, h, lds + threadIdx.x, 256u, wc);
+        if (occ) w.flags[p] &= ~F_LIGHT;  // only this thread touches p here
+    }
+}
+
+// Light weight, MIS BSDF ray and the continuation sample
+// (Core/Integrator.cu:57-99, :186-223).
+template <bool MIS>
+__global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
+                                                    IntegratorView cfg,
+                                                    uint32_t cur, uint32_t depth) {
+    const uint32_t n = w.counters[kCCont];
+    const uint32_t nxt = cur ^ 1u;
+    PERSISTENT_LOOP(i, n) {
+        uint32_t p = w.qCont[i];
+        uint32_t fl = w.flags[p];
+        Ray r = loadRay(w.rayO, w.rayD, p);
+        Hit h = loadHit(w.hit, p);
+        Isect it = makeIsect(s, r, h);
+        const GMaterial& m = s.mats[s.primMat[h.prim]];
+        BSDF bsdf = makeBSDF(m, it.frame);
+        Sampler smp = loadSampler(w, p);
+        Spec beta = ldS(w.beta, p);
+        Spec rad = ldS(w.rad, p);
+        LightSample ls = noLight();
+        if (fl & F_LIGHT) {
+            float4 a = w.lgt[p], b = w.lgn[p];
+            LightGeom g;
+            g.ld = mk3(a.x, a.y, a.z);
+            g.d2 = a.w;
+            g.ln = mk3(b.x, b.y, b.z);
+            g.area = b.w;
+            g.eid = w.lgId[p];
+            g.valid = true;
+            ls = lightFinish(s, g, bsdf, it);
+        }
+        bool misRay = false;
+        if (MIS) {
+            if (ls.pdf <= 0.0f) {
+                rad = rad + sp(0.f) * beta;
+            } else {
+                Spec value = misLightPart(ls, bsdf, it);
+                Ray mr;
+                Spec bw;
+                float bp;
+                if (misBsdfRay(bsdf, smp, it, mr, bw, bp)) {
+                    misRay = true;
+                    storeRay(w.misO, w.misD, p, mr);
+                    stS(w.misVal, p, value, ls.selPdf);
+                    stS(w.misBw, p, bw, bp);
+                    w.misPos[p] = make_float4(it.pos.x, it.pos.y, it.pos.z,
+                                              __uint_as_float(ls.id));
+                    stS(w.misBeta, p, beta);
+                } else {
+                    rad = rad + (value / ls.selPdf) * beta;
+                }
+            }
+        } else if (ls.pdf > 0.0f) {
+            rad = rad + ls.weight * beta / ls.pdf / ls.selPdf;
+        }
+        // Continuation (Core/Integrator.cu:195-223).
+        bool alive = true;
+        F3 wi;
+        float pdf = 0.0f;
+        Spec f = bsdfSampleEval(bsdf, smp, it.view, wi, pdf);
+        if (pdf <= 0.0f) {
+            alive = false;
+        } else {
+            beta = beta * (f / pdf);
+            Ray nr{offsetRayOrigin(it.pos, oriented(it)), r.tMin, wi, r.tMax};
+            storeRay(w.rayO, w.rayD, p, nr);
+            if (smp.next1D() < cfg.rrThreshold) alive = false;
+            else if (!roulette(cfg, depth, smp, beta)) alive = false;
+        }
+        if (depth + 1 >= cfg.maxDepth) alive = false;
+        w.rng[p] = smp.state;
+        stS(w.beta, p, beta);
+        stS(w.rad, p, rad);
+        w.flags[p] = (fl & ~F_LIGHT) | (misRay ? F_MIS : 0u);
+        uint32_t slot = wavePush(&w.counters[kCExt0 + nxt], alive);
+        if (alive) w.qExt[nxt][slot] = p;
+        slot = wavePush(&w.counters[kCMis0 + nxt], misRay);
+        if (misRay) w.qMis[nxt][slot] = p;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
+                         const IntegratorView& cfg, uint32_t si, WfState& w,
+                         const WfLaunch& L, bool countWork, hipStream_t st) {
+    const size_t lds = size_t(s.stackCap) * 256u * sizeof(uint2);
+    const dim3 b256(256);
+    hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
+                       (1u << kCExt1) | (1u << kCMis0) | (1u << kCMis1) |
+                           (1u << kCShadow) | (1u << kCCont));
+    hipLaunchKernelGGL(kWfGenerate, dim3(L.shadeGrid), b256, 0, st, w, cam, si);
+    uint32_t cur = 0;
+    for (uint32_t d = 0; d < cfg.maxDepth; ++d) {
+        const uint32_t nxt = cur ^ 1u;
+        if (countWork)
+            hipLaunchKernelGGL((kWfExtend<true>), dim3(L.traceGrid), b256, lds, st, s, w, cur);
+        else
+            hipLaunchKernelGGL((kWfExtend<false>), dim3(L.traceGrid), b256, lds, st, s, w, cur);
+        hipLaunchKernelGGL(kWfMisResolve, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
+        hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
+                           (1u << (kCExt0 + nxt)) | (1u << (kCMis0 + nxt)) |
+                               (1u << kCShadow) | (1u << kCCont));
+        hipLaunchKernelGGL(kWfShadeHit, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
+        hipLaunchKernelGGL(kWfShadow, dim3(L.traceGrid), b256, lds, st, s, w);
+        if (cfg.useMIS)
+            hipLaunchKernelGGL((kWfShadeCont<true>), dim3(L.shadeGrid), b256, 0, st, s, w,
+                               cfg, cur, d);
+        else
+            hipLaunchKernelGGL((kWfShadeCont<false>), dim3(L.shadeGrid), b256, 0, st, s,
+                               w, cfg, cur, d);
+        cur = nxt;
+    }
+    // MIS rays spawned at the last bounce: trace and resolve them.
+    hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
+                       1u << (kCExt0 + cur));
+    hipLaunchKernelGGL((kWfExtend<false>), dim3(L.traceGrid), b256, lds, st, s, w, cur);
+    hipLaunchKernelGGL(kWfMisResolve, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
+    return hipGetLastError();
+}
+
+}  // namespace crh

@@ -806,9 +806,12 @@ inline Spec emissionOf(const crh_material& m) {
     return Spec(m.emission[0], m.emission[1], m.emission[2]);
 }
 
+// SceneView::intersect as the integrator uses it.  A reported hit without a
+// recorded primitive (all-NaN comparisons) counts as a miss; the reference
+// would read an uninitialised primitive index (Scene/Scene.cu:31-58).
 bool intersect(const Scene& s, RayHit& rh, Tally& tl) {
     tl.rays++;
-    return bvhIntersect(s, rh, nullptr, false);
+    return bvhIntersect(s, rh, nullptr, false) && rh.prim != CRH_INVALID_PRIM;
 }
 bool occluded(const Scene& s, const Ray& r, Tally& tl) {
     tl.rays++;

@@ -1,0 +1,101 @@
+// Exercises the C++ host API (include/craystal/CRaystal.h) the way the
+// reference's host code uses its classes (Source/CRayli/CRayli.cpp:7-40).
+// Without arguments: host-only checks (BVH build, material finalize, camera,
+// errors).  With --gpu: also builds a Scene, renders 2 spp and traces rays.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "craystal/CRaystal.h"
+
+using namespace CRay;
+
+#define EXPECT(c)                                                      \
+    do {                                                               \
+        if (!(c)) {                                                    \
+            std::fprintf(stderr, "FAILED %s at line %d\n", #c, __LINE__); \
+            return 1;                                                  \
+        }                                                              \
+    } while (0)
+
+static MeshData quad(Float3 c, Float3 a, Float3 b, Float3 n, uint32_t mat) {
+    MeshData m;
+    m.position = {c, {c[0] + a[0], c[1] + a[1], c[2] + a[2]},
+                  {c[0] + a[0] + b[0], c[1] + a[1] + b[1], c[2] + a[2] + b[2]},
+                  {c[0] + b[0], c[1] + b[1], c[2] + b[2]}};
+    m.normal = {n, n, n, n};
+    m.index = {0, 1, 2, 0, 2, 3};
+    m.materialID = mat;
+    return m;
+}
+
+int main(int argc, char** argv) {
+    bool gpu = argc > 1 && std::strcmp(argv[1], "--gpu") == 0;
+    SceneData data;
+    MaterialData white;
+    white.diffuseRefl = {0.7f, 0.7f, 0.7f};
+    white.finalize();
+    MaterialData light;
+    light.emission = {10.f, 10.f, 10.f};
+    light.finalize();
+    EXPECT(light.isEmissive() && !white.isEmissive() && white.isDiffuse());
+    data.materials = {white, light};
+    data.meshes.push_back(quad({-2, 0, 2}, {4, 0, 0}, {0, 0, -4}, {0, 1, 0}, 0));
+    data.meshes.push_back(quad({-0.5f, 2, -0.5f}, {1, 0, 0}, {0, 0, 1}, {0, -1, 0}, 1));
+    data.emissiveIndex = {2, 3};
+
+    BVH bvh;
+    bvh.build(data);
+    crh_bvh_stats st = bvh.stats();
+    EXPECT(st.prim_count == 4 && st.node_count == 1 && st.root == 0);
+    std::vector<crh_bvh_node_ref> nodes;
+    std::vector<uint32_t> prims;
+    bvh.exportNodes(nodes, prims);
+    EXPECT(nodes.size() == 1 && nodes[0].tag == 0 && nodes[0].b == 4);
+
+    Camera cam;
+    cam.setWorldPosition({0, 1, 4});
+    cam.setTarget({0, 0.5f, 0});
+    cam.calculateCameraData();
+    EXPECT(std::fabs(cam.getData().w[2] + 0.992278f) < 1e-4f);
+
+    bool threw = false;
+    try {
+        SceneData bad = data;
+        bad.meshes[0].index[0] = 99;
+        BVH b2;
+        b2.build(bad);
+    } catch (const RuntimeError& e) {
+        threw = e.code() == CRH_ERR_INVALID_ARG;
+    }
+    EXPECT(threw);
+    if (!gpu) {
+        std::printf("cpp host api ok\n");
+        return 0;
+    }
+
+    auto scene = std::make_shared<Scene>(std::move(data));
+    auto sensor = std::make_shared<Sensor>(UInt2{32, 24});
+    auto camRef = std::make_shared<Camera>(cam);
+    camRef->setSensor(sensor);
+    scene->setCamera(camRef);
+    PathTraceIntegrator::Configs cfg;
+    cfg.maxDepth = 4;
+    PathTraceIntegrator integ(cfg);
+    crh_stats rs = integ.dispatch(*scene, 2);
+    auto img = sensor->createImage();
+    double mean = 0;
+    for (float v : img) mean += v;
+    mean /= img.size();
+    EXPECT(rs.rays > 0 && std::isfinite(mean) && mean > 0);
+    std::vector<crh_ray> rays(2);
+    rays[0] = crh_ray{{0, 1, 0}, 0.f, {0, -1, 0}, INFINITY};
+    rays[1] = crh_ray{{0, 1, 0}, 0.f, {0, 1, 0}, INFINITY};
+    auto hits = scene->intersect(rays);
+    EXPECT(hits[0].prim < 2 && std::fabs(hits[0].t - 1.f) < 1e-6f);
+    EXPECT(hits[1].prim >= 2 && std::fabs(hits[1].t - 1.f) < 1e-6f);
+    auto occ = scene->intersectOcclusion(rays);
+    EXPECT(occ[0] == 1 && occ[1] == 1);
+    std::printf("cpp gpu api ok: mean %.5f rays %llu\n", mean, (unsigned long long)rs.rays);
+    return 0;
+}
