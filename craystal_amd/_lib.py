@@ -93,6 +93,11 @@ class crh_integrator_cfg(C.Structure):
                 ("reserved", u32 * 2)]
 
 
+class crh_tuning(C.Structure):
+    _fields_ = [("stack", u32), ("trace_block", u32), ("wf_trace_grid", u32),
+                ("wf_trace_block", u32), ("wf_shade_grid", u32), ("reserved", u32 * 3)]
+
+
 class crh_stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("main_kernel_ms", C.c_double),
                 ("total_ms", C.c_double), ("rays", C.c_uint64),
@@ -116,6 +121,8 @@ SIGNATURES = {
     "crh_version": [C.POINTER(u32)],
     "crh_context_create": [C.c_int, C.POINTER(_P)],
     "crh_context_destroy": [_P],
+    "crh_context_set_tuning": [_P, C.POINTER(crh_tuning)],
+    "crh_context_get_tuning": [_P, C.POINTER(crh_tuning)],
     "crh_buffer_create": [_P, C.c_size_t, C.POINTER(_P)],
     "crh_buffer_upload": [_P, _P, C.c_size_t],
     "crh_buffer_download": [_P, _P, C.c_size_t],

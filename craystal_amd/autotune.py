@@ -1,0 +1,94 @@
+"""Autotuner for launch shapes that do not change results.
+
+Picks, by short timed trials on the device, the traversal-stack placement
+(LDS or private memory), threads per block of the ray-batch kernels, and the
+grid and block sizes of the wavefront kernels (its queue work distribution).
+Every candidate renders / traces the same inputs, and the tuner checks that
+all candidates produce bit-identical outputs before applying the fastest.
+"""
+from __future__ import annotations
+
+import itertools
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .core import (Configs, DeviceBuffer, PathTraceIntegrator, Scene, get_context,
+                   trace_closest_device)
+
+
+def _median(xs):
+    return float(np.median(np.asarray(xs)))
+
+
+def tune_trace(scene: Scene, rays: np.ndarray, reps: int = 3, device: int = 0,
+               stacks=(1, 2), blocks=(64, 128, 256)) -> dict:
+    """Tune the closest-hit batch kernel; returns the chosen tuning and the
+    per-candidate timings (ms)."""
+    ctx = get_context(device)
+    saved = ctx.get_tuning()
+    n = len(rays)
+    rb = DeviceBuffer(rays.nbytes, device)
+    hb = DeviceBuffer(n * 16, device)
+    rb.copyFromHost(np.ascontiguousarray(rays, dtype=L.RAY_DTYPE))
+    ref = None
+    results = {}
+    for st, bl in itertools.product(stacks, blocks):
+        ctx.set_tuning(stack=st, trace_block=bl)
+        trace_closest_device(scene, rb, n, hb)
+        ms = [trace_closest_device(scene, rb, n, hb)["kernel_ms"] for _ in range(reps)]
+        out = np.zeros(n, L.HIT_DTYPE)
+        hb.copyToHost(out)
+        if ref is None:
+            ref = out
+        elif out.tobytes() != ref.tobytes():
+            ctx.set_tuning(**saved)
+            raise AssertionError(f"tuning stack={st} block={bl} changed the hits")
+        results[(st, bl)] = _median(ms)
+    best = min(results, key=results.get)
+    ctx.set_tuning(stack=best[0], trace_block=best[1])
+    return {"stack": best[0], "trace_block": best[1],
+            "timings_ms": {f"stack={k[0]},block={k[1]}": round(v, 4) for k, v in results.items()}}
+
+
+def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
+                stacks=(1, 2), wf_grids=(1024, 2048, 4096), wf_blocks=(128, 256),
+                first_sample: int = 1 << 20) -> dict:
+    """Tune the megakernel stack placement and the wavefront grid/block
+    sizes on the scene's current camera/sensor; returns the choice and the
+    timings (ms per spp) of both integrators."""
+    ctx = get_context(device)
+    sensor = scene.getCamera().getSensor()
+    res_mk, res_wf = {}, {}
+    ref = None
+
+    def run(c):
+        sensor.clear()
+        st = PathTraceIntegrator(c).dispatch(scene, spp, first_sample=first_sample)
+        return st["kernel_ms"] / spp, sensor.data.copy()
+
+    for st in stacks:
+        ctx.set_tuning(stack=st)
+        run(cfg)
+        ms, img = run(cfg)
+        ref = img if ref is None else ref
+        assert img.tobytes() == ref.tobytes(), "stack placement changed the image"
+        res_mk[st] = ms
+    best_stack = min(res_mk, key=res_mk.get)
+    wf_cfg = Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
+    for st, g, b in itertools.product(stacks, wf_grids, wf_blocks):
+        ctx.set_tuning(stack=st, wf_trace_grid=g, wf_shade_grid=g, wf_trace_block=b)
+        ms, img = run(wf_cfg)
+        assert img.tobytes() == ref.tobytes(), "wavefront launch shape changed the image"
+        res_wf[(st, g, b)] = ms
+    best_wf = min(res_wf, key=res_wf.get)
+    mk_ms, wf_ms = res_mk[best_stack], res_wf[best_wf]
+    use_wf = wf_ms < mk_ms
+    stack = best_wf[0] if use_wf else best_stack
+    ctx.set_tuning(stack=stack, wf_trace_grid=best_wf[1], wf_shade_grid=best_wf[1],
+                   wf_trace_block=best_wf[2])
+    return {"mode": L.MODE_WAVEFRONT if use_wf else L.MODE_MEGAKERNEL,
+            "stack": stack, "wf_grid": best_wf[1], "wf_trace_block": best_wf[2],
+            "megakernel_ms": {str(k): round(v, 3) for k, v in res_mk.items()},
+            "wavefront_ms": {str(k): round(v, 3) for k, v in res_wf.items()}}

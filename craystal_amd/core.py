@@ -36,6 +36,23 @@ class Context:
         self.handle = h
         self.device = device
 
+    TUNING_FIELDS = ("stack", "trace_block", "wf_trace_grid", "wf_trace_block",
+                     "wf_shade_grid")
+
+    def set_tuning(self, **kw):
+        """Launch choices that never change results (see crh_tuning)."""
+        t = L.crh_tuning()
+        for k, v in kw.items():
+            if k not in self.TUNING_FIELDS:
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        L.call("crh_context_set_tuning", self.handle, C.byref(t))
+
+    def get_tuning(self) -> dict:
+        t = L.crh_tuning()
+        L.call("crh_context_get_tuning", self.handle, C.byref(t))
+        return {k: getattr(t, k) for k in self.TUNING_FIELDS}
+
     def __del__(self):
         if getattr(self, "handle", None) and L._lib is not None:
             L._lib.crh_context_destroy(self.handle)

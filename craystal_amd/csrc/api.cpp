@@ -115,7 +115,25 @@ struct crh_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    crh::Tuning tuning;
 };
+
+namespace {
+// The private stack holds kPrivStack levels; deeper trees use the LDS stack.
+crh::Tuning effective(const crh::Tuning& t, const crh::SceneView& v) {
+    crh::Tuning e = t;
+    if (v.stackCap > uint32_t(crh::kPrivStackLevels)) e.privStack = 0;
+    return e;
+}
+crh::WfLaunch wfLaunch(const crh::Tuning& t) {
+    crh::WfLaunch w;
+    w.traceGrid = t.wfTraceGrid;
+    w.traceBlock = t.wfTraceBlock;
+    w.shadeGrid = t.wfShadeGrid;
+    w.privStack = t.privStack;
+    return w;
+}
+}  // namespace
 
 struct crh_buffer {
     crh_context* ctx = nullptr;
@@ -249,6 +267,34 @@ int crh_context_destroy(crh_context* ctx) {
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+    return CRH_OK;
+}
+
+int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
+    ARG_CHECK(ctx && t, "NULL argument");
+    auto okBlock = [](uint32_t b) { return b == 0 || (b >= 64 && b <= 256 && b % 64 == 0); };
+    ARG_CHECK(t->stack <= 2, "stack must be 0 (keep), 1 (LDS) or 2 (private)");
+    ARG_CHECK(okBlock(t->trace_block) && okBlock(t->wf_trace_block),
+              "block sizes must be multiples of 64 in [64, 256]");
+    ARG_CHECK(t->wf_trace_grid <= (1u << 20) && t->wf_shade_grid <= (1u << 20),
+              "grid too large");
+    crh::Tuning& o = ctx->tuning;
+    if (t->stack) o.privStack = t->stack == 2 ? 1u : 0u;
+    if (t->trace_block) o.traceBlock = t->trace_block;
+    if (t->wf_trace_grid) o.wfTraceGrid = t->wf_trace_grid;
+    if (t->wf_trace_block) o.wfTraceBlock = t->wf_trace_block;
+    if (t->wf_shade_grid) o.wfShadeGrid = t->wf_shade_grid;
+    return CRH_OK;
+}
+
+int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
+    ARG_CHECK(ctx && t, "NULL argument");
+    std::memset(t, 0, sizeof(*t));
+    t->stack = ctx->tuning.privStack ? 2u : 1u;
+    t->trace_block = ctx->tuning.traceBlock;
+    t->wf_trace_grid = ctx->tuning.wfTraceGrid;
+    t->wf_trace_block = ctx->tuning.wfTraceBlock;
+    t->wf_shade_grid = ctx->tuning.wfShadeGrid;
     return CRH_OK;
 }
 
@@ -542,7 +588,8 @@ static int traceClosestDev(crh_scene* s, const crh_ray* rays, uint64_t n,
     double t0 = nowMs();
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-    HIP_TRY(crh::launchTraceClosest(s->view, rays, n, hits, work, ctx->stream));
+    HIP_TRY(crh::launchTraceClosest(s->view, rays, n, hits, work,
+                                    effective(ctx->tuning, s->view), ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev1));
     if (stats) {
@@ -602,7 +649,8 @@ int crh_trace_any_device(crh_scene* s, const crh_buffer* rays, uint64_t n,
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     HIP_TRY(crh::launchTraceAny(s->view, rays->mem.as<crh_ray>(), n,
-                                occ->mem.as<uint8_t>(), ctx->stream));
+                                occ->mem.as<uint8_t>(), effective(ctx->tuning, s->view),
+                                ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev1));
     if (stats) {
@@ -648,7 +696,7 @@ int crh_trace_any(crh_scene* s, const crh_ray* rays, uint64_t n,
     HIP_TRY(dox.alloc(n));
     HIP_TRY(hipMemcpy(dr.p, rays, n * sizeof(crh_ray), hipMemcpyHostToDevice));
     HIP_TRY(crh::launchTraceAny(s->view, dr.as<crh_ray>(), n, dox.as<uint8_t>(),
-                                ctx->stream));
+                                effective(ctx->tuning, s->view), ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     HIP_TRY(hipMemcpy(occluded, dox.p, n, hipMemcpyDeviceToHost));
     return CRH_OK;
@@ -758,7 +806,8 @@ int crh_render(crh_scene* s, const crh_camera* cam,
         HIP_TRY(ensureWavefront(film));
         HIP_TRY(hipMemsetAsync(film->wf.work, 0, 64, ctx->stream));
     }
-    crh::WfLaunch wl;
+    const crh::Tuning tun = effective(ctx->tuning, s->view);
+    const crh::WfLaunch wl = wfLaunch(tun);
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     for (uint32_t i = 0; i < spp; ++i) {
         if (stats) HIP_TRY(hipEventRecord(evs[2 * i], ctx->stream));
@@ -769,7 +818,7 @@ int crh_render(crh_scene* s, const crh_camera* cam,
             HIP_TRY(crh::launchRenderPass(s->view, cv, iv, first_sample + i,
                                           film->L.as<float4>(), film->P.as<float2>(),
                                           counter.as<unsigned long long>(),
-                                          cfg->count_work != 0, ctx->stream));
+                                          cfg->count_work != 0, tun, ctx->stream));
         if (stats) HIP_TRY(hipEventRecord(evs[2 * i + 1], ctx->stream));
         HIP_TRY(crh::launchFilmGather(film->L.as<float4>(), film->P.as<float2>(),
                                       film->w, film->h, film->sum.as<float>(),

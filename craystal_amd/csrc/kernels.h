@@ -4,21 +4,31 @@
 
 #include "../../include/crayhip.h"
 #include "layout.h"
+#include "traverse_limits.h"
 
 namespace crh {
 
-constexpr uint32_t kTraceBlock = 256;
+constexpr uint32_t kTraceBlock = 256;   // upper bound; launches may use fewer
 constexpr uint32_t kRenderBlock = 256;
+
+// Launch choices that do not change results (set by the autotuner).
+struct Tuning {
+    uint32_t privStack = 0;     // 1: traversal stack in private memory
+    uint32_t traceBlock = 256;  // threads per block, ray-batch kernels (64..256)
+    uint32_t wfTraceGrid = 2048;
+    uint32_t wfShadeGrid = 2048;
+    uint32_t wfTraceBlock = 256;
+};
 
 hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
                               uint64_t n, crh_hit* hits, uint32_t* work,
-                              hipStream_t st);
+                              const Tuning& t, hipStream_t st);
 hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
-                          uint8_t* occ, hipStream_t st);
+                          uint8_t* occ, const Tuning& t, hipStream_t st);
 hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
                             const IntegratorView& cfg, uint32_t si, float4* L,
                             float2* P, unsigned long long* counters,
-                            bool countWork, hipStream_t st);
+                            bool countWork, const Tuning& t, hipStream_t st);
 hipError_t launchFilmGather(const float4* L, const float2* P, uint32_t W,
                             uint32_t H, float* sum, float* wsum,
                             hipStream_t st);

@@ -11,19 +11,20 @@ namespace crh {
 // ---------------------------------------------------------------------------
 // Ray batch queries
 // ---------------------------------------------------------------------------
-template <bool COUNT>
+template <bool COUNT, bool PRIV>
 __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(
     SceneView s, const crh_ray* __restrict__ rays, uint64_t n,
     crh_hit* __restrict__ hits, uint32_t* __restrict__ work) {
     extern __shared__ uint2 lds[];
-    uint64_t i = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4* rp = reinterpret_cast<const float4*>(rays + i);
     float4 a = rp[0], b = rp[1];
     Ray ray{mk3(a.x, a.y, a.z), a.w, mk3(b.x, b.y, b.z), b.w};
     Hit h;
     WorkCount wc{0, 0};
-    traverse<false, COUNT>(s, ray, h, lds + threadIdx.x, kTraceBlock, wc);
+    auto stk = makeStack<PRIV>(lds);
+    traverse<false, COUNT>(s, ray, h, stk, wc);
     reinterpret_cast<uint4*>(hits)[i] =
         make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.u),
                    __float_as_uint(h.v));
@@ -33,49 +34,48 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(
     }
 }
 
+template <bool PRIV>
 __global__ __launch_bounds__(kTraceBlock) void kTraceAny(
     SceneView s, const crh_ray* __restrict__ rays, uint64_t n,
     uint8_t* __restrict__ occ) {
     extern __shared__ uint2 lds[];
-    uint64_t i = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4* rp = reinterpret_cast<const float4*>(rays + i);
     float4 a = rp[0], b = rp[1];
     Ray ray{mk3(a.x, a.y, a.z), a.w, mk3(b.x, b.y, b.z), b.w};
     Hit h;
     WorkCount wc{0, 0};
-    occ[i] = traverse<true, false>(s, ray, h, lds + threadIdx.x, kTraceBlock, wc)
-                 ? 1
-                 : 0;
+    auto stk = makeStack<PRIV>(lds);
+    occ[i] = traverse<true, false>(s, ray, h, stk, wc) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
 // Megakernel integrator (Core/Integrator.cu:13-227)
 // ---------------------------------------------------------------------------
-template <bool COUNT>
+template <bool COUNT, class Stack>
 struct PathCtx {
     const SceneView& s;
-    uint2* stack;
-    uint32_t stride;
+    Stack& stack;
     uint32_t rays;
     WorkCount wc;
 };
 
-template <bool COUNT>
-CRH_D bool closest(PathCtx<COUNT>& c, const Ray& r, Hit& h) {
+template <bool COUNT, class Stack>
+CRH_D bool closest(PathCtx<COUNT, Stack>& c, const Ray& r, Hit& h) {
     c.rays++;
-    return closestHit<COUNT>(c.s, r, h, c.stack, c.stride, c.wc);
+    return closestHit<COUNT>(c.s, r, h, c.stack, c.wc);
 }
-template <bool COUNT>
-CRH_D bool occluded(PathCtx<COUNT>& c, const Ray& r) {
+template <bool COUNT, class Stack>
+CRH_D bool occluded(PathCtx<COUNT, Stack>& c, const Ray& r) {
     Hit h;
     c.rays++;
-    return traverse<true, COUNT>(c.s, r, h, c.stack, c.stride, c.wc);
+    return traverse<true, COUNT>(c.s, r, h, c.stack, c.wc);
 }
 
 // sampleLight (Core/Integrator.cu:21-72)
-template <bool COUNT>
-CRH_D LightSample sampleLight(PathCtx<COUNT>& c, const Isect& it,
+template <bool COUNT, class Stack>
+CRH_D LightSample sampleLight(PathCtx<COUNT, Stack>& c, const Isect& it,
                               const BSDF& bsdf, Sampler& smp) {
     LightGeom g = lightGeom(c.s, it, smp);
     if (!g.valid) return noLight();
@@ -84,8 +84,8 @@ CRH_D LightSample sampleLight(PathCtx<COUNT>& c, const Isect& it,
 }
 
 // evalMIS (Core/Integrator.cu:74-140)
-template <bool COUNT>
-CRH_D Spec evalMIS(PathCtx<COUNT>& c, const Isect& it, const BSDF& bsdf,
+template <bool COUNT, class Stack>
+CRH_D Spec evalMIS(PathCtx<COUNT, Stack>& c, const Isect& it, const BSDF& bsdf,
                    Sampler& smp) {
     LightSample ls = sampleLight(c, it, bsdf, smp);
     if (ls.pdf <= 0.0f) return sp(0.f);
@@ -113,7 +113,7 @@ CRH_D void tileCoord(uint32_t tilesX, uint32_t& x, uint32_t& y) {
 // pathTraceKernel (Core/Integrator.cu:142-227): radiance of sample index
 // `si` for every pixel into L (float4 per pixel, w unused) and the jittered
 // film position into P.
-template <bool MIS, bool COUNT>
+template <bool MIS, bool COUNT, bool PRIV>
 __global__ __launch_bounds__(kRenderBlock) void kRenderPass(
     SceneView s, CameraView cam, IntegratorView cfg, uint32_t si,
     uint32_t tilesX, float4* __restrict__ L, float2* __restrict__ P,
@@ -123,7 +123,8 @@ __global__ __launch_bounds__(kRenderBlock) void kRenderPass(
     tileCoord(tilesX, x, y);
     uint32_t rays = 0, nodes = 0, tris = 0;
     if (x < cam.width && y < cam.height) {
-        PathCtx<COUNT> c{s, lds + threadIdx.x, kRenderBlock, 0, {0, 0}};
+        auto stk = makeStack<PRIV>(lds);
+        PathCtx<COUNT, typename StackOf<PRIV>::T> c{s, stk, 0, {0, 0}};
         Sampler smp(x, y, si);
         F2 j = smp.next2D();
         F2 pix{float(x) + j.x, float(y) + j.y};
@@ -232,50 +233,68 @@ __global__ __launch_bounds__(256) void kFilmGather(
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
+static size_t stackLds(const SceneView& s, const Tuning& t, uint32_t block) {
+    return t.privStack ? 0 : size_t(s.stackCap) * block * sizeof(uint2);
+}
+
 hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
                               uint64_t n, crh_hit* hits, uint32_t* work,
-                              hipStream_t st) {
+                              const Tuning& t, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    size_t lds = size_t(s.stackCap) * kTraceBlock * sizeof(uint2);
-    dim3 grid(uint32_t((n + kTraceBlock - 1) / kTraceBlock));
-    if (work)
-        hipLaunchKernelGGL((kTraceClosest<true>), grid, dim3(kTraceBlock), lds,
-                           st, s, rays, n, hits, work);
+    const uint32_t B = t.traceBlock;
+    size_t lds = stackLds(s, t, B);
+    dim3 grid(uint32_t((n + B - 1) / B)), block(B);
+    if (work && t.privStack)
+        hipLaunchKernelGGL((kTraceClosest<true, true>), grid, block, lds, st, s, rays, n, hits, work);
+    else if (work)
+        hipLaunchKernelGGL((kTraceClosest<true, false>), grid, block, lds, st, s, rays, n, hits, work);
+    else if (t.privStack)
+        hipLaunchKernelGGL((kTraceClosest<false, true>), grid, block, lds, st, s, rays, n, hits, work);
     else
-        hipLaunchKernelGGL((kTraceClosest<false>), grid, dim3(kTraceBlock), lds,
-                           st, s, rays, n, hits, work);
+        hipLaunchKernelGGL((kTraceClosest<false, false>), grid, block, lds, st, s, rays, n, hits, work);
     return hipGetLastError();
 }
 
 hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
-                          uint8_t* occ, hipStream_t st) {
+                          uint8_t* occ, const Tuning& t, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    size_t lds = size_t(s.stackCap) * kTraceBlock * sizeof(uint2);
-    dim3 grid(uint32_t((n + kTraceBlock - 1) / kTraceBlock));
-    hipLaunchKernelGGL(kTraceAny, grid, dim3(kTraceBlock), lds, st, s, rays, n,
-                       occ);
+    const uint32_t B = t.traceBlock;
+    size_t lds = stackLds(s, t, B);
+    dim3 grid(uint32_t((n + B - 1) / B)), block(B);
+    if (t.privStack)
+        hipLaunchKernelGGL((kTraceAny<true>), grid, block, lds, st, s, rays, n, occ);
+    else
+        hipLaunchKernelGGL((kTraceAny<false>), grid, block, lds, st, s, rays, n, occ);
     return hipGetLastError();
+}
+
+template <bool MIS, bool COUNT, bool PRIV>
+static void renderLaunch(dim3 grid, size_t lds, hipStream_t st, const SceneView& s,
+                         const CameraView& cam, const IntegratorView& cfg, uint32_t si,
+                         uint32_t tilesX, float4* L, float2* P,
+                         unsigned long long* counters) {
+    hipLaunchKernelGGL((kRenderPass<MIS, COUNT, PRIV>), grid, dim3(kRenderBlock), lds, st, s,
+                       cam, cfg, si, tilesX, L, P, counters);
 }
 
 hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
                             const IntegratorView& cfg, uint32_t si, float4* L,
                             float2* P, unsigned long long* counters,
-                            bool countWork, hipStream_t st) {
+                            bool countWork, const Tuning& t, hipStream_t st) {
     uint32_t tilesX = (cam.width + 15u) / 16u, tilesY = (cam.height + 15u) / 16u;
-    size_t lds = size_t(s.stackCap) * kRenderBlock * sizeof(uint2);
-    dim3 grid(tilesX * tilesY), block(kRenderBlock);
-    if (cfg.useMIS && countWork)
-        hipLaunchKernelGGL((kRenderPass<true, true>), grid, block, lds, st, s, cam,
-                           cfg, si, tilesX, L, P, counters);
-    else if (cfg.useMIS)
-        hipLaunchKernelGGL((kRenderPass<true, false>), grid, block, lds, st, s, cam,
-                           cfg, si, tilesX, L, P, counters);
-    else if (countWork)
-        hipLaunchKernelGGL((kRenderPass<false, true>), grid, block, lds, st, s, cam,
-                           cfg, si, tilesX, L, P, counters);
-    else
-        hipLaunchKernelGGL((kRenderPass<false, false>), grid, block, lds, st, s,
-                           cam, cfg, si, tilesX, L, P, counters);
+    size_t lds = stackLds(s, t, kRenderBlock);
+    dim3 grid(tilesX * tilesY);
+    const int sel = (cfg.useMIS ? 4 : 0) | (countWork ? 2 : 0) | (t.privStack ? 1 : 0);
+    switch (sel) {
+        case 0: renderLaunch<false, false, false>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
+        case 1: renderLaunch<false, false, true>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
+        case 2: renderLaunch<false, true, false>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
+        case 3: renderLaunch<false, true, true>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
+        case 4: renderLaunch<true, false, false>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
+        case 5: renderLaunch<true, false, true>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
+        case 6: renderLaunch<true, true, false>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
+        default: renderLaunch<true, true, true>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
+    }
     return hipGetLastError();
 }
 

@@ -12,11 +12,12 @@
 // popped near child would never fail the `entry.tMin > closestT` test (it was
 // pushed with t < closestT and closestT has not changed), so the visit order
 // and every comparison are identical.  With that, the stack never holds more
-// entries than the tree has levels, and it lives in LDS: entry k of a thread
-// is at stack[k * stride], stride = threads per block, so a wave's lanes hit
-// consecutive 8-byte slots.
+// entries than the tree has levels.  It lives in LDS, entry k of a thread at
+// stack[k * stride] with stride = threads per block so a wave's lanes hit
+// consecutive 8-byte slots (LdsStack), or in private memory (PrivStack).
 #pragma once
 #include "device_math.h"
+#include "traverse_limits.h"
 
 namespace crh {
 
@@ -27,6 +28,24 @@ struct Hit {
 
 struct WorkCount {
     uint32_t nodes, tris;
+};
+
+// Traversal stack in LDS: entry k of a thread at base[k * stride], stride =
+// threads per block, so a wave's lanes touch consecutive 8-byte slots.
+struct LdsStack {
+    uint2* base;
+    uint32_t stride;
+    CRH_D void put(int k, uint2 v) { base[k * stride] = v; }
+    CRH_D uint2 get(int k) const { return base[k * stride]; }
+};
+
+// Traversal stack in private (scratch) memory: no LDS, so occupancy is set
+// by registers alone; entries are cached in L1/L2.  Holds kPrivStack levels.
+constexpr int kPrivStack = kPrivStackLevels;
+struct PrivStack {
+    uint2 a[kPrivStack];
+    CRH_D void put(int k, uint2 v) { a[k] = v; }
+    CRH_D uint2 get(int k) const { return a[k]; }
 };
 
 CRH_D void loadNode(const GNode* nodes, uint32_t i, float4& a, float4& b,
@@ -40,9 +59,9 @@ CRH_D void loadNode(const GNode* nodes, uint32_t i, float4& a, float4& b,
 
 // Returns true if any primitive was hit inside [tMin, tMax] (the reference's
 // return value).  ANY: stop at the first such primitive.
-template <bool ANY, bool COUNT>
-CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, uint2* stack,
-                    uint32_t stride, WorkCount& wc) {
+template <bool ANY, bool COUNT, class Stack>
+CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, Stack& stack,
+                    WorkCount& wc) {
     hit.prim = 0xFFFFFFFFu;
     hit.t = __builtin_inff();
     hit.u = 0.f;
@@ -104,7 +123,7 @@ CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, uint2* stack,
                 float farT = rightNear ? lt : rt;
                 if (nearT < closestT) {
                     if (farT < closestT) {
-                        stack[sp * stride] = make_uint2(farRef, __float_as_uint(farT));
+                        stack.put(sp, make_uint2(farRef, __float_as_uint(farT)));
                         ++sp;
                     }
                     cur = nearRef;
@@ -122,7 +141,7 @@ CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, uint2* stack,
         bool found = false;
         while (sp > 0) {
             --sp;
-            uint2 e = stack[sp * stride];
+            uint2 e = stack.get(sp);
             if (__uint_as_float(e.y) > closestT) continue;
             cur = e.x;
             found = true;
@@ -137,11 +156,32 @@ CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, uint2* stack,
 // hit without recording a primitive (only possible when every comparison
 // sees a NaN, e.g. a NaN ray direction) is treated as a miss: the reference
 // would go on to read an uninitialised primitive index.
-template <bool COUNT>
+template <bool COUNT, class Stack>
 CRH_D bool closestHit(const SceneView& s, const Ray& ray, Hit& hit,
-                      uint2* stack, uint32_t stride, WorkCount& wc) {
-    bool h = traverse<false, COUNT>(s, ray, hit, stack, stride, wc);
+                      Stack& stack, WorkCount& wc) {
+    bool h = traverse<false, COUNT>(s, ray, hit, stack, wc);
     return h && hit.prim != 0xFFFFFFFFu;
+}
+
+// Stack selection: PRIV = private (scratch) stack, else LDS stack.
+template <bool PRIV>
+struct StackOf;
+template <>
+struct StackOf<false> {
+    using T = LdsStack;
+};
+template <>
+struct StackOf<true> {
+    using T = PrivStack;
+};
+
+template <bool PRIV>
+CRH_D typename StackOf<PRIV>::T makeStack(uint2* lds) {
+    if constexpr (PRIV) {
+        return PrivStack{};
+    } else {
+        return LdsStack{lds + threadIdx.x, blockDim.x};
+    }
 }
 
 }  // namespace crh

@@ -48,8 +48,10 @@ struct WfState {
 };
 
 struct WfLaunch {
-    uint32_t traceGrid = 2048;  // blocks of 256 for the traversal kernels
-    uint32_t shadeGrid = 2048;  // blocks of 256 for the shading kernels
+    uint32_t traceGrid = 2048;   // blocks for the traversal kernels
+    uint32_t traceBlock = 256;   // threads per traversal block (64..256)
+    uint32_t shadeGrid = 2048;   // blocks of 256 for the shading kernels
+    uint32_t privStack = 0;      // 1: traversal stack in private memory
 };
 
 hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,

@@ -118,10 +118,11 @@ __global__ __launch_bounds__(256) void kWfGenerate(WfState w, CameraView cam,
 }
 
 // Closest hit for the extension queue followed by the MIS queue.
-template <bool COUNT>
+template <bool COUNT, bool PRIV>
 __global__ __launch_bounds__(256) void kWfExtend(SceneView s, WfState w,
                                                  uint32_t cur) {
     extern __shared__ uint2 lds[];
+    auto stk = makeStack<PRIV>(lds);
     const uint32_t nExt = w.counters[kCExt0 + cur];
     const uint32_t nMis = w.counters[kCMis0 + cur];
     if (blockIdx.x == 0 && threadIdx.x == 0 && nExt + nMis)
@@ -132,13 +133,13 @@ __global__ __launch_bounds__(256) void kWfExtend(SceneView s, WfState w,
         if (i < nExt) {
             uint32_t p = w.qExt[cur][i];
             Ray r = loadRay(w.rayO, w.rayD, p);
-            bool hit = closestHit<COUNT>(s, r, h, lds + threadIdx.x, 256u, wc);
+            bool hit = closestHit<COUNT>(s, r, h, stk, wc);
             storeHit(w.hit, p, h);
             if (hit) atomicOr(&w.flags[p], F_EXTHIT);
         } else {
             uint32_t p = w.qMis[cur][i - nExt];
             Ray r = loadRay(w.misO, w.misD, p);
-            bool hit = closestHit<COUNT>(s, r, h, lds + threadIdx.x, 256u, wc);
+            bool hit = closestHit<COUNT>(s, r, h, stk, wc);
             storeHit(w.misHit, p, h);
             if (hit) atomicOr(&w.flags[p], F_MISHIT);
         }
@@ -223,8 +224,10 @@ __global__ __launch_bounds__(256) void kWfShadeHit(SceneView s, WfState w,
     }
 }
 
+template <bool PRIV>
 __global__ __launch_bounds__(256) void kWfShadow(SceneView s, WfState w) {
     extern __shared__ uint2 lds[];
+    auto stk = makeStack<PRIV>(lds);
     const uint32_t n = w.counters[kCShadow];
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(w.work, (unsigned long long)n);
     WorkCount wc{0, 0};
@@ -232,7 +235,7 @@ __global__ __launch_bounds__(256) void kWfShadow(SceneView s, WfState w) {
         uint32_t p = w.qShadow[i];
         Ray r = loadRay(w.shO, w.shD, p);
         Hit h;
-        bool occ = traverse<true, false>(s, r, h, lds + threadIdx.x, 256u, wc);
+        bool occ = traverse<true, false>(s, r, h, stk, wc);
         if (occ) w.flags[p] &= ~F_LIGHT;  // only this thread touches p here
     }
 }
@@ -321,11 +324,27 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+template <bool PRIV>
+static void launchExtend(bool count, dim3 g, dim3 b, size_t lds, hipStream_t st,
+                         const SceneView& s, const WfState& w, uint32_t cur) {
+    if (count)
+        hipLaunchKernelGGL((kWfExtend<true, PRIV>), g, b, lds, st, s, w, cur);
+    else
+        hipLaunchKernelGGL((kWfExtend<false, PRIV>), g, b, lds, st, s, w, cur);
+}
+
 hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                          const IntegratorView& cfg, uint32_t si, WfState& w,
                          const WfLaunch& L, bool countWork, hipStream_t st) {
-    const size_t lds = size_t(s.stackCap) * 256u * sizeof(uint2);
-    const dim3 b256(256);
+    const uint32_t TB = L.traceBlock;
+    const size_t lds = L.privStack ? 0 : size_t(s.stackCap) * TB * sizeof(uint2);
+    const dim3 b256(256), bT(TB);
+    auto extend = [&](uint32_t cur, bool count) {
+        if (L.privStack)
+            launchExtend<true>(count, dim3(L.traceGrid), bT, lds, st, s, w, cur);
+        else
+            launchExtend<false>(count, dim3(L.traceGrid), bT, lds, st, s, w, cur);
+    };
     hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
                        (1u << kCExt1) | (1u << kCMis0) | (1u << kCMis1) |
                            (1u << kCShadow) | (1u << kCCont));
@@ -333,16 +352,16 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
     uint32_t cur = 0;
     for (uint32_t d = 0; d < cfg.maxDepth; ++d) {
         const uint32_t nxt = cur ^ 1u;
-        if (countWork)
-            hipLaunchKernelGGL((kWfExtend<true>), dim3(L.traceGrid), b256, lds, st, s, w, cur);
-        else
-            hipLaunchKernelGGL((kWfExtend<false>), dim3(L.traceGrid), b256, lds, st, s, w, cur);
+        extend(cur, countWork);
         hipLaunchKernelGGL(kWfMisResolve, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
         hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
                            (1u << (kCExt0 + nxt)) | (1u << (kCMis0 + nxt)) |
                                (1u << kCShadow) | (1u << kCCont));
         hipLaunchKernelGGL(kWfShadeHit, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
-        hipLaunchKernelGGL(kWfShadow, dim3(L.traceGrid), b256, lds, st, s, w);
+        if (L.privStack)
+            hipLaunchKernelGGL((kWfShadow<true>), dim3(L.traceGrid), bT, lds, st, s, w);
+        else
+            hipLaunchKernelGGL((kWfShadow<false>), dim3(L.traceGrid), bT, lds, st, s, w);
         if (cfg.useMIS)
             hipLaunchKernelGGL((kWfShadeCont<true>), dim3(L.shadeGrid), b256, 0, st, s, w,
                                cfg, cur, d);
@@ -354,7 +373,7 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
     // MIS rays spawned at the last bounce: trace and resolve them.
     hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
                        1u << (kCExt0 + cur));
-    hipLaunchKernelGGL((kWfExtend<false>), dim3(L.traceGrid), b256, lds, st, s, w, cur);
+    extend(cur, false);
     hipLaunchKernelGGL(kWfMisResolve, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
     return hipGetLastError();
 }

@@ -194,6 +194,17 @@ typedef struct crh_stats {
     uint64_t launches;
 } crh_stats;
 
+/* Launch choices that never change results (picked by the autotuner,
+ * craystal_amd/autotune.py).  Zero fields keep the current value. */
+typedef struct crh_tuning {
+    uint32_t stack;          /* 1 = LDS traversal stack, 2 = private (scratch) */
+    uint32_t trace_block;    /* threads per block, ray-batch kernels: 64..256 */
+    uint32_t wf_trace_grid;  /* wavefront traversal kernels: blocks */
+    uint32_t wf_trace_block; /* wavefront traversal kernels: threads (64..256) */
+    uint32_t wf_shade_grid;  /* wavefront shading kernels: blocks of 256 */
+    uint32_t reserved[3];
+} crh_tuning;
+
 /* ---- errors / device ------------------------------------------------- */
 const char* crh_last_error(void);
 int crh_device_count(int* count);
@@ -202,6 +213,10 @@ int crh_version(uint32_t* version);
 /* ---- context (replaces the implicit CUDA runtime context) ------------- */
 int crh_context_create(int device, crh_context** out);
 int crh_context_destroy(crh_context* ctx);
+/* Launch tuning for every later call on this context (no counterpart in the
+ * reference, whose launch shape is fixed, Core/Integrator.cu:248-253). */
+int crh_context_set_tuning(crh_context* ctx, const crh_tuning* tuning);
+int crh_context_get_tuning(const crh_context* ctx, crh_tuning* tuning);
 
 /* ---- device buffers (DeviceBuffer, Core/Buffer.h:13-44; size_t sizes) - */
 int crh_buffer_create(crh_context* ctx, size_t bytes, crh_buffer** out);

@@ -1,0 +1,56 @@
+"""Launch tuning (stack placement, block and grid sizes) never changes
+results; the autotuner checks that itself and applies the fastest choice."""
+import numpy as np
+import pytest
+
+import craystal_amd as cr
+from craystal_amd import _lib as L
+from craystal_amd import autotune, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def test_trace_tuning_invariance_and_autotune(gpu):
+    data, cam = scenes.cornell_box()
+    scene = cr.Scene(data)
+    rays = scenes.mixed_batch(data, cam, 50000)
+    ctx = cr.get_context(0)
+    saved = ctx.get_tuning()
+    try:
+        res = autotune.tune_trace(scene, rays, reps=2)
+        assert res["stack"] in (1, 2) and res["trace_block"] in (64, 128, 256)
+        assert len(res["timings_ms"]) == 6
+        assert ctx.get_tuning()["trace_block"] == res["trace_block"]
+        # any-hit with the private stack equals the LDS stack
+        ctx.set_tuning(stack=1)
+        a = scene.trace_any(rays)
+        ctx.set_tuning(stack=2)
+        b = scene.trace_any(rays)
+        assert np.array_equal(a, b)
+    finally:
+        ctx.set_tuning(**saved)
+
+
+def test_render_tuning_invariance(gpu):
+    data, camd = scenes.cornell_box()
+    scene = cr.Scene(data)
+    sensor = cr.Sensor((64, 64))
+    cam = scenes.make_camera(camd, sensor)
+    scene.setCamera(cam)
+    cfg = cr.Configs(maxDepth=8, rrThreshold=0.05, useMIS=True, rrStartDepth=3)
+    ctx = cr.get_context(0)
+    saved = ctx.get_tuning()
+    try:
+        res = autotune.tune_render(scene, cfg, spp=1, wf_grids=(64, 512), wf_blocks=(64, 256))
+        assert res["mode"] in (L.MODE_MEGAKERNEL, L.MODE_WAVEFRONT)
+        assert len(res["wavefront_ms"]) == 8
+    finally:
+        ctx.set_tuning(**saved)
+
+
+def test_tuning_rejects_bad_values(gpu):
+    ctx = cr.get_context(0)
+    with pytest.raises(cr.RuntimeError):
+        ctx.set_tuning(trace_block=96)
+    with pytest.raises(cr.RuntimeError):
+        ctx.set_tuning(stack=7)
