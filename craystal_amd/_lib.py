@@ -149,7 +149,15 @@ SIGNATURES = {
     "crh_film_destroy": [_P],
     "crh_render": [_P, C.POINTER(crh_camera), C.POINTER(crh_integrator_cfg), u32, u32, _P,
                    C.POINTER(crh_stats)],
+    "crh_guide_create": [_P, _P, _P, u32, C.POINTER(_P)],
+    "crh_guide_set_params": [_P, _P, u32],
+    "crh_guide_get_params": [_P, _P, u32],
+    "crh_guide_eval": [_P, _P, _P, _P, u32, _P, _P],
+    "crh_guide_train": [_P, _P, u32, f32, C.c_int, _P, _P],
+    "crh_guide_destroy": [_P],
 }
+GUIDE_PARAMS = 19760
+GUIDE_OUTPUTS = 48
 
 _lib = None
 

@@ -44,7 +44,7 @@ def build(verbose: bool = True) -> Path:
         o = OBJ / (src + ".o")
         _run(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
         objs.append(str(o))
-    for src in ("kernels.hip", "wavefront.hip"):
+    for src in ("kernels.hip", "wavefront.hip", "guiding.hip"):
         o = OBJ / (src + ".o")
         _run([str(ROCM / "bin" / "hipcc"), *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
         objs.append(str(o))

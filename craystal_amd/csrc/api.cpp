@@ -11,6 +11,7 @@
 // back on one stream instead of synchronising after each.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -24,6 +25,7 @@
 #include "bvh_build.h"
 #include "kernels.h"
 #include "wavefront.h"
+#include "guiding_kernels.h"
 
 static_assert(sizeof(crh_ray) == 32, "crh_ray layout");
 static_assert(sizeof(crh_hit) == 16, "crh_hit layout");
@@ -248,6 +250,7 @@ int crh_context_create(int device, crh_context** out) {
     auto* c = new crh_context();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = crh::guideSetLdsLimits();
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e != hipSuccess) {
@@ -849,6 +852,185 @@ int crh_render(crh_scene* s, const crh_camera* cam,
     }
     return CRH_OK;
     GUARD_END
+}
+
+// ---- learned path guiding ----------------------------------------------------
+}  // extern "C"
+
+struct crh_guide {
+    crh_context* ctx = nullptr;
+    DevMem theta, m, v, grad, wq, wqT, loss, samples, hsave, dsave;
+    crh::GuideView view{};
+    uint32_t step = 0;
+    size_t capSamples = 0;
+};
+
+namespace {
+
+// He-uniform init (fan-in 64) from SplitMix64; biases zero.
+void heInit(std::vector<float>& th, uint32_t seed) {
+    uint64_t x = 0x9E3779B97F4A7C15ull * (uint64_t(seed) + 1);
+    auto next = [&]() {
+        x += 0x9E3779B97F4A7C15ull;
+        uint64_t z = x;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return double(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    th.assign(crh::kGParams, 0.f);
+    const double lim = std::sqrt(6.0 / 64.0);
+    for (int l = 0; l < crh::kGLayers; ++l) {
+        int w0 = crh::gWOff(l), N = crh::gLayerN(l);
+        for (int i = 0; i < 64 * N; ++i) th[w0 + i] = float((2.0 * next() - 1.0) * lim);
+    }
+}
+
+int guideUpload(crh_guide* g, const float* theta) {
+    crh_context* ctx = g->ctx;
+    HIP_TRY(hipMemcpyAsync(g->theta.p, theta, crh::kGParams * 4, hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipMemsetAsync(g->m.p, 0, g->m.bytes, ctx->stream));
+    HIP_TRY(hipMemsetAsync(g->v.p, 0, g->v.bytes, ctx->stream));
+    HIP_TRY(hipMemsetAsync(g->grad.p, 0, g->grad.bytes, ctx->stream));
+    HIP_TRY(crh::launchGuideQuantize(g->view, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    g->step = 0;
+    return CRH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int crh_guide_create(crh_context* ctx, const float* bmin, const float* bmax, uint32_t seed,
+                     crh_guide** out) {
+    ARG_CHECK(ctx && bmin && bmax && out, "NULL argument");
+    *out = nullptr;
+    GUARD_BEGIN
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto* g = new crh_guide();
+    g->ctx = ctx;
+    hipError_t e = g->theta.alloc(crh::kGParams * 4);
+    if (e == hipSuccess) e = g->m.alloc(crh::kGParams * 4);
+    if (e == hipSuccess) e = g->v.alloc(crh::kGParams * 4);
+    if (e == hipSuccess) e = g->grad.alloc(crh::kGParams * 4);
+    if (e == hipSuccess) e = g->wq.alloc(crh::kGQElems * 2);
+    if (e == hipSuccess) e = g->wqT.alloc(crh::kGQElems * 2);
+    if (e == hipSuccess) e = g->loss.alloc(256);
+    if (e != hipSuccess) {
+        delete g;
+        return hipFail("guide allocation", e, __FILE__, __LINE__);
+    }
+    crh::GuideView& v = g->view;
+    v.theta = g->theta.as<float>();
+    v.m = g->m.as<float>();
+    v.v = g->v.as<float>();
+    v.grad = g->grad.as<float>();
+    v.wq = g->wq.as<uint16_t>();
+    v.wqT = g->wqT.as<uint16_t>();
+    for (int k = 0; k < 3; ++k) {
+        v.bmin[k] = bmin[k];
+        v.bext[k] = bmax[k] - bmin[k];
+    }
+    std::vector<float> th;
+    heInit(th, seed);
+    int rc = guideUpload(g, th.data());
+    if (rc != CRH_OK) {
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_guide_set_params(crh_guide* g, const float* theta, uint32_t n) {
+    ARG_CHECK(g && theta, "NULL argument");
+    ARG_CHECK(n == crh::kGParams, "parameter count must be CRH_GUIDE_PARAMS");
+    HIP_TRY(hipSetDevice(g->ctx->device));
+    return guideUpload(g, theta);
+}
+
+int crh_guide_get_params(const crh_guide* g, float* theta, uint32_t n) {
+    ARG_CHECK(g && theta, "NULL argument");
+    ARG_CHECK(n == crh::kGParams, "parameter count must be CRH_GUIDE_PARAMS");
+    HIP_TRY(hipSetDevice(g->ctx->device));
+    HIP_TRY(hipMemcpy(theta, g->theta.p, crh::kGParams * 4, hipMemcpyDeviceToHost));
+    return CRH_OK;
+}
+
+int crh_guide_eval(crh_guide* g, const float* pos, const float* nrm, const float* wo,
+                   uint32_t n, float* raw, float* mixture) {
+    ARG_CHECK(g && pos && nrm && wo, "NULL argument");
+    if (n == 0) return CRH_OK;
+    GUARD_BEGIN
+    crh_context* ctx = g->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<float4> hp(n), hn(n), hw(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        hp[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.f);
+        hn[i] = make_float4(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2], 0.f);
+        hw[i] = make_float4(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2], 0.f);
+    }
+    DevMem dp, dn, dw, dmix, draw;
+    HIP_TRY(dp.alloc(size_t(n) * 16));
+    HIP_TRY(dn.alloc(size_t(n) * 16));
+    HIP_TRY(dw.alloc(size_t(n) * 16));
+    HIP_TRY(dmix.alloc(size_t(n) * 40 * 4));
+    HIP_TRY(draw.alloc(size_t(n) * crh::kGOut * 4));
+    HIP_TRY(hipMemcpy(dp.p, hp.data(), dp.bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dn.p, hn.data(), dn.bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dw.p, hw.data(), dw.bytes, hipMemcpyHostToDevice));
+    uint32_t grid = std::min<uint32_t>((n + 63) / 64, 1024u);
+    HIP_TRY(crh::launchGuideInfer(g->view, nullptr, nullptr, n, dp.as<float4>(), dn.as<float4>(),
+                                  dw.as<float4>(), dmix.as<float>(), draw.as<float>(), grid,
+                                  ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (raw) HIP_TRY(hipMemcpy(raw, draw.p, draw.bytes, hipMemcpyDeviceToHost));
+    if (mixture) HIP_TRY(hipMemcpy(mixture, dmix.p, dmix.bytes, hipMemcpyDeviceToHost));
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_guide_train(crh_guide* g, const float* samples, uint32_t n, float lr, int apply_adam,
+                    float* loss, float* grad) {
+    ARG_CHECK(g && samples, "NULL argument");
+    ARG_CHECK(n > 0 && n <= (1u << 24), "sample count must be in [1, 2^24]");
+    GUARD_BEGIN
+    crh_context* ctx = g->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t padN = (size_t(n) + 63) & ~size_t(63);
+    if (padN > g->capSamples) {
+        HIP_TRY(g->samples.alloc(padN * sizeof(crh::GuideSample)));
+        HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * padN * 64 * 2));
+        HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * padN * 64 * 2));
+        g->capSamples = padN;
+    }
+    HIP_TRY(hipMemcpyAsync(g->samples.p, samples, size_t(n) * sizeof(crh::GuideSample),
+                           hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(g->loss.p, 0, 4, ctx->stream));
+    if (apply_adam) g->step++;
+    HIP_TRY(crh::launchGuideTrainStep(g->view, g->samples.as<crh::GuideSample>(), n,
+                                      g->hsave.as<uint16_t>(), g->dsave.as<uint16_t>(),
+                                      g->loss.as<float>(), lr, g->step, apply_adam != 0,
+                                      ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (loss) HIP_TRY(hipMemcpy(loss, g->loss.p, 4, hipMemcpyDeviceToHost));
+    if (!apply_adam) {
+        if (grad) HIP_TRY(hipMemcpy(grad, g->grad.p, crh::kGParams * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemset(g->grad.p, 0, g->grad.bytes));
+    }
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_guide_destroy(crh_guide* g) {
+    if (!g) return CRH_OK;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    delete g;
+    return CRH_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,474 @@
+// Path-guiding network kernels for gfx950: inference and training of the
+// 64-wide MLP with v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32
+// accumulation), the von Mises-Fisher mixture head and Adam.
+//
+// Tiles: one wave computes 16 samples; a block of 4 waves moves through the
+// batch in lockstep (64 samples per iteration) so block barriers stay in
+// uniform control flow.  Activations are row-major [16][72] bf16 tiles in
+// LDS (72 = 64 + 8 pad: row starts 36 dwords apart, 2-way bank aliasing at
+// most).  Forward weights are staged once per block in LDS, already in MFMA
+// B-operand order, so each operand is one 16-byte LDS read per lane.
+#include <hip/hip_runtime.h>
+
+#include "guiding.h"
+#include "guiding_kernels.h"
+
+namespace crh {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kRow = 72;           // LDS row stride of an activation tile
+constexpr int kTile = 16 * kRow;   // elements per activation tile
+constexpr int kWavesPerBlock = 4;
+
+namespace {
+
+__device__ __forceinline__ __bf16 tobf(float v) { return static_cast<__bf16>(v); }
+__device__ __forceinline__ float frombf(__bf16 v) { return static_cast<float>(v); }
+
+__device__ __forceinline__ float softplusf(float x) {
+    return x > 20.f ? x : log1pf(expf(fminf(x, 20.f)));
+}
+
+// Stage the forward weights (5 layers x 4096 bf16) in LDS.
+__device__ void stageWeights(const __bf16* __restrict__ wq, __bf16* ldsW) {
+    const uint4* src = reinterpret_cast<const uint4*>(wq);
+    uint4* dst = reinterpret_cast<uint4*>(ldsW);
+    for (int i = threadIdx.x; i < kGQElems / 8; i += blockDim.x) dst[i] = src[i];
+}
+
+// Input features of sample m of the tile into row m (lanes cover 16
+// features each).  valid = false writes a zero row.
+__device__ void featureRow(__bf16* H, int m, int chunk, bool valid, const float* pos,
+                           const float* nrm, const float* wo, const float* bmin,
+                           const float* bext) {
+    for (int j = 0; j < 16; ++j) {
+        int f = chunk * 16 + j;
+        float v = 0.f;
+        if (valid) {
+            if (f < 30) {
+                int a = f / 10, r = f % 10, lvl = r >> 1;
+                float e = bext[a] > 0.f ? bext[a] : 1.f;
+                float p = (pos[a] - bmin[a]) / e;
+                float arg = float(1 << lvl) * 3.14159265358979323846f * p;
+                v = (r & 1) ? cosf(arg) : sinf(arg);
+            } else if (f >= 32 && f < 35) {
+                v = nrm[f - 32];
+            } else if (f >= 35 && f < 38) {
+                v = wo[f - 35];
+            }
+        }
+        H[m * kRow + f] = tobf(v);
+    }
+}
+
+// One layer: out = act(in * W_l + b_l) for the wave's 16-row tile.
+// For the last layer the fp32 result goes to outF ([16][48]).
+__device__ void layerForward(int l, const __bf16* in, __bf16* out, float* outF,
+                             const __bf16* ldsW, const float* __restrict__ theta) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
+    const int N = gLayerN(l), NT = N >> 4;
+    const float* bias = theta + gBOff(l);
+    for (int t = 0; t < NT; ++t) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            bf16x8 a = *reinterpret_cast<const bf16x8*>(in + r16 * kRow + 32 * s + 8 * g);
+            bf16x8 b = *reinterpret_cast<const bf16x8*>(ldsW + gQOff(l) +
+                                                         ((s * NT + t) * 64 + lane) * 8);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        }
+        const int col = 16 * t + r16;
+        const float bc = bias[col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 4 * g + i;
+            float v = acc[i] + bc;
+            if (l < kGLayers - 1)
+                out[row * kRow + col] = tobf(fmaxf(v, 0.f));
+            else
+                outF[row * kGOut + col] = v;
+        }
+    }
+}
+
+struct Head {
+    float w[kGLobes], mu[kGLobes][3], k[kGLobes], kraw[kGLobes], mlen[kGLobes];
+};
+
+__device__ void computeHead(const float* o, Head& h) {
+    float amax = -INFINITY;
+    for (int j = 0; j < kGLobes; ++j) amax = fmaxf(amax, o[5 * j]);
+    float sum = 0.f;
+    for (int j = 0; j < kGLobes; ++j) {
+        h.w[j] = expf(o[5 * j] - amax);
+        sum += h.w[j];
+    }
+    for (int j = 0; j < kGLobes; ++j) {
+        h.w[j] /= sum;
+        float mx = o[5 * j + 1], my = o[5 * j + 2], mz = o[5 * j + 3];
+        float len = fmaxf(sqrtf(mx * mx + my * my + mz * mz), 1e-6f);
+        h.mlen[j] = len;
+        h.mu[j][0] = mx / len;
+        h.mu[j][1] = my / len;
+        h.mu[j][2] = mz / len;
+        h.kraw[j] = o[5 * j + 4];
+        h.k[j] = softplusf(h.kraw[j]) + kGKappaMin;
+    }
+}
+
+__device__ __forceinline__ float vmfPdf(float k, float c) {
+    // k / (2 pi (1 - e^{-2k})) * exp(k (c - 1))
+    return k / (2.f * 3.14159265358979323846f * (-expm1f(-2.f * k))) * expf(k * (c - 1.f));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Inference: per queued path, mixture parameters
+//   out[p * 40 + ...] = w[8], mu[8][3], kappa[8]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void kGuideInfer(GuideView gv, const uint32_t* __restrict__ queue,
+                                                   const uint32_t* __restrict__ count,
+                                                   uint32_t countDirect,
+                                                   const float4* __restrict__ pos,
+                                                   const float4* __restrict__ nrm,
+                                                   const float4* __restrict__ wo,
+                                                   float* __restrict__ params,
+                                                   float* __restrict__ rawOut) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __bf16* ldsW = reinterpret_cast<__bf16*>(smem);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __bf16* H0 = ldsW + kGQElems + wave * (2 * kTile);
+    __bf16* H1 = H0 + kTile;
+    float* outF = reinterpret_cast<float*>(ldsW + kGQElems + kWavesPerBlock * 2 * kTile) +
+                  wave * 16 * kGOut;
+    stageWeights(reinterpret_cast<const __bf16*>(gv.wq), ldsW);
+    const uint32_t n = count ? *count : countDirect;
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        __syncthreads();
+        const uint32_t tile0 = base + wave * 16;
+        {
+            int m = lane & 15, chunk = lane >> 4;
+            uint32_t idx = tile0 + m;
+            bool valid = idx < n;
+            uint32_t p = valid ? (queue ? queue[idx] : idx) : 0u;
+            float4 P = valid ? pos[p] : make_float4(0, 0, 0, 0);
+            float4 Nn = valid ? nrm[p] : make_float4(0, 0, 0, 0);
+            float4 W = valid ? wo[p] : make_float4(0, 0, 0, 0);
+            const float pa[3] = {P.x, P.y, P.z}, na[3] = {Nn.x, Nn.y, Nn.z},
+                        wa[3] = {W.x, W.y, W.z};
+            featureRow(H0, m, chunk, valid, pa, na, wa, gv.bmin, gv.bext);
+        }
+        __syncthreads();
+        __bf16* in = H0;
+        __bf16* out = H1;
+        for (int l = 0; l < kGLayers; ++l) {
+            layerForward(l, in, out, outF, ldsW, gv.theta);
+            __syncthreads();
+            __bf16* t = in;
+            in = out;
+            out = t;
+        }
+        if (lane < 16) {
+            uint32_t idx = tile0 + lane;
+            if (idx < n) {
+                uint32_t p = queue ? queue[idx] : idx;
+                const float* o = outF + lane * kGOut;
+                if (rawOut)
+                    for (int c = 0; c < kGOut; ++c) rawOut[size_t(p) * kGOut + c] = o[c];
+                Head h;
+                computeHead(o, h);
+                float* dst = params + size_t(p) * 40;
+                for (int j = 0; j < kGLobes; ++j) {
+                    dst[j] = h.w[j];
+                    dst[8 + 3 * j] = h.mu[j][0];
+                    dst[8 + 3 * j + 1] = h.mu[j][1];
+                    dst[8 + 3 * j + 2] = h.mu[j][2];
+                    dst[32 + j] = h.k[j];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Training, pass 1: forward, KL-loss gradient at the outputs, backward
+// through the layers; saves every layer's input H_l and output gradient
+// delta_l (bf16 rows of 64) for the weight-gradient GEMM.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void kGuideFwdBwd(GuideView gv,
+                                                    const GuideSample* __restrict__ samples,
+                                                    uint32_t n, float invN,
+                                                    __bf16* __restrict__ Hsave,
+                                                    __bf16* __restrict__ Dsave,
+                                                    float* __restrict__ lossSum) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __bf16* ldsW = reinterpret_cast<__bf16*>(smem);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4,
+              r16 = lane & 15;
+    // per wave: H_0..H_4 (5 tiles) + two delta tiles
+    __bf16* Hw = ldsW + kGQElems + wave * (7 * kTile);
+    __bf16* Da = Hw + 5 * kTile;
+    __bf16* Db = Da + kTile;
+    float* outF = reinterpret_cast<float*>(ldsW + kGQElems + kWavesPerBlock * 7 * kTile) +
+                  wave * 16 * kGOut;
+    stageWeights(reinterpret_cast<const __bf16*>(gv.wq), ldsW);
+    const size_t padN = (size_t(n) + 63) & ~size_t(63);
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        __syncthreads();
+        const uint32_t tile0 = base + wave * 16;
+        {
+            int m = lane & 15, chunk = lane >> 4;
+            uint32_t idx = tile0 + m;
+            bool valid = idx < n;
+            const GuideSample& smp = samples[valid ? idx : 0];
+            featureRow(Hw, m, chunk, valid, smp.pos, smp.nrm, smp.wo, gv.bmin, gv.bext);
+        }
+        __syncthreads();
+        for (int l = 0; l < kGLayers; ++l) {
+            layerForward(l, Hw + l * kTile, l + 1 < kGLayers ? Hw + (l + 1) * kTile : nullptr,
+                         outF, ldsW, gv.theta);
+            __syncthreads();
+        }
+        // Output gradient (oracle/guiding.py loss_and_grad_out), lanes 0..15.
+        for (int c = lane; c < 16 * kRow; c += 64) Da[c] = tobf(0.f);
+        __syncthreads();
+        if (lane < 16) {
+            uint32_t idx = tile0 + lane;
+            if (idx < n) {
+                const GuideSample& smp = samples[idx];
+                const float* o = outF + lane * kGOut;
+                Head h;
+                computeHead(o, h);
+                float d[3] = {smp.wi[0], smp.wi[1], smp.wi[2]};
+                float v[kGLobes], q = 0.f, cmu[kGLobes];
+                for (int j = 0; j < kGLobes; ++j) {
+                    cmu[j] = h.mu[j][0] * d[0] + h.mu[j][1] * d[1] + h.mu[j][2] * d[2];
+                    v[j] = vmfPdf(h.k[j], cmu[j]);
+                    q += h.w[j] * v[j];
+                }
+                q = fmaxf(q, 1e-30f);
+                float wt = smp.wi[3];
+                atomicAdd(lossSum, -wt * logf(q) * invN);
+                float gsc = -wt * invN;
+                for (int j = 0; j < kGLobes; ++j) {
+                    float resp = h.w[j] * v[j] / q;
+                    float ga = gsc * (resp - h.w[j]);
+                    float e = expf(-2.f * h.k[j]);
+                    float dlv = 1.f / h.k[j] - 2.f * e / (1.f - e) + (cmu[j] - 1.f);
+                    float sig = 1.f / (1.f + expf(-h.kraw[j]));
+                    float gk = gsc * resp * dlv * sig;
+                    float s = gsc * resp * h.k[j] / h.mlen[j];
+                    __bf16* row = Da + lane * kRow;
+                    row[5 * j] = tobf(ga);
+                    for (int c = 0; c < 3; ++c)
+                        row[5 * j + 1 + c] = tobf(s * (d[c] - cmu[j] * h.mu[j][c]));
+                    row[5 * j + 4] = tobf(gk);
+                }
+            }
+        }
+        __syncthreads();
+        // Save H_l rows and delta_4; then backward: delta_{l-1} =
+        // (delta_l W_l^T) masked by H_l > 0.
+        for (int c = lane; c < 16 * 64; c += 64) {
+            int m = c >> 6, f = c & 63;
+            uint32_t idx = tile0 + m;
+            if (idx < padN) {
+                for (int l = 0; l < kGLayers; ++l)
+                    Hsave[(size_t(l) * padN + idx) * 64 + f] = Hw[l * kTile + m * kRow + f];
+                Dsave[(size_t(4) * padN + idx) * 64 + f] = Da[m * kRow + f];
+            }
+        }
+        __bf16* dIn = Da;
+        __bf16* dOut = Db;
+        for (int l = kGLayers - 1; l >= 1; --l) {
+            const __bf16* H = Hw + l * kTile;
+            for (int t = 0; t < 4; ++t) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    bf16x8 a = *reinterpret_cast<const bf16x8*>(dIn + r16 * kRow + 32 * s + 8 * g);
+                    bf16x8 b = *reinterpret_cast<const bf16x8*>(
+                        reinterpret_cast<const __bf16*>(gv.wqT) + gQOff(l) +
+                                                                 ((s * 4 + t) * 64 + lane) * 8);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+                }
+                const int col = 16 * t + r16;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = 4 * g + i;
+                    float hv = frombf(H[row * kRow + col]);
+                    __bf16 dv = tobf(hv > 0.f ? acc[i] : 0.f);
+                    dOut[row * kRow + col] = dv;
+                    uint32_t idx = tile0 + row;
+                    if (idx < padN) Dsave[(size_t(l - 1) * padN + idx) * 64 + col] = dv;
+                }
+            }
+            __syncthreads();
+            __bf16* t2 = dIn;
+            dIn = dOut;
+            dOut = t2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Training, pass 2: dW_l = H_l^T delta_l over the batch (K = samples, 32 per
+// MFMA step) and db_l = sum delta_l, accumulated into grad with atomics.
+// grid.x = 16 output tiles per layer x 5 layers, grid.y = batch chunks.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void kGuideWeightGrad(GuideView gv, const __bf16* __restrict__ Hsave,
+                                                       const __bf16* __restrict__ Dsave, uint32_t n,
+                                                       uint32_t chunk) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
+    const int l = blockIdx.x / 16, tile = blockIdx.x % 16, a = tile / 4, b = tile % 4;
+    const int N = gLayerN(l);
+    if (16 * b >= N) return;
+    const size_t padN = (size_t(n) + 63) & ~size_t(63);
+    const uint32_t s0 = blockIdx.y * chunk, s1 = min(n, s0 + chunk);
+    const __bf16* H = Hsave + size_t(l) * padN * 64;
+    const __bf16* D = Dsave + size_t(l) * padN * 64;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;
+    for (uint32_t s = s0; s < s1; s += 32) {
+        bf16x8 av, bv;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            uint32_t smp = s + 8 * g + i;
+            bool ok = smp < s1;
+            av[i] = ok ? H[size_t(smp) * 64 + 16 * a + r16] : tobf(0.f);
+            bv[i] = ok ? D[size_t(smp) * 64 + 16 * b + r16] : tobf(0.f);
+            if (a == 0) bsum += frombf(bv[i]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+    }
+    float* grad = gv.grad;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int row = 16 * a + 4 * g + i, col = 16 * b + r16;
+        atomicAdd(&grad[gWOff(l) + row * N + col], acc[i]);
+    }
+    if (a == 0) {
+        // lanes with the same column (r16) hold partial sums for g = 0..3
+        bsum += __shfl_xor(bsum, 16, 64);
+        bsum += __shfl_xor(bsum, 32, 64);
+        if (g == 0) atomicAdd(&grad[gBOff(l) + 16 * b + r16], bsum);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adam on the fp32 master parameters; refreshes both bf16 operand copies
+// and clears the gradient for the next step.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void kGuideAdam(GuideView gv, float lr, float b1, float b2,
+                                                  float eps, float c1, float c2) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kGParams) return;
+    float gr = gv.grad[i];
+    float m = b1 * gv.m[i] + (1.f - b1) * gr;
+    float v = b2 * gv.v[i] + (1.f - b2) * gr * gr;
+    gv.m[i] = m;
+    gv.v[i] = v;
+    float th = gv.theta[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+    gv.theta[i] = th;
+    gv.grad[i] = 0.f;
+    // weight element?  find its layer
+    for (int l = 0; l < kGLayers; ++l) {
+        int w0 = gWOff(l), N = gLayerN(l);
+        if (i >= w0 && i < w0 + 64 * N) {
+            int k = (i - w0) / N, c = (i - w0) % N;
+            reinterpret_cast<__bf16*>(gv.wq)[gQOff(l) + gSwz(k, c, N)] = tobf(th);
+            reinterpret_cast<__bf16*>(gv.wqT)[gQOff(l) + gSwz(c, k, 64)] = tobf(th);
+        }
+    }
+}
+
+// Rebuild the bf16 operand copies from theta (after a host upload).
+__global__ __launch_bounds__(256) void kGuideQuantize(GuideView gv) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kGQElems) return;
+    reinterpret_cast<__bf16*>(gv.wqT)[i] = tobf(0.f);  // padding rows stay zero
+    for (int l = 0; l < kGLayers; ++l) {
+        int w0 = gWOff(l), N = gLayerN(l);
+        int j = i - l * 4096;
+        if (j >= 0 && j < 64 * N) {
+            int k = j / N, c = j % N;
+            float th = gv.theta[w0 + j];
+            reinterpret_cast<__bf16*>(gv.wq)[gQOff(l) + gSwz(k, c, N)] = tobf(th);
+        }
+    }
+}
+__global__ __launch_bounds__(256) void kGuideQuantizeT(GuideView gv) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    for (int l = 0; l < kGLayers; ++l) {
+        int w0 = gWOff(l), N = gLayerN(l);
+        int j = i - l * 4096;
+        if (j >= 0 && j < 64 * N) {
+            int k = j / N, c = j % N;
+            reinterpret_cast<__bf16*>(gv.wqT)[gQOff(l) + gSwz(c, k, 64)] = tobf(gv.theta[w0 + j]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+size_t guideInferLds() {
+    return size_t(kGQElems) * 2 + size_t(kWavesPerBlock) * 2 * kTile * 2 +
+           size_t(kWavesPerBlock) * 16 * kGOut * 4;
+}
+size_t guideTrainLds() {
+    return size_t(kGQElems) * 2 + size_t(kWavesPerBlock) * 7 * kTile * 2 +
+           size_t(kWavesPerBlock) * 16 * kGOut * 4;
+}
+
+hipError_t guideSetLdsLimits() {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideInfer),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(guideInferLds()));
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideFwdBwd),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(guideTrainLds()));
+}
+
+hipError_t launchGuideInfer(const GuideView& gv, const uint32_t* queue, const uint32_t* count,
+                            uint32_t countDirect, const float4* pos, const float4* nrm,
+                            const float4* wo, float* params, float* rawOut, uint32_t grid,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(kGuideInfer, dim3(grid), dim3(256), guideInferLds(), st, gv, queue, count,
+                       countDirect, pos, nrm, wo, params, rawOut);
+    return hipGetLastError();
+}
+
+hipError_t launchGuideQuantize(const GuideView& gv, hipStream_t st) {
+    hipLaunchKernelGGL(kGuideQuantize, dim3(kGQElems / 256), dim3(256), 0, st, gv);
+    hipLaunchKernelGGL(kGuideQuantizeT, dim3(kGQElems / 256), dim3(256), 0, st, gv);
+    return hipGetLastError();
+}
+
+hipError_t launchGuideTrainStep(const GuideView& gv, const GuideSample* samples, uint32_t n,
+                                uint16_t* Hsave16, uint16_t* Dsave16, float* lossSum, float lr,
+                                uint32_t step, bool applyAdam, hipStream_t st) {
+    __bf16* Hsave = reinterpret_cast<__bf16*>(Hsave16);
+    __bf16* Dsave = reinterpret_cast<__bf16*>(Dsave16);
+    if (n == 0) return hipSuccess;
+    uint32_t blocks = (n + 63) / 64;
+    if (blocks > 512) blocks = 512;
+    hipLaunchKernelGGL(kGuideFwdBwd, dim3(blocks), dim3(256), guideTrainLds(), st, gv, samples, n,
+                       1.0f / float(n), Hsave, Dsave, lossSum);
+    const uint32_t chunk = 2048;
+    dim3 wg(16 * kGLayers, (n + chunk - 1) / chunk);
+    hipLaunchKernelGGL(kGuideWeightGrad, wg, dim3(64), 0, st, gv, Hsave, Dsave, n, chunk);
+    if (applyAdam) {
+        const float b1 = 0.9f, b2 = 0.999f;
+        float c1 = 1.f - powf(b1, float(step)), c2 = 1.f - powf(b2, float(step));
+        hipLaunchKernelGGL(kGuideAdam, dim3((kGParams + 255) / 256), dim3(256), 0, st, gv, lr, b1,
+                           b2, 1e-8f, c1, c2);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace crh

@@ -1,0 +1,85 @@
+"""Host wrapper of the learned path guide (crh_guide_* in include/crayhip.h).
+
+The network maps (position, shading normal, outgoing direction) to an
+8-lobe von Mises-Fisher mixture over incident directions; it runs and trains
+on the GPU (BF16 MFMA, FP32 accumulation, Adam).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .core import get_context
+
+
+class Guide:
+    def __init__(self, bmin, bmax, seed: int = 1234, device: int = 0):
+        self.ctx = get_context(device)
+        lo = np.ascontiguousarray(bmin, np.float32)
+        hi = np.ascontiguousarray(bmax, np.float32)
+        h = C.c_void_p()
+        L.call("crh_guide_create", self.ctx.handle, L.ptr(lo), L.ptr(hi), seed, C.byref(h))
+        self.handle = h
+        self.bmin, self.bmax = lo, hi
+
+    def set_params(self, theta: np.ndarray):
+        t = np.ascontiguousarray(theta, np.float32).reshape(-1)
+        L.call("crh_guide_set_params", self.handle, L.ptr(t), len(t))
+
+    def get_params(self) -> np.ndarray:
+        t = np.zeros(L.GUIDE_PARAMS, np.float32)
+        L.call("crh_guide_get_params", self.handle, L.ptr(t), len(t))
+        return t
+
+    def eval(self, pos, nrm, wo):
+        """Returns (raw outputs (n, 48), mixture (n, 40))."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+        wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+        n = len(pos)
+        raw = np.zeros((n, L.GUIDE_OUTPUTS), np.float32)
+        mix = np.zeros((n, 40), np.float32)
+        L.call("crh_guide_eval", self.handle, L.ptr(pos), L.ptr(nrm), L.ptr(wo), n, L.ptr(raw),
+               L.ptr(mix))
+        return raw, mix
+
+    @staticmethod
+    def pack_samples(pos, nrm, wo, wi, weight) -> np.ndarray:
+        n = len(pos)
+        s = np.zeros((n, 16), np.float32)
+        s[:, 0:3], s[:, 4:7], s[:, 8:11], s[:, 12:15] = pos, nrm, wo, wi
+        s[:, 15] = weight
+        return s
+
+    def train(self, samples: np.ndarray, lr: float = 1e-2, apply_adam: bool = True):
+        """One step; returns (loss, gradient or None)."""
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 16)
+        loss = np.zeros(1, np.float32)
+        grad = None if apply_adam else np.zeros(L.GUIDE_PARAMS, np.float32)
+        L.call("crh_guide_train", self.handle, L.ptr(s), len(s), lr, 1 if apply_adam else 0,
+               L.ptr(loss), None if grad is None else L.ptr(grad))
+        return float(loss[0]), grad
+
+    def __del__(self):
+        if getattr(self, "handle", None) and L._lib is not None:
+            L._lib.crh_guide_destroy(self.handle)
+            self.handle = None
+
+
+def flatten_params(P) -> np.ndarray:
+    """oracle.guiding parameter list -> flat crh layout (W row-major, b)."""
+    return np.concatenate([np.concatenate([W.reshape(-1), b]) for W, b in P]).astype(np.float32)
+
+
+def unflatten_params(theta: np.ndarray):
+    shapes = [(64, 64), (64, 64), (64, 64), (64, 64), (64, 48)]
+    P, off = [], 0
+    for k, nn in shapes:
+        W = theta[off:off + k * nn].reshape(k, nn).astype(np.float64)
+        off += k * nn
+        b = theta[off:off + nn].astype(np.float64)
+        off += nn
+        P.append([W, b])
+    return P

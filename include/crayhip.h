@@ -39,6 +39,7 @@ typedef struct crh_bvh crh_bvh;
 typedef struct crh_scene crh_scene;
 typedef struct crh_film crh_film;
 typedef struct crh_buffer crh_buffer;
+typedef struct crh_guide crh_guide;
 
 /* ---- value types (layouts fixed; static-asserted in the library) ------ */
 
@@ -284,6 +285,31 @@ int crh_film_destroy(crh_film* film);
 int crh_render(crh_scene* scene, const crh_camera* cam,
                const crh_integrator_cfg* cfg, uint32_t first_sample,
                uint32_t spp, crh_film* film, crh_stats* stats);
+
+/* ---- learned path guiding (no counterpart in the reference) ------------
+ * A 64-wide MLP (4 hidden ReLU layers, BF16 weights on MFMA, FP32
+ * accumulation) maps (position, shading normal, outgoing direction) to an
+ * 8-lobe von Mises-Fisher mixture; trained online with Adam on the KL
+ * divergence to the renderer's radiance samples.  Parameter layout: for each
+ * of the 5 layers, W (64 x N row-major, N = 64, 64, 64, 64, 48) then b (N);
+ * CRH_GUIDE_PARAMS floats in total. */
+#define CRH_GUIDE_PARAMS 19760u
+#define CRH_GUIDE_OUTPUTS 48u
+/* scene bounds normalise the position encoding; seed picks the He init */
+int crh_guide_create(crh_context* ctx, const float* bmin, const float* bmax,
+                     uint32_t seed, crh_guide** out);
+int crh_guide_set_params(crh_guide* g, const float* theta, uint32_t n);
+int crh_guide_get_params(const crh_guide* g, float* theta, uint32_t n);
+/* n queries (pos, nrm, wo: 3 floats each).  raw: n*48 network outputs;
+ * mixture: n*40 = weights[8], means[8][3], kappas[8].  Either may be NULL. */
+int crh_guide_eval(crh_guide* g, const float* pos, const float* nrm,
+                   const float* wo, uint32_t n, float* raw, float* mixture);
+/* One training step on n samples of 16 floats (pos, _, nrm, _, wo, _, wi,
+ * target/pdf).  apply_adam = 0 leaves the parameters and returns the
+ * gradient in grad (CRH_GUIDE_PARAMS floats, may be NULL); loss may be NULL. */
+int crh_guide_train(crh_guide* g, const float* samples, uint32_t n, float lr,
+                    int apply_adam, float* loss, float* grad);
+int crh_guide_destroy(crh_guide* g);
 
 #ifdef __cplusplus
 }

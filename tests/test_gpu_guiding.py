@@ -1,0 +1,105 @@
+"""GPU guiding network vs the float64 oracle (oracle/guiding.py): forward
+outputs, mixture head, gradients (BF16-rounding-aware tolerances) and Adam."""
+import numpy as np
+import pytest
+
+from craystal_amd import _lib as L
+from craystal_amd.guiding import Guide, flatten_params, unflatten_params
+from oracle import guiding as G
+
+pytestmark = pytest.mark.gpu
+
+BMIN, BMAX = np.array([-1.0, -1.0, -1.0]), np.array([2.0, 2.0, 2.0])
+
+
+def _inputs(n, seed=0):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-1, 2, (n, 3)).astype(np.float32)
+    nrm = rng.normal(size=(n, 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    wo = rng.normal(size=(n, 3))
+    wo = (wo / np.linalg.norm(wo, axis=1, keepdims=True)).astype(np.float32)
+    wi = rng.normal(size=(n, 3))
+    wi = (wi / np.linalg.norm(wi, axis=1, keepdims=True)).astype(np.float32)
+    y = rng.uniform(0.1, 2.0, n).astype(np.float32)
+    return pos, nrm, wo, wi, y
+
+
+def _bf16_tol(ref, rel=0.02):
+    return rel * (np.abs(ref) + 0.05 * np.abs(ref).max() + 1e-6)
+
+
+def test_mfma_forward_matches_oracle(gpu):
+    P = G.init_params(3)
+    g = Guide(BMIN, BMAX)
+    g.set_params(flatten_params(P))
+    assert np.array_equal(g.get_params(), flatten_params(P))
+    pos, nrm, wo, _, _ = _inputs(1000)        # not a multiple of 64: ragged tail
+    raw, mix = g.eval(pos, nrm, wo)
+    X = G.features(pos, nrm, wo, BMIN, BMAX)
+    ref, _ = G.forward(P, X, quant=True)
+    assert np.all(np.abs(raw - ref) <= _bf16_tol(ref)), np.abs(raw - ref).max()
+    w, mu, kap = G.head(raw.astype(np.float64))
+    assert np.allclose(mix[:, :8], w, atol=1e-5)
+    assert np.allclose(mix[:, 8:32].reshape(-1, 8, 3), mu, atol=1e-5)
+    assert np.allclose(mix[:, 32:40], kap, rtol=1e-5)
+    assert np.allclose(mix[:, :8].sum(1), 1, atol=1e-5)
+
+
+def test_gradients_match_oracle(gpu):
+    P = G.init_params(5)
+    g = Guide(BMIN, BMAX)
+    g.set_params(flatten_params(P))
+    pos, nrm, wo, wi, y = _inputs(2048, 1)
+    q = np.full(len(pos), 1 / (4 * np.pi), np.float32)
+    loss, grad = g.train(Guide.pack_samples(pos, nrm, wo, wi, y / q), apply_adam=False)
+    X = G.features(pos, nrm, wo, BMIN, BMAX)
+    out, acts = G.forward(P, X, quant=True)
+    ref_loss, Gout = G.loss_and_grad_out(out, wi.astype(np.float64), y, q)
+    ref = flatten_params([[gw, gb] for gw, gb in G.backward(P, acts, G.bf16(Gout))])
+    assert abs(loss - ref_loss) <= 2e-3 * abs(ref_loss)
+    cos = grad @ ref / (np.linalg.norm(grad) * np.linalg.norm(ref))
+    assert cos > 0.995, cos
+    assert np.linalg.norm(grad - ref) <= 0.06 * np.linalg.norm(ref)
+    # the parameters did not move
+    assert np.array_equal(g.get_params(), flatten_params(P))
+
+
+def test_adam_step_matches_oracle(gpu):
+    P = G.init_params(9)
+    g = Guide(BMIN, BMAX)
+    g.set_params(flatten_params(P))
+    pos, nrm, wo, wi, y = _inputs(1024, 2)
+    q = np.full(len(pos), 1 / (4 * np.pi), np.float32)
+    s = Guide.pack_samples(pos, nrm, wo, wi, y / q)
+    _, grad = g.train(s, apply_adam=False)
+    g.train(s, lr=1e-2, apply_adam=True)
+    theta = g.get_params()
+    opt = G.Adam([[W.copy(), b.copy()] for W, b in P], lr=1e-2)
+    expect = flatten_params(opt.step([[W.copy(), b.copy()] for W, b in P],
+                                     [tuple(x) for x in unflatten_params(grad)]))
+    moved = theta - flatten_params(P)
+    # first Adam step: |delta| = lr where the gradient is not ~0
+    big = np.abs(grad) > 1e-6
+    assert np.allclose(np.abs(moved[big]), 1e-2, rtol=1e-3)
+    assert np.mean(np.sign(moved[big]) == np.sign(expect[big] - flatten_params(P)[big])) > 0.99
+
+
+def test_training_fits_a_target_lobe(gpu):
+    """On-GPU training against a fixed target distribution lowers the KL loss
+    and concentrates the mixture toward the target mean."""
+    g = Guide(BMIN, BMAX, seed=21)
+    rng = np.random.default_rng(7)
+    pos, nrm, wo, _, _ = _inputs(4096, 3)
+    mu = np.array([0.0, 0.0, 1.0])
+    losses = []
+    for it in range(120):
+        wi = rng.normal(size=(4096, 3))
+        wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+        y = G.vmf_pdf(mu[None], np.array([10.0]), wi)
+        loss, _ = g.train(Guide.pack_samples(pos, nrm, wo, wi, y * 4 * np.pi), lr=1e-2)
+        losses.append(loss)
+    assert np.mean(losses[-10:]) < 0.7 * np.mean(losses[:10])
+    raw, _ = g.eval(pos[:256], nrm[:256], wo[:256])
+    pdf = G.mixture_pdf(raw.astype(np.float64), np.tile(mu, (256, 1)))
+    assert np.median(pdf) > 4 / (4 * np.pi)
