@@ -98,6 +98,11 @@ class crh_tuning(C.Structure):
                 ("wf_trace_block", u32), ("wf_shade_grid", u32), ("reserved", u32 * 3)]
 
 
+class crh_guided_cfg(C.Structure):
+    _fields_ = [("alpha", f32), ("train", u32), ("train_batch", u32), ("train_steps", u32),
+                ("lr", f32), ("reserved", u32 * 3)]
+
+
 class crh_stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("main_kernel_ms", C.c_double),
                 ("total_ms", C.c_double), ("rays", C.c_uint64),
@@ -155,6 +160,8 @@ SIGNATURES = {
     "crh_guide_eval": [_P, _P, _P, _P, u32, _P, _P],
     "crh_guide_train": [_P, _P, u32, f32, C.c_int, _P, _P],
     "crh_guide_destroy": [_P],
+    "crh_render_guided": [_P, C.POINTER(crh_camera), C.POINTER(crh_integrator_cfg), _P,
+                          C.POINTER(crh_guided_cfg), u32, u32, _P, C.POINTER(crh_stats)],
 }
 GUIDE_PARAMS = 19760
 GUIDE_OUTPUTS = 48

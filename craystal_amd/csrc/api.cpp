@@ -161,6 +161,9 @@ struct crh_film {
     DevMem wfArena;
     crh::WfState wf{};
     bool wfReady = false;
+    // Guided-mode state, allocated on first guided render.
+    DevMem gArena, gSamples;
+    bool guidedReady = false;
 };
 
 namespace {
@@ -213,7 +216,43 @@ hipError_t ensureWavefront(crh_film* f) {
     w.work = (unsigned long long*)take(align(64));
     w.rad = f->L.as<float4>();
     w.P = f->P.as<float2>();
+    w.gPos = w.gNrm = w.gWo = w.rec = nullptr;
+    w.gMix = nullptr;
+    w.recId = nullptr;
+    w.recCap = 0;
     f->wfReady = true;
+    return hipSuccess;
+}
+
+// Guided-mode buffers: per-path shading frame and mixture, vertex records
+// (4 per path) and the training samples made from them.
+hipError_t ensureGuided(crh_film* f) {
+    hipError_t e = ensureWavefront(f);
+    if (e != hipSuccess || f->guidedReady) return e;
+    const size_t n = size_t(f->w) * f->h;
+    const size_t cap = n * 4;
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t total = 3 * align(n * 16) + align(n * 40 * 4) + align(n * 4) +
+                         align(cap * crh::kRecF4 * 16);
+    e = f->gArena.alloc(total);
+    if (e == hipSuccess) e = f->gSamples.alloc(cap * sizeof(crh::GuideSample));
+    if (e != hipSuccess) return e;
+    char* base = f->gArena.as<char>();
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base + off;
+        off += align(bytes);
+        return p;
+    };
+    crh::WfState& w = f->wf;
+    w.gPos = (float4*)take(n * 16);
+    w.gNrm = (float4*)take(n * 16);
+    w.gWo = (float4*)take(n * 16);
+    w.gMix = (float*)take(n * 40 * 4);
+    w.recId = (uint32_t*)take(n * 4);
+    w.rec = (float4*)take(cap * crh::kRecF4 * 16);
+    w.recCap = uint32_t(cap);
+    f->guidedReady = true;
     return hipSuccess;
 }
 
@@ -1031,6 +1070,87 @@ int crh_guide_destroy(crh_guide* g) {
     (void)hipStreamSynchronize(g->ctx->stream);
     delete g;
     return CRH_OK;
+}
+
+int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_cfg* cfg,
+                      crh_guide* g, const crh_guided_cfg* gc, uint32_t first_sample,
+                      uint32_t spp, crh_film* film, crh_stats* stats) {
+    ARG_CHECK(s && cam && cfg && g && gc && film, "NULL argument");
+    ARG_CHECK(film->ctx == s->ctx && g->ctx == s->ctx, "objects belong to different contexts");
+    ARG_CHECK(gc->alpha >= 0.f && gc->alpha <= 1.f, "alpha must be in [0, 1]");
+    ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
+    ARG_CHECK(!gc->train || (gc->train_batch >= 64 && gc->train_batch <= (1u << 22)),
+              "train_batch must be in [64, 2^22]");
+    GUARD_BEGIN
+    crh_context* ctx = s->ctx;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    crh::CameraView cv;
+    for (int k = 0; k < 3; ++k) {
+        cv.pos[k] = cam->pos[k];
+        cv.u[k] = cam->u[k];
+        cv.v[k] = cam->v[k];
+        cv.w[k] = cam->w[k];
+    }
+    cv.hh = float(std::tan(double(cam->fov_y) * 0.5 * 0.01745329251994329576923690768489));
+    cv.aspect = float(film->w) / float(film->h);
+    cv.width = film->w;
+    cv.height = film->h;
+    crh::IntegratorView iv{cfg->max_depth, cfg->rr_threshold, cfg->rr_start_depth,
+                           cfg->use_mis ? 1u : 0u};
+    double t0 = nowMs();
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(ensureGuided(film));
+    HIP_TRY(hipMemsetAsync(film->wf.work, 0, 64, ctx->stream));
+    const crh::Tuning tun = effective(ctx->tuning, s->view);
+    const crh::WfLaunch wl = wfLaunch(tun);
+    const uint32_t batch = gc->train_batch;
+    if (gc->train && size_t(batch) > g->capSamples) {
+        HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * batch * 64 * 2));
+        HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * batch * 64 * 2));
+        g->capSamples = batch;
+    }
+    double mainMs = 0.0;
+    uint32_t lastSamples = 0;
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    for (uint32_t i = 0; i < spp; ++i) {
+        HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, film->wf, wl, false,
+                                   ctx->stream, &g->view, gc->alpha,
+                                   gc->train ? film->gSamples.as<crh::GuideSample>() : nullptr));
+        HIP_TRY(crh::launchFilmGather(film->L.as<float4>(), film->P.as<float2>(), film->w,
+                                      film->h, film->sum.as<float>(), film->wsum.as<float>(),
+                                      ctx->stream));
+        if (gc->train) {
+            uint32_t produced = 0;
+            HIP_TRY(hipMemcpyAsync(&produced, film->wf.counters + crh::kCTrain, 4,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            lastSamples = produced;
+            const crh::GuideSample* smp = film->gSamples.as<crh::GuideSample>();
+            for (uint32_t k = 0; k < gc->train_steps && size_t(k + 1) * batch <= produced; ++k) {
+                g->step++;
+                HIP_TRY(crh::launchGuideTrainStep(g->view, smp + size_t(k) * batch, batch,
+                                                  g->hsave.as<uint16_t>(), g->dsave.as<uint16_t>(),
+                                                  g->loss.as<float>(), gc->lr, g->step, true,
+                                                  ctx->stream));
+            }
+        }
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    if (stats) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        unsigned long long c[3] = {0, 0, 0};
+        HIP_TRY(hipMemcpy(c, film->wf.work, sizeof(c), hipMemcpyDeviceToHost));
+        stats->kernel_ms = ms;
+        stats->main_kernel_ms = mainMs;
+        stats->total_ms = nowMs() - t0;
+        stats->rays = c[0];
+        stats->node_visits = lastSamples;
+        stats->launches = spp;
+    }
+    return CRH_OK;
+    GUARD_END
 }
 
 }  // extern "C"

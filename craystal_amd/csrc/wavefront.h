@@ -14,7 +14,9 @@ enum : uint32_t {
     kCMis1 = 3,  // MIS-ray queue, pong
     kCShadow = 4,
     kCCont = 5,  // paths to shade after the shadow query
-    kWfCounters = 6,
+    kCRec = 6,   // training records written this pass
+    kCTrain = 7, // training samples produced from the records
+    kWfCounters = 8,
 };
 
 struct WfState {
@@ -45,7 +47,18 @@ struct WfState {
     uint32_t* qCont;
     uint32_t* counters;
     unsigned long long* work;  // rays, node visits, triangle tests
+    // Guided mode only (null otherwise).
+    float4* gPos;     // shading point of the vertex being continued
+    float4* gNrm;     // shading normal
+    float4* gWo;      // outgoing (view) direction
+    float* gMix;      // 40 floats per path: mixture weights, means, kappas
+    float4* rec;      // training records, kRecF4 float4 each
+    uint32_t* recId;  // per path: record of its last guided vertex
+    uint32_t recCap;
 };
+
+constexpr uint32_t kRecF4 = 7;  // float4s per training record
+constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
 
 struct WfLaunch {
     uint32_t traceGrid = 2048;   // blocks for the traversal kernels
@@ -54,8 +67,16 @@ struct WfLaunch {
     uint32_t privStack = 0;      // 1: traversal stack in private memory
 };
 
+struct GuideView;
+struct GuideSample;
+
+// guide != null: guided continuation sampling (one-sample MIS with the
+// guide's mixture, fraction alpha for BSDF sampling).  samples != null:
+// also convert this pass's vertex records into training samples.
 hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                          const IntegratorView& cfg, uint32_t si, WfState& w,
-                         const WfLaunch& L, bool countWork, hipStream_t st);
+                         const WfLaunch& L, bool countWork, hipStream_t st,
+                         const GuideView* guide = nullptr, float alpha = 0.5f,
+                         GuideSample* samples = nullptr);
 
 }  // namespace crh

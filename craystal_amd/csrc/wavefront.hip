@@ -26,6 +26,8 @@
 // needed between bounces.
 #include <hip/hip_runtime.h>
 
+#include "guide_device.h"
+#include "guiding_kernels.h"
 #include "integrator.h"
 #include "wavefront.h"
 
@@ -111,6 +113,7 @@ __global__ __launch_bounds__(256) void kWfGenerate(WfState w, CameraView cam,
         stS(w.beta, p, sp(1.f));
         stS(w.rad, p, sp(0.f));
         w.flags[p] = 0u;
+        if (w.recId) w.recId[p] = kNoRecord;
         w.P[p] = make_float2(pix.x, pix.y);
         w.qExt[0][p] = p;
     }
@@ -182,6 +185,7 @@ __global__ __launch_bounds__(256) void kWfMisResolve(SceneView s, WfState w,
 
 // Process the extension hit: environment / emitter terms, or light
 // selection producing a shadow ray (Core/Integrator.cu:165-190, :25-53).
+template <bool GUIDED>
 __global__ __launch_bounds__(256) void kWfShadeHit(SceneView s, WfState w,
                                                    uint32_t cur) {
     const uint32_t n = w.counters[kCExt0 + cur];
@@ -189,6 +193,16 @@ __global__ __launch_bounds__(256) void kWfShadeHit(SceneView s, WfState w,
         uint32_t p = w.qExt[cur][i];
         uint32_t fl = w.flags[p];
         Spec beta = ldS(w.beta, p);
+        if (GUIDED) {
+            // Radiance before anything found along the guided direction of
+            // the previous vertex: the start of that direction's estimate.
+            uint32_t rid = w.recId[p];
+            if (rid != kNoRecord) {
+                float4 r = w.rad[p];
+                w.rec[size_t(rid) * kRecF4 + 6] = make_float4(r.x, r.y, r.z, 1.f);
+                w.recId[p] = kNoRecord;
+            }
+        }
         bool shade = false, shadow = false;
         if (!(fl & F_EXTHIT)) {
             stS(w.rad, p, ldS(w.rad, p) + sp3(s.env) * beta);
@@ -201,6 +215,11 @@ __global__ __launch_bounds__(256) void kWfShadeHit(SceneView s, WfState w,
             } else {
                 shade = true;
                 Isect it = makeIsect(s, r, h);
+                if (GUIDED) {
+                    w.gPos[p] = make_float4(it.pos.x, it.pos.y, it.pos.z, 0.f);
+                    w.gNrm[p] = make_float4(it.frame.N.x, it.frame.N.y, it.frame.N.z, 0.f);
+                    w.gWo[p] = make_float4(it.view.x, it.view.y, it.view.z, 0.f);
+                }
                 Sampler smp = loadSampler(w, p);
                 LightGeom g = lightGeom(s, it, smp);
                 w.rng[p] = smp.state;
@@ -242,10 +261,11 @@ __global__ __launch_bounds__(256) void kWfShadow(SceneView s, WfState w) {
 
 // Light weight, MIS BSDF ray and the continuation sample
 // (Core/Integrator.cu:57-99, :186-223).
-template <bool MIS>
+template <bool MIS, bool GUIDED>
 __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
                                                     IntegratorView cfg,
-                                                    uint32_t cur, uint32_t depth) {
+                                                    uint32_t cur, uint32_t depth,
+                                                    float alpha) {
     const uint32_t n = w.counters[kCCont];
     const uint32_t nxt = cur ^ 1u;
     PERSISTENT_LOOP(i, n) {
@@ -299,17 +319,68 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
         bool alive = true;
         F3 wi;
         float pdf = 0.0f;
-        Spec f = bsdfSampleEval(bsdf, smp, it.view, wi, pdf);
+        Spec f;
+        bool guided = GUIDED && bsdf.type != 2u;
+        float qmix = 0.f;
+        if (!guided) {
+            f = bsdfSampleEval(bsdf, smp, it.view, wi, pdf);
+        } else {
+            // One-sample MIS of BSDF sampling (fraction alpha) and the
+            // guide's mixture.  The weight f * (p_act / p_rep) / q_mix has
+            // the same expectation as the plain estimator's f / p_rep with
+            // wi ~ p_act, so guided and plain images converge to the same
+            // (reference) result; p_rep is the pdf the reference reports,
+            // p_act the density its sampler actually has.
+            Mixture mix = loadMixture(w.gMix + size_t(p) * 40);
+            F3 woL = toLocal(bsdf.frame, it.view);
+            F3 wiL;
+            if (smp.next1D() < alpha) {
+                F2 u = smp.next2D();
+                float dummy;
+                wiL = sampleLocal(bsdf, woL, u, dummy);
+                wi = toWorld(bsdf.frame, wiL);
+            } else {
+                float u0 = smp.next1D();
+                F2 u = smp.next2D();
+                wi = mixtureSample(mix, u0, u.x, u.y);
+                wiL = toLocal(bsdf.frame, wi);
+            }
+            f = evalLocal(bsdf, woL, wiL);
+            float prep = pdfLocal(bsdf, woL, wiL);
+            float pact = actualPdfLocal(bsdf, woL, wiL);
+            qmix = alpha * pact + (1.f - alpha) * mixturePdf(mix, wi);
+            pdf = (prep > 0.f && pact > 0.f && qmix > 0.f) ? qmix * prep / pact : 0.f;
+        }
+        bool record = false;
+        Spec betaW = beta;
         if (pdf <= 0.0f) {
             alive = false;
         } else {
             beta = beta * (f / pdf);
+            betaW = beta;
+            record = guided;
             Ray nr{offsetRayOrigin(it.pos, oriented(it)), r.tMin, wi, r.tMax};
             storeRay(w.rayO, w.rayD, p, nr);
             if (smp.next1D() < cfg.rrThreshold) alive = false;
             else if (!roulette(cfg, depth, smp, beta)) alive = false;
         }
         if (depth + 1 >= cfg.maxDepth) alive = false;
+        if (GUIDED) {
+            uint32_t slot = wavePush(&w.counters[kCRec], record);
+            if (record && slot < w.recCap) {
+                float4* R = w.rec + size_t(slot) * kRecF4;
+                R[0] = make_float4(it.pos.x, it.pos.y, it.pos.z, qmix);
+                R[1] = make_float4(it.frame.N.x, it.frame.N.y, it.frame.N.z, 0.f);
+                R[2] = make_float4(it.view.x, it.view.y, it.view.z, 0.f);
+                R[3] = make_float4(wi.x, wi.y, wi.z, __uint_as_float(p));
+                R[4] = make_float4(f.r, f.g, f.b, 0.f);
+                R[5] = make_float4(betaW.r, betaW.g, betaW.b, 0.f);  // before roulette
+                R[6] = make_float4(0.f, 0.f, 0.f, 0.f);
+                w.recId[p] = slot;
+            } else {
+                w.recId[p] = kNoRecord;
+            }
+        }
         w.rng[p] = smp.state;
         stS(w.beta, p, beta);
         stS(w.rad, p, rad);
@@ -318,6 +389,40 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
         if (alive) w.qExt[nxt][slot] = p;
         slot = wavePush(&w.counters[kCMis0 + nxt], misRay);
         if (misRay) w.qMis[nxt][slot] = p;
+    }
+}
+
+// Training samples from this pass's records: incident radiance along the
+// guided direction = (final radiance - radiance before that direction's
+// contributions) / throughput; target = luminance of (BSDF * cos) x that,
+// weight = target / sampling pdf.  Records whose path ended before the next
+// vertex (no snapshot) or with zero target are skipped.
+__global__ __launch_bounds__(256) void kWfRecordsToSamples(WfState w, GuideSample* out) {
+    const uint32_t n = min(w.counters[kCRec], w.recCap);
+    PERSISTENT_LOOP(r, n) {
+        const float4* R = w.rec + size_t(r) * kRecF4;
+        float4 snap = R[6];
+        bool ok = snap.w == 1.f;
+        float weight = 0.f;
+        if (ok) {
+            uint32_t p = __float_as_uint(R[3].w);
+            float4 e = w.rad[p], f = R[4], bw = R[5];
+            float li[3] = {bw.x > 0.f ? (e.x - snap.x) / bw.x : 0.f,
+                           bw.y > 0.f ? (e.y - snap.y) / bw.y : 0.f,
+                           bw.z > 0.f ? (e.z - snap.z) / bw.z : 0.f};
+            float y = (f.x * li[0] + f.y * li[1] + f.z * li[2]) / 3.f;
+            weight = y / R[0].w;
+            ok = weight > 0.f && isfinite(weight);
+        }
+        uint32_t slot = wavePush(&w.counters[kCTrain], ok);
+        if (ok) {
+            GuideSample& g = out[slot];
+            float4 a = R[0], b = R[1], c = R[2], d = R[3];
+            g.pos[0] = a.x; g.pos[1] = a.y; g.pos[2] = a.z; g.pos[3] = 0.f;
+            g.nrm[0] = b.x; g.nrm[1] = b.y; g.nrm[2] = b.z; g.nrm[3] = 0.f;
+            g.wo[0] = c.x; g.wo[1] = c.y; g.wo[2] = c.z; g.wo[3] = 0.f;
+            g.wi[0] = d.x; g.wi[1] = d.y; g.wi[2] = d.z; g.wi[3] = weight;
+        }
     }
 }
 
@@ -335,10 +440,12 @@ static void launchExtend(bool count, dim3 g, dim3 b, size_t lds, hipStream_t st,
 
 hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                          const IntegratorView& cfg, uint32_t si, WfState& w,
-                         const WfLaunch& L, bool countWork, hipStream_t st) {
+                         const WfLaunch& L, bool countWork, hipStream_t st,
+                         const GuideView* guide, float alpha, GuideSample* samples) {
     const uint32_t TB = L.traceBlock;
     const size_t lds = L.privStack ? 0 : size_t(s.stackCap) * TB * sizeof(uint2);
     const dim3 b256(256), bT(TB);
+    const bool guided = guide != nullptr;
     auto extend = [&](uint32_t cur, bool count) {
         if (L.privStack)
             launchExtend<true>(count, dim3(L.traceGrid), bT, lds, st, s, w, cur);
@@ -347,7 +454,8 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
     };
     hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
                        (1u << kCExt1) | (1u << kCMis0) | (1u << kCMis1) |
-                           (1u << kCShadow) | (1u << kCCont));
+                           (1u << kCShadow) | (1u << kCCont) | (1u << kCRec) |
+                           (1u << kCTrain));
     hipLaunchKernelGGL(kWfGenerate, dim3(L.shadeGrid), b256, 0, st, w, cam, si);
     uint32_t cur = 0;
     for (uint32_t d = 0; d < cfg.maxDepth; ++d) {
@@ -357,17 +465,31 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
         hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
                            (1u << (kCExt0 + nxt)) | (1u << (kCMis0 + nxt)) |
                                (1u << kCShadow) | (1u << kCCont));
-        hipLaunchKernelGGL(kWfShadeHit, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
+        if (guided)
+            hipLaunchKernelGGL((kWfShadeHit<true>), dim3(L.shadeGrid), b256, 0, st, s, w, cur);
+        else
+            hipLaunchKernelGGL((kWfShadeHit<false>), dim3(L.shadeGrid), b256, 0, st, s, w, cur);
         if (L.privStack)
             hipLaunchKernelGGL((kWfShadow<true>), dim3(L.traceGrid), bT, lds, st, s, w);
         else
             hipLaunchKernelGGL((kWfShadow<false>), dim3(L.traceGrid), bT, lds, st, s, w);
-        if (cfg.useMIS)
-            hipLaunchKernelGGL((kWfShadeCont<true>), dim3(L.shadeGrid), b256, 0, st, s, w,
-                               cfg, cur, d);
-        else
-            hipLaunchKernelGGL((kWfShadeCont<false>), dim3(L.shadeGrid), b256, 0, st, s,
-                               w, cfg, cur, d);
+        if (guided) {
+            hipError_t e = launchGuideInfer(*guide, w.qCont, w.counters + kCCont, 0u, w.gPos,
+                                            w.gNrm, w.gWo, w.gMix, nullptr, L.shadeGrid, st);
+            if (e != hipSuccess) return e;
+            if (cfg.useMIS)
+                hipLaunchKernelGGL((kWfShadeCont<true, true>), dim3(L.shadeGrid), b256, 0, st, s,
+                                   w, cfg, cur, d, alpha);
+            else
+                hipLaunchKernelGGL((kWfShadeCont<false, true>), dim3(L.shadeGrid), b256, 0, st, s,
+                                   w, cfg, cur, d, alpha);
+        } else if (cfg.useMIS) {
+            hipLaunchKernelGGL((kWfShadeCont<true, false>), dim3(L.shadeGrid), b256, 0, st, s, w,
+                               cfg, cur, d, alpha);
+        } else {
+            hipLaunchKernelGGL((kWfShadeCont<false, false>), dim3(L.shadeGrid), b256, 0, st, s,
+                               w, cfg, cur, d, alpha);
+        }
         cur = nxt;
     }
     // MIS rays spawned at the last bounce: trace and resolve them.
@@ -375,6 +497,8 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                        1u << (kCExt0 + cur));
     extend(cur, false);
     hipLaunchKernelGGL(kWfMisResolve, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
+    if (guided && samples)
+        hipLaunchKernelGGL(kWfRecordsToSamples, dim3(L.shadeGrid), b256, 0, st, w, samples);
     return hipGetLastError();
 }
 

@@ -68,6 +68,40 @@ class Guide:
             self.handle = None
 
 
+class GuidedIntegrator:
+    """Wavefront path tracer with one-sample MIS between BSDF sampling
+    (fraction alpha) and the guide; optionally trains the guide online on
+    every pass's vertices (Adam, KL divergence)."""
+
+    def __init__(self, configs, guide: Guide, alpha: float = 0.5, lr: float = 1e-2,
+                 train_batch: int = 65536, train_steps: int = 4):
+        self.configs = configs
+        self.guide = guide
+        self.alpha, self.lr = alpha, lr
+        self.train_batch, self.train_steps = train_batch, train_steps
+        self.last_stats: dict = {}
+
+    def dispatch(self, scene, spp: int, first_sample: int = 0, train: bool = False,
+                 readback: bool = True) -> dict:
+        cam = scene.getCamera()
+        sensor = cam.getSensor()
+        g = L.crh_guided_cfg()
+        g.alpha = self.alpha
+        g.train = 1 if train else 0
+        g.train_batch = self.train_batch
+        g.train_steps = self.train_steps
+        g.lr = self.lr
+        st = L.crh_stats()
+        cfg = self.configs.to_c()
+        L.call("crh_render_guided", scene.handle, C.byref(cam.data), C.byref(cfg),
+               self.guide.handle, C.byref(g), first_sample, spp, sensor.handle, C.byref(st))
+        if readback:
+            sensor.readbackDeviceData()
+        self.last_stats = {"kernel_ms": st.kernel_ms, "total_ms": st.total_ms, "rays": st.rays,
+                           "train_samples": st.node_visits}
+        return self.last_stats
+
+
 def flatten_params(P) -> np.ndarray:
     """oracle.guiding parameter list -> flat crh layout (W row-major, b)."""
     return np.concatenate([np.concatenate([W.reshape(-1), b]) for W, b in P]).astype(np.float32)

@@ -234,6 +234,57 @@ def spheres_scene(n_spheres: int = 48, subdiv: int = 5, seed: int = SCENE_SEED,
     return data, cam
 
 
+def occluded_room(n_objects: int = 200, subdiv: int = 3, seed: int = SCENE_SEED,
+                  light_radiance: float = 50.0):
+    """Closed 10 x 10 x 3 m room lit only through a 0.5 x 0.5 m opening in
+    the +x wall by an exterior 4 x 4 m area light (radiance 50), furnished
+    with seeded boxes and icospheres (about 150k triangles at the default
+    sizes) in diffuse and glossy Principled materials (roughness
+    U[0.1, 0.6]).  1024 x 1024, max depth 16."""
+    rng = Stream(seed)
+    mats = [diffuse((0.7, 0.7, 0.7)), emitter((light_radiance,) * 3)]
+    W = 0
+    meshes = [
+        quad((-5, 0, 5), (10, 0, 0), (0, 0, -10), W),      # floor, +y
+        quad((-5, 3, -5), (10, 0, 0), (0, 0, 10), W),      # ceiling, -y
+        quad((-5, 0, -5), (10, 0, 0), (0, 3, 0), W),       # z = -5, +z
+        quad((5, 0, 5), (-10, 0, 0), (0, 3, 0), W),        # z = +5, -z
+        quad((-5, 0, 5), (0, 0, -10), (0, 3, 0), W),       # x = -5, +x
+    ]
+    # x = +5 wall (normal -x) with a 0.5 x 0.5 opening at y in [1.25, 1.75],
+    # z in [-0.25, 0.25]: four quads around the hole.
+    meshes += [
+        quad((5, 0, -5), (0, 0, 10), (0, 1.25, 0), W),             # below
+        quad((5, 1.75, -5), (0, 0, 10), (0, 1.25, 0), W),          # above
+        quad((5, 1.25, -5), (0, 0, 4.75), (0, 0.5, 0), W),         # z < -0.25
+        quad((5, 1.25, 0.25), (0, 0, 4.75), (0, 0.5, 0), W),       # z > 0.25
+    ]
+    # exterior light facing -x, 2 m outside the opening
+    meshes.append(quad((7, -0.5, -2), (0, 0, 4), (0, 4, 0), 1))   # faces -x
+    for i in range(n_objects):
+        x, z = rng.uniform(-4.6, 4.6), rng.uniform(-4.6, 4.6)
+        if rng.uniform() < 0.5:
+            mats.append(diffuse((rng.uniform(0.3, 0.9), rng.uniform(0.3, 0.9),
+                                 rng.uniform(0.3, 0.9))))
+        else:
+            mats.append(principled((rng.uniform(), rng.uniform(), rng.uniform()),
+                                   rng.uniform(0.1, 0.6), 0.0))
+        mid = len(mats) - 1
+        if rng.uniform() < 0.5:
+            r = rng.uniform(0.15, 0.45)
+            sd = subdiv + 1 if rng.uniform() < 0.05 else subdiv
+            meshes.append(icosphere((x, r, z), r, sd, mid))
+        else:
+            sx, sy, sz = rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8)
+            meshes.append(box_mesh((x, sy / 2, z), (sx, sy, sz),
+                                   math.radians(rng.uniform(0, 90)), mid))
+    data = SceneData(meshes=meshes, materials=mats)
+    data.emissiveIndex = _emissive_prims(data)
+    cam = dict(pos=(-4.2, 1.7, -4.2), target=(2.0, 0.7, 2.0), up=(0.0, 1.0, 0.0), fov=60.0,
+               size=(1024, 1024), spp=None, max_depth=16, rr_start_depth=3)
+    return data, cam
+
+
 def make_camera(cam: dict, sensor=None) -> Camera:
     c = Camera()
     c.setWorldPosition(cam["pos"])

@@ -311,6 +311,26 @@ int crh_guide_train(crh_guide* g, const float* samples, uint32_t n, float lr,
                     int apply_adam, float* loss, float* grad);
 int crh_guide_destroy(crh_guide* g);
 
+/* Guided rendering: the wavefront integrator with one-sample MIS between
+ * BSDF sampling (fraction alpha) and the guide's mixture at every
+ * Lambertian / Principled vertex; the estimator's expectation equals the
+ * plain integrator's.  With train != 0, every pass's vertices become
+ * training samples and up to train_steps Adam steps of train_batch samples
+ * follow the pass. */
+typedef struct crh_guided_cfg {
+    float alpha;
+    uint32_t train;
+    uint32_t train_batch;
+    uint32_t train_steps;
+    float lr;
+    uint32_t reserved[3];
+} crh_guided_cfg;
+/* stats->node_visits receives the training samples produced (last pass). */
+int crh_render_guided(crh_scene* scene, const crh_camera* cam,
+                      const crh_integrator_cfg* cfg, crh_guide* guide,
+                      const crh_guided_cfg* gcfg, uint32_t first_sample,
+                      uint32_t spp, crh_film* film, crh_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,3 +103,44 @@ def test_training_fits_a_target_lobe(gpu):
     raw, _ = g.eval(pos[:256], nrm[:256], wo[:256])
     pdf = G.mixture_pdf(raw.astype(np.float64), np.tile(mu, (256, 1)))
     assert np.median(pdf) > 4 / (4 * np.pi)
+
+
+def _room():
+    from craystal_amd import scenes
+    return scenes.occluded_room(n_objects=12, subdiv=1)
+
+
+def test_guided_render_is_unbiased(gpu):
+    """Guided (untrained and trained) and plain renders of the occluded-light
+    room converge to the same image mean: per-pass means agree within 3
+    sigma (the guided weight keeps the plain estimator's expectation)."""
+    import craystal_amd as cr
+    from craystal_amd import scenes
+    from craystal_amd.guiding import GuidedIntegrator
+    data, camd = _room()
+    scene = cr.Scene(data)
+    size = (64, 64)
+    cfg = cr.Configs(maxDepth=8, rrThreshold=0.05, useMIS=True, rrStartDepth=3,
+                     mode=L.MODE_WAVEFRONT)
+    P = np.array(data.concatenated()[0])
+    guide = Guide(P.min(0), P.max(0), seed=3)
+    gi = GuidedIntegrator(cfg, guide, train_batch=4096, train_steps=2)
+    sensor = cr.Sensor(size)
+    cam = scenes.make_camera(camd, sensor)
+    scene.setCamera(cam)
+    for k in range(6):                       # train a little first
+        sensor.clear()
+        st = gi.dispatch(scene, 1, first_sample=10000 + k, train=True)
+    assert st["train_samples"] > 0
+    plain, guided = [], []
+    for k in range(24):
+        sensor.clear()
+        cr.PathTraceIntegrator(cfg).dispatch(scene, 1, first_sample=k)
+        plain.append(sensor.createImage().mean())
+        sensor.clear()
+        gi.dispatch(scene, 1, first_sample=500 + k)
+        guided.append(sensor.createImage().mean())
+    plain, guided = np.array(plain), np.array(guided)
+    assert np.isfinite(guided).all()
+    se = np.sqrt(plain.var(ddof=1) / len(plain) + guided.var(ddof=1) / len(guided))
+    assert abs(plain.mean() - guided.mean()) <= 3 * se, (plain.mean(), guided.mean(), se)
