@@ -44,7 +44,37 @@ def parse():
     ap.add_argument("--trace-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-trace", action="store_true", help="skip the closest-hit batch")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="counter-collection mode: one 256 MiB device copy (traffic "
+                         "calibration), one 16,777,216-ray closest-hit launch and one render "
+                         "pass; prints the launch sizes and exits")
     return ap.parse_args()
+
+
+def profile_kernels(cr, L, scenes):
+    """Minimal launches for rocprofv3 --pmc runs (FETCH_SIZE / WRITE_SIZE)."""
+    nbytes = 256 << 20
+    a, b = cr.DeviceBuffer(nbytes), cr.DeviceBuffer(nbytes)
+    a.memset(1)
+    b.copyFrom(a)
+    data, camd = scenes.spheres_scene()
+    scene = cr.Scene(data)
+    n = 1 << 24
+    rays = scenes.mixed_batch(data, camd, n)
+    rb, hb = cr.DeviceBuffer(rays.nbytes), cr.DeviceBuffer(n * 16)
+    rb.copyFromHost(rays)
+    cnt = cr.trace_closest_device(scene, rb, n, hb, count_work=True)
+    cr.trace_closest_device(scene, rb, n, hb)
+    W, H = camd["size"]
+    sensor = cr.Sensor((W, H))
+    cam = scenes.make_camera(camd, sensor)
+    scene.setCamera(cam)
+    cfg = cr.Configs(maxDepth=camd["max_depth"], rrThreshold=0.05, useMIS=True,
+                     rrStartDepth=camd["rr_start_depth"])
+    st = cr.PathTraceIntegrator(cfg).dispatch(scene, 1, first_sample=0, readback=False)
+    print(json.dumps({"copy_bytes": nbytes, "trace_rays": n,
+                      "trace_node_visits": cnt["node_visits"], "trace_tri_tests": cnt["tri_tests"],
+                      "render_samples": W * H, "render_rays": st["rays"]}), flush=True)
 
 
 def main():
@@ -62,6 +92,9 @@ def main():
     cr.load()
     if cr.device_count() < 1:
         raise SystemExit("no GPU visible to libcrayhip.so")
+    if a.profile_kernels:
+        profile_kernels(cr, L, scenes)
+        return
 
     data, camd = scenes.spheres_scene()
     W, H = camd["size"]

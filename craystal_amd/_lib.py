@@ -133,6 +133,7 @@ SIGNATURES = {
     "crh_buffer_download": [_P, _P, C.c_size_t],
     "crh_buffer_memset": [_P, C.c_int],
     "crh_buffer_size": [_P, C.POINTER(C.c_size_t)],
+    "crh_buffer_copy": [_P, _P, C.c_size_t],
     "crh_buffer_destroy": [_P],
     "crh_bvh_build": [_P, u32, _P, u32, C.POINTER(crh_bvh_params), C.POINTER(_P)],
     "crh_bvh_get_stats": [_P, C.POINTER(crh_bvh_stats)],

@@ -384,6 +384,11 @@ class DeviceBuffer:
     def memset(self, value: int = 0):
         L.call("crh_buffer_memset", self.handle, value)
 
+    def copyFrom(self, src: "DeviceBuffer", nbytes: int | None = None):
+        """Device-to-device copy."""
+        L.call("crh_buffer_copy", self.handle, src.handle,
+               int(min(self.nbytes, src.nbytes) if nbytes is None else nbytes))
+
     def size(self) -> int:
         n = C.c_size_t()
         L.call("crh_buffer_size", self.handle, C.byref(n))

@@ -389,6 +389,18 @@ int crh_buffer_memset(crh_buffer* buf, int value) {
     return CRH_OK;
 }
 
+int crh_buffer_copy(crh_buffer* dst, const crh_buffer* src, size_t bytes) {
+    ARG_CHECK(dst && src, "NULL argument");
+    ARG_CHECK(dst->ctx == src->ctx, "buffers belong to different contexts");
+    ARG_CHECK(bytes <= dst->mem.bytes && bytes <= src->mem.bytes, "copy larger than a buffer");
+    if (bytes == 0) return CRH_OK;
+    HIP_TRY(hipSetDevice(dst->ctx->device));
+    HIP_TRY(hipMemcpyAsync(dst->mem.p, src->mem.p, bytes, hipMemcpyDeviceToDevice,
+                           dst->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(dst->ctx->stream));
+    return CRH_OK;
+}
+
 int crh_buffer_size(const crh_buffer* buf, size_t* bytes) {
     ARG_CHECK(buf && bytes, "NULL argument");
     *bytes = buf->mem.bytes;
@@ -1040,7 +1052,7 @@ int crh_guide_train(crh_guide* g, const float* samples, uint32_t n, float lr, in
     crh_context* ctx = g->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t padN = (size_t(n) + 63) & ~size_t(63);
-    if (padN > g->capSamples) {
+    if (padN > g->capSamples || padN * sizeof(crh::GuideSample) > g->samples.bytes) {
         HIP_TRY(g->samples.alloc(padN * sizeof(crh::GuideSample)));
         HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * padN * 64 * 2));
         HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * padN * 64 * 2));
@@ -1104,10 +1116,11 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
     const crh::Tuning tun = effective(ctx->tuning, s->view);
     const crh::WfLaunch wl = wfLaunch(tun);
     const uint32_t batch = gc->train_batch;
-    if (gc->train && size_t(batch) > g->capSamples) {
-        HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * batch * 64 * 2));
-        HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * batch * 64 * 2));
-        g->capSamples = batch;
+    const size_t padB = (size_t(batch) + 63) & ~size_t(63);
+    if (gc->train && padB > g->capSamples) {
+        HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * padB * 64 * 2));
+        HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * padB * 64 * 2));
+        g->capSamples = padB;
     }
     double mainMs = 0.0;
     uint32_t lastSamples = 0;

@@ -224,6 +224,9 @@ int crh_buffer_create(crh_context* ctx, size_t bytes, crh_buffer** out);
 int crh_buffer_upload(crh_buffer* buf, const void* src, size_t bytes);
 int crh_buffer_download(const crh_buffer* buf, void* dst, size_t bytes);
 int crh_buffer_memset(crh_buffer* buf, int value);
+/* Device-to-device copy of bytes from src to dst (no reference counterpart;
+ * also the known-size copy used to calibrate HBM traffic counters). */
+int crh_buffer_copy(crh_buffer* dst, const crh_buffer* src, size_t bytes);
 int crh_buffer_size(const crh_buffer* buf, size_t* bytes);
 int crh_buffer_destroy(crh_buffer* buf);
 
