@@ -95,7 +95,8 @@ class crh_integrator_cfg(C.Structure):
 
 class crh_tuning(C.Structure):
     _fields_ = [("stack", u32), ("trace_block", u32), ("wf_trace_grid", u32),
-                ("wf_trace_block", u32), ("wf_shade_grid", u32), ("reserved", u32 * 3)]
+                ("wf_trace_block", u32), ("wf_shade_grid", u32), ("persistent", u32),
+                ("refill", u32), ("pt_grid", u32)]
 
 
 class crh_guided_cfg(C.Structure):

@@ -23,9 +23,11 @@ def _median(xs):
 
 
 def tune_trace(scene: Scene, rays: np.ndarray, reps: int = 3, device: int = 0,
-               stacks=(1, 2), blocks=(64, 128, 256)) -> dict:
-    """Tune the closest-hit batch kernel; returns the chosen tuning and the
-    per-candidate timings (ms)."""
+               stacks=(1, 2), blocks=(64, 128, 256), persistent=((2, 0),),
+               pt_grids=(1024,)) -> dict:
+    """Tune the closest-hit batch kernel over stack placement, block size and
+    persistence ((mode, refill) pairs; mode 1 = lanes refill, 2 = off);
+    returns the chosen tuning and the per-candidate timings (ms)."""
     ctx = get_context(device)
     saved = ctx.get_tuning()
     n = len(rays)
@@ -34,8 +36,15 @@ def tune_trace(scene: Scene, rays: np.ndarray, reps: int = 3, device: int = 0,
     rb.copyFromHost(np.ascontiguousarray(rays, dtype=L.RAY_DTYPE))
     ref = None
     results = {}
-    for st, bl in itertools.product(stacks, blocks):
-        ctx.set_tuning(stack=st, trace_block=bl)
+    cands = []
+    for st, bl, (pm, rf) in itertools.product(stacks, blocks, persistent):
+        for pg in (pt_grids if pm == 1 else (0,)):
+            cands.append((st, bl, pm, rf, pg))
+    for st, bl, pm, rf, pg in cands:
+        kw = dict(stack=st, trace_block=bl, persistent=pm)
+        if pm == 1:
+            kw.update(refill=rf, pt_grid=pg)
+        ctx.set_tuning(**kw)
         trace_closest_device(scene, rb, n, hb)
         ms = [trace_closest_device(scene, rb, n, hb)["kernel_ms"] for _ in range(reps)]
         out = np.zeros(n, L.HIT_DTYPE)
@@ -44,17 +53,22 @@ def tune_trace(scene: Scene, rays: np.ndarray, reps: int = 3, device: int = 0,
             ref = out
         elif out.tobytes() != ref.tobytes():
             ctx.set_tuning(**saved)
-            raise AssertionError(f"tuning stack={st} block={bl} changed the hits")
-        results[(st, bl)] = _median(ms)
+            raise AssertionError(f"tuning {kw} changed the hits")
+        results[(st, bl, pm, rf, pg)] = _median(ms)
     best = min(results, key=results.get)
-    ctx.set_tuning(stack=best[0], trace_block=best[1])
-    return {"stack": best[0], "trace_block": best[1],
-            "timings_ms": {f"stack={k[0]},block={k[1]}": round(v, 4) for k, v in results.items()}}
+    st, bl, pm, rf, pg = best
+    kw = dict(stack=st, trace_block=bl, persistent=pm)
+    if pm == 1:
+        kw.update(refill=rf, pt_grid=pg)
+    ctx.set_tuning(**kw)
+    return {"stack": st, "trace_block": bl, "persistent": pm, "refill": rf, "pt_grid": pg,
+            "timings_ms": {f"stack={k[0]},block={k[1]},persistent={k[2]},refill={k[3]},"
+                           f"grid={k[4]}": round(v, 4) for k, v in results.items()}}
 
 
 def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
                 stacks=(1, 2), wf_grids=(1024, 2048, 4096), wf_blocks=(128, 256),
-                first_sample: int = 1 << 20) -> dict:
+                wf_persistent=(2, 1), first_sample: int = 1 << 20) -> dict:
     """Tune the megakernel stack placement and the wavefront grid/block
     sizes on the scene's current camera/sensor; returns the choice and the
     timings (ms per spp) of both integrators."""
@@ -77,18 +91,20 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
         res_mk[st] = ms
     best_stack = min(res_mk, key=res_mk.get)
     wf_cfg = Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
-    for st, g, b in itertools.product(stacks, wf_grids, wf_blocks):
-        ctx.set_tuning(stack=st, wf_trace_grid=g, wf_shade_grid=g, wf_trace_block=b)
+    for st, g, b, pm in itertools.product(stacks, wf_grids, wf_blocks, wf_persistent):
+        ctx.set_tuning(stack=st, wf_trace_grid=g, wf_shade_grid=g, wf_trace_block=b,
+                       persistent=pm)
         ms, img = run(wf_cfg)
         assert img.tobytes() == ref.tobytes(), "wavefront launch shape changed the image"
-        res_wf[(st, g, b)] = ms
+        res_wf[(st, g, b, pm)] = ms
     best_wf = min(res_wf, key=res_wf.get)
     mk_ms, wf_ms = res_mk[best_stack], res_wf[best_wf]
     use_wf = wf_ms < mk_ms
     stack = best_wf[0] if use_wf else best_stack
     ctx.set_tuning(stack=stack, wf_trace_grid=best_wf[1], wf_shade_grid=best_wf[1],
-                   wf_trace_block=best_wf[2])
+                   wf_trace_block=best_wf[2], persistent=best_wf[3])
     return {"mode": L.MODE_WAVEFRONT if use_wf else L.MODE_MEGAKERNEL,
             "stack": stack, "wf_grid": best_wf[1], "wf_trace_block": best_wf[2],
+            "wf_persistent": best_wf[3],
             "megakernel_ms": {str(k): round(v, 3) for k, v in res_mk.items()},
             "wavefront_ms": {str(k): round(v, 3) for k, v in res_wf.items()}}

@@ -37,7 +37,7 @@ class Context:
         self.device = device
 
     TUNING_FIELDS = ("stack", "trace_block", "wf_trace_grid", "wf_trace_block",
-                     "wf_shade_grid")
+                     "wf_shade_grid", "persistent", "refill", "pt_grid")
 
     def set_tuning(self, **kw):
         """Launch choices that never change results (see crh_tuning)."""

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -118,9 +119,46 @@ struct crh_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     crh::Tuning tuning;
+    uint32_t* counter = nullptr;  // work counter of the persistent kernels
+    // One reference for the creator (dropped by crh_context_destroy) and one
+    // per live buffer, scene, film and guide made on it: the stream and the
+    // events outlive every object that uses them, whatever order the caller
+    // destroys things in (a garbage-collected host may finalise a context
+    // before the objects that point at it).
+    std::atomic<int> refs{1};
 };
 
 namespace {
+
+void contextRelease(crh_context* c) {
+    if (c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->counter) (void)hipFree(c->counter);
+    delete c;
+}
+
+// A counted reference to the owning context.  Declared as the first member of
+// each object so that it is released after the object's device memory.
+struct CtxRef {
+    crh_context* p = nullptr;
+    CtxRef() = default;
+    CtxRef(const CtxRef&) = delete;
+    CtxRef& operator=(const CtxRef&) = delete;
+    ~CtxRef() {
+        if (p) contextRelease(p);
+    }
+    void set(crh_context* c) {
+        c->refs.fetch_add(1, std::memory_order_relaxed);
+        p = c;
+    }
+    crh_context* operator->() const { return p; }
+    operator crh_context*() const { return p; }
+};
+
 // The private stack holds kPrivStack levels; deeper trees use the LDS stack.
 crh::Tuning effective(const crh::Tuning& t, const crh::SceneView& v) {
     crh::Tuning e = t;
@@ -133,12 +171,14 @@ crh::WfLaunch wfLaunch(const crh::Tuning& t) {
     w.traceBlock = t.wfTraceBlock;
     w.shadeGrid = t.wfShadeGrid;
     w.privStack = t.privStack;
+    w.persistent = t.persistent;
+    w.refill = t.refill;
     return w;
 }
 }  // namespace
 
 struct crh_buffer {
-    crh_context* ctx = nullptr;
+    CtxRef ctx;
     DevMem mem;
 };
 
@@ -147,14 +187,14 @@ struct crh_bvh {
 };
 
 struct crh_scene {
-    crh_context* ctx = nullptr;
+    CtxRef ctx;
     crh::BvhResult bvh;
     DevMem nodes, tris, pos, nrm, uv, idx, primMat, mats, emissive;
     crh::SceneView view{};
 };
 
 struct crh_film {
-    crh_context* ctx = nullptr;
+    CtxRef ctx;
     uint32_t w = 0, h = 0;
     DevMem sum, wsum, L, P;
     // Wavefront path state, allocated on first wavefront render.
@@ -290,6 +330,7 @@ int crh_context_create(int device, crh_context** out) {
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = crh::guideSetLdsLimits();
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->counter), 256);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e != hipSuccess) {
@@ -303,12 +344,7 @@ int crh_context_create(int device, crh_context** out) {
 
 int crh_context_destroy(crh_context* ctx) {
     if (!ctx) return CRH_OK;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    contextRelease(ctx);
     return CRH_OK;
 }
 
@@ -318,14 +354,20 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(t->stack <= 2, "stack must be 0 (keep), 1 (LDS) or 2 (private)");
     ARG_CHECK(okBlock(t->trace_block) && okBlock(t->wf_trace_block),
               "block sizes must be multiples of 64 in [64, 256]");
-    ARG_CHECK(t->wf_trace_grid <= (1u << 20) && t->wf_shade_grid <= (1u << 20),
+    ARG_CHECK(t->wf_trace_grid <= (1u << 20) && t->wf_shade_grid <= (1u << 20) &&
+                  t->pt_grid <= (1u << 20),
               "grid too large");
+    ARG_CHECK(t->persistent <= 2, "persistent must be 0 (keep), 1 (on) or 2 (off)");
+    ARG_CHECK(t->refill <= 64, "refill must be in [1, 64]");
     crh::Tuning& o = ctx->tuning;
     if (t->stack) o.privStack = t->stack == 2 ? 1u : 0u;
     if (t->trace_block) o.traceBlock = t->trace_block;
     if (t->wf_trace_grid) o.wfTraceGrid = t->wf_trace_grid;
     if (t->wf_trace_block) o.wfTraceBlock = t->wf_trace_block;
     if (t->wf_shade_grid) o.wfShadeGrid = t->wf_shade_grid;
+    if (t->persistent) o.persistent = t->persistent == 1 ? 1u : 0u;
+    if (t->refill) o.refill = t->refill;
+    if (t->pt_grid) o.ptGrid = t->pt_grid;
     return CRH_OK;
 }
 
@@ -337,6 +379,9 @@ int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
     t->wf_trace_grid = ctx->tuning.wfTraceGrid;
     t->wf_trace_block = ctx->tuning.wfTraceBlock;
     t->wf_shade_grid = ctx->tuning.wfShadeGrid;
+    t->persistent = ctx->tuning.persistent ? 1u : 2u;
+    t->refill = ctx->tuning.refill;
+    t->pt_grid = ctx->tuning.ptGrid;
     return CRH_OK;
 }
 
@@ -347,7 +392,7 @@ int crh_buffer_create(crh_context* ctx, size_t bytes, crh_buffer** out) {
     GUARD_BEGIN
     HIP_TRY(hipSetDevice(ctx->device));
     auto* b = new crh_buffer();
-    b->ctx = ctx;
+    b->ctx.set(ctx);
     hipError_t e = b->mem.alloc(bytes);
     if (e != hipSuccess) {
         delete b;
@@ -524,7 +569,7 @@ int crh_scene_create(crh_context* ctx, const crh_scene_desc* d,
     ARG_CHECK(bp.bins >= 2 && bp.bins <= 1024, "bins must be in [2, 1024]");
 
     auto* s = new crh_scene();
-    s->ctx = ctx;
+    s->ctx.set(ctx);
     s->bvh = crh::buildBvh(pos.data(), uint32_t(pos.size() / 3), idx.data(),
                            ntri, bp);
     crh::GpuBvh g = crh::toGpuLayout(s->bvh, pos.data(), idx.data());
@@ -643,7 +688,7 @@ static int traceClosestDev(crh_scene* s, const crh_ray* rays, uint64_t n,
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     HIP_TRY(crh::launchTraceClosest(s->view, rays, n, hits, work,
-                                    effective(ctx->tuning, s->view), ctx->stream));
+                                    effective(ctx->tuning, s->view), ctx->counter, ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev1));
     if (stats) {
@@ -704,7 +749,7 @@ int crh_trace_any_device(crh_scene* s, const crh_buffer* rays, uint64_t n,
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     HIP_TRY(crh::launchTraceAny(s->view, rays->mem.as<crh_ray>(), n,
                                 occ->mem.as<uint8_t>(), effective(ctx->tuning, s->view),
-                                ctx->stream));
+                                ctx->counter, ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev1));
     if (stats) {
@@ -750,7 +795,7 @@ int crh_trace_any(crh_scene* s, const crh_ray* rays, uint64_t n,
     HIP_TRY(dox.alloc(n));
     HIP_TRY(hipMemcpy(dr.p, rays, n * sizeof(crh_ray), hipMemcpyHostToDevice));
     HIP_TRY(crh::launchTraceAny(s->view, dr.as<crh_ray>(), n, dox.as<uint8_t>(),
-                                effective(ctx->tuning, s->view), ctx->stream));
+                                effective(ctx->tuning, s->view), ctx->counter, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     HIP_TRY(hipMemcpy(occluded, dox.p, n, hipMemcpyDeviceToHost));
     return CRH_OK;
@@ -766,7 +811,7 @@ int crh_film_create(crh_context* ctx, uint32_t w, uint32_t h, crh_film** out) {
     GUARD_BEGIN
     HIP_TRY(hipSetDevice(ctx->device));
     auto* f = new crh_film();
-    f->ctx = ctx;
+    f->ctx.set(ctx);
     f->w = w;
     f->h = h;
     size_t n = size_t(w) * h;
@@ -909,7 +954,7 @@ int crh_render(crh_scene* s, const crh_camera* cam,
 }  // extern "C"
 
 struct crh_guide {
-    crh_context* ctx = nullptr;
+    CtxRef ctx;
     DevMem theta, m, v, grad, wq, wqT, loss, samples, hsave, dsave;
     crh::GuideView view{};
     uint32_t step = 0;
@@ -961,7 +1006,7 @@ int crh_guide_create(crh_context* ctx, const float* bmin, const float* bmax, uin
     GUARD_BEGIN
     HIP_TRY(hipSetDevice(ctx->device));
     auto* g = new crh_guide();
-    g->ctx = ctx;
+    g->ctx.set(ctx);
     hipError_t e = g->theta.alloc(crh::kGParams * 4);
     if (e == hipSuccess) e = g->m.alloc(crh::kGParams * 4);
     if (e == hipSuccess) e = g->v.alloc(crh::kGParams * 4);

@@ -18,13 +18,17 @@ struct Tuning {
     uint32_t wfTraceGrid = 2048;
     uint32_t wfShadeGrid = 2048;
     uint32_t wfTraceBlock = 256;
+    uint32_t persistent = 0;    // 1: ray-batch kernels refill lanes with new rays
+    uint32_t refill = 16;       // idle lanes per wave that trigger a refill
+    uint32_t ptGrid = 2048;     // blocks of the persistent ray-batch kernels
 };
 
+// counter: one device uint32 used by the persistent variants (may be null).
 hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
                               uint64_t n, crh_hit* hits, uint32_t* work,
-                              const Tuning& t, hipStream_t st);
+                              const Tuning& t, uint32_t* counter, hipStream_t st);
 hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
-                          uint8_t* occ, const Tuning& t, hipStream_t st);
+                          uint8_t* occ, const Tuning& t, uint32_t* counter, hipStream_t st);
 hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
                             const IntegratorView& cfg, uint32_t si, float4* L,
                             float2* P, unsigned long long* counters,

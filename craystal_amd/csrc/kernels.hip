@@ -50,6 +50,37 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(
     occ[i] = traverse<true, false>(s, ray, h, stk, wc) ? 1 : 0;
 }
 
+// Persistent variants: lanes refill with new rays as theirs finish.
+struct BatchSrc {
+    const crh_ray* rays;
+    crh_hit* hits;
+    uint8_t* occ;
+    CRH_D Ray load(uint32_t i) const {
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        float4 a = rp[0], b = rp[1];
+        return Ray{mk3(a.x, a.y, a.z), a.w, mk3(b.x, b.y, b.z), b.w};
+    }
+    CRH_D void store(uint32_t i, bool isHit, const Hit& h) const {
+        if (hits)
+            reinterpret_cast<uint4*>(hits)[i] =
+                make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.u), __float_as_uint(h.v));
+        if (occ) occ[i] = isHit ? 1 : 0;
+    }
+};
+
+__global__ __launch_bounds__(64) void kResetCounter(uint32_t* c) {
+    if (threadIdx.x == 0) *c = 0u;
+}
+
+template <bool ANY, bool PRIV>
+__global__ __launch_bounds__(kTraceBlock) void kTracePersistent(
+    SceneView s, BatchSrc src, uint32_t n, uint32_t* __restrict__ counter, uint32_t refill) {
+    extern __shared__ uint2 lds[];
+    auto stk = makeStack<PRIV>(lds);
+    WorkCount wc{0, 0};
+    persistentTrace<ANY, false>(s, src, counter, n, refill, stk, wc);
+}
+
 // ---------------------------------------------------------------------------
 // Megakernel integrator (Core/Integrator.cu:13-227)
 // ---------------------------------------------------------------------------
@@ -237,10 +268,29 @@ static size_t stackLds(const SceneView& s, const Tuning& t, uint32_t block) {
     return t.privStack ? 0 : size_t(s.stackCap) * block * sizeof(uint2);
 }
 
+template <bool ANY>
+static hipError_t launchPersistent(const SceneView& s, const BatchSrc& src, uint32_t n,
+                                   uint32_t* counter, const Tuning& t, hipStream_t st) {
+    const uint32_t B = t.traceBlock;
+    size_t lds = stackLds(s, t, B);
+    hipLaunchKernelGGL(kResetCounter, dim3(1), dim3(64), 0, st, counter);
+    dim3 grid(t.ptGrid), block(B);
+    if (t.privStack)
+        hipLaunchKernelGGL((kTracePersistent<ANY, true>), grid, block, lds, st, s, src, n, counter,
+                           t.refill);
+    else
+        hipLaunchKernelGGL((kTracePersistent<ANY, false>), grid, block, lds, st, s, src, n,
+                           counter, t.refill);
+    return hipGetLastError();
+}
+
 hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
                               uint64_t n, crh_hit* hits, uint32_t* work,
-                              const Tuning& t, hipStream_t st) {
+                              const Tuning& t, uint32_t* counter, hipStream_t st) {
     if (n == 0) return hipSuccess;
+    if (t.persistent && !work && counter && n < (uint64_t(1) << 32))
+        return launchPersistent<false>(s, BatchSrc{rays, hits, nullptr}, uint32_t(n), counter, t,
+                                       st);
     const uint32_t B = t.traceBlock;
     size_t lds = stackLds(s, t, B);
     dim3 grid(uint32_t((n + B - 1) / B)), block(B);
@@ -256,8 +306,10 @@ hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
 }
 
 hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
-                          uint8_t* occ, const Tuning& t, hipStream_t st) {
+                          uint8_t* occ, const Tuning& t, uint32_t* counter, hipStream_t st) {
     if (n == 0) return hipSuccess;
+    if (t.persistent && counter && n < (uint64_t(1) << 32))
+        return launchPersistent<true>(s, BatchSrc{rays, nullptr, occ}, uint32_t(n), counter, t, st);
     const uint32_t B = t.traceBlock;
     size_t lds = stackLds(s, t, B);
     dim3 grid(uint32_t((n + B - 1) / B)), block(B);

@@ -152,6 +152,164 @@ CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, Stack& stack,
     return isHit;
 }
 
+// ---------------------------------------------------------------------------
+// The same traversal as a resumable state machine: travInit + repeated
+// travStep perform exactly the operations of traverse(), one loop iteration
+// (one internal node or one leaf) per step, so a persistent kernel can
+// interleave rays per lane without changing any result.
+// ---------------------------------------------------------------------------
+struct TravState {
+    RayPre r;
+    Hit hit;
+    float closestT;
+    uint32_t cur;
+    int sp;
+    bool isHit;
+};
+
+// Returns true if the ray is already finished (empty scene / root miss).
+CRH_D bool travInit(const SceneView& s, const Ray& ray, TravState& t) {
+    t.hit.prim = 0xFFFFFFFFu;
+    t.hit.t = __builtin_inff();
+    t.hit.u = 0.f;
+    t.hit.v = 0.f;
+    t.isHit = false;
+    t.closestT = t.hit.t;
+    t.sp = 0;
+    t.cur = s.rootRef;
+    if (!s.hasGeom) return true;
+    t.r = prep(ray);
+    float dummy;
+    return !boxHit(s.rootMin, s.rootMax, t.r, dummy);
+}
+
+// One iteration of traverse()'s loop; returns true when the ray is done.
+template <bool ANY, bool COUNT, class Stack>
+CRH_D bool travStep(const SceneView& s, TravState& t, Stack& stack, WorkCount& wc) {
+    const RayPre& r = t.r;
+    if (t.cur & kLeafBit) {
+        uint32_t i = t.cur & ~kLeafBit;
+        while (true) {
+            const float4* tp = reinterpret_cast<const float4*>(s.tris + i);
+            float4 a = tp[0], b = tp[1], c = tp[2];
+            uint32_t pw = __float_as_uint(a.w);
+            if (COUNT) wc.tris++;
+            float u, v, tt;
+            if (triHit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), r, u, v,
+                       tt)) {
+                if (!(tt < r.tMin || tt > r.tMax)) {
+                    t.isHit = true;
+                    if (tt < t.hit.t) {
+                        t.hit.t = tt;
+                        t.hit.u = u;
+                        t.hit.v = v;
+                        t.hit.prim = pw & ~kLeafBit;
+                    }
+                    t.closestT = smin(t.closestT, t.hit.t);
+                    if (ANY) return true;
+                }
+            }
+            if (pw & kLeafBit) break;
+            ++i;
+        }
+    } else {
+        if (COUNT) wc.nodes++;
+        float4 a, b, c;
+        uint4 d;
+        loadNode(s.nodes, t.cur, a, b, c, d);
+        const float lmin[3] = {a.x, a.y, a.z};
+        const float lmax[3] = {a.w, b.x, b.y};
+        const float rmin[3] = {b.z, b.w, c.x};
+        const float rmax[3] = {c.y, c.z, c.w};
+        float lt = __builtin_inff(), rt = __builtin_inff();
+        bool hl = boxHit(lmin, lmax, r, lt);
+        bool hr = boxHit(rmin, rmax, r, rt);
+        if (hl && hr) {
+            bool rightNear = lt > rt;
+            uint32_t nearRef = rightNear ? d.y : d.x;
+            uint32_t farRef = rightNear ? d.x : d.y;
+            float nearT = rightNear ? rt : lt;
+            float farT = rightNear ? lt : rt;
+            if (nearT < t.closestT) {
+                if (farT < t.closestT) {
+                    stack.put(t.sp, make_uint2(farRef, __float_as_uint(farT)));
+                    ++t.sp;
+                }
+                t.cur = nearRef;
+                return false;
+            }
+        } else if (hl && lt < t.closestT) {
+            t.cur = d.x;
+            return false;
+        } else if (hr && rt < t.closestT) {
+            t.cur = d.y;
+            return false;
+        }
+    }
+    while (t.sp > 0) {
+        --t.sp;
+        uint2 e = stack.get(t.sp);
+        if (__uint_as_float(e.y) > t.closestT) continue;
+        t.cur = e.x;
+        return false;
+    }
+    return true;
+}
+
+// Persistent traversal with per-lane ray refill.  Rays [0, n) are handed
+// out through *counter (zero at launch): lanes without a ray take the next
+// ones, one atomic per wave (ballot + mbcnt); a wave keeps stepping its
+// lanes and refills once `refill` or more of them are idle while rays
+// remain.  src.load(i) gives ray i, src.store(i, isHit, hit) its result.
+template <bool ANY, bool COUNT, class Stack, class Src>
+CRH_D void persistentTrace(const SceneView& s, Src& src, uint32_t* counter, uint32_t n,
+                           uint32_t refill, Stack& stack, WorkCount& wc) {
+    const uint32_t kNone = 0xFFFFFFFFu;
+    uint32_t mine = kNone;
+    bool exhausted = false;  // wave-uniform: no rays left to hand out
+    TravState t;
+    while (true) {
+        if (!exhausted) {
+            uint64_t need = __ballot(mine == kNone);
+            if (need) {
+                uint32_t lo = uint32_t(need), hi = uint32_t(need >> 32);
+                uint32_t below = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+                int leader = __ffsll((unsigned long long)need) - 1;
+                uint32_t base = 0;
+                if (int(__lane_id()) == leader) base = atomicAdd(counter, uint32_t(__popcll(need)));
+                base = __shfl(base, leader, 64);
+                if (base + uint32_t(__popcll(need)) >= n) exhausted = true;
+                if (mine == kNone) {
+                    uint32_t idx = base + below;
+                    if (idx < n) {
+                        mine = idx;
+                        if (travInit(s, src.load(idx), t)) {
+                            src.store(idx, t.isHit, t.hit);
+                            mine = kNone;
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot(mine != kNone) == 0) {
+            if (exhausted) break;
+            continue;  // every lane finished at init: fetch again
+        }
+        while (true) {
+            if (mine != kNone) {
+                if (travStep<ANY, COUNT>(s, t, stack, wc)) {
+                    src.store(mine, t.isHit, t.hit);
+                    mine = kNone;
+                }
+            }
+            uint64_t idle = __ballot(mine == kNone);
+            int nIdle = __popcll(idle);
+            if (nIdle == 64) break;
+            if (!exhausted && nIdle >= int(refill)) break;
+        }
+    }
+}
+
 // SceneView::intersect as the integrator uses it.  A query that reports a
 // hit without recording a primitive (only possible when every comparison
 // sees a NaN, e.g. a NaN ray direction) is treated as a miss: the reference

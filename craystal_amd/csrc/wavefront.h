@@ -16,7 +16,9 @@ enum : uint32_t {
     kCCont = 5,  // paths to shade after the shadow query
     kCRec = 6,   // training records written this pass
     kCTrain = 7, // training samples produced from the records
-    kWfCounters = 8,
+    kCPtExt = 8,     // ray hand-out counter of the persistent extend kernel
+    kCPtShadow = 9,  // ray hand-out counter of the persistent shadow kernel
+    kWfCounters = 10,
 };
 
 struct WfState {
@@ -65,6 +67,8 @@ struct WfLaunch {
     uint32_t traceBlock = 256;   // threads per traversal block (64..256)
     uint32_t shadeGrid = 2048;   // blocks of 256 for the shading kernels
     uint32_t privStack = 0;      // 1: traversal stack in private memory
+    uint32_t persistent = 0;     // 1: traversal lanes refill with new rays
+    uint32_t refill = 16;
 };
 
 struct GuideView;

@@ -160,6 +160,61 @@ __global__ __launch_bounds__(256) void kWfExtend(SceneView s, WfState w,
     }
 }
 
+// Persistent variants: lanes take the next queue entry when their ray ends.
+struct ExtendSrc {
+    WfState w;
+    uint32_t cur, nExt;
+    CRH_D Ray load(uint32_t i) const {
+        if (i < nExt) return loadRay(w.rayO, w.rayD, w.qExt[cur][i]);
+        return loadRay(w.misO, w.misD, w.qMis[cur][i - nExt]);
+    }
+    CRH_D void store(uint32_t i, bool isHit, const Hit& h) const {
+        bool hit = isHit && h.prim != 0xFFFFFFFFu;  // closestHit()
+        if (i < nExt) {
+            uint32_t p = w.qExt[cur][i];
+            storeHit(w.hit, p, h);
+            if (hit) atomicOr(&w.flags[p], F_EXTHIT);
+        } else {
+            uint32_t p = w.qMis[cur][i - nExt];
+            storeHit(w.misHit, p, h);
+            if (hit) atomicOr(&w.flags[p], F_MISHIT);
+        }
+    }
+};
+
+template <bool PRIV>
+__global__ __launch_bounds__(256) void kWfExtendPT(SceneView s, WfState w, uint32_t cur,
+                                                   uint32_t refill) {
+    extern __shared__ uint2 lds[];
+    auto stk = makeStack<PRIV>(lds);
+    const uint32_t nExt = w.counters[kCExt0 + cur];
+    const uint32_t nMis = w.counters[kCMis0 + cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && nExt + nMis)
+        atomicAdd(w.work, (unsigned long long)(nExt + nMis));
+    ExtendSrc src{w, cur, nExt};
+    WorkCount wc{0, 0};
+    persistentTrace<false, false>(s, src, &w.counters[kCPtExt], nExt + nMis, refill, stk, wc);
+}
+
+struct ShadowSrc {
+    WfState w;
+    CRH_D Ray load(uint32_t i) const { return loadRay(w.shO, w.shD, w.qShadow[i]); }
+    CRH_D void store(uint32_t i, bool isHit, const Hit&) const {
+        if (isHit) w.flags[w.qShadow[i]] &= ~F_LIGHT;
+    }
+};
+
+template <bool PRIV>
+__global__ __launch_bounds__(256) void kWfShadowPT(SceneView s, WfState w, uint32_t refill) {
+    extern __shared__ uint2 lds[];
+    auto stk = makeStack<PRIV>(lds);
+    const uint32_t n = w.counters[kCShadow];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(w.work, (unsigned long long)n);
+    ShadowSrc src{w};
+    WorkCount wc{0, 0};
+    persistentTrace<true, false>(s, src, &w.counters[kCPtShadow], n, refill, stk, wc);
+}
+
 // Resolve the MIS estimate of the previous bounce:
 // radiance += (value / selPdf) * beta_k  (Core/Integrator.cu:139, :186).
 __global__ __launch_bounds__(256) void kWfMisResolve(SceneView s, WfState w,
@@ -446,11 +501,36 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
     const size_t lds = L.privStack ? 0 : size_t(s.stackCap) * TB * sizeof(uint2);
     const dim3 b256(256), bT(TB);
     const bool guided = guide != nullptr;
+    const bool pt = L.persistent != 0;
     auto extend = [&](uint32_t cur, bool count) {
-        if (L.privStack)
+        if (pt && !count) {
+            hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters, 1u << kCPtExt);
+            if (L.privStack)
+                hipLaunchKernelGGL((kWfExtendPT<true>), dim3(L.traceGrid), bT, lds, st, s, w, cur,
+                                   L.refill);
+            else
+                hipLaunchKernelGGL((kWfExtendPT<false>), dim3(L.traceGrid), bT, lds, st, s, w, cur,
+                                   L.refill);
+        } else if (L.privStack) {
             launchExtend<true>(count, dim3(L.traceGrid), bT, lds, st, s, w, cur);
-        else
+        } else {
             launchExtend<false>(count, dim3(L.traceGrid), bT, lds, st, s, w, cur);
+        }
+    };
+    auto shadow = [&]() {
+        if (pt) {
+            hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters, 1u << kCPtShadow);
+            if (L.privStack)
+                hipLaunchKernelGGL((kWfShadowPT<true>), dim3(L.traceGrid), bT, lds, st, s, w,
+                                   L.refill);
+            else
+                hipLaunchKernelGGL((kWfShadowPT<false>), dim3(L.traceGrid), bT, lds, st, s, w,
+                                   L.refill);
+        } else if (L.privStack) {
+            hipLaunchKernelGGL((kWfShadow<true>), dim3(L.traceGrid), bT, lds, st, s, w);
+        } else {
+            hipLaunchKernelGGL((kWfShadow<false>), dim3(L.traceGrid), bT, lds, st, s, w);
+        }
     };
     hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
                        (1u << kCExt1) | (1u << kCMis0) | (1u << kCMis1) |
@@ -469,10 +549,7 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
             hipLaunchKernelGGL((kWfShadeHit<true>), dim3(L.shadeGrid), b256, 0, st, s, w, cur);
         else
             hipLaunchKernelGGL((kWfShadeHit<false>), dim3(L.shadeGrid), b256, 0, st, s, w, cur);
-        if (L.privStack)
-            hipLaunchKernelGGL((kWfShadow<true>), dim3(L.traceGrid), bT, lds, st, s, w);
-        else
-            hipLaunchKernelGGL((kWfShadow<false>), dim3(L.traceGrid), bT, lds, st, s, w);
+        shadow();
         if (guided) {
             hipError_t e = launchGuideInfer(*guide, w.qCont, w.counters + kCCont, 0u, w.gPos,
                                             w.gNrm, w.gWo, w.gMix, nullptr, L.shadeGrid, st);

@@ -203,7 +203,9 @@ typedef struct crh_tuning {
     uint32_t wf_trace_grid;  /* wavefront traversal kernels: blocks */
     uint32_t wf_trace_block; /* wavefront traversal kernels: threads (64..256) */
     uint32_t wf_shade_grid;  /* wavefront shading kernels: blocks of 256 */
-    uint32_t reserved[3];
+    uint32_t persistent;     /* 1 = ray batches refill lanes (persistent), 2 = off */
+    uint32_t refill;         /* idle lanes per wave that trigger a refill, 1..64 */
+    uint32_t pt_grid;        /* blocks of the persistent ray-batch kernels */
 } crh_tuning;
 
 /* ---- errors / device ------------------------------------------------- */
@@ -213,6 +215,9 @@ int crh_version(uint32_t* version);
 
 /* ---- context (replaces the implicit CUDA runtime context) ------------- */
 int crh_context_create(int device, crh_context** out);
+/* Drops the caller's reference.  Buffers, scenes, films and guides created
+ * on the context hold references of their own, so objects may be destroyed
+ * in any order; the stream is released with the last of them. */
 int crh_context_destroy(crh_context* ctx);
 /* Launch tuning for every later call on this context (no counterpart in the
  * reference, whose launch shape is fixed, Core/Integrator.cu:248-253). */

@@ -96,6 +96,20 @@ int main(int argc, char** argv) {
     EXPECT(hits[1].prim >= 2 && std::fabs(hits[1].t - 1.f) < 1e-6f);
     auto occ = scene->intersectOcclusion(rays);
     EXPECT(occ[0] == 1 && occ[1] == 1);
+
+    // Objects keep their context alive: destroy the context first, then use
+    // and destroy a buffer made on it.
+    crh_context* c2 = nullptr;
+    crh_buffer* b2 = nullptr;
+    EXPECT(crh_context_create(0, &c2) == CRH_OK);
+    EXPECT(crh_buffer_create(c2, 64, &b2) == CRH_OK);
+    EXPECT(crh_context_destroy(c2) == CRH_OK);
+    uint32_t in[16], out[16] = {};
+    for (int i = 0; i < 16; ++i) in[i] = 0x9E3779B9u * (i + 1);
+    EXPECT(crh_buffer_upload(b2, in, 64) == CRH_OK);
+    EXPECT(crh_buffer_download(b2, out, 64) == CRH_OK);
+    EXPECT(std::memcmp(in, out, 64) == 0);
+    EXPECT(crh_buffer_destroy(b2) == CRH_OK);
     std::printf("cpp gpu api ok: mean %.5f rays %llu\n", mean, (unsigned long long)rs.rays);
     return 0;
 }

@@ -107,3 +107,24 @@ def test_spheres_1m_batch_device_resident(gpu, orc):
     assert st["node_visits"] == int(nv.sum(dtype=np.uint64))
     assert st["tri_tests"] == int(nt.sum(dtype=np.uint64))
     assert (hits["prim"] != L.CRH_INVALID_PRIM).mean() > 0.3
+
+
+def test_persistent_refill_bit_identical(gpu, orc):
+    """Lanes refilling with new rays (persistent kernels) give the same hits
+    as one ray per thread, on the 983k-triangle scene and ragged sizes."""
+    data, cam = scenes.spheres_scene()
+    gs, os_ = _scene_pair(data, orc)
+    ctx = cr.get_context(0)
+    saved = ctx.get_tuning()
+    try:
+        for n, refill, grid in [(1 << 18, 16, 2048), (77777, 1, 7), (4099, 64, 3)]:
+            rays = scenes.mixed_batch(data, cam, n)
+            ctx.set_tuning(persistent=2)
+            base = gs.trace_closest(rays)
+            ctx.set_tuning(persistent=1, refill=refill, pt_grid=grid)
+            pt = gs.trace_closest(rays)
+            assert pt.tobytes() == base.tobytes()
+            assert np.array_equal(gs.trace_any(rays), os_.trace_any(rays))
+        _assert_hits_equal(pt, os_.trace_closest(rays))
+    finally:
+        ctx.set_tuning(**saved)

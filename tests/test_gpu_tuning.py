@@ -17,16 +17,19 @@ def test_trace_tuning_invariance_and_autotune(gpu):
     ctx = cr.get_context(0)
     saved = ctx.get_tuning()
     try:
-        res = autotune.tune_trace(scene, rays, reps=2)
+        res = autotune.tune_trace(scene, rays, reps=2,
+                                  persistent=((2, 0), (1, 8), (1, 32)), pt_grids=(64, 1024))
         assert res["stack"] in (1, 2) and res["trace_block"] in (64, 128, 256)
-        assert len(res["timings_ms"]) == 6
+        assert len(res["timings_ms"]) == 6 * 5
         assert ctx.get_tuning()["trace_block"] == res["trace_block"]
-        # any-hit with the private stack equals the LDS stack
-        ctx.set_tuning(stack=1)
+        # any-hit: private stack and persistent refill equal the LDS stack
+        ctx.set_tuning(stack=1, persistent=2)
         a = scene.trace_any(rays)
         ctx.set_tuning(stack=2)
         b = scene.trace_any(rays)
-        assert np.array_equal(a, b)
+        ctx.set_tuning(stack=1, persistent=1, refill=4, pt_grid=32)
+        c = scene.trace_any(rays)
+        assert np.array_equal(a, b) and np.array_equal(a, c)
     finally:
         ctx.set_tuning(**saved)
 
@@ -43,7 +46,7 @@ def test_render_tuning_invariance(gpu):
     try:
         res = autotune.tune_render(scene, cfg, spp=1, wf_grids=(64, 512), wf_blocks=(64, 256))
         assert res["mode"] in (L.MODE_MEGAKERNEL, L.MODE_WAVEFRONT)
-        assert len(res["wavefront_ms"]) == 8
+        assert len(res["wavefront_ms"]) == 16
     finally:
         ctx.set_tuning(**saved)
 
