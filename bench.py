@@ -109,6 +109,13 @@ def main():
                      rrStartDepth=camd["rr_start_depth"])
     integ = cr.PathTraceIntegrator(cfg)
 
+    # launch shapes picked by the autotuner (outside the timed region; every
+    # candidate must render the same image bit for bit)
+    from craystal_amd import autotune
+    tuned = autotune.tune_render(scene, cfg, spp=1, wf_grids=(4096, 16384), wf_blocks=(128, 256),
+                                 wf_persistent=((2, 0), (1, 16)))
+    sensor.clear()
+
     # warmup (untimed)
     if a.warmup > 0:
         integ.dispatch(scene, a.warmup, first_sample=0, readback=False)
@@ -182,6 +189,8 @@ def main():
     t0 = time.perf_counter()
     wst = wf.dispatch(scene, a.steps, first_sample=a.warmup, readback=False)
     wel = time.perf_counter() - t0
+    out["tuning"] = {k: tuned[k] for k in ("render_stack", "wf_stack", "wf_grid",
+                                           "wf_trace_block", "wf_persistent", "wf_refill")}
     out["wavefront"] = {"msamples_per_s": round(samples / wel / 1e6, 3),
                         "ms_per_step": round(wel / a.steps * 1e3, 3),
                         "grays_per_s": round(wst["rays"] / wel / 1e9, 4)}
@@ -193,6 +202,8 @@ def main():
         rb = cr.DeviceBuffer(rays.nbytes)
         hb = cr.DeviceBuffer(n * 16)
         rb.copyFromHost(rays)
+        ttuned = autotune.tune_trace(scene, rays, reps=2, persistent=((2, 0), (1, 16)),
+                                     pt_grids=(4096,))
         cr.trace_closest_device(scene, rb, n, hb)  # warmup
         cnt = cr.trace_closest_device(scene, rb, n, hb, count_work=True)
         times = []
@@ -206,6 +217,8 @@ def main():
             "kernel_ms": round(kms, 3), "kernel": "kTraceClosest",
             "node_visits_per_ray": round(cnt["node_visits"] / n, 3),
             "tri_tests_per_ray": round(cnt["tri_tests"] / n, 3),
+            "tuning": {k: ttuned[k] for k in ("stack", "trace_block", "persistent", "refill",
+                                              "pt_grid")},
             "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
                          "bytes_per_ray": round(b_ray, 1)}}
@@ -215,14 +228,18 @@ def main():
         import oracle
         orc_scene = oracle.Scene(data)
         threads = int(oracle.lib().orc_max_threads())
-        t0 = time.perf_counter()
-        _, _, crays = orc_scene.render_samples(cam, (W, H), cfg, 0)
+        # spp passes of the full image until at least 10 s of CPU work
+        passes, crays, t0 = 0, 0, time.perf_counter()
+        while passes == 0 or time.perf_counter() - t0 < 10.0:
+            _, _, r = orc_scene.render_samples(cam, (W, H), cfg, passes)
+            crays += r
+            passes += 1
         cs = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(px / cs / 1e6, 4), "unit": "Msamples/s",
+        out["cpu_baseline"] = {"value": round(passes * px / cs / 1e6, 4), "unit": "Msamples/s",
                                "cores": threads, "kind": "port",
-                               "sample": f"one spp pass of the same 1920x1080 image "
-                                         f"({px} paths, {crays} rays) by the C++ oracle, "
-                                         f"OpenMP over rows",
+                               "sample": f"{passes} spp passes of the same 1920x1080 image "
+                                         f"({passes * px} paths, {crays} rays) by the C++ "
+                                         f"oracle, OpenMP over rows",
                                "seconds": round(cs, 3),
                                "grays_per_s": round(crays / cs / 1e9, 5)}
     print(json.dumps(out), flush=True)

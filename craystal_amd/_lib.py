@@ -94,14 +94,18 @@ class crh_integrator_cfg(C.Structure):
 
 
 class crh_tuning(C.Structure):
-    _fields_ = [("stack", u32), ("trace_block", u32), ("wf_trace_grid", u32),
-                ("wf_trace_block", u32), ("wf_shade_grid", u32), ("persistent", u32),
-                ("refill", u32), ("pt_grid", u32)]
+    _fields_ = [(k, u32) for k in ("stack", "trace_block", "persistent", "refill", "pt_grid",
+                                   "render_stack", "wf_stack", "wf_trace_grid", "wf_trace_block",
+                                   "wf_shade_grid", "wf_persistent", "wf_refill")]
 
 
 class crh_guided_cfg(C.Structure):
     _fields_ = [("alpha", f32), ("train", u32), ("train_batch", u32), ("train_steps", u32),
                 ("lr", f32), ("reserved", u32 * 3)]
+
+
+class crh_guide_info(C.Structure):
+    _fields_ = [("steps", u32), ("last_samples", u32), ("last_loss", f32), ("reserved", u32)]
 
 
 class crh_stats(C.Structure):
@@ -161,6 +165,8 @@ SIGNATURES = {
     "crh_guide_get_params": [_P, _P, u32],
     "crh_guide_eval": [_P, _P, _P, _P, u32, _P, _P],
     "crh_guide_train": [_P, _P, u32, f32, C.c_int, _P, _P],
+    "crh_guide_get_info": [_P, C.POINTER(crh_guide_info)],
+    "crh_guide_get_losses": [_P, _P, u32],
     "crh_guide_destroy": [_P],
     "crh_render_guided": [_P, C.POINTER(crh_camera), C.POINTER(crh_integrator_cfg), _P,
                           C.POINTER(crh_guided_cfg), u32, u32, _P, C.POINTER(crh_stats)],

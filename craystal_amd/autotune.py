@@ -68,10 +68,12 @@ def tune_trace(scene: Scene, rays: np.ndarray, reps: int = 3, device: int = 0,
 
 def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
                 stacks=(1, 2), wf_grids=(1024, 2048, 4096), wf_blocks=(128, 256),
-                wf_persistent=(2, 1), first_sample: int = 1 << 20) -> dict:
-    """Tune the megakernel stack placement and the wavefront grid/block
-    sizes on the scene's current camera/sensor; returns the choice and the
-    timings (ms per spp) of both integrators."""
+                wf_persistent=((2, 0), (1, 16)), first_sample: int = 1 << 20) -> dict:
+    """Tune the megakernel stack placement and the wavefront stack placement,
+    grid/block sizes and persistence ((mode, refill) pairs) on the scene's
+    current camera/sensor; every candidate must give the same image bit for
+    bit.  Applies the best choice of each integrator and returns the faster
+    integrator's mode with the timings (ms per spp) of both."""
     ctx = get_context(device)
     sensor = scene.getCamera().getSensor()
     res_mk, res_wf = {}, {}
@@ -83,28 +85,28 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
         return st["kernel_ms"] / spp, sensor.data.copy()
 
     for st in stacks:
-        ctx.set_tuning(stack=st)
+        ctx.set_tuning(render_stack=st)
         run(cfg)
         ms, img = run(cfg)
         ref = img if ref is None else ref
         assert img.tobytes() == ref.tobytes(), "stack placement changed the image"
         res_mk[st] = ms
     best_stack = min(res_mk, key=res_mk.get)
+    ctx.set_tuning(render_stack=best_stack)
     wf_cfg = Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
-    for st, g, b, pm in itertools.product(stacks, wf_grids, wf_blocks, wf_persistent):
-        ctx.set_tuning(stack=st, wf_trace_grid=g, wf_shade_grid=g, wf_trace_block=b,
-                       persistent=pm)
+    for st, g, b, (pm, rf) in itertools.product(stacks, wf_grids, wf_blocks, wf_persistent):
+        ctx.set_tuning(wf_stack=st, wf_trace_grid=g, wf_shade_grid=g, wf_trace_block=b,
+                       wf_persistent=pm, **({"wf_refill": rf} if pm == 1 else {}))
         ms, img = run(wf_cfg)
         assert img.tobytes() == ref.tobytes(), "wavefront launch shape changed the image"
-        res_wf[(st, g, b, pm)] = ms
+        res_wf[(st, g, b, pm, rf)] = ms
     best_wf = min(res_wf, key=res_wf.get)
+    st, g, b, pm, rf = best_wf
+    ctx.set_tuning(wf_stack=st, wf_trace_grid=g, wf_shade_grid=g, wf_trace_block=b,
+                   wf_persistent=pm, **({"wf_refill": rf} if pm == 1 else {}))
     mk_ms, wf_ms = res_mk[best_stack], res_wf[best_wf]
-    use_wf = wf_ms < mk_ms
-    stack = best_wf[0] if use_wf else best_stack
-    ctx.set_tuning(stack=stack, wf_trace_grid=best_wf[1], wf_shade_grid=best_wf[1],
-                   wf_trace_block=best_wf[2], persistent=best_wf[3])
-    return {"mode": L.MODE_WAVEFRONT if use_wf else L.MODE_MEGAKERNEL,
-            "stack": stack, "wf_grid": best_wf[1], "wf_trace_block": best_wf[2],
-            "wf_persistent": best_wf[3],
+    return {"mode": L.MODE_WAVEFRONT if wf_ms < mk_ms else L.MODE_MEGAKERNEL,
+            "render_stack": best_stack, "wf_stack": st, "wf_grid": g, "wf_trace_block": b,
+            "wf_persistent": pm, "wf_refill": rf,
             "megakernel_ms": {str(k): round(v, 3) for k, v in res_mk.items()},
             "wavefront_ms": {str(k): round(v, 3) for k, v in res_wf.items()}}

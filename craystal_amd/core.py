@@ -36,8 +36,7 @@ class Context:
         self.handle = h
         self.device = device
 
-    TUNING_FIELDS = ("stack", "trace_block", "wf_trace_grid", "wf_trace_block",
-                     "wf_shade_grid", "persistent", "refill", "pt_grid")
+    TUNING_FIELDS = tuple(k for k, _ in L.crh_tuning._fields_)
 
     def set_tuning(self, **kw):
         """Launch choices that never change results (see crh_tuning)."""

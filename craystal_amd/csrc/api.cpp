@@ -162,7 +162,7 @@ struct CtxRef {
 // The private stack holds kPrivStack levels; deeper trees use the LDS stack.
 crh::Tuning effective(const crh::Tuning& t, const crh::SceneView& v) {
     crh::Tuning e = t;
-    if (v.stackCap > uint32_t(crh::kPrivStackLevels)) e.privStack = 0;
+    if (v.stackCap > uint32_t(crh::kPrivStackLevels)) e.privStack = e.renderPriv = e.wfPriv = 0;
     return e;
 }
 crh::WfLaunch wfLaunch(const crh::Tuning& t) {
@@ -170,9 +170,9 @@ crh::WfLaunch wfLaunch(const crh::Tuning& t) {
     w.traceGrid = t.wfTraceGrid;
     w.traceBlock = t.wfTraceBlock;
     w.shadeGrid = t.wfShadeGrid;
-    w.privStack = t.privStack;
-    w.persistent = t.persistent;
-    w.refill = t.refill;
+    w.privStack = t.wfPriv;
+    w.persistent = t.wfPersistent;
+    w.refill = t.wfRefill;
     return w;
 }
 }  // namespace
@@ -351,37 +351,49 @@ int crh_context_destroy(crh_context* ctx) {
 int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(ctx && t, "NULL argument");
     auto okBlock = [](uint32_t b) { return b == 0 || (b >= 64 && b <= 256 && b % 64 == 0); };
-    ARG_CHECK(t->stack <= 2, "stack must be 0 (keep), 1 (LDS) or 2 (private)");
+    auto okStack = [](uint32_t v) { return v <= 2; };
+    ARG_CHECK(okStack(t->stack) && okStack(t->render_stack) && okStack(t->wf_stack),
+              "stack must be 0 (keep), 1 (LDS) or 2 (private)");
     ARG_CHECK(okBlock(t->trace_block) && okBlock(t->wf_trace_block),
               "block sizes must be multiples of 64 in [64, 256]");
     ARG_CHECK(t->wf_trace_grid <= (1u << 20) && t->wf_shade_grid <= (1u << 20) &&
                   t->pt_grid <= (1u << 20),
               "grid too large");
-    ARG_CHECK(t->persistent <= 2, "persistent must be 0 (keep), 1 (on) or 2 (off)");
-    ARG_CHECK(t->refill <= 64, "refill must be in [1, 64]");
+    ARG_CHECK(t->persistent <= 2 && t->wf_persistent <= 2,
+              "persistent must be 0 (keep), 1 (on) or 2 (off)");
+    ARG_CHECK(t->refill <= 64 && t->wf_refill <= 64, "refill must be in [1, 64]");
     crh::Tuning& o = ctx->tuning;
     if (t->stack) o.privStack = t->stack == 2 ? 1u : 0u;
     if (t->trace_block) o.traceBlock = t->trace_block;
-    if (t->wf_trace_grid) o.wfTraceGrid = t->wf_trace_grid;
-    if (t->wf_trace_block) o.wfTraceBlock = t->wf_trace_block;
-    if (t->wf_shade_grid) o.wfShadeGrid = t->wf_shade_grid;
     if (t->persistent) o.persistent = t->persistent == 1 ? 1u : 0u;
     if (t->refill) o.refill = t->refill;
     if (t->pt_grid) o.ptGrid = t->pt_grid;
+    if (t->render_stack) o.renderPriv = t->render_stack == 2 ? 1u : 0u;
+    if (t->wf_stack) o.wfPriv = t->wf_stack == 2 ? 1u : 0u;
+    if (t->wf_trace_grid) o.wfTraceGrid = t->wf_trace_grid;
+    if (t->wf_trace_block) o.wfTraceBlock = t->wf_trace_block;
+    if (t->wf_shade_grid) o.wfShadeGrid = t->wf_shade_grid;
+    if (t->wf_persistent) o.wfPersistent = t->wf_persistent == 1 ? 1u : 0u;
+    if (t->wf_refill) o.wfRefill = t->wf_refill;
     return CRH_OK;
 }
 
 int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
     ARG_CHECK(ctx && t, "NULL argument");
     std::memset(t, 0, sizeof(*t));
-    t->stack = ctx->tuning.privStack ? 2u : 1u;
-    t->trace_block = ctx->tuning.traceBlock;
-    t->wf_trace_grid = ctx->tuning.wfTraceGrid;
-    t->wf_trace_block = ctx->tuning.wfTraceBlock;
-    t->wf_shade_grid = ctx->tuning.wfShadeGrid;
-    t->persistent = ctx->tuning.persistent ? 1u : 2u;
-    t->refill = ctx->tuning.refill;
-    t->pt_grid = ctx->tuning.ptGrid;
+    const crh::Tuning& o = ctx->tuning;
+    t->stack = o.privStack ? 2u : 1u;
+    t->trace_block = o.traceBlock;
+    t->persistent = o.persistent ? 1u : 2u;
+    t->refill = o.refill;
+    t->pt_grid = o.ptGrid;
+    t->render_stack = o.renderPriv ? 2u : 1u;
+    t->wf_stack = o.wfPriv ? 2u : 1u;
+    t->wf_trace_grid = o.wfTraceGrid;
+    t->wf_trace_block = o.wfTraceBlock;
+    t->wf_shade_grid = o.wfShadeGrid;
+    t->wf_persistent = o.wfPersistent ? 1u : 2u;
+    t->wf_refill = o.wfRefill;
     return CRH_OK;
 }
 
@@ -955,13 +967,20 @@ int crh_render(crh_scene* s, const crh_camera* cam,
 
 struct crh_guide {
     CtxRef ctx;
+    // loss: the losses of the last kLossHist Adam steps (step k in slot
+    // k % kLossHist), then one slot for gradient-only steps.
     DevMem theta, m, v, grad, wq, wqT, loss, samples, hsave, dsave;
     crh::GuideView view{};
     uint32_t step = 0;
+    uint32_t lastSamples = 0;
     size_t capSamples = 0;
 };
 
 namespace {
+
+constexpr uint32_t kLossHist = 1024;
+
+float* lossSlot(crh_guide* g) { return g->loss.as<float>() + (g->step % kLossHist); }
 
 // He-uniform init (fan-in 64) from SplitMix64; biases zero.
 void heInit(std::vector<float>& th, uint32_t seed) {
@@ -1013,7 +1032,7 @@ int crh_guide_create(crh_context* ctx, const float* bmin, const float* bmax, uin
     if (e == hipSuccess) e = g->grad.alloc(crh::kGParams * 4);
     if (e == hipSuccess) e = g->wq.alloc(crh::kGQElems * 2);
     if (e == hipSuccess) e = g->wqT.alloc(crh::kGQElems * 2);
-    if (e == hipSuccess) e = g->loss.alloc(256);
+    if (e == hipSuccess) e = g->loss.alloc((kLossHist + 1) * sizeof(float));
     if (e != hipSuccess) {
         delete g;
         return hipFail("guide allocation", e, __FILE__, __LINE__);
@@ -1105,18 +1124,48 @@ int crh_guide_train(crh_guide* g, const float* samples, uint32_t n, float lr, in
     }
     HIP_TRY(hipMemcpyAsync(g->samples.p, samples, size_t(n) * sizeof(crh::GuideSample),
                            hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemsetAsync(g->loss.p, 0, 4, ctx->stream));
     if (apply_adam) g->step++;
+    float* slot = apply_adam ? lossSlot(g) : g->loss.as<float>() + kLossHist;
+    HIP_TRY(hipMemsetAsync(slot, 0, 4, ctx->stream));
     HIP_TRY(crh::launchGuideTrainStep(g->view, g->samples.as<crh::GuideSample>(), n,
                                       g->hsave.as<uint16_t>(), g->dsave.as<uint16_t>(),
-                                      g->loss.as<float>(), lr, g->step, apply_adam != 0,
-                                      ctx->stream));
+                                      slot, lr, g->step, apply_adam != 0, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (loss) HIP_TRY(hipMemcpy(loss, g->loss.p, 4, hipMemcpyDeviceToHost));
+    if (loss) HIP_TRY(hipMemcpy(loss, slot, 4, hipMemcpyDeviceToHost));
     if (!apply_adam) {
         if (grad) HIP_TRY(hipMemcpy(grad, g->grad.p, crh::kGParams * 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemset(g->grad.p, 0, g->grad.bytes));
     }
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_guide_get_info(const crh_guide* g, crh_guide_info* info) {
+    ARG_CHECK(g && info, "NULL argument");
+    GUARD_BEGIN
+    std::memset(info, 0, sizeof(*info));
+    info->steps = g->step;
+    info->last_samples = g->lastSamples;
+    if (g->step) {
+        HIP_TRY(hipSetDevice(g->ctx->device));
+        HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+        HIP_TRY(hipMemcpy(&info->last_loss, g->loss.as<float>() + (g->step % kLossHist), 4,
+                          hipMemcpyDeviceToHost));
+    }
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_guide_get_losses(const crh_guide* g, float* losses, uint32_t n) {
+    ARG_CHECK(g && (losses || n == 0), "NULL argument");
+    ARG_CHECK(n <= std::min(g->step, kLossHist), "n exceeds the recorded steps");
+    GUARD_BEGIN
+    if (n == 0) return CRH_OK;
+    HIP_TRY(hipSetDevice(g->ctx->device));
+    HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+    std::vector<float> all(kLossHist);
+    HIP_TRY(hipMemcpy(all.data(), g->loss.p, kLossHist * sizeof(float), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) losses[i] = all[(g->step - n + 1 + i) % kLossHist];
     return CRH_OK;
     GUARD_END
 }
@@ -1183,13 +1232,15 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
                                    hipMemcpyDeviceToHost, ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
             lastSamples = produced;
+            g->lastSamples = produced;
             const crh::GuideSample* smp = film->gSamples.as<crh::GuideSample>();
             for (uint32_t k = 0; k < gc->train_steps && size_t(k + 1) * batch <= produced; ++k) {
                 g->step++;
+                float* slot = lossSlot(g);
+                HIP_TRY(hipMemsetAsync(slot, 0, 4, ctx->stream));
                 HIP_TRY(crh::launchGuideTrainStep(g->view, smp + size_t(k) * batch, batch,
                                                   g->hsave.as<uint16_t>(), g->dsave.as<uint16_t>(),
-                                                  g->loss.as<float>(), gc->lr, g->step, true,
-                                                  ctx->stream));
+                                                  slot, gc->lr, g->step, true, ctx->stream));
             }
         }
     }

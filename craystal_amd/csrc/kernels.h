@@ -13,14 +13,21 @@ constexpr uint32_t kRenderBlock = 256;
 
 // Launch choices that do not change results (set by the autotuner).
 struct Tuning {
-    uint32_t privStack = 0;     // 1: traversal stack in private memory
-    uint32_t traceBlock = 256;  // threads per block, ray-batch kernels (64..256)
-    uint32_t wfTraceGrid = 2048;
-    uint32_t wfShadeGrid = 2048;
-    uint32_t wfTraceBlock = 256;
-    uint32_t persistent = 0;    // 1: ray-batch kernels refill lanes with new rays
+    // ray-batch kernels (crh_trace_*)
+    uint32_t privStack = 1;     // 1: traversal stack in private memory
+    uint32_t traceBlock = 64;   // threads per block (64..256)
+    uint32_t persistent = 0;    // 1: lanes refill with new rays from a work counter
     uint32_t refill = 16;       // idle lanes per wave that trigger a refill
-    uint32_t ptGrid = 2048;     // blocks of the persistent ray-batch kernels
+    uint32_t ptGrid = 2048;     // blocks of the persistent variant
+    // megakernel
+    uint32_t renderPriv = 0;
+    // wavefront
+    uint32_t wfPriv = 0;
+    uint32_t wfTraceGrid = 16384;
+    uint32_t wfTraceBlock = 128;
+    uint32_t wfShadeGrid = 16384;
+    uint32_t wfPersistent = 0;
+    uint32_t wfRefill = 16;
 };
 
 // counter: one device uint32 used by the persistent variants (may be null).

@@ -264,15 +264,15 @@ __global__ __launch_bounds__(256) void kFilmGather(
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-static size_t stackLds(const SceneView& s, const Tuning& t, uint32_t block) {
-    return t.privStack ? 0 : size_t(s.stackCap) * block * sizeof(uint2);
+static size_t stackLds(const SceneView& s, uint32_t priv, uint32_t block) {
+    return priv ? 0 : size_t(s.stackCap) * block * sizeof(uint2);
 }
 
 template <bool ANY>
 static hipError_t launchPersistent(const SceneView& s, const BatchSrc& src, uint32_t n,
                                    uint32_t* counter, const Tuning& t, hipStream_t st) {
     const uint32_t B = t.traceBlock;
-    size_t lds = stackLds(s, t, B);
+    size_t lds = stackLds(s, t.privStack, B);
     hipLaunchKernelGGL(kResetCounter, dim3(1), dim3(64), 0, st, counter);
     dim3 grid(t.ptGrid), block(B);
     if (t.privStack)
@@ -292,7 +292,7 @@ hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
         return launchPersistent<false>(s, BatchSrc{rays, hits, nullptr}, uint32_t(n), counter, t,
                                        st);
     const uint32_t B = t.traceBlock;
-    size_t lds = stackLds(s, t, B);
+    size_t lds = stackLds(s, t.privStack, B);
     dim3 grid(uint32_t((n + B - 1) / B)), block(B);
     if (work && t.privStack)
         hipLaunchKernelGGL((kTraceClosest<true, true>), grid, block, lds, st, s, rays, n, hits, work);
@@ -311,7 +311,7 @@ hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
     if (t.persistent && counter && n < (uint64_t(1) << 32))
         return launchPersistent<true>(s, BatchSrc{rays, nullptr, occ}, uint32_t(n), counter, t, st);
     const uint32_t B = t.traceBlock;
-    size_t lds = stackLds(s, t, B);
+    size_t lds = stackLds(s, t.privStack, B);
     dim3 grid(uint32_t((n + B - 1) / B)), block(B);
     if (t.privStack)
         hipLaunchKernelGGL((kTraceAny<true>), grid, block, lds, st, s, rays, n, occ);
@@ -334,9 +334,9 @@ hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
                             float2* P, unsigned long long* counters,
                             bool countWork, const Tuning& t, hipStream_t st) {
     uint32_t tilesX = (cam.width + 15u) / 16u, tilesY = (cam.height + 15u) / 16u;
-    size_t lds = stackLds(s, t, kRenderBlock);
+    size_t lds = stackLds(s, t.renderPriv, kRenderBlock);
     dim3 grid(tilesX * tilesY);
-    const int sel = (cfg.useMIS ? 4 : 0) | (countWork ? 2 : 0) | (t.privStack ? 1 : 0);
+    const int sel = (cfg.useMIS ? 4 : 0) | (countWork ? 2 : 0) | (t.renderPriv ? 1 : 0);
     switch (sel) {
         case 0: renderLaunch<false, false, false>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
         case 1: renderLaunch<false, false, true>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;

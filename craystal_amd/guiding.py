@@ -62,6 +62,20 @@ class Guide:
                L.ptr(loss), None if grad is None else L.ptr(grad))
         return float(loss[0]), grad
 
+    def info(self) -> dict:
+        """Adam steps taken, samples of the last trained pass, last loss."""
+        i = L.crh_guide_info()
+        L.call("crh_guide_get_info", self.handle, C.byref(i))
+        return {"steps": i.steps, "last_samples": i.last_samples, "last_loss": i.last_loss}
+
+    def losses(self, n: int | None = None) -> np.ndarray:
+        """Losses of the last n Adam steps (at most 1024), oldest first."""
+        steps = self.info()["steps"]
+        n = min(steps, 1024) if n is None else n
+        out = np.zeros(n, np.float32)
+        L.call("crh_guide_get_losses", self.handle, L.ptr(out), n)
+        return out
+
     def __del__(self):
         if getattr(self, "handle", None) and L._lib is not None:
             L._lib.crh_guide_destroy(self.handle)

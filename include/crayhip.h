@@ -198,14 +198,18 @@ typedef struct crh_stats {
 /* Launch choices that never change results (picked by the autotuner,
  * craystal_amd/autotune.py).  Zero fields keep the current value. */
 typedef struct crh_tuning {
-    uint32_t stack;          /* 1 = LDS traversal stack, 2 = private (scratch) */
-    uint32_t trace_block;    /* threads per block, ray-batch kernels: 64..256 */
+    uint32_t stack;          /* ray batches: 1 = LDS traversal stack, 2 = private (scratch) */
+    uint32_t trace_block;    /* ray batches: threads per block, 64..256 */
+    uint32_t persistent;     /* ray batches: 1 = lanes refill from a work counter, 2 = off */
+    uint32_t refill;         /* ray batches: idle lanes per wave that trigger a refill, 1..64 */
+    uint32_t pt_grid;        /* ray batches: blocks of the persistent kernels */
+    uint32_t render_stack;   /* megakernel: 1 = LDS, 2 = private */
+    uint32_t wf_stack;       /* wavefront traversal kernels: 1 = LDS, 2 = private */
     uint32_t wf_trace_grid;  /* wavefront traversal kernels: blocks */
     uint32_t wf_trace_block; /* wavefront traversal kernels: threads (64..256) */
     uint32_t wf_shade_grid;  /* wavefront shading kernels: blocks of 256 */
-    uint32_t persistent;     /* 1 = ray batches refill lanes (persistent), 2 = off */
-    uint32_t refill;         /* idle lanes per wave that trigger a refill, 1..64 */
-    uint32_t pt_grid;        /* blocks of the persistent ray-batch kernels */
+    uint32_t wf_persistent;  /* wavefront traversal: 1 = lanes refill, 2 = off */
+    uint32_t wf_refill;      /* wavefront traversal: idle lanes per wave that trigger a refill */
 } crh_tuning;
 
 /* ---- errors / device ------------------------------------------------- */
@@ -317,6 +321,17 @@ int crh_guide_eval(crh_guide* g, const float* pos, const float* nrm,
  * gradient in grad (CRH_GUIDE_PARAMS floats, may be NULL); loss may be NULL. */
 int crh_guide_train(crh_guide* g, const float* samples, uint32_t n, float lr,
                     int apply_adam, float* loss, float* grad);
+/* Training progress: Adam steps taken, training samples the last trained
+ * render pass produced, loss of the last step. */
+typedef struct crh_guide_info {
+    uint32_t steps;
+    uint32_t last_samples;
+    float last_loss;
+    uint32_t reserved;
+} crh_guide_info;
+int crh_guide_get_info(const crh_guide* g, crh_guide_info* info);
+/* Losses of the last n Adam steps, oldest first; n <= min(steps, 1024). */
+int crh_guide_get_losses(const crh_guide* g, float* losses, uint32_t n);
 int crh_guide_destroy(crh_guide* g);
 
 /* Guided rendering: the wavefront integrator with one-sample MIS between

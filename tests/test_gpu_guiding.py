@@ -142,6 +142,10 @@ def test_guided_render_is_unbiased(gpu):
         sensor.clear()
         st = gi.dispatch(scene, 1, first_sample=10000 + k, train=True)
     assert st["train_samples"] > 0
+    info = guide.info()
+    assert info["steps"] == 12 and info["last_samples"] == st["train_samples"]
+    losses = guide.losses()
+    assert len(losses) == 12 and np.isfinite(losses).all() and losses[-1] == info["last_loss"]
     plain, guided = [], []
     for k in range(24):
         sensor.clear()

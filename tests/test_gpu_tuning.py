@@ -47,6 +47,8 @@ def test_render_tuning_invariance(gpu):
         res = autotune.tune_render(scene, cfg, spp=1, wf_grids=(64, 512), wf_blocks=(64, 256))
         assert res["mode"] in (L.MODE_MEGAKERNEL, L.MODE_WAVEFRONT)
         assert len(res["wavefront_ms"]) == 16
+        t = ctx.get_tuning()
+        assert t["render_stack"] == res["render_stack"] and t["wf_stack"] == res["wf_stack"]
     finally:
         ctx.set_tuning(**saved)
 
@@ -57,3 +59,5 @@ def test_tuning_rejects_bad_values(gpu):
         ctx.set_tuning(trace_block=96)
     with pytest.raises(cr.RuntimeError):
         ctx.set_tuning(stack=7)
+    with pytest.raises(cr.RuntimeError):
+        ctx.set_tuning(wf_refill=65)
