@@ -96,7 +96,14 @@ class crh_integrator_cfg(C.Structure):
 class crh_tuning(C.Structure):
     _fields_ = [(k, u32) for k in ("stack", "trace_block", "persistent", "refill", "pt_grid",
                                    "render_stack", "wf_stack", "wf_trace_grid", "wf_trace_block",
-                                   "wf_shade_grid", "wf_persistent", "wf_refill")]
+                                   "wf_shade_grid", "wf_persistent", "wf_refill",
+                                   "batch_stack", "batch_tile_w", "batch_grid", "batch_chunks",
+                                   "batch_graph")]
+
+
+class crh_batch_info(C.Structure):
+    _fields_ = [("scenes", u32), ("max_stack", u32), ("pool_bytes", C.c_uint64),
+                ("nodes", C.c_uint64), ("triangles", C.c_uint64)]
 
 
 class crh_guided_cfg(C.Structure):
@@ -168,6 +175,11 @@ SIGNATURES = {
     "crh_guide_get_info": [_P, C.POINTER(crh_guide_info)],
     "crh_guide_get_losses": [_P, _P, u32],
     "crh_guide_destroy": [_P],
+    "crh_batch_create": [_P, _P, u32, _P],
+    "crh_batch_get_info": [_P, C.POINTER(crh_batch_info)],
+    "crh_batch_render": [_P, _P, _P, _P, C.POINTER(crh_integrator_cfg), u32,
+                         C.POINTER(crh_stats)],
+    "crh_batch_destroy": [_P],
     "crh_render_guided": [_P, C.POINTER(crh_camera), C.POINTER(crh_integrator_cfg), _P,
                           C.POINTER(crh_guided_cfg), u32, u32, _P, C.POINTER(crh_stats)],
 }

@@ -110,3 +110,37 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
             "wf_persistent": pm, "wf_refill": rf,
             "megakernel_ms": {str(k): round(v, 3) for k, v in res_mk.items()},
             "wavefront_ms": {str(k): round(v, 3) for k, v in res_wf.items()}}
+
+
+def tune_batch(batch, cameras, spp, cfg: Configs, device: int = 0, tile_ws=(8, 16, 32, 64),
+               grids=(512, 1024, 2048), chunks=(1, 8), stacks=(1, 2), reps: int = 2,
+               first_sample: int = 1 << 20) -> dict:
+    """Tune the batched renderer's tile shape, persistent grid, work ranges
+    and stack placement.  Every candidate renders the whole batch; all
+    films must come out bit-identical before the fastest choice is applied."""
+    ctx = get_context(device)
+    sensors = [c.getSensor() for c in cameras]
+    ref, res = None, {}
+    for tw, g, ch, st in itertools.product(tile_ws, grids, chunks, stacks):
+        ctx.set_tuning(batch_tile_w=tw, batch_grid=g, batch_chunks=ch, batch_stack=st,
+                       batch_graph=1)
+        ms = []
+        for r in range(reps + 1):
+            for s in sensors:
+                s.clear()
+            out = batch.render(cameras, spp, cfg, first_sample=first_sample, readback=(r == 0))
+            if r == 0:
+                imgs = b"".join(s.data.tobytes() + s.weight.tobytes() for s in sensors)
+                if ref is None:
+                    ref = imgs
+                elif imgs != ref:
+                    raise AssertionError(f"batch tuning {(tw, g, ch, st)} changed an image")
+            else:
+                ms.append(out["kernel_ms"])
+        res[(tw, g, ch, st)] = _median(ms)
+    best = min(res, key=res.get)
+    tw, g, ch, st = best
+    ctx.set_tuning(batch_tile_w=tw, batch_grid=g, batch_chunks=ch, batch_stack=st, batch_graph=1)
+    return {"batch_tile_w": tw, "batch_grid": g, "batch_chunks": ch, "batch_stack": st,
+            "timings_ms": {f"tile_w={k[0]},grid={k[1]},chunks={k[2]},stack={k[3]}": round(v, 4)
+                           for k, v in res.items()}}

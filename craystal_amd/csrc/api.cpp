@@ -162,8 +162,25 @@ struct CtxRef {
 // The private stack holds kPrivStack levels; deeper trees use the LDS stack.
 crh::Tuning effective(const crh::Tuning& t, const crh::SceneView& v) {
     crh::Tuning e = t;
-    if (v.stackCap > uint32_t(crh::kPrivStackLevels)) e.privStack = e.renderPriv = e.wfPriv = 0;
+    if (v.stackCap > uint32_t(crh::kPrivStackLevels))
+        e.privStack = e.renderPriv = e.wfPriv = e.batchPriv = 0;
     return e;
+}
+// Core/CameraProxy.h:25-40 view of a camera for a w x h film: aspect in
+// float, tan(radians(fovY * 0.5)) in double (fovY * 0.5 promotes), narrowed.
+crh::CameraView makeCameraView(const crh_camera* cam, uint32_t w, uint32_t h) {
+    crh::CameraView cv;
+    for (int k = 0; k < 3; ++k) {
+        cv.pos[k] = cam->pos[k];
+        cv.u[k] = cam->u[k];
+        cv.v[k] = cam->v[k];
+        cv.w[k] = cam->w[k];
+    }
+    cv.hh = float(std::tan(double(cam->fov_y) * 0.5 * 0.01745329251994329576923690768489));
+    cv.aspect = float(w) / float(h);
+    cv.width = w;
+    cv.height = h;
+    return cv;
 }
 crh::WfLaunch wfLaunch(const crh::Tuning& t) {
     crh::WfLaunch w;
@@ -204,6 +221,9 @@ struct crh_film {
     // Guided-mode state, allocated on first guided render.
     DevMem gArena, gSamples;
     bool guidedReady = false;
+    // Training samples waiting in gSamples (accumulated over passes until a
+    // batch is full) and the buffer's capacity in samples.
+    size_t gPending = 0, gCap = 0;
 };
 
 namespace {
@@ -275,7 +295,6 @@ hipError_t ensureGuided(crh_film* f) {
     const size_t total = 3 * align(n * 16) + align(n * 40 * 4) + align(n * 4) +
                          align(cap * crh::kRecF4 * 16);
     e = f->gArena.alloc(total);
-    if (e == hipSuccess) e = f->gSamples.alloc(cap * sizeof(crh::GuideSample));
     if (e != hipSuccess) return e;
     char* base = f->gArena.as<char>();
     size_t off = 0;
@@ -351,9 +370,9 @@ int crh_context_destroy(crh_context* ctx) {
 int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(ctx && t, "NULL argument");
     auto okBlock = [](uint32_t b) { return b == 0 || (b >= 64 && b <= 256 && b % 64 == 0); };
-    auto okStack = [](uint32_t v) { return v <= 2; };
+    auto okStack = [](uint32_t v) { return v <= 3; };
     ARG_CHECK(okStack(t->stack) && okStack(t->render_stack) && okStack(t->wf_stack),
-              "stack must be 0 (keep), 1 (LDS) or 2 (private)");
+              "stack must be 0 (keep), 1 (LDS), 2 (private) or 3 (short LDS + private)");
     ARG_CHECK(okBlock(t->trace_block) && okBlock(t->wf_trace_block),
               "block sizes must be multiples of 64 in [64, 256]");
     ARG_CHECK(t->wf_trace_grid <= (1u << 20) && t->wf_shade_grid <= (1u << 20) &&
@@ -362,19 +381,32 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(t->persistent <= 2 && t->wf_persistent <= 2,
               "persistent must be 0 (keep), 1 (on) or 2 (off)");
     ARG_CHECK(t->refill <= 64 && t->wf_refill <= 64, "refill must be in [1, 64]");
+    ARG_CHECK(okStack(t->batch_stack),
+              "batch_stack must be 0 (keep), 1 (LDS), 2 (private) or 3 (short LDS + private)");
+    ARG_CHECK(t->batch_tile_w == 0 || t->batch_tile_w == 8 || t->batch_tile_w == 16 ||
+                  t->batch_tile_w == 32 || t->batch_tile_w == 64,
+              "batch_tile_w must be 8, 16, 32 or 64");
+    ARG_CHECK(t->batch_grid <= (1u << 20), "grid too large");
+    ARG_CHECK(t->batch_chunks <= crh::kBatchMaxChunks, "batch_chunks must be in [1, 8]");
+    ARG_CHECK(t->batch_graph <= 2, "batch_graph must be 0 (keep), 1 (on) or 2 (off)");
     crh::Tuning& o = ctx->tuning;
-    if (t->stack) o.privStack = t->stack == 2 ? 1u : 0u;
+    if (t->stack) o.privStack = t->stack - 1u;
     if (t->trace_block) o.traceBlock = t->trace_block;
     if (t->persistent) o.persistent = t->persistent == 1 ? 1u : 0u;
     if (t->refill) o.refill = t->refill;
     if (t->pt_grid) o.ptGrid = t->pt_grid;
-    if (t->render_stack) o.renderPriv = t->render_stack == 2 ? 1u : 0u;
-    if (t->wf_stack) o.wfPriv = t->wf_stack == 2 ? 1u : 0u;
+    if (t->render_stack) o.renderPriv = t->render_stack - 1u;
+    if (t->wf_stack) o.wfPriv = t->wf_stack - 1u;
     if (t->wf_trace_grid) o.wfTraceGrid = t->wf_trace_grid;
     if (t->wf_trace_block) o.wfTraceBlock = t->wf_trace_block;
     if (t->wf_shade_grid) o.wfShadeGrid = t->wf_shade_grid;
     if (t->wf_persistent) o.wfPersistent = t->wf_persistent == 1 ? 1u : 0u;
     if (t->wf_refill) o.wfRefill = t->wf_refill;
+    if (t->batch_stack) o.batchPriv = t->batch_stack - 1u;
+    if (t->batch_tile_w) o.batchTileW = t->batch_tile_w;
+    if (t->batch_grid) o.batchGrid = t->batch_grid;
+    if (t->batch_chunks) o.batchChunks = t->batch_chunks;
+    if (t->batch_graph) o.batchGraph = t->batch_graph == 1 ? 1u : 0u;
     return CRH_OK;
 }
 
@@ -382,18 +414,23 @@ int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
     ARG_CHECK(ctx && t, "NULL argument");
     std::memset(t, 0, sizeof(*t));
     const crh::Tuning& o = ctx->tuning;
-    t->stack = o.privStack ? 2u : 1u;
+    t->stack = o.privStack + 1u;
     t->trace_block = o.traceBlock;
     t->persistent = o.persistent ? 1u : 2u;
     t->refill = o.refill;
     t->pt_grid = o.ptGrid;
-    t->render_stack = o.renderPriv ? 2u : 1u;
-    t->wf_stack = o.wfPriv ? 2u : 1u;
+    t->render_stack = o.renderPriv + 1u;
+    t->wf_stack = o.wfPriv + 1u;
     t->wf_trace_grid = o.wfTraceGrid;
     t->wf_trace_block = o.wfTraceBlock;
     t->wf_shade_grid = o.wfShadeGrid;
     t->wf_persistent = o.wfPersistent ? 1u : 2u;
     t->wf_refill = o.wfRefill;
+    t->batch_stack = o.batchPriv + 1u;
+    t->batch_tile_w = o.batchTileW;
+    t->batch_grid = o.batchGrid;
+    t->batch_chunks = o.batchChunks;
+    t->batch_graph = o.batchGraph ? 1u : 2u;
     return CRH_OK;
 }
 
@@ -879,20 +916,7 @@ int crh_render(crh_scene* s, const crh_camera* cam,
     GUARD_BEGIN
     crh_context* ctx = s->ctx;
     if (stats) std::memset(stats, 0, sizeof(*stats));
-    crh::CameraView cv;
-    for (int k = 0; k < 3; ++k) {
-        cv.pos[k] = cam->pos[k];
-        cv.u[k] = cam->u[k];
-        cv.v[k] = cam->v[k];
-        cv.w[k] = cam->w[k];
-    }
-    // Core/CameraProxy.h:31-34: aspect in float, tan(radians(fovY * 0.5)) in
-    // double (fovY * 0.5 promotes), then narrowed.
-    cv.hh = float(std::tan(double(cam->fov_y) * 0.5 *
-                           0.01745329251994329576923690768489));
-    cv.aspect = float(film->w) / float(film->h);
-    cv.width = film->w;
-    cv.height = film->h;
+    const crh::CameraView cv = makeCameraView(cam, film->w, film->h);
     crh::IntegratorView iv{cfg->max_depth, cfg->rr_threshold,
                            cfg->rr_start_depth, cfg->use_mis ? 1u : 0u};
     double t0 = nowMs();
@@ -1190,17 +1214,7 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
     GUARD_BEGIN
     crh_context* ctx = s->ctx;
     if (stats) std::memset(stats, 0, sizeof(*stats));
-    crh::CameraView cv;
-    for (int k = 0; k < 3; ++k) {
-        cv.pos[k] = cam->pos[k];
-        cv.u[k] = cam->u[k];
-        cv.v[k] = cam->v[k];
-        cv.w[k] = cam->w[k];
-    }
-    cv.hh = float(std::tan(double(cam->fov_y) * 0.5 * 0.01745329251994329576923690768489));
-    cv.aspect = float(film->w) / float(film->h);
-    cv.width = film->w;
-    cv.height = film->h;
+    const crh::CameraView cv = makeCameraView(cam, film->w, film->h);
     crh::IntegratorView iv{cfg->max_depth, cfg->rr_threshold, cfg->rr_start_depth,
                            cfg->use_mis ? 1u : 0u};
     double t0 = nowMs();
@@ -1216,13 +1230,22 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
         HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * padB * 64 * 2));
         g->capSamples = padB;
     }
+    // Sample buffer: one pass's records on top of a batch still filling.
+    const size_t needCap = size_t(film->wf.recCap) + batch;
+    if (gc->train && film->gCap < needCap) {
+        HIP_TRY(film->gSamples.alloc(needCap * sizeof(crh::GuideSample)));
+        film->gCap = needCap;
+        film->gPending = 0;
+    }
     double mainMs = 0.0;
     uint32_t lastSamples = 0;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     for (uint32_t i = 0; i < spp; ++i) {
+        crh::GuideSample* smpBuf = film->gSamples.as<crh::GuideSample>();
         HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, film->wf, wl, false,
                                    ctx->stream, &g->view, gc->alpha,
-                                   gc->train ? film->gSamples.as<crh::GuideSample>() : nullptr));
+                                   gc->train ? smpBuf + film->gPending : nullptr,
+                                   uint32_t(film->gCap - film->gPending)));
         HIP_TRY(crh::launchFilmGather(film->L.as<float4>(), film->P.as<float2>(), film->w,
                                       film->h, film->sum.as<float>(), film->wsum.as<float>(),
                                       ctx->stream));
@@ -1231,17 +1254,30 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
             HIP_TRY(hipMemcpyAsync(&produced, film->wf.counters + crh::kCTrain, 4,
                                    hipMemcpyDeviceToHost, ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
+            produced = uint32_t(std::min<size_t>(produced, film->gCap - film->gPending));
             lastSamples = produced;
             g->lastSamples = produced;
-            const crh::GuideSample* smp = film->gSamples.as<crh::GuideSample>();
-            for (uint32_t k = 0; k < gc->train_steps && size_t(k + 1) * batch <= produced; ++k) {
+            const size_t pending = film->gPending + produced;
+            uint32_t k = 0;
+            for (; k < gc->train_steps && size_t(k + 1) * batch <= pending; ++k) {
                 g->step++;
                 float* slot = lossSlot(g);
                 HIP_TRY(hipMemsetAsync(slot, 0, 4, ctx->stream));
-                HIP_TRY(crh::launchGuideTrainStep(g->view, smp + size_t(k) * batch, batch,
+                HIP_TRY(crh::launchGuideTrainStep(g->view, smpBuf + size_t(k) * batch, batch,
                                                   g->hsave.as<uint16_t>(), g->dsave.as<uint16_t>(),
                                                   slot, gc->lr, g->step, true, ctx->stream));
             }
+            // Keep the newest unused samples (at most k batches' worth, so the
+            // move never overlaps) at the front for the next pass.
+            size_t rem = pending - size_t(k) * batch;
+            if (k > 0) {
+                rem = std::min(rem, size_t(k) * batch);
+                if (rem)
+                    HIP_TRY(hipMemcpyAsync(smpBuf, smpBuf + (pending - rem),
+                                           rem * sizeof(crh::GuideSample),
+                                           hipMemcpyDeviceToDevice, ctx->stream));
+            }
+            film->gPending = rem;
         }
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1260,6 +1296,243 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
     }
     return CRH_OK;
     GUARD_END
+}
+
+// ---- batched rendering -----------------------------------------------------
+struct crh_batch {
+    CtxRef ctx;
+    DevMem pool;  // every scene's nodes, then triangles, vertices, ... (256 B aligned slices)
+    std::vector<crh::SceneView> views;
+    uint64_t nodes = 0, tris = 0;
+    uint32_t maxStack = 1;
+    DevMem table, prefix, next, graphCounter;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> sig;  // what the captured graph was built for
+    ~crh_batch() { dropGraph(); }
+    void dropGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        sig.clear();
+    }
+};
+
+int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh_batch** out) {
+    ARG_CHECK(ctx && out && (scenes || n == 0), "NULL argument");
+    ARG_CHECK(n > 0 && n <= 4096, "scene count must be in [1, 4096]");
+    *out = nullptr;
+    for (uint32_t i = 0; i < n; ++i) {
+        ARG_CHECK(scenes[i], "NULL scene");
+        ARG_CHECK(scenes[i]->ctx == ctx, "scene belongs to a different context");
+    }
+    GUARD_BEGIN
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    // Region r of the pool holds member r of every scene, scene after scene.
+    DevMem crh_scene::*members[] = {&crh_scene::nodes, &crh_scene::tris,    &crh_scene::pos,
+                                    &crh_scene::nrm,   &crh_scene::uv,      &crh_scene::idx,
+                                    &crh_scene::primMat, &crh_scene::mats, &crh_scene::emissive};
+    constexpr int kR = 9;
+    std::vector<size_t> off(size_t(n) * kR);
+    size_t total = 0;
+    for (int r = 0; r < kR; ++r)
+        for (uint32_t i = 0; i < n; ++i) {
+            off[size_t(i) * kR + r] = total;
+            total += align((scenes[i]->*members[r]).bytes);
+        }
+    auto* b = new crh_batch();
+    b->ctx.set(ctx);
+    hipError_t e = b->pool.alloc(std::max<size_t>(total, 256));
+    if (e == hipSuccess) e = b->next.alloc(crh::kBatchMaxChunks * sizeof(uint32_t));
+    char* base = b->pool.as<char>();
+    for (uint32_t i = 0; i < n && e == hipSuccess; ++i)
+        for (int r = 0; r < kR && e == hipSuccess; ++r) {
+            const DevMem& src = scenes[i]->*members[r];
+            if (src.bytes)
+                e = hipMemcpyAsync(base + off[size_t(i) * kR + r], src.p, src.bytes,
+                                   hipMemcpyDeviceToDevice, ctx->stream);
+        }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        delete b;
+        return hipFail("batch pool", e, __FILE__, __LINE__);
+    }
+    b->views.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const crh_scene* sc = scenes[i];
+        crh::SceneView v = sc->view;
+        auto at = [&](int r) { return base + off[size_t(i) * kR + r]; };
+        v.nodes = reinterpret_cast<const crh::GNode*>(at(0));
+        v.tris = reinterpret_cast<const crh::GTri*>(at(1));
+        v.pos = reinterpret_cast<const float*>(at(2));
+        v.nrm = reinterpret_cast<const float*>(at(3));
+        v.uv = reinterpret_cast<const float*>(at(4));
+        v.idx = reinterpret_cast<const uint32_t*>(at(5));
+        v.primMat = reinterpret_cast<const uint32_t*>(at(6));
+        v.mats = reinterpret_cast<const crh::GMaterial*>(at(7));
+        v.emissive = reinterpret_cast<const uint32_t*>(at(8));
+        b->views[i] = v;
+        b->nodes += sc->nodes.bytes / sizeof(crh::GNode);
+        b->tris += sc->tris.bytes / sizeof(crh::GTri);
+        b->maxStack = std::max(b->maxStack, v.stackCap);
+    }
+    *out = b;
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_batch_get_info(const crh_batch* b, crh_batch_info* info) {
+    ARG_CHECK(b && info, "NULL argument");
+    std::memset(info, 0, sizeof(*info));
+    info->scenes = uint32_t(b->views.size());
+    info->max_stack = b->maxStack;
+    info->pool_bytes = b->pool.bytes;
+    info->nodes = b->nodes;
+    info->triangles = b->tris;
+    return CRH_OK;
+}
+
+int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* films,
+                     const uint32_t* spp, const crh_integrator_cfg* cfg, uint32_t first_sample,
+                     crh_stats* stats) {
+    ARG_CHECK(b && cams && films && spp && cfg, "NULL argument");
+    ARG_CHECK(cfg->mode == CRH_MODE_MEGAKERNEL, "batched rendering uses the megakernel mode");
+    ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
+    const uint32_t n = uint32_t(b->views.size());
+    uint32_t maxSpp = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        ARG_CHECK(films[i], "NULL film");
+        ARG_CHECK(films[i]->ctx == b->ctx, "film belongs to a different context");
+        for (uint32_t j = 0; j < i; ++j) ARG_CHECK(films[j] != films[i], "films must differ");
+        ARG_CHECK(spp[i] <= (1u << 20), "spp too large");
+        maxSpp = std::max(maxSpp, spp[i]);
+    }
+    GUARD_BEGIN
+    crh_context* ctx = b->ctx;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    double t0 = nowMs();
+    HIP_TRY(hipSetDevice(ctx->device));
+    crh::Tuning tun = ctx->tuning;
+    if (b->maxStack > uint32_t(crh::kPrivStackLevels)) tun.batchPriv = 0;
+    // Scenes by decreasing spp: those still sampling at pass k form a prefix.
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t c) { return spp[a] > spp[c]; });
+    const uint32_t tw = tun.batchTileW, th = crh::kRenderBlock / tw;
+    std::vector<crh::BatchScene> tab(n);
+    std::vector<uint32_t> pre(2 * (size_t(n) + 1), 0);  // tile prefix, then pixel prefix
+    uint32_t* tilePre = pre.data();
+    uint32_t* pixPre = pre.data() + n + 1;
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t i = order[j];
+        crh_film* f = films[i];
+        crh::BatchScene& t = tab[j];
+        std::memset(&t, 0, sizeof(t));
+        t.view = b->views[i];
+        t.cam = makeCameraView(&cams[i], f->w, f->h);
+        t.L = f->L.as<float>();
+        t.P = f->P.as<float>();
+        t.sum = f->sum.as<float>();
+        t.wsum = f->wsum.as<float>();
+        t.tilesX = (f->w + tw - 1) / tw;
+        const uint64_t tiles = uint64_t(t.tilesX) * ((f->h + th - 1) / th);
+        ARG_CHECK(uint64_t(tilePre[j]) + tiles < (uint64_t(1) << 32) &&
+                      uint64_t(pixPre[j]) + uint64_t(f->w) * f->h < (uint64_t(1) << 32),
+                  "batch too large for 32-bit item indices");
+        tilePre[j + 1] = tilePre[j] + uint32_t(tiles);
+        pixPre[j + 1] = pixPre[j] + f->w * f->h;
+    }
+    if (b->table.bytes < tab.size() * sizeof(crh::BatchScene))
+        HIP_TRY(b->table.alloc(tab.size() * sizeof(crh::BatchScene)));
+    if (b->prefix.bytes < pre.size() * sizeof(uint32_t)) {
+        HIP_TRY(b->prefix.alloc(pre.size() * sizeof(uint32_t)));
+        b->dropGraph();  // the captured graph points at the old buffers
+    }
+    DevMem counter;
+    HIP_TRY(counter.alloc(3 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(counter.p, 0, counter.bytes, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(b->table.p, tab.data(), tab.size() * sizeof(crh::BatchScene),
+                           hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(b->prefix.p, pre.data(), pre.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, ctx->stream));
+    const crh::IntegratorView iv{cfg->max_depth, cfg->rr_threshold, cfg->rr_start_depth,
+                                 cfg->use_mis ? 1u : 0u};
+    const auto* dTab = b->table.as<crh::BatchScene>();
+    const uint32_t* dTile = b->prefix.as<uint32_t>();
+    const uint32_t* dPix = dTile + n + 1;
+    auto* dCount = counter.as<unsigned long long>();
+    // The pass sequence; launched directly or recorded into a graph.
+    auto passes = [&](hipStream_t st, unsigned long long* cnt) -> hipError_t {
+        uint32_t active = n;
+        for (uint32_t k = 0; k < maxSpp; ++k) {
+            while (active > 0 && spp[order[active - 1]] <= k) --active;
+            hipError_t e = crh::launchBatchPass(dTab, dTile, dPix, active, tilePre[active],
+                                                pixPre[active], iv, first_sample + k, b->maxStack,
+                                                tun, b->next.as<uint32_t>(), cnt, st);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    if (tun.batchGraph) {
+        // The graph bakes in pass counts, item counts, launch shapes and the
+        // counter address; cameras and films come from the table each call.
+        std::vector<uint64_t> sig = {n, first_sample, maxSpp, iv.maxDepth,
+                                     uint64_t(__builtin_bit_cast(uint32_t, iv.rrThreshold)),
+                                     iv.rrStartDepth, iv.useMIS, tun.batchPriv, tw,
+                                     tun.batchGrid, tun.batchChunks};
+        for (uint32_t j = 0; j < n; ++j) sig.push_back(spp[order[j]]);
+        sig.insert(sig.end(), pre.begin(), pre.end());
+        if (!b->exec || sig != b->sig) {
+            b->dropGraph();
+            // counts of graph replays go to a counter the batch keeps
+            if (!b->graphCounter.p) HIP_TRY(b->graphCounter.alloc(3 * sizeof(unsigned long long)));
+            HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = passes(ctx->stream, b->graphCounter.as<unsigned long long>());
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+            if (e == hipSuccess) e = e2;
+            if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            b->graph = g;
+            if (e != hipSuccess) {
+                b->dropGraph();
+                return hipFail("batch graph capture", e, __FILE__, __LINE__);
+            }
+            b->sig = sig;
+        }
+        HIP_TRY(hipMemsetAsync(b->graphCounter.p, 0, b->graphCounter.bytes, ctx->stream));
+        HIP_TRY(hipGraphLaunch(b->exec, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(dCount, b->graphCounter.p, b->graphCounter.bytes,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        HIP_TRY(passes(ctx->stream, dCount));
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    if (stats) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        unsigned long long c[3] = {0, 0, 0};
+        HIP_TRY(hipMemcpy(c, dCount, sizeof(c), hipMemcpyDeviceToHost));
+        stats->kernel_ms = ms;
+        stats->main_kernel_ms = ms;
+        stats->total_ms = nowMs() - t0;
+        stats->rays = c[0];
+        stats->launches = tun.batchGraph ? 1 : uint64_t(maxSpp) * 3;
+    }
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_batch_destroy(crh_batch* b) {
+    if (!b) return CRH_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+    return CRH_OK;
 }
 
 }  // extern "C"

@@ -6,6 +6,8 @@
 #include "layout.h"
 #include "traverse_limits.h"
 
+#include <type_traits>
+
 namespace crh {
 
 constexpr uint32_t kTraceBlock = 256;   // upper bound; launches may use fewer
@@ -13,8 +15,10 @@ constexpr uint32_t kRenderBlock = 256;
 
 // Launch choices that do not change results (set by the autotuner).
 struct Tuning {
+    // Traversal stack mode of each kernel family: 0 LDS, 1 private, 2 short
+    // LDS stack with private overflow.
     // ray-batch kernels (crh_trace_*)
-    uint32_t privStack = 1;     // 1: traversal stack in private memory
+    uint32_t privStack = 1;
     uint32_t traceBlock = 64;   // threads per block (64..256)
     uint32_t persistent = 0;    // 1: lanes refill with new rays from a work counter
     uint32_t refill = 16;       // idle lanes per wave that trigger a refill
@@ -28,7 +32,26 @@ struct Tuning {
     uint32_t wfShadeGrid = 16384;
     uint32_t wfPersistent = 0;
     uint32_t wfRefill = 16;
+    // batched rendering
+    uint32_t batchPriv = 0;
+    uint32_t batchTileW = 16;   // tile width in pixels; height = 256 / width
+    uint32_t batchGrid = 1024;  // persistent blocks
+    uint32_t batchChunks = 8;   // item ranges with their own counters (1..8)
+    uint32_t batchGraph = 1;    // 1: passes replayed from a captured hipGraph
 };
+
+constexpr uint32_t kBatchMaxChunks = 8;
+
+// Traversal stack modes of the tuning: 0 LDS, 1 private, 2 short LDS stack
+// with private overflow.  withStack calls f(std::integral_constant<int, mode>).
+template <class F>
+inline void withStack(uint32_t mode, F&& f) {
+    if (mode == 1) f(std::integral_constant<int, 1>{});
+    else if (mode == 2) f(std::integral_constant<int, 2>{});
+    else f(std::integral_constant<int, 0>{});
+}
+// LDS bytes per block of a stack mode.
+size_t stackLdsBytes(uint32_t mode, uint32_t stackCap, uint32_t block);
 
 // counter: one device uint32 used by the persistent variants (may be null).
 hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
@@ -40,6 +63,14 @@ hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
                             const IntegratorView& cfg, uint32_t si, float4* L,
                             float2* P, unsigned long long* counters,
                             bool countWork, const Tuning& t, hipStream_t st);
+// One sample index over the first nActive scenes of the table: persistent
+// megakernel over `items` tiles, then the film gather over `pixels` pixels.
+// next: kBatchMaxChunks device counters.
+hipError_t launchBatchPass(const BatchScene* tab, const uint32_t* tilePrefix,
+                           const uint32_t* pixPrefix, uint32_t nActive, uint32_t items,
+                           uint32_t pixels, const IntegratorView& cfg, uint32_t si,
+                           uint32_t stackCap, const Tuning& t, uint32_t* next,
+                           unsigned long long* counters, hipStream_t st);
 hipError_t launchFilmGather(const float4* L, const float2* P, uint32_t W,
                             uint32_t H, float* sum, float* wsum,
                             hipStream_t st);

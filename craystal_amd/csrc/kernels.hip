@@ -11,7 +11,7 @@ namespace crh {
 // ---------------------------------------------------------------------------
 // Ray batch queries
 // ---------------------------------------------------------------------------
-template <bool COUNT, bool PRIV>
+template <bool COUNT, int SM>
 __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(
     SceneView s, const crh_ray* __restrict__ rays, uint64_t n,
     crh_hit* __restrict__ hits, uint32_t* __restrict__ work) {
@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(
     Ray ray{mk3(a.x, a.y, a.z), a.w, mk3(b.x, b.y, b.z), b.w};
     Hit h;
     WorkCount wc{0, 0};
-    auto stk = makeStack<PRIV>(lds);
+    auto stk = makeStack<SM>(lds);
     traverse<false, COUNT>(s, ray, h, stk, wc);
     reinterpret_cast<uint4*>(hits)[i] =
         make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.u),
@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(
     }
 }
 
-template <bool PRIV>
+template <int SM>
 __global__ __launch_bounds__(kTraceBlock) void kTraceAny(
     SceneView s, const crh_ray* __restrict__ rays, uint64_t n,
     uint8_t* __restrict__ occ) {
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(
     Ray ray{mk3(a.x, a.y, a.z), a.w, mk3(b.x, b.y, b.z), b.w};
     Hit h;
     WorkCount wc{0, 0};
-    auto stk = makeStack<PRIV>(lds);
+    auto stk = makeStack<SM>(lds);
     occ[i] = traverse<true, false>(s, ray, h, stk, wc) ? 1 : 0;
 }
 
@@ -72,11 +72,11 @@ __global__ __launch_bounds__(64) void kResetCounter(uint32_t* c) {
     if (threadIdx.x == 0) *c = 0u;
 }
 
-template <bool ANY, bool PRIV>
+template <bool ANY, int SM>
 __global__ __launch_bounds__(kTraceBlock) void kTracePersistent(
     SceneView s, BatchSrc src, uint32_t n, uint32_t* __restrict__ counter, uint32_t refill) {
     extern __shared__ uint2 lds[];
-    auto stk = makeStack<PRIV>(lds);
+    auto stk = makeStack<SM>(lds);
     WorkCount wc{0, 0};
     persistentTrace<ANY, false>(s, src, counter, n, refill, stk, wc);
 }
@@ -141,68 +141,66 @@ CRH_D void tileCoord(uint32_t tilesX, uint32_t& x, uint32_t& y) {
     y = ty * 16u + (wave >> 1) * 8u + (lane >> 3);
 }
 
-// pathTraceKernel (Core/Integrator.cu:142-227): radiance of sample index
-// `si` for every pixel into L (float4 per pixel, w unused) and the jittered
-// film position into P.
-template <bool MIS, bool COUNT, bool PRIV>
-__global__ __launch_bounds__(kRenderBlock) void kRenderPass(
-    SceneView s, CameraView cam, IntegratorView cfg, uint32_t si,
-    uint32_t tilesX, float4* __restrict__ L, float2* __restrict__ P,
-    unsigned long long* __restrict__ counters) {
-    extern __shared__ uint2 lds[];
-    uint32_t x, y;
-    tileCoord(tilesX, x, y);
-    uint32_t rays = 0, nodes = 0, tris = 0;
-    if (x < cam.width && y < cam.height) {
-        auto stk = makeStack<PRIV>(lds);
-        PathCtx<COUNT, typename StackOf<PRIV>::T> c{s, stk, 0, {0, 0}};
-        Sampler smp(x, y, si);
-        F2 j = smp.next2D();
-        F2 pix{float(x) + j.x, float(y) + j.y};
-        Ray ray = cameraRay(cam, pix);
-        Spec rad = sp(0.f), beta = sp(1.f);
-        for (uint32_t depth = 0; depth < cfg.maxDepth; ++depth) {
-            bool terminate = false;
-            Hit h;
-            if (closest(c, ray, h)) {
-                Isect it = makeIsect(s, ray, h);
-                const GMaterial& m = s.mats[s.primMat[h.prim]];
-                if (m.flags & 1u) {
-                    rad = rad + sp3(m.emission) * beta;
-                    break;
-                }
-                BSDF bsdf = makeBSDF(m, it.frame);
-                if (MIS) {
-                    rad = rad + evalMIS(c, it, bsdf, smp) * beta;
-                } else {
-                    LightSample ls = sampleLight(c, it, bsdf, smp);
-                    if (ls.pdf > 0.0f)
-                        rad = rad + ls.weight * beta / ls.pdf / ls.selPdf;
-                }
-                F3 wi;
-                float pdf = 0.0f;
-                Spec f = bsdfSampleEval(bsdf, smp, it.view, wi, pdf);
-                if (pdf <= 0.0f) break;
-                beta = beta * (f / pdf);
-                ray.o = offsetRayOrigin(it.pos, oriented(it));
-                ray.d = wi;
-            } else {
-                rad = rad + sp3(s.env) * beta;
-                terminate = true;
+// pathTraceKernel (Core/Integrator.cu:142-227) for one pixel: radiance of
+// sample index `si` through pixel (x, y) into L[id] (float4, w unused) and
+// the jittered film position into P[id]; adds the rays traced to `rays`.
+template <bool MIS, bool COUNT, class Stack>
+CRH_D void tracePixel(const SceneView& s, Stack& stk, const CameraView& cam,
+                      const IntegratorView& cfg, uint32_t x, uint32_t y, uint32_t si,
+                      float4* __restrict__ L, float2* __restrict__ P, uint32_t& rays,
+                      WorkCount& wc) {
+    PathCtx<COUNT, Stack> c{s, stk, 0, {0, 0}};
+    Sampler smp(x, y, si);
+    F2 j = smp.next2D();
+    F2 pix{float(x) + j.x, float(y) + j.y};
+    Ray ray = cameraRay(cam, pix);
+    Spec rad = sp(0.f), beta = sp(1.f);
+    for (uint32_t depth = 0; depth < cfg.maxDepth; ++depth) {
+        bool terminate = false;
+        Hit h;
+        if (closest(c, ray, h)) {
+            Isect it = makeIsect(s, ray, h);
+            const GMaterial& m = s.mats[s.primMat[h.prim]];
+            if (m.flags & 1u) {
+                rad = rad + sp3(m.emission) * beta;
+                break;
             }
-            if (smp.next1D() < cfg.rrThreshold) terminate = true;
-            if (terminate) break;
-            if (!roulette(cfg, depth, smp, beta)) break;
+            BSDF bsdf = makeBSDF(m, it.frame);
+            if (MIS) {
+                rad = rad + evalMIS(c, it, bsdf, smp) * beta;
+            } else {
+                LightSample ls = sampleLight(c, it, bsdf, smp);
+                if (ls.pdf > 0.0f)
+                    rad = rad + ls.weight * beta / ls.pdf / ls.selPdf;
+            }
+            F3 wi;
+            float pdf = 0.0f;
+            Spec f = bsdfSampleEval(bsdf, smp, it.view, wi, pdf);
+            if (pdf <= 0.0f) break;
+            beta = beta * (f / pdf);
+            ray.o = offsetRayOrigin(it.pos, oriented(it));
+            ray.d = wi;
+        } else {
+            rad = rad + sp3(s.env) * beta;
+            terminate = true;
         }
-        size_t id = size_t(y) * cam.width + x;
-        L[id] = make_float4(rad.r, rad.g, rad.b, 0.f);
-        P[id] = make_float2(pix.x, pix.y);
-        rays = c.rays;
-        nodes = c.wc.nodes;
-        tris = c.wc.tris;
+        if (smp.next1D() < cfg.rrThreshold) terminate = true;
+        if (terminate) break;
+        if (!roulette(cfg, depth, smp, beta)) break;
     }
-    // Wave-level reductions, one atomic per wave and counter:
-    // counters = {rays, node visits, triangle tests}.
+    size_t id = size_t(y) * cam.width + x;
+    L[id] = make_float4(rad.r, rad.g, rad.b, 0.f);
+    P[id] = make_float2(pix.x, pix.y);
+    rays += c.rays;
+    wc.nodes += c.wc.nodes;
+    wc.tris += c.wc.tris;
+}
+
+// Wave-level reductions, one atomic per wave and counter:
+// counters = {rays, node visits, triangle tests}.
+template <bool COUNT>
+CRH_D void flushCounters(unsigned long long* counters, uint32_t rays, uint32_t nodes,
+                         uint32_t tris) {
     unsigned long long r64 = rays, n64 = nodes, t64 = tris;
     for (int off = 32; off > 0; off >>= 1) {
         r64 += __shfl_down(r64, off, 64);
@@ -220,16 +218,31 @@ __global__ __launch_bounds__(kRenderBlock) void kRenderPass(
     }
 }
 
+template <bool MIS, bool COUNT, int SM>
+__global__ __launch_bounds__(kRenderBlock) void kRenderPass(
+    SceneView s, CameraView cam, IntegratorView cfg, uint32_t si,
+    uint32_t tilesX, float4* __restrict__ L, float2* __restrict__ P,
+    unsigned long long* __restrict__ counters) {
+    extern __shared__ uint2 lds[];
+    uint32_t x, y;
+    tileCoord(tilesX, x, y);
+    uint32_t rays = 0;
+    WorkCount wc{0, 0};
+    if (x < cam.width && y < cam.height) {
+        auto stk = makeStack<SM>(lds);
+        tracePixel<MIS, COUNT>(s, stk, cam, cfg, x, y, si, L, P, rays, wc);
+    }
+    flushCounters<COUNT>(counters, rays, wc.nodes, wc.tris);
+}
+
 // Deterministic film update for one pass.  Each pixel gathers the tent-filter
 // contributions of the 3x3 neighbouring samples in row-major sample order,
 // which is the order a sequential SensorData::addSample loop over the pass
 // (Core/Sensor.cu:19-43) delivers them; samples with NaN/Inf are dropped and
 // zero weights skipped (they add exactly nothing).
-__global__ __launch_bounds__(256) void kFilmGather(
-    const float4* __restrict__ L, const float2* __restrict__ P, uint32_t W,
-    uint32_t H, float* __restrict__ sum, float* __restrict__ wsum) {
-    uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= W * H) return;
+CRH_D void gatherPixel(const float4* __restrict__ L, const float2* __restrict__ P,
+                       uint32_t W, uint32_t H, float* __restrict__ sum,
+                       float* __restrict__ wsum, uint32_t i) {
     int x = int(i % W), y = int(i / W);
     float s0 = sum[3 * size_t(i)], s1 = sum[3 * size_t(i) + 1],
           s2 = sum[3 * size_t(i) + 2], ws = wsum[i];
@@ -261,26 +274,109 @@ __global__ __launch_bounds__(256) void kFilmGather(
     wsum[i] = ws;
 }
 
+__global__ __launch_bounds__(256) void kFilmGather(
+    const float4* __restrict__ L, const float2* __restrict__ P, uint32_t W,
+    uint32_t H, float* __restrict__ sum, float* __restrict__ wsum) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= W * H) return;
+    gatherPixel(L, P, W, H, sum, wsum, i);
+}
+
+// ---------------------------------------------------------------------------
+// Batched rendering: many scenes, one persistent launch per sample index
+// ---------------------------------------------------------------------------
+// The table holds the active scenes first (sorted by decreasing spp, so the
+// scenes that still render at pass k are a prefix of it).  tilePrefix[j] is
+// the first work item (tile) of scene j; tilePrefix[nActive] = item count.
+
+// Index j with prefix[j] <= item < prefix[j + 1] (prefix[0] = 0).
+CRH_D uint32_t findScene(const uint32_t* __restrict__ prefix, uint32_t n, uint32_t item) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void kBatchReset(uint32_t* __restrict__ next) {
+    if (threadIdx.x < kBatchMaxChunks) next[threadIdx.x] = 0u;
+}
+
+// Persistent blocks take tiles from `chunks` contiguous ranges of the item
+// list, each with its own counter; block b starts on range b % chunks (the
+// hardware deals blocks to the 8 XCDs round-robin, so with 8 ranges each
+// XCD's L2 sees mostly its own scenes) and then moves on to the others.
+// Every block leaves after one failed take per range.
+template <bool MIS, int SM>
+__global__ __launch_bounds__(kRenderBlock) void kBatchRender(
+    const BatchScene* __restrict__ tab, const uint32_t* __restrict__ tilePrefix,
+    uint32_t nActive, IntegratorView cfg, uint32_t si, uint32_t tileW, uint32_t chunks,
+    uint32_t* __restrict__ next, unsigned long long* __restrict__ counters) {
+    extern __shared__ uint2 lds[];
+    __shared__ uint32_t taken;
+    const uint32_t T = tilePrefix[nActive];
+    const uint32_t tileH = kRenderBlock / tileW;
+    const uint32_t home = blockIdx.x % chunks;
+    uint32_t rays = 0;
+    WorkCount wc{0, 0};
+    auto stk = makeStack<SM>(lds);
+    for (uint32_t k = 0; k < chunks; ++k) {
+        const uint32_t c = (home + k) % chunks;
+        const uint32_t lo = uint32_t(uint64_t(T) * c / chunks);
+        const uint32_t hi = uint32_t(uint64_t(T) * (c + 1) / chunks);
+        for (;;) {
+            if (threadIdx.x == 0) taken = lo + atomicAdd(&next[c], 1u);
+            __syncthreads();
+            const uint32_t item = __builtin_amdgcn_readfirstlane(taken);
+            __syncthreads();
+            if (item >= hi) break;
+            const uint32_t j = __builtin_amdgcn_readfirstlane(findScene(tilePrefix, nActive, item));
+            const BatchScene& b = tab[j];
+            const uint32_t tile = item - tilePrefix[j];
+            const uint32_t x = (tile % b.tilesX) * tileW + threadIdx.x % tileW;
+            const uint32_t y = (tile / b.tilesX) * tileH + threadIdx.x / tileW;
+            if (x < b.cam.width && y < b.cam.height)
+                tracePixel<MIS, false>(b.view, stk, b.cam, cfg, x, y, si,
+                                       reinterpret_cast<float4*>(b.L),
+                                       reinterpret_cast<float2*>(b.P), rays, wc);
+        }
+    }
+    flushCounters<false>(counters, rays, 0, 0);
+}
+
+// Film update of every active scene: pixPrefix[j] is scene j's first pixel.
+__global__ __launch_bounds__(256) void kBatchGather(const BatchScene* __restrict__ tab,
+                                                    const uint32_t* __restrict__ pixPrefix,
+                                                    uint32_t nActive) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= pixPrefix[nActive]) return;
+    const uint32_t j = findScene(pixPrefix, nActive, i);
+    const BatchScene& b = tab[j];
+    gatherPixel(reinterpret_cast<const float4*>(b.L), reinterpret_cast<const float2*>(b.P),
+                b.cam.width, b.cam.height, b.sum, b.wsum, i - pixPrefix[j]);
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-static size_t stackLds(const SceneView& s, uint32_t priv, uint32_t block) {
-    return priv ? 0 : size_t(s.stackCap) * block * sizeof(uint2);
+size_t stackLdsBytes(uint32_t mode, uint32_t stackCap, uint32_t block) {
+    const uint32_t entries = mode == 1 ? 0u : (mode == 2 ? uint32_t(kShortStackLds) : stackCap);
+    return size_t(entries) * block * sizeof(uint2);
 }
 
 template <bool ANY>
 static hipError_t launchPersistent(const SceneView& s, const BatchSrc& src, uint32_t n,
                                    uint32_t* counter, const Tuning& t, hipStream_t st) {
     const uint32_t B = t.traceBlock;
-    size_t lds = stackLds(s, t.privStack, B);
+    const size_t lds = stackLdsBytes(t.privStack, s.stackCap, B);
     hipLaunchKernelGGL(kResetCounter, dim3(1), dim3(64), 0, st, counter);
-    dim3 grid(t.ptGrid), block(B);
-    if (t.privStack)
-        hipLaunchKernelGGL((kTracePersistent<ANY, true>), grid, block, lds, st, s, src, n, counter,
-                           t.refill);
-    else
-        hipLaunchKernelGGL((kTracePersistent<ANY, false>), grid, block, lds, st, s, src, n,
-                           counter, t.refill);
+    const dim3 grid(t.ptGrid), block(B);
+    withStack(t.privStack, [&](auto sm) {
+        hipLaunchKernelGGL((kTracePersistent<ANY, decltype(sm)::value>), grid, block, lds, st, s,
+                           src, n, counter, t.refill);
+    });
     return hipGetLastError();
 }
 
@@ -292,16 +388,17 @@ hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
         return launchPersistent<false>(s, BatchSrc{rays, hits, nullptr}, uint32_t(n), counter, t,
                                        st);
     const uint32_t B = t.traceBlock;
-    size_t lds = stackLds(s, t.privStack, B);
-    dim3 grid(uint32_t((n + B - 1) / B)), block(B);
-    if (work && t.privStack)
-        hipLaunchKernelGGL((kTraceClosest<true, true>), grid, block, lds, st, s, rays, n, hits, work);
-    else if (work)
-        hipLaunchKernelGGL((kTraceClosest<true, false>), grid, block, lds, st, s, rays, n, hits, work);
-    else if (t.privStack)
-        hipLaunchKernelGGL((kTraceClosest<false, true>), grid, block, lds, st, s, rays, n, hits, work);
-    else
-        hipLaunchKernelGGL((kTraceClosest<false, false>), grid, block, lds, st, s, rays, n, hits, work);
+    const size_t lds = stackLdsBytes(t.privStack, s.stackCap, B);
+    const dim3 grid(uint32_t((n + B - 1) / B)), block(B);
+    withStack(t.privStack, [&](auto sm) {
+        constexpr int SM = decltype(sm)::value;
+        if (work)
+            hipLaunchKernelGGL((kTraceClosest<true, SM>), grid, block, lds, st, s, rays, n, hits,
+                               work);
+        else
+            hipLaunchKernelGGL((kTraceClosest<false, SM>), grid, block, lds, st, s, rays, n, hits,
+                               work);
+    });
     return hipGetLastError();
 }
 
@@ -311,42 +408,61 @@ hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
     if (t.persistent && counter && n < (uint64_t(1) << 32))
         return launchPersistent<true>(s, BatchSrc{rays, nullptr, occ}, uint32_t(n), counter, t, st);
     const uint32_t B = t.traceBlock;
-    size_t lds = stackLds(s, t.privStack, B);
-    dim3 grid(uint32_t((n + B - 1) / B)), block(B);
-    if (t.privStack)
-        hipLaunchKernelGGL((kTraceAny<true>), grid, block, lds, st, s, rays, n, occ);
-    else
-        hipLaunchKernelGGL((kTraceAny<false>), grid, block, lds, st, s, rays, n, occ);
+    const size_t lds = stackLdsBytes(t.privStack, s.stackCap, B);
+    const dim3 grid(uint32_t((n + B - 1) / B)), block(B);
+    withStack(t.privStack, [&](auto sm) {
+        hipLaunchKernelGGL((kTraceAny<decltype(sm)::value>), grid, block, lds, st, s, rays, n,
+                           occ);
+    });
     return hipGetLastError();
-}
-
-template <bool MIS, bool COUNT, bool PRIV>
-static void renderLaunch(dim3 grid, size_t lds, hipStream_t st, const SceneView& s,
-                         const CameraView& cam, const IntegratorView& cfg, uint32_t si,
-                         uint32_t tilesX, float4* L, float2* P,
-                         unsigned long long* counters) {
-    hipLaunchKernelGGL((kRenderPass<MIS, COUNT, PRIV>), grid, dim3(kRenderBlock), lds, st, s,
-                       cam, cfg, si, tilesX, L, P, counters);
 }
 
 hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
                             const IntegratorView& cfg, uint32_t si, float4* L,
                             float2* P, unsigned long long* counters,
                             bool countWork, const Tuning& t, hipStream_t st) {
-    uint32_t tilesX = (cam.width + 15u) / 16u, tilesY = (cam.height + 15u) / 16u;
-    size_t lds = stackLds(s, t.renderPriv, kRenderBlock);
-    dim3 grid(tilesX * tilesY);
-    const int sel = (cfg.useMIS ? 4 : 0) | (countWork ? 2 : 0) | (t.renderPriv ? 1 : 0);
-    switch (sel) {
-        case 0: renderLaunch<false, false, false>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
-        case 1: renderLaunch<false, false, true>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
-        case 2: renderLaunch<false, true, false>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
-        case 3: renderLaunch<false, true, true>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
-        case 4: renderLaunch<true, false, false>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
-        case 5: renderLaunch<true, false, true>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
-        case 6: renderLaunch<true, true, false>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
-        default: renderLaunch<true, true, true>(grid, lds, st, s, cam, cfg, si, tilesX, L, P, counters); break;
-    }
+    const uint32_t tilesX = (cam.width + 15u) / 16u, tilesY = (cam.height + 15u) / 16u;
+    const size_t lds = stackLdsBytes(t.renderPriv, s.stackCap, kRenderBlock);
+    const dim3 grid(tilesX * tilesY), block(kRenderBlock);
+    withStack(t.renderPriv, [&](auto sm) {
+        constexpr int SM = decltype(sm)::value;
+        if (cfg.useMIS && countWork)
+            hipLaunchKernelGGL((kRenderPass<true, true, SM>), grid, block, lds, st, s, cam, cfg,
+                               si, tilesX, L, P, counters);
+        else if (cfg.useMIS)
+            hipLaunchKernelGGL((kRenderPass<true, false, SM>), grid, block, lds, st, s, cam, cfg,
+                               si, tilesX, L, P, counters);
+        else if (countWork)
+            hipLaunchKernelGGL((kRenderPass<false, true, SM>), grid, block, lds, st, s, cam, cfg,
+                               si, tilesX, L, P, counters);
+        else
+            hipLaunchKernelGGL((kRenderPass<false, false, SM>), grid, block, lds, st, s, cam, cfg,
+                               si, tilesX, L, P, counters);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launchBatchPass(const BatchScene* tab, const uint32_t* tilePrefix,
+                           const uint32_t* pixPrefix, uint32_t nActive, uint32_t items,
+                           uint32_t pixels, const IntegratorView& cfg, uint32_t si,
+                           uint32_t stackCap, const Tuning& t, uint32_t* next,
+                           unsigned long long* counters, hipStream_t st) {
+    if (nActive == 0 || items == 0) return hipSuccess;
+    hipLaunchKernelGGL(kBatchReset, dim3(1), dim3(64), 0, st, next);
+    const size_t lds = stackLdsBytes(t.batchPriv, stackCap, kRenderBlock);
+    const dim3 grid(items < t.batchGrid ? items : t.batchGrid), block(kRenderBlock);
+    const uint32_t tw = t.batchTileW, ch = t.batchChunks;
+    withStack(t.batchPriv, [&](auto sm) {
+        constexpr int SM = decltype(sm)::value;
+        if (cfg.useMIS)
+            hipLaunchKernelGGL((kBatchRender<true, SM>), grid, block, lds, st, tab, tilePrefix,
+                               nActive, cfg, si, tw, ch, next, counters);
+        else
+            hipLaunchKernelGGL((kBatchRender<false, SM>), grid, block, lds, st, tab, tilePrefix,
+                               nActive, cfg, si, tw, ch, next, counters);
+    });
+    hipLaunchKernelGGL(kBatchGather, dim3((pixels + 255u) / 256u), dim3(256), 0, st, tab,
+                       pixPrefix, nActive);
     return hipGetLastError();
 }
 

@@ -95,4 +95,17 @@ struct IntegratorView {
     uint32_t useMIS;
 };
 
+// One scene of a batched render: its view into the pooled scene data, its
+// camera, the per-pass sample buffers and the film of its sensor.
+struct BatchScene {
+    SceneView view;
+    CameraView cam;
+    float* L;  // float4 per pixel
+    float* P;  // float2 per pixel
+    float* sum;
+    float* wsum;
+    uint32_t tilesX;
+    uint32_t pad[3];
+};
+
 }  // namespace crh

@@ -14,7 +14,8 @@
 // and every comparison are identical.  With that, the stack never holds more
 // entries than the tree has levels.  It lives in LDS, entry k of a thread at
 // stack[k * stride] with stride = threads per block so a wave's lanes hit
-// consecutive 8-byte slots (LdsStack), or in private memory (PrivStack).
+// consecutive 8-byte slots (LdsStack), in private memory (PrivStack), or the
+// first kShortStack levels in LDS and the rest in private memory (ShortStack).
 #pragma once
 #include "device_math.h"
 #include "traverse_limits.h"
@@ -46,6 +47,23 @@ struct PrivStack {
     uint2 a[kPrivStack];
     CRH_D void put(int k, uint2 v) { a[k] = v; }
     CRH_D uint2 get(int k) const { return a[k]; }
+};
+
+// Short stack: levels below kShortStack in LDS (laid out as LdsStack), deeper
+// levels in private memory.  Typical
+// traversals stay within the LDS part, so the private part (and its scratch
+// traffic) is touched only by deep paths; the LDS part is small enough not to
+// limit occupancy.
+constexpr int kShortStack = kShortStackLds;
+struct ShortStack {
+    uint2* base;
+    uint32_t stride;
+    uint2 a[kPrivStack - kShortStack];
+    CRH_D void put(int k, uint2 v) {
+        if (k < kShortStack) base[k * stride] = v;
+        else a[k - kShortStack] = v;
+    }
+    CRH_D uint2 get(int k) const { return k < kShortStack ? base[k * stride] : a[k - kShortStack]; }
 };
 
 CRH_D void loadNode(const GNode* nodes, uint32_t i, float4& a, float4& b,
@@ -321,22 +339,32 @@ CRH_D bool closestHit(const SceneView& s, const Ray& ray, Hit& hit,
     return h && hit.prim != 0xFFFFFFFFu;
 }
 
-// Stack selection: PRIV = private (scratch) stack, else LDS stack.
-template <bool PRIV>
+// Stack selection: SM = 0 LDS stack, 1 private (scratch) stack, 2 short LDS
+// stack with private overflow.
+template <int SM>
 struct StackOf;
 template <>
-struct StackOf<false> {
+struct StackOf<0> {
     using T = LdsStack;
 };
 template <>
-struct StackOf<true> {
+struct StackOf<1> {
     using T = PrivStack;
 };
+template <>
+struct StackOf<2> {
+    using T = ShortStack;
+};
 
-template <bool PRIV>
-CRH_D typename StackOf<PRIV>::T makeStack(uint2* lds) {
-    if constexpr (PRIV) {
+template <int SM>
+CRH_D typename StackOf<SM>::T makeStack(uint2* lds) {
+    if constexpr (SM == 1) {
         return PrivStack{};
+    } else if constexpr (SM == 2) {
+        ShortStack s;
+        s.base = lds + threadIdx.x;
+        s.stride = blockDim.x;
+        return s;
     } else {
         return LdsStack{lds + threadIdx.x, blockDim.x};
     }

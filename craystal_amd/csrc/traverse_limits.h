@@ -6,5 +6,8 @@ namespace crh {
 // Levels held by the private (scratch) traversal stack; trees that need more
 // stack entries fall back to the LDS stack.
 constexpr int kPrivStackLevels = 32;
+// Levels of the short stack kept in LDS (the rest of its kPrivStackLevels in
+// private memory).
+constexpr int kShortStackLds = 8;
 
 }  // namespace crh

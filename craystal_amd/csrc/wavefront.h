@@ -66,7 +66,7 @@ struct WfLaunch {
     uint32_t traceGrid = 2048;   // blocks for the traversal kernels
     uint32_t traceBlock = 256;   // threads per traversal block (64..256)
     uint32_t shadeGrid = 2048;   // blocks of 256 for the shading kernels
-    uint32_t privStack = 0;      // 1: traversal stack in private memory
+    uint32_t privStack = 0;      // stack mode: 0 LDS, 1 private, 2 short LDS + private
     uint32_t persistent = 0;     // 1: traversal lanes refill with new rays
     uint32_t refill = 16;
 };
@@ -81,6 +81,6 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                          const IntegratorView& cfg, uint32_t si, WfState& w,
                          const WfLaunch& L, bool countWork, hipStream_t st,
                          const GuideView* guide = nullptr, float alpha = 0.5f,
-                         GuideSample* samples = nullptr);
+                         GuideSample* samples = nullptr, uint32_t sampleCap = 0);
 
 }  // namespace crh

@@ -30,6 +30,7 @@
 #include "guiding_kernels.h"
 #include "integrator.h"
 #include "wavefront.h"
+#include "kernels.h"
 
 namespace crh {
 
@@ -121,11 +122,11 @@ __global__ __launch_bounds__(256) void kWfGenerate(WfState w, CameraView cam,
 }
 
 // Closest hit for the extension queue followed by the MIS queue.
-template <bool COUNT, bool PRIV>
+template <bool COUNT, int SM>
 __global__ __launch_bounds__(256) void kWfExtend(SceneView s, WfState w,
                                                  uint32_t cur) {
     extern __shared__ uint2 lds[];
-    auto stk = makeStack<PRIV>(lds);
+    auto stk = makeStack<SM>(lds);
     const uint32_t nExt = w.counters[kCExt0 + cur];
     const uint32_t nMis = w.counters[kCMis0 + cur];
     if (blockIdx.x == 0 && threadIdx.x == 0 && nExt + nMis)
@@ -182,11 +183,11 @@ struct ExtendSrc {
     }
 };
 
-template <bool PRIV>
+template <int SM>
 __global__ __launch_bounds__(256) void kWfExtendPT(SceneView s, WfState w, uint32_t cur,
                                                    uint32_t refill) {
     extern __shared__ uint2 lds[];
-    auto stk = makeStack<PRIV>(lds);
+    auto stk = makeStack<SM>(lds);
     const uint32_t nExt = w.counters[kCExt0 + cur];
     const uint32_t nMis = w.counters[kCMis0 + cur];
     if (blockIdx.x == 0 && threadIdx.x == 0 && nExt + nMis)
@@ -204,10 +205,10 @@ struct ShadowSrc {
     }
 };
 
-template <bool PRIV>
+template <int SM>
 __global__ __launch_bounds__(256) void kWfShadowPT(SceneView s, WfState w, uint32_t refill) {
     extern __shared__ uint2 lds[];
-    auto stk = makeStack<PRIV>(lds);
+    auto stk = makeStack<SM>(lds);
     const uint32_t n = w.counters[kCShadow];
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(w.work, (unsigned long long)n);
     ShadowSrc src{w};
@@ -298,10 +299,10 @@ __global__ __launch_bounds__(256) void kWfShadeHit(SceneView s, WfState w,
     }
 }
 
-template <bool PRIV>
+template <int SM>
 __global__ __launch_bounds__(256) void kWfShadow(SceneView s, WfState w) {
     extern __shared__ uint2 lds[];
-    auto stk = makeStack<PRIV>(lds);
+    auto stk = makeStack<SM>(lds);
     const uint32_t n = w.counters[kCShadow];
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(w.work, (unsigned long long)n);
     WorkCount wc{0, 0};
@@ -451,8 +452,10 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
 // guided direction = (final radiance - radiance before that direction's
 // contributions) / throughput; target = luminance of (BSDF * cos) x that,
 // weight = target / sampling pdf.  Records whose path ended before the next
-// vertex (no snapshot) or with zero target are skipped.
-__global__ __launch_bounds__(256) void kWfRecordsToSamples(WfState w, GuideSample* out) {
+// vertex (no snapshot) or with zero target are skipped; at most cap samples
+// are written (the counter still counts them all).
+__global__ __launch_bounds__(256) void kWfRecordsToSamples(WfState w, GuideSample* out,
+                                                           uint32_t cap) {
     const uint32_t n = min(w.counters[kCRec], w.recCap);
     PERSISTENT_LOOP(r, n) {
         const float4* R = w.rec + size_t(r) * kRecF4;
@@ -470,7 +473,7 @@ __global__ __launch_bounds__(256) void kWfRecordsToSamples(WfState w, GuideSampl
             ok = weight > 0.f && isfinite(weight);
         }
         uint32_t slot = wavePush(&w.counters[kCTrain], ok);
-        if (ok) {
+        if (ok && slot < cap) {
             GuideSample& g = out[slot];
             float4 a = R[0], b = R[1], c = R[2], d = R[3];
             g.pos[0] = a.x; g.pos[1] = a.y; g.pos[2] = a.z; g.pos[3] = 0.f;
@@ -484,53 +487,48 @@ __global__ __launch_bounds__(256) void kWfRecordsToSamples(WfState w, GuideSampl
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-template <bool PRIV>
+template <int SM>
 static void launchExtend(bool count, dim3 g, dim3 b, size_t lds, hipStream_t st,
                          const SceneView& s, const WfState& w, uint32_t cur) {
     if (count)
-        hipLaunchKernelGGL((kWfExtend<true, PRIV>), g, b, lds, st, s, w, cur);
+        hipLaunchKernelGGL((kWfExtend<true, SM>), g, b, lds, st, s, w, cur);
     else
-        hipLaunchKernelGGL((kWfExtend<false, PRIV>), g, b, lds, st, s, w, cur);
+        hipLaunchKernelGGL((kWfExtend<false, SM>), g, b, lds, st, s, w, cur);
 }
 
 hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                          const IntegratorView& cfg, uint32_t si, WfState& w,
                          const WfLaunch& L, bool countWork, hipStream_t st,
-                         const GuideView* guide, float alpha, GuideSample* samples) {
+                         const GuideView* guide, float alpha, GuideSample* samples,
+                         uint32_t sampleCap) {
     const uint32_t TB = L.traceBlock;
-    const size_t lds = L.privStack ? 0 : size_t(s.stackCap) * TB * sizeof(uint2);
+    const size_t lds = stackLdsBytes(L.privStack, s.stackCap, TB);
     const dim3 b256(256), bT(TB);
     const bool guided = guide != nullptr;
     const bool pt = L.persistent != 0;
     auto extend = [&](uint32_t cur, bool count) {
-        if (pt && !count) {
+        if (pt && !count)
             hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters, 1u << kCPtExt);
-            if (L.privStack)
-                hipLaunchKernelGGL((kWfExtendPT<true>), dim3(L.traceGrid), bT, lds, st, s, w, cur,
+        withStack(L.privStack, [&](auto sm) {
+            constexpr int SM = decltype(sm)::value;
+            if (pt && !count)
+                hipLaunchKernelGGL((kWfExtendPT<SM>), dim3(L.traceGrid), bT, lds, st, s, w, cur,
                                    L.refill);
             else
-                hipLaunchKernelGGL((kWfExtendPT<false>), dim3(L.traceGrid), bT, lds, st, s, w, cur,
-                                   L.refill);
-        } else if (L.privStack) {
-            launchExtend<true>(count, dim3(L.traceGrid), bT, lds, st, s, w, cur);
-        } else {
-            launchExtend<false>(count, dim3(L.traceGrid), bT, lds, st, s, w, cur);
-        }
+                launchExtend<SM>(count, dim3(L.traceGrid), bT, lds, st, s, w, cur);
+        });
     };
     auto shadow = [&]() {
-        if (pt) {
+        if (pt)
             hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters, 1u << kCPtShadow);
-            if (L.privStack)
-                hipLaunchKernelGGL((kWfShadowPT<true>), dim3(L.traceGrid), bT, lds, st, s, w,
+        withStack(L.privStack, [&](auto sm) {
+            constexpr int SM = decltype(sm)::value;
+            if (pt)
+                hipLaunchKernelGGL((kWfShadowPT<SM>), dim3(L.traceGrid), bT, lds, st, s, w,
                                    L.refill);
             else
-                hipLaunchKernelGGL((kWfShadowPT<false>), dim3(L.traceGrid), bT, lds, st, s, w,
-                                   L.refill);
-        } else if (L.privStack) {
-            hipLaunchKernelGGL((kWfShadow<true>), dim3(L.traceGrid), bT, lds, st, s, w);
-        } else {
-            hipLaunchKernelGGL((kWfShadow<false>), dim3(L.traceGrid), bT, lds, st, s, w);
-        }
+                hipLaunchKernelGGL((kWfShadow<SM>), dim3(L.traceGrid), bT, lds, st, s, w);
+        });
     };
     hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
                        (1u << kCExt1) | (1u << kCMis0) | (1u << kCMis1) |
@@ -575,7 +573,8 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
     extend(cur, false);
     hipLaunchKernelGGL(kWfMisResolve, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
     if (guided && samples)
-        hipLaunchKernelGGL(kWfRecordsToSamples, dim3(L.shadeGrid), b256, 0, st, w, samples);
+        hipLaunchKernelGGL(kWfRecordsToSamples, dim3(L.shadeGrid), b256, 0, st, w, samples,
+                           sampleCap);
     return hipGetLastError();
 }
 

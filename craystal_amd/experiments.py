@@ -45,7 +45,8 @@ def reference_image(scene: Scene, camd: dict, size, cfg: Configs, spp: int,
 def _progressive(render_chunk, sensor: Sensor, ref: np.ndarray, budget_s: float, target: float,
                  max_spp: int):
     """Runs render_chunk(first_sample) -> (spp rendered, seconds) until the
-    relMSE target or the time budget; returns the curve [(t, spp, relmse)]."""
+    relMSE target (0 = never) or the time budget; returns the curve
+    [(t, spp, relmse)]."""
     t, spp, curve = 0.0, 0, []
     while t < budget_s and spp < max_spp:
         k, dt = render_chunk(spp)
@@ -54,7 +55,7 @@ def _progressive(render_chunk, sensor: Sensor, ref: np.ndarray, budget_s: float,
         sensor.readbackDeviceData()
         e = relmse(sensor.createImage(), ref)
         curve.append((t, spp, e))
-        if e <= target:
+        if target > 0 and e <= target:
             break
     return curve
 
@@ -66,12 +67,27 @@ def time_to_target(curve, target):
     return None, None
 
 
+def extrapolate_time(curve, target, tail: float = 0.5):
+    """Time at which relMSE reaches target under relMSE(t) = a / t, with a
+    fitted (least squares in a) on the last `tail` fraction of the curve."""
+    pts = [(t, e) for t, _, e in curve[int(len(curve) * (1 - tail)):] if t > 0 and e > 0]
+    if not pts:
+        return None
+    inv = np.array([1.0 / t for t, _ in pts])
+    err = np.array([e for _, e in pts])
+    a = float(inv @ err / (inv @ inv))
+    return a / target
+
+
 def guided_vs_plain(data, camd, size, cfg: Configs, ref: np.ndarray, target: float = 0.01,
                     budget_s: float = 60.0, chunk: int = 8, train_fraction: float = 0.25,
-                    train_batch: int = 65536, train_steps: int = 4, lr: float = 1e-2,
-                    plain_budget_s: float | None = None, max_spp: int = 1 << 16) -> dict:
+                    train_batch: int = 16384, train_steps: int = 4, lr: float = 1e-2,
+                    alpha: float = 0.5, max_spp: int = 1 << 20, seed: int = 7) -> dict:
+    """Plain megakernel and guided wavefront, each for budget_s seconds of
+    render time (the guided time includes its training steps, which run
+    during the first train_fraction of its budget); relMSE against ref after
+    every chunk of passes."""
     scene = Scene(data)
-    # plain megakernel
     sensor = Sensor(size)
     cam = scenes.make_camera(camd, sensor)
     scene.setCamera(cam)
@@ -83,52 +99,53 @@ def guided_vs_plain(data, camd, size, cfg: Configs, ref: np.ndarray, target: flo
         st = plain.dispatch(scene, chunk, first_sample=first, readback=False)
         return chunk, st["kernel_ms"] / 1e3
 
-    pcurve = _progressive(plain_chunk, sensor, ref, plain_budget_s or budget_s, target, max_spp)
-    t_plain, spp_plain = time_to_target(pcurve, target)
+    pcurve = _progressive(plain_chunk, sensor, ref, budget_s, 0.0, max_spp)
 
-    # guided wavefront
     P = data.concatenated()[0]
-    guide = Guide(P.min(0), P.max(0), seed=7)
+    guide = Guide(P.min(0), P.max(0), seed=seed)
     gcfg = Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
-    gi = GuidedIntegrator(gcfg, guide, alpha=0.5, lr=lr, train_batch=train_batch,
+    gi = GuidedIntegrator(gcfg, guide, alpha=alpha, lr=lr, train_batch=train_batch,
                           train_steps=train_steps)
     sensor.clear()
     gi.dispatch(scene, 1, first_sample=1 << 22, readback=False)       # warm up (no training)
     sensor.clear()
-    budget = (t_plain if t_plain else (plain_budget_s or budget_s))
-    train_until = train_fraction * budget
-    state = {"t": 0.0}
+    train_until = train_fraction * budget_s
+    state = {"t": 0.0, "train_s": 0.0}
 
     def guided_chunk(first):
         train = state["t"] < train_until
-        t0 = time.perf_counter()
         st = gi.dispatch(scene, chunk, first_sample=first, train=train, readback=False)
-        dt = max(st["kernel_ms"] / 1e3, 0.0) if not train else time.perf_counter() - t0
+        dt = st["kernel_ms"] / 1e3
         state["t"] += dt
+        if train:
+            state["train_s"] += dt
         return chunk, dt
 
-    gcurve = _progressive(guided_chunk, sensor, ref, budget_s, target, max_spp)
+    gcurve = _progressive(guided_chunk, sensor, ref, budget_s, 0.0, max_spp)
+    t_plain, spp_plain = time_to_target(pcurve, target)
     t_guided, spp_guided = time_to_target(gcurve, target)
+    x_plain, x_guided = extrapolate_time(pcurve, target), extrapolate_time(gcurve, target)
+    px = size[0] * size[1]
     out = {
-        "target_relmse": target,
-        "plain": {"seconds": t_plain, "spp": spp_plain,
-                  "final_relmse": pcurve[-1][2] if pcurve else None,
-                  "msamples_per_s": (pcurve[-1][1] * size[0] * size[1] / pcurve[-1][0] / 1e6)
-                  if pcurve else None},
-        "guided": {"seconds": t_guided, "spp": spp_guided,
-                   "final_relmse": gcurve[-1][2] if gcurve else None,
-                   "train_seconds": round(train_until, 3),
-                   "msamples_per_s": (gcurve[-1][1] * size[0] * size[1] / gcurve[-1][0] / 1e6)
-                   if gcurve else None},
-        "curve_plain": [(round(t, 4), s, round(e, 5)) for t, s, e in pcurve[::max(1, len(pcurve) // 20)]],
-        "curve_guided": [(round(t, 4), s, round(e, 5)) for t, s, e in gcurve[::max(1, len(gcurve) // 20)]],
+        "target_relmse": target, "budget_s": budget_s, "pixels": px,
+        "plain": {"seconds_to_target": t_plain, "spp_to_target": spp_plain,
+                  "extrapolated_seconds": x_plain, "final_relmse": pcurve[-1][2],
+                  "final_spp": pcurve[-1][1],
+                  "msamples_per_s": pcurve[-1][1] * px / pcurve[-1][0] / 1e6},
+        "guided": {"seconds_to_target": t_guided, "spp_to_target": spp_guided,
+                   "extrapolated_seconds": x_guided, "final_relmse": gcurve[-1][2],
+                   "final_spp": gcurve[-1][1], "train_seconds": round(state["train_s"], 3),
+                   "train_steps": guide.info()["steps"],
+                   "msamples_per_s": gcurve[-1][1] * px / gcurve[-1][0] / 1e6},
+        "curve_plain": [(round(t, 4), k, round(e, 5)) for t, k, e in pcurve[::max(1, len(pcurve) // 24)]],
+        "curve_guided": [(round(t, 4), k, round(e, 5)) for t, k, e in gcurve[::max(1, len(gcurve) // 24)]],
     }
+    tg = gcurve[-1][0]
+    pe = [e for t, k, e in pcurve if t <= tg]
+    out["equal_time"] = {"seconds": round(tg, 4), "plain_relmse": pe[-1] if pe else None,
+                         "guided_relmse": gcurve[-1][2]}
     if t_plain and t_guided:
-        out["speedup"] = round(t_plain / t_guided, 3)
-    # relMSE at equal time (the guided run's end vs plain at the same time)
-    if pcurve and gcurve:
-        tg = gcurve[-1][0]
-        pe = [e for t, s, e in pcurve if t <= tg]
-        out["equal_time"] = {"seconds": round(tg, 4), "plain_relmse": pe[-1] if pe else None,
-                             "guided_relmse": gcurve[-1][2]}
+        out["speedup_measured"] = t_plain / t_guided
+    if x_plain and x_guided:
+        out["speedup_extrapolated"] = x_plain / x_guided
     return out

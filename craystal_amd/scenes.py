@@ -138,6 +138,39 @@ def icosphere(center, radius, subdiv, material) -> MeshData:
     return MeshData(P, V.astype(np.float32), F, material)
 
 
+def heightfield(cells: int, extent: float, amplitude: float, seed: int, material,
+                octaves: int = 4) -> MeshData:
+    """A (cells x cells)-quad height field over [-extent/2, extent/2]^2 in xz
+    (2 cells^2 triangles): a sum of seeded sinusoid octaves, vertex normals
+    from central differences."""
+    rng = Stream(seed)
+    n = cells + 1
+    t = np.linspace(-extent / 2, extent / 2, n)
+    X, Z = np.meshgrid(t, t, indexing="xy")
+    Y = np.zeros_like(X)
+    dX, dZ = np.zeros_like(X), np.zeros_like(X)
+    for o in range(octaves):
+        f = (2.0 ** o) * 2.0 * math.pi / extent * rng.uniform(0.8, 1.6)
+        a = amplitude / (2.0 ** o)
+        ph1, ph2, ang = rng.uniform(0, 6.283), rng.uniform(0, 6.283), rng.uniform(0, 3.1416)
+        u = math.cos(ang) * X + math.sin(ang) * Z
+        v = -math.sin(ang) * X + math.cos(ang) * Z
+        Y += a * np.sin(f * u + ph1) * np.cos(f * v + ph2)
+        du = a * f * np.cos(f * u + ph1) * np.cos(f * v + ph2)
+        dv = -a * f * np.sin(f * u + ph1) * np.sin(f * v + ph2)
+        dX += du * math.cos(ang) - dv * math.sin(ang)
+        dZ += du * math.sin(ang) + dv * math.cos(ang)
+    P = np.stack([X, Y, Z], -1).reshape(-1, 3).astype(np.float32)
+    N = np.stack([-dX, np.ones_like(X), -dZ], -1).reshape(-1, 3)
+    N = (N / np.linalg.norm(N, axis=1, keepdims=True)).astype(np.float32)
+    i, j = np.meshgrid(np.arange(cells), np.arange(cells), indexing="xy")
+    a = (j * n + i).reshape(-1)
+    b, c, d = a + 1, a + n + 1, a + n
+    # counter-clockwise seen from +y: (a, d, c), (a, c, b)
+    I = np.stack([a, d, c, a, c, b], -1).reshape(-1).astype(np.uint32)
+    return MeshData(P, N, I, material)
+
+
 def principled(base, roughness, metallic) -> MaterialData:
     """A Principled material expressed in the reference's (kd, ks, shininess)
     parameters: roughness = sqrt(2/(n+1)) and metallic = avg(ks)
@@ -282,6 +315,57 @@ def occluded_room(n_objects: int = 200, subdiv: int = 3, seed: int = SCENE_SEED,
     data.emissiveIndex = _emissive_prims(data)
     cam = dict(pos=(-4.2, 1.7, -4.2), target=(2.0, 0.7, 2.0), up=(0.0, 1.0, 0.0), fov=60.0,
                size=(1024, 1024), spp=None, max_depth=16, rr_start_depth=3)
+    return data, cam
+
+
+def batch_scene(index: int, seed: int = SCENE_SEED, tri_range=(1_000, 500_000),
+                resolutions=(128, 256, 512, 1024), spps=(4, 16, 64), max_depth: int = 12):
+    """One scene of the many-scene batch: a triangle budget drawn
+    log-uniformly from tri_range, spent on a height-field ground (about 40%),
+    icospheres and boxes; each object gets a Phong (exponent U[1, 1000]) or a
+    Principled (roughness U[0.05, 1], metallic Bernoulli(0.3)) material; a
+    1.5 x 1.5 m area light and a constant environment.  Square resolution
+    and spp drawn from the given sets.  Returns (SceneData, camera kwargs)."""
+    rng = Stream(seed * 7919 + 1000 + index)
+    lo, hi = math.log(tri_range[0]), math.log(tri_range[1])
+    budget = int(math.exp(rng.uniform(lo, hi)))
+    mats = [diffuse((0.6, 0.6, 0.6)), emitter((30.0, 28.0, 24.0))]
+
+    def material():
+        if rng.uniform() < 0.5:
+            kd = (rng.uniform(0.1, 0.8), rng.uniform(0.1, 0.8), rng.uniform(0.1, 0.8))
+            ks = rng.uniform(0.05, 0.4)
+            mats.append(phong(kd, (ks, ks, ks), rng.uniform(1.0, 1000.0)))
+        else:
+            base = (rng.uniform(), rng.uniform(), rng.uniform())
+            mats.append(principled(base, rng.uniform(0.05, 1.0),
+                                   1.0 if rng.uniform() < 0.3 else 0.0))
+        return len(mats) - 1
+
+    cells = max(2, int(math.sqrt(0.4 * budget / 2.0)))
+    meshes = [heightfield(cells, 12.0, rng.uniform(0.1, 0.8), seed + index, material())]
+    left = budget - 2 * cells * cells
+    sub = 1
+    while sub < 6 and 20 * 4 ** (sub + 1) * 4 <= left:
+        sub += 1
+    for _ in range(max(1, min(40, left // (20 * 4 ** sub)))):
+        r = rng.uniform(0.2, 0.9)
+        c = (rng.uniform(-4.5, 4.5), rng.uniform(0.8, 2.5), rng.uniform(-4.5, 4.5))
+        meshes.append(icosphere(c, r, sub, material()))
+    for _ in range(6):
+        size = (rng.uniform(0.3, 1.2), rng.uniform(0.3, 1.5), rng.uniform(0.3, 1.2))
+        c = (rng.uniform(-4.5, 4.5), size[1] / 2 + 0.5, rng.uniform(-4.5, 4.5))
+        meshes.append(box_mesh(c, size, rng.uniform(0.0, 1.57), material()))
+    meshes.append(quad((-0.75, 6.0, -0.75), (1.5, 0, 0), (0, 0, 1.5), 1))  # faces -y
+    env = rng.uniform(0.02, 0.1)
+    data = SceneData(meshes=meshes, materials=mats, envRadiance=(env, env * 1.05, env * 1.15))
+    data.emissiveIndex = _emissive_prims(data)
+    res = resolutions[min(len(resolutions) - 1, int(rng.uniform() * len(resolutions)))]
+    spp = spps[min(len(spps) - 1, int(rng.uniform() * len(spps)))]
+    ang = rng.uniform(0.0, 6.283)
+    cam = dict(pos=(9.0 * math.cos(ang), 5.5, 9.0 * math.sin(ang)), target=(0.0, 0.5, 0.0),
+               up=(0.0, 1.0, 0.0), fov=50.0, size=(res, res), spp=spp, max_depth=max_depth,
+               rr_start_depth=3)
     return data, cam
 
 

