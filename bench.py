@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--trace-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-trace", action="store_true", help="skip the closest-hit batch")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc runs that measure HBM traffic")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="counter-collection mode: one 256 MiB device copy (traffic "
                          "calibration), one 16,777,216-ray closest-hit launch and one render "
@@ -77,6 +79,53 @@ def profile_kernels(cr, L, scenes):
                       "render_samples": W * H, "render_rays": st["rays"]}), flush=True)
 
 
+def measure_traffic() -> dict | None:
+    """HBM bytes per dispatch of the profiled kernels from two rocprofv3 --pmc
+    runs of `bench.py --profile-kernels` (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch), each scaled by what the counter reads for the 256 MiB copy
+    that mode starts with.  Run as child processes before this process
+    touches the GPU; None when rocprofv3 is missing or a run fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None
+    vals: dict[str, dict[str, float]] = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="crh_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "240", exe, "--pmc", ctr, "--kernel-trace",
+               "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+               str(ROOT / "bench.py"), "--profile-kernels"]
+        r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
+                           capture_output=True, text=True)
+        files = list(Path(d).rglob("*counter_collection.csv"))
+        if r.returncode != 0 or not files:
+            return None
+        for row in csv.DictReader(open(files[0])):
+            name = row["Kernel_Name"]
+            key = ("copy" if "copyBuffer" in name else
+                   "kRenderPass" if "kRenderPass<" in name else
+                   "kTraceClosest" if "kTraceClosest<false" in name else None)
+            if key:
+                # the calibration copy is the largest copy dispatch
+                v = float(row["Counter_Value"])
+                d = vals.setdefault(key, {})
+                d[ctr] = max(d.get(ctr, 0.0), v) if key == "copy" else v
+        shutil.rmtree(d, ignore_errors=True)
+    try:
+        copy_kib = 256 * 1024
+        fs = copy_kib / vals["copy"]["FETCH_SIZE"]
+        ws = copy_kib / vals["copy"]["WRITE_SIZE"]
+        out = {"fetch_scale": round(fs, 5), "write_scale": round(ws, 5)}
+        for k in ("kRenderPass", "kTraceClosest"):
+            out[k] = int((vals[k]["FETCH_SIZE"] * fs + vals[k]["WRITE_SIZE"] * ws) * 1024)
+        return out
+    except (KeyError, ZeroDivisionError):
+        return None
+
+
 def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -84,6 +133,10 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not "
                                    "part of it (use --gpus 1)"}), flush=True)
         sys.exit(2)
+
+    traffic = None
+    if not a.profile_kernels and not a.no_traffic:
+        traffic = measure_traffic()
 
     import craystal_amd as cr
     from craystal_amd import _lib as L
@@ -175,7 +228,8 @@ def main():
                 "max_leaf": bst["max_leaf"], "build_s": round(build_s, 3)},
         "roofline": {"bound": "hbm", "kernel": "kRenderPass",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": traffic["kRenderPass"] if traffic else None,
                      "bytes_per_sample": round(b_sample, 1),
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                      "node_visits_per_sample": round(nodes_ps, 3),
@@ -189,6 +243,11 @@ def main():
     t0 = time.perf_counter()
     wst = wf.dispatch(scene, a.steps, first_sample=a.warmup, readback=False)
     wel = time.perf_counter() - t0
+    if traffic:
+        out["roofline"]["traffic_note"] = (
+            "HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of "
+            "bench.py --profile-kernels (one 1920x1080 pass), scaled by a 256 MiB copy "
+            f"(fetch x {traffic['fetch_scale']}, write x {traffic['write_scale']})")
     out["tuning"] = {k: tuned[k] for k in ("render_stack", "wf_stack", "wf_grid",
                                            "wf_trace_block", "wf_persistent", "wf_refill")}
     out["wavefront"] = {"msamples_per_s": round(samples / wel / 1e6, 3),
@@ -221,6 +280,7 @@ def main():
                                               "pt_grid")},
             "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
+                         "traffic": traffic["kTraceClosest"] if traffic else None,
                          "bytes_per_ray": round(b_ray, 1)}}
         del rb, hb
 

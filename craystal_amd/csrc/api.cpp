@@ -228,13 +228,16 @@ struct crh_film {
 
 namespace {
 
-// Carves the wavefront structure-of-arrays state out of one allocation.
-hipError_t ensureWavefront(crh_film* f) {
-    if (f->wfReady) return hipSuccess;
-    const size_t n = size_t(f->w) * f->h;
+// Carves the wavefront structure-of-arrays state for `slots` samples per
+// pixel out of one allocation (reallocated when more slots are needed).
+hipError_t ensureWavefront(crh_film* f, uint32_t slots) {
+    if (f->wfReady && f->wf.slots >= slots) return hipSuccess;
+    f->wfReady = f->guidedReady = false;
+    f->gArena.reset();
+    const size_t n = size_t(f->w) * f->h * slots;
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
-    const size_t v4 = align(n * 16), u1 = align(n * 4);
-    const size_t total = 16 * v4 + 9 * u1 + align(64) + align(64);
+    const size_t v4 = align(n * 16), u1 = align(n * 4), u2 = align(n * 8);
+    const size_t total = 17 * v4 + u2 + 9 * u1 + align(64) + align(64);
     hipError_t e = f->wfArena.alloc(total);
     if (e != hipSuccess) return e;
     char* base = f->wfArena.as<char>();
@@ -247,10 +250,14 @@ hipError_t ensureWavefront(crh_film* f) {
     crh::WfState& w = f->wf;
     w.width = f->w;
     w.height = f->h;
+    w.npix = f->w * f->h;
+    w.slots = slots;
+    w.active = slots;
     w.rayO = (float4*)take(v4);
     w.rayD = (float4*)take(v4);
     w.hit = (uint4*)take(v4);
     w.beta = (float4*)take(v4);
+    w.rad = (float4*)take(v4);
     w.shO = (float4*)take(v4);
     w.shD = (float4*)take(v4);
     w.lgt = (float4*)take(v4);
@@ -263,6 +270,7 @@ hipError_t ensureWavefront(crh_film* f) {
     w.misPos = (float4*)take(v4);
     w.misBeta = (float4*)take(v4);
     take(v4);  // spare
+    w.P = (float2*)take(u2);
     w.rng = (uint32_t*)take(u1);
     w.flags = (uint32_t*)take(u1);
     w.lgId = (uint32_t*)take(u1);
@@ -274,8 +282,6 @@ hipError_t ensureWavefront(crh_film* f) {
     w.qCont = (uint32_t*)take(u1);
     w.counters = (uint32_t*)take(align(64));
     w.work = (unsigned long long*)take(align(64));
-    w.rad = f->L.as<float4>();
-    w.P = f->P.as<float2>();
     w.gPos = w.gNrm = w.gWo = w.rec = nullptr;
     w.gMix = nullptr;
     w.recId = nullptr;
@@ -286,10 +292,10 @@ hipError_t ensureWavefront(crh_film* f) {
 
 // Guided-mode buffers: per-path shading frame and mixture, vertex records
 // (4 per path) and the training samples made from them.
-hipError_t ensureGuided(crh_film* f) {
-    hipError_t e = ensureWavefront(f);
+hipError_t ensureGuided(crh_film* f, uint32_t slots) {
+    hipError_t e = ensureWavefront(f, slots);
     if (e != hipSuccess || f->guidedReady) return e;
-    const size_t n = size_t(f->w) * f->h;
+    const size_t n = size_t(f->w) * f->h * f->wf.slots;
     const size_t cap = n * 4;
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t total = 3 * align(n * 16) + align(n * 40 * 4) + align(n * 4) +
@@ -310,9 +316,19 @@ hipError_t ensureGuided(crh_film* f) {
     w.gMix = (float*)take(n * 40 * 4);
     w.recId = (uint32_t*)take(n * 4);
     w.rec = (float4*)take(cap * crh::kRecF4 * 16);
-    w.recCap = uint32_t(cap);
+    w.recCap = uint32_t(std::min<size_t>(cap, 0xFFFFFFF0u));
     f->guidedReady = true;
     return hipSuccess;
+}
+
+// Samples per pixel the wavefront keeps in flight: enough paths to fill the
+// pool the tuning asks for, no more than the call renders.
+uint32_t wfSlots(const crh::Tuning& t, const crh_film* f, uint32_t spp) {
+    const uint64_t npix = uint64_t(f->w) * f->h;
+    uint64_t k = std::max<uint64_t>(1, t.wfPool / npix);
+    k = std::min<uint64_t>(k, std::max<uint32_t>(spp, 1u));
+    k = std::min<uint64_t>(k, std::max<uint64_t>(1, (uint64_t(1) << 31) / npix / 4));
+    return uint32_t(k);
 }
 
 }  // namespace
@@ -381,6 +397,7 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(t->persistent <= 2 && t->wf_persistent <= 2,
               "persistent must be 0 (keep), 1 (on) or 2 (off)");
     ARG_CHECK(t->refill <= 64 && t->wf_refill <= 64, "refill must be in [1, 64]");
+    ARG_CHECK(t->wf_pool <= (1u << 28), "wf_pool must be at most 2^28 paths");
     ARG_CHECK(okStack(t->batch_stack),
               "batch_stack must be 0 (keep), 1 (LDS), 2 (private) or 3 (short LDS + private)");
     ARG_CHECK(t->batch_tile_w == 0 || t->batch_tile_w == 8 || t->batch_tile_w == 16 ||
@@ -402,6 +419,7 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     if (t->wf_shade_grid) o.wfShadeGrid = t->wf_shade_grid;
     if (t->wf_persistent) o.wfPersistent = t->wf_persistent == 1 ? 1u : 0u;
     if (t->wf_refill) o.wfRefill = t->wf_refill;
+    if (t->wf_pool) o.wfPool = t->wf_pool;
     if (t->batch_stack) o.batchPriv = t->batch_stack - 1u;
     if (t->batch_tile_w) o.batchTileW = t->batch_tile_w;
     if (t->batch_grid) o.batchGrid = t->batch_grid;
@@ -426,6 +444,7 @@ int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
     t->wf_shade_grid = o.wfShadeGrid;
     t->wf_persistent = o.wfPersistent ? 1u : 2u;
     t->wf_refill = o.wfRefill;
+    t->wf_pool = o.wfPool;
     t->batch_stack = o.batchPriv + 1u;
     t->batch_tile_w = o.batchTileW;
     t->batch_grid = o.batchGrid;
@@ -937,17 +956,22 @@ int crh_render(crh_scene* s, const crh_camera* cam,
         for (auto& e : evs) HIP_TRY(hipEventCreate(&e));
     }
     const bool wavefront = cfg->mode == CRH_MODE_WAVEFRONT;
-    if (wavefront) {
-        HIP_TRY(ensureWavefront(film));
-        HIP_TRY(hipMemsetAsync(film->wf.work, 0, 64, ctx->stream));
-    }
     const crh::Tuning tun = effective(ctx->tuning, s->view);
     const crh::WfLaunch wl = wfLaunch(tun);
+    const uint32_t slots = wavefront ? wfSlots(tun, film, spp) : 1u;
+    if (wavefront) {
+        HIP_TRY(ensureWavefront(film, slots));
+        HIP_TRY(hipMemsetAsync(film->wf.work, 0, 64, ctx->stream));
+    }
+    const size_t npix = size_t(film->w) * film->h;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-    for (uint32_t i = 0; i < spp; ++i) {
+    // Pass groups: the megakernel renders one sample index per launch; the
+    // wavefront keeps `slots` of them in flight and gathers them in order.
+    for (uint32_t i = 0; i < spp; i += slots) {
+        const uint32_t active = std::min(slots, spp - i);
         if (stats) HIP_TRY(hipEventRecord(evs[2 * i], ctx->stream));
         if (wavefront)
-            HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, film->wf, wl,
+            HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, active, film->wf, wl,
                                        cfg->count_work != 0, ctx->stream));
         else
             HIP_TRY(crh::launchRenderPass(s->view, cv, iv, first_sample + i,
@@ -955,9 +979,12 @@ int crh_render(crh_scene* s, const crh_camera* cam,
                                           counter.as<unsigned long long>(),
                                           cfg->count_work != 0, tun, ctx->stream));
         if (stats) HIP_TRY(hipEventRecord(evs[2 * i + 1], ctx->stream));
-        HIP_TRY(crh::launchFilmGather(film->L.as<float4>(), film->P.as<float2>(),
-                                      film->w, film->h, film->sum.as<float>(),
-                                      film->wsum.as<float>(), ctx->stream));
+        for (uint32_t k = 0; k < active; ++k) {
+            const float4* Ls = wavefront ? film->wf.rad + k * npix : film->L.as<float4>();
+            const float2* Ps = wavefront ? film->wf.P + k * npix : film->P.as<float2>();
+            HIP_TRY(crh::launchFilmGather(Ls, Ps, film->w, film->h, film->sum.as<float>(),
+                                          film->wsum.as<float>(), ctx->stream));
+        }
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev1));
@@ -965,7 +992,7 @@ int crh_render(crh_scene* s, const crh_camera* cam,
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         double mainMs = 0.0;
-        for (uint32_t i = 0; i < spp; ++i) {
+        for (uint32_t i = 0; i < spp; i += slots) {
             float m = 0.f;
             HIP_TRY(hipEventElapsedTime(&m, evs[2 * i], evs[2 * i + 1]));
             mainMs += m;
@@ -979,7 +1006,8 @@ int crh_render(crh_scene* s, const crh_camera* cam,
         stats->rays = c[0];
         stats->node_visits = c[1];
         stats->tri_tests = c[2];
-        stats->launches = wavefront ? uint64_t(spp) * (6u * cfg->max_depth + 5u)
+        const uint64_t groups = (spp + slots - 1) / slots;
+        stats->launches = wavefront ? groups * (6u * cfg->max_depth + 5u) + spp
                                     : uint64_t(spp) * 2;
     }
     return CRH_OK;
@@ -1219,10 +1247,12 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
                            cfg->use_mis ? 1u : 0u};
     double t0 = nowMs();
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(ensureGuided(film));
-    HIP_TRY(hipMemsetAsync(film->wf.work, 0, 64, ctx->stream));
     const crh::Tuning tun = effective(ctx->tuning, s->view);
     const crh::WfLaunch wl = wfLaunch(tun);
+    const uint32_t slots = wfSlots(tun, film, spp);
+    HIP_TRY(ensureGuided(film, slots));
+    HIP_TRY(hipMemsetAsync(film->wf.work, 0, 64, ctx->stream));
+    const size_t npix = size_t(film->w) * film->h;
     const uint32_t batch = gc->train_batch;
     const size_t padB = (size_t(batch) + 63) & ~size_t(63);
     if (gc->train && padB > g->capSamples) {
@@ -1230,7 +1260,7 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
         HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * padB * 64 * 2));
         g->capSamples = padB;
     }
-    // Sample buffer: one pass's records on top of a batch still filling.
+    // Sample buffer: one pass group's records on top of a batch still filling.
     const size_t needCap = size_t(film->wf.recCap) + batch;
     if (gc->train && film->gCap < needCap) {
         HIP_TRY(film->gSamples.alloc(needCap * sizeof(crh::GuideSample)));
@@ -1240,15 +1270,17 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
     double mainMs = 0.0;
     uint32_t lastSamples = 0;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-    for (uint32_t i = 0; i < spp; ++i) {
+    for (uint32_t i = 0; i < spp; i += slots) {
+        const uint32_t active = std::min(slots, spp - i);
         crh::GuideSample* smpBuf = film->gSamples.as<crh::GuideSample>();
-        HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, film->wf, wl, false,
-                                   ctx->stream, &g->view, gc->alpha,
+        HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, active, film->wf, wl,
+                                   false, ctx->stream, &g->view, gc->alpha,
                                    gc->train ? smpBuf + film->gPending : nullptr,
                                    uint32_t(film->gCap - film->gPending)));
-        HIP_TRY(crh::launchFilmGather(film->L.as<float4>(), film->P.as<float2>(), film->w,
-                                      film->h, film->sum.as<float>(), film->wsum.as<float>(),
-                                      ctx->stream));
+        for (uint32_t k = 0; k < active; ++k)
+            HIP_TRY(crh::launchFilmGather(film->wf.rad + k * npix, film->wf.P + k * npix,
+                                          film->w, film->h, film->sum.as<float>(),
+                                          film->wsum.as<float>(), ctx->stream));
         if (gc->train) {
             uint32_t produced = 0;
             HIP_TRY(hipMemcpyAsync(&produced, film->wf.counters + crh::kCTrain, 4,
@@ -1259,7 +1291,7 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
             g->lastSamples = produced;
             const size_t pending = film->gPending + produced;
             uint32_t k = 0;
-            for (; k < gc->train_steps && size_t(k + 1) * batch <= pending; ++k) {
+            for (; k < gc->train_steps * active && size_t(k + 1) * batch <= pending; ++k) {
                 g->step++;
                 float* slot = lossSlot(g);
                 HIP_TRY(hipMemsetAsync(slot, 0, 4, ctx->stream));
@@ -1292,7 +1324,7 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
         stats->total_ms = nowMs() - t0;
         stats->rays = c[0];
         stats->node_visits = lastSamples;
-        stats->launches = spp;
+        stats->launches = (spp + slots - 1) / slots;
     }
     return CRH_OK;
     GUARD_END

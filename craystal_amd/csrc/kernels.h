@@ -32,6 +32,7 @@ struct Tuning {
     uint32_t wfShadeGrid = 16384;
     uint32_t wfPersistent = 0;
     uint32_t wfRefill = 16;
+    uint32_t wfPool = 1u << 22;  // paths in flight (several samples per pixel for small images)
     // batched rendering
     uint32_t batchPriv = 0;
     uint32_t batchTileW = 16;   // tile width in pixels; height = 256 / width

@@ -23,14 +23,17 @@ enum : uint32_t {
 
 struct WfState {
     uint32_t width, height;
+    uint32_t npix;    // width * height
+    uint32_t slots;   // samples per pixel the state holds (paths = slots * npix)
+    uint32_t active;  // samples per pixel of the current pass group
     float4* rayO;  // extension ray: origin, tMin
     float4* rayD;  // direction, tMax
     uint4* hit;    // closest hit of the extension ray
     float4* beta;
-    float4* rad;   // radiance (aliases the film's per-pass sample buffer)
+    float4* rad;   // radiance; path p = slot * npix + pixel
     uint32_t* rng; // Sampler state (the increment is recomputed from the pixel)
     uint32_t* flags;
-    float2* P;     // jittered film position (aliases the film buffer)
+    float2* P;     // jittered film position
     float4* shO;   // shadow ray
     float4* shD;
     float4* lgt;   // light direction, d^2
@@ -77,8 +80,11 @@ struct GuideSample;
 // guide != null: guided continuation sampling (one-sample MIS with the
 // guide's mixture, fraction alpha for BSDF sampling).  samples != null:
 // also convert this pass's vertex records into training samples.
+// Renders sample indices si .. si + active - 1 of every pixel at once (one
+// path per pixel and sample, active <= w.slots); the caller gathers the
+// slots into the film in sample order.
 hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
-                         const IntegratorView& cfg, uint32_t si, WfState& w,
+                         const IntegratorView& cfg, uint32_t si, uint32_t active, WfState& w,
                          const WfLaunch& L, bool countWork, hipStream_t st,
                          const GuideView* guide = nullptr, float alpha = 0.5f,
                          GuideSample* samples = nullptr, uint32_t sampleCap = 0);

@@ -81,7 +81,8 @@ CRH_D void stS(float4* a, uint32_t p, Spec s, float w = 0.f) {
     a[p] = make_float4(s.r, s.g, s.b, w);
 }
 CRH_D Sampler loadSampler(const WfState& w, uint32_t p) {
-    uint32_t x = p % w.width, y = p / w.width;
+    uint32_t pix = p % w.npix;
+    uint32_t x = pix % w.width, y = pix / w.width;
     Sampler s(0u, 0u, 0u);
     s.state = w.rng[p];
     s.inc = x * 1234567891u + y * 987654321u + 1u;
@@ -102,10 +103,11 @@ __global__ __launch_bounds__(64) void kWfReset(uint32_t* c, uint32_t mask) {
 
 __global__ __launch_bounds__(256) void kWfGenerate(WfState w, CameraView cam,
                                                    uint32_t si) {
-    uint32_t n = w.width * w.height;
+    uint32_t n = w.active * w.npix;
     PERSISTENT_LOOP(p, n) {
-        uint32_t x = p % w.width, y = p / w.width;
-        Sampler smp(x, y, si);
+        uint32_t px = p % w.npix;
+        uint32_t x = px % w.width, y = px / w.width;
+        Sampler smp(x, y, si + p / w.npix);
         F2 j = smp.next2D();
         F2 pix{float(x) + j.x, float(y) + j.y};
         Ray r = cameraRay(cam, pix);
@@ -497,10 +499,12 @@ static void launchExtend(bool count, dim3 g, dim3 b, size_t lds, hipStream_t st,
 }
 
 hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
-                         const IntegratorView& cfg, uint32_t si, WfState& w,
+                         const IntegratorView& cfg, uint32_t si, uint32_t active, WfState& w,
                          const WfLaunch& L, bool countWork, hipStream_t st,
                          const GuideView* guide, float alpha, GuideSample* samples,
                          uint32_t sampleCap) {
+    if (active == 0 || active > w.slots) return hipErrorInvalidValue;
+    w.active = active;
     const uint32_t TB = L.traceBlock;
     const size_t lds = stackLdsBytes(L.privStack, s.stackCap, TB);
     const dim3 b256(256), bT(TB);

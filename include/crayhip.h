@@ -211,6 +211,8 @@ typedef struct crh_tuning {
     uint32_t wf_shade_grid;  /* wavefront shading kernels: blocks of 256 */
     uint32_t wf_persistent;  /* wavefront traversal: 1 = lanes refill, 2 = off */
     uint32_t wf_refill;      /* wavefront traversal: idle lanes per wave that trigger a refill */
+    uint32_t wf_pool;        /* wavefront: paths kept in flight; samples per pixel rendered
+                                together = max(1, wf_pool / pixels), capped by the spp asked */
     uint32_t batch_stack;    /* batched rendering: stack as above */
     uint32_t batch_tile_w;   /* batched rendering: tile width 8, 16, 32 or 64 (256 px tiles) */
     uint32_t batch_grid;     /* batched rendering: persistent blocks per pass */
@@ -343,10 +345,11 @@ int crh_guide_destroy(crh_guide* g);
 /* Guided rendering: the wavefront integrator with one-sample MIS between
  * BSDF sampling (fraction alpha) and the guide's mixture at every
  * Lambertian / Principled vertex; the estimator's expectation equals the
- * plain integrator's.  With train != 0, every pass's vertices with non-zero
- * radiance become training samples; they accumulate on the film over passes
- * (and calls) and each pass is followed by up to train_steps Adam steps, one
- * per full batch of train_batch samples. */
+ * plain integrator's.  Passes run in groups of samples per pixel kept in
+ * flight together (crh_tuning.wf_pool).  With train != 0, the vertices with
+ * non-zero radiance become training samples; they accumulate on the film over
+ * groups (and calls) and each group of k passes is followed by up to
+ * k * train_steps Adam steps, one per full batch of train_batch samples. */
 typedef struct crh_guided_cfg {
     float alpha;
     uint32_t train;
@@ -355,7 +358,8 @@ typedef struct crh_guided_cfg {
     float lr;
     uint32_t reserved[3];
 } crh_guided_cfg;
-/* stats->node_visits receives the training samples the last pass produced. */
+/* stats->node_visits receives the training samples the last pass group
+ * produced; stats->launches the number of pass groups. */
 int crh_render_guided(crh_scene* scene, const crh_camera* cam,
                       const crh_integrator_cfg* cfg, crh_guide* guide,
                       const crh_guided_cfg* gcfg, uint32_t first_sample,

@@ -150,19 +150,29 @@ def _film(scene, camd, size, cfg, spp, first=0):
 @pytest.mark.parametrize("mis", [True, False])
 def test_wavefront_film_bit_identical_to_megakernel(gpu, mis):
     """Same RNG streams, same per-path arithmetic (integrator.h), different
-    schedule: the two films are bit-identical, and so is the ray count."""
+    schedule: the two films are bit-identical, and so is the ray count, with
+    one sample per pixel in flight or several (path pools of 1, 3 and 5
+    samples per pixel over 5 passes, the last group ragged)."""
     from craystal_amd import _lib as L
-    for data, camd, size in [(*scenes.cornell_box(), (64, 64)),
-                             (*_materials_scene(), (64, 48))]:
-        scene = cr.Scene(data)
-        base = dict(maxDepth=8, rrThreshold=0.05, useMIS=mis, rrStartDepth=3)
-        d0, w0, s0 = _film(scene, camd, size, cr.Configs(**base), 4, first=5)
-        d1, w1, s1 = _film(scene, camd, size, cr.Configs(**base, mode=L.MODE_WAVEFRONT), 4,
-                           first=5)
-        assert np.array_equal(w0.view(np.uint32), w1.view(np.uint32))
-        assert np.array_equal(d0.view(np.uint32), d1.view(np.uint32)), \
-            np.argwhere(d0 != d1)[:5]
-        assert s0["rays"] == s1["rays"]
+    ctx = cr.get_context(0)
+    saved = ctx.get_tuning()
+    try:
+        for data, camd, size in [(*scenes.cornell_box(), (64, 64)),
+                                 (*_materials_scene(), (64, 48))]:
+            scene = cr.Scene(data)
+            base = dict(maxDepth=8, rrThreshold=0.05, useMIS=mis, rrStartDepth=3)
+            d0, w0, s0 = _film(scene, camd, size, cr.Configs(**base), 5, first=5)
+            npix = size[0] * size[1]
+            for k in (1, 3, 5):
+                ctx.set_tuning(wf_pool=k * npix)
+                d1, w1, s1 = _film(scene, camd, size,
+                                   cr.Configs(**base, mode=L.MODE_WAVEFRONT), 5, first=5)
+                assert np.array_equal(w0.view(np.uint32), w1.view(np.uint32)), k
+                assert np.array_equal(d0.view(np.uint32), d1.view(np.uint32)), \
+                    (k, np.argwhere(d0 != d1)[:5])
+                assert s0["rays"] == s1["rays"], k
+    finally:
+        ctx.set_tuning(**saved)
 
 
 def _materials_scene():
