@@ -165,8 +165,9 @@ def main():
     # launch shapes picked by the autotuner (outside the timed region; every
     # candidate must render the same image bit for bit)
     from craystal_amd import autotune
-    tuned = autotune.tune_render(scene, cfg, spp=1, wf_grids=(4096, 16384), wf_blocks=(128, 256),
-                                 wf_persistent=((2, 0), (1, 16)))
+    tuned = autotune.tune_render(scene, cfg, spp=4, stacks=(1, 2, 3), wf_grids=(16384,),
+                                 wf_blocks=(128, 256), wf_persistent=((2, 0),),
+                                 wf_pools=(1 << 21, 1 << 22, 1 << 23))
     sensor.clear()
 
     # warmup (untimed)
@@ -249,7 +250,8 @@ def main():
             "bench.py --profile-kernels (one 1920x1080 pass), scaled by a 256 MiB copy "
             f"(fetch x {traffic['fetch_scale']}, write x {traffic['write_scale']})")
     out["tuning"] = {k: tuned[k] for k in ("render_stack", "wf_stack", "wf_grid",
-                                           "wf_trace_block", "wf_persistent", "wf_refill")}
+                                           "wf_trace_block", "wf_persistent", "wf_refill",
+                                           "wf_pool")}
     out["wavefront"] = {"msamples_per_s": round(samples / wel / 1e6, 3),
                         "ms_per_step": round(wel / a.steps * 1e3, 3),
                         "grays_per_s": round(wst["rays"] / wel / 1e9, 4)}
@@ -261,8 +263,8 @@ def main():
         rb = cr.DeviceBuffer(rays.nbytes)
         hb = cr.DeviceBuffer(n * 16)
         rb.copyFromHost(rays)
-        ttuned = autotune.tune_trace(scene, rays, reps=2, persistent=((2, 0), (1, 16)),
-                                     pt_grids=(4096,))
+        ttuned = autotune.tune_trace(scene, rays, reps=2, stacks=(1, 2, 3), blocks=(64, 128),
+                                     persistent=((2, 0),))
         cr.trace_closest_device(scene, rb, n, hb)  # warmup
         cnt = cr.trace_closest_device(scene, rb, n, hb, count_work=True)
         times = []

@@ -68,11 +68,13 @@ def tune_trace(scene: Scene, rays: np.ndarray, reps: int = 3, device: int = 0,
 
 def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
                 stacks=(1, 2), wf_grids=(1024, 2048, 4096), wf_blocks=(128, 256),
-                wf_persistent=((2, 0), (1, 16)), first_sample: int = 1 << 20) -> dict:
+                wf_persistent=((2, 0), (1, 16)), wf_pools=(None,),
+                first_sample: int = 1 << 20) -> dict:
     """Tune the megakernel stack placement and the wavefront stack placement,
-    grid/block sizes and persistence ((mode, refill) pairs) on the scene's
-    current camera/sensor; every candidate must give the same image bit for
-    bit.  Applies the best choice of each integrator and returns the faster
+    grid/block sizes, persistence ((mode, refill) pairs) and path pool (paths
+    in flight; None keeps the current one) on the scene's current
+    camera/sensor; every candidate must give the same image bit for bit.
+    Applies the best choice of each integrator and returns the faster
     integrator's mode with the timings (ms per spp) of both."""
     ctx = get_context(device)
     sensor = scene.getCamera().getSensor()
@@ -94,20 +96,26 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
     best_stack = min(res_mk, key=res_mk.get)
     ctx.set_tuning(render_stack=best_stack)
     wf_cfg = Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
-    for st, g, b, (pm, rf) in itertools.product(stacks, wf_grids, wf_blocks, wf_persistent):
-        ctx.set_tuning(wf_stack=st, wf_trace_grid=g, wf_shade_grid=g, wf_trace_block=b,
-                       wf_persistent=pm, **({"wf_refill": rf} if pm == 1 else {}))
+    pool0 = ctx.get_tuning()["wf_pool"]
+    pools = [pool0 if p is None else p for p in wf_pools]
+
+    def wf_kw(st, g, b, pm, rf, pool):
+        return dict(wf_stack=st, wf_trace_grid=g, wf_shade_grid=g, wf_trace_block=b,
+                    wf_persistent=pm, wf_pool=pool, **({"wf_refill": rf} if pm == 1 else {}))
+
+    for st, g, b, (pm, rf), pool in itertools.product(stacks, wf_grids, wf_blocks,
+                                                      wf_persistent, pools):
+        ctx.set_tuning(**wf_kw(st, g, b, pm, rf, pool))
         ms, img = run(wf_cfg)
         assert img.tobytes() == ref.tobytes(), "wavefront launch shape changed the image"
-        res_wf[(st, g, b, pm, rf)] = ms
+        res_wf[(st, g, b, pm, rf, pool)] = ms
     best_wf = min(res_wf, key=res_wf.get)
-    st, g, b, pm, rf = best_wf
-    ctx.set_tuning(wf_stack=st, wf_trace_grid=g, wf_shade_grid=g, wf_trace_block=b,
-                   wf_persistent=pm, **({"wf_refill": rf} if pm == 1 else {}))
+    st, g, b, pm, rf, pool = best_wf
+    ctx.set_tuning(**wf_kw(st, g, b, pm, rf, pool))
     mk_ms, wf_ms = res_mk[best_stack], res_wf[best_wf]
     return {"mode": L.MODE_WAVEFRONT if wf_ms < mk_ms else L.MODE_MEGAKERNEL,
             "render_stack": best_stack, "wf_stack": st, "wf_grid": g, "wf_trace_block": b,
-            "wf_persistent": pm, "wf_refill": rf,
+            "wf_persistent": pm, "wf_refill": rf, "wf_pool": pool,
             "megakernel_ms": {str(k): round(v, 3) for k, v in res_mk.items()},
             "wavefront_ms": {str(k): round(v, 3) for k, v in res_wf.items()}}
 

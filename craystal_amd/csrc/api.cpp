@@ -321,14 +321,26 @@ hipError_t ensureGuided(crh_film* f, uint32_t slots) {
     return hipSuccess;
 }
 
-// Samples per pixel the wavefront keeps in flight: enough paths to fill the
-// pool the tuning asks for, no more than the call renders.
-uint32_t wfSlots(const crh::Tuning& t, const crh_film* f, uint32_t spp) {
+// Samples per pixel rendered together: enough paths to fill `pool`, no
+// more than the call renders.
+uint32_t poolSlots(uint32_t pool, const crh_film* f, uint32_t spp) {
     const uint64_t npix = uint64_t(f->w) * f->h;
-    uint64_t k = std::max<uint64_t>(1, t.wfPool / npix);
+    uint64_t k = std::max<uint64_t>(1, pool / npix);
     k = std::min<uint64_t>(k, std::max<uint32_t>(spp, 1u));
     k = std::min<uint64_t>(k, std::max<uint64_t>(1, (uint64_t(1) << 31) / npix / 4));
     return uint32_t(k);
+}
+uint32_t wfSlots(const crh::Tuning& t, const crh_film* f, uint32_t spp) {
+    return poolSlots(t.wfPool, f, spp);
+}
+
+// Per-pass sample buffers of the film for `slots` sample indices per launch.
+hipError_t ensureFilmSlots(crh_film* f, uint32_t slots) {
+    const size_t n = size_t(f->w) * f->h * slots;
+    if (f->L.bytes >= n * sizeof(float4) && f->P.bytes >= n * sizeof(float2)) return hipSuccess;
+    hipError_t e = f->L.alloc(n * sizeof(float4));
+    if (e == hipSuccess) e = f->P.alloc(n * sizeof(float2));
+    return e;
 }
 
 }  // namespace
@@ -397,7 +409,8 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(t->persistent <= 2 && t->wf_persistent <= 2,
               "persistent must be 0 (keep), 1 (on) or 2 (off)");
     ARG_CHECK(t->refill <= 64 && t->wf_refill <= 64, "refill must be in [1, 64]");
-    ARG_CHECK(t->wf_pool <= (1u << 28), "wf_pool must be at most 2^28 paths");
+    ARG_CHECK(t->wf_pool <= (1u << 28) && t->render_pool <= (1u << 28),
+              "wf_pool and render_pool must be at most 2^28 paths");
     ARG_CHECK(okStack(t->batch_stack),
               "batch_stack must be 0 (keep), 1 (LDS), 2 (private) or 3 (short LDS + private)");
     ARG_CHECK(t->batch_tile_w == 0 || t->batch_tile_w == 8 || t->batch_tile_w == 16 ||
@@ -420,6 +433,7 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     if (t->wf_persistent) o.wfPersistent = t->wf_persistent == 1 ? 1u : 0u;
     if (t->wf_refill) o.wfRefill = t->wf_refill;
     if (t->wf_pool) o.wfPool = t->wf_pool;
+    if (t->render_pool) o.renderPool = t->render_pool;
     if (t->batch_stack) o.batchPriv = t->batch_stack - 1u;
     if (t->batch_tile_w) o.batchTileW = t->batch_tile_w;
     if (t->batch_grid) o.batchGrid = t->batch_grid;
@@ -445,6 +459,7 @@ int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
     t->wf_persistent = o.wfPersistent ? 1u : 2u;
     t->wf_refill = o.wfRefill;
     t->wf_pool = o.wfPool;
+    t->render_pool = o.renderPool;
     t->batch_stack = o.batchPriv + 1u;
     t->batch_tile_w = o.batchTileW;
     t->batch_grid = o.batchGrid;
@@ -958,15 +973,18 @@ int crh_render(crh_scene* s, const crh_camera* cam,
     const bool wavefront = cfg->mode == CRH_MODE_WAVEFRONT;
     const crh::Tuning tun = effective(ctx->tuning, s->view);
     const crh::WfLaunch wl = wfLaunch(tun);
-    const uint32_t slots = wavefront ? wfSlots(tun, film, spp) : 1u;
+    const uint32_t slots = wavefront ? wfSlots(tun, film, spp)
+                                     : poolSlots(tun.renderPool, film, spp);
     if (wavefront) {
         HIP_TRY(ensureWavefront(film, slots));
         HIP_TRY(hipMemsetAsync(film->wf.work, 0, 64, ctx->stream));
+    } else {
+        HIP_TRY(ensureFilmSlots(film, slots));
     }
     const size_t npix = size_t(film->w) * film->h;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-    // Pass groups: the megakernel renders one sample index per launch; the
-    // wavefront keeps `slots` of them in flight and gathers them in order.
+    // Pass groups: `slots` sample indices per megakernel launch or wavefront
+    // pass, gathered into the film in sample order.
     for (uint32_t i = 0; i < spp; i += slots) {
         const uint32_t active = std::min(slots, spp - i);
         if (stats) HIP_TRY(hipEventRecord(evs[2 * i], ctx->stream));
@@ -974,14 +992,14 @@ int crh_render(crh_scene* s, const crh_camera* cam,
             HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, active, film->wf, wl,
                                        cfg->count_work != 0, ctx->stream));
         else
-            HIP_TRY(crh::launchRenderPass(s->view, cv, iv, first_sample + i,
+            HIP_TRY(crh::launchRenderPass(s->view, cv, iv, first_sample + i, active,
                                           film->L.as<float4>(), film->P.as<float2>(),
                                           counter.as<unsigned long long>(),
                                           cfg->count_work != 0, tun, ctx->stream));
         if (stats) HIP_TRY(hipEventRecord(evs[2 * i + 1], ctx->stream));
         for (uint32_t k = 0; k < active; ++k) {
-            const float4* Ls = wavefront ? film->wf.rad + k * npix : film->L.as<float4>();
-            const float2* Ps = wavefront ? film->wf.P + k * npix : film->P.as<float2>();
+            const float4* Ls = (wavefront ? film->wf.rad : film->L.as<float4>()) + k * npix;
+            const float2* Ps = (wavefront ? film->wf.P : film->P.as<float2>()) + k * npix;
             HIP_TRY(crh::launchFilmGather(Ls, Ps, film->w, film->h, film->sum.as<float>(),
                                           film->wsum.as<float>(), ctx->stream));
         }
@@ -1008,7 +1026,7 @@ int crh_render(crh_scene* s, const crh_camera* cam,
         stats->tri_tests = c[2];
         const uint64_t groups = (spp + slots - 1) / slots;
         stats->launches = wavefront ? groups * (6u * cfg->max_depth + 5u) + spp
-                                    : uint64_t(spp) * 2;
+                                    : groups + spp;
     }
     return CRH_OK;
     GUARD_END
@@ -1236,6 +1254,7 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
     ARG_CHECK(s && cam && cfg && g && gc && film, "NULL argument");
     ARG_CHECK(film->ctx == s->ctx && g->ctx == s->ctx, "objects belong to different contexts");
     ARG_CHECK(gc->alpha >= 0.f && gc->alpha <= 1.f, "alpha must be in [0, 1]");
+    ARG_CHECK(gc->materials <= 3u, "materials must be a mask of bits 0 and 1");
     ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
     ARG_CHECK(!gc->train || (gc->train_batch >= 64 && gc->train_batch <= (1u << 22)),
               "train_batch must be in [64, 2^22]");
@@ -1276,7 +1295,8 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
         HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, active, film->wf, wl,
                                    false, ctx->stream, &g->view, gc->alpha,
                                    gc->train ? smpBuf + film->gPending : nullptr,
-                                   uint32_t(film->gCap - film->gPending)));
+                                   uint32_t(film->gCap - film->gPending),
+                                   gc->materials ? gc->materials : 3u, gc->max_depth));
         for (uint32_t k = 0; k < active; ++k)
             HIP_TRY(crh::launchFilmGather(film->wf.rad + k * npix, film->wf.P + k * npix,
                                           film->w, film->h, film->sum.as<float>(),

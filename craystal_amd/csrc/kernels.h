@@ -33,6 +33,7 @@ struct Tuning {
     uint32_t wfPersistent = 0;
     uint32_t wfRefill = 16;
     uint32_t wfPool = 1u << 22;  // paths in flight (several samples per pixel for small images)
+    uint32_t renderPool = 1u << 21;  // megakernel: paths per launch, the same way
     // batched rendering
     uint32_t batchPriv = 0;
     uint32_t batchTileW = 16;   // tile width in pixels; height = 256 / width
@@ -60,8 +61,10 @@ hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
                               const Tuning& t, uint32_t* counter, hipStream_t st);
 hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
                           uint8_t* occ, const Tuning& t, uint32_t* counter, hipStream_t st);
+// Sample indices si .. si + active - 1 in one launch; slot k of L and P
+// (w * h entries each) receives sample si + k.
 hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
-                            const IntegratorView& cfg, uint32_t si, float4* L,
+                            const IntegratorView& cfg, uint32_t si, uint32_t active, float4* L,
                             float2* P, unsigned long long* counters,
                             bool countWork, const Tuning& t, hipStream_t st);
 // One sample index over the first nActive scenes of the table: persistent

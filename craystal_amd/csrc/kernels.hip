@@ -132,9 +132,8 @@ CRH_D Spec evalMIS(PathCtx<COUNT, Stack>& c, const Isect& it, const BSDF& bsdf,
     return value / ls.selPdf;
 }
 
-CRH_D void tileCoord(uint32_t tilesX, uint32_t& x, uint32_t& y) {
+CRH_D void tileCoord(uint32_t tile, uint32_t tilesX, uint32_t& x, uint32_t& y) {
     // 16x16 pixel tile per block; each wave covers an 8x8 quarter.
-    uint32_t tile = blockIdx.x;
     uint32_t tx = tile % tilesX, ty = tile / tilesX;
     uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     x = tx * 16u + (wave & 1u) * 8u + (lane & 7u);
@@ -218,19 +217,23 @@ CRH_D void flushCounters(unsigned long long* counters, uint32_t rays, uint32_t n
     }
 }
 
+// Block b renders tile b % tiles of sample index si + b / tiles into slot
+// b / tiles of L and P (npix entries per slot).
 template <bool MIS, bool COUNT, int SM>
 __global__ __launch_bounds__(kRenderBlock) void kRenderPass(
     SceneView s, CameraView cam, IntegratorView cfg, uint32_t si,
-    uint32_t tilesX, float4* __restrict__ L, float2* __restrict__ P,
+    uint32_t tilesX, uint32_t tiles, float4* __restrict__ L, float2* __restrict__ P,
     unsigned long long* __restrict__ counters) {
     extern __shared__ uint2 lds[];
+    const uint32_t slot = blockIdx.x / tiles;
+    const size_t off = size_t(slot) * cam.width * cam.height;
     uint32_t x, y;
-    tileCoord(tilesX, x, y);
+    tileCoord(blockIdx.x - slot * tiles, tilesX, x, y);
     uint32_t rays = 0;
     WorkCount wc{0, 0};
     if (x < cam.width && y < cam.height) {
         auto stk = makeStack<SM>(lds);
-        tracePixel<MIS, COUNT>(s, stk, cam, cfg, x, y, si, L, P, rays, wc);
+        tracePixel<MIS, COUNT>(s, stk, cam, cfg, x, y, si + slot, L + off, P + off, rays, wc);
     }
     flushCounters<COUNT>(counters, rays, wc.nodes, wc.tris);
 }
@@ -418,26 +421,28 @@ hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
 }
 
 hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
-                            const IntegratorView& cfg, uint32_t si, float4* L,
+                            const IntegratorView& cfg, uint32_t si, uint32_t active, float4* L,
                             float2* P, unsigned long long* counters,
                             bool countWork, const Tuning& t, hipStream_t st) {
     const uint32_t tilesX = (cam.width + 15u) / 16u, tilesY = (cam.height + 15u) / 16u;
+    const uint32_t tiles = tilesX * tilesY;
+    if (uint64_t(tiles) * active >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
     const size_t lds = stackLdsBytes(t.renderPriv, s.stackCap, kRenderBlock);
-    const dim3 grid(tilesX * tilesY), block(kRenderBlock);
+    const dim3 grid(tiles * active), block(kRenderBlock);
     withStack(t.renderPriv, [&](auto sm) {
         constexpr int SM = decltype(sm)::value;
         if (cfg.useMIS && countWork)
             hipLaunchKernelGGL((kRenderPass<true, true, SM>), grid, block, lds, st, s, cam, cfg,
-                               si, tilesX, L, P, counters);
+                               si, tilesX, tiles, L, P, counters);
         else if (cfg.useMIS)
             hipLaunchKernelGGL((kRenderPass<true, false, SM>), grid, block, lds, st, s, cam, cfg,
-                               si, tilesX, L, P, counters);
+                               si, tilesX, tiles, L, P, counters);
         else if (countWork)
             hipLaunchKernelGGL((kRenderPass<false, true, SM>), grid, block, lds, st, s, cam, cfg,
-                               si, tilesX, L, P, counters);
+                               si, tilesX, tiles, L, P, counters);
         else
             hipLaunchKernelGGL((kRenderPass<false, false, SM>), grid, block, lds, st, s, cam, cfg,
-                               si, tilesX, L, P, counters);
+                               si, tilesX, tiles, L, P, counters);
     });
     return hipGetLastError();
 }

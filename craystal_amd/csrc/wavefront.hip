@@ -323,7 +323,7 @@ template <bool MIS, bool GUIDED>
 __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
                                                     IntegratorView cfg,
                                                     uint32_t cur, uint32_t depth,
-                                                    float alpha) {
+                                                    float alpha, uint32_t gmask) {
     const uint32_t n = w.counters[kCCont];
     const uint32_t nxt = cur ^ 1u;
     PERSISTENT_LOOP(i, n) {
@@ -378,7 +378,9 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
         F3 wi;
         float pdf = 0.0f;
         Spec f;
-        bool guided = GUIDED && bsdf.type != 2u;
+        // gmask: bit t set = guide BSDF type t (0 Lambertian, 1 Principled);
+        // Phong is never guided (its sampled density is not the mixture's).
+        bool guided = GUIDED && bsdf.type != 2u && ((gmask >> bsdf.type) & 1u);
         float qmix = 0.f;
         if (!guided) {
             f = bsdfSampleEval(bsdf, smp, it.view, wi, pdf);
@@ -502,7 +504,7 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                          const IntegratorView& cfg, uint32_t si, uint32_t active, WfState& w,
                          const WfLaunch& L, bool countWork, hipStream_t st,
                          const GuideView* guide, float alpha, GuideSample* samples,
-                         uint32_t sampleCap) {
+                         uint32_t sampleCap, uint32_t guideMask, uint32_t guideDepth) {
     if (active == 0 || active > w.slots) return hipErrorInvalidValue;
     w.active = active;
     const uint32_t TB = L.traceBlock;
@@ -552,22 +554,26 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
         else
             hipLaunchKernelGGL((kWfShadeHit<false>), dim3(L.shadeGrid), b256, 0, st, s, w, cur);
         shadow();
-        if (guided) {
+        // Past guideDepth the guided render continues like the plain one (no
+        // inference); records of earlier vertices still get their snapshots.
+        const bool guideHere = guided && guideMask != 0u && (guideDepth == 0u || d < guideDepth);
+        const uint32_t gm = guideMask;
+        if (guideHere) {
             hipError_t e = launchGuideInfer(*guide, w.qCont, w.counters + kCCont, 0u, w.gPos,
                                             w.gNrm, w.gWo, w.gMix, nullptr, L.shadeGrid, st);
             if (e != hipSuccess) return e;
             if (cfg.useMIS)
                 hipLaunchKernelGGL((kWfShadeCont<true, true>), dim3(L.shadeGrid), b256, 0, st, s,
-                                   w, cfg, cur, d, alpha);
+                                   w, cfg, cur, d, alpha, gm);
             else
                 hipLaunchKernelGGL((kWfShadeCont<false, true>), dim3(L.shadeGrid), b256, 0, st, s,
-                                   w, cfg, cur, d, alpha);
+                                   w, cfg, cur, d, alpha, gm);
         } else if (cfg.useMIS) {
             hipLaunchKernelGGL((kWfShadeCont<true, false>), dim3(L.shadeGrid), b256, 0, st, s, w,
-                               cfg, cur, d, alpha);
+                               cfg, cur, d, alpha, 0u);
         } else {
             hipLaunchKernelGGL((kWfShadeCont<false, false>), dim3(L.shadeGrid), b256, 0, st, s,
-                               w, cfg, cur, d, alpha);
+                               w, cfg, cur, d, alpha, 0u);
         }
         cur = nxt;
     }

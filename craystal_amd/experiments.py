@@ -79,40 +79,40 @@ def extrapolate_time(curve, target, tail: float = 0.5):
     return a / target
 
 
-def guided_vs_plain(data, camd, size, cfg: Configs, ref: np.ndarray, target: float = 0.01,
-                    budget_s: float = 60.0, chunk: int = 8, train_fraction: float = 0.25,
-                    train_batch: int = 16384, train_steps: int = 4, lr: float = 1e-2,
-                    alpha: float = 0.5, max_spp: int = 1 << 20, seed: int = 7) -> dict:
-    """Plain megakernel and guided wavefront, each for budget_s seconds of
-    render time (the guided time includes its training steps, which run
-    during the first train_fraction of its budget); relMSE against ref after
-    every chunk of passes."""
-    scene = Scene(data)
-    sensor = Sensor(size)
-    cam = scenes.make_camera(camd, sensor)
-    scene.setCamera(cam)
-    plain = PathTraceIntegrator(Configs(**{**cfg.__dict__, "mode": L.MODE_MEGAKERNEL}))
+def plain_curve(scene: Scene, sensor: Sensor, cfg: Configs, ref: np.ndarray, budget_s: float,
+                chunk: int, mode: int = L.MODE_MEGAKERNEL, max_spp: int = 1 << 20):
+    """Progressive plain render (megakernel or wavefront) for budget_s
+    seconds of render time; relMSE against ref after every chunk."""
+    plain = PathTraceIntegrator(Configs(**{**cfg.__dict__, "mode": mode}))
     plain.dispatch(scene, 1, first_sample=1 << 22, readback=False)     # warm up
     sensor.clear()
 
-    def plain_chunk(first):
+    def run(first):
         st = plain.dispatch(scene, chunk, first_sample=first, readback=False)
         return chunk, st["kernel_ms"] / 1e3
 
-    pcurve = _progressive(plain_chunk, sensor, ref, budget_s, 0.0, max_spp)
+    return _progressive(run, sensor, ref, budget_s, 0.0, max_spp)
 
+
+def guided_curve(scene: Scene, sensor: Sensor, data, cfg: Configs, ref: np.ndarray,
+                 budget_s: float, chunk: int, train_fraction: float = 0.25,
+                 train_batch: int = 4096, train_steps: int = 4, lr: float = 1e-2,
+                 alpha: float = 0.5, max_depth: int = 0, materials: int = 0, seed: int = 7,
+                 max_spp: int = 1 << 20):
+    """Progressive guided wavefront render for budget_s seconds of render
+    time, the guide training during the first train_fraction of it (its
+    training steps count as render time).  Returns (curve, info)."""
     P = data.concatenated()[0]
     guide = Guide(P.min(0), P.max(0), seed=seed)
     gcfg = Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
     gi = GuidedIntegrator(gcfg, guide, alpha=alpha, lr=lr, train_batch=train_batch,
-                          train_steps=train_steps)
-    sensor.clear()
+                          train_steps=train_steps, max_depth=max_depth, materials=materials)
     gi.dispatch(scene, 1, first_sample=1 << 22, readback=False)       # warm up (no training)
     sensor.clear()
     train_until = train_fraction * budget_s
     state = {"t": 0.0, "train_s": 0.0}
 
-    def guided_chunk(first):
+    def run(first):
         train = state["t"] < train_until
         st = gi.dispatch(scene, chunk, first_sample=first, train=train, readback=False)
         dt = st["kernel_ms"] / 1e3
@@ -121,31 +121,45 @@ def guided_vs_plain(data, camd, size, cfg: Configs, ref: np.ndarray, target: flo
             state["train_s"] += dt
         return chunk, dt
 
-    gcurve = _progressive(guided_chunk, sensor, ref, budget_s, 0.0, max_spp)
-    t_plain, spp_plain = time_to_target(pcurve, target)
-    t_guided, spp_guided = time_to_target(gcurve, target)
-    x_plain, x_guided = extrapolate_time(pcurve, target), extrapolate_time(gcurve, target)
+    curve = _progressive(run, sensor, ref, budget_s, 0.0, max_spp)
+    return curve, {"train_seconds": round(state["train_s"], 3),
+                   "train_steps": guide.info()["steps"]}
+
+
+def summarize(curve, target: float, pixels: int) -> dict:
+    t_hit, spp_hit = time_to_target(curve, target)
+    return {"seconds_to_target": t_hit, "spp_to_target": spp_hit,
+            "extrapolated_seconds": extrapolate_time(curve, target),
+            "final_relmse": curve[-1][2], "final_spp": curve[-1][1],
+            "final_seconds": curve[-1][0],
+            "msamples_per_s": curve[-1][1] * pixels / curve[-1][0] / 1e6,
+            "curve": [(round(t, 4), k, round(e, 5)) for t, k, e in curve[::max(1, len(curve) // 24)]]}
+
+
+def guided_vs_plain(data, camd, size, cfg: Configs, ref: np.ndarray, target: float = 0.01,
+                    budget_s: float = 60.0, chunk: int = 8, plain_wavefront: bool = True,
+                    **guided_kw) -> dict:
+    """Plain megakernel, plain wavefront and guided wavefront, each for
+    budget_s seconds of render time; relMSE against ref after every chunk."""
+    scene = Scene(data)
+    sensor = Sensor(size)
+    scene.setCamera(scenes.make_camera(camd, sensor))
     px = size[0] * size[1]
-    out = {
-        "target_relmse": target, "budget_s": budget_s, "pixels": px,
-        "plain": {"seconds_to_target": t_plain, "spp_to_target": spp_plain,
-                  "extrapolated_seconds": x_plain, "final_relmse": pcurve[-1][2],
-                  "final_spp": pcurve[-1][1],
-                  "msamples_per_s": pcurve[-1][1] * px / pcurve[-1][0] / 1e6},
-        "guided": {"seconds_to_target": t_guided, "spp_to_target": spp_guided,
-                   "extrapolated_seconds": x_guided, "final_relmse": gcurve[-1][2],
-                   "final_spp": gcurve[-1][1], "train_seconds": round(state["train_s"], 3),
-                   "train_steps": guide.info()["steps"],
-                   "msamples_per_s": gcurve[-1][1] * px / gcurve[-1][0] / 1e6},
-        "curve_plain": [(round(t, 4), k, round(e, 5)) for t, k, e in pcurve[::max(1, len(pcurve) // 24)]],
-        "curve_guided": [(round(t, 4), k, round(e, 5)) for t, k, e in gcurve[::max(1, len(gcurve) // 24)]],
-    }
-    tg = gcurve[-1][0]
-    pe = [e for t, k, e in pcurve if t <= tg]
+    out = {"target_relmse": target, "budget_s": budget_s, "pixels": px}
+    pc = plain_curve(scene, sensor, cfg, ref, budget_s, chunk)
+    out["plain"] = summarize(pc, target, px)
+    if plain_wavefront:
+        wc = plain_curve(scene, sensor, cfg, ref, budget_s, chunk, mode=L.MODE_WAVEFRONT)
+        out["plain_wavefront"] = summarize(wc, target, px)
+    gc, info = guided_curve(scene, sensor, data, cfg, ref, budget_s, chunk, **guided_kw)
+    out["guided"] = {**summarize(gc, target, px), **info}
+    p, g = out["plain"], out["guided"]
+    if p["seconds_to_target"] and g["seconds_to_target"]:
+        out["speedup_measured"] = p["seconds_to_target"] / g["seconds_to_target"]
+    if p["extrapolated_seconds"] and g["extrapolated_seconds"]:
+        out["speedup_extrapolated"] = p["extrapolated_seconds"] / g["extrapolated_seconds"]
+    tg = gc[-1][0]
+    pe = [e for t, k, e in pc if t <= tg]
     out["equal_time"] = {"seconds": round(tg, 4), "plain_relmse": pe[-1] if pe else None,
-                         "guided_relmse": gcurve[-1][2]}
-    if t_plain and t_guided:
-        out["speedup_measured"] = t_plain / t_guided
-    if x_plain and x_guided:
-        out["speedup_extrapolated"] = x_plain / x_guided
+                         "guided_relmse": gc[-1][2]}
     return out

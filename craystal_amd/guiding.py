@@ -87,12 +87,18 @@ class GuidedIntegrator:
     (fraction alpha) and the guide; optionally trains the guide online on
     every pass's vertices (Adam, KL divergence)."""
 
+    LAMBERTIAN, PRINCIPLED = 1, 2
+
     def __init__(self, configs, guide: Guide, alpha: float = 0.5, lr: float = 1e-2,
-                 train_batch: int = 65536, train_steps: int = 4):
+                 train_batch: int = 4096, train_steps: int = 4, max_depth: int = 0,
+                 materials: int = 0):
+        """max_depth: guide only vertices at depth < max_depth (0 = all);
+        materials: mask of guided BSDFs (LAMBERTIAN | PRINCIPLED, 0 = both)."""
         self.configs = configs
         self.guide = guide
         self.alpha, self.lr = alpha, lr
         self.train_batch, self.train_steps = train_batch, train_steps
+        self.max_depth, self.materials = max_depth, materials
         self.last_stats: dict = {}
 
     def dispatch(self, scene, spp: int, first_sample: int = 0, train: bool = False,
@@ -105,6 +111,8 @@ class GuidedIntegrator:
         g.train_batch = self.train_batch
         g.train_steps = self.train_steps
         g.lr = self.lr
+        g.max_depth = self.max_depth
+        g.materials = self.materials
         st = L.crh_stats()
         cfg = self.configs.to_c()
         L.call("crh_render_guided", scene.handle, C.byref(cam.data), C.byref(cfg),

@@ -205,6 +205,8 @@ typedef struct crh_tuning {
     uint32_t refill;         /* ray batches: idle lanes per wave that trigger a refill, 1..64 */
     uint32_t pt_grid;        /* ray batches: blocks of the persistent kernels */
     uint32_t render_stack;   /* megakernel: stack as above */
+    uint32_t render_pool;    /* megakernel: paths per launch; sample indices per launch =
+                                max(1, render_pool / pixels), capped by the spp asked */
     uint32_t wf_stack;       /* wavefront traversal kernels: stack as above */
     uint32_t wf_trace_grid;  /* wavefront traversal kernels: blocks */
     uint32_t wf_trace_block; /* wavefront traversal kernels: threads (64..256) */
@@ -356,7 +358,10 @@ typedef struct crh_guided_cfg {
     uint32_t train_batch;
     uint32_t train_steps;
     float lr;
-    uint32_t reserved[3];
+    uint32_t max_depth;  /* guide only vertices at depth < max_depth (0 = every depth) */
+    uint32_t materials;  /* guided BSDFs: bit 0 Lambertian, bit 1 Principled (0 = both);
+                            the others sample their BSDF as the plain integrator does */
+    uint32_t reserved;
 } crh_guided_cfg;
 /* stats->node_visits receives the training samples the last pass group
  * produced; stats->launches the number of pass groups. */
