@@ -35,6 +35,11 @@ B_TRI = 52                 # 48 B index + vertices, 4 B primitive map
 B_FILM = 16 * 9            # (3 + 1) floats x up to 9 pixels
 
 
+def progress(msg: str):
+    """One line per finished leg on stderr (the JSON result goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,6 +49,10 @@ def parse():
     ap.add_argument("--trace-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-trace", action="store_true", help="skip the closest-hit batch")
+    ap.add_argument("--guided-seconds", type=float, default=20.0,
+                    help="render time of each leg of the guided-vs-plain comparison (0 = skip)")
+    ap.add_argument("--batch-scenes", type=int, default=64,
+                    help="scenes of the many-scene batch leg (0 = skip)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc runs that measure HBM traffic")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -77,6 +86,63 @@ def profile_kernels(cr, L, scenes):
     print(json.dumps({"copy_bytes": nbytes, "trace_rays": n,
                       "trace_node_visits": cnt["node_visits"], "trace_tri_tests": cnt["tri_tests"],
                       "render_samples": W * H, "render_rays": st["rays"]}), flush=True)
+
+
+def batch_leg(cr, scenes, n: int) -> dict:
+    """The many-scene batch: n seeded scenes (triangles log-uniform in [1k,
+    500k], resolutions 128^2..1024^2, 4/16/64 spp, max depth 12) rendered in
+    one persistent launch per sample index, replayed from a captured graph,
+    after the batch autotuner picked its tile shape, grid, ranges and stack.
+    Also times the same scenes rendered one after another."""
+    from craystal_amd import autotune
+    from craystal_amd.batch import Batch
+    items = [scenes.batch_scene(i) for i in range(n)]
+    t0 = time.perf_counter()
+    scene_list = [cr.Scene(d) for d, _ in items]
+    build_s = time.perf_counter() - t0
+    batch = Batch(scene_list)
+    info = batch.info()
+    sensors = [cr.Sensor(c["size"]) for _, c in items]
+    cams = [scenes.make_camera(c, s) for (_, c), s in zip(items, sensors)]
+    spp = [c["spp"] for _, c in items]
+    cfg = cr.Configs(maxDepth=12, rrThreshold=0.05, useMIS=True, rrStartDepth=3)
+    tuned = autotune.tune_batch(batch, cams, [1] * n, cfg, tile_ws=(8, 16, 32),
+                                grids=(1024, 2048), chunks=(1, 8), stacks=(1, 3), reps=1)
+    for s in sensors:
+        s.clear()
+    batch.render(cams, spp, cfg, first_sample=0, readback=False)     # captures the graph
+    for s in sensors:
+        s.clear()
+    st = batch.render(cams, spp, cfg, first_sample=0, readback=False)
+    samples = sum(c["size"][0] * c["size"][1] * k for (_, c), k in zip(items, spp))
+    solo_ms = 0.0
+    for sc, cam, k in zip(scene_list, cams, spp):
+        sc.setCamera(cam)
+        solo_ms += cr.PathTraceIntegrator(cfg).dispatch(sc, k, readback=False)["kernel_ms"]
+    return {"scenes": n, "triangles": info["triangles"], "pool_mb": round(info["pool_bytes"] / 2**20, 1),
+            "bvh_build_s": round(build_s, 2), "samples": samples,
+            "kernel_ms": round(st["kernel_ms"], 3),
+            "msamples_per_s": round(samples / (st["kernel_ms"] / 1e3) / 1e6, 2),
+            "grays_per_s": round(st["rays"] / (st["kernel_ms"] / 1e3) / 1e9, 4),
+            "solo_sequential_ms": round(solo_ms, 3),
+            "tuning": {k: tuned[k] for k in ("batch_tile_w", "batch_grid", "batch_chunks",
+                                              "batch_stack")}}
+
+
+def guided_leg(cr, scenes, seconds: float) -> dict:
+    """Time-to-relMSE 0.01 on the occluded-light room at 256 x 256 (max depth
+    16): the plain megakernel and the guided wavefront each render for
+    `seconds` of render time against the committed 65,536-spp plain image."""
+    from craystal_amd import experiments as X
+    ref = np.load(ROOT / "craystal_amd" / "data" / "room256_plain65536.npy")
+    data, camd = scenes.occluded_room()
+    cfg = cr.Configs(maxDepth=16, rrThreshold=0.05, useMIS=True, rrStartDepth=3)
+    out = X.guided_vs_plain(data, camd, (256, 256), cfg, ref, target=0.01, budget_s=seconds,
+                            chunk=64, plain_wavefront=False, train_batch=4096, train_steps=4,
+                            lr=1e-2, alpha=0.5, max_depth=3, materials=3)
+    for k in ("plain", "guided"):
+        out[k].pop("curve", None)
+    return out
 
 
 def measure_traffic() -> dict | None:
@@ -137,6 +203,7 @@ def main():
     traffic = None
     if not a.profile_kernels and not a.no_traffic:
         traffic = measure_traffic()
+        progress(f"traffic counters: {traffic}")
 
     import craystal_amd as cr
     from craystal_amd import _lib as L
@@ -167,9 +234,10 @@ def main():
     from craystal_amd import autotune
     tuned = autotune.tune_render(scene, cfg, spp=4, stacks=(1, 2, 3), wf_grids=(16384,),
                                  wf_blocks=(128, 256), wf_persistent=((2, 0),),
-                                 wf_pools=(1 << 21, 1 << 22, 1 << 23))
+                                 wf_pools=(1 << 21, 1 << 22, 1 << 23), render_waves=(1, 4))
     sensor.clear()
 
+    progress(f"scene built, autotuned: {tuned['render_stack']=} {tuned['wf_pool']=}")
     # warmup (untimed)
     if a.warmup > 0:
         integ.dispatch(scene, a.warmup, first_sample=0, readback=False)
@@ -237,6 +305,7 @@ def main():
                      "tri_tests_per_sample": round(tris_ps, 3)},
     }
 
+    progress(f"megakernel: {msps:.1f} Msamples/s")
     # wavefront integrator on the same workload and timed steps
     wf_cfg = cr.Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
     wf = cr.PathTraceIntegrator(wf_cfg)
@@ -249,13 +318,14 @@ def main():
             "HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of "
             "bench.py --profile-kernels (one 1920x1080 pass), scaled by a 256 MiB copy "
             f"(fetch x {traffic['fetch_scale']}, write x {traffic['write_scale']})")
-    out["tuning"] = {k: tuned[k] for k in ("render_stack", "wf_stack", "wf_grid",
+    out["tuning"] = {k: tuned[k] for k in ("render_stack", "render_waves", "wf_stack", "wf_grid",
                                            "wf_trace_block", "wf_persistent", "wf_refill",
                                            "wf_pool")}
     out["wavefront"] = {"msamples_per_s": round(samples / wel / 1e6, 3),
                         "ms_per_step": round(wel / a.steps * 1e3, 3),
                         "grays_per_s": round(wst["rays"] / wel / 1e9, 4)}
 
+    progress(f"wavefront: {out['wavefront']['msamples_per_s']} Msamples/s")
     rays = None
     if not a.no_trace:
         n = a.trace_rays
@@ -285,6 +355,15 @@ def main():
                          "traffic": traffic["kTraceClosest"] if traffic else None,
                          "bytes_per_ray": round(b_ray, 1)}}
         del rb, hb
+
+    if "closest_hit" in out:
+        progress(f"closest hit: {out['closest_hit']['grays_per_s']} Grays/s")
+    if a.batch_scenes > 0:
+        out["batch"] = batch_leg(cr, scenes, a.batch_scenes)
+        progress(f"batch: {out['batch']['msamples_per_s']} Msamples/s")
+    if a.guided_seconds > 0:
+        out["guided"] = guided_leg(cr, scenes, a.guided_seconds)
+        progress(f"guided: {out['guided']['equal_time']}")
 
     if not a.no_cpu:
         import oracle

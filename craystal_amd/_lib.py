@@ -95,7 +95,7 @@ class crh_integrator_cfg(C.Structure):
 
 class crh_tuning(C.Structure):
     _fields_ = [(k, u32) for k in ("stack", "trace_block", "persistent", "refill", "pt_grid",
-                                   "render_stack", "render_pool", "wf_stack", "wf_trace_grid", "wf_trace_block",
+                                   "render_stack", "render_pool", "render_waves", "wf_stack", "wf_trace_grid", "wf_trace_block",
                                    "wf_shade_grid", "wf_persistent", "wf_refill", "wf_pool",
                                    "batch_stack", "batch_tile_w", "batch_grid", "batch_chunks",
                                    "batch_graph")]
@@ -108,7 +108,7 @@ class crh_batch_info(C.Structure):
 
 class crh_guided_cfg(C.Structure):
     _fields_ = [("alpha", f32), ("train", u32), ("train_batch", u32), ("train_steps", u32),
-                ("lr", f32), ("max_depth", u32), ("materials", u32), ("reserved", u32)]
+                ("lr", f32), ("max_depth", u32), ("materials", u32), ("target", u32)]
 
 
 class crh_guide_info(C.Structure):

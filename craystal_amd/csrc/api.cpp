@@ -409,6 +409,8 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(t->persistent <= 2 && t->wf_persistent <= 2,
               "persistent must be 0 (keep), 1 (on) or 2 (off)");
     ARG_CHECK(t->refill <= 64 && t->wf_refill <= 64, "refill must be in [1, 64]");
+    ARG_CHECK(t->render_waves == 0 || t->render_waves == 1 || t->render_waves == 4,
+              "render_waves must be 0 (keep), 1 or 4");
     ARG_CHECK(t->wf_pool <= (1u << 28) && t->render_pool <= (1u << 28),
               "wf_pool and render_pool must be at most 2^28 paths");
     ARG_CHECK(okStack(t->batch_stack),
@@ -434,6 +436,7 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     if (t->wf_refill) o.wfRefill = t->wf_refill;
     if (t->wf_pool) o.wfPool = t->wf_pool;
     if (t->render_pool) o.renderPool = t->render_pool;
+    if (t->render_waves) o.renderWaves = t->render_waves;
     if (t->batch_stack) o.batchPriv = t->batch_stack - 1u;
     if (t->batch_tile_w) o.batchTileW = t->batch_tile_w;
     if (t->batch_grid) o.batchGrid = t->batch_grid;
@@ -460,6 +463,7 @@ int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
     t->wf_refill = o.wfRefill;
     t->wf_pool = o.wfPool;
     t->render_pool = o.renderPool;
+    t->render_waves = o.renderWaves;
     t->batch_stack = o.batchPriv + 1u;
     t->batch_tile_w = o.batchTileW;
     t->batch_grid = o.batchGrid;
@@ -1255,6 +1259,7 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
     ARG_CHECK(film->ctx == s->ctx && g->ctx == s->ctx, "objects belong to different contexts");
     ARG_CHECK(gc->alpha >= 0.f && gc->alpha <= 1.f, "alpha must be in [0, 1]");
     ARG_CHECK(gc->materials <= 3u, "materials must be a mask of bits 0 and 1");
+    ARG_CHECK(gc->target <= 1u, "target must be 0 (BSDF x radiance) or 1 (radiance)");
     ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
     ARG_CHECK(!gc->train || (gc->train_batch >= 64 && gc->train_batch <= (1u << 22)),
               "train_batch must be in [64, 2^22]");
@@ -1296,7 +1301,8 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
                                    false, ctx->stream, &g->view, gc->alpha,
                                    gc->train ? smpBuf + film->gPending : nullptr,
                                    uint32_t(film->gCap - film->gPending),
-                                   gc->materials ? gc->materials : 3u, gc->max_depth));
+                                   gc->materials ? gc->materials : 3u, gc->max_depth,
+                                   gc->target));
         for (uint32_t k = 0; k < active; ++k)
             HIP_TRY(crh::launchFilmGather(film->wf.rad + k * npix, film->wf.P + k * npix,
                                           film->w, film->h, film->sum.as<float>(),

@@ -18,7 +18,7 @@ struct Tuning {
     // Traversal stack mode of each kernel family: 0 LDS, 1 private, 2 short
     // LDS stack with private overflow.
     // ray-batch kernels (crh_trace_*)
-    uint32_t privStack = 1;
+    uint32_t privStack = 2;
     uint32_t traceBlock = 64;   // threads per block (64..256)
     uint32_t persistent = 0;    // 1: lanes refill with new rays from a work counter
     uint32_t refill = 16;       // idle lanes per wave that trigger a refill
@@ -26,14 +26,15 @@ struct Tuning {
     // megakernel
     uint32_t renderPriv = 0;
     // wavefront
-    uint32_t wfPriv = 0;
+    uint32_t wfPriv = 2;
     uint32_t wfTraceGrid = 16384;
     uint32_t wfTraceBlock = 128;
     uint32_t wfShadeGrid = 16384;
     uint32_t wfPersistent = 0;
     uint32_t wfRefill = 16;
-    uint32_t wfPool = 1u << 22;  // paths in flight (several samples per pixel for small images)
+    uint32_t wfPool = 1u << 23;  // paths in flight (several samples per pixel for small images)
     uint32_t renderPool = 1u << 21;  // megakernel: paths per launch, the same way
+    uint32_t renderWaves = 1;        // megakernels: minimum waves per SIMD (1 or 4)
     // batched rendering
     uint32_t batchPriv = 0;
     uint32_t batchTileW = 16;   // tile width in pixels; height = 256 / width
@@ -51,6 +52,13 @@ inline void withStack(uint32_t mode, F&& f) {
     if (mode == 1) f(std::integral_constant<int, 1>{});
     else if (mode == 2) f(std::integral_constant<int, 2>{});
     else f(std::integral_constant<int, 0>{});
+}
+// Register budget of the megakernels: 1 = the compiler's choice, 4 = at most
+// 128 VGPRs (4 waves per SIMD).  withWaves calls f(integral_constant<int, w>).
+template <class F>
+inline void withWaves(uint32_t w, F&& f) {
+    if (w == 4) f(std::integral_constant<int, 4>{});
+    else f(std::integral_constant<int, 1>{});
 }
 // LDS bytes per block of a stack mode.
 size_t stackLdsBytes(uint32_t mode, uint32_t stackCap, uint32_t block);

@@ -219,8 +219,10 @@ CRH_D void flushCounters(unsigned long long* counters, uint32_t rays, uint32_t n
 
 // Block b renders tile b % tiles of sample index si + b / tiles into slot
 // b / tiles of L and P (npix entries per slot).
-template <bool MIS, bool COUNT, int SM>
-__global__ __launch_bounds__(kRenderBlock) void kRenderPass(
+// W: minimum waves per SIMD the register allocation must allow (1 = the
+// compiler's choice; 4 caps the kernel at 128 VGPRs).
+template <bool MIS, bool COUNT, int SM, int W>
+__global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W, 8))) void kRenderPass(
     SceneView s, CameraView cam, IntegratorView cfg, uint32_t si,
     uint32_t tilesX, uint32_t tiles, float4* __restrict__ L, float2* __restrict__ P,
     unsigned long long* __restrict__ counters) {
@@ -312,8 +314,8 @@ __global__ void kBatchReset(uint32_t* __restrict__ next) {
 // hardware deals blocks to the 8 XCDs round-robin, so with 8 ranges each
 // XCD's L2 sees mostly its own scenes) and then moves on to the others.
 // Every block leaves after one failed take per range.
-template <bool MIS, int SM>
-__global__ __launch_bounds__(kRenderBlock) void kBatchRender(
+template <bool MIS, int SM, int W>
+__global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W, 8))) void kBatchRender(
     const BatchScene* __restrict__ tab, const uint32_t* __restrict__ tilePrefix,
     uint32_t nActive, IntegratorView cfg, uint32_t si, uint32_t tileW, uint32_t chunks,
     uint32_t* __restrict__ next, unsigned long long* __restrict__ counters) {
@@ -431,18 +433,21 @@ hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
     const dim3 grid(tiles * active), block(kRenderBlock);
     withStack(t.renderPriv, [&](auto sm) {
         constexpr int SM = decltype(sm)::value;
-        if (cfg.useMIS && countWork)
-            hipLaunchKernelGGL((kRenderPass<true, true, SM>), grid, block, lds, st, s, cam, cfg,
-                               si, tilesX, tiles, L, P, counters);
-        else if (cfg.useMIS)
-            hipLaunchKernelGGL((kRenderPass<true, false, SM>), grid, block, lds, st, s, cam, cfg,
-                               si, tilesX, tiles, L, P, counters);
-        else if (countWork)
-            hipLaunchKernelGGL((kRenderPass<false, true, SM>), grid, block, lds, st, s, cam, cfg,
-                               si, tilesX, tiles, L, P, counters);
-        else
-            hipLaunchKernelGGL((kRenderPass<false, false, SM>), grid, block, lds, st, s, cam, cfg,
-                               si, tilesX, tiles, L, P, counters);
+        withWaves(t.renderWaves, [&](auto wv) {
+            constexpr int W = decltype(wv)::value;
+            if (cfg.useMIS && countWork)
+                hipLaunchKernelGGL((kRenderPass<true, true, SM, W>), grid, block, lds, st, s, cam,
+                                   cfg, si, tilesX, tiles, L, P, counters);
+            else if (cfg.useMIS)
+                hipLaunchKernelGGL((kRenderPass<true, false, SM, W>), grid, block, lds, st, s, cam,
+                                   cfg, si, tilesX, tiles, L, P, counters);
+            else if (countWork)
+                hipLaunchKernelGGL((kRenderPass<false, true, SM, W>), grid, block, lds, st, s, cam,
+                                   cfg, si, tilesX, tiles, L, P, counters);
+            else
+                hipLaunchKernelGGL((kRenderPass<false, false, SM, W>), grid, block, lds, st, s,
+                                   cam, cfg, si, tilesX, tiles, L, P, counters);
+        });
     });
     return hipGetLastError();
 }
@@ -459,12 +464,15 @@ hipError_t launchBatchPass(const BatchScene* tab, const uint32_t* tilePrefix,
     const uint32_t tw = t.batchTileW, ch = t.batchChunks;
     withStack(t.batchPriv, [&](auto sm) {
         constexpr int SM = decltype(sm)::value;
-        if (cfg.useMIS)
-            hipLaunchKernelGGL((kBatchRender<true, SM>), grid, block, lds, st, tab, tilePrefix,
-                               nActive, cfg, si, tw, ch, next, counters);
-        else
-            hipLaunchKernelGGL((kBatchRender<false, SM>), grid, block, lds, st, tab, tilePrefix,
-                               nActive, cfg, si, tw, ch, next, counters);
+        withWaves(t.renderWaves, [&](auto wv) {
+            constexpr int W = decltype(wv)::value;
+            if (cfg.useMIS)
+                hipLaunchKernelGGL((kBatchRender<true, SM, W>), grid, block, lds, st, tab,
+                                   tilePrefix, nActive, cfg, si, tw, ch, next, counters);
+            else
+                hipLaunchKernelGGL((kBatchRender<false, SM, W>), grid, block, lds, st, tab,
+                                   tilePrefix, nActive, cfg, si, tw, ch, next, counters);
+        });
     });
     hipLaunchKernelGGL(kBatchGather, dim3((pixels + 255u) / 256u), dim3(256), 0, st, tab,
                        pixPrefix, nActive);

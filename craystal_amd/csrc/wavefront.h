@@ -88,6 +88,7 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                          const WfLaunch& L, bool countWork, hipStream_t st,
                          const GuideView* guide = nullptr, float alpha = 0.5f,
                          GuideSample* samples = nullptr, uint32_t sampleCap = 0,
-                         uint32_t guideMask = 3u, uint32_t guideDepth = 0u);
+                         uint32_t guideMask = 3u, uint32_t guideDepth = 0u,
+                         uint32_t guideTarget = 0u);
 
 }  // namespace crh

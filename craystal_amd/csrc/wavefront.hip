@@ -455,11 +455,13 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
 // Training samples from this pass's records: incident radiance along the
 // guided direction = (final radiance - radiance before that direction's
 // contributions) / throughput; target = luminance of (BSDF * cos) x that,
-// weight = target / sampling pdf.  Records whose path ended before the next
+// weight = target / sampling pdf.  With radianceTarget the target is the
+// mean of the incident radiance alone (no BSDF factor).  Records whose path ended before the next
 // vertex (no snapshot) or with zero target are skipped; at most cap samples
 // are written (the counter still counts them all).
 __global__ __launch_bounds__(256) void kWfRecordsToSamples(WfState w, GuideSample* out,
-                                                           uint32_t cap) {
+                                                           uint32_t cap,
+                                                           uint32_t radianceTarget) {
     const uint32_t n = min(w.counters[kCRec], w.recCap);
     PERSISTENT_LOOP(r, n) {
         const float4* R = w.rec + size_t(r) * kRecF4;
@@ -472,7 +474,9 @@ __global__ __launch_bounds__(256) void kWfRecordsToSamples(WfState w, GuideSampl
             float li[3] = {bw.x > 0.f ? (e.x - snap.x) / bw.x : 0.f,
                            bw.y > 0.f ? (e.y - snap.y) / bw.y : 0.f,
                            bw.z > 0.f ? (e.z - snap.z) / bw.z : 0.f};
-            float y = (f.x * li[0] + f.y * li[1] + f.z * li[2]) / 3.f;
+            float y = radianceTarget
+                          ? (li[0] + li[1] + li[2]) / 3.f
+                          : (f.x * li[0] + f.y * li[1] + f.z * li[2]) / 3.f;
             weight = y / R[0].w;
             ok = weight > 0.f && isfinite(weight);
         }
@@ -504,7 +508,8 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                          const IntegratorView& cfg, uint32_t si, uint32_t active, WfState& w,
                          const WfLaunch& L, bool countWork, hipStream_t st,
                          const GuideView* guide, float alpha, GuideSample* samples,
-                         uint32_t sampleCap, uint32_t guideMask, uint32_t guideDepth) {
+                         uint32_t sampleCap, uint32_t guideMask, uint32_t guideDepth,
+                         uint32_t guideTarget) {
     if (active == 0 || active > w.slots) return hipErrorInvalidValue;
     w.active = active;
     const uint32_t TB = L.traceBlock;
@@ -584,7 +589,7 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
     hipLaunchKernelGGL(kWfMisResolve, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
     if (guided && samples)
         hipLaunchKernelGGL(kWfRecordsToSamples, dim3(L.shadeGrid), b256, 0, st, w, samples,
-                           sampleCap);
+                           sampleCap, guideTarget);
     return hipGetLastError();
 }
 

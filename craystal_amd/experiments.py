@@ -97,8 +97,8 @@ def plain_curve(scene: Scene, sensor: Sensor, cfg: Configs, ref: np.ndarray, bud
 def guided_curve(scene: Scene, sensor: Sensor, data, cfg: Configs, ref: np.ndarray,
                  budget_s: float, chunk: int, train_fraction: float = 0.25,
                  train_batch: int = 4096, train_steps: int = 4, lr: float = 1e-2,
-                 alpha: float = 0.5, max_depth: int = 0, materials: int = 0, seed: int = 7,
-                 max_spp: int = 1 << 20):
+                 alpha: float = 0.5, max_depth: int = 0, materials: int = 0, target: int = 0,
+                 seed: int = 7, max_spp: int = 1 << 20):
     """Progressive guided wavefront render for budget_s seconds of render
     time, the guide training during the first train_fraction of it (its
     training steps count as render time).  Returns (curve, info)."""
@@ -106,7 +106,8 @@ def guided_curve(scene: Scene, sensor: Sensor, data, cfg: Configs, ref: np.ndarr
     guide = Guide(P.min(0), P.max(0), seed=seed)
     gcfg = Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
     gi = GuidedIntegrator(gcfg, guide, alpha=alpha, lr=lr, train_batch=train_batch,
-                          train_steps=train_steps, max_depth=max_depth, materials=materials)
+                          train_steps=train_steps, max_depth=max_depth, materials=materials,
+                          target=target)
     gi.dispatch(scene, 1, first_sample=1 << 22, readback=False)       # warm up (no training)
     sensor.clear()
     train_until = train_fraction * budget_s

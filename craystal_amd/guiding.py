@@ -91,14 +91,15 @@ class GuidedIntegrator:
 
     def __init__(self, configs, guide: Guide, alpha: float = 0.5, lr: float = 1e-2,
                  train_batch: int = 4096, train_steps: int = 4, max_depth: int = 0,
-                 materials: int = 0):
+                 materials: int = 0, target: int = 0):
         """max_depth: guide only vertices at depth < max_depth (0 = all);
-        materials: mask of guided BSDFs (LAMBERTIAN | PRINCIPLED, 0 = both)."""
+        materials: mask of guided BSDFs (LAMBERTIAN | PRINCIPLED, 0 = both);
+        target: 0 learns BSDF x cos x incident radiance, 1 incident radiance."""
         self.configs = configs
         self.guide = guide
         self.alpha, self.lr = alpha, lr
         self.train_batch, self.train_steps = train_batch, train_steps
-        self.max_depth, self.materials = max_depth, materials
+        self.max_depth, self.materials, self.target = max_depth, materials, target
         self.last_stats: dict = {}
 
     def dispatch(self, scene, spp: int, first_sample: int = 0, train: bool = False,
@@ -113,6 +114,7 @@ class GuidedIntegrator:
         g.lr = self.lr
         g.max_depth = self.max_depth
         g.materials = self.materials
+        g.target = self.target
         st = L.crh_stats()
         cfg = self.configs.to_c()
         L.call("crh_render_guided", scene.handle, C.byref(cam.data), C.byref(cfg),

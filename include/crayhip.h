@@ -207,6 +207,8 @@ typedef struct crh_tuning {
     uint32_t render_stack;   /* megakernel: stack as above */
     uint32_t render_pool;    /* megakernel: paths per launch; sample indices per launch =
                                 max(1, render_pool / pixels), capped by the spp asked */
+    uint32_t render_waves;   /* megakernel and batch: 1 = compiler's register budget,
+                                4 = at most 128 VGPRs so 4 waves fit per SIMD */
     uint32_t wf_stack;       /* wavefront traversal kernels: stack as above */
     uint32_t wf_trace_grid;  /* wavefront traversal kernels: blocks */
     uint32_t wf_trace_block; /* wavefront traversal kernels: threads (64..256) */
@@ -361,7 +363,8 @@ typedef struct crh_guided_cfg {
     uint32_t max_depth;  /* guide only vertices at depth < max_depth (0 = every depth) */
     uint32_t materials;  /* guided BSDFs: bit 0 Lambertian, bit 1 Principled (0 = both);
                             the others sample their BSDF as the plain integrator does */
-    uint32_t reserved;
+    uint32_t target;     /* what the guide learns: 0 = BSDF x cos x incident radiance,
+                            1 = incident radiance */
 } crh_guided_cfg;
 /* stats->node_visits receives the training samples the last pass group
  * produced; stats->launches the number of pass groups. */
