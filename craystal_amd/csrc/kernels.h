@@ -24,7 +24,7 @@ struct Tuning {
     uint32_t refill = 16;       // idle lanes per wave that trigger a refill
     uint32_t ptGrid = 2048;     // blocks of the persistent variant
     // megakernel
-    uint32_t renderPriv = 0;
+    uint32_t renderPriv = 2;
     // wavefront
     uint32_t wfPriv = 2;
     uint32_t wfTraceGrid = 16384;
@@ -34,9 +34,9 @@ struct Tuning {
     uint32_t wfRefill = 16;
     uint32_t wfPool = 1u << 23;  // paths in flight (several samples per pixel for small images)
     uint32_t renderPool = 1u << 21;  // megakernel: paths per launch, the same way
-    uint32_t renderWaves = 1;        // megakernels: minimum waves per SIMD (1 or 4)
+    uint32_t renderWaves = 4;        // megakernels: minimum waves per SIMD (1 or 4)
     // batched rendering
-    uint32_t batchPriv = 0;
+    uint32_t batchPriv = 2;
     uint32_t batchTileW = 16;   // tile width in pixels; height = 256 / width
     uint32_t batchGrid = 1024;  // persistent blocks
     uint32_t batchChunks = 8;   // item ranges with their own counters (1..8)

@@ -80,15 +80,16 @@ def extrapolate_time(curve, target, tail: float = 0.5):
 
 
 def plain_curve(scene: Scene, sensor: Sensor, cfg: Configs, ref: np.ndarray, budget_s: float,
-                chunk: int, mode: int = L.MODE_MEGAKERNEL, max_spp: int = 1 << 20):
+                chunk: int, mode: int = L.MODE_MEGAKERNEL, max_spp: int = 1 << 20,
+                first: int = 0):
     """Progressive plain render (megakernel or wavefront) for budget_s
     seconds of render time; relMSE against ref after every chunk."""
     plain = PathTraceIntegrator(Configs(**{**cfg.__dict__, "mode": mode}))
     plain.dispatch(scene, 1, first_sample=1 << 22, readback=False)     # warm up
     sensor.clear()
 
-    def run(first):
-        st = plain.dispatch(scene, chunk, first_sample=first, readback=False)
+    def run(done):
+        st = plain.dispatch(scene, chunk, first_sample=first + done, readback=False)
         return chunk, st["kernel_ms"] / 1e3
 
     return _progressive(run, sensor, ref, budget_s, 0.0, max_spp)
@@ -98,7 +99,7 @@ def guided_curve(scene: Scene, sensor: Sensor, data, cfg: Configs, ref: np.ndarr
                  budget_s: float, chunk: int, train_fraction: float = 0.25,
                  train_batch: int = 4096, train_steps: int = 4, lr: float = 1e-2,
                  alpha: float = 0.5, max_depth: int = 0, materials: int = 0, target: int = 0,
-                 seed: int = 7, max_spp: int = 1 << 20):
+                 seed: int = 7, max_spp: int = 1 << 20, first: int = 0):
     """Progressive guided wavefront render for budget_s seconds of render
     time, the guide training during the first train_fraction of it (its
     training steps count as render time).  Returns (curve, info)."""
@@ -113,9 +114,9 @@ def guided_curve(scene: Scene, sensor: Sensor, data, cfg: Configs, ref: np.ndarr
     train_until = train_fraction * budget_s
     state = {"t": 0.0, "train_s": 0.0}
 
-    def run(first):
+    def run(done):
         train = state["t"] < train_until
-        st = gi.dispatch(scene, chunk, first_sample=first, train=train, readback=False)
+        st = gi.dispatch(scene, chunk, first_sample=first + done, train=train, readback=False)
         dt = st["kernel_ms"] / 1e3
         state["t"] += dt
         if train:
