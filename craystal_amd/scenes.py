@@ -369,6 +369,32 @@ def batch_scene(index: int, seed: int = SCENE_SEED, tri_range=(1_000, 500_000),
     return data, cam
 
 
+def lampshade_room(light_radiance: float = 40.0):
+    """A 4 x 4 x 2.5 m diffuse room lit only indirectly: a 0.4 x 0.4 m
+    emitter 0.25 m below the ceiling faces up, a slightly larger diffuse
+    shade under it hides its back, so every surface the camera sees is lit
+    by the bright ceiling patch around the lamp.  All materials Lambertian.
+    Returns (SceneData, camera kwargs)."""
+    white, shade = diffuse((0.75, 0.75, 0.75)), diffuse((0.6, 0.6, 0.6))
+    mats = [white, shade, emitter((light_radiance,) * 3), diffuse((0.7, 0.3, 0.2))]
+    meshes = [
+        quad((-2, 0, 2), (4, 0, 0), (0, 0, -4), 0),        # floor, +y
+        quad((-2, 2.5, -2), (4, 0, 0), (0, 0, 4), 0),      # ceiling, -y
+        quad((-2, 0, -2), (4, 0, 0), (0, 2.5, 0), 0),      # z = -2, +z
+        quad((2, 0, 2), (-4, 0, 0), (0, 2.5, 0), 0),       # z = +2, -z
+        quad((-2, 0, 2), (0, 0, -4), (0, 2.5, 0), 3),      # x = -2, +x
+        quad((2, 0, -2), (0, 0, 4), (0, 2.5, 0), 0),       # x = +2, -x
+        quad((-0.2, 2.25, 0.2), (0.4, 0, 0), (0, 0, -0.4), 2),   # lamp, faces +y
+        quad((-0.3, 2.24, -0.3), (0.6, 0, 0), (0, 0, 0.6), 1),   # shade, faces -y
+        box_mesh((0.6, 0.4, -0.5), (0.8, 0.8, 0.8), 0.5, 0),
+    ]
+    data = SceneData(meshes=meshes, materials=mats)
+    data.emissiveIndex = _emissive_prims(data)
+    cam = dict(pos=(-1.6, 1.2, 1.7), target=(0.5, 0.8, -1.0), up=(0.0, 1.0, 0.0), fov=70.0,
+               size=(64, 64), spp=16, max_depth=8, rr_start_depth=3)
+    return data, cam
+
+
 def make_camera(cam: dict, sensor=None) -> Camera:
     c = Camera()
     c.setWorldPosition(cam["pos"])
