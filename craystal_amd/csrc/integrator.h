@@ -72,10 +72,12 @@ CRH_D LightGeom lightGeom(const SceneView& s, const Isect& it, Sampler& smp) {
 
 // The unoccluded light sample (Core/Integrator.cu:57-70).
 CRH_D LightSample lightFinish(const SceneView& s, const LightGeom& g,
-                              const BSDF& bsdf, const Isect& it) {
+                              const BSDF& bsdf, const Isect& it,
+                              const Lanes& w = rgbLanes()) {
     LightSample ls;
-    const GMaterial& m = s.mats[s.primMat[g.eid]];
-    ls.weight = bsdfEval(bsdf, it.view, g.ld) * sp3(m.emission);
+    const uint32_t mid = s.primMat[g.eid];
+    const GMaterial& m = s.mats[mid];
+    ls.weight = bsdfEval(bsdf, it.view, g.ld) * emissionOf(m, mid, w);
     ls.dir = g.ld;
     ls.pdf = g.d2 / (g.area * absDot(g.ld, g.ln));
     ls.id = g.eid;
@@ -111,10 +113,11 @@ CRH_D bool misBsdfRay(const BSDF& bsdf, Sampler& smp, const Isect& it, Ray& r,
 // result (`isHit` = the reference's intersect() return).  Adds into value.
 CRH_D void misBsdfPart(const SceneView& s, F3 pos, const Ray& r, bool isHit,
                        const Hit& h, uint32_t eid, Spec bw, float bp,
-                       Spec& value) {
+                       Spec& value, const Lanes& w = rgbLanes()) {
     if (!(isHit && h.prim == eid)) return;
     Isect lit = makeIsect(s, r, h);
-    const GMaterial& m = s.mats[s.primMat[h.prim]];
+    const uint32_t mid = s.primMat[h.prim];
+    const GMaterial& m = s.mats[mid];
     if (!(m.flags & 1u)) return;
     F3 toL = lit.pos - pos;
     float d2 = dot(toL, toL);
@@ -123,7 +126,7 @@ CRH_D void misBsdfPart(const SceneView& s, F3 pos, const Ray& r, bool isHit,
     if (cl != 0.0f) {
         float area = triArea(getTri(s, h.prim));
         float lpdf = d2 / (area * cl);
-        value = value + powerHeur(bp, lpdf) * sp3(m.emission) * bw / bp;
+        value = value + powerHeur(bp, lpdf) * emissionOf(m, mid, w) * bw / bp;
     }
 }
 

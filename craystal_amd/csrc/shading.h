@@ -31,6 +31,45 @@ CRH_D bool sbad(Spec a) {
            isinf(a.g) || isinf(a.b);
 }
 
+// --- Spectral transport -----------------------------------------------------
+// Three wavelength bins (of the 95 bins of 5 nm from 360 nm) per path ride in
+// the three lanes of Spec.  tab == nullptr: RGB transport.  The spectral
+// table (SceneView::spec) holds, in floats: the CIE table premultiplied for
+// the film (kSpecCmf: x, y, z, pad per bin), the environment spectrum
+// (kSpecEnv), then one block of kSpecMatStride per material: the averages of
+// its diffuse and specular spectra over all bins, then its diffuse, specular
+// and emission spectra.
+constexpr int kSpecBins = 95;
+constexpr int kSpecCmf = 0;
+constexpr int kSpecEnv = 4 * kSpecBins;            // 380
+constexpr int kSpecMat0 = kSpecEnv + 96;            // 476
+constexpr int kSpecMatStride = 296;
+constexpr int kSpecAvgKd = 0, kSpecAvgKs = 1, kSpecKd = 8, kSpecKs = 8 + kSpecBins,
+              kSpecEm = 8 + 2 * kSpecBins;
+struct Lanes {
+    const float* tab;
+    uint32_t b0, b1, b2;
+};
+CRH_D Lanes rgbLanes() { return Lanes{nullptr, 0u, 0u, 0u}; }
+// Bins h, h + 32, h + 64 (mod 95) for the hero bin h: each lane's bin is
+// uniform over the 95 bins.
+CRH_D Lanes heroLanes(const float* tab, float u) {
+    uint32_t h = min(uint32_t(u * float(kSpecBins)), uint32_t(kSpecBins - 1));
+    return Lanes{tab, h, (h + 32u) % kSpecBins, (h + 64u) % kSpecBins};
+}
+CRH_D Spec laneSpec(const float* p, const Lanes& w) { return {p[w.b0], p[w.b1], p[w.b2]}; }
+CRH_D const float* specMat(const Lanes& w, uint32_t mid) {
+    return w.tab + kSpecMat0 + size_t(mid) * kSpecMatStride;
+}
+// XYZ of the three lanes' radiance (the table folds in 95/3, 5 nm and the
+// normalisation that maps a constant spectrum 1 to Y = 1).
+CRH_D Spec laneXYZ(const Spec& L, const Lanes& w) {
+    const float4* c = reinterpret_cast<const float4*>(w.tab + kSpecCmf);
+    float4 a = c[w.b0], b = c[w.b1], d = c[w.b2];
+    return {(L.r * a.x + L.g * b.x) + L.b * d.x, (L.r * a.y + L.g * b.y) + L.b * d.y,
+            (L.r * a.z + L.g * b.z) + L.b * d.z};
+}
+
 // --- Triangle data (Scene/Triangle.cu:9-37, 78-92) -----------------------
 struct TriData {
     F3 p[3], n[3];
@@ -134,23 +173,26 @@ struct BSDF {
     uint32_t type;
 };
 
-CRH_D BSDF makeBSDF(const GMaterial& m, const Frame& f) {
+// avgKd / avgKs: the reference's Spectrum::averageValue of the reflectances
+// (over the three RGB channels, or over all 95 bins in spectral transport).
+CRH_D BSDF makeBSDFWith(const GMaterial& m, const Frame& f, Spec kd, Spec ks, float avgKd,
+                        float avgKs) {
     BSDF b;
     b.frame = f;
     b.type = m.type;
-    b.kd = sp3(m.kd);
-    b.ks = sp3(m.ks);
+    b.kd = kd;
+    b.ks = ks;
     b.n = m.shininess;
     b.pD = b.pS = 0.f;
     b.rough = b.metal = 0.f;
     if (m.type == 2u) {  // Phong (PhongBRDF.cu:10-19)
-        float d = smax(savg(b.kd), 0.0f), s = smax(savg(b.ks), 0.0f);
+        float d = smax(avgKd, 0.0f), s = smax(avgKs, 0.0f);
         float sum = smax(d + s, kEps);
         b.pD = d / sum;
         b.pS = s / sum;
     } else if (m.type == 1u) {  // Principled (PrincipledBRDF.cu:47-58)
         float r = sqrtf(2.0f / (b.n + 1.0f));
-        float mt = savg(b.ks);
+        float mt = avgKs;
         b.rough = fminf(fmaxf(r, 0.0f), 1.0f);
         b.metal = fminf(fmaxf(mt, 0.0f), 1.0f);
         float spr = 0.1f * (1.0f - b.metal) + 1.0f * b.metal;
@@ -159,6 +201,22 @@ CRH_D BSDF makeBSDF(const GMaterial& m, const Frame& f) {
         b.pS = spr / (pd + spr);
     }
     return b;
+}
+
+CRH_D BSDF makeBSDF(const GMaterial& m, const Frame& f) {
+    Spec kd = sp3(m.kd), ks = sp3(m.ks);
+    return makeBSDFWith(m, f, kd, ks, savg(kd), savg(ks));
+}
+
+// The material of primitive `mid` under the path's lanes.
+CRH_D BSDF makeBSDF(const GMaterial& m, uint32_t mid, const Frame& f, const Lanes& w) {
+    if (!w.tab) return makeBSDF(m, f);
+    const float* sm = specMat(w, mid);
+    return makeBSDFWith(m, f, laneSpec(sm + kSpecKd, w), laneSpec(sm + kSpecKs, w),
+                        sm[kSpecAvgKd], sm[kSpecAvgKs]);
+}
+CRH_D Spec emissionOf(const GMaterial& m, uint32_t mid, const Lanes& w) {
+    return w.tab ? laneSpec(specMat(w, mid) + kSpecEm, w) : sp3(m.emission);
 }
 
 // Phong (PhongBRDF.cu:21-68)
