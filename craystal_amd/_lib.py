@@ -78,7 +78,8 @@ class crh_scene_desc(C.Structure):
     _fields_ = [("meshes", C.POINTER(crh_mesh)), ("nmesh", u32),
                 ("materials", C.POINTER(crh_material)), ("nmaterial", u32),
                 ("emissive_index", C.POINTER(u32)), ("nemissive", u32),
-                ("env_radiance", f32 * 3), ("bvh", crh_bvh_params)]
+                ("env_radiance", f32 * 3), ("bvh", crh_bvh_params),
+                ("spectra", C.POINTER(f32)), ("env_spectrum", C.POINTER(f32))]
 
 
 class crh_camera(C.Structure):
@@ -90,7 +91,7 @@ class crh_camera(C.Structure):
 class crh_integrator_cfg(C.Structure):
     _fields_ = [("max_depth", u32), ("rr_threshold", f32), ("rr_start_depth", u32),
                 ("use_mis", u32), ("mode", u32), ("count_work", u32),
-                ("reserved", u32 * 2)]
+                ("spectral", u32), ("reserved", u32)]
 
 
 class crh_tuning(C.Structure):
@@ -175,6 +176,8 @@ SIGNATURES = {
     "crh_guide_get_info": [_P, C.POINTER(crh_guide_info)],
     "crh_guide_get_losses": [_P, _P, u32],
     "crh_guide_destroy": [_P],
+    "crh_cie_cmf": [_P],
+    "crh_spectrum_to_xyz": [_P, _P],
     "crh_batch_create": [_P, _P, u32, _P],
     "crh_batch_get_info": [_P, C.POINTER(crh_batch_info)],
     "crh_batch_render": [_P, _P, _P, _P, C.POINTER(crh_integrator_cfg), u32,

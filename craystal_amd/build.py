@@ -1,6 +1,6 @@
 """Build libcrayhip.so (gfx950 HIP kernels + C-ABI) in-tree.
 
-Host C++ (api.cpp, bvh_build.cpp) is compiled with g++ against the HIP
+Host C++ (api.cpp, bvh_build.cpp, spectral.cpp) is compiled with g++ against the HIP
 runtime headers; the kernels with hipcc for gfx950.  Everything is built with
 -ffp-contract=off so that traversal arithmetic is bit-identical to the
 host-side restatement of the reference.  Run as ``python -m craystal_amd.build``.
@@ -40,7 +40,7 @@ def _run(cmd: list[str]) -> None:
 def build(verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
     objs = []
-    for src in ("api.cpp", "bvh_build.cpp"):
+    for src in ("api.cpp", "bvh_build.cpp", "spectral.cpp"):
         o = OBJ / (src + ".o")
         _run(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
         objs.append(str(o))

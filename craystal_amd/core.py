@@ -19,6 +19,25 @@ import numpy as np
 
 from . import _lib as L
 
+SPECTRUM_BINS = 95          # 5 nm bins from 360 nm to 830 nm
+SPECTRUM_LAMBDA = 360.0 + 5.0 * np.arange(SPECTRUM_BINS)
+
+
+def cie_cmf() -> np.ndarray:
+    """(95, 3) x, y, z-bar used by spectral transport (crh_cie_cmf)."""
+    out = np.zeros((SPECTRUM_BINS, 3), np.float32)
+    L.call("crh_cie_cmf", L.ptr(out))
+    return out
+
+
+def spectrum_to_xyz(spectrum) -> np.ndarray:
+    """XYZ of a 95-bin spectrum; a constant 1 gives Y = 1."""
+    s = np.ascontiguousarray(spectrum, np.float32).reshape(SPECTRUM_BINS)
+    out = np.zeros(3, np.float32)
+    L.call("crh_spectrum_to_xyz", L.ptr(s), L.ptr(out))
+    return out
+
+
 # sRGB -> XYZ (Core/SpectrumTabData.h:63-65), used for the XYZ film output.
 SRGB_TO_XYZ = np.array([[0.412453, 0.357580, 0.180423],
                         [0.212671, 0.715160, 0.072169],
@@ -143,6 +162,10 @@ class SceneData:
     envRadiance: tuple = (0.0, 0.0, 0.0)
     # BVH parameters (bins, max leaf, max depth); zeros take the reference's.
     bvhParams: tuple = (32, 4, 20)
+    # Spectral transport (optional): (nmaterial, 3, 95) diffuse, specular and
+    # emission spectra, 5 nm bins from 360 nm; the environment's 95 bins.
+    spectra: object = None
+    envSpectrum: object = None
 
     @property
     def triangle_count(self) -> int:
@@ -188,6 +211,18 @@ class SceneData:
         d.nemissive = len(em)
         d.env_radiance[:] = [float(x) for x in self.envRadiance]
         d.bvh.bins, d.bvh.max_leaf, d.bvh.max_depth = self.bvhParams
+        if self.spectra is not None:
+            sp = np.ascontiguousarray(self.spectra, dtype=np.float32)
+            if sp.shape != (len(self.materials), 3, SPECTRUM_BINS):
+                raise ValueError(f"spectra must be (nmaterial, 3, {SPECTRUM_BINS})")
+            keep.append(sp)
+            d.spectra = sp.ctypes.data_as(C.POINTER(C.c_float))
+            if self.envSpectrum is not None:
+                env = np.ascontiguousarray(self.envSpectrum, dtype=np.float32).reshape(-1)
+                if env.shape != (SPECTRUM_BINS,):
+                    raise ValueError(f"envSpectrum must have {SPECTRUM_BINS} bins")
+                keep.append(env)
+                d.env_spectrum = env.ctypes.data_as(C.POINTER(C.c_float))
         return d, keep
 
 
@@ -409,6 +444,9 @@ class Configs:
     useMIS: bool = True
     rrStartDepth: int = 6
     mode: int = L.MODE_MEGAKERNEL
+    # Spectral transport: three of the 95 wavelength bins per path; the film
+    # then holds CIE XYZ (the scene needs SceneData.spectra).
+    spectral: bool = False
 
     @staticmethod
     def reference(useMIS: bool = True) -> "Configs":
@@ -421,6 +459,7 @@ class Configs:
         c.rr_start_depth = self.rrStartDepth
         c.use_mis = 1 if self.useMIS else 0
         c.mode = self.mode
+        c.spectral = 1 if self.spectral else 0
         return c
 
 

@@ -25,6 +25,7 @@
 #include "../../include/crayhip.h"
 #include "bvh_build.h"
 #include "kernels.h"
+#include "spectral.h"
 #include "wavefront.h"
 #include "guiding_kernels.h"
 
@@ -206,7 +207,7 @@ struct crh_bvh {
 struct crh_scene {
     CtxRef ctx;
     crh::BvhResult bvh;
-    DevMem nodes, tris, pos, nrm, uv, idx, primMat, mats, emissive;
+    DevMem nodes, tris, pos, nrm, uv, idx, primMat, mats, emissive, spec;
     crh::SceneView view{};
 };
 
@@ -686,6 +687,9 @@ int crh_scene_create(crh_context* ctx, const crh_scene_desc* d,
     if (e == hipSuccess) e = upload(s->primMat, primMat, st);
     if (e == hipSuccess) e = upload(s->mats, gm, st);
     if (e == hipSuccess) e = upload(s->emissive, em, st);
+    std::vector<float> specTab;
+    if (d->spectra) specTab = crh::buildSpectralTable(d->spectra, d->nmaterial, d->env_spectrum);
+    if (e == hipSuccess && d->spectra) e = upload(s->spec, specTab, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         delete s;
@@ -710,6 +714,7 @@ int crh_scene_create(crh_context* ctx, const crh_scene_desc* d,
     v.primMat = s->primMat.as<uint32_t>();
     v.mats = s->mats.as<crh::GMaterial>();
     v.emissive = s->emissive.as<uint32_t>();
+    v.spec = d->spectra ? s->spec.as<float>() : nullptr;
     *out = s;
     return CRH_OK;
     GUARD_END
@@ -951,6 +956,10 @@ int crh_render(crh_scene* s, const crh_camera* cam,
     ARG_CHECK(cfg->mode == CRH_MODE_MEGAKERNEL || cfg->mode == CRH_MODE_WAVEFRONT,
               "unknown integrator mode");
     ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
+    ARG_CHECK(!cfg->spectral || s->view.spec, "spectral transport needs a scene with spectra");
+    ARG_CHECK(!cfg->spectral || cfg->mode == CRH_MODE_MEGAKERNEL,
+              "spectral transport runs on the megakernel");
+    ARG_CHECK(!cfg->spectral || !cfg->count_work, "spectral transport has no counting variant");
     GUARD_BEGIN
     crh_context* ctx = s->ctx;
     if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -999,7 +1008,8 @@ int crh_render(crh_scene* s, const crh_camera* cam,
             HIP_TRY(crh::launchRenderPass(s->view, cv, iv, first_sample + i, active,
                                           film->L.as<float4>(), film->P.as<float2>(),
                                           counter.as<unsigned long long>(),
-                                          cfg->count_work != 0, tun, ctx->stream));
+                                          cfg->count_work != 0, tun, ctx->stream,
+                                          cfg->spectral != 0));
         if (stats) HIP_TRY(hipEventRecord(evs[2 * i + 1], ctx->stream));
         for (uint32_t k = 0; k < active; ++k) {
             const float4* Ls = (wavefront ? film->wf.rad : film->L.as<float4>()) + k * npix;
@@ -1258,6 +1268,7 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
     ARG_CHECK(s && cam && cfg && g && gc && film, "NULL argument");
     ARG_CHECK(film->ctx == s->ctx && g->ctx == s->ctx, "objects belong to different contexts");
     ARG_CHECK(gc->alpha >= 0.f && gc->alpha <= 1.f, "alpha must be in [0, 1]");
+    ARG_CHECK(!cfg->spectral, "guided rendering transports RGB");
     ARG_CHECK(gc->materials <= 3u, "materials must be a mask of bits 0 and 1");
     ARG_CHECK(gc->target <= 1u, "target must be 0 (BSDF x radiance) or 1 (radiance)");
     ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
@@ -1431,6 +1442,7 @@ int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh
         v.primMat = reinterpret_cast<const uint32_t*>(at(6));
         v.mats = reinterpret_cast<const crh::GMaterial*>(at(7));
         v.emissive = reinterpret_cast<const uint32_t*>(at(8));
+        v.spec = nullptr;  // batched rendering transports RGB
         b->views[i] = v;
         b->nodes += sc->nodes.bytes / sizeof(crh::GNode);
         b->tris += sc->tris.bytes / sizeof(crh::GTri);
@@ -1439,6 +1451,22 @@ int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh
     *out = b;
     return CRH_OK;
     GUARD_END
+}
+
+int crh_cie_cmf(float* xyz) {
+    ARG_CHECK(xyz, "NULL argument");
+    double c[3];
+    for (int i = 0; i < crh::kCieBins; ++i) {
+        crh::cieFit(crh::kCieLambda0 + crh::kCieStep * i, c);
+        for (int j = 0; j < 3; ++j) xyz[3 * i + j] = float(c[j]);
+    }
+    return CRH_OK;
+}
+
+int crh_spectrum_to_xyz(const float* spectrum, float* xyz) {
+    ARG_CHECK(spectrum && xyz, "NULL argument");
+    crh::spectrumToXYZ(spectrum, xyz);
+    return CRH_OK;
 }
 
 int crh_batch_get_info(const crh_batch* b, crh_batch_info* info) {
@@ -1457,6 +1485,7 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
                      crh_stats* stats) {
     ARG_CHECK(b && cams && films && spp && cfg, "NULL argument");
     ARG_CHECK(cfg->mode == CRH_MODE_MEGAKERNEL, "batched rendering uses the megakernel mode");
+    ARG_CHECK(!cfg->spectral, "batched rendering transports RGB");
     ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
     const uint32_t n = uint32_t(b->views.size());
     uint32_t maxSpp = 0;

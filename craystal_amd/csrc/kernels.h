@@ -74,7 +74,8 @@ hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
 hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
                             const IntegratorView& cfg, uint32_t si, uint32_t active, float4* L,
                             float2* P, unsigned long long* counters,
-                            bool countWork, const Tuning& t, hipStream_t st);
+                            bool countWork, const Tuning& t, hipStream_t st,
+                            bool spectral = false);
 // One sample index over the first nActive scenes of the table: persistent
 // megakernel over `items` tiles, then the film gather over `pixels` pixels.
 // next: kBatchMaxChunks device counters.

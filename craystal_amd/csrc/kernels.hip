@@ -107,18 +107,18 @@ CRH_D bool occluded(PathCtx<COUNT, Stack>& c, const Ray& r) {
 // sampleLight (Core/Integrator.cu:21-72)
 template <bool COUNT, class Stack>
 CRH_D LightSample sampleLight(PathCtx<COUNT, Stack>& c, const Isect& it,
-                              const BSDF& bsdf, Sampler& smp) {
+                              const BSDF& bsdf, Sampler& smp, const Lanes& w) {
     LightGeom g = lightGeom(c.s, it, smp);
     if (!g.valid) return noLight();
     if (occluded(c, g.shadow)) return noLight();
-    return lightFinish(c.s, g, bsdf, it);
+    return lightFinish(c.s, g, bsdf, it, w);
 }
 
 // evalMIS (Core/Integrator.cu:74-140)
 template <bool COUNT, class Stack>
 CRH_D Spec evalMIS(PathCtx<COUNT, Stack>& c, const Isect& it, const BSDF& bsdf,
-                   Sampler& smp) {
-    LightSample ls = sampleLight(c, it, bsdf, smp);
+                   Sampler& smp, const Lanes& w) {
+    LightSample ls = sampleLight(c, it, bsdf, smp, w);
     if (ls.pdf <= 0.0f) return sp(0.f);
     Spec value = misLightPart(ls, bsdf, it);
     Ray r;
@@ -127,7 +127,7 @@ CRH_D Spec evalMIS(PathCtx<COUNT, Stack>& c, const Isect& it, const BSDF& bsdf,
     if (misBsdfRay(bsdf, smp, it, r, bw, bp)) {
         Hit h;
         bool isHit = closest(c, r, h);
-        misBsdfPart(c.s, it.pos, r, isHit, h, ls.id, bw, bp, value);
+        misBsdfPart(c.s, it.pos, r, isHit, h, ls.id, bw, bp, value, w);
     }
     return value / ls.selPdf;
 }
@@ -143,7 +143,9 @@ CRH_D void tileCoord(uint32_t tile, uint32_t tilesX, uint32_t& x, uint32_t& y) {
 // pathTraceKernel (Core/Integrator.cu:142-227) for one pixel: radiance of
 // sample index `si` through pixel (x, y) into L[id] (float4, w unused) and
 // the jittered film position into P[id]; adds the rays traced to `rays`.
-template <bool MIS, bool COUNT, class Stack>
+// SPEC: spectral transport; the path draws its hero wavelength bin right
+// after the pixel jitter, and L receives CIE XYZ instead of RGB.
+template <bool MIS, bool COUNT, bool SPEC, class Stack>
 CRH_D void tracePixel(const SceneView& s, Stack& stk, const CameraView& cam,
                       const IntegratorView& cfg, uint32_t x, uint32_t y, uint32_t si,
                       float4* __restrict__ L, float2* __restrict__ P, uint32_t& rays,
@@ -152,6 +154,7 @@ CRH_D void tracePixel(const SceneView& s, Stack& stk, const CameraView& cam,
     Sampler smp(x, y, si);
     F2 j = smp.next2D();
     F2 pix{float(x) + j.x, float(y) + j.y};
+    const Lanes w = SPEC ? heroLanes(s.spec, smp.next1D()) : rgbLanes();
     Ray ray = cameraRay(cam, pix);
     Spec rad = sp(0.f), beta = sp(1.f);
     for (uint32_t depth = 0; depth < cfg.maxDepth; ++depth) {
@@ -159,16 +162,17 @@ CRH_D void tracePixel(const SceneView& s, Stack& stk, const CameraView& cam,
         Hit h;
         if (closest(c, ray, h)) {
             Isect it = makeIsect(s, ray, h);
-            const GMaterial& m = s.mats[s.primMat[h.prim]];
+            const uint32_t mid = s.primMat[h.prim];
+            const GMaterial& m = s.mats[mid];
             if (m.flags & 1u) {
-                rad = rad + sp3(m.emission) * beta;
+                rad = rad + emissionOf(m, mid, w) * beta;
                 break;
             }
-            BSDF bsdf = makeBSDF(m, it.frame);
+            BSDF bsdf = makeBSDF(m, mid, it.frame, w);
             if (MIS) {
-                rad = rad + evalMIS(c, it, bsdf, smp) * beta;
+                rad = rad + evalMIS(c, it, bsdf, smp, w) * beta;
             } else {
-                LightSample ls = sampleLight(c, it, bsdf, smp);
+                LightSample ls = sampleLight(c, it, bsdf, smp, w);
                 if (ls.pdf > 0.0f)
                     rad = rad + ls.weight * beta / ls.pdf / ls.selPdf;
             }
@@ -180,13 +184,14 @@ CRH_D void tracePixel(const SceneView& s, Stack& stk, const CameraView& cam,
             ray.o = offsetRayOrigin(it.pos, oriented(it));
             ray.d = wi;
         } else {
-            rad = rad + sp3(s.env) * beta;
+            rad = rad + (SPEC ? laneSpec(s.spec + kSpecEnv, w) : sp3(s.env)) * beta;
             terminate = true;
         }
         if (smp.next1D() < cfg.rrThreshold) terminate = true;
         if (terminate) break;
         if (!roulette(cfg, depth, smp, beta)) break;
     }
+    if (SPEC) rad = laneXYZ(rad, w);
     size_t id = size_t(y) * cam.width + x;
     L[id] = make_float4(rad.r, rad.g, rad.b, 0.f);
     P[id] = make_float2(pix.x, pix.y);
@@ -221,7 +226,7 @@ CRH_D void flushCounters(unsigned long long* counters, uint32_t rays, uint32_t n
 // b / tiles of L and P (npix entries per slot).
 // W: minimum waves per SIMD the register allocation must allow (1 = the
 // compiler's choice; 4 caps the kernel at 128 VGPRs).
-template <bool MIS, bool COUNT, int SM, int W>
+template <bool MIS, bool COUNT, int SM, int W, bool SPEC = false>
 __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W, 8))) void kRenderPass(
     SceneView s, CameraView cam, IntegratorView cfg, uint32_t si,
     uint32_t tilesX, uint32_t tiles, float4* __restrict__ L, float2* __restrict__ P,
@@ -235,7 +240,8 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W,
     WorkCount wc{0, 0};
     if (x < cam.width && y < cam.height) {
         auto stk = makeStack<SM>(lds);
-        tracePixel<MIS, COUNT>(s, stk, cam, cfg, x, y, si + slot, L + off, P + off, rays, wc);
+        tracePixel<MIS, COUNT, SPEC>(s, stk, cam, cfg, x, y, si + slot, L + off, P + off, rays,
+                                     wc);
     }
     flushCounters<COUNT>(counters, rays, wc.nodes, wc.tris);
 }
@@ -343,7 +349,7 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W,
             const uint32_t x = (tile % b.tilesX) * tileW + threadIdx.x % tileW;
             const uint32_t y = (tile / b.tilesX) * tileH + threadIdx.x / tileW;
             if (x < b.cam.width && y < b.cam.height)
-                tracePixel<MIS, false>(b.view, stk, b.cam, cfg, x, y, si,
+                tracePixel<MIS, false, false>(b.view, stk, b.cam, cfg, x, y, si,
                                        reinterpret_cast<float4*>(b.L),
                                        reinterpret_cast<float2*>(b.P), rays, wc);
         }
@@ -425,7 +431,8 @@ hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
 hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
                             const IntegratorView& cfg, uint32_t si, uint32_t active, float4* L,
                             float2* P, unsigned long long* counters,
-                            bool countWork, const Tuning& t, hipStream_t st) {
+                            bool countWork, const Tuning& t, hipStream_t st, bool spectral) {
+    if (spectral && (!s.spec || countWork)) return hipErrorInvalidValue;
     const uint32_t tilesX = (cam.width + 15u) / 16u, tilesY = (cam.height + 15u) / 16u;
     const uint32_t tiles = tilesX * tilesY;
     if (uint64_t(tiles) * active >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
@@ -435,7 +442,13 @@ hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
         constexpr int SM = decltype(sm)::value;
         withWaves(t.renderWaves, [&](auto wv) {
             constexpr int W = decltype(wv)::value;
-            if (cfg.useMIS && countWork)
+            if (spectral && cfg.useMIS)
+                hipLaunchKernelGGL((kRenderPass<true, false, SM, W, true>), grid, block, lds, st, s,
+                                   cam, cfg, si, tilesX, tiles, L, P, counters);
+            else if (spectral)
+                hipLaunchKernelGGL((kRenderPass<false, false, SM, W, true>), grid, block, lds, st,
+                                   s, cam, cfg, si, tilesX, tiles, L, P, counters);
+            else if (cfg.useMIS && countWork)
                 hipLaunchKernelGGL((kRenderPass<true, true, SM, W>), grid, block, lds, st, s, cam,
                                    cfg, si, tilesX, tiles, L, P, counters);
             else if (cfg.useMIS)

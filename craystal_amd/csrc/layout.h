@@ -75,6 +75,9 @@ struct SceneView {
     const uint32_t* emissive;
     float env[3];
     uint32_t pad;
+    // Spectral transport table (layout in shading.h, kSpec*), null when the
+    // scene has no spectra.
+    const float* spec;
 };
 
 struct CameraView {

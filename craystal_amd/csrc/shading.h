@@ -10,6 +10,7 @@
 // Spectrum = RGB (Core/Spectrum.h:326).
 #pragma once
 #include "traverse.h"
+#include "spec_layout.h"
 
 namespace crh {
 
@@ -32,20 +33,9 @@ CRH_D bool sbad(Spec a) {
 }
 
 // --- Spectral transport -----------------------------------------------------
-// Three wavelength bins (of the 95 bins of 5 nm from 360 nm) per path ride in
-// the three lanes of Spec.  tab == nullptr: RGB transport.  The spectral
-// table (SceneView::spec) holds, in floats: the CIE table premultiplied for
-// the film (kSpecCmf: x, y, z, pad per bin), the environment spectrum
-// (kSpecEnv), then one block of kSpecMatStride per material: the averages of
-// its diffuse and specular spectra over all bins, then its diffuse, specular
-// and emission spectra.
-constexpr int kSpecBins = 95;
-constexpr int kSpecCmf = 0;
-constexpr int kSpecEnv = 4 * kSpecBins;            // 380
-constexpr int kSpecMat0 = kSpecEnv + 96;            // 476
-constexpr int kSpecMatStride = 296;
-constexpr int kSpecAvgKd = 0, kSpecAvgKs = 1, kSpecKd = 8, kSpecKs = 8 + kSpecBins,
-              kSpecEm = 8 + 2 * kSpecBins;
+// Three wavelength bins (of the kSpecBins bins) per path ride in the three
+// lanes of Spec; Lanes::tab points at the scene's spectral table
+// (spec_layout.h), nullptr for RGB transport.
 struct Lanes {
     const float* tab;
     uint32_t b0, b1, b2;

@@ -1,0 +1,18 @@
+// Layout of the per-scene spectral transport table (host and device), in
+// floats: the film's CIE weights (x, y, z, pad per bin), the environment
+// spectrum, then one block per material: the averages of its diffuse and
+// specular spectra over all bins, then its diffuse, specular and emission
+// spectra.  Bins are 5 nm wide from 360 nm to 830 nm.
+#pragma once
+
+namespace crh {
+
+constexpr int kSpecBins = 95;
+constexpr int kSpecCmf = 0;
+constexpr int kSpecEnv = 4 * kSpecBins;  // 380
+constexpr int kSpecMat0 = kSpecEnv + 96;  // 476
+constexpr int kSpecMatStride = 296;
+constexpr int kSpecAvgKd = 0, kSpecAvgKs = 1, kSpecKd = 8, kSpecKs = 8 + kSpecBins,
+              kSpecEm = 8 + 2 * kSpecBins;
+
+}  // namespace crh

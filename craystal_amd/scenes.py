@@ -238,6 +238,41 @@ def cornell_box(seed: int = SCENE_SEED, light_radiance=(17.0, 12.0, 4.0)):
     return data, cam
 
 
+def smooth_spectrum(kind: str, scale: float = 1.0) -> np.ndarray:
+    """Smooth 95-bin reflectance curves over 360-830 nm for the spectral
+    Cornell box: 'white' (nearly flat), 'red' (a logistic edge at 600 nm),
+    'green' (a bump at 530 nm), 'flat'."""
+    lam = 360.0 + 5.0 * np.arange(95)
+    if kind == "white":
+        v = 0.73 + 0.03 * np.sin((lam - 360.0) / 470.0 * np.pi)
+    elif kind == "red":
+        v = 0.05 + 0.6 / (1.0 + np.exp(-(lam - 600.0) / 15.0))
+    elif kind == "green":
+        v = 0.06 + 0.45 * np.exp(-0.5 * ((lam - 530.0) / 35.0) ** 2)
+    elif kind == "flat":
+        v = np.ones_like(lam)
+    else:
+        raise ValueError(kind)
+    return (v * scale).astype(np.float32)
+
+
+def cornell_box_spectral(seed: int = SCENE_SEED, light_radiance: float = 12.0):
+    """The Cornell-style box with spectral materials: walls and boxes with
+    smooth reflectance curves (white, red, green), a constant-spectrum
+    emitter.  The RGB fields hold each spectrum's mean (used by RGB
+    transport only).  Returns (SceneData, camera kwargs)."""
+    data, cam = cornell_box(seed, light_radiance=(light_radiance,) * 3)
+    kinds = ["white", "red", "green"]
+    spectra = np.zeros((len(data.materials), 3, 95), np.float32)
+    for i, k in enumerate(kinds):
+        spectra[i, 0] = smooth_spectrum(k)
+        m = float(spectra[i, 0].mean())
+        data.materials[i] = diffuse((m, m, m))
+    spectra[3, 2] = smooth_spectrum("flat", light_radiance)
+    data.spectra = spectra
+    return data, cam
+
+
 def spheres_scene(n_spheres: int = 48, subdiv: int = 5, seed: int = SCENE_SEED,
                   env=(0.05, 0.06, 0.08), light_radiance=(60.0, 60.0, 60.0)):
     """About 1.0M triangles: 48 icospheres (subdivision 5) with radii U[0.2, 1]

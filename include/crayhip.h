@@ -144,6 +144,13 @@ typedef struct crh_scene_desc {
     uint32_t nemissive;
     float env_radiance[3];
     crh_bvh_params bvh;
+    /* Spectral transport (optional, no counterpart in the reference's RGB
+     * integrator): nmaterial x 3 x 95 floats (diffuse, specular, emission
+     * spectra of each material, bins of 5 nm from 360 nm to 830 nm) and the
+     * environment's 95-bin spectrum (may be NULL = 0).  NULL spectra: the
+     * scene renders in RGB only. */
+    const float* spectra;
+    const float* env_spectrum;
 } crh_scene_desc;
 
 /* Camera: the reference CameraProxy layout, 96 B (Core/CameraProxy.h:10-24).
@@ -180,7 +187,10 @@ typedef struct crh_integrator_cfg {
     uint32_t mode;
     uint32_t count_work; /* != 0: run the counting variant (fills
                             node_visits / tri_tests; slower) */
-    uint32_t reserved[2];
+    uint32_t spectral;   /* != 0: spectral transport (megakernel, scenes created
+                            with spectra): each path carries three of the 95
+                            wavelength bins and the film receives CIE XYZ */
+    uint32_t reserved;
 } crh_integrator_cfg;
 
 /* Timing/counters returned by the render and trace entry points. */
@@ -372,6 +382,14 @@ int crh_render_guided(crh_scene* scene, const crh_camera* cam,
                       const crh_integrator_cfg* cfg, crh_guide* guide,
                       const crh_guided_cfg* gcfg, uint32_t first_sample,
                       uint32_t spp, crh_film* film, crh_stats* stats);
+
+/* ---- spectra (no reference counterpart is used by its integrator;
+ * BroadSpectrum::toXYZ, Core/Spectrum.cpp:36-44, is the analogue) -------- */
+/* The colour matching functions used: x, y, z-bar at 360 + 5 i nm, i < 95
+ * (an analytic fit, see craystal_amd/csrc/spectral.h), 285 floats. */
+int crh_cie_cmf(float* xyz);
+/* XYZ of a 95-bin spectrum, normalised so that a constant 1 has Y = 1. */
+int crh_spectrum_to_xyz(const float* spectrum, float* xyz);
 
 /* ---- batched rendering (no counterpart in the reference, which renders
  * one scene per PathTraceIntegrator::dispatch, Core/Integrator.cu:233-262) - */

@@ -41,6 +41,7 @@ def lib() -> C.CDLL:
             "orc_bvh_export": (None, [P, P, P]),
             "orc_bvh_free": (None, [P]),
             "orc_scene_create": (P, [P]),
+            "orc_cie_cmf": (None, [P]),
             "orc_scene_free": (None, [P]),
             "orc_scene_bvh_stats": (None, [P, P]),
             "orc_scene_bvh_export": (None, [P, P, P]),

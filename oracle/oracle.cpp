@@ -150,6 +150,21 @@ struct BVHOut {
     double buildMs = 0;
 };
 
+// CIE 1931 colour matching functions at lambda (nm): the piecewise-Gaussian
+// fit of Wyman, Sloan and Shirley, "Simple Analytic Approximations to the CIE
+// XYZ Color Matching Functions", JCGT 2(2), 2013 (coefficients of its
+// multi-lobe fit).
+inline void cieXYZ(double l, double out[3]) {
+    auto g = [](double x, double mu, double s1, double s2) {
+        double t = (x - mu) / (x < mu ? s1 : s2);
+        return std::exp(-0.5 * t * t);
+    };
+    out[0] = 1.056 * g(l, 599.8, 37.9, 31.0) + 0.362 * g(l, 442.0, 16.0, 26.7) -
+             0.065 * g(l, 501.1, 20.4, 26.2);
+    out[1] = 0.821 * g(l, 568.8, 46.9, 40.5) + 0.286 * g(l, 530.9, 16.3, 31.1);
+    out[2] = 1.217 * g(l, 437.0, 11.8, 36.0) + 0.681 * g(l, 459.0, 26.0, 13.8);
+}
+
 struct Scene {
     std::vector<V3> pos, nrm;
     std::vector<V2> uv;
@@ -160,6 +175,12 @@ struct Scene {
     std::vector<uint32_t> emissive;
     Spec env;
     BVHOut bvh;
+    // Spectral transport (this build's addition): per material 3 x 95 bins
+    // (diffuse, specular, emission), their bin averages, the environment
+    // spectrum and the film's per-lane CIE weights.
+    bool spectral = false;
+    std::vector<float> spectra, envSpec, avgKd, avgKs;
+    float filmW[95][3] = {};
 };
 
 // ---------------------------------------------------------------------------
@@ -566,20 +587,28 @@ struct BSDF {
     Frame frame;
 
     void init(const crh_material& m, const Frame& f) {  // BSDF.cu:6-32
+        Spec d(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+        Spec sp(m.specular[0], m.specular[1], m.specular[2]);
+        initWith(m, f, d, sp, d.averageValue(), sp.averageValue());
+    }
+    // avgKd / avgKs: averageValue of the reflectances (all 95 bins in
+    // spectral transport, where kd / ks hold the path's three bins).
+    void initWith(const crh_material& m, const Frame& f, Spec d, Spec sp, float avgKd,
+                  float avgKs) {
         frame = f;
         type = m.type;
-        kd = Spec(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
-        ks = Spec(m.specular[0], m.specular[1], m.specular[2]);
+        kd = d;
+        ks = sp;
         n = m.shininess;
         if (type == CRH_MAT_PHONG) {
-            pD = fmax_std(kd.averageValue(), 0.0f);
-            pS = fmax_std(ks.averageValue(), 0.0f);
+            pD = fmax_std(avgKd, 0.0f);
+            pS = fmax_std(avgKs, 0.0f);
             float sum = fmax_std(pD + pS, kEps);
             pD /= sum;
             pS /= sum;
         } else if (type == CRH_MAT_PRINCIPLED) {
             float r = std::sqrt(2.0f / (n + 1.0f));
-            float mt = ks.averageValue();
+            float mt = avgKs;
             rough = std::clamp(r, 0.0f, 1.0f);
             metal = std::clamp(mt, 0.0f, 1.0f);
             float sp = 0.1f * (1.0f - metal) + 1.0f * metal;  // lerp(0.1,1,m)
@@ -802,7 +831,19 @@ struct Tally {
 inline const crh_material& matOf(const Scene& s, uint32_t prim) {
     return s.mats[s.meshMat[s.primMesh[prim]]];
 }
-inline Spec emissionOf(const crh_material& m) {
+inline uint32_t matIdOf(const Scene& s, uint32_t prim) { return s.meshMat[s.primMesh[prim]]; }
+
+// Spectral transport: a path carries the bins h, h + 32, h + 64 (mod 95) of
+// its hero bin h in the three Spec channels.
+struct Wl {
+    bool on = false;
+    uint32_t b[3] = {0, 0, 0};
+};
+inline Spec lanesOf(const float* p, const Wl& w) { return Spec(p[w.b[0]], p[w.b[1]], p[w.b[2]]); }
+inline Spec emissionOf(const Scene& s, uint32_t prim, const Wl& w) {
+    if (w.on)
+        return lanesOf(s.spectra.data() + (size_t(matIdOf(s, prim)) * 3 + 2) * 95, w);
+    const crh_material& m = matOf(s, prim);
     return Spec(m.emission[0], m.emission[1], m.emission[2]);
 }
 
@@ -821,7 +862,7 @@ bool occluded(const Scene& s, const Ray& r, Tally& tl) {
 }
 
 LightSample sampleLight(const Scene& s, const Isect& it, const BSDF& bsdf,
-                        Sampler& smp, Tally& tl) {  // :21-72
+                        Sampler& smp, Tally& tl, const Wl& wl) {  // :21-72
     int cnt = int(s.emissive.size());
     if (cnt == 0) return {};
     int sel = std::min<int>(int(float(cnt) * smp.next1D()), cnt - 1);
@@ -842,9 +883,8 @@ LightSample sampleLight(const Scene& s, const Isect& it, const BSDF& bsdf,
     sr.d = ld;
     sr.tMax = length(sr.o - target) - kEps;
     if (occluded(s, sr, tl)) return {};
-    const crh_material& m = matOf(s, eid);
     LightSample ls;
-    ls.weight = bsdf.evaluate(it.viewW, ld) * emissionOf(m);
+    ls.weight = bsdf.evaluate(it.viewW, ld) * emissionOf(s, eid, wl);
     ls.dirW = ld;
     ls.pdf = d2 / (tri.area() * absDot(ld, vd.n));
     ls.emissiveID = eid;
@@ -853,9 +893,9 @@ LightSample sampleLight(const Scene& s, const Isect& it, const BSDF& bsdf,
 }
 
 Spec evalMIS(const Scene& s, const Isect& it, const BSDF& bsdf, Sampler& smp,
-             Tally& tl) {  // :74-140
+             Tally& tl, const Wl& wl) {  // :74-140
     Spec value(0.0f);
-    LightSample ls = sampleLight(s, it, bsdf, smp, tl);
+    LightSample ls = sampleLight(s, it, bsdf, smp, tl, wl);
     if (ls.pdf <= 0.0) return value;
     {
         float lightPdf = ls.pdf;
@@ -878,7 +918,7 @@ Spec evalMIS(const Scene& s, const Isect& it, const BSDF& bsdf, Sampler& smp,
                 Isect lit = createIntersection(s, rh);
                 const crh_material& m = matOf(s, rh.prim);
                 if (m.flags & CRH_MATF_EMISSIVE) {
-                    Spec em = emissionOf(m);
+                    Spec em = emissionOf(s, rh.prim, wl);
                     V3 toLight = lit.posW - it.posW;
                     float d2 = dot(toLight, toLight);
                     V3 lv = toLight / std::sqrt(d2);
@@ -904,6 +944,14 @@ Spec tracePath(const Scene& s, const crh_camera& cam, uint32_t W, uint32_t H,
     V2 j = smp.next2D();
     V2 pixel(float(x) + j.x, float(y) + j.y);
     pixelOut = pixel;
+    Wl wl;
+    if (cfg.spectral) {  // hero bin drawn right after the jitter
+        uint32_t h = std::min<uint32_t>(uint32_t(smp.next1D() * 95.0f), 94u);
+        wl.on = true;
+        wl.b[0] = h;
+        wl.b[1] = (h + 32u) % 95u;
+        wl.b[2] = (h + 64u) % 95u;
+    }
     Ray ray = generateRay(cam, W, H, pixel);
     Spec radiance(0.f), beta(1.f);
     const float rr = cfg.rr_threshold;
@@ -915,15 +963,22 @@ Spec tracePath(const Scene& s, const crh_camera& cam, uint32_t W, uint32_t H,
             Isect it = createIntersection(s, rh);
             const crh_material& m = matOf(s, rh.prim);
             if (m.flags & CRH_MATF_EMISSIVE) {
-                radiance = radiance + emissionOf(m) * beta;
+                radiance = radiance + emissionOf(s, rh.prim, wl) * beta;
                 break;
             }
             BSDF bsdf;
-            bsdf.init(m, it.frame);
-            if (cfg.use_mis) {
-                radiance = radiance + evalMIS(s, it, bsdf, smp, tl) * beta;
+            if (wl.on) {
+                uint32_t mid = matIdOf(s, rh.prim);
+                const float* sp = s.spectra.data() + size_t(mid) * 3 * 95;
+                bsdf.initWith(m, it.frame, lanesOf(sp, wl), lanesOf(sp + 95, wl), s.avgKd[mid],
+                              s.avgKs[mid]);
             } else {
-                LightSample ls = sampleLight(s, it, bsdf, smp, tl);
+                bsdf.init(m, it.frame);
+            }
+            if (cfg.use_mis) {
+                radiance = radiance + evalMIS(s, it, bsdf, smp, tl, wl) * beta;
+            } else {
+                LightSample ls = sampleLight(s, it, bsdf, smp, tl, wl);
                 if (ls.pdf > 0.0)
                     radiance = radiance +
                                ls.weight * beta / ls.pdf / ls.lightSelectPdf;
@@ -937,7 +992,7 @@ Spec tracePath(const Scene& s, const crh_camera& cam, uint32_t W, uint32_t H,
             ray.d = wi;
         } else {
             // Constant environment (this build's addition; zero = reference).
-            radiance = radiance + s.env * beta;
+            radiance = radiance + (wl.on ? lanesOf(s.envSpec.data(), wl) : s.env) * beta;
             terminate = true;
         }
         if (smp.next1D() < rr) terminate = true;
@@ -948,6 +1003,15 @@ Spec tracePath(const Scene& s, const crh_camera& cam, uint32_t W, uint32_t H,
             if (smp.next1D() < q) break;
             beta = beta / (1.0f - q);
         }
+    }
+    if (wl.on) {  // CIE XYZ of the three lanes
+        const float* a = s.filmW[wl.b[0]];
+        const float* b = s.filmW[wl.b[1]];
+        const float* c = s.filmW[wl.b[2]];
+        const float* L = radiance.v;
+        radiance = Spec((L[0] * a[0] + L[1] * b[0]) + L[2] * c[0],
+                        (L[0] * a[1] + L[1] * b[1]) + L[2] * c[1],
+                        (L[0] * a[2] + L[1] * b[2]) + L[2] * c[2]);
     }
     return radiance;
 }
@@ -995,6 +1059,29 @@ Scene* makeScene(const crh_scene_desc& d) {
     s->mats.assign(d.materials, d.materials + d.nmaterial);
     s->emissive.assign(d.emissive_index, d.emissive_index + d.nemissive);
     s->env = Spec(d.env_radiance[0], d.env_radiance[1], d.env_radiance[2]);
+    if (d.spectra) {
+        s->spectral = true;
+        s->spectra.assign(d.spectra, d.spectra + size_t(d.nmaterial) * 3 * 95);
+        s->envSpec.assign(95, 0.0f);
+        if (d.env_spectrum) s->envSpec.assign(d.env_spectrum, d.env_spectrum + 95);
+        for (uint32_t m = 0; m < d.nmaterial; ++m) {
+            float a = 0.0f, b = 0.0f;
+            for (int i = 0; i < 95; ++i) a += s->spectra[size_t(m) * 285 + i];
+            for (int i = 0; i < 95; ++i) b += s->spectra[size_t(m) * 285 + 95 + i];
+            s->avgKd.push_back(a / 95.0f);
+            s->avgKs.push_back(b / 95.0f);
+        }
+        double ysum = 0.0, c[3];
+        for (int i = 0; i < 95; ++i) {
+            cieXYZ(360.0 + 5.0 * i, c);
+            ysum += c[1] * 5.0;
+        }
+        const double k = (1.0 / ysum) * 5.0 * 95.0 / 3.0;
+        for (int i = 0; i < 95; ++i) {
+            cieXYZ(360.0 + 5.0 * i, c);
+            for (int j = 0; j < 3; ++j) s->filmW[i][j] = float(c[j] * k);
+        }
+    }
     s->bvh = buildBVH(s->pos, s->idx, d.bvh.bins, d.bvh.max_leaf,
                       d.bvh.max_depth);
     return s;
@@ -1242,5 +1329,13 @@ uint64_t orc_render(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
 }
 
 int orc_max_threads(void) { return omp_get_max_threads(); }
+
+void orc_cie_cmf(float* xyz) {
+    double c[3];
+    for (int i = 0; i < 95; ++i) {
+        cieXYZ(360.0 + 5.0 * i, c);
+        for (int j = 0; j < 3; ++j) xyz[3 * i + j] = float(c[j]);
+    }
+}
 
 }  // extern "C"

@@ -515,3 +515,78 @@ def test_integrator_direct_light_double_count(orc, mis):
                            for k in range(24)])
     se = vals.std() / math.sqrt(len(vals))
     assert abs(vals.mean() - 1.95 * D) < 4 * se + 1e-3 * D, (vals.mean(), 1.95 * D, se)
+
+
+# ---------------------------------------------------------------------------
+# Spectral transport (this build's addition): CIE fit, XYZ normalisation and
+# the reduction of flat spectra to the (reference-pinned) RGB estimator.
+# ---------------------------------------------------------------------------
+def _oracle_cmf(orc):
+    out = np.zeros((95, 3), np.float32)
+    orc.lib().orc_cie_cmf(out.ctypes.data)
+    return out
+
+
+def test_cie_fit_known_values_and_product_table(orc):
+    c = _oracle_cmf(orc)
+    lam = 360 + 5 * np.arange(95)
+    # CIE 1931 2-degree observer: y(555) = 1.0, x(600) = 1.0622, z(445) = 1.7826
+    assert abs(c[lam == 555][0, 1] - 1.0) < 0.01
+    assert abs(c[lam == 600][0, 0] - 1.0622) < 0.02
+    assert abs(c[lam == 445][0, 2] - 1.7826) < 0.06
+    # peaks where the standard observer has them
+    assert lam[np.argmax(c[:, 1])] in (555, 560)
+    assert lam[np.argmax(c[:, 0])] in (595, 600, 605)
+    assert lam[np.argmax(c[:, 2])] in (440, 445, 450)
+    # equal-energy white sits at x = y = 1/3 in CIE 1931: X, Y, Z integrals agree to 3 %
+    tot = c.sum(0)
+    assert np.all(np.abs(tot / tot[1] - 1.0) < 0.03)
+    # the product's table is the same fit
+    assert np.array_equal(cr.cie_cmf(), c)
+
+
+def test_spectrum_to_xyz_normalisation_and_linearity():
+    one = np.ones(95, np.float32)
+    xyz = cr.spectrum_to_xyz(one)
+    assert abs(xyz[1] - 1.0) < 1e-6
+    rng = np.random.default_rng(4)
+    a, b = rng.random(95).astype(np.float32), rng.random(95).astype(np.float32)
+    lhs = cr.spectrum_to_xyz(2.0 * a + 3.0 * b)
+    rhs = 2.0 * cr.spectrum_to_xyz(a) + 3.0 * cr.spectrum_to_xyz(b)
+    assert np.allclose(lhs, rhs, rtol=1e-5)
+    c = cr.cie_cmf().astype(np.float64)
+    assert np.allclose(xyz, c.sum(0) / c[:, 1].sum(), rtol=1e-5)
+
+
+def test_flat_spectra_reduce_to_the_rgb_estimator(orc):
+    """Spectral transport with constant spectra equal to gray RGB materials:
+    the expected XYZ is the RGB estimator's gray value times the white point
+    (X, 1, Z of a constant spectrum 1); per-pass image means agree within 3
+    sigma (the two modes draw different random streams)."""
+    data, camd = scenes.cornell_box(light_radiance=(12.0, 12.0, 12.0))
+    grays = [0.73, 0.5, 0.3, 0.0]
+    for i, g in enumerate(grays[:3]):
+        data.materials[i] = scenes.diffuse((g, g, g))
+    sp = np.zeros((4, 3, 95), np.float32)
+    for i, g in enumerate(grays[:3]):
+        sp[i, 0] = g
+    sp[3, 2] = 12.0
+    data.spectra = sp
+    white = cr.spectrum_to_xyz(np.ones(95, np.float32))
+    osc = orc.Scene(data)
+    cam = scenes.make_camera(camd)
+    size = (24, 24)
+    rgb_cfg = cr.Configs(maxDepth=6, rrThreshold=0.05, useMIS=True, rrStartDepth=3)
+    spec_cfg = cr.Configs(maxDepth=6, rrThreshold=0.05, useMIS=True, rrStartDepth=3,
+                          spectral=True)
+    gm, sm = [], []
+    for k in range(48):
+        Lg, _, _ = osc.render_samples(cam, size, rgb_cfg, k)
+        Ls, _, _ = osc.render_samples(cam, size, spec_cfg, 10000 + k)
+        gm.append(Lg[..., 1].mean())
+        sm.append(Ls.reshape(-1, 3).mean(0))
+    gm, sm = np.array(gm), np.array(sm)
+    for ch in range(3):
+        exp = gm * white[ch]
+        se = np.sqrt(exp.var(ddof=1) / len(exp) + sm[:, ch].var(ddof=1) / len(sm))
+        assert abs(exp.mean() - sm[:, ch].mean()) <= 3 * se, (ch, exp.mean(), sm[:, ch].mean(), se)
