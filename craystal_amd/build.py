@@ -56,16 +56,28 @@ def build(verbose: bool = True) -> Path:
 
 
 def build_cpp_tests() -> Path:
-    """Compile tests/cpp/api_test.cpp against include/ and libcrayhip.so."""
+    """Compile tests/cpp/api_test.cpp and import_test.cpp against include/ and
+    libcrayhip.so."""
     root = PKG.parent
-    out = root / "tests" / "cpp" / "_build" / "api_test"
-    out.parent.mkdir(parents=True, exist_ok=True)
-    _run(["g++", "-O1", "-std=c++17", f"-I{root / 'include'}", str(root / "tests" / "cpp" / "api_test.cpp"),
-          f"-L{PKG}", "-lcrayhip", f"-Wl,-rpath,{PKG}", "-o", str(out)])
+    outdir = root / "tests" / "cpp" / "_build"
+    outdir.mkdir(parents=True, exist_ok=True)
+    for name in ("api_test", "import_test"):
+        _run(["g++", "-O1", "-std=c++17", f"-I{root / 'include'}",
+              str(root / "tests" / "cpp" / f"{name}.cpp"), f"-L{PKG}", "-lcrayhip",
+              f"-Wl,-rpath,{PKG}", "-o", str(outdir / name)])
+    return outdir / "api_test"
+
+
+def build_cli() -> Path:
+    """The crayli command line renderer next to libcrayhip.so."""
+    out = PKG / "crayli"
+    _run(["g++", "-O2", "-std=c++17", f"-I{PKG.parent / 'include'}", str(CSRC / "crayli.cpp"),
+          f"-L{PKG}", "-lcrayhip", "-Wl,-rpath,$ORIGIN", "-o", str(out)])
     return out
 
 
 if __name__ == "__main__":
     build()
+    build_cli()
     build_cpp_tests()
     sys.exit(0)

@@ -51,7 +51,8 @@ def write_exr(path, rgb: np.ndarray) -> None:
     table_at = len(head)
     first = table_at + 8 * h
     offsets = struct.pack(f"<{h}Q", *[first + y * (8 + line_bytes) for y in range(h)])
-    half = img[..., :3].astype(np.float16)
+    with np.errstate(over="ignore"):   # beyond the half range -> inf, as the format stores it
+        half = img[..., :3].astype(np.float16)
     blocks = []
     for y in range(h):
         data = b"".join(half[y, :, c].astype("<f2").tobytes() for _, c in chans)
