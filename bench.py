@@ -204,8 +204,10 @@ def measure_traffic() -> dict | None:
             return None
         for row in csv.DictReader(open(files[0])):
             name = row["Kernel_Name"]
+            if "kRenderPass<" in name or "kRenderRegen<" in name:
+                vals.setdefault("render_kernel", {})["name"] = name.split("<")[0]
             key = ("copy" if "copyBuffer" in name else
-                   "kRenderPass" if "kRenderPass<" in name else
+                   "render" if ("kRenderPass<" in name or "kRenderRegen<" in name) else
                    "kTraceClosest" if "kTraceClosest<false" in name else None)
             if key:
                 # the calibration copy is the largest copy dispatch
@@ -217,8 +219,9 @@ def measure_traffic() -> dict | None:
         copy_kib = 256 * 1024
         fs = copy_kib / vals["copy"]["FETCH_SIZE"]
         ws = copy_kib / vals["copy"]["WRITE_SIZE"]
-        out = {"fetch_scale": round(fs, 5), "write_scale": round(ws, 5)}
-        for k in ("kRenderPass", "kTraceClosest"):
+        out = {"fetch_scale": round(fs, 5), "write_scale": round(ws, 5),
+               "render_kernel": vals["render_kernel"]["name"]}
+        for k in ("render", "kTraceClosest"):
             out[k] = int((vals[k]["FETCH_SIZE"] * fs + vals[k]["WRITE_SIZE"] * ws) * 1024)
         return out
     except (KeyError, ZeroDivisionError):
@@ -267,7 +270,8 @@ def main():
     from craystal_amd import autotune
     tuned = autotune.tune_render(scene, cfg, spp=4, stacks=(1, 2, 3), wf_grids=(16384,),
                                  wf_blocks=(128, 256), wf_persistent=((2, 0),),
-                                 wf_pools=(1 << 21, 1 << 22, 1 << 23), render_waves=(1, 4))
+                                 wf_pools=(1 << 21, 1 << 22, 1 << 23), render_waves=(1, 4),
+                                 render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)))
     sensor.clear()
 
     progress(f"scene built, autotuned: {tuned['render_stack']=} {tuned['wf_pool']=}")
@@ -328,10 +332,11 @@ def main():
         "gpu_kernel_ms": round(main_ms, 3),
         "bvh": {"nodes": bst["node_count"], "max_depth": bst["max_depth"],
                 "max_leaf": bst["max_leaf"], "build_s": round(build_s, 3)},
-        "roofline": {"bound": "hbm", "kernel": "kRenderPass",
+        "roofline": {"bound": "hbm",
+                     "kernel": "kRenderRegen" if tuned["render_regen"] == 1 else "kRenderPass",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": traffic["kRenderPass"] if traffic else None,
+                     "traffic": traffic["render"] if traffic else None,
                      "bytes_per_sample": round(b_sample, 1),
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                      "node_visits_per_sample": round(nodes_ps, 3),
@@ -349,11 +354,14 @@ def main():
     if traffic:
         out["roofline"]["traffic_note"] = (
             "HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of "
-            "bench.py --profile-kernels (one 1920x1080 pass), scaled by a 256 MiB copy "
+            "bench.py --profile-kernels (one 1920x1080 pass of "
+            f"{traffic['render_kernel']} at the default launch shape), scaled by a 256 MiB copy "
             f"(fetch x {traffic['fetch_scale']}, write x {traffic['write_scale']})")
-    out["tuning"] = {k: tuned[k] for k in ("render_stack", "render_waves", "wf_stack", "wf_grid",
+    out["tuning"] = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen",
+                                           "render_grid", "wf_stack", "wf_grid",
                                            "wf_trace_block", "wf_persistent", "wf_refill",
                                            "wf_pool")}
+    out["tuning"]["megakernel_ms_per_spp"] = tuned["megakernel_ms"]
     out["wavefront"] = {"msamples_per_s": round(samples / wel / 1e6, 3),
                         "ms_per_step": round(wel / a.steps * 1e3, 3),
                         "grays_per_s": round(wst["rays"] / wel / 1e9, 4)}

@@ -412,6 +412,8 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(t->refill <= 64 && t->wf_refill <= 64, "refill must be in [1, 64]");
     ARG_CHECK(t->render_waves == 0 || t->render_waves == 1 || t->render_waves == 4,
               "render_waves must be 0 (keep), 1 or 4");
+    ARG_CHECK(t->render_regen <= 2, "render_regen must be 0 (keep), 1 (on) or 2 (off)");
+    ARG_CHECK(t->render_grid <= (1u << 20), "render_grid too large");
     ARG_CHECK(t->wf_pool <= (1u << 28) && t->render_pool <= (1u << 28),
               "wf_pool and render_pool must be at most 2^28 paths");
     ARG_CHECK(okStack(t->batch_stack),
@@ -438,6 +440,8 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     if (t->wf_pool) o.wfPool = t->wf_pool;
     if (t->render_pool) o.renderPool = t->render_pool;
     if (t->render_waves) o.renderWaves = t->render_waves;
+    if (t->render_regen) o.renderRegen = t->render_regen == 1 ? 1u : 0u;
+    if (t->render_grid) o.renderGrid = t->render_grid;
     if (t->batch_stack) o.batchPriv = t->batch_stack - 1u;
     if (t->batch_tile_w) o.batchTileW = t->batch_tile_w;
     if (t->batch_grid) o.batchGrid = t->batch_grid;
@@ -465,6 +469,8 @@ int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
     t->wf_pool = o.wfPool;
     t->render_pool = o.renderPool;
     t->render_waves = o.renderWaves;
+    t->render_regen = o.renderRegen ? 1u : 2u;
+    t->render_grid = o.renderGrid;
     t->batch_stack = o.batchPriv + 1u;
     t->batch_tile_w = o.batchTileW;
     t->batch_grid = o.batchGrid;
@@ -1009,7 +1015,7 @@ int crh_render(crh_scene* s, const crh_camera* cam,
                                           film->L.as<float4>(), film->P.as<float2>(),
                                           counter.as<unsigned long long>(),
                                           cfg->count_work != 0, tun, ctx->stream,
-                                          cfg->spectral != 0));
+                                          cfg->spectral != 0, ctx->counter));
         if (stats) HIP_TRY(hipEventRecord(evs[2 * i + 1], ctx->stream));
         for (uint32_t k = 0; k < active; ++k) {
             const float4* Ls = (wavefront ? film->wf.rad : film->L.as<float4>()) + k * npix;

@@ -35,6 +35,8 @@ struct Tuning {
     uint32_t wfPool = 1u << 23;  // paths in flight (several samples per pixel for small images)
     uint32_t renderPool = 1u << 21;  // megakernel: paths per launch, the same way
     uint32_t renderWaves = 4;        // megakernels: minimum waves per SIMD (1 or 4)
+    uint32_t renderRegen = 1;        // megakernel: lanes take a new path when theirs ends
+    uint32_t renderGrid = 2048;      // megakernel with regeneration: persistent blocks
     // batched rendering
     uint32_t batchPriv = 2;
     uint32_t batchTileW = 16;   // tile width in pixels; height = 256 / width
@@ -70,12 +72,14 @@ hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
 hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
                           uint8_t* occ, const Tuning& t, uint32_t* counter, hipStream_t st);
 // Sample indices si .. si + active - 1 in one launch; slot k of L and P
-// (w * h entries each) receives sample si + k.
+// (w * h entries each) receives sample si + k.  next: a device counter for
+// path regeneration (used when the tuning asks for it and work is not
+// counted).
 hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
                             const IntegratorView& cfg, uint32_t si, uint32_t active, float4* L,
                             float2* P, unsigned long long* counters,
                             bool countWork, const Tuning& t, hipStream_t st,
-                            bool spectral = false);
+                            bool spectral = false, uint32_t* next = nullptr);
 // One sample index over the first nActive scenes of the table: persistent
 // megakernel over `items` tiles, then the film gather over `pixels` pixels.
 // next: kBatchMaxChunks device counters.

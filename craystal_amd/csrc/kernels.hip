@@ -5,6 +5,7 @@
 
 #include "kernels.h"
 #include "integrator.h"
+#include "wave.h"
 
 namespace crh {
 
@@ -132,12 +133,17 @@ CRH_D Spec evalMIS(PathCtx<COUNT, Stack>& c, const Isect& it, const BSDF& bsdf,
     return value / ls.selPdf;
 }
 
-CRH_D void tileCoord(uint32_t tile, uint32_t tilesX, uint32_t& x, uint32_t& y) {
-    // 16x16 pixel tile per block; each wave covers an 8x8 quarter.
+// Pixel `inTile` (0..255) of a 16x16 tile; each 64-pixel quarter is an 8x8
+// block (one wave of a 256-thread block).
+CRH_D void tileCoordOf(uint32_t tile, uint32_t inTile, uint32_t tilesX, uint32_t& x,
+                       uint32_t& y) {
     uint32_t tx = tile % tilesX, ty = tile / tilesX;
-    uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    uint32_t wave = inTile >> 6, lane = inTile & 63u;
     x = tx * 16u + (wave & 1u) * 8u + (lane & 7u);
     y = ty * 16u + (wave >> 1) * 8u + (lane >> 3);
+}
+CRH_D void tileCoord(uint32_t tile, uint32_t tilesX, uint32_t& x, uint32_t& y) {
+    tileCoordOf(tile, threadIdx.x, tilesX, x, y);
 }
 
 // pathTraceKernel (Core/Integrator.cu:142-227) for one pixel: radiance of
@@ -244,6 +250,115 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W,
                                      wc);
     }
     flushCounters<COUNT>(counters, rays, wc.nodes, wc.tris);
+}
+
+// Path regeneration: persistent blocks; every lane runs one bounce of its
+// path per iteration and, when its path ends, writes it and takes the next
+// path index (one atomic per wave for all lanes that need one), so lanes do
+// not idle while the longest path of their wave finishes.  Path p is pixel
+// p % (tiles * 256) (tile order, 8x8 per wave as kRenderPass) of sample
+// slot p / (tiles * 256).  Each path runs exactly tracePixel's steps on the
+// same random stream, so L and P equal kRenderPass's.
+template <bool MIS, bool SPEC, int SM, int W>
+__global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W, 8))) void
+kRenderRegen(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint32_t tilesX,
+             uint32_t tiles, uint32_t total, uint32_t* __restrict__ next,
+             float4* __restrict__ L, float2* __restrict__ P,
+             unsigned long long* __restrict__ counters) {
+    extern __shared__ uint2 lds[];
+    auto stk = makeStack<SM>(lds);
+    PathCtx<false, typename StackOf<SM>::T> c{s, stk, 0, {0, 0}};
+    const size_t npix = size_t(cam.width) * cam.height;
+    const uint32_t perSlot = tiles * 256u;
+    bool live = false, exhausted = false;
+    uint32_t depth = 0;
+    size_t out = 0;
+    Sampler smp(0u, 0u, 0u);
+    Ray ray{};
+    Spec rad = sp(0.f), beta = sp(1.f);
+    F2 pix{0.f, 0.f};
+    Lanes w = rgbLanes();
+    auto finish = [&]() {
+        Spec v = SPEC ? laneXYZ(rad, w) : rad;
+        L[out] = make_float4(v.r, v.g, v.b, 0.f);
+        P[out] = make_float2(pix.x, pix.y);
+        live = false;
+    };
+    while (true) {
+        if (!exhausted) {
+            const bool want = !live;
+            const uint32_t p = wavePush(next, want);
+            exhausted = __any(want && p >= total);
+            if (want && p < total) {
+                const uint32_t slot = p / perSlot, q = p - slot * perSlot;
+                uint32_t x, y;
+                tileCoordOf(q >> 8, q & 255u, tilesX, x, y);
+                if (x < cam.width && y < cam.height) {
+                    smp = Sampler(x, y, si + slot);
+                    F2 j = smp.next2D();
+                    pix = F2{float(x) + j.x, float(y) + j.y};
+                    w = SPEC ? heroLanes(s.spec, smp.next1D()) : rgbLanes();
+                    ray = cameraRay(cam, pix);
+                    rad = sp(0.f);
+                    beta = sp(1.f);
+                    depth = 0;
+                    out = slot * npix + size_t(y) * cam.width + x;
+                    live = true;
+                    if (cfg.maxDepth == 0u) finish();
+                }
+            }
+        }
+        if (!__any(live)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (live) {  // one bounce of pathTraceKernel's loop (Core/Integrator.cu:160-223)
+            bool done = false, terminate = false;
+            Hit h;
+            if (closest(c, ray, h)) {
+                Isect it = makeIsect(s, ray, h);
+                const uint32_t mid = s.primMat[h.prim];
+                const GMaterial& m = s.mats[mid];
+                if (m.flags & 1u) {
+                    rad = rad + emissionOf(m, mid, w) * beta;
+                    done = true;
+                } else {
+                    BSDF bsdf = makeBSDF(m, mid, it.frame, w);
+                    if (MIS) {
+                        rad = rad + evalMIS(c, it, bsdf, smp, w) * beta;
+                    } else {
+                        LightSample ls = sampleLight(c, it, bsdf, smp, w);
+                        if (ls.pdf > 0.0f) rad = rad + ls.weight * beta / ls.pdf / ls.selPdf;
+                    }
+                    F3 wi;
+                    float pdf = 0.0f;
+                    Spec f = bsdfSampleEval(bsdf, smp, it.view, wi, pdf);
+                    if (pdf <= 0.0f) {
+                        done = true;
+                    } else {
+                        beta = beta * (f / pdf);
+                        ray.o = offsetRayOrigin(it.pos, oriented(it));
+                        ray.d = wi;
+                    }
+                }
+            } else {
+                rad = rad + (SPEC ? laneSpec(s.spec + kSpecEnv, w) : sp3(s.env)) * beta;
+                terminate = true;
+            }
+            if (!done) {
+                if (smp.next1D() < cfg.rrThreshold) terminate = true;
+                if (terminate) done = true;
+                else if (!roulette(cfg, depth, smp, beta)) done = true;
+                else if (++depth >= cfg.maxDepth) done = true;
+            }
+            if (done) finish();
+        }
+    }
+    flushCounters<false>(counters, c.rays, 0, 0);
+}
+
+__global__ void kResetNext(uint32_t* __restrict__ next) {
+    if (threadIdx.x == 0) *next = 0u;
 }
 
 // Deterministic film update for one pass.  Each pixel gathers the tent-filter
@@ -431,13 +546,42 @@ hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
 hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
                             const IntegratorView& cfg, uint32_t si, uint32_t active, float4* L,
                             float2* P, unsigned long long* counters,
-                            bool countWork, const Tuning& t, hipStream_t st, bool spectral) {
+                            bool countWork, const Tuning& t, hipStream_t st, bool spectral,
+                            uint32_t* next) {
     if (spectral && (!s.spec || countWork)) return hipErrorInvalidValue;
     const uint32_t tilesX = (cam.width + 15u) / 16u, tilesY = (cam.height + 15u) / 16u;
     const uint32_t tiles = tilesX * tilesY;
-    if (uint64_t(tiles) * active >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
+    if (uint64_t(tiles) * active >= (uint64_t(1) << 23)) return hipErrorInvalidValue;
     const size_t lds = stackLdsBytes(t.renderPriv, s.stackCap, kRenderBlock);
-    const dim3 grid(tiles * active), block(kRenderBlock);
+    const dim3 block(kRenderBlock);
+    if (t.renderRegen && !countWork && next) {
+        const uint32_t total = tiles * active * 256u;
+        const dim3 grid(std::min<uint32_t>(tiles * active, t.renderGrid));
+        hipLaunchKernelGGL(kResetNext, dim3(1), dim3(64), 0, st, next);
+        withStack(t.renderPriv, [&](auto sm) {
+            constexpr int SM = decltype(sm)::value;
+            withWaves(t.renderWaves, [&](auto wv) {
+                constexpr int W = decltype(wv)::value;
+                if (spectral && cfg.useMIS)
+                    hipLaunchKernelGGL((kRenderRegen<true, true, SM, W>), grid, block, lds, st, s,
+                                       cam, cfg, si, tilesX, tiles, total, next, L, P, counters);
+                else if (spectral)
+                    hipLaunchKernelGGL((kRenderRegen<false, true, SM, W>), grid, block, lds, st,
+                                       s, cam, cfg, si, tilesX, tiles, total, next, L, P,
+                                       counters);
+                else if (cfg.useMIS)
+                    hipLaunchKernelGGL((kRenderRegen<true, false, SM, W>), grid, block, lds, st,
+                                       s, cam, cfg, si, tilesX, tiles, total, next, L, P,
+                                       counters);
+                else
+                    hipLaunchKernelGGL((kRenderRegen<false, false, SM, W>), grid, block, lds, st,
+                                       s, cam, cfg, si, tilesX, tiles, total, next, L, P,
+                                       counters);
+            });
+        });
+        return hipGetLastError();
+    }
+    const dim3 grid(tiles * active);
     withStack(t.renderPriv, [&](auto sm) {
         constexpr int SM = decltype(sm)::value;
         withWaves(t.renderWaves, [&](auto wv) {

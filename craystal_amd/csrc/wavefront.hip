@@ -31,6 +31,7 @@
 #include "integrator.h"
 #include "wavefront.h"
 #include "kernels.h"
+#include "wave.h"
 
 namespace crh {
 
@@ -44,19 +45,6 @@ enum : uint32_t {
 };
 
 CRH_D uint32_t laneId() { return __lane_id(); }
-
-// Reserve `pred ? 1 : 0` slots of a queue for each active lane of the wave.
-CRH_D uint32_t wavePush(uint32_t* counter, bool pred) {
-    uint64_t mask = __ballot(pred);
-    if (mask == 0) return 0xFFFFFFFFu;
-    uint32_t lo = uint32_t(mask), hi = uint32_t(mask >> 32);
-    uint32_t below = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (int(laneId()) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
-    base = __shfl(base, leader, 64);
-    return pred ? base + below : 0xFFFFFFFFu;
-}
 
 CRH_D Ray loadRay(const float4* o, const float4* d, uint32_t p) {
     float4 a = o[p], b = d[p];

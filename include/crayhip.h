@@ -219,6 +219,9 @@ typedef struct crh_tuning {
                                 max(1, render_pool / pixels), capped by the spp asked */
     uint32_t render_waves;   /* megakernel and batch: 1 = compiler's register budget,
                                 4 = at most 128 VGPRs so 4 waves fit per SIMD */
+    uint32_t render_regen;   /* megakernel: 1 = persistent blocks whose lanes start a new
+                                path as soon as theirs ends, 2 = one path per thread */
+    uint32_t render_grid;    /* megakernel with regeneration: persistent blocks */
     uint32_t wf_stack;       /* wavefront traversal kernels: stack as above */
     uint32_t wf_trace_grid;  /* wavefront traversal kernels: blocks */
     uint32_t wf_trace_block; /* wavefront traversal kernels: threads (64..256) */

@@ -175,6 +175,44 @@ def test_wavefront_film_bit_identical_to_megakernel(gpu, mis):
         ctx.set_tuning(**saved)
 
 
+@pytest.mark.parametrize("spectral", [False, True])
+def test_path_regeneration_bit_identical(gpu, spectral):
+    """Persistent megakernel whose lanes start a new path as soon as theirs
+    ends vs one path per thread: bit-identical films and ray counts, over
+    image sizes that are not multiples of the 16x16 tile, pass groups of 1
+    and 3 samples per pixel (the last group ragged), MIS on and off, depth
+    limits 0, 1 and 8 and persistent grids smaller and larger than the
+    work."""
+    from craystal_amd import _lib as L
+    ctx = cr.get_context(0)
+    saved = ctx.get_tuning()
+    if spectral:
+        cases = [(*scenes.cornell_box_spectral(), (37, 29))]
+    else:
+        cases = [(*scenes.cornell_box(), (64, 64)), (*_materials_scene(), (61, 45))]
+    try:
+        for data, camd, size in cases:
+            scene = cr.Scene(data)
+            npix = size[0] * size[1]
+            for mis, depth in [(True, 8), (False, 8), (True, 1), (True, 0)]:
+                cfg = cr.Configs(maxDepth=depth, rrThreshold=0.05, useMIS=mis, rrStartDepth=3,
+                                 spectral=spectral)
+                ctx.set_tuning(render_regen=2, render_pool=npix)
+                d0, w0, s0 = _film(scene, camd, size, cfg, 4, first=7)
+                for pool, grid in [(npix, 1), (npix, 3), (3 * npix, 2048)]:
+                    ctx.set_tuning(render_regen=1, render_pool=pool, render_grid=grid)
+                    d1, w1, s1 = _film(scene, camd, size, cfg, 4, first=7)
+                    key = (size, mis, depth, pool // npix, grid)
+                    assert np.array_equal(w0.view(np.uint32), w1.view(np.uint32)), key
+                    assert np.array_equal(d0.view(np.uint32), d1.view(np.uint32)), \
+                        (key, np.argwhere(d0 != d1)[:5])
+                    assert s0["rays"] == s1["rays"], key
+                if depth:
+                    assert d0.max() > 0
+    finally:
+        ctx.set_tuning(**saved)
+
+
 def _materials_scene():
     mats = [scenes.diffuse((0.6, 0.6, 0.6)), scenes.emitter((20.0, 20.0, 20.0)),
             scenes.principled((0.8, 0.3, 0.2), 0.3, 0.0),
