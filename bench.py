@@ -61,7 +61,37 @@ def parse():
                     help="counter-collection mode: one 256 MiB device copy (traffic "
                          "calibration), one 16,777,216-ray closest-hit launch and one render "
                          "pass; prints the launch sizes and exits")
+    ap.add_argument("--profile-guided", action="store_true",
+                    help="counter-collection mode for the guiding kernels: the occluded room "
+                         "at 256x256, 16 spp of the guided wavefront with training, 16 without, "
+                         "then 16 spp of the plain megakernel and of the plain wavefront")
     return ap.parse_args()
+
+
+def profile_guided(cr, L, scenes):
+    """Launches of the guided wavefront (training and inference), the plain
+    megakernel and the plain wavefront on the occluded room, 256 x 256, max
+    depth 16, for rocprofv3 kernel traces and counter runs."""
+    from craystal_amd.guiding import Guide, GuidedIntegrator
+    data, camd = scenes.occluded_room()
+    cfg = cr.Configs(maxDepth=16, rrThreshold=0.05, useMIS=True, rrStartDepth=3)
+    scene = cr.Scene(data)
+    sensor = cr.Sensor((256, 256))
+    scene.setCamera(scenes.make_camera(camd, sensor))
+    P = data.concatenated()[0]
+    guide = Guide(P.min(0), P.max(0), seed=11)
+    gi = GuidedIntegrator(cr.Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT}), guide,
+                          alpha=0.5, lr=1e-2, train_batch=4096, train_steps=4, materials=1,
+                          target=0)
+    tr = gi.dispatch(scene, 16, first_sample=0, train=True, readback=False)
+    inf = gi.dispatch(scene, 16, first_sample=16, train=False, readback=False)
+    mk = cr.PathTraceIntegrator(cfg).dispatch(scene, 16, first_sample=32, readback=False)
+    wf = cr.PathTraceIntegrator(cr.Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})).dispatch(
+        scene, 16, first_sample=48, readback=False)
+    print(json.dumps({"pixels": 65536, "spp_each": 16, "guided_train_ms": tr["kernel_ms"],
+                      "guided_ms": inf["kernel_ms"], "megakernel_ms": mk["kernel_ms"],
+                      "wavefront_ms": wf["kernel_ms"], "guide_steps": guide.info()["steps"]}),
+          flush=True)
 
 
 def profile_kernels(cr, L, scenes):
@@ -237,7 +267,7 @@ def main():
         sys.exit(2)
 
     traffic = None
-    if not a.profile_kernels and not a.no_traffic:
+    if not a.profile_kernels and not a.profile_guided and not a.no_traffic:
         traffic = measure_traffic()
         progress(f"traffic counters: {traffic}")
 
@@ -250,6 +280,9 @@ def main():
         raise SystemExit("no GPU visible to libcrayhip.so")
     if a.profile_kernels:
         profile_kernels(cr, L, scenes)
+        return
+    if a.profile_guided:
+        profile_guided(cr, L, scenes)
         return
 
     data, camd = scenes.spheres_scene()
