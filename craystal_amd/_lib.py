@@ -179,6 +179,7 @@ SIGNATURES = {
     "crh_guide_destroy": [_P],
     "crh_cie_cmf": [_P],
     "crh_spectrum_to_xyz": [_P, _P],
+    "crh_rgb_to_spectrum": [_P, _P],
     "crh_batch_create": [_P, _P, u32, _P],
     "crh_batch_get_info": [_P, C.POINTER(crh_batch_info)],
     "crh_batch_render": [_P, _P, _P, _P, C.POINTER(crh_integrator_cfg), u32,

@@ -38,6 +38,17 @@ def spectrum_to_xyz(spectrum) -> np.ndarray:
     return out
 
 
+def rgb_to_spectrum(rgb) -> np.ndarray:
+    """95-bin spectrum of an RGB value (crh_rgb_to_spectrum: Smits's
+    decomposition; the replacement of BroadSpectrum::fromRGB,
+    Core/Spectrum.cpp:49).  The white-balanced linear sRGB of the result
+    equals rgb; gray (k, k, k) gives the flat spectrum k."""
+    c = np.ascontiguousarray(rgb, np.float32).reshape(3)
+    out = np.zeros(SPECTRUM_BINS, np.float32)
+    L.call("crh_rgb_to_spectrum", L.ptr(c), L.ptr(out))
+    return out
+
+
 # sRGB -> XYZ (Core/SpectrumTabData.h:63-65), used for the XYZ film output.
 SRGB_TO_XYZ = np.array([[0.412453, 0.357580, 0.180423],
                         [0.212671, 0.715160, 0.072169],
@@ -162,8 +173,9 @@ class SceneData:
     envRadiance: tuple = (0.0, 0.0, 0.0)
     # BVH parameters (bins, max leaf, max depth); zeros take the reference's.
     bvhParams: tuple = (32, 4, 20)
-    # Spectral transport (optional): (nmaterial, 3, 95) diffuse, specular and
-    # emission spectra, 5 nm bins from 360 nm; the environment's 95 bins.
+    # Spectral transport: (nmaterial, 3, 95) diffuse, specular and emission
+    # spectra, 5 nm bins from 360 nm; the environment's 95 bins.  None: the
+    # RGB values are uplifted (rgb_to_spectrum) when the scene is created.
     spectra: object = None
     envSpectrum: object = None
 
@@ -445,7 +457,7 @@ class Configs:
     rrStartDepth: int = 6
     mode: int = L.MODE_MEGAKERNEL
     # Spectral transport: three of the 95 wavelength bins per path; the film
-    # then holds CIE XYZ (the scene needs SceneData.spectra).
+    # then holds CIE XYZ (spectra from SceneData.spectra or uplifted RGB).
     spectral: bool = False
 
     @staticmethod

@@ -693,9 +693,25 @@ int crh_scene_create(crh_context* ctx, const crh_scene_desc* d,
     if (e == hipSuccess) e = upload(s->primMat, primMat, st);
     if (e == hipSuccess) e = upload(s->mats, gm, st);
     if (e == hipSuccess) e = upload(s->emissive, em, st);
+    // Spectral table: the given spectra, or the materials' and environment's
+    // RGB uplifted to 95 bins (RGB-authored scenes render spectrally too).
     std::vector<float> specTab;
-    if (d->spectra) specTab = crh::buildSpectralTable(d->spectra, d->nmaterial, d->env_spectrum);
-    if (e == hipSuccess && d->spectra) e = upload(s->spec, specTab, st);
+    if (d->spectra) {
+        specTab = crh::buildSpectralTable(d->spectra, d->nmaterial, d->env_spectrum);
+    } else {
+        std::vector<float> rgb(size_t(d->nmaterial) * 9);
+        for (uint32_t i = 0; i < d->nmaterial; ++i)
+            for (int k = 0; k < 3; ++k) {
+                rgb[9 * i + k] = d->materials[i].diffuse[k];
+                rgb[9 * i + 3 + k] = d->materials[i].specular[k];
+                rgb[9 * i + 6 + k] = d->materials[i].emission[k];
+            }
+        std::vector<float> sp = crh::upliftedSpectra(rgb.data(), d->nmaterial, 9);
+        float env[crh::kCieBins];
+        crh::rgbToSpectrum(d->env_radiance, env);
+        specTab = crh::buildSpectralTable(sp.data(), d->nmaterial, env);
+    }
+    if (e == hipSuccess) e = upload(s->spec, specTab, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         delete s;
@@ -720,7 +736,7 @@ int crh_scene_create(crh_context* ctx, const crh_scene_desc* d,
     v.primMat = s->primMat.as<uint32_t>();
     v.mats = s->mats.as<crh::GMaterial>();
     v.emissive = s->emissive.as<uint32_t>();
-    v.spec = d->spectra ? s->spec.as<float>() : nullptr;
+    v.spec = s->spec.as<float>();
     *out = s;
     return CRH_OK;
     GUARD_END
@@ -1467,6 +1483,14 @@ int crh_cie_cmf(float* xyz) {
         for (int j = 0; j < 3; ++j) xyz[3 * i + j] = float(c[j]);
     }
     return CRH_OK;
+}
+
+int crh_rgb_to_spectrum(const float* rgb, float* spectrum) {
+    GUARD_BEGIN
+    ARG_CHECK(rgb && spectrum, "NULL argument");
+    crh::rgbToSpectrum(rgb, spectrum);
+    return CRH_OK;
+    GUARD_END
 }
 
 int crh_spectrum_to_xyz(const float* spectrum, float* xyz) {

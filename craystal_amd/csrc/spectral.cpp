@@ -1,7 +1,9 @@
 #include "spectral.h"
 
+#include "rgb_basis.h"
 #include "spec_layout.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace crh {
@@ -49,6 +51,40 @@ void spectrumToXYZ(const float* s, float xyz[3]) {
         for (int j = 0; j < 3; ++j) acc[j] += c[j] * double(s[i]) * kCieStep * k;
     }
     for (int j = 0; j < 3; ++j) xyz[j] = float(acc[j]);
+}
+
+void rgbToSpectrum(const float rgb[3], float out[kCieBins]) {
+    enum { kC, kM, kY, kR, kG, kB };
+    const float r = std::max(rgb[0], 0.0f), g = std::max(rgb[1], 0.0f),
+                b = std::max(rgb[2], 0.0f);
+    float white, w1, w2;
+    int b1, b2;
+    if (r <= g && r <= b) {  // red smallest: white + cyan + (blue or green)
+        white = r;
+        b1 = kC;
+        if (g <= b) { w1 = g - r; b2 = kB; w2 = b - g; }
+        else        { w1 = b - r; b2 = kG; w2 = g - b; }
+    } else if (g <= r && g <= b) {  // green smallest: white + magenta + ...
+        white = g;
+        b1 = kM;
+        if (r <= b) { w1 = r - g; b2 = kB; w2 = b - r; }
+        else        { w1 = b - g; b2 = kR; w2 = r - b; }
+    } else {  // blue smallest: white + yellow + (green or red)
+        white = b;
+        b1 = kY;
+        if (r <= g) { w1 = r - b; b2 = kG; w2 = g - r; }
+        else        { w1 = g - b; b2 = kR; w2 = r - g; }
+    }
+    for (int i = 0; i < kCieBins; ++i)
+        out[i] = white + w1 * kRgbBasis[b1][i] + w2 * kRgbBasis[b2][i];
+}
+
+std::vector<float> upliftedSpectra(const float* rgb9, uint32_t nmaterial, size_t stride) {
+    std::vector<float> s(size_t(nmaterial) * 3 * kCieBins);
+    for (uint32_t m = 0; m < nmaterial; ++m)
+        for (int k = 0; k < 3; ++k)
+            rgbToSpectrum(rgb9 + m * stride + 3 * k, s.data() + (size_t(m) * 3 + k) * kCieBins);
+    return s;
 }
 
 std::vector<float> buildSpectralTable(const float* spectra, uint32_t nmaterial,

@@ -303,12 +303,14 @@ def spheres_scene(n_spheres: int = 48, subdiv: int = 5, seed: int = SCENE_SEED,
 
 
 def occluded_room(n_objects: int = 200, subdiv: int = 3, seed: int = SCENE_SEED,
-                  light_radiance: float = 50.0):
+                  light_radiance: float = 50.0, glossy: bool = True):
     """Closed 10 x 10 x 3 m room lit only through a 0.5 x 0.5 m opening in
     the +x wall by an exterior 4 x 4 m area light (radiance 50), furnished
     with seeded boxes and icospheres (about 150k triangles at the default
     sizes) in diffuse and glossy Principled materials (roughness
-    U[0.1, 0.6]).  1024 x 1024, max depth 16."""
+    U[0.1, 0.6]).  1024 x 1024, max depth 16.  glossy=False gives every
+    Principled object a Lambertian material of the same base colour (same
+    geometry and random draws)."""
     rng = Stream(seed)
     mats = [diffuse((0.7, 0.7, 0.7)), emitter((light_radiance,) * 3)]
     W = 0
@@ -335,8 +337,9 @@ def occluded_room(n_objects: int = 200, subdiv: int = 3, seed: int = SCENE_SEED,
             mats.append(diffuse((rng.uniform(0.3, 0.9), rng.uniform(0.3, 0.9),
                                  rng.uniform(0.3, 0.9))))
         else:
-            mats.append(principled((rng.uniform(), rng.uniform(), rng.uniform()),
-                                   rng.uniform(0.1, 0.6), 0.0))
+            base = (rng.uniform(), rng.uniform(), rng.uniform())
+            rough = rng.uniform(0.1, 0.6)
+            mats.append(principled(base, rough, 0.0) if glossy else diffuse(base))
         mid = len(mats) - 1
         if rng.uniform() < 0.5:
             r = rng.uniform(0.15, 0.45)

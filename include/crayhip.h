@@ -148,7 +148,8 @@ typedef struct crh_scene_desc {
      * integrator): nmaterial x 3 x 95 floats (diffuse, specular, emission
      * spectra of each material, bins of 5 nm from 360 nm to 830 nm) and the
      * environment's 95-bin spectrum (may be NULL = 0).  NULL spectra: the
-     * scene renders in RGB only. */
+     * materials' and the environment's RGB values are uplifted to spectra by
+     * crh_rgb_to_spectrum, so every scene can also render spectrally. */
     const float* spectra;
     const float* env_spectrum;
 } crh_scene_desc;
@@ -393,6 +394,16 @@ int crh_render_guided(crh_scene* scene, const crh_camera* cam,
 int crh_cie_cmf(float* xyz);
 /* XYZ of a 95-bin spectrum, normalised so that a constant 1 has Y = 1. */
 int crh_spectrum_to_xyz(const float* spectrum, float* xyz);
+/* RGB -> 95-bin spectrum, replacing BroadSpectrum::fromRGB
+ * (Core/Spectrum.cpp:49, which returns zeros; its Smits tables are
+ * Core/SpectrumTabData.h:67-257): Smits's decomposition into the flat white
+ * spectrum plus one secondary and one primary basis spectrum, the basis
+ * recomputed for this library's colour matching functions so that the
+ * white-balanced linear sRGB of the result (flat 1 -> (1, 1, 1)) equals the
+ * input.  Reflectances in [0, 1] map into [0, 1]; homogeneous of degree 1, so
+ * emission uses it too.  Negative channels count as 0.  rgb: 3 floats;
+ * spectrum: 95 floats out. */
+int crh_rgb_to_spectrum(const float* rgb, float* spectrum);
 
 /* ---- batched rendering (no counterpart in the reference, which renders
  * one scene per PathTraceIntegrator::dispatch, Core/Integrator.cu:233-262) - */
