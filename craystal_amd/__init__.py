@@ -8,11 +8,11 @@ PathTraceIntegrator API.  Build the library with
 from ._lib import RuntimeError, load  # noqa: A004
 from .core import (BVH, SPECTRUM_BINS, SPECTRUM_LAMBDA, Camera, Configs, DeviceBuffer,
                    MaterialData, MaterialFlags, MaterialType, MeshData, PathTraceIntegrator,
-                   Scene, SceneData, Sensor, cie_cmf, device_count, get_context,
+                   Scene, SceneData, Sensor, SphereData, cie_cmf, device_count, get_context,
                    rgb_to_spectrum, spectrum_to_xyz, trace_any_device, trace_closest_device)
 
 __all__ = ["RuntimeError", "load", "BVH", "Camera", "Configs", "DeviceBuffer", "MaterialData",
            "MaterialFlags", "MaterialType", "MeshData", "PathTraceIntegrator", "Scene",
-           "SceneData", "Sensor", "device_count", "get_context", "trace_any_device",
+           "SceneData", "SphereData", "Sensor", "device_count", "get_context", "trace_any_device",
            "trace_closest_device", "SPECTRUM_BINS", "SPECTRUM_LAMBDA", "cie_cmf",
            "spectrum_to_xyz", "rgb_to_spectrum"]

@@ -26,6 +26,7 @@ CRH_ERR_OOM = 3
 CRH_ERR_UNSUPPORTED = 4
 CRH_ERR_INTERNAL = 5
 CRH_INVALID_PRIM = 0xFFFFFFFF
+CRH_HIT_SPHERE = 0x40000000   # crh_hit.prim of a sphere hit: index | this bit
 
 MAT_DIFFUSE, MAT_PRINCIPLED, MAT_PHONG = 0, 1, 2
 MATF_EMISSIVE, MATF_DIFFUSE, MATF_DIFFUSE_TEXTURE = 1, 2, 4
@@ -74,12 +75,19 @@ class crh_mesh(C.Structure):
                 ("nindex", u32), ("material_id", u32)]
 
 
+class crh_texture(C.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("channels", u32), ("data", C.POINTER(f32))]
+
+
 class crh_scene_desc(C.Structure):
     _fields_ = [("meshes", C.POINTER(crh_mesh)), ("nmesh", u32),
                 ("materials", C.POINTER(crh_material)), ("nmaterial", u32),
                 ("emissive_index", C.POINTER(u32)), ("nemissive", u32),
                 ("env_radiance", f32 * 3), ("bvh", crh_bvh_params),
-                ("spectra", C.POINTER(f32)), ("env_spectrum", C.POINTER(f32))]
+                ("spectra", C.POINTER(f32)), ("env_spectrum", C.POINTER(f32)),
+                ("sphere_center", C.POINTER(f32)), ("sphere_radius", C.POINTER(f32)),
+                ("nsphere", u32), ("sphere_material", C.POINTER(u32)),
+                ("textures", C.POINTER(crh_texture)), ("ntexture", u32)]
 
 
 class crh_camera(C.Structure):

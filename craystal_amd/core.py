@@ -165,10 +165,25 @@ class MeshData:
 
 
 @dataclass
+class SphereData:
+    """SphereData (Scene/Sphere.h)."""
+    center: tuple = (0.0, 0.0, 0.0)
+    radius: float = 1.0
+
+
+@dataclass
 class SceneData:
+    """SceneData (Scene/Scene.h:40-47): meshes, spheres, materials, texture
+    images ((H, W, C) float arrays, C = 1..4) and the emissive triangle list,
+    plus this library's environment, BVH parameters and spectra."""
     meshes: list = field(default_factory=list)
     materials: list = field(default_factory=list)
     emissiveIndex: list = field(default_factory=list)
+    spheres: list = field(default_factory=list)
+    textureImages: list = field(default_factory=list)
+    # Per-sphere material ids; None = the reference's rule (a sphere takes the
+    # material of the triangle with its index).
+    sphereMaterials: object = None
     # Constant environment radiance on a miss (zero = the reference).
     envRadiance: tuple = (0.0, 0.0, 0.0)
     # BVH parameters (bins, max leaf, max depth); zeros take the reference's.
@@ -235,6 +250,32 @@ class SceneData:
                     raise ValueError(f"envSpectrum must have {SPECTRUM_BINS} bins")
                 keep.append(env)
                 d.env_spectrum = env.ctypes.data_as(C.POINTER(C.c_float))
+        if self.spheres:
+            cen = np.ascontiguousarray([sp.center for sp in self.spheres], np.float32)
+            rad = np.ascontiguousarray([sp.radius for sp in self.spheres], np.float32)
+            keep += [cen, rad]
+            d.sphere_center = cen.ctypes.data_as(C.POINTER(C.c_float))
+            d.sphere_radius = rad.ctypes.data_as(C.POINTER(C.c_float))
+            d.nsphere = len(self.spheres)
+            if self.sphereMaterials is not None:
+                sm = np.ascontiguousarray(self.sphereMaterials, np.uint32).reshape(-1)
+                if len(sm) != len(self.spheres):
+                    raise ValueError("sphereMaterials needs one id per sphere")
+                keep.append(sm)
+                d.sphere_material = sm.ctypes.data_as(C.POINTER(C.c_uint32))
+        if self.textureImages:
+            tex = (L.crh_texture * len(self.textureImages))()
+            for i, im in enumerate(self.textureImages):
+                a = np.asarray(im, np.float32)
+                if a.ndim == 2:
+                    a = a[..., None]
+                a = np.ascontiguousarray(a)
+                keep.append(a)
+                tex[i].height, tex[i].width, tex[i].channels = a.shape
+                tex[i].data = a.ctypes.data_as(C.POINTER(C.c_float))
+            keep.append(tex)
+            d.textures = C.cast(tex, C.POINTER(L.crh_texture))
+            d.ntexture = len(self.textureImages)
         return d, keep
 
 

@@ -208,6 +208,7 @@ struct crh_scene {
     CtxRef ctx;
     crh::BvhResult bvh;
     DevMem nodes, tris, pos, nrm, uv, idx, primMat, mats, emissive, spec;
+    DevMem spheres, sphereMat, texels, texInfo;
     crh::SceneView view{};
 };
 
@@ -659,6 +660,46 @@ int crh_scene_create(crh_context* ctx, const crh_scene_desc* d,
     const uint32_t ntri = uint32_t(idx.size() / 3);
     for (uint32_t i = 0; i < d->nemissive; ++i)
         ARG_CHECK(d->emissive_index[i] < ntri, "emissive index out of range");
+    // Spheres: centre + radius as 4 floats; their materials.
+    ARG_CHECK(d->nsphere == 0 || (d->sphere_center && d->sphere_radius),
+              "sphere_center / sphere_radius is NULL");
+    ARG_CHECK(d->nsphere < crh::kSphereBit, "too many spheres");
+    std::vector<float> sph(4 * size_t(d->nsphere));
+    for (uint32_t i = 0; i < d->nsphere; ++i) {
+        for (int k = 0; k < 3; ++k) sph[4 * size_t(i) + k] = d->sphere_center[3 * size_t(i) + k];
+        sph[4 * size_t(i) + 3] = d->sphere_radius[i];
+    }
+    std::vector<uint32_t> sphMat;
+    if (d->nsphere && d->sphere_material) {
+        sphMat.assign(d->sphere_material, d->sphere_material + d->nsphere);
+        for (uint32_t id : sphMat) ARG_CHECK(id < d->nmaterial, "sphere material out of range");
+    } else {
+        ARG_CHECK(d->nsphere <= ntri,
+                  "without sphere_material a sphere takes the material of the triangle with "
+                  "its index, so the sphere count must not exceed the triangle count");
+    }
+    // Textures: every image as 4-float texels, back to back.
+    ARG_CHECK(d->ntexture == 0 || d->textures, "textures is NULL");
+    std::vector<float> texels;
+    std::vector<uint32_t> texInfo(4 * size_t(d->ntexture), 0u);
+    for (uint32_t t = 0; t < d->ntexture; ++t) {
+        const crh_texture& tx = d->textures[t];
+        ARG_CHECK(tx.width > 0 && tx.height > 0 && tx.data, "empty texture");
+        ARG_CHECK(tx.channels >= 1 && tx.channels <= 4, "texture channels must be 1..4");
+        const size_t first = texels.size() / 4, n = size_t(tx.width) * tx.height;
+        ARG_CHECK(first + n < (size_t(1) << 32), "texture data too large");
+        texInfo[4 * size_t(t)] = uint32_t(first);
+        texInfo[4 * size_t(t) + 1] = tx.width;
+        texInfo[4 * size_t(t) + 2] = tx.height;
+        texels.resize(4 * (first + n), 0.0f);
+        for (size_t i = 0; i < n; ++i)
+            for (uint32_t c = 0; c < tx.channels; ++c)
+                texels[4 * (first + i) + c] = tx.data[i * tx.channels + c];
+    }
+    for (uint32_t i = 0; i < d->nmaterial; ++i)
+        ARG_CHECK(!(d->materials[i].flags & CRH_MATF_DIFFUSE_TEXTURE) ||
+                      d->materials[i].diffuse_tex < d->ntexture,
+                  "diffuse_tex out of range");
     crh::BuildParams bp = crh::resolveParams(&d->bvh);
     ARG_CHECK(bp.bins >= 2 && bp.bins <= 1024, "bins must be in [2, 1024]");
 
@@ -679,6 +720,7 @@ int crh_scene_create(crh_context* ctx, const crh_scene_desc* d,
         o.type = m.type;
         o.flags = m.flags;
         o.shininess = m.shininess;
+        o.tex = m.diffuse_tex;
     }
     std::vector<uint32_t> em(d->emissive_index, d->emissive_index + d->nemissive);
 
@@ -712,6 +754,10 @@ int crh_scene_create(crh_context* ctx, const crh_scene_desc* d,
         specTab = crh::buildSpectralTable(sp.data(), d->nmaterial, env);
     }
     if (e == hipSuccess) e = upload(s->spec, specTab, st);
+    if (e == hipSuccess) e = upload(s->spheres, sph, st);
+    if (e == hipSuccess) e = upload(s->sphereMat, sphMat, st);
+    if (e == hipSuccess) e = upload(s->texels, texels, st);
+    if (e == hipSuccess) e = upload(s->texInfo, texInfo, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         delete s;
@@ -737,6 +783,12 @@ int crh_scene_create(crh_context* ctx, const crh_scene_desc* d,
     v.mats = s->mats.as<crh::GMaterial>();
     v.emissive = s->emissive.as<uint32_t>();
     v.spec = s->spec.as<float>();
+    v.spheres = s->spheres.as<float>();
+    v.sphereMat = sphMat.empty() ? nullptr : s->sphereMat.as<uint32_t>();
+    v.nSphere = d->nsphere;
+    v.texels = s->texels.as<float>();
+    v.texInfo = s->texInfo.as<uint32_t>();
+    v.nTexture = d->ntexture;
     *out = s;
     return CRH_OK;
     GUARD_END
@@ -1422,10 +1474,12 @@ int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh
     HIP_TRY(hipSetDevice(ctx->device));
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     // Region r of the pool holds member r of every scene, scene after scene.
-    DevMem crh_scene::*members[] = {&crh_scene::nodes, &crh_scene::tris,    &crh_scene::pos,
-                                    &crh_scene::nrm,   &crh_scene::uv,      &crh_scene::idx,
-                                    &crh_scene::primMat, &crh_scene::mats, &crh_scene::emissive};
-    constexpr int kR = 9;
+    DevMem crh_scene::*members[] = {
+        &crh_scene::nodes,    &crh_scene::tris,    &crh_scene::pos,     &crh_scene::nrm,
+        &crh_scene::uv,       &crh_scene::idx,     &crh_scene::primMat, &crh_scene::mats,
+        &crh_scene::emissive, &crh_scene::spheres, &crh_scene::sphereMat,
+        &crh_scene::texels,   &crh_scene::texInfo};
+    constexpr int kR = 13;
     std::vector<size_t> off(size_t(n) * kR);
     size_t total = 0;
     for (int r = 0; r < kR; ++r)
@@ -1464,6 +1518,10 @@ int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh
         v.primMat = reinterpret_cast<const uint32_t*>(at(6));
         v.mats = reinterpret_cast<const crh::GMaterial*>(at(7));
         v.emissive = reinterpret_cast<const uint32_t*>(at(8));
+        v.spheres = reinterpret_cast<const float*>(at(9));
+        v.sphereMat = sc->view.sphereMat ? reinterpret_cast<const uint32_t*>(at(10)) : nullptr;
+        v.texels = reinterpret_cast<const float*>(at(11));
+        v.texInfo = reinterpret_cast<const uint32_t*>(at(12));
         v.spec = nullptr;  // batched rendering transports RGB
         b->views[i] = v;
         b->nodes += sc->nodes.bytes / sizeof(crh::GNode);

@@ -114,9 +114,9 @@ CRH_D bool misBsdfRay(const BSDF& bsdf, Sampler& smp, const Isect& it, Ray& r,
 CRH_D void misBsdfPart(const SceneView& s, F3 pos, const Ray& r, bool isHit,
                        const Hit& h, uint32_t eid, Spec bw, float bp,
                        Spec& value, const Lanes& w = rgbLanes()) {
-    if (!(isHit && h.prim == eid)) return;
+    if (!(isHit && hitsLight(s, h.prim, eid))) return;
     Isect lit = makeIsect(s, r, h);
-    const uint32_t mid = s.primMat[h.prim];
+    const uint32_t mid = matIdOf(s, h.prim);
     const GMaterial& m = s.mats[mid];
     if (!(m.flags & 1u)) return;
     F3 toL = lit.pos - pos;
@@ -124,7 +124,7 @@ CRH_D void misBsdfPart(const SceneView& s, F3 pos, const Ray& r, bool isHit,
     F3 lv = toL / sqrtf(d2);
     float cl = absDot(lv, lit.faceN);
     if (cl != 0.0f) {
-        float area = triArea(getTri(s, h.prim));
+        float area = triArea(getTri(s, eid));  // the matched index's triangle
         float lpdf = d2 / (area * cl);
         value = value + powerHeur(bp, lpdf) * emissionOf(m, mid, w) * bw / bp;
     }

@@ -168,13 +168,13 @@ CRH_D void tracePixel(const SceneView& s, Stack& stk, const CameraView& cam,
         Hit h;
         if (closest(c, ray, h)) {
             Isect it = makeIsect(s, ray, h);
-            const uint32_t mid = s.primMat[h.prim];
+            const uint32_t mid = matIdOf(s, h.prim);
             const GMaterial& m = s.mats[mid];
             if (m.flags & 1u) {
                 rad = rad + emissionOf(m, mid, w) * beta;
                 break;
             }
-            BSDF bsdf = makeBSDF(m, mid, it.frame, w);
+            BSDF bsdf = makeBSDF(s, m, mid, it, h, w);
             if (MIS) {
                 rad = rad + evalMIS(c, it, bsdf, smp, w) * beta;
             } else {
@@ -317,13 +317,13 @@ kRenderRegen(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint3
             Hit h;
             if (closest(c, ray, h)) {
                 Isect it = makeIsect(s, ray, h);
-                const uint32_t mid = s.primMat[h.prim];
+                const uint32_t mid = matIdOf(s, h.prim);
                 const GMaterial& m = s.mats[mid];
                 if (m.flags & 1u) {
                     rad = rad + emissionOf(m, mid, w) * beta;
                     done = true;
                 } else {
-                    BSDF bsdf = makeBSDF(m, mid, it.frame, w);
+                    BSDF bsdf = makeBSDF(s, m, mid, it, h, w);
                     if (MIS) {
                         rad = rad + evalMIS(c, it, bsdf, smp, w) * beta;
                     } else {

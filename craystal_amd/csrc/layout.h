@@ -20,6 +20,8 @@
 namespace crh {
 
 constexpr uint32_t kLeafBit = 0x80000000u;
+// Hit::prim of a sphere hit: the sphere index with this bit set.
+constexpr uint32_t kSphereBit = 0x40000000u;
 
 struct alignas(16) GNode {
     float lmin[3];
@@ -52,8 +54,10 @@ struct alignas(16) GMaterial {
     uint32_t flags;
     float emission[3];
     float shininess;
+    uint32_t tex;  // diffuse texture (MaterialData::diffuseHandle)
+    uint32_t pad[3];
 };
-static_assert(sizeof(GMaterial) == 48, "GMaterial must be 48 B");
+static_assert(sizeof(GMaterial) == 64, "GMaterial must be 64 B");
 
 // Everything a kernel needs about one scene, passed by value.
 struct SceneView {
@@ -75,9 +79,19 @@ struct SceneView {
     const uint32_t* emissive;
     float env[3];
     uint32_t pad;
-    // Spectral transport table (layout in shading.h, kSpec*), null when the
-    // scene has no spectra.
+    // Spectral transport table (layout in spec_layout.h, kSpec*).
     const float* spec;
+    // Spheres (SphereSOA, Scene/Sphere.h): 4 floats each (centre, radius);
+    // sphereMat null = the reference's material rule (the material of the
+    // triangle with the same index).
+    const float* spheres;
+    const uint32_t* sphereMat;
+    // Textures: 4 floats per texel, all images back to back; per texture 4
+    // words (first texel, width, height, 0).
+    const float* texels;
+    const uint32_t* texInfo;
+    uint32_t nSphere;
+    uint32_t nTexture;
 };
 
 struct CameraView {

@@ -123,13 +123,83 @@ CRH_D Isect makeIsect(const SceneView& s, const Ray& ray, const Hit& h) {
     Isect it;
     it.pos = ray.o + h.t * ray.d;
     it.view = -normalize(ray.d);
-    F3 w = mk3(1.f - h.u - h.v, h.u, h.v);
-    TriData tri = getTri(s, h.prim);
-    it.faceN = faceNormal(tri);
-    F3 sn = normalize(interp3(tri.n, w));
+    F3 sn;
+    if (h.prim & kSphereBit) {  // HitType::Sphere: both normals radial
+        const float* c = s.spheres + 4 * size_t(h.prim & ~kSphereBit);
+        it.faceN = normalize(it.pos - mk3(c[0], c[1], c[2]));
+        sn = it.faceN;
+    } else {
+        F3 w = mk3(1.f - h.u - h.v, h.u, h.v);
+        TriData tri = getTri(s, h.prim);
+        it.faceN = faceNormal(tri);
+        sn = normalize(interp3(tri.n, w));
+    }
     it.front = dot(it.view, it.faceN) > 0.0f;
     it.frame = makeFrame(it.front ? sn : -sn);
     return it;
+}
+
+// Material of a hit primitive: Core/Integrator.cu:172-178 reads the mesh
+// descriptor of primitiveIndex whatever the hit type, so a sphere takes the
+// material of the triangle with its index unless the scene gives spheres
+// their own materials.
+CRH_D uint32_t matIdOf(const SceneView& s, uint32_t prim) {
+    if (prim & kSphereBit) {
+        const uint32_t i = prim & ~kSphereBit;
+        return s.sphereMat ? s.sphereMat[i] : s.primMat[i];
+    }
+    return s.primMat[prim];
+}
+// evalMIS's test that the BSDF ray found the sampled light
+// (Core/Integrator.cu:108) compares primitive indices only; with the
+// reference's sphere materials a sphere of the same index matches too.
+CRH_D bool hitsLight(const SceneView& s, uint32_t prim, uint32_t eid) {
+    return s.sphereMat ? prim == eid : (prim & ~kSphereBit) == eid;
+}
+
+// The hit's texture coordinate (Scene/Scene.cu:36-55): a triangle's
+// interpolated vertex coordinates; a sphere's (u, v) parameterisation of the
+// unit normal at the hit point (Scene/Sphere.cu:30-37).
+CRH_D F2 hitUV(const SceneView& s, const Hit& h, const Isect& it) {
+    if (h.prim & kSphereBit) {
+        const F3 n = it.faceN;
+        float theta = acosf(-n.y);
+        float phi = atan2f(-n.z, n.x) + kPi;
+        return F2{phi * (0.5f * kInvPi), theta * kInvPi};
+    }
+    F3 w = mk3(1.f - h.u - h.v, h.u, h.v);
+    const uint32_t* vi = s.idx + 3 * size_t(h.prim);
+    F2 t0{s.uv[2 * size_t(vi[0])], s.uv[2 * size_t(vi[0]) + 1]};
+    F2 t1{s.uv[2 * size_t(vi[1])], s.uv[2 * size_t(vi[1]) + 1]};
+    F2 t2{s.uv[2 * size_t(vi[2])], s.uv[2 * size_t(vi[2]) + 1]};
+    return F2{t0.x * w.x + t1.x * w.y + t2.x * w.z, t0.y * w.x + t1.y * w.y + t2.y * w.z};
+}
+
+// --- Textures (Core/Texture.cu:6-35) ----------------------------------------
+// float -> uint32 as the GPU converts: NaN and negatives 0, saturating.
+CRH_D uint32_t satU32(float f) {
+    return f >= 4294967296.0f ? 0xFFFFFFFFu : (f > 0.0f ? uint32_t(f) : 0u);
+}
+CRH_D float4 lerp4(float4 a, float4 b, float t) {
+    const float r = 1.0f - t;
+    return make_float4(a.x * r + b.x * t, a.y * r + b.y * t, a.z * r + b.z * t,
+                       a.w * r + b.w * t);
+}
+// TextureView::sample: bilinear over texels clamped to the image.
+CRH_D float4 sampleTexture(const SceneView& s, uint32_t tex, F2 uv) {
+    const uint4 ti = reinterpret_cast<const uint4*>(s.texInfo)[tex];
+    const float4* px = reinterpret_cast<const float4*>(s.texels) + ti.x;
+    const uint32_t w = ti.y, h = ti.z;
+    const float fxp = uv.x * float(w - 1u);
+    const float fyp = uv.y * float(h - 1u);
+    const uint32_t x0 = satU32(floorf(fxp)), y0 = satU32(floorf(fyp));
+    const uint32_t x1 = min(x0 + 1u, w - 1u), y1 = min(y0 + 1u, h - 1u);
+    const float fx = fxp - float(x0), fy = fyp - float(y0);
+    const uint32_t cx0 = min(x0, w - 1u), cy0 = min(y0, h - 1u);
+    const uint32_t cx1 = min(x1, w - 1u), cy1 = min(y1, h - 1u);
+    float4 c00 = px[size_t(cy0) * w + cx0], c10 = px[size_t(cy0) * w + cx1];
+    float4 c01 = px[size_t(cy1) * w + cx0], c11 = px[size_t(cy1) * w + cx1];
+    return lerp4(lerp4(c00, c10, fx), lerp4(c01, c11, fx), fy);
 }
 
 // --- Sampling warps (Math/Sampling.cu) -------------------------------------
@@ -198,12 +268,58 @@ CRH_D BSDF makeBSDF(const GMaterial& m, const Frame& f) {
     return makeBSDFWith(m, f, kd, ks, savg(kd), savg(ks));
 }
 
-// The material of primitive `mid` under the path's lanes.
-CRH_D BSDF makeBSDF(const GMaterial& m, uint32_t mid, const Frame& f, const Lanes& w) {
-    if (!w.tab) return makeBSDF(m, f);
+// Smits's RGB -> spectrum decomposition (crh_rgb_to_spectrum) of one
+// texel, evaluated at the path's three bins; avg = its average over all
+// bins.  The basis rows and their averages live in the spectral table.
+CRH_D Spec upliftLanes(const float* tab, const Lanes& w, float r, float g, float b, float& avg) {
+    r = smax(r, 0.0f);
+    g = smax(g, 0.0f);
+    b = smax(b, 0.0f);
+    float white, w1, w2;
+    uint32_t b1, b2;
+    if (r <= g && r <= b) {
+        white = r;
+        b1 = 0u;
+        if (g <= b) { w1 = g - r; b2 = 5u; w2 = b - g; }
+        else        { w1 = b - r; b2 = 4u; w2 = g - b; }
+    } else if (g <= r && g <= b) {
+        white = g;
+        b1 = 1u;
+        if (r <= b) { w1 = r - g; b2 = 5u; w2 = b - r; }
+        else        { w1 = b - g; b2 = 3u; w2 = r - b; }
+    } else {
+        white = b;
+        b1 = 2u;
+        if (r <= g) { w1 = r - b; b2 = 4u; w2 = g - r; }
+        else        { w1 = g - b; b2 = 3u; w2 = r - g; }
+    }
+    const float* p1 = tab + kSpecBasis + b1 * kSpecBins;
+    const float* p2 = tab + kSpecBasis + b2 * kSpecBins;
+    avg = white + w1 * tab[kSpecBasisAvg + b1] + w2 * tab[kSpecBasisAvg + b2];
+    return {white + w1 * p1[w.b0] + w2 * p2[w.b0], white + w1 * p1[w.b1] + w2 * p2[w.b1],
+            white + w1 * p1[w.b2] + w2 * p2[w.b2]};
+}
+
+// The BSDF at a hit (getBSDF, Material/BSDF.cu:6-32): the material's
+// diffuse reflectance or its texture at the hit's coordinate, under the
+// path's lanes.
+CRH_D BSDF makeBSDF(const SceneView& s, const GMaterial& m, uint32_t mid, const Isect& it,
+                    const Hit& h, const Lanes& w) {
+    if (!(m.flags & 4u)) {
+        if (!w.tab) return makeBSDF(m, it.frame);
+        const float* sm = specMat(w, mid);
+        return makeBSDFWith(m, it.frame, laneSpec(sm + kSpecKd, w), laneSpec(sm + kSpecKs, w),
+                            sm[kSpecAvgKd], sm[kSpecAvgKs]);
+    }
+    const float4 t = sampleTexture(s, m.tex, hitUV(s, h, it));
+    if (!w.tab) {
+        Spec kd{t.x, t.y, t.z}, ks = sp3(m.ks);
+        return makeBSDFWith(m, it.frame, kd, ks, savg(kd), savg(ks));
+    }
     const float* sm = specMat(w, mid);
-    return makeBSDFWith(m, f, laneSpec(sm + kSpecKd, w), laneSpec(sm + kSpecKs, w),
-                        sm[kSpecAvgKd], sm[kSpecAvgKs]);
+    float avgKd;
+    Spec kd = upliftLanes(w.tab, w, t.x, t.y, t.z, avgKd);
+    return makeBSDFWith(m, it.frame, kd, laneSpec(sm + kSpecKs, w), avgKd, sm[kSpecAvgKs]);
 }
 CRH_D Spec emissionOf(const GMaterial& m, uint32_t mid, const Lanes& w) {
     return w.tab ? laneSpec(specMat(w, mid) + kSpecEm, w) : sp3(m.emission);

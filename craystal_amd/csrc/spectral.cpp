@@ -100,6 +100,10 @@ std::vector<float> buildSpectralTable(const float* spectra, uint32_t nmaterial,
     }
     if (envSpectrum)
         for (int i = 0; i < kCieBins; ++i) t[kEnv + i] = envSpectrum[i];
+    for (int b = 0; b < 6; ++b) {
+        for (int i = 0; i < kCieBins; ++i) t[kSpecBasis + b * kCieBins + i] = kRgbBasis[b][i];
+        t[kSpecBasisAvg + b] = average(kRgbBasis[b]);
+    }
     for (uint32_t m = 0; m < nmaterial; ++m) {
         float* dst = t.data() + kMat0 + size_t(m) * kMatStride;
         const float* src = spectra + size_t(m) * 3 * kCieBins;

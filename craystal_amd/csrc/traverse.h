@@ -66,6 +66,56 @@ struct ShortStack {
     CRH_D uint2 get(int k) const { return k < kShortStack ? base[k * stride] : a[k - kShortStack]; }
 };
 
+// SphereSOA::intersect (Scene/Sphere.cu:6-40) with intersectShape's range
+// test (Scene/Shape.h:17-37): the nearer root if it lies in [tMin, tMax],
+// else the farther one.  (u, v) are not computed here; shading derives them
+// from the hit point when a texture needs them.
+CRH_D bool sphereHit(const float* sph, const Ray& ray, float& t) {
+    const float4 q = *reinterpret_cast<const float4*>(sph);
+    F3 oc = ray.o - mk3(q.x, q.y, q.z);
+    float a = dot(ray.d, ray.d);
+    float b = 2.0f * dot(oc, ray.d);
+    float c = dot(oc, oc) - q.w * q.w;
+    float disc = b * b - 4.0f * a * c;
+    if (disc < 0) return false;
+    float sq = sqrtf(disc);
+    float t0 = (-b - sq) / (2.0f * a);
+    float t1 = (-b + sq) / (2.0f * a);
+    if (t0 > t1) {
+        float x = t0;
+        t0 = t1;
+        t1 = x;
+    }
+    if (t0 < ray.tMin || t0 > ray.tMax) {
+        if (t1 < ray.tMin || t1 > ray.tMax) return false;
+        t0 = t1;
+    }
+    t = t0;
+    return true;
+}
+
+// The sphere loop SceneView::intersect runs before the BVH
+// (Scene/Scene.cu:5-29): every sphere in index order, a hit replacing the
+// current one only if strictly nearer.  ANY: true at the first hit.
+template <bool ANY>
+CRH_D bool sphereLoop(const SceneView& s, const Ray& ray, Hit& hit) {
+    bool isHit = false;
+    for (uint32_t i = 0; i < s.nSphere; ++i) {
+        float t;
+        if (sphereHit(s.spheres + 4 * size_t(i), ray, t)) {
+            isHit = true;
+            if (ANY) return true;
+            if (t < hit.t) {
+                hit.t = t;
+                hit.u = 0.f;
+                hit.v = 0.f;
+                hit.prim = i | kSphereBit;
+            }
+        }
+    }
+    return isHit;
+}
+
 CRH_D void loadNode(const GNode* nodes, uint32_t i, float4& a, float4& b,
                     float4& c, uint4& d) {
     const float4* p = reinterpret_cast<const float4*>(nodes + i);
@@ -84,12 +134,16 @@ CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, Stack& stack,
     hit.t = __builtin_inff();
     hit.u = 0.f;
     hit.v = 0.f;
-    if (!s.hasGeom) return false;
+    bool isHit = false;
+    if (s.nSphere) {
+        isHit = sphereLoop<ANY>(s, ray, hit);
+        if (ANY && isHit) return true;
+    }
+    if (!s.hasGeom) return isHit;
     RayPre r = prep(ray);
     float dummy;
-    if (!boxHit(s.rootMin, s.rootMax, r, dummy)) return false;
+    if (!boxHit(s.rootMin, s.rootMax, r, dummy)) return isHit;
 
-    bool isHit = false;
     float closestT = hit.t;
     uint32_t cur = s.rootRef;
     int sp = 0;
@@ -185,13 +239,19 @@ struct TravState {
     bool isHit;
 };
 
-// Returns true if the ray is already finished (empty scene / root miss).
+// Returns true if the ray is already finished (empty scene / root miss /
+// any-hit on a sphere).
+template <bool ANY>
 CRH_D bool travInit(const SceneView& s, const Ray& ray, TravState& t) {
     t.hit.prim = 0xFFFFFFFFu;
     t.hit.t = __builtin_inff();
     t.hit.u = 0.f;
     t.hit.v = 0.f;
     t.isHit = false;
+    if (s.nSphere) {
+        t.isHit = sphereLoop<ANY>(s, ray, t.hit);
+        if (ANY && t.isHit) return true;
+    }
     t.closestT = t.hit.t;
     t.sp = 0;
     t.cur = s.rootRef;
@@ -301,7 +361,7 @@ CRH_D void persistentTrace(const SceneView& s, Src& src, uint32_t* counter, uint
                     uint32_t idx = base + below;
                     if (idx < n) {
                         mine = idx;
-                        if (travInit(s, src.load(idx), t)) {
+                        if (travInit<ANY>(s, src.load(idx), t)) {
                             src.store(idx, t.isHit, t.hit);
                             mine = kNone;
                         }

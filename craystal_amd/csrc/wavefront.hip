@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void kWfShadeHit(SceneView s, WfState w,
         } else {
             Ray r = loadRay(w.rayO, w.rayD, p);
             Hit h = loadHit(w.hit, p);
-            const GMaterial& m = s.mats[s.primMat[h.prim]];
+            const GMaterial& m = s.mats[matIdOf(s, h.prim)];
             if (m.flags & 1u) {
                 stS(w.rad, p, ldS(w.rad, p) + sp3(m.emission) * beta);
             } else {
@@ -320,8 +320,9 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
         Ray r = loadRay(w.rayO, w.rayD, p);
         Hit h = loadHit(w.hit, p);
         Isect it = makeIsect(s, r, h);
-        const GMaterial& m = s.mats[s.primMat[h.prim]];
-        BSDF bsdf = makeBSDF(m, it.frame);
+        const uint32_t mid = matIdOf(s, h.prim);
+        const GMaterial& m = s.mats[mid];
+        BSDF bsdf = makeBSDF(s, m, mid, it, h, rgbLanes());
         Sampler smp = loadSampler(w, p);
         Spec beta = ldS(w.beta, p);
         Spec rad = ldS(w.rad, p);

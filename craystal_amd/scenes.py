@@ -11,7 +11,8 @@ import math
 
 import numpy as np
 
-from .core import Camera, MaterialData, MaterialType, MeshData, SceneData
+from .core import (Camera, MaterialData, MaterialFlags, MaterialType, MeshData, SceneData,
+                   SphereData)
 
 SCENE_SEED = 0xC0FFEE
 RAY_SEED = 0x5EED
@@ -235,6 +236,43 @@ def cornell_box(seed: int = SCENE_SEED, light_radiance=(17.0, 12.0, 4.0)):
     data.emissiveIndex = _emissive_prims(data)
     cam = dict(pos=(0.0, 1.0, 3.9), target=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0), fov=40.0,
                size=(256, 256), spp=16, max_depth=8, rr_start_depth=3)
+    return data, cam
+
+
+def textured_sphere_box(sphere_materials: bool = False, seed: int = SCENE_SEED):
+    """The Cornell-style box with three spheres and two textures: an 8 x 8
+    RGB checker on the back wall (Lambertian) and a 16 x 4 RGBA gradient on
+    the floor (Phong).  sphere_materials=False keeps the reference's rule (a
+    sphere takes the material of the triangle with its index: spheres 0
+    and 1 the textured floor's, sphere 2 the ceiling's); True gives them a
+    textured Principled, the red wall's and the floor's materials.  Returns
+    (SceneData, camera kwargs)."""
+    data, cam = cornell_box(seed)
+    yy, xx = np.mgrid[0:8, 0:8]
+    check = np.where(((xx + yy) % 2 == 0)[..., None], (0.85, 0.8, 0.3), (0.15, 0.3, 0.7))
+    u = np.linspace(0.0, 1.0, 16, dtype=np.float32)
+    grad = np.zeros((4, 16, 4), np.float32)
+    grad[..., 0] = u
+    grad[..., 1] = 0.6 - 0.4 * u
+    grad[..., 2] = np.linspace(0.2, 0.8, 4, dtype=np.float32)[:, None]
+    grad[..., 3] = 1.0
+    data.textureImages = [check.astype(np.float32), grad]
+    tex_flag = MaterialFlags.hasDiffuseTexture
+    m4 = MaterialData(diffuseRefl=(0.5, 0.5, 0.5), diffuseHandle=0,
+                      type=MaterialType.FullDiffuse).finalize()
+    m5 = MaterialData(diffuseRefl=(0.5, 0.5, 0.5), specularRefl=(0.2, 0.2, 0.2), shininess=30.0,
+                      diffuseHandle=1, type=MaterialType.Phong).finalize()
+    m6 = MaterialData(diffuseRefl=(0.5, 0.5, 0.5), specularRefl=(0.0, 0.0, 0.0), shininess=40.0,
+                      diffuseHandle=0, type=MaterialType.Principled).finalize()
+    for m in (m4, m5, m6):
+        m.flags |= tex_flag
+    data.materials += [m4, m5, m6]
+    data.meshes[2].materialID = 4      # back wall
+    data.meshes[0].materialID = 5      # floor
+    data.spheres = [SphereData((-0.45, 0.25, 0.45), 0.25), SphereData((0.45, 0.85, 0.2), 0.22),
+                    SphereData((0.05, 1.45, -0.2), 0.18)]
+    if sphere_materials:
+        data.sphereMaterials = [6, 1, 5]
     return data, cam
 
 

@@ -53,7 +53,11 @@ typedef struct crh_ray {
 
 /* Closest hit, 16 B.  prim = mesh primitive ID (HitInfo::primitiveIndex,
  * Core/Hit.h:15-24), t = RayHit::hitT, (u, v) = HitInfo::barycentric.
- * A miss is prim = CRH_INVALID_PRIM, t = +inf. */
+ * A miss is prim = CRH_INVALID_PRIM, t = +inf.  A sphere hit
+ * (HitType::Sphere) is prim = sphere index | CRH_HIT_SPHERE with u = v = 0;
+ * its (u, v) parameterisation is needed only for texture lookups, where the
+ * integrator derives it from the hit point. */
+#define CRH_HIT_SPHERE 0x40000000u
 typedef struct crh_hit {
     uint32_t prim;
     float t;
@@ -119,6 +123,17 @@ typedef struct crh_material {
     uint32_t flags;
 } crh_material;
 
+/* A texture image (Image as Texture(const Image&) reads it,
+ * Core/Texture.cu:46-62): width x height pixels of `channels` (1..4) floats,
+ * pixel (x, y) at data[(y * width + x) * channels]; missing channels read 0
+ * (a one-channel image is red only, as in the reference). */
+typedef struct crh_texture {
+    uint32_t width;
+    uint32_t height;
+    uint32_t channels;
+    const float* data;
+} crh_texture;
+
 /* One mesh (MeshData, Scene/Triangle.h:36-42).  Indices are local to the
  * mesh; meshes are concatenated in order and the primitive ID is the global
  * triangle index in that order (TriangleMeshManager, Scene/Triangle.cu:98-165).
@@ -133,8 +148,8 @@ typedef struct crh_mesh {
     uint32_t material_id;
 } crh_mesh;
 
-/* SceneData (Scene/Scene.h:40-47) minus spheres/textures, plus the constant
- * environment radiance added by this build (0 = the reference's black miss). */
+/* SceneData (Scene/Scene.h:40-47), plus the constant environment radiance
+ * added by this build (0 = the reference's black miss). */
 typedef struct crh_scene_desc {
     const crh_mesh* meshes;
     uint32_t nmesh;
@@ -152,6 +167,29 @@ typedef struct crh_scene_desc {
      * crh_rgb_to_spectrum, so every scene can also render spectrally. */
     const float* spectra;
     const float* env_spectrum;
+    /* Spheres (SceneData::spheres, Scene/Scene.h:40-47; SphereData,
+     * Scene/Sphere.h): centre (3 floats) and radius of each, tested by every
+     * ray before the triangle BVH as SceneView::intersect does
+     * (Scene/Scene.cu:5-29).  The reference gives a sphere no material: it
+     * takes the material of the mesh holding the triangle with the same
+     * index (Core/Integrator.cu:172-178 reads getMeshDesc(primitiveIndex)),
+     * so with sphere_material = NULL every sphere index must be below the
+     * triangle count, and a sphere whose index equals the sampled emissive
+     * triangle's counts as that light in the MIS BSDF test
+     * (Core/Integrator.cu:108).  sphere_material (optional, nsphere ids)
+     * gives each sphere its own material instead, and MIS then matches
+     * triangles only. */
+    const float* sphere_center;
+    const float* sphere_radius;
+    uint32_t nsphere;
+    const uint32_t* sphere_material;
+    /* Diffuse textures (SceneData::textureImages), indexed by
+     * crh_material.diffuse_tex when its flags hold CRH_MATF_DIFFUSE_TEXTURE;
+     * sampled bilinearly at the hit's texture coordinate
+     * (TextureView::sample, Core/Texture.cu:15-35; getBSDF,
+     * Material/BSDF.cu:11-15). */
+    const crh_texture* textures;
+    uint32_t ntexture;
 } crh_scene_desc;
 
 /* Camera: the reference CameraProxy layout, 96 B (Core/CameraProxy.h:10-24).

@@ -109,12 +109,30 @@ struct MeshData {
     uint32_t materialID = 0;
 };
 
+// SphereData (Scene/Sphere.h).
+struct SphereData {
+    Float3 center{0, 0, 0};
+    float radius = 1.f;
+};
+
+// An Image as Texture(const Image&) reads it (Core/Texture.cu:46-62):
+// pixel (x, y) at pixels[(y * width + x) * channels].
+struct TextureImage {
+    uint32_t width = 0, height = 0, channels = 4;
+    std::vector<float> pixels;
+};
+
 struct SceneData {
+    std::vector<SphereData> spheres;
     std::vector<MeshData> meshes;
     std::vector<MaterialData> materials;
+    std::vector<TextureImage> textureImages;
     std::vector<uint32_t> emissiveIndex;
     Float3 envRadiance{0, 0, 0};  // constant environment; zero = reference
     crh_bvh_params bvhParams{32, 4, 20, 0};
+    // Per-sphere materials; empty = the reference's rule (a sphere takes the
+    // material of the triangle with its index).
+    std::vector<uint32_t> sphereMaterials;
 };
 
 class BVH {
@@ -266,6 +284,20 @@ class Scene {
         d.nemissive = uint32_t(data.emissiveIndex.size());
         for (int k = 0; k < 3; ++k) d.env_radiance[k] = data.envRadiance[k];
         d.bvh = data.bvhParams;
+        std::vector<float> cen, rad;
+        for (const auto& sp : data.spheres) {
+            cen.insert(cen.end(), {sp.center[0], sp.center[1], sp.center[2]});
+            rad.push_back(sp.radius);
+        }
+        d.sphere_center = cen.data();
+        d.sphere_radius = rad.data();
+        d.nsphere = uint32_t(data.spheres.size());
+        d.sphere_material = data.sphereMaterials.empty() ? nullptr : data.sphereMaterials.data();
+        std::vector<crh_texture> tex;
+        for (const auto& t : data.textureImages)
+            tex.push_back(crh_texture{t.width, t.height, t.channels, t.pixels.data()});
+        d.textures = tex.data();
+        d.ntexture = uint32_t(tex.size());
         detail::check(crh_scene_create(detail::context(device), &d, &mScene));
     }
     Scene(const Scene&) = delete;

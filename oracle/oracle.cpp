@@ -22,6 +22,11 @@
 //   BSDFs              closed forms (Lambert, Schlick, GGX normalisation)
 //   Integrator         furnace-style closed forms (emitter-only, white box)
 //   Film               tent-filter closed form, constant-image KAT
+//   Spheres            float64 roots, inside / range / miss cases, the (u, v)
+//                      closed form, brute force over spheres + triangles
+//   Textures           bilinear closed form, channel fill, saturating casts
+//   RGB uplift         colour round trip through the CIE table, bounds,
+//                      homogeneity, continuity across channel orderings
 // Parity unpinned at bit level: glm's exact operation order beyond the
 // camera KAT tolerance and CUDA libdevice transcendental results.
 #include <omp.h>
@@ -34,6 +39,7 @@
 #include <numeric>
 #include <vector>
 
+#include "../craystal_amd/csrc/rgb_basis.h"  // basis data of the RGB uplift
 #include "../include/crayhip.h"
 #include "vecmath.h"
 
@@ -181,6 +187,17 @@ struct Scene {
     bool spectral = false;
     std::vector<float> spectra, envSpec, avgKd, avgKs;
     float filmW[95][3] = {};
+    // Spheres (SphereSOA, Scene/Sphere.h): centre and radius; sphereMat
+    // empty = the reference's rule (the triangle with the same index).
+    std::vector<V3> sphC;
+    std::vector<float> sphR;
+    std::vector<uint32_t> sphereMat;
+    // Textures (Texture, Core/Texture.h): 4 floats per texel.
+    struct Tex {
+        uint32_t w, h;
+        std::vector<float> px;
+    };
+    std::vector<Tex> textures;
 };
 
 // ---------------------------------------------------------------------------
@@ -414,6 +431,7 @@ struct RayHit {
     Ray ray;
     V2 bary;
     uint32_t prim = CRH_INVALID_PRIM;
+    bool sphere = false;   // HitInfo::type == HitType::Sphere
     float hitT = kFltInf;  // Core/Hit.h:29
 };
 
@@ -426,6 +444,49 @@ inline bool intersectShape(const Scene& s, uint32_t id, RayHit& rh) {
     if (t < rh.hitT) {
         rh.bary = V2(u, v);
         rh.prim = id;
+        rh.sphere = false;
+        rh.hitT = t;
+    }
+    return true;
+}
+
+// SphereSOA::intersect (Scene/Sphere.cu:6-49): nearer root in range, else
+// the farther; (u, v) from the unit normal at the hit point.
+bool sphereIntersect(const Scene& s, uint32_t id, const Ray& ray, V2& uv, float& hitT) {
+    V3 center = s.sphC[id];
+    float radius = s.sphR[id];
+    V3 oc = ray.o - center;
+    float a = dot(ray.d, ray.d);
+    float b = 2.0f * dot(oc, ray.d);
+    float c = dot(oc, oc) - radius * radius;
+    float disc = b * b - 4.0f * a * c;
+    if (disc < 0) return false;
+    float sq = std::sqrt(disc);
+    float t0 = (-b - sq) / (2.0f * a);
+    float t1 = (-b + sq) / (2.0f * a);
+    if (t0 > t1) std::swap(t0, t1);
+    if (t0 < ray.tMin || t0 > ray.tMax) {
+        if (t1 < ray.tMin || t1 > ray.tMax) return false;
+        t0 = t1;
+    }
+    hitT = t0;
+    V3 hp = ray.o + ray.d * hitT;
+    V3 n = normalize(hp - center);
+    float theta = std::acos(-n.y);
+    float phi = std::atan2(-n.z, n.x) + kPi;
+    uv = V2(phi * (0.5f * kInvPi), theta * kInvPi);
+    return true;
+}
+// intersectShape (Scene/Shape.h:17-37) on a sphere.
+inline bool intersectSphereShape(const Scene& s, uint32_t id, RayHit& rh) {
+    V2 uv;
+    float t = kFltInf;
+    if (!sphereIntersect(s, id, rh.ray, uv, t)) return false;
+    if (t < rh.ray.tMin || t > rh.ray.tMax) return false;
+    if (t < rh.hitT) {
+        rh.bary = uv;
+        rh.prim = id;
+        rh.sphere = true;
         rh.hitT = t;
     }
     return true;
@@ -491,6 +552,20 @@ bool bvhIntersect(const Scene& s, RayHit& rh, Counters* cnt, bool anyHit) {
     return isHit;
 }
 
+// SceneView::intersect / intersectOcclusion (Scene/Scene.cu:5-29): every
+// sphere in index order, then the BVH over the triangles.  anyHit: the same
+// boolean with an early exit.
+bool sceneIntersect(const Scene& s, RayHit& rh, Counters* cnt, bool anyHit) {
+    bool hit = false;
+    for (uint32_t i = 0; i < uint32_t(s.sphR.size()); ++i)
+        if (intersectSphereShape(s, i, rh)) {
+            hit = true;
+            if (anyHit) return true;
+        }
+    if (bvhIntersect(s, rh, cnt, anyHit)) hit = true;
+    return hit;
+}
+
 // ---------------------------------------------------------------------------
 // Frame / Intersection (Core/Frame.cu:6-31, Core/Intersection.cu:4-10,
 // Scene/Scene.cu:31-58)
@@ -533,15 +608,96 @@ Isect createIntersection(const Scene& s, const RayHit& rh) {
     Isect it;
     it.posW = rh.ray.o + rh.hitT * rh.ray.d;
     it.viewW = -normalize(rh.ray.d);
-    V3 w(1.f - rh.bary.x - rh.bary.y, rh.bary.x, rh.bary.y);  // Hit.h:20-23
-    Tri tri = getTriangle(s, rh.prim);
-    it.faceN = tri.faceNormal();
-    Vtx vd = tri.interpolate(w);
-    V3 sn = normalize(vd.n);
-    it.uv = vd.uv;
+    V3 sn;
+    if (rh.sphere) {  // HitType::Sphere (:45-49)
+        it.faceN = normalize(it.posW - s.sphC[rh.prim]);
+        sn = it.faceN;
+        it.uv = rh.bary;
+    } else {
+        V3 w(1.f - rh.bary.x - rh.bary.y, rh.bary.x, rh.bary.y);  // Hit.h:20-23
+        Tri tri = getTriangle(s, rh.prim);
+        it.faceN = tri.faceNormal();
+        Vtx vd = tri.interpolate(w);
+        sn = normalize(vd.n);
+        it.uv = vd.uv;
+    }
     it.front = dot(it.viewW, it.faceN) > 0.0;
     it.frame = Frame(it.front ? sn : -sn);
     return it;
+}
+
+// TextureView::fetchTexel / sample (Core/Texture.cu:6-35).  The float ->
+// uint32 casts follow the GPU's conversion (NaN and negatives to 0,
+// saturating), which is what the reference's device code computes.
+inline uint32_t gpuU32(float f) {
+    if (!(f > 0.0f)) return 0u;
+    if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+    return uint32_t(f);
+}
+struct F4 {
+    float v[4];
+};
+inline F4 lerpF4(const F4& a, const F4& b, float t) {  // Math/CRayMath.h:31-33
+    F4 r;
+    for (int i = 0; i < 4; ++i) r.v[i] = a.v[i] * (1.0f - t) + b.v[i] * t;
+    return r;
+}
+F4 textureSample(const Scene::Tex& tx, V2 uv) {
+    auto fetch = [&](uint32_t x, uint32_t y) {
+        x = std::clamp<uint32_t>(x, 0, tx.w - 1);
+        y = std::clamp<uint32_t>(y, 0, tx.h - 1);
+        const float* p = tx.px.data() + 4 * (size_t(y) * tx.w + x);
+        return F4{{p[0], p[1], p[2], p[3]}};
+    };
+    float px = uv.x * float(tx.w - 1);
+    float py = uv.y * float(tx.h - 1);
+    uint32_t x0 = gpuU32(std::floor(px)), y0 = gpuU32(std::floor(py));
+    uint32_t x1 = std::min(x0 + 1, tx.w - 1), y1 = std::min(y0 + 1, tx.h - 1);
+    float fx = px - float(x0), fy = py - float(y0);
+    F4 c0 = lerpF4(fetch(x0, y0), fetch(x1, y0), fx);
+    F4 c1 = lerpF4(fetch(x0, y1), fetch(x1, y1), fx);
+    return lerpF4(c0, c1, fy);
+}
+
+// RGB -> 95-bin spectrum (crh_rgb_to_spectrum; Smits 1999): the flat white
+// plus one secondary and one primary basis spectrum weighted by the sorted
+// channel differences.  Returns the bins b[0..2] and the 95-bin average.
+Spec upliftBins(float r, float g, float b, const uint32_t* bins, float& avg) {
+    r = std::max(r, 0.0f);
+    g = std::max(g, 0.0f);
+    b = std::max(b, 0.0f);
+    enum { C, M, Y, R, G, B };
+    float white, w1, w2;
+    int k1, k2;
+    if (r <= g && r <= b) {
+        white = r; k1 = C;
+        if (g <= b) { w1 = g - r; k2 = B; w2 = b - g; } else { w1 = b - r; k2 = G; w2 = g - b; }
+    } else if (g <= r && g <= b) {
+        white = g; k1 = M;
+        if (r <= b) { w1 = r - g; k2 = B; w2 = b - r; } else { w1 = b - g; k2 = R; w2 = r - b; }
+    } else {
+        white = b; k1 = Y;
+        if (r <= g) { w1 = r - b; k2 = G; w2 = g - r; } else { w1 = g - b; k2 = R; w2 = r - g; }
+    }
+    auto mean = [](const float* v) {
+        float a = 0.0f;
+        for (int i = 0; i < 95; ++i) a += v[i];
+        return a / 95.0f;
+    };
+    const float* p1 = crh::kRgbBasis[k1];
+    const float* p2 = crh::kRgbBasis[k2];
+    avg = white + w1 * mean(p1) + w2 * mean(p2);
+    Spec out;
+    for (int j = 0; j < 3; ++j) out.v[j] = white + w1 * p1[bins[j]] + w2 * p2[bins[j]];
+    return out;
+}
+void upliftAll(float r, float g, float b, float* out95) {
+    uint32_t bins[3];
+    float avg;
+    for (uint32_t i = 0; i < 95; ++i) {
+        bins[0] = bins[1] = bins[2] = i;
+        out95[i] = upliftBins(r, g, b, bins, avg).v[0];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -832,6 +988,12 @@ inline const crh_material& matOf(const Scene& s, uint32_t prim) {
     return s.mats[s.meshMat[s.primMesh[prim]]];
 }
 inline uint32_t matIdOf(const Scene& s, uint32_t prim) { return s.meshMat[s.primMesh[prim]]; }
+// The material of a hit: getMeshDesc(primitiveIndex) whatever the hit type
+// (Core/Integrator.cu:172-178), unless the scene gives spheres materials.
+inline uint32_t matIdOfHit(const Scene& s, const RayHit& rh) {
+    if (rh.sphere && !s.sphereMat.empty()) return s.sphereMat[rh.prim];
+    return matIdOf(s, rh.prim);
+}
 
 // Spectral transport: a path carries the bins h, h + 32, h + 64 (mod 95) of
 // its hero bin h in the three Spec channels.
@@ -840,11 +1002,13 @@ struct Wl {
     uint32_t b[3] = {0, 0, 0};
 };
 inline Spec lanesOf(const float* p, const Wl& w) { return Spec(p[w.b[0]], p[w.b[1]], p[w.b[2]]); }
-inline Spec emissionOf(const Scene& s, uint32_t prim, const Wl& w) {
-    if (w.on)
-        return lanesOf(s.spectra.data() + (size_t(matIdOf(s, prim)) * 3 + 2) * 95, w);
-    const crh_material& m = matOf(s, prim);
+inline Spec emissionOfMat(const Scene& s, uint32_t mid, const Wl& w) {
+    if (w.on) return lanesOf(s.spectra.data() + (size_t(mid) * 3 + 2) * 95, w);
+    const crh_material& m = s.mats[mid];
     return Spec(m.emission[0], m.emission[1], m.emission[2]);
+}
+inline Spec emissionOf(const Scene& s, uint32_t prim, const Wl& w) {
+    return emissionOfMat(s, matIdOf(s, prim), w);
 }
 
 // SceneView::intersect as the integrator uses it.  A reported hit without a
@@ -852,13 +1016,13 @@ inline Spec emissionOf(const Scene& s, uint32_t prim, const Wl& w) {
 // would read an uninitialised primitive index (Scene/Scene.cu:31-58).
 bool intersect(const Scene& s, RayHit& rh, Tally& tl) {
     tl.rays++;
-    return bvhIntersect(s, rh, nullptr, false) && rh.prim != CRH_INVALID_PRIM;
+    return sceneIntersect(s, rh, nullptr, false) && rh.prim != CRH_INVALID_PRIM;
 }
 bool occluded(const Scene& s, const Ray& r, Tally& tl) {
     tl.rays++;
     RayHit rh;
     rh.ray = r;
-    return bvhIntersect(s, rh, nullptr, true);
+    return sceneIntersect(s, rh, nullptr, true);
 }
 
 LightSample sampleLight(const Scene& s, const Isect& it, const BSDF& bsdf,
@@ -914,11 +1078,15 @@ Spec evalMIS(const Scene& s, const Isect& it, const BSDF& bsdf, Sampler& smp,
             RayHit rh;
             rh.ray.o = offsetRayOrigin(it.posW, it.orientedFaceNormal());
             rh.ray.d = dir;
-            if (intersect(s, rh, tl) && rh.prim == ls.emissiveID) {
+            // :108 compares primitive indices only: with the reference's
+            // sphere materials a sphere of the light's index matches too.
+            if (intersect(s, rh, tl) && rh.prim == ls.emissiveID &&
+                (!rh.sphere || s.sphereMat.empty())) {
                 Isect lit = createIntersection(s, rh);
-                const crh_material& m = matOf(s, rh.prim);
+                const uint32_t mid = matIdOfHit(s, rh);
+                const crh_material& m = s.mats[mid];
                 if (m.flags & CRH_MATF_EMISSIVE) {
-                    Spec em = emissionOf(s, rh.prim, wl);
+                    Spec em = emissionOfMat(s, mid, wl);
                     V3 toLight = lit.posW - it.posW;
                     float d2 = dot(toLight, toLight);
                     V3 lv = toLight / std::sqrt(d2);
@@ -961,15 +1129,26 @@ Spec tracePath(const Scene& s, const crh_camera& cam, uint32_t W, uint32_t H,
         bool terminate = false;
         if (intersect(s, rh, tl)) {
             Isect it = createIntersection(s, rh);
-            const crh_material& m = matOf(s, rh.prim);
+            const uint32_t mid = matIdOfHit(s, rh);
+            const crh_material& m = s.mats[mid];
             if (m.flags & CRH_MATF_EMISSIVE) {
-                radiance = radiance + emissionOf(s, rh.prim, wl) * beta;
+                radiance = radiance + emissionOfMat(s, mid, wl) * beta;
                 break;
             }
-            BSDF bsdf;
-            if (wl.on) {
-                uint32_t mid = matIdOf(s, rh.prim);
-                const float* sp = s.spectra.data() + size_t(mid) * 3 * 95;
+            BSDF bsdf;  // getBSDF (Material/BSDF.cu:6-32)
+            const float* sp = wl.on ? s.spectra.data() + size_t(mid) * 3 * 95 : nullptr;
+            if (m.flags & CRH_MATF_DIFFUSE_TEXTURE) {
+                F4 t = textureSample(s.textures[m.diffuse_tex], it.uv);
+                if (wl.on) {
+                    float avgKd;
+                    Spec kd = upliftBins(t.v[0], t.v[1], t.v[2], wl.b, avgKd);
+                    bsdf.initWith(m, it.frame, kd, lanesOf(sp + 95, wl), avgKd, s.avgKs[mid]);
+                } else {
+                    Spec kd(t.v[0], t.v[1], t.v[2]);
+                    Spec ks(m.specular[0], m.specular[1], m.specular[2]);
+                    bsdf.initWith(m, it.frame, kd, ks, kd.averageValue(), ks.averageValue());
+                }
+            } else if (wl.on) {
                 bsdf.initWith(m, it.frame, lanesOf(sp, wl), lanesOf(sp + 95, wl), s.avgKd[mid],
                               s.avgKs[mid]);
             } else {
@@ -1059,11 +1238,39 @@ Scene* makeScene(const crh_scene_desc& d) {
     s->mats.assign(d.materials, d.materials + d.nmaterial);
     s->emissive.assign(d.emissive_index, d.emissive_index + d.nemissive);
     s->env = Spec(d.env_radiance[0], d.env_radiance[1], d.env_radiance[2]);
+    for (uint32_t i = 0; i < d.nsphere; ++i) {
+        s->sphC.push_back(V3(d.sphere_center[3 * i], d.sphere_center[3 * i + 1],
+                             d.sphere_center[3 * i + 2]));
+        s->sphR.push_back(d.sphere_radius[i]);
+    }
+    if (d.nsphere && d.sphere_material)
+        s->sphereMat.assign(d.sphere_material, d.sphere_material + d.nsphere);
+    for (uint32_t t = 0; t < d.ntexture; ++t) {  // Texture(const Image&), Core/Texture.cu:46-62
+        const crh_texture& tx = d.textures[t];
+        Scene::Tex o{tx.width, tx.height, std::vector<float>(4 * size_t(tx.width) * tx.height, 0.f)};
+        for (size_t i = 0; i < size_t(tx.width) * tx.height; ++i)
+            for (uint32_t c = 0; c < tx.channels; ++c) o.px[4 * i + c] = tx.data[i * tx.channels + c];
+        s->textures.push_back(std::move(o));
+    }
+    // Spectral transport: the given spectra, or every RGB value uplifted.
+    s->spectral = true;
     if (d.spectra) {
-        s->spectral = true;
         s->spectra.assign(d.spectra, d.spectra + size_t(d.nmaterial) * 3 * 95);
+    } else {
+        s->spectra.assign(size_t(d.nmaterial) * 3 * 95, 0.0f);
+        for (uint32_t m = 0; m < d.nmaterial; ++m) {
+            const crh_material& mt = d.materials[m];
+            float* o = s->spectra.data() + size_t(m) * 285;
+            upliftAll(mt.diffuse[0], mt.diffuse[1], mt.diffuse[2], o);
+            upliftAll(mt.specular[0], mt.specular[1], mt.specular[2], o + 95);
+            upliftAll(mt.emission[0], mt.emission[1], mt.emission[2], o + 190);
+        }
+    }
+    {
         s->envSpec.assign(95, 0.0f);
-        if (d.env_spectrum) s->envSpec.assign(d.env_spectrum, d.env_spectrum + 95);
+        if (d.spectra && d.env_spectrum) s->envSpec.assign(d.env_spectrum, d.env_spectrum + 95);
+        if (!d.spectra) upliftAll(d.env_radiance[0], d.env_radiance[1], d.env_radiance[2],
+                                  s->envSpec.data());
         for (uint32_t m = 0; m < d.nmaterial; ++m) {
             float a = 0.0f, b = 0.0f;
             for (int i = 0; i < 95; ++i) a += s->spectra[size_t(m) * 285 + i];
@@ -1202,6 +1409,15 @@ void orc_scene_bvh_export(void* s, crh_bvh_node_ref* nodes, uint32_t* prims) {
 }
 
 // Closest hit per ray; node_visits / tri_tests (optional) per ray.
+// crh_hit of a RayHit: sphere hits carry CRH_HIT_SPHERE and u = v = 0.
+static void writeHit(const RayHit& rh, crh_hit& h) {
+    const bool none = rh.prim == CRH_INVALID_PRIM;
+    h.prim = none ? rh.prim : (rh.prim | (rh.sphere ? CRH_HIT_SPHERE : 0u));
+    h.t = rh.hitT;
+    h.u = none || rh.sphere ? 0.f : rh.bary.x;
+    h.v = none || rh.sphere ? 0.f : rh.bary.y;
+}
+
 void orc_trace_closest(void* sp, const crh_ray* rays, uint64_t n,
                        crh_hit* hits, uint32_t* node_visits,
                        uint32_t* tri_tests, int nthreads) {
@@ -1212,11 +1428,8 @@ void orc_trace_closest(void* sp, const crh_ray* rays, uint64_t n,
         RayHit rh;
         rh.ray = toRay(rays[i]);
         Counters c;
-        bvhIntersect(s, rh, &c, false);
-        hits[i].prim = rh.prim;
-        hits[i].t = rh.hitT;
-        hits[i].u = rh.prim == CRH_INVALID_PRIM ? 0.f : rh.bary.x;
-        hits[i].v = rh.prim == CRH_INVALID_PRIM ? 0.f : rh.bary.y;
+        sceneIntersect(s, rh, &c, false);
+        writeHit(rh, hits[i]);
         if (node_visits) node_visits[i] = uint32_t(c.nodes);
         if (tri_tests) tri_tests[i] = uint32_t(c.tris);
     }
@@ -1230,7 +1443,7 @@ void orc_trace_any(void* sp, const crh_ray* rays, uint64_t n, uint8_t* occ,
     for (int64_t i = 0; i < int64_t(n); ++i) {
         RayHit rh;
         rh.ray = toRay(rays[i]);
-        occ[i] = bvhIntersect(s, rh, nullptr, true) ? 1 : 0;
+        occ[i] = sceneIntersect(s, rh, nullptr, true) ? 1 : 0;
     }
 }
 
@@ -1243,11 +1456,9 @@ void orc_trace_brute(void* sp, const crh_ray* rays, uint64_t n, crh_hit* hits) {
     for (int64_t i = 0; i < int64_t(n); ++i) {
         RayHit rh;
         rh.ray = toRay(rays[i]);
+        for (uint32_t k = 0; k < uint32_t(s.sphR.size()); ++k) intersectSphereShape(s, k, rh);
         for (uint32_t t = 0; t < ntri; ++t) intersectShape(s, t, rh);
-        hits[i].prim = rh.prim;
-        hits[i].t = rh.hitT;
-        hits[i].u = rh.prim == CRH_INVALID_PRIM ? 0.f : rh.bary.x;
-        hits[i].v = rh.prim == CRH_INVALID_PRIM ? 0.f : rh.bary.y;
+        writeHit(rh, hits[i]);
     }
 }
 
@@ -1336,6 +1547,31 @@ void orc_cie_cmf(float* xyz) {
         cieXYZ(360.0 + 5.0 * i, c);
         for (int j = 0; j < 3; ++j) xyz[3 * i + j] = float(c[j]);
     }
+}
+
+// One sphere test (SphereSOA::intersect + intersectShape's range test) on a
+// one-sphere scene; out = (t, u, v).  Returns 1 on a hit.
+int orc_sphere_hit(const float* center, float radius, const crh_ray* r, float* out) {
+    Scene s;
+    s.sphC.push_back(V3(center[0], center[1], center[2]));
+    s.sphR.push_back(radius);
+    RayHit rh;
+    rh.ray = toRay(*r);
+    if (!intersectSphereShape(s, 0, rh) || rh.prim == CRH_INVALID_PRIM) return 0;
+    out[0] = rh.hitT;
+    out[1] = rh.bary.x;
+    out[2] = rh.bary.y;
+    return 1;
+}
+
+// TextureView::sample of one image at (u, v); out = 4 floats.
+void orc_texture_sample(const crh_texture* tx, float u, float v, float* out) {
+    Scene::Tex o{tx->width, tx->height,
+                 std::vector<float>(4 * size_t(tx->width) * tx->height, 0.f)};
+    for (size_t i = 0; i < size_t(tx->width) * tx->height; ++i)
+        for (uint32_t c = 0; c < tx->channels; ++c) o.px[4 * i + c] = tx->data[i * tx->channels + c];
+    F4 r = textureSample(o, V2(u, v));
+    for (int i = 0; i < 4; ++i) out[i] = r.v[i];
 }
 
 }  // extern "C"
