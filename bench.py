@@ -139,7 +139,8 @@ def batch_leg(cr, scenes, n: int) -> dict:
     spp = [c["spp"] for _, c in items]
     cfg = cr.Configs(maxDepth=12, rrThreshold=0.05, useMIS=True, rrStartDepth=3)
     tuned = autotune.tune_batch(batch, cams, [1] * n, cfg, tile_ws=(8, 16, 32),
-                                grids=(1024, 2048), chunks=(1, 8), stacks=(1, 3), reps=1)
+                                grids=(1024, 2048), chunks=(1, 8), stacks=(1, 3), waves=(1, 4),
+                                reps=1)
     for s in sensors:
         s.clear()
     batch.render(cams, spp, cfg, first_sample=0, readback=False)     # captures the graph
@@ -158,7 +159,7 @@ def batch_leg(cr, scenes, n: int) -> dict:
             "grays_per_s": round(st["rays"] / (st["kernel_ms"] / 1e3) / 1e9, 4),
             "solo_sequential_ms": round(solo_ms, 3),
             "tuning": {k: tuned[k] for k in ("batch_tile_w", "batch_grid", "batch_chunks",
-                                              "batch_stack")}}
+                                              "batch_stack", "batch_waves")}}
 
 
 def guided_leg(cr, scenes, seconds: float, repeats: int) -> dict:
@@ -303,8 +304,9 @@ def main():
     from craystal_amd import autotune
     tuned = autotune.tune_render(scene, cfg, spp=4, stacks=(1, 2, 3), wf_grids=(16384,),
                                  wf_blocks=(128, 256), wf_persistent=((2, 0),),
-                                 wf_pools=(1 << 21, 1 << 22, 1 << 23), render_waves=(1, 4),
-                                 render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)))
+                                 wf_pools=(1 << 22, 1 << 23), render_waves=(1, 4),
+                                 render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
+                                 wf_finishes=(65, 1, 2, 3, 4))
     sensor.clear()
 
     progress(f"scene built, autotuned: {tuned['render_stack']=} {tuned['wf_pool']=}")
@@ -393,7 +395,8 @@ def main():
     out["tuning"] = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen",
                                            "render_grid", "wf_stack", "wf_grid",
                                            "wf_trace_block", "wf_persistent", "wf_refill",
-                                           "wf_pool")}
+                                           "wf_pool", "wf_finish")}
+    out["tuning"]["wavefront_ms_per_spp"] = tuned["wavefront_ms"]
     out["tuning"]["megakernel_ms_per_spp"] = tuned["megakernel_ms"]
     out["wavefront"] = {"msamples_per_s": round(samples / wel / 1e6, 3),
                         "ms_per_step": round(wel / a.steps * 1e3, 3),

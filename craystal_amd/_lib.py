@@ -105,10 +105,10 @@ class crh_integrator_cfg(C.Structure):
 class crh_tuning(C.Structure):
     _fields_ = [(k, u32) for k in ("stack", "trace_block", "persistent", "refill", "pt_grid",
                                    "render_stack", "render_pool", "render_waves", "render_regen",
-                                   "render_grid", "wf_stack", "wf_trace_grid", "wf_trace_block",
+                                   "render_grid", "wf_finish", "wf_stack", "wf_trace_grid", "wf_trace_block",
                                    "wf_shade_grid", "wf_persistent", "wf_refill", "wf_pool",
                                    "batch_stack", "batch_tile_w", "batch_grid", "batch_chunks",
-                                   "batch_graph")]
+                                   "batch_graph", "batch_waves")]
 
 
 class crh_batch_info(C.Structure):

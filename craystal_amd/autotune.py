@@ -69,7 +69,7 @@ def tune_trace(scene: Scene, rays: np.ndarray, reps: int = 3, device: int = 0,
 def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
                 stacks=(1, 2), wf_grids=(1024, 2048, 4096), wf_blocks=(128, 256),
                 wf_persistent=((2, 0), (1, 16)), wf_pools=(None,), render_waves=(1,),
-                render_regen=((2, 0), (1, 1024), (1, 2048)),
+                render_regen=((2, 0), (1, 1024), (1, 2048)), wf_finishes=(65,),
                 first_sample: int = 1 << 20) -> dict:
     """Tune the megakernel stack placement, register budget and path
     regeneration ((mode, persistent grid) pairs, mode 2 = one path per
@@ -107,41 +107,43 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
     pool0 = ctx.get_tuning()["wf_pool"]
     pools = [pool0 if p is None else p for p in wf_pools]
 
-    def wf_kw(st, g, b, pm, rf, pool):
+    def wf_kw(st, g, b, pm, rf, pool, fin):
         return dict(wf_stack=st, wf_trace_grid=g, wf_shade_grid=g, wf_trace_block=b,
-                    wf_persistent=pm, wf_pool=pool, **({"wf_refill": rf} if pm == 1 else {}))
+                    wf_persistent=pm, wf_pool=pool, wf_finish=fin,
+                    **({"wf_refill": rf} if pm == 1 else {}))
 
-    for st, g, b, (pm, rf), pool in itertools.product(stacks, wf_grids, wf_blocks,
-                                                      wf_persistent, pools):
-        ctx.set_tuning(**wf_kw(st, g, b, pm, rf, pool))
+    for st, g, b, (pm, rf), pool, fin in itertools.product(stacks, wf_grids, wf_blocks,
+                                                           wf_persistent, pools, wf_finishes):
+        ctx.set_tuning(**wf_kw(st, g, b, pm, rf, pool, fin))
         ms, img = run(wf_cfg)
         assert img.tobytes() == ref.tobytes(), "wavefront launch shape changed the image"
-        res_wf[(st, g, b, pm, rf, pool)] = ms
+        res_wf[(st, g, b, pm, rf, pool, fin)] = ms
     best_wf = min(res_wf, key=res_wf.get)
-    st, g, b, pm, rf, pool = best_wf
-    ctx.set_tuning(**wf_kw(st, g, b, pm, rf, pool))
+    st, g, b, pm, rf, pool, fin = best_wf
+    ctx.set_tuning(**wf_kw(st, g, b, pm, rf, pool, fin))
     mk_ms, wf_ms = res_mk[best_mk], res_wf[best_wf]
     return {"mode": L.MODE_WAVEFRONT if wf_ms < mk_ms else L.MODE_MEGAKERNEL,
             "render_stack": best_stack, "render_waves": best_waves, "render_regen": best_regen,
             "render_grid": best_grid, "wf_stack": st, "wf_grid": g, "wf_trace_block": b,
-            "wf_persistent": pm, "wf_refill": rf, "wf_pool": pool,
+            "wf_persistent": pm, "wf_refill": rf, "wf_pool": pool, "wf_finish": fin,
             "megakernel_ms": {f"stack={k[0]},waves={k[1]},regen={k[2]},grid={k[3]}": round(v, 3)
                               for k, v in res_mk.items()},
             "wavefront_ms": {str(k): round(v, 3) for k, v in res_wf.items()}}
 
 
 def tune_batch(batch, cameras, spp, cfg: Configs, device: int = 0, tile_ws=(8, 16, 32, 64),
-               grids=(512, 1024, 2048), chunks=(1, 8), stacks=(1, 2), reps: int = 2,
-               first_sample: int = 1 << 20) -> dict:
-    """Tune the batched renderer's tile shape, persistent grid, work ranges
-    and stack placement.  Every candidate renders the whole batch; all
-    films must come out bit-identical before the fastest choice is applied."""
+               grids=(512, 1024, 2048), chunks=(1, 8), stacks=(1, 2), waves=(4,),
+               reps: int = 2, first_sample: int = 1 << 20) -> dict:
+    """Tune the batched renderer's tile shape, persistent grid, work ranges,
+    stack placement and register budget.  Every candidate renders the whole
+    batch; all films must come out bit-identical before the fastest choice
+    is applied."""
     ctx = get_context(device)
     sensors = [c.getSensor() for c in cameras]
     ref, res = None, {}
-    for tw, g, ch, st in itertools.product(tile_ws, grids, chunks, stacks):
+    for tw, g, ch, st, wv in itertools.product(tile_ws, grids, chunks, stacks, waves):
         ctx.set_tuning(batch_tile_w=tw, batch_grid=g, batch_chunks=ch, batch_stack=st,
-                       batch_graph=1)
+                       batch_graph=1, batch_waves=wv)
         ms = []
         for r in range(reps + 1):
             for s in sensors:
@@ -152,13 +154,15 @@ def tune_batch(batch, cameras, spp, cfg: Configs, device: int = 0, tile_ws=(8, 1
                 if ref is None:
                     ref = imgs
                 elif imgs != ref:
-                    raise AssertionError(f"batch tuning {(tw, g, ch, st)} changed an image")
+                    raise AssertionError(f"batch tuning {(tw, g, ch, st, wv)} changed an image")
             else:
                 ms.append(out["kernel_ms"])
-        res[(tw, g, ch, st)] = _median(ms)
+        res[(tw, g, ch, st, wv)] = _median(ms)
     best = min(res, key=res.get)
-    tw, g, ch, st = best
-    ctx.set_tuning(batch_tile_w=tw, batch_grid=g, batch_chunks=ch, batch_stack=st, batch_graph=1)
+    tw, g, ch, st, wv = best
+    ctx.set_tuning(batch_tile_w=tw, batch_grid=g, batch_chunks=ch, batch_stack=st, batch_graph=1,
+                   batch_waves=wv)
     return {"batch_tile_w": tw, "batch_grid": g, "batch_chunks": ch, "batch_stack": st,
-            "timings_ms": {f"tile_w={k[0]},grid={k[1]},chunks={k[2]},stack={k[3]}": round(v, 4)
-                           for k, v in res.items()}}
+            "batch_waves": wv,
+            "timings_ms": {f"tile_w={k[0]},grid={k[1]},chunks={k[2]},stack={k[3]},waves={k[4]}":
+                           round(v, 4) for k, v in res.items()}}

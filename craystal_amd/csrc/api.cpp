@@ -191,6 +191,9 @@ crh::WfLaunch wfLaunch(const crh::Tuning& t) {
     w.privStack = t.wfPriv;
     w.persistent = t.wfPersistent;
     w.refill = t.wfRefill;
+    w.finishDepth = t.wfFinish;
+    w.finishWaves = t.renderWaves;
+    w.finishGrid = t.renderGrid;
     return w;
 }
 }  // namespace
@@ -413,8 +416,11 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(t->refill <= 64 && t->wf_refill <= 64, "refill must be in [1, 64]");
     ARG_CHECK(t->render_waves == 0 || t->render_waves == 1 || t->render_waves == 4,
               "render_waves must be 0 (keep), 1 or 4");
+    ARG_CHECK(t->batch_waves == 0 || t->batch_waves == 1 || t->batch_waves == 4,
+              "batch_waves must be 0 (keep), 1 or 4");
     ARG_CHECK(t->render_regen <= 2, "render_regen must be 0 (keep), 1 (on) or 2 (off)");
     ARG_CHECK(t->render_grid <= (1u << 20), "render_grid too large");
+    ARG_CHECK(t->wf_finish <= 65, "wf_finish must be 0 (keep), 1..64 (bounces) or 65 (off)");
     ARG_CHECK(t->wf_pool <= (1u << 28) && t->render_pool <= (1u << 28),
               "wf_pool and render_pool must be at most 2^28 paths");
     ARG_CHECK(okStack(t->batch_stack),
@@ -443,7 +449,9 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     if (t->render_waves) o.renderWaves = t->render_waves;
     if (t->render_regen) o.renderRegen = t->render_regen == 1 ? 1u : 0u;
     if (t->render_grid) o.renderGrid = t->render_grid;
+    if (t->wf_finish) o.wfFinish = t->wf_finish == 65 ? 0u : t->wf_finish;
     if (t->batch_stack) o.batchPriv = t->batch_stack - 1u;
+    if (t->batch_waves) o.batchWaves = t->batch_waves;
     if (t->batch_tile_w) o.batchTileW = t->batch_tile_w;
     if (t->batch_grid) o.batchGrid = t->batch_grid;
     if (t->batch_chunks) o.batchChunks = t->batch_chunks;
@@ -472,7 +480,9 @@ int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
     t->render_waves = o.renderWaves;
     t->render_regen = o.renderRegen ? 1u : 2u;
     t->render_grid = o.renderGrid;
+    t->wf_finish = o.wfFinish ? o.wfFinish : 65u;
     t->batch_stack = o.batchPriv + 1u;
+    t->batch_waves = o.batchWaves;
     t->batch_tile_w = o.batchTileW;
     t->batch_grid = o.batchGrid;
     t->batch_chunks = o.batchChunks;
@@ -1357,7 +1367,12 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
     double t0 = nowMs();
     HIP_TRY(hipSetDevice(ctx->device));
     const crh::Tuning tun = effective(ctx->tuning, s->view);
-    const crh::WfLaunch wl = wfLaunch(tun);
+    crh::WfLaunch wl = wfLaunch(tun);
+    // The per-path finish never takes guided vertices away: with guiding at
+    // every depth it is off, else it starts no earlier than max_depth.
+    wl.finishDepth = (gc->max_depth == 0u || wl.finishDepth == 0u)
+                         ? 0u
+                         : std::max(wl.finishDepth, gc->max_depth);
     const uint32_t slots = wfSlots(tun, film, spp);
     HIP_TRY(ensureGuided(film, slots));
     HIP_TRY(hipMemsetAsync(film->wf.work, 0, 64, ctx->stream));
@@ -1658,7 +1673,7 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
         std::vector<uint64_t> sig = {n, first_sample, maxSpp, iv.maxDepth,
                                      uint64_t(__builtin_bit_cast(uint32_t, iv.rrThreshold)),
                                      iv.rrStartDepth, iv.useMIS, tun.batchPriv, tw,
-                                     tun.batchGrid, tun.batchChunks};
+                                     tun.batchGrid, tun.batchChunks, tun.batchWaves};
         for (uint32_t j = 0; j < n; ++j) sig.push_back(spp[order[j]]);
         sig.insert(sig.end(), pre.begin(), pre.end());
         if (!b->exec || sig != b->sig) {

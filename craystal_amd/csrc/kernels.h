@@ -35,8 +35,10 @@ struct Tuning {
     uint32_t wfPool = 1u << 23;  // paths in flight (several samples per pixel for small images)
     uint32_t renderPool = 1u << 21;  // megakernel: paths per launch, the same way
     uint32_t renderWaves = 4;        // megakernels: minimum waves per SIMD (1 or 4)
+    uint32_t batchWaves = 4;         // batch megakernel: the same for kBatchRender
     uint32_t renderRegen = 1;        // megakernel: lanes take a new path when theirs ends
     uint32_t renderGrid = 2048;      // megakernel with regeneration: persistent blocks
+    uint32_t wfFinish = 2;           // wavefront: finish per path after this many bounces (0 off)
     // batched rendering
     uint32_t batchPriv = 2;
     uint32_t batchTileW = 16;   // tile width in pixels; height = 256 / width

@@ -5,6 +5,7 @@
 
 #include "kernels.h"
 #include "integrator.h"
+#include "path.h"
 #include "wave.h"
 
 namespace crh {
@@ -83,56 +84,8 @@ __global__ __launch_bounds__(kTraceBlock) void kTracePersistent(
 }
 
 // ---------------------------------------------------------------------------
-// Megakernel integrator (Core/Integrator.cu:13-227)
+// Megakernel integrator (Core/Integrator.cu:142-227); per-path steps in path.h
 // ---------------------------------------------------------------------------
-template <bool COUNT, class Stack>
-struct PathCtx {
-    const SceneView& s;
-    Stack& stack;
-    uint32_t rays;
-    WorkCount wc;
-};
-
-template <bool COUNT, class Stack>
-CRH_D bool closest(PathCtx<COUNT, Stack>& c, const Ray& r, Hit& h) {
-    c.rays++;
-    return closestHit<COUNT>(c.s, r, h, c.stack, c.wc);
-}
-template <bool COUNT, class Stack>
-CRH_D bool occluded(PathCtx<COUNT, Stack>& c, const Ray& r) {
-    Hit h;
-    c.rays++;
-    return traverse<true, COUNT>(c.s, r, h, c.stack, c.wc);
-}
-
-// sampleLight (Core/Integrator.cu:21-72)
-template <bool COUNT, class Stack>
-CRH_D LightSample sampleLight(PathCtx<COUNT, Stack>& c, const Isect& it,
-                              const BSDF& bsdf, Sampler& smp, const Lanes& w) {
-    LightGeom g = lightGeom(c.s, it, smp);
-    if (!g.valid) return noLight();
-    if (occluded(c, g.shadow)) return noLight();
-    return lightFinish(c.s, g, bsdf, it, w);
-}
-
-// evalMIS (Core/Integrator.cu:74-140)
-template <bool COUNT, class Stack>
-CRH_D Spec evalMIS(PathCtx<COUNT, Stack>& c, const Isect& it, const BSDF& bsdf,
-                   Sampler& smp, const Lanes& w) {
-    LightSample ls = sampleLight(c, it, bsdf, smp, w);
-    if (ls.pdf <= 0.0f) return sp(0.f);
-    Spec value = misLightPart(ls, bsdf, it);
-    Ray r;
-    Spec bw;
-    float bp;
-    if (misBsdfRay(bsdf, smp, it, r, bw, bp)) {
-        Hit h;
-        bool isHit = closest(c, r, h);
-        misBsdfPart(c.s, it.pos, r, isHit, h, ls.id, bw, bp, value, w);
-    }
-    return value / ls.selPdf;
-}
-
 // Pixel `inTile` (0..255) of a 16x16 tile; each 64-pixel quarter is an 8x8
 // block (one wave of a 256-thread block).
 CRH_D void tileCoordOf(uint32_t tile, uint32_t inTile, uint32_t tilesX, uint32_t& x,
@@ -163,40 +116,8 @@ CRH_D void tracePixel(const SceneView& s, Stack& stk, const CameraView& cam,
     const Lanes w = SPEC ? heroLanes(s.spec, smp.next1D()) : rgbLanes();
     Ray ray = cameraRay(cam, pix);
     Spec rad = sp(0.f), beta = sp(1.f);
-    for (uint32_t depth = 0; depth < cfg.maxDepth; ++depth) {
-        bool terminate = false;
-        Hit h;
-        if (closest(c, ray, h)) {
-            Isect it = makeIsect(s, ray, h);
-            const uint32_t mid = matIdOf(s, h.prim);
-            const GMaterial& m = s.mats[mid];
-            if (m.flags & 1u) {
-                rad = rad + emissionOf(m, mid, w) * beta;
-                break;
-            }
-            BSDF bsdf = makeBSDF(s, m, mid, it, h, w);
-            if (MIS) {
-                rad = rad + evalMIS(c, it, bsdf, smp, w) * beta;
-            } else {
-                LightSample ls = sampleLight(c, it, bsdf, smp, w);
-                if (ls.pdf > 0.0f)
-                    rad = rad + ls.weight * beta / ls.pdf / ls.selPdf;
-            }
-            F3 wi;
-            float pdf = 0.0f;
-            Spec f = bsdfSampleEval(bsdf, smp, it.view, wi, pdf);
-            if (pdf <= 0.0f) break;
-            beta = beta * (f / pdf);
-            ray.o = offsetRayOrigin(it.pos, oriented(it));
-            ray.d = wi;
-        } else {
-            rad = rad + (SPEC ? laneSpec(s.spec + kSpecEnv, w) : sp3(s.env)) * beta;
-            terminate = true;
-        }
-        if (smp.next1D() < cfg.rrThreshold) terminate = true;
-        if (terminate) break;
-        if (!roulette(cfg, depth, smp, beta)) break;
-    }
+    for (uint32_t depth = 0; depth < cfg.maxDepth; ++depth)
+        if (!pathBounce<MIS, SPEC>(c, cfg, depth, ray, rad, beta, smp, w)) break;
     if (SPEC) rad = laneXYZ(rad, w);
     size_t id = size_t(y) * cam.width + x;
     L[id] = make_float4(rad.r, rad.g, rad.b, 0.f);
@@ -312,45 +233,9 @@ kRenderRegen(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint3
             if (exhausted) break;
             continue;
         }
-        if (live) {  // one bounce of pathTraceKernel's loop (Core/Integrator.cu:160-223)
-            bool done = false, terminate = false;
-            Hit h;
-            if (closest(c, ray, h)) {
-                Isect it = makeIsect(s, ray, h);
-                const uint32_t mid = matIdOf(s, h.prim);
-                const GMaterial& m = s.mats[mid];
-                if (m.flags & 1u) {
-                    rad = rad + emissionOf(m, mid, w) * beta;
-                    done = true;
-                } else {
-                    BSDF bsdf = makeBSDF(s, m, mid, it, h, w);
-                    if (MIS) {
-                        rad = rad + evalMIS(c, it, bsdf, smp, w) * beta;
-                    } else {
-                        LightSample ls = sampleLight(c, it, bsdf, smp, w);
-                        if (ls.pdf > 0.0f) rad = rad + ls.weight * beta / ls.pdf / ls.selPdf;
-                    }
-                    F3 wi;
-                    float pdf = 0.0f;
-                    Spec f = bsdfSampleEval(bsdf, smp, it.view, wi, pdf);
-                    if (pdf <= 0.0f) {
-                        done = true;
-                    } else {
-                        beta = beta * (f / pdf);
-                        ray.o = offsetRayOrigin(it.pos, oriented(it));
-                        ray.d = wi;
-                    }
-                }
-            } else {
-                rad = rad + (SPEC ? laneSpec(s.spec + kSpecEnv, w) : sp3(s.env)) * beta;
-                terminate = true;
-            }
-            if (!done) {
-                if (smp.next1D() < cfg.rrThreshold) terminate = true;
-                if (terminate) done = true;
-                else if (!roulette(cfg, depth, smp, beta)) done = true;
-                else if (++depth >= cfg.maxDepth) done = true;
-            }
+        if (live) {
+            bool done = !pathBounce<MIS, SPEC>(c, cfg, depth, ray, rad, beta, smp, w);
+            if (!done && ++depth >= cfg.maxDepth) done = true;
             if (done) finish();
         }
     }
@@ -621,7 +506,7 @@ hipError_t launchBatchPass(const BatchScene* tab, const uint32_t* tilePrefix,
     const uint32_t tw = t.batchTileW, ch = t.batchChunks;
     withStack(t.batchPriv, [&](auto sm) {
         constexpr int SM = decltype(sm)::value;
-        withWaves(t.renderWaves, [&](auto wv) {
+        withWaves(t.batchWaves, [&](auto wv) {
             constexpr int W = decltype(wv)::value;
             if (cfg.useMIS)
                 hipLaunchKernelGGL((kBatchRender<true, SM, W>), grid, block, lds, st, tab,

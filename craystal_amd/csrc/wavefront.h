@@ -72,6 +72,9 @@ struct WfLaunch {
     uint32_t privStack = 0;      // stack mode: 0 LDS, 1 private, 2 short LDS + private
     uint32_t persistent = 0;     // 1: traversal lanes refill with new rays
     uint32_t refill = 16;
+    uint32_t finishDepth = 0;    // > 0: bounces from this depth on run per path in one launch
+    uint32_t finishWaves = 4;    // register budget of that launch (1 or 4 waves per SIMD)
+    uint32_t finishGrid = 2048;  // its persistent blocks of 256
 };
 
 struct GuideView;

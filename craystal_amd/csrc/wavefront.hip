@@ -29,6 +29,7 @@
 #include "guide_device.h"
 #include "guiding_kernels.h"
 #include "integrator.h"
+#include "path.h"
 #include "wavefront.h"
 #include "kernels.h"
 #include "wave.h"
@@ -42,6 +43,7 @@ enum : uint32_t {
     F_LIGHT = 2u,    // light geometry valid, shadow ray traced
     F_MISHIT = 4u,   // MIS ray query returned a hit (intersect() == true)
     F_EXTHIT = 8u,   // extension query returned a hit
+    F_ALIVE = 16u,   // the path continues (it is in the next extension queue)
 };
 
 CRH_D uint32_t laneId() { return __lane_id(); }
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(256) void kWfGenerate(WfState w, CameraView cam,
         w.rng[p] = smp.state;
         stS(w.beta, p, sp(1.f));
         stS(w.rad, p, sp(0.f));
-        w.flags[p] = 0u;
+        w.flags[p] = F_ALIVE;
         if (w.recId) w.recId[p] = kNoRecord;
         w.P[p] = make_float2(pix.x, pix.y);
         w.qExt[0][p] = p;
@@ -433,12 +435,99 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
         w.rng[p] = smp.state;
         stS(w.beta, p, beta);
         stS(w.rad, p, rad);
-        w.flags[p] = (fl & ~F_LIGHT) | (misRay ? F_MIS : 0u);
+        w.flags[p] = (fl & ~(F_LIGHT | F_ALIVE)) | (misRay ? F_MIS : 0u) | (alive ? F_ALIVE : 0u);
         uint32_t slot = wavePush(&w.counters[kCExt0 + nxt], alive);
         if (alive) w.qExt[nxt][slot] = p;
         slot = wavePush(&w.counters[kCMis0 + nxt], misRay);
         if (misRay) w.qMis[nxt][slot] = p;
     }
+}
+
+// Finishes every path still in flight after `d0` wavefront bounces in one
+// persistent launch: a lane takes the next entry of the extension and MIS
+// queues (one atomic per wave), traces and resolves the pending MIS ray of
+// the path's last vertex (as kWfMisResolve), gives the record of its last
+// guided vertex its snapshot (as kWfShadeHit), and, if the path continues,
+// runs pathTraceKernel's loop from depth d0 one bounce per iteration (as
+// the megakernel, without guiding); a lane whose path ends takes the next
+// entry.  Each path performs the operations of the pure wavefront in the
+// same order, so the film is the same; the launch replaces the tail of
+// nearly empty bounces, each of which costs a full traversal launch.
+template <bool MIS, int SM, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void kWfFinish(
+    SceneView s, WfState w, IntegratorView cfg, uint32_t cur, uint32_t d0) {
+    extern __shared__ uint2 lds[];
+    auto stk = makeStack<SM>(lds);
+    PathCtx<false, typename StackOf<SM>::T> c{s, stk, 0, {0, 0}};
+    const uint32_t nExt = w.counters[kCExt0 + cur];
+    const uint32_t total = nExt + w.counters[kCMis0 + cur];
+    bool live = false, exhausted = false;
+    uint32_t p = 0, depth = 0;
+    Ray ray{};
+    Spec rad = sp(0.f), beta = sp(1.f);
+    Sampler smp(0u, 0u, 0u);
+    while (true) {
+        if (!exhausted) {
+            const bool want = !live;
+            const uint32_t i = wavePush(&w.counters[kCPtExt], want);
+            exhausted = __any(want && i >= total);
+            if (want && i < total) {
+                const bool alive = i < nExt;
+                p = alive ? w.qExt[cur][i] : w.qMis[cur][i - nExt];
+                const uint32_t fl = w.flags[p];
+                // the MIS entry of a path that continues is handled with its
+                // extension entry (flags are not written here, so this test
+                // holds whichever entry a lane takes first)
+                if (alive || !(fl & F_ALIVE)) {
+                    rad = ldS(w.rad, p);
+                    if (fl & F_MIS) {
+                        Ray r = loadRay(w.misO, w.misD, p);
+                        Hit h;
+                        bool hit = closest(c, r, h);
+                        float4 vs = w.misVal[p];
+                        Spec value{vs.x, vs.y, vs.z};
+                        float4 bw4 = w.misBw[p];
+                        float4 pe = w.misPos[p];
+                        misBsdfPart(s, mk3(pe.x, pe.y, pe.z), r, hit, h, __float_as_uint(pe.w),
+                                    Spec{bw4.x, bw4.y, bw4.z}, bw4.w, value);
+                        rad = rad + (value / vs.w) * ldS(w.misBeta, p);
+                    }
+                    if (alive) {
+                        if (w.recId) {
+                            const uint32_t rid = w.recId[p];
+                            if (rid != kNoRecord) {
+                                w.rec[size_t(rid) * kRecF4 + 6] =
+                                    make_float4(rad.r, rad.g, rad.b, 1.f);
+                                w.recId[p] = kNoRecord;
+                            }
+                        }
+                        ray = loadRay(w.rayO, w.rayD, p);
+                        beta = ldS(w.beta, p);
+                        smp = loadSampler(w, p);
+                        depth = d0;
+                        live = true;
+                    } else {
+                        stS(w.rad, p, rad);
+                    }
+                }
+            }
+        }
+        if (!__any(live)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (live) {
+            bool done = !pathBounce<MIS, false>(c, cfg, depth, ray, rad, beta, smp, rgbLanes());
+            if (!done && ++depth >= cfg.maxDepth) done = true;
+            if (done) {
+                stS(w.rad, p, rad);
+                live = false;
+            }
+        }
+    }
+    unsigned long long r64 = c.rays;
+    for (int off = 32; off > 0; off >>= 1) r64 += __shfl_down(r64, off, 64);
+    if (laneId() == 0 && r64) atomicAdd(w.work, r64);
 }
 
 // Training samples from this pass's records: incident radiance along the
@@ -536,7 +625,11 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                            (1u << kCTrain));
     hipLaunchKernelGGL(kWfGenerate, dim3(L.shadeGrid), b256, 0, st, w, cam, si);
     uint32_t cur = 0;
-    for (uint32_t d = 0; d < cfg.maxDepth; ++d) {
+    // Bounces run as wavefront launches up to finishDepth; the paths still
+    // in flight then finish in one per-path launch.
+    const bool finish = L.finishDepth > 0u && L.finishDepth < cfg.maxDepth && !countWork;
+    const uint32_t dEnd = finish ? L.finishDepth : cfg.maxDepth;
+    for (uint32_t d = 0; d < dEnd; ++d) {
         const uint32_t nxt = cur ^ 1u;
         extend(cur, countWork);
         hipLaunchKernelGGL(kWfMisResolve, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
@@ -571,11 +664,28 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
         }
         cur = nxt;
     }
-    // MIS rays spawned at the last bounce: trace and resolve them.
-    hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
-                       1u << (kCExt0 + cur));
-    extend(cur, false);
-    hipLaunchKernelGGL(kWfMisResolve, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
+    if (finish) {
+        const size_t flds = stackLdsBytes(L.privStack, s.stackCap, 256);
+        hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters, 1u << kCPtExt);
+        withStack(L.privStack, [&](auto sm) {
+            constexpr int SM = decltype(sm)::value;
+            withWaves(L.finishWaves, [&](auto wv) {
+                constexpr int W = decltype(wv)::value;
+                if (cfg.useMIS)
+                    hipLaunchKernelGGL((kWfFinish<true, SM, W>), dim3(L.finishGrid), b256, flds,
+                                       st, s, w, cfg, cur, dEnd);
+                else
+                    hipLaunchKernelGGL((kWfFinish<false, SM, W>), dim3(L.finishGrid), b256, flds,
+                                       st, s, w, cfg, cur, dEnd);
+            });
+        });
+    } else {
+        // MIS rays spawned at the last bounce: trace and resolve them.
+        hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
+                           1u << (kCExt0 + cur));
+        extend(cur, false);
+        hipLaunchKernelGGL(kWfMisResolve, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
+    }
     if (guided && samples)
         hipLaunchKernelGGL(kWfRecordsToSamples, dim3(L.shadeGrid), b256, 0, st, w, samples,
                            sampleCap, guideTarget);

@@ -261,6 +261,12 @@ typedef struct crh_tuning {
     uint32_t render_regen;   /* megakernel: 1 = persistent blocks whose lanes start a new
                                 path as soon as theirs ends, 2 = one path per thread */
     uint32_t render_grid;    /* megakernel with regeneration: persistent blocks */
+    uint32_t wf_finish;      /* wavefront: 1..64 = after that many bounces the paths still in
+                                flight finish in one persistent launch (a lane per path,
+                                refilled as paths end); 65 = every bounce as wavefront
+                                launches.  Same film either way.  Guided rendering never
+                                finishes before crh_guided_cfg.max_depth (off when every
+                                depth is guided) */
     uint32_t wf_stack;       /* wavefront traversal kernels: stack as above */
     uint32_t wf_trace_grid;  /* wavefront traversal kernels: blocks */
     uint32_t wf_trace_block; /* wavefront traversal kernels: threads (64..256) */
@@ -274,6 +280,7 @@ typedef struct crh_tuning {
     uint32_t batch_grid;     /* batched rendering: persistent blocks per pass */
     uint32_t batch_chunks;   /* batched rendering: work ranges with own counters, 1..8 */
     uint32_t batch_graph;    /* batched rendering: 1 = passes replayed from a hipGraph, 2 = off */
+    uint32_t batch_waves;    /* batch megakernel register budget: 1 or 4 waves per SIMD */
 } crh_tuning;
 
 /* ---- errors / device ------------------------------------------------- */

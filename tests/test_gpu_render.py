@@ -163,14 +163,15 @@ def test_wavefront_film_bit_identical_to_megakernel(gpu, mis):
             base = dict(maxDepth=8, rrThreshold=0.05, useMIS=mis, rrStartDepth=3)
             d0, w0, s0 = _film(scene, camd, size, cr.Configs(**base), 5, first=5)
             npix = size[0] * size[1]
-            for k in (1, 3, 5):
-                ctx.set_tuning(wf_pool=k * npix)
+            # (samples per pixel in flight, bounces before the per-path finish; 65 = none)
+            for k, fin in ((1, 65), (3, 65), (5, 65), (3, 1), (1, 2), (5, 5), (2, 8)):
+                ctx.set_tuning(wf_pool=k * npix, wf_finish=fin)
                 d1, w1, s1 = _film(scene, camd, size,
                                    cr.Configs(**base, mode=L.MODE_WAVEFRONT), 5, first=5)
-                assert np.array_equal(w0.view(np.uint32), w1.view(np.uint32)), k
+                assert np.array_equal(w0.view(np.uint32), w1.view(np.uint32)), (k, fin)
                 assert np.array_equal(d0.view(np.uint32), d1.view(np.uint32)), \
-                    (k, np.argwhere(d0 != d1)[:5])
-                assert s0["rays"] == s1["rays"], k
+                    (k, fin, np.argwhere(d0 != d1)[:5])
+                assert s0["rays"] == s1["rays"], (k, fin)
     finally:
         ctx.set_tuning(**saved)
 
@@ -230,13 +231,25 @@ def _materials_scene():
 
 
 def test_wavefront_large_image_matches_megakernel(gpu):
-    """Full 1080p pass on the 983k-triangle scene (queues of 2M paths)."""
+    """1080p passes on the 983k-triangle scene (queues of 2M paths and more),
+    every bounce as wavefront launches and with the per-path finish after
+    1 and 3 bounces (at this size the threads finishing a path's extension
+    entry and its MIS entry run far apart, which the small images do not
+    exercise)."""
     from craystal_amd import _lib as L
     data, camd = scenes.spheres_scene()
     scene = cr.Scene(data)
     base = dict(maxDepth=16, rrThreshold=0.05, useMIS=True, rrStartDepth=3)
-    d0, w0, s0 = _film(scene, camd, (1920, 1080), cr.Configs(**base), 1, first=2)
-    d1, w1, s1 = _film(scene, camd, (1920, 1080), cr.Configs(**base, mode=L.MODE_WAVEFRONT), 1,
-                       first=2)
-    assert np.array_equal(d0.view(np.uint32), d1.view(np.uint32))
-    assert s0["rays"] == s1["rays"] > 2 * 1920 * 1080
+    ctx = cr.get_context(0)
+    saved = ctx.get_tuning()
+    try:
+        d0, w0, s0 = _film(scene, camd, (1920, 1080), cr.Configs(**base), 2, first=2)
+        for fin, pool in ((65, 1 << 21), (1, 1 << 22), (3, 1 << 21)):
+            ctx.set_tuning(wf_finish=fin, wf_pool=pool)
+            d1, w1, s1 = _film(scene, camd, (1920, 1080),
+                               cr.Configs(**base, mode=L.MODE_WAVEFRONT), 2, first=2)
+            assert np.array_equal(d0.view(np.uint32), d1.view(np.uint32)), \
+                (fin, int((d0 != d1).any(-1).sum()))
+            assert s0["rays"] == s1["rays"] > 4 * 1920 * 1080, fin
+    finally:
+        ctx.set_tuning(**saved)
