@@ -61,6 +61,8 @@ def parse():
                     help="counter-collection mode: one 256 MiB device copy (traffic "
                          "calibration), one 16,777,216-ray closest-hit launch and one render "
                          "pass; prints the launch sizes and exits")
+    ap.add_argument("--profile-tuning", default=None,
+                    help="with --profile-kernels: JSON of launch-shape fields to apply first")
     ap.add_argument("--profile-guided", action="store_true",
                     help="counter-collection mode for the guiding kernels: the occluded room "
                          "at 256x256, 16 spp of the guided wavefront with training, 16 without, "
@@ -94,8 +96,12 @@ def profile_guided(cr, L, scenes):
           flush=True)
 
 
-def profile_kernels(cr, L, scenes):
-    """Minimal launches for rocprofv3 --pmc runs (FETCH_SIZE / WRITE_SIZE)."""
+def profile_kernels(cr, L, scenes, tuning: str | None = None):
+    """Minimal launches for rocprofv3 --pmc runs (FETCH_SIZE / WRITE_SIZE).
+    tuning: JSON of crh_tuning fields to apply first (the launch shapes the
+    timed run uses)."""
+    if tuning:
+        cr.get_context(0).set_tuning(**json.loads(tuning))
     nbytes = 256 << 20
     a, b = cr.DeviceBuffer(nbytes), cr.DeviceBuffer(nbytes)
     a.memset(1)
@@ -209,7 +215,7 @@ def guided_leg(cr, scenes, seconds: float, repeats: int) -> dict:
     return out
 
 
-def measure_traffic() -> dict | None:
+def measure_traffic(tuning: dict | None = None) -> dict | None:
     """HBM bytes per dispatch of the profiled kernels from two rocprofv3 --pmc
     runs of `bench.py --profile-kernels` (FETCH_SIZE, then WRITE_SIZE, KiB per
     dispatch), each scaled by what the counter reads for the 256 MiB copy
@@ -228,6 +234,8 @@ def measure_traffic() -> dict | None:
         cmd = ["timeout", "-s", "KILL", "240", exe, "--pmc", ctr, "--kernel-trace",
                "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
                str(ROOT / "bench.py"), "--profile-kernels"]
+        if tuning:
+            cmd += ["--profile-tuning", json.dumps(tuning)]
         r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
                            capture_output=True, text=True)
         files = list(Path(d).rglob("*counter_collection.csv"))
@@ -268,9 +276,6 @@ def main():
         sys.exit(2)
 
     traffic = None
-    if not a.profile_kernels and not a.profile_guided and not a.no_traffic:
-        traffic = measure_traffic()
-        progress(f"traffic counters: {traffic}")
 
     import craystal_amd as cr
     from craystal_amd import _lib as L
@@ -280,7 +285,7 @@ def main():
     if cr.device_count() < 1:
         raise SystemExit("no GPU visible to libcrayhip.so")
     if a.profile_kernels:
-        profile_kernels(cr, L, scenes)
+        profile_kernels(cr, L, scenes, a.profile_tuning)
         return
     if a.profile_guided:
         profile_guided(cr, L, scenes)
@@ -310,6 +315,15 @@ def main():
     sensor.clear()
 
     progress(f"scene built, autotuned: {tuned['render_stack']=} {tuned['wf_pool']=}")
+    if not a.no_traffic:
+        # HBM traffic of the megakernel at the launch shape just chosen (and
+        # of the closest-hit kernel at its default one), from two rocprofv3
+        # child runs while this process is idle
+        mk_tuning = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen")}
+        if tuned["render_regen"] == 1:
+            mk_tuning["render_grid"] = tuned["render_grid"]
+        traffic = measure_traffic(mk_tuning)
+        progress(f"traffic counters: {traffic}")
     # warmup (untimed)
     if a.warmup > 0:
         integ.dispatch(scene, a.warmup, first_sample=0, readback=False)
@@ -390,7 +404,8 @@ def main():
         out["roofline"]["traffic_note"] = (
             "HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of "
             "bench.py --profile-kernels (one 1920x1080 pass of "
-            f"{traffic['render_kernel']} at the default launch shape), scaled by a 256 MiB copy "
+            f"{traffic['render_kernel']} at the tuned launch shape; the closest-hit batch at "
+            "its default one), scaled by a 256 MiB copy "
             f"(fetch x {traffic['fetch_scale']}, write x {traffic['write_scale']})")
     out["tuning"] = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen",
                                            "render_grid", "wf_stack", "wf_grid",
