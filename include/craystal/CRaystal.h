@@ -338,6 +338,7 @@ class PathTraceIntegrator {
         bool useMIS = true;
         uint32_t rrStartDepth = 6u;
         uint32_t mode = CRH_MODE_MEGAKERNEL;
+        bool spectral = false;  // three of 95 wavelength bins per path; the film holds XYZ
         static Configs reference() { return Configs{}; }
     };
 
@@ -357,6 +358,7 @@ class PathTraceIntegrator {
         c.rr_start_depth = mConfigs.rrStartDepth;
         c.use_mis = mConfigs.useMIS ? 1u : 0u;
         c.mode = mConfigs.mode;
+        c.spectral = mConfigs.spectral ? 1u : 0u;
         crh_stats st{};
         detail::check(crh_render(scene.handle(), &cam->getData(), &c, firstSample,
                                  uint32_t(spp), sensor->handle(), &st));

@@ -69,10 +69,31 @@ int main(int argc, char** argv) {
         threw = e.code() == CRH_ERR_INVALID_ARG;
     }
     EXPECT(threw);
+    // RGB -> spectrum uplift: grays are flat, white's XYZ is the flat spectrum's
+    float gray[3] = {0.25f, 0.25f, 0.25f}, sp[95], xyz[3], xyzFlat[3], flat[95];
+    EXPECT(crh_rgb_to_spectrum(gray, sp) == CRH_OK);
+    for (float v : sp) EXPECT(v == 0.25f);
+    for (float& v : flat) v = 1.f;
+    float one[3] = {1.f, 1.f, 1.f};
+    EXPECT(crh_rgb_to_spectrum(one, sp) == CRH_OK);
+    EXPECT(crh_spectrum_to_xyz(sp, xyz) == CRH_OK && crh_spectrum_to_xyz(flat, xyzFlat) == CRH_OK);
+    for (int k = 0; k < 3; ++k) EXPECT(std::fabs(xyz[k] - xyzFlat[k]) < 1e-6f);
     if (!gpu) {
         std::printf("cpp host api ok\n");
         return 0;
     }
+    // Spheres with their own materials and a 2 x 2 texture on the floor.
+    SceneData sdata = data;
+    MaterialData tex = white;
+    tex.diffuseHandle = 0;
+    tex.flags |= CRH_MATF_DIFFUSE_TEXTURE;
+    sdata.materials.push_back(tex);
+    sdata.meshes[0].texCrd = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+    sdata.meshes[0].materialID = 2;
+    sdata.textureImages.push_back(TextureImage{2, 2, 3, {0.9f, 0.1f, 0.1f, 0.1f, 0.9f, 0.1f,
+                                                         0.1f, 0.1f, 0.9f, 0.9f, 0.9f, 0.9f}});
+    sdata.spheres = {SphereData{{0.f, 0.5f, 0.f}, 0.4f}, SphereData{{1.f, 0.3f, 1.f}, 0.3f}};
+    sdata.sphereMaterials = {0, 2};
 
     auto scene = std::make_shared<Scene>(std::move(data));
     auto sensor = std::make_shared<Sensor>(UInt2{32, 24});
@@ -110,6 +131,25 @@ int main(int argc, char** argv) {
     EXPECT(crh_buffer_download(b2, out, 64) == CRH_OK);
     EXPECT(std::memcmp(in, out, 64) == 0);
     EXPECT(crh_buffer_destroy(b2) == CRH_OK);
+
+    auto sscene = std::make_shared<Scene>(std::move(sdata));
+    std::vector<crh_ray> sr = {crh_ray{{0, 2, 0}, 0.f, {0, -1, 0}, INFINITY},
+                               crh_ray{{0, 0.5f, 0}, 0.f, {1, 0, 0}, INFINITY}};
+    auto sh = sscene->intersect(sr);
+    EXPECT(sh[0].prim == (0u | CRH_HIT_SPHERE) && std::fabs(sh[0].t - 1.1f) < 1e-5f);
+    EXPECT(sh[1].prim == (0u | CRH_HIT_SPHERE) && std::fabs(sh[1].t - 0.4f) < 1e-6f);  // from inside
+    auto ssensor = std::make_shared<Sensor>(UInt2{32, 24});
+    auto scam = std::make_shared<Camera>(cam);
+    scam->setSensor(ssensor);
+    sscene->setCamera(scam);
+    PathTraceIntegrator::Configs scfg = cfg;
+    scfg.spectral = true;
+    crh_stats ss = PathTraceIntegrator(scfg).dispatch(*sscene, 2);
+    auto simg = ssensor->createImage();
+    double smean = 0;
+    for (float v : simg) smean += v;
+    smean /= simg.size();
+    EXPECT(ss.rays > 0 && std::isfinite(smean) && smean > 0 && std::fabs(smean - mean) > 1e-6);
     std::printf("cpp gpu api ok: mean %.5f rays %llu\n", mean, (unsigned long long)rs.rays);
     return 0;
 }
