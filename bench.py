@@ -247,7 +247,8 @@ def measure_traffic(tuning: dict | None = None) -> dict | None:
                 vals.setdefault("render_kernel", {})["name"] = name.split("<")[0]
             key = ("copy" if "copyBuffer" in name else
                    "render" if ("kRenderPass<" in name or "kRenderRegen<" in name) else
-                   "kTraceClosest" if "kTraceClosest<false" in name else None)
+                   "kTraceClosest" if ("kTraceClosest<false" in name or
+                                       "kTracePersistent<false" in name) else None)
             if key:
                 # the calibration copy is the largest copy dispatch
                 v = float(row["Counter_Value"])
@@ -308,22 +309,17 @@ def main():
     # candidate must render the same image bit for bit)
     from craystal_amd import autotune
     tuned = autotune.tune_render(scene, cfg, spp=4, stacks=(1, 2, 3), wf_grids=(16384,),
-                                 wf_blocks=(128, 256), wf_persistent=((2, 0),),
+                                 wf_blocks=(128, 256), wf_persistent=((2, 0), (1, 16)),
                                  wf_pools=(1 << 22, 1 << 23), render_waves=(1, 4),
                                  render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
                                  wf_finishes=(65, 1, 2, 3, 4))
     sensor.clear()
 
     progress(f"scene built, autotuned: {tuned['render_stack']=} {tuned['wf_pool']=}")
-    if not a.no_traffic:
-        # HBM traffic of the megakernel at the launch shape just chosen (and
-        # of the closest-hit kernel at its default one), from two rocprofv3
-        # child runs while this process is idle
-        mk_tuning = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen")}
-        if tuned["render_regen"] == 1:
-            mk_tuning["render_grid"] = tuned["render_grid"]
-        traffic = measure_traffic(mk_tuning)
-        progress(f"traffic counters: {traffic}")
+    # launch shapes the HBM-traffic child runs use (the timed ones)
+    prof_tuning = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen")}
+    if tuned["render_regen"] == 1:
+        prof_tuning["render_grid"] = tuned["render_grid"]
     # warmup (untimed)
     if a.warmup > 0:
         integ.dispatch(scene, a.warmup, first_sample=0, readback=False)
@@ -400,13 +396,6 @@ def main():
     t0 = time.perf_counter()
     wst = wf.dispatch(scene, a.steps, first_sample=a.warmup, readback=False)
     wel = time.perf_counter() - t0
-    if traffic:
-        out["roofline"]["traffic_note"] = (
-            "HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of "
-            "bench.py --profile-kernels (one 1920x1080 pass of "
-            f"{traffic['render_kernel']} at the tuned launch shape; the closest-hit batch at "
-            "its default one), scaled by a 256 MiB copy "
-            f"(fetch x {traffic['fetch_scale']}, write x {traffic['write_scale']})")
     out["tuning"] = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen",
                                            "render_grid", "wf_stack", "wf_grid",
                                            "wf_trace_block", "wf_persistent", "wf_refill",
@@ -425,8 +414,20 @@ def main():
         rb = cr.DeviceBuffer(rays.nbytes)
         hb = cr.DeviceBuffer(n * 16)
         rb.copyFromHost(rays)
-        ttuned = autotune.tune_trace(scene, rays, reps=2, stacks=(1, 2, 3), blocks=(64, 128),
-                                     persistent=((2, 0),))
+        ttuned = autotune.tune_trace(scene, rays, reps=2, stacks=(2, 3), blocks=(64, 128, 256),
+                                     persistent=((2, 0), (1, 16), (1, 32)),
+                                     pt_grids=(2048, 4096))
+        prof_tuning.update(stack=ttuned["stack"], trace_block=ttuned["trace_block"],
+                           persistent=ttuned["persistent"])
+        if ttuned["persistent"] == 1:
+            prof_tuning.update(refill=ttuned["refill"], pt_grid=ttuned["pt_grid"])
+        if not a.no_traffic:
+            # HBM traffic of both kernels at the launch shapes chosen, from two
+            # rocprofv3 child runs while this process is idle
+            traffic = measure_traffic(prof_tuning)
+            progress(f"traffic counters: {traffic}")
+            if traffic:
+                out["roofline"]["traffic"] = traffic["render"]
         cr.trace_closest_device(scene, rb, n, hb)  # warmup
         cnt = cr.trace_closest_device(scene, rb, n, hb, count_work=True)
         times = []
@@ -437,7 +438,8 @@ def main():
         ach = b_ray * n / (kms / 1e3) / 1e9
         out["closest_hit"] = {
             "rays": n, "grays_per_s": round(n / (kms / 1e3) / 1e9, 4),
-            "kernel_ms": round(kms, 3), "kernel": "kTraceClosest",
+            "kernel_ms": round(kms, 3),
+            "kernel": "kTracePersistent" if ttuned["persistent"] == 1 else "kTraceClosest",
             "node_visits_per_ray": round(cnt["node_visits"] / n, 3),
             "tri_tests_per_ray": round(cnt["tri_tests"] / n, 3),
             "tuning": {k: ttuned[k] for k in ("stack", "trace_block", "persistent", "refill",
@@ -448,6 +450,17 @@ def main():
                          "bytes_per_ray": round(b_ray, 1)}}
         del rb, hb
 
+    if a.no_trace and not a.no_traffic:
+        traffic = measure_traffic(prof_tuning)
+        if traffic:
+            out["roofline"]["traffic"] = traffic["render"]
+    if traffic:
+        out["roofline"]["traffic_note"] = (
+            "HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of "
+            "bench.py --profile-kernels (one 1920x1080 pass of "
+            f"{traffic['render_kernel']} at the tuned launch shape; the closest-hit batch "
+            "likewise), scaled by a 256 MiB copy "
+            f"(fetch x {traffic['fetch_scale']}, write x {traffic['write_scale']})")
     if "closest_hit" in out:
         progress(f"closest hit: {out['closest_hit']['grays_per_s']} Grays/s")
     if a.batch_scenes > 0:

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W,
 
 // Path regeneration: persistent blocks; every lane runs one bounce of its
 // path per iteration and, when its path ends, writes it and takes the next
-// path index (one atomic per wave for all lanes that need one), so lanes do
+// path index (a WaveDispenser: one atomic per chunk of paths), so lanes do
 // not idle while the longest path of their wave finishes.  Path p is pixel
 // p % (tiles * 256) (tile order, 8x8 per wave as kRenderPass) of sample
 // slot p / (tiles * 256).  Each path runs exactly tracePixel's steps on the
@@ -191,7 +191,9 @@ kRenderRegen(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint3
     PathCtx<false, typename StackOf<SM>::T> c{s, stk, 0, {0, 0}};
     const size_t npix = size_t(cam.width) * cam.height;
     const uint32_t perSlot = tiles * 256u;
-    bool live = false, exhausted = false;
+    bool live = false;
+    WaveDispenser disp;
+    const uint32_t chunk = dispenserChunk(total);
     uint32_t depth = 0;
     size_t out = 0;
     Sampler smp(0u, 0u, 0u);
@@ -206,11 +208,9 @@ kRenderRegen(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint3
         live = false;
     };
     while (true) {
-        if (!exhausted) {
-            const bool want = !live;
-            const uint32_t p = wavePush(next, want);
-            exhausted = __any(want && p >= total);
-            if (want && p < total) {
+        {
+            const uint32_t p = disp.take(next, total, chunk, !live);
+            if (p < total) {
                 const uint32_t slot = p / perSlot, q = p - slot * perSlot;
                 uint32_t x, y;
                 tileCoordOf(q >> 8, q & 255u, tilesX, x, y);
@@ -230,7 +230,7 @@ kRenderRegen(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint3
             }
         }
         if (!__any(live)) {
-            if (exhausted) break;
+            if (disp.exhausted) break;
             continue;
         }
         if (live) {

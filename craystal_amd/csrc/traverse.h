@@ -19,6 +19,7 @@
 #pragma once
 #include "device_math.h"
 #include "traverse_limits.h"
+#include "wave.h"
 
 namespace crh {
 
@@ -335,42 +336,29 @@ CRH_D bool travStep(const SceneView& s, TravState& t, Stack& stack, WorkCount& w
 }
 
 // Persistent traversal with per-lane ray refill.  Rays [0, n) are handed
-// out through *counter (zero at launch): lanes without a ray take the next
-// ones, one atomic per wave (ballot + mbcnt); a wave keeps stepping its
-// lanes and refills once `refill` or more of them are idle while rays
-// remain.  src.load(i) gives ray i, src.store(i, isHit, hit) its result.
+// out by a WaveDispenser over *counter (zero at launch), one atomic per
+// chunk.  A wave keeps stepping its lanes and refills once `refill` or more
+// of them are idle while rays remain.  src.load(i) gives ray i,
+// src.store(i, isHit, hit) its result.
 template <bool ANY, bool COUNT, class Stack, class Src>
 CRH_D void persistentTrace(const SceneView& s, Src& src, uint32_t* counter, uint32_t n,
                            uint32_t refill, Stack& stack, WorkCount& wc) {
     const uint32_t kNone = 0xFFFFFFFFu;
+    const uint32_t chunk = dispenserChunk(n);
     uint32_t mine = kNone;
-    bool exhausted = false;  // wave-uniform: no rays left to hand out
+    WaveDispenser disp;
     TravState t;
     while (true) {
-        if (!exhausted) {
-            uint64_t need = __ballot(mine == kNone);
-            if (need) {
-                uint32_t lo = uint32_t(need), hi = uint32_t(need >> 32);
-                uint32_t below = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
-                int leader = __ffsll((unsigned long long)need) - 1;
-                uint32_t base = 0;
-                if (int(__lane_id()) == leader) base = atomicAdd(counter, uint32_t(__popcll(need)));
-                base = __shfl(base, leader, 64);
-                if (base + uint32_t(__popcll(need)) >= n) exhausted = true;
-                if (mine == kNone) {
-                    uint32_t idx = base + below;
-                    if (idx < n) {
-                        mine = idx;
-                        if (travInit<ANY>(s, src.load(idx), t)) {
-                            src.store(idx, t.isHit, t.hit);
-                            mine = kNone;
-                        }
-                    }
-                }
+        const uint32_t idx = disp.take(counter, n, chunk, mine == kNone);
+        if (idx < n) {
+            mine = idx;
+            if (travInit<ANY>(s, src.load(idx), t)) {
+                src.store(idx, t.isHit, t.hit);
+                mine = kNone;
             }
         }
         if (__ballot(mine != kNone) == 0) {
-            if (exhausted) break;
+            if (disp.exhausted) break;
             continue;  // every lane finished at init: fetch again
         }
         while (true) {
@@ -383,7 +371,7 @@ CRH_D void persistentTrace(const SceneView& s, Src& src, uint32_t* counter, uint
             uint64_t idle = __ballot(mine == kNone);
             int nIdle = __popcll(idle);
             if (nIdle == 64) break;
-            if (!exhausted && nIdle >= int(refill)) break;
+            if (!disp.exhausted && nIdle >= int(refill)) break;
         }
     }
 }

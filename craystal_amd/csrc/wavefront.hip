@@ -445,7 +445,7 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
 
 // Finishes every path still in flight after `d0` wavefront bounces in one
 // persistent launch: a lane takes the next entry of the extension and MIS
-// queues (one atomic per wave), traces and resolves the pending MIS ray of
+// queues (a WaveDispenser: one atomic per chunk), traces and resolves the pending MIS ray of
 // the path's last vertex (as kWfMisResolve), gives the record of its last
 // guided vertex its snapshot (as kWfShadeHit), and, if the path continues,
 // runs pathTraceKernel's loop from depth d0 one bounce per iteration (as
@@ -461,17 +461,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     PathCtx<false, typename StackOf<SM>::T> c{s, stk, 0, {0, 0}};
     const uint32_t nExt = w.counters[kCExt0 + cur];
     const uint32_t total = nExt + w.counters[kCMis0 + cur];
-    bool live = false, exhausted = false;
+    bool live = false;
+    WaveDispenser disp;
+    const uint32_t chunk = dispenserChunk(total);
     uint32_t p = 0, depth = 0;
     Ray ray{};
     Spec rad = sp(0.f), beta = sp(1.f);
     Sampler smp(0u, 0u, 0u);
     while (true) {
-        if (!exhausted) {
-            const bool want = !live;
-            const uint32_t i = wavePush(&w.counters[kCPtExt], want);
-            exhausted = __any(want && i >= total);
-            if (want && i < total) {
+        {
+            const uint32_t i = disp.take(&w.counters[kCPtExt], total, chunk, !live);
+            if (i < total) {
                 const bool alive = i < nExt;
                 p = alive ? w.qExt[cur][i] : w.qMis[cur][i - nExt];
                 const uint32_t fl = w.flags[p];
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
             }
         }
         if (!__any(live)) {
-            if (exhausted) break;
+            if (disp.exhausted) break;
             continue;
         }
         if (live) {
