@@ -49,9 +49,9 @@ def parse():
     ap.add_argument("--trace-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-trace", action="store_true", help="skip the closest-hit batch")
-    ap.add_argument("--guided-seconds", type=float, default=10.0,
+    ap.add_argument("--guided-seconds", type=float, default=8.0,
                     help="render time of each run of the guided-vs-plain comparison (0 = skip)")
-    ap.add_argument("--guided-runs", type=int, default=3,
+    ap.add_argument("--guided-runs", type=int, default=5,
                     help="independent runs per method in the guided-vs-plain comparison")
     ap.add_argument("--batch-scenes", type=int, default=64,
                     help="scenes of the many-scene batch leg (0 = skip)")
@@ -208,9 +208,17 @@ def guided_leg(cr, scenes, seconds: float, repeats: int) -> dict:
                   "median_extrapolated_seconds": float(np.median([x["extrapolated_seconds"]
                                                                  for x in v])),
                   "msamples_per_s": round(float(np.mean([x["msamples_per_s"] for x in v])), 2),
-                  "reached_target_s": [x["seconds_to_target"] for x in v]}
+                  "reached_target_s": [x["seconds_to_target"] for x in v],
+                  "relmse_trimmed_at_end": [round(x["final_relmse_trimmed"], 5) for x in v],
+                  "median_relmse_trimmed_at_end": float(np.median(
+                      [x["final_relmse_trimmed"] for x in v]))}
     out["speedup_median_time_to_target"] = round(
         out["plain"]["median_extrapolated_seconds"] / out["guided"]["median_extrapolated_seconds"],
+        3)
+    # diagnostic: the same ratio on relMSE without the 0.1 % of pixels with the
+    # largest error (equal render times, relMSE ~ 1/t)
+    out["speedup_trimmed_diagnostic"] = round(
+        out["plain"]["median_relmse_trimmed_at_end"] / out["guided"]["median_relmse_trimmed_at_end"],
         3)
     return out
 
@@ -309,10 +317,11 @@ def main():
     # candidate must render the same image bit for bit)
     from craystal_amd import autotune
     tuned = autotune.tune_render(scene, cfg, spp=4, stacks=(1, 2, 3), wf_grids=(16384,),
-                                 wf_blocks=(128, 256), wf_persistent=((2, 0), (1, 16)),
-                                 wf_pools=(1 << 22, 1 << 23), render_waves=(1, 4),
+                                 wf_blocks=(128, 256),
+                                 wf_persistent=((2, 0), (1, 16, 2048), (1, 16, 4096)),
+                                 wf_pools=(1 << 23,), render_waves=(1, 4),
                                  render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
-                                 wf_finishes=(65, 1, 2, 3, 4))
+                                 wf_finishes=(65, 1, 2, 3))
     sensor.clear()
 
     progress(f"scene built, autotuned: {tuned['render_stack']=} {tuned['wf_pool']=}")
@@ -398,7 +407,8 @@ def main():
     wel = time.perf_counter() - t0
     out["tuning"] = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen",
                                            "render_grid", "wf_stack", "wf_grid",
-                                           "wf_trace_block", "wf_persistent", "wf_refill",
+                                           "wf_trace_block", "wf_trace_grid", "wf_persistent",
+                                           "wf_refill",
                                            "wf_pool", "wf_finish")}
     out["tuning"]["wavefront_ms_per_spp"] = tuned["wavefront_ms"]
     out["tuning"]["megakernel_ms_per_spp"] = tuned["megakernel_ms"]

@@ -74,8 +74,9 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
     """Tune the megakernel stack placement, register budget and path
     regeneration ((mode, persistent grid) pairs, mode 2 = one path per
     thread) and the wavefront stack placement,
-    grid/block sizes, persistence ((mode, refill) pairs) and path pool (paths
-    in flight; None keeps the current one) on the scene's current
+    grid/block sizes, persistence ((mode, refill) or (mode, refill, traversal
+    grid) tuples; mode 1 = lanes refill, 2 = off), path pool (paths in
+    flight; None keeps the current one) and finishing depth on the scene's current
     camera/sensor; every candidate must give the same image bit for bit.
     Applies the best choice of each integrator and returns the faster
     integrator's mode with the timings (ms per spp) of both."""
@@ -107,25 +108,27 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
     pool0 = ctx.get_tuning()["wf_pool"]
     pools = [pool0 if p is None else p for p in wf_pools]
 
-    def wf_kw(st, g, b, pm, rf, pool, fin):
-        return dict(wf_stack=st, wf_trace_grid=g, wf_shade_grid=g, wf_trace_block=b,
+    def wf_kw(st, g, b, pm, rf, tg, pool, fin):
+        return dict(wf_stack=st, wf_trace_grid=tg or g, wf_shade_grid=g, wf_trace_block=b,
                     wf_persistent=pm, wf_pool=pool, wf_finish=fin,
                     **({"wf_refill": rf} if pm == 1 else {}))
 
-    for st, g, b, (pm, rf), pool, fin in itertools.product(stacks, wf_grids, wf_blocks,
-                                                           wf_persistent, pools, wf_finishes):
-        ctx.set_tuning(**wf_kw(st, g, b, pm, rf, pool, fin))
+    pers = [tuple(p) + (0,) * (3 - len(p)) for p in wf_persistent]
+    for st, g, b, (pm, rf, tg), pool, fin in itertools.product(stacks, wf_grids, wf_blocks,
+                                                               pers, pools, wf_finishes):
+        ctx.set_tuning(**wf_kw(st, g, b, pm, rf, tg, pool, fin))
         ms, img = run(wf_cfg)
         assert img.tobytes() == ref.tobytes(), "wavefront launch shape changed the image"
-        res_wf[(st, g, b, pm, rf, pool, fin)] = ms
+        res_wf[(st, g, b, pm, rf, tg, pool, fin)] = ms
     best_wf = min(res_wf, key=res_wf.get)
-    st, g, b, pm, rf, pool, fin = best_wf
-    ctx.set_tuning(**wf_kw(st, g, b, pm, rf, pool, fin))
+    st, g, b, pm, rf, tg, pool, fin = best_wf
+    ctx.set_tuning(**wf_kw(st, g, b, pm, rf, tg, pool, fin))
     mk_ms, wf_ms = res_mk[best_mk], res_wf[best_wf]
     return {"mode": L.MODE_WAVEFRONT if wf_ms < mk_ms else L.MODE_MEGAKERNEL,
             "render_stack": best_stack, "render_waves": best_waves, "render_regen": best_regen,
             "render_grid": best_grid, "wf_stack": st, "wf_grid": g, "wf_trace_block": b,
-            "wf_persistent": pm, "wf_refill": rf, "wf_pool": pool, "wf_finish": fin,
+            "wf_persistent": pm, "wf_refill": rf, "wf_trace_grid": tg or g, "wf_pool": pool,
+            "wf_finish": fin,
             "megakernel_ms": {f"stack={k[0]},waves={k[1]},regen={k[2]},grid={k[3]}": round(v, 3)
                               for k, v in res_mk.items()},
             "wavefront_ms": {str(k): round(v, 3) for k, v in res_wf.items()}}

@@ -27,6 +27,20 @@ def relmse(img: np.ndarray, ref: np.ndarray) -> float:
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + EPS)))
 
 
+def relmse_trimmed(img: np.ndarray, ref: np.ndarray, frac: float = 1e-3) -> float:
+    """relMSE without the `frac` of pixels with the largest error: a
+    diagnostic of the bulk of the image that single extreme samples do not
+    move (the metric itself is relmse)."""
+    e = ((img - ref) ** 2 / (ref ** 2 + EPS)).mean(axis=-1).ravel()
+    keep = len(e) - int(len(e) * frac)
+    return float(np.sort(e)[:keep].mean()) if keep > 0 else float("nan")
+
+
+class Curve(list):
+    """[(t, spp, relmse)] with the final image's trimmed relMSE."""
+    trimmed = None
+
+
 def reference_image(scene: Scene, camd: dict, size, cfg: Configs, spp: int,
                     chunk: int = 256, first: int = 1 << 24) -> np.ndarray:
     sensor = Sensor(size)
@@ -47,16 +61,20 @@ def _progressive(render_chunk, sensor: Sensor, ref: np.ndarray, budget_s: float,
     """Runs render_chunk(first_sample) -> (spp rendered, seconds) until the
     relMSE target (0 = never) or the time budget; returns the curve
     [(t, spp, relmse)]."""
-    t, spp, curve = 0.0, 0, []
+    t, spp, curve = 0.0, 0, Curve()
+    img = None
     while t < budget_s and spp < max_spp:
         k, dt = render_chunk(spp)
         t += dt
         spp += k
         sensor.readbackDeviceData()
-        e = relmse(sensor.createImage(), ref)
+        img = sensor.createImage()
+        e = relmse(img, ref)
         curve.append((t, spp, e))
         if target > 0 and e <= target:
             break
+    if img is not None:
+        curve.trimmed = relmse_trimmed(img, ref)
     return curve
 
 
@@ -133,6 +151,7 @@ def summarize(curve, target: float, pixels: int) -> dict:
     return {"seconds_to_target": t_hit, "spp_to_target": spp_hit,
             "extrapolated_seconds": extrapolate_time(curve, target),
             "final_relmse": curve[-1][2], "final_spp": curve[-1][1],
+            "final_relmse_trimmed": getattr(curve, "trimmed", None),
             "final_seconds": curve[-1][0],
             "msamples_per_s": curve[-1][1] * pixels / curve[-1][0] / 1e6,
             "curve": [(round(t, 4), k, round(e, 5)) for t, k, e in curve[::max(1, len(curve) // 24)]]}
