@@ -321,12 +321,13 @@ def main():
                                  wf_persistent=((2, 0), (1, 16, 2048), (1, 16, 4096)),
                                  wf_pools=(1 << 23,), render_waves=(1, 4),
                                  render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
-                                 wf_finishes=(65, 1, 2, 3))
+                                 wf_finishes=(65, 1, 2, 3), render_bands=(1, 2))
     sensor.clear()
 
     progress(f"scene built, autotuned: {tuned['render_stack']=} {tuned['wf_pool']=}")
     # launch shapes the HBM-traffic child runs use (the timed ones)
-    prof_tuning = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen")}
+    prof_tuning = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen",
+                                         "render_band")}
     if tuned["render_regen"] == 1:
         prof_tuning["render_grid"] = tuned["render_grid"]
     # warmup (untimed)
@@ -406,7 +407,7 @@ def main():
     wst = wf.dispatch(scene, a.steps, first_sample=a.warmup, readback=False)
     wel = time.perf_counter() - t0
     out["tuning"] = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen",
-                                           "render_grid", "wf_stack", "wf_grid",
+                                           "render_grid", "render_band", "wf_stack", "wf_grid",
                                            "wf_trace_block", "wf_trace_grid", "wf_persistent",
                                            "wf_refill",
                                            "wf_pool", "wf_finish")}

@@ -70,7 +70,7 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
                 stacks=(1, 2), wf_grids=(1024, 2048, 4096), wf_blocks=(128, 256),
                 wf_persistent=((2, 0), (1, 16)), wf_pools=(None,), render_waves=(1,),
                 render_regen=((2, 0), (1, 1024), (1, 2048)), wf_finishes=(65,),
-                first_sample: int = 1 << 20) -> dict:
+                render_bands=(1,), first_sample: int = 1 << 20) -> dict:
     """Tune the megakernel stack placement, register budget and path
     regeneration ((mode, persistent grid) pairs, mode 2 = one path per
     thread) and the wavefront stack placement,
@@ -90,19 +90,22 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
         st = PathTraceIntegrator(c).dispatch(scene, spp, first_sample=first_sample)
         return st["kernel_ms"] / spp, sensor.data.copy()
 
-    def mk_kw(st, wv, rg, grid):
-        return dict(render_stack=st, render_waves=wv, render_regen=rg,
+    def mk_kw(st, wv, rg, grid, band):
+        return dict(render_stack=st, render_waves=wv, render_regen=rg, render_band=band,
                     **({"render_grid": grid} if rg == 1 else {}))
 
-    for st, wv, (rg, grid) in itertools.product(stacks, render_waves, render_regen):
-        ctx.set_tuning(**mk_kw(st, wv, rg, grid))
+    # (regeneration, grid, band): the XCD banding applies to one path per thread
+    mk_modes = [(rg, grid, band) for rg, grid in render_regen
+                for band in (render_bands if rg == 2 else render_bands[:1])]
+    for st, wv, (rg, grid, band) in itertools.product(stacks, render_waves, mk_modes):
+        ctx.set_tuning(**mk_kw(st, wv, rg, grid, band))
         run(cfg)
         ms, img = run(cfg)
         ref = img if ref is None else ref
         assert img.tobytes() == ref.tobytes(), "megakernel launch shape changed the image"
-        res_mk[(st, wv, rg, grid)] = ms
+        res_mk[(st, wv, rg, grid, band)] = ms
     best_mk = min(res_mk, key=res_mk.get)
-    best_stack, best_waves, best_regen, best_grid = best_mk
+    best_stack, best_waves, best_regen, best_grid, best_band = best_mk
     ctx.set_tuning(**mk_kw(*best_mk))
     wf_cfg = Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
     pool0 = ctx.get_tuning()["wf_pool"]
@@ -126,11 +129,11 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
     mk_ms, wf_ms = res_mk[best_mk], res_wf[best_wf]
     return {"mode": L.MODE_WAVEFRONT if wf_ms < mk_ms else L.MODE_MEGAKERNEL,
             "render_stack": best_stack, "render_waves": best_waves, "render_regen": best_regen,
-            "render_grid": best_grid, "wf_stack": st, "wf_grid": g, "wf_trace_block": b,
+            "render_grid": best_grid, "render_band": best_band, "wf_stack": st, "wf_grid": g, "wf_trace_block": b,
             "wf_persistent": pm, "wf_refill": rf, "wf_trace_grid": tg or g, "wf_pool": pool,
             "wf_finish": fin,
-            "megakernel_ms": {f"stack={k[0]},waves={k[1]},regen={k[2]},grid={k[3]}": round(v, 3)
-                              for k, v in res_mk.items()},
+            "megakernel_ms": {f"stack={k[0]},waves={k[1]},regen={k[2]},grid={k[3]},band={k[4]}":
+                              round(v, 3) for k, v in res_mk.items()},
             "wavefront_ms": {str(k): round(v, 3) for k, v in res_wf.items()}}
 
 

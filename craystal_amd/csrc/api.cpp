@@ -421,6 +421,7 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(t->render_regen <= 2, "render_regen must be 0 (keep), 1 (on) or 2 (off)");
     ARG_CHECK(t->render_grid <= (1u << 20), "render_grid too large");
     ARG_CHECK(t->wf_finish <= 65, "wf_finish must be 0 (keep), 1..64 (bounces) or 65 (off)");
+    ARG_CHECK(t->render_band <= 2, "render_band must be 0 (keep), 1 (on) or 2 (off)");
     ARG_CHECK(t->wf_pool <= (1u << 28) && t->render_pool <= (1u << 28),
               "wf_pool and render_pool must be at most 2^28 paths");
     ARG_CHECK(okStack(t->batch_stack),
@@ -450,6 +451,7 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     if (t->render_regen) o.renderRegen = t->render_regen == 1 ? 1u : 0u;
     if (t->render_grid) o.renderGrid = t->render_grid;
     if (t->wf_finish) o.wfFinish = t->wf_finish == 65 ? 0u : t->wf_finish;
+    if (t->render_band) o.xcdBand = t->render_band == 1 ? 1u : 0u;
     if (t->batch_stack) o.batchPriv = t->batch_stack - 1u;
     if (t->batch_waves) o.batchWaves = t->batch_waves;
     if (t->batch_tile_w) o.batchTileW = t->batch_tile_w;
@@ -481,6 +483,7 @@ int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
     t->render_regen = o.renderRegen ? 1u : 2u;
     t->render_grid = o.renderGrid;
     t->wf_finish = o.wfFinish ? o.wfFinish : 65u;
+    t->render_band = o.xcdBand ? 1u : 2u;
     t->batch_stack = o.batchPriv + 1u;
     t->batch_waves = o.batchWaves;
     t->batch_tile_w = o.batchTileW;

@@ -38,6 +38,7 @@ struct Tuning {
     uint32_t batchWaves = 4;         // batch megakernel: the same for kBatchRender
     uint32_t renderRegen = 1;        // megakernel: lanes take a new path when theirs ends
     uint32_t renderGrid = 2048;      // megakernel with regeneration: persistent blocks
+    uint32_t xcdBand = 0;            // megakernel: each XCD takes a contiguous band of tiles
     uint32_t wfFinish = 2;           // wavefront: finish per path after this many bounces (0 off)
     // batched rendering
     uint32_t batchPriv = 2;

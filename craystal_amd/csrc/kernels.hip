@@ -86,6 +86,17 @@ __global__ __launch_bounds__(kTraceBlock) void kTracePersistent(
 // ---------------------------------------------------------------------------
 // Megakernel integrator (Core/Integrator.cu:142-227); per-path steps in path.h
 // ---------------------------------------------------------------------------
+// Workgroups go to the 8 XCDs round robin (block b to XCD b mod 8), and
+// each XCD has its own L2.  This bijection on [0, n) gives the blocks of
+// one XCD a contiguous run of work items (a band of tiles of the image), so
+// neighbouring primary rays share their XCD's L2; the last n mod 8 items
+// map to themselves.
+CRH_D uint32_t xcdBand(uint32_t b, uint32_t n) {
+    const uint32_t per = n / 8u;
+    if (b >= 8u * per) return b;
+    return (b % 8u) * per + b / 8u;
+}
+
 // Pixel `inTile` (0..255) of a 16x16 tile; each 64-pixel quarter is an 8x8
 // block (one wave of a 256-thread block).
 CRH_D void tileCoordOf(uint32_t tile, uint32_t inTile, uint32_t tilesX, uint32_t& x,
@@ -149,20 +160,21 @@ CRH_D void flushCounters(unsigned long long* counters, uint32_t rays, uint32_t n
     }
 }
 
-// Block b renders tile b % tiles of sample index si + b / tiles into slot
-// b / tiles of L and P (npix entries per slot).
+// Work item w = xcdBand(block) renders tile w % tiles of sample index
+// si + w / tiles into slot w / tiles of L and P (npix entries per slot).
 // W: minimum waves per SIMD the register allocation must allow (1 = the
 // compiler's choice; 4 caps the kernel at 128 VGPRs).
 template <bool MIS, bool COUNT, int SM, int W, bool SPEC = false>
 __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W, 8))) void kRenderPass(
     SceneView s, CameraView cam, IntegratorView cfg, uint32_t si,
-    uint32_t tilesX, uint32_t tiles, float4* __restrict__ L, float2* __restrict__ P,
-    unsigned long long* __restrict__ counters) {
+    uint32_t tilesX, uint32_t tiles, uint32_t band, float4* __restrict__ L,
+    float2* __restrict__ P, unsigned long long* __restrict__ counters) {
     extern __shared__ uint2 lds[];
-    const uint32_t slot = blockIdx.x / tiles;
+    const uint32_t b = band ? xcdBand(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t slot = b / tiles;
     const size_t off = size_t(slot) * cam.width * cam.height;
     uint32_t x, y;
-    tileCoord(blockIdx.x - slot * tiles, tilesX, x, y);
+    tileCoord(b - slot * tiles, tilesX, x, y);
     uint32_t rays = 0;
     WorkCount wc{0, 0};
     if (x < cam.width && y < cam.height) {
@@ -467,28 +479,29 @@ hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
         return hipGetLastError();
     }
     const dim3 grid(tiles * active);
+    const uint32_t band = t.xcdBand;
     withStack(t.renderPriv, [&](auto sm) {
         constexpr int SM = decltype(sm)::value;
         withWaves(t.renderWaves, [&](auto wv) {
             constexpr int W = decltype(wv)::value;
             if (spectral && cfg.useMIS)
                 hipLaunchKernelGGL((kRenderPass<true, false, SM, W, true>), grid, block, lds, st, s,
-                                   cam, cfg, si, tilesX, tiles, L, P, counters);
+                                   cam, cfg, si, tilesX, tiles, band, L, P, counters);
             else if (spectral)
                 hipLaunchKernelGGL((kRenderPass<false, false, SM, W, true>), grid, block, lds, st,
-                                   s, cam, cfg, si, tilesX, tiles, L, P, counters);
+                                   s, cam, cfg, si, tilesX, tiles, band, L, P, counters);
             else if (cfg.useMIS && countWork)
                 hipLaunchKernelGGL((kRenderPass<true, true, SM, W>), grid, block, lds, st, s, cam,
-                                   cfg, si, tilesX, tiles, L, P, counters);
+                                   cfg, si, tilesX, tiles, band, L, P, counters);
             else if (cfg.useMIS)
                 hipLaunchKernelGGL((kRenderPass<true, false, SM, W>), grid, block, lds, st, s, cam,
-                                   cfg, si, tilesX, tiles, L, P, counters);
+                                   cfg, si, tilesX, tiles, band, L, P, counters);
             else if (countWork)
                 hipLaunchKernelGGL((kRenderPass<false, true, SM, W>), grid, block, lds, st, s, cam,
-                                   cfg, si, tilesX, tiles, L, P, counters);
+                                   cfg, si, tilesX, tiles, band, L, P, counters);
             else
                 hipLaunchKernelGGL((kRenderPass<false, false, SM, W>), grid, block, lds, st, s,
-                                   cam, cfg, si, tilesX, tiles, L, P, counters);
+                                   cam, cfg, si, tilesX, tiles, band, L, P, counters);
         });
     });
     return hipGetLastError();

@@ -281,6 +281,9 @@ typedef struct crh_tuning {
     uint32_t batch_chunks;   /* batched rendering: work ranges with own counters, 1..8 */
     uint32_t batch_graph;    /* batched rendering: 1 = passes replayed from a hipGraph, 2 = off */
     uint32_t batch_waves;    /* batch megakernel register budget: 1 or 4 waves per SIMD */
+    uint32_t render_band;    /* megakernel (one path per thread): 1 = the blocks of each XCD
+                                take a contiguous band of tiles (its own L2), 2 = tiles in
+                                block order */
 } crh_tuning;
 
 /* ---- errors / device ------------------------------------------------- */

@@ -44,7 +44,9 @@ def test_render_tuning_invariance(gpu):
     ctx = cr.get_context(0)
     saved = ctx.get_tuning()
     try:
-        res = autotune.tune_render(scene, cfg, spp=1, wf_grids=(64, 512), wf_blocks=(64, 256))
+        res = autotune.tune_render(scene, cfg, spp=1, wf_grids=(64, 512), wf_blocks=(64, 256),
+                                   render_bands=(1, 2))
+        assert len(res["megakernel_ms"]) == 2 * 4   # stacks x (2 bands + 2 regeneration grids)
         assert res["mode"] in (L.MODE_MEGAKERNEL, L.MODE_WAVEFRONT)
         assert len(res["wavefront_ms"]) == 16
         t = ctx.get_tuning()
