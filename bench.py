@@ -177,7 +177,10 @@ def guided_leg(cr, scenes, seconds: float, repeats: int) -> dict:
     renders `repeats` independent runs (disjoint sample indices, fresh guide)
     of `seconds` render time, and the medians are reported.  Guided: the
     wavefront with one-sample MIS (alpha 0.5) at Lambertian vertices, the
-    guide training during the first 25% of each run."""
+    guide training during the first 25% of each run.  The plain megakernel's
+    launch shape is tuned on this scene first (path regeneration pays off
+    here), so both methods run at their best."""
+    from craystal_amd import autotune
     from craystal_amd import experiments as X
     ref = np.load(ROOT / "craystal_amd" / "data" / "room256_plain65536.npy")
     data, camd = scenes.occluded_room()
@@ -187,18 +190,27 @@ def guided_leg(cr, scenes, seconds: float, repeats: int) -> dict:
     sensor = cr.Sensor(size)
     scene.setCamera(scenes.make_camera(camd, sensor))
     px = size[0] * size[1]
+    tuned = autotune.tune_render(scene, cfg, spp=16, stacks=(2, 3), render_waves=(4,),
+                                 render_regen=((2, 0), (1, 1024), (1, 2048)),
+                                 wf_grids=(16384,), wf_blocks=(128,), wf_persistent=((2, 0),),
+                                 wf_finishes=(65,))
+    # the guided wavefront keeps 256 samples per pixel in flight (16M paths):
+    # its per-bounce launches then cost less per sample than at 64
+    cr.get_context(0).set_tuning(wf_pool=1 << 24)
+    sensor.clear()
     runs = {"plain": [], "guided": []}
     for r in range(repeats):
         first = (r + 1) << 21
         pc = X.plain_curve(scene, sensor, cfg, ref, seconds, 64, first=first)
         runs["plain"].append(X.summarize(pc, 0.01, px))
-        gc, info = X.guided_curve(scene, sensor, data, cfg, ref, seconds, 64, first=first,
+        gc, info = X.guided_curve(scene, sensor, data, cfg, ref, seconds, 256, first=first,
                                   seed=11 + r, materials=1, target=0, train_batch=4096,
                                   train_steps=4, lr=1e-2, alpha=0.5)
         runs["guided"].append({**X.summarize(gc, 0.01, px), **info})
         progress(f"guided leg run {r}: plain {runs['plain'][-1]['final_relmse']:.4f} "
                  f"guided {runs['guided'][-1]['final_relmse']:.4f}")
     out = {"scene": "occluded room, 171,388 tris, 256x256, max depth 16", "target_relmse": 0.01,
+           "plain_tuning": {k: tuned[k] for k in ("render_stack", "render_regen", "render_grid")},
            "seconds_per_run": seconds, "runs": repeats,
            "reference": "craystal_amd/data/room256_plain65536.npy (plain megakernel, 65,536 spp)"}
     for k, v in runs.items():
