@@ -63,11 +63,31 @@ def parse():
                          "pass; prints the launch sizes and exits")
     ap.add_argument("--profile-tuning", default=None,
                     help="with --profile-kernels: JSON of launch-shape fields to apply first")
+    ap.add_argument("--profile-wavefront", action="store_true",
+                    help="counter-collection mode for the wavefront: the spheres scene at "
+                         "1920x1080, 8 spp (with --profile-tuning applied first)")
     ap.add_argument("--profile-guided", action="store_true",
                     help="counter-collection mode for the guiding kernels: the occluded room "
                          "at 256x256, 16 spp of the guided wavefront with training, 16 without, "
                          "then 16 spp of the plain megakernel and of the plain wavefront")
     return ap.parse_args()
+
+
+def profile_wavefront(cr, L, scenes, tuning: str | None = None):
+    """The wavefront integrator on the 1080p workload: one warm-up spp and 8
+    timed spp (depth 16, MIS), for rocprofv3 kernel traces."""
+    if tuning:
+        cr.get_context(0).set_tuning(**json.loads(tuning))
+    data, camd = scenes.spheres_scene()
+    scene = cr.Scene(data)
+    sensor = cr.Sensor(camd["size"])
+    scene.setCamera(scenes.make_camera(camd, sensor))
+    cfg = cr.Configs(maxDepth=16, rrThreshold=0.05, useMIS=True, rrStartDepth=3,
+                     mode=L.MODE_WAVEFRONT)
+    integ = cr.PathTraceIntegrator(cfg)
+    integ.dispatch(scene, 1, readback=False)
+    st = integ.dispatch(scene, 8, first_sample=1, readback=False)
+    print(json.dumps({"spp": 8, "ms_per_spp": st["kernel_ms"] / 8}), flush=True)
 
 
 def profile_guided(cr, L, scenes):
@@ -310,6 +330,9 @@ def main():
         return
     if a.profile_guided:
         profile_guided(cr, L, scenes)
+        return
+    if a.profile_wavefront:
+        profile_wavefront(cr, L, scenes, a.profile_tuning)
         return
 
     data, camd = scenes.spheres_scene()
