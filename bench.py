@@ -354,7 +354,7 @@ def main():
     tuned = autotune.tune_render(scene, cfg, spp=4, stacks=(1, 2, 3), wf_grids=(16384,),
                                  wf_blocks=(128, 256),
                                  wf_persistent=((2, 0), (1, 16, 2048), (1, 16, 4096)),
-                                 wf_pools=(1 << 23,), render_waves=(1, 4),
+                                 wf_pools=(1 << 23,), render_waves=(1, 4, 5),
                                  render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
                                  wf_finishes=(65, 1, 2, 3), render_bands=(1, 2))
     sensor.clear()

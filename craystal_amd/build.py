@@ -37,17 +37,30 @@ def _run(cmd: list[str]) -> None:
     subprocess.run(cmd, check=True)
 
 
+def _run_all(cmds: list[list[str]]) -> None:
+    """The compile commands at once (independent translation units)."""
+    for c in cmds:
+        print(" ".join(c), flush=True)
+    procs = [subprocess.Popen(c) for c in cmds]
+    codes = [p.wait() for p in procs]
+    for c, rc in zip(cmds, codes):
+        if rc != 0:
+            raise subprocess.CalledProcessError(rc, c)
+
+
 def build(verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in ("api.cpp", "bvh_build.cpp", "spectral.cpp"):
         o = OBJ / (src + ".o")
-        _run(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
+        cmds.append(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
         objs.append(str(o))
     for src in ("kernels.hip", "wavefront.hip", "guiding.hip"):
         o = OBJ / (src + ".o")
-        _run([str(ROCM / "bin" / "hipcc"), *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
+        cmds.append([str(ROCM / "bin" / "hipcc"), *HIPCC_FLAGS, "-c", str(CSRC / src), "-o",
+                     str(o)])
         objs.append(str(o))
+    _run_all(cmds)
     tmp = OUT.with_suffix(".so.tmp")
     _run([str(ROCM / "bin" / "hipcc"), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
           *objs, "-lpthread"])

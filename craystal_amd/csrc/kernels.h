@@ -59,10 +59,11 @@ inline void withStack(uint32_t mode, F&& f) {
     else f(std::integral_constant<int, 0>{});
 }
 // Register budget of the megakernels: 1 = the compiler's choice, 4 = at most
-// 128 VGPRs (4 waves per SIMD).  withWaves calls f(integral_constant<int, w>).
+// 128 VGPRs (4 waves per SIMD), 5 = at most 96 VGPRs (5 waves, more spills).  withWaves calls f(integral_constant<int, w>).
 template <class F>
 inline void withWaves(uint32_t w, F&& f) {
     if (w == 4) f(std::integral_constant<int, 4>{});
+    else if (w == 5) f(std::integral_constant<int, 5>{});
     else f(std::integral_constant<int, 1>{});
 }
 // LDS bytes per block of a stack mode.

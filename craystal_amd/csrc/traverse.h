@@ -149,30 +149,31 @@ CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, Stack& stack,
     uint32_t cur = s.rootRef;
     int sp = 0;
     while (true) {
+        // One node or one triangle per iteration: a lane in a long leaf does
+        // not hold the others of its wave for the whole leaf.
         if (cur & kLeafBit) {
             uint32_t i = cur & ~kLeafBit;
-            while (true) {
-                const float4* tp = reinterpret_cast<const float4*>(s.tris + i);
-                float4 a = tp[0], b = tp[1], c = tp[2];
-                uint32_t pw = __float_as_uint(a.w);
-                if (COUNT) wc.tris++;
-                float u, v, t;
-                if (triHit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z),
-                           mk3(c.x, c.y, c.z), r, u, v, t)) {
-                    if (!(t < r.tMin || t > r.tMax)) {
-                        isHit = true;
-                        if (t < hit.t) {
-                            hit.t = t;
-                            hit.u = u;
-                            hit.v = v;
-                            hit.prim = pw & ~kLeafBit;
-                        }
-                        closestT = smin(closestT, hit.t);
-                        if (ANY) return true;
+            const float4* tp = reinterpret_cast<const float4*>(s.tris + i);
+            float4 a = tp[0], b = tp[1], c = tp[2];
+            uint32_t pw = __float_as_uint(a.w);
+            if (COUNT) wc.tris++;
+            float u, v, t;
+            if (triHit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), r, u, v, t)) {
+                if (!(t < r.tMin || t > r.tMax)) {
+                    isHit = true;
+                    if (t < hit.t) {
+                        hit.t = t;
+                        hit.u = u;
+                        hit.v = v;
+                        hit.prim = pw & ~kLeafBit;
                     }
+                    closestT = smin(closestT, hit.t);
+                    if (ANY) return true;
                 }
-                if (pw & kLeafBit) break;
-                ++i;
+            }
+            if (!(pw & kLeafBit)) {  // the leaf goes on with its next triangle
+                cur = (i + 1u) | kLeafBit;
+                continue;
             }
         } else {
             if (COUNT) wc.nodes++;
@@ -262,34 +263,34 @@ CRH_D bool travInit(const SceneView& s, const Ray& ray, TravState& t) {
     return !boxHit(s.rootMin, s.rootMax, t.r, dummy);
 }
 
-// One iteration of traverse()'s loop; returns true when the ray is done.
+// One iteration of traverse()'s loop (one node or one triangle); returns
+// true when the ray is done.
 template <bool ANY, bool COUNT, class Stack>
 CRH_D bool travStep(const SceneView& s, TravState& t, Stack& stack, WorkCount& wc) {
     const RayPre& r = t.r;
-    if (t.cur & kLeafBit) {
+    if (t.cur & kLeafBit) {  // one triangle per step
         uint32_t i = t.cur & ~kLeafBit;
-        while (true) {
-            const float4* tp = reinterpret_cast<const float4*>(s.tris + i);
-            float4 a = tp[0], b = tp[1], c = tp[2];
-            uint32_t pw = __float_as_uint(a.w);
-            if (COUNT) wc.tris++;
-            float u, v, tt;
-            if (triHit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), r, u, v,
-                       tt)) {
-                if (!(tt < r.tMin || tt > r.tMax)) {
-                    t.isHit = true;
-                    if (tt < t.hit.t) {
-                        t.hit.t = tt;
-                        t.hit.u = u;
-                        t.hit.v = v;
-                        t.hit.prim = pw & ~kLeafBit;
-                    }
-                    t.closestT = smin(t.closestT, t.hit.t);
-                    if (ANY) return true;
+        const float4* tp = reinterpret_cast<const float4*>(s.tris + i);
+        float4 a = tp[0], b = tp[1], c = tp[2];
+        uint32_t pw = __float_as_uint(a.w);
+        if (COUNT) wc.tris++;
+        float u, v, tt;
+        if (triHit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), r, u, v, tt)) {
+            if (!(tt < r.tMin || tt > r.tMax)) {
+                t.isHit = true;
+                if (tt < t.hit.t) {
+                    t.hit.t = tt;
+                    t.hit.u = u;
+                    t.hit.v = v;
+                    t.hit.prim = pw & ~kLeafBit;
                 }
+                t.closestT = smin(t.closestT, t.hit.t);
+                if (ANY) return true;
             }
-            if (pw & kLeafBit) break;
-            ++i;
+        }
+        if (!(pw & kLeafBit)) {
+            t.cur = (i + 1u) | kLeafBit;
+            return false;
         }
     } else {
         if (COUNT) wc.nodes++;
