@@ -149,31 +149,34 @@ CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, Stack& stack,
     uint32_t cur = s.rootRef;
     int sp = 0;
     while (true) {
-        // One node or one triangle per iteration: a lane in a long leaf does
-        // not hold the others of its wave for the whole leaf.
+        // The whole leaf in an inner loop.  (The persistent trace kernel's
+        // travStep takes one triangle per step instead, so that lanes in long
+        // leaves do not hold their wave; here, where each lane carries a whole
+        // path, the inner loop measured faster.)
         if (cur & kLeafBit) {
             uint32_t i = cur & ~kLeafBit;
-            const float4* tp = reinterpret_cast<const float4*>(s.tris + i);
-            float4 a = tp[0], b = tp[1], c = tp[2];
-            uint32_t pw = __float_as_uint(a.w);
-            if (COUNT) wc.tris++;
-            float u, v, t;
-            if (triHit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), r, u, v, t)) {
-                if (!(t < r.tMin || t > r.tMax)) {
-                    isHit = true;
-                    if (t < hit.t) {
-                        hit.t = t;
-                        hit.u = u;
-                        hit.v = v;
-                        hit.prim = pw & ~kLeafBit;
+            while (true) {
+                const float4* tp = reinterpret_cast<const float4*>(s.tris + i);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                uint32_t pw = __float_as_uint(a.w);
+                if (COUNT) wc.tris++;
+                float u, v, t;
+                if (triHit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), r, u, v,
+                           t)) {
+                    if (!(t < r.tMin || t > r.tMax)) {
+                        isHit = true;
+                        if (t < hit.t) {
+                            hit.t = t;
+                            hit.u = u;
+                            hit.v = v;
+                            hit.prim = pw & ~kLeafBit;
+                        }
+                        closestT = smin(closestT, hit.t);
+                        if (ANY) return true;
                     }
-                    closestT = smin(closestT, hit.t);
-                    if (ANY) return true;
                 }
-            }
-            if (!(pw & kLeafBit)) {  // the leaf goes on with its next triangle
-                cur = (i + 1u) | kLeafBit;
-                continue;
+                if (pw & kLeafBit) break;  // the leaf's last triangle
+                ++i;
             }
         } else {
             if (COUNT) wc.nodes++;
@@ -228,9 +231,9 @@ CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, Stack& stack,
 
 // ---------------------------------------------------------------------------
 // The same traversal as a resumable state machine: travInit + repeated
-// travStep perform exactly the operations of traverse(), one loop iteration
-// (one internal node or one leaf) per step, so a persistent kernel can
-// interleave rays per lane without changing any result.
+// travStep perform exactly the operations of traverse(), one internal node or
+// one triangle per step, so a persistent kernel can interleave rays per lane
+// without changing any result.
 // ---------------------------------------------------------------------------
 struct TravState {
     RayPre r;
@@ -263,8 +266,8 @@ CRH_D bool travInit(const SceneView& s, const Ray& ray, TravState& t) {
     return !boxHit(s.rootMin, s.rootMax, t.r, dummy);
 }
 
-// One iteration of traverse()'s loop (one node or one triangle); returns
-// true when the ray is done.
+// One step of traverse()'s work (one internal node, or one triangle of a
+// leaf); returns true when the ray is done.
 template <bool ANY, bool COUNT, class Stack>
 CRH_D bool travStep(const SceneView& s, TravState& t, Stack& stack, WorkCount& wc) {
     const RayPre& r = t.r;
