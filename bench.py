@@ -55,6 +55,11 @@ def parse():
                     help="independent runs per method in the guided-vs-plain comparison")
     ap.add_argument("--batch-scenes", type=int, default=64,
                     help="scenes of the many-scene batch leg (0 = skip)")
+    ap.add_argument("--terrain", action="store_true",
+                    help="add the large-scale leg: the 43.8M-triangle terrain, four "
+                         "3840x2160 views in one batched launch, and its closest-hit batch")
+    ap.add_argument("--terrain-spp", type=int, default=2,
+                    help="timed spp per view of the terrain leg")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc runs that measure HBM traffic")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -186,6 +191,78 @@ def batch_leg(cr, scenes, n: int) -> dict:
             "solo_sequential_ms": round(solo_ms, 3),
             "tuning": {k: tuned[k] for k in ("batch_tile_w", "batch_grid", "batch_chunks",
                                               "batch_stack", "batch_waves")}}
+
+
+def terrain_leg(cr, scenes, spp: int, trace_rays: int) -> dict:
+    """The large-scale workload: the fBm terrain (4096 x 4096 vertices,
+    33,538,050 triangles) with 2,000 subdivision-4 icospheres (43,778,066
+    triangles with the sun disk), four 3840 x 2160 camera views rendered in
+    one persistent launch per sample index (the scene held once in the pool),
+    max depth 24, MIS.  One warm-up spp, then `spp` timed spp of every view.
+    Also the closest-hit batch of `trace_rays` mixed rays from the first
+    view, at the shape the autotuner picks for it."""
+    from craystal_amd import autotune
+    from craystal_amd.batch import Batch
+    t0 = time.perf_counter()
+    data, cams = scenes.terrain_scene()
+    gen_s = time.perf_counter() - t0
+    progress(f"terrain generated: {data.triangle_count} triangles in {gen_s:.1f} s")
+    t0 = time.perf_counter()
+    scene = cr.Scene(data)
+    build_s = time.perf_counter() - t0
+    bst = scene.bvh_stats()
+    progress(f"terrain scene built in {build_s:.1f} s: {bst}")
+    batch = Batch([scene] * len(cams))
+    info = batch.info()
+    sensors = [cr.Sensor(c["size"]) for c in cams]
+    cam_objs = [scenes.make_camera(c, s) for c, s in zip(cams, sensors)]
+    cfg = cr.Configs(maxDepth=cams[0]["max_depth"], rrThreshold=0.05, useMIS=True,
+                     rrStartDepth=cams[0]["rr_start_depth"])
+    batch.render(cam_objs, [1] * len(cams), cfg, first_sample=0, readback=False)
+    t0 = time.perf_counter()
+    st = batch.render(cam_objs, [spp] * len(cams), cfg, first_sample=1, readback=False)
+    el = time.perf_counter() - t0
+    W, H = cams[0]["size"]
+    samples = W * H * spp * len(cams)
+    out = {"scene": "fBm terrain 4096^2 vertices (8 octaves, 50 m over 1 km) + 2,000 "
+                    "icospheres s4 + sun disk + sky",
+           "triangles": int(data.triangle_count), "views": len(cams), "width": W, "height": H,
+           "max_depth": cams[0]["max_depth"], "spp_timed": spp,
+           "generate_s": round(gen_s, 2), "scene_create_s": round(build_s, 2),
+           "bvh": {"nodes": bst["node_count"], "max_depth": bst["max_depth"],
+                   "max_leaf": bst["max_leaf"], "build_s": round(bst["build_ms"] / 1e3, 3)},
+           "pool_mb": round(info["pool_bytes"] / 2**20, 1),
+           "kernel_ms": round(st["kernel_ms"], 3), "wall_ms": round(el * 1e3, 3),
+           "msamples_per_s": round(samples / el / 1e6, 2),
+           "grays_per_s": round(st["rays"] / el / 1e9, 4),
+           "rays_per_sample": round(st["rays"] / samples, 4)}
+    progress(f"terrain render: {out['msamples_per_s']} Msamples/s")
+    del sensors, cam_objs, batch
+    if trace_rays > 0:
+        n = trace_rays
+        rays = scenes.mixed_batch(data, cams[0], n)
+        rb, hb = cr.DeviceBuffer(rays.nbytes), cr.DeviceBuffer(n * 16)
+        rb.copyFromHost(rays)
+        tt = autotune.tune_trace(scene, rays, reps=1, stacks=(2, 3), blocks=(256,),
+                                 persistent=((2, 0), (1, 16)), pt_grids=(2048, 4096))
+        cr.trace_closest_device(scene, rb, n, hb)
+        cnt = cr.trace_closest_device(scene, rb, n, hb, count_work=True)
+        kms = float(np.mean([cr.trace_closest_device(scene, rb, n, hb)["kernel_ms"]
+                             for _ in range(3)]))
+        b_ray = B_RAY_FIXED + cnt["node_visits"] / n * B_NODE + cnt["tri_tests"] / n * B_TRI
+        ach = b_ray * n / (kms / 1e3) / 1e9
+        out["closest_hit"] = {
+            "rays": n, "grays_per_s": round(n / (kms / 1e3) / 1e9, 4),
+            "kernel_ms": round(kms, 3),
+            "node_visits_per_ray": round(cnt["node_visits"] / n, 3),
+            "tri_tests_per_ray": round(cnt["tri_tests"] / n, 3),
+            "tuning": {k: tt[k] for k in ("stack", "trace_block", "persistent", "refill",
+                                          "pt_grid")},
+            "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
+                         "bytes_per_ray": round(b_ray, 1)}}
+        progress(f"terrain closest hit: {out['closest_hit']['grays_per_s']} Grays/s")
+    return out
 
 
 def guided_leg(cr, scenes, seconds: float, repeats: int) -> dict:
@@ -514,6 +591,8 @@ def main():
         progress(f"batch: {out['batch']['msamples_per_s']} Msamples/s")
     if a.guided_seconds > 0 and a.guided_runs > 0:
         out["guided"] = guided_leg(cr, scenes, a.guided_seconds, a.guided_runs)
+    if a.terrain:
+        out["terrain"] = terrain_leg(cr, scenes, a.terrain_spp, a.trace_rays)
 
     if not a.no_cpu:
         import oracle

@@ -20,6 +20,7 @@
 #include <exception>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/crayhip.h"
@@ -672,6 +673,7 @@ int crh_scene_create(crh_context* ctx, const crh_scene_desc* d,
         }
         primMat.insert(primMat.end(), mesh.nindex / 3, mesh.material_id);
     }
+    ARG_CHECK(idx.size() / 3 < crh::kSphereBit, "too many triangles (at most 2^30 - 1)");
     const uint32_t ntri = uint32_t(idx.size() / 3);
     for (uint32_t i = 0; i < d->nemissive; ++i)
         ARG_CHECK(d->emissive_index[i] < ntri, "emissive index out of range");
@@ -1493,27 +1495,35 @@ int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh
     GUARD_BEGIN
     HIP_TRY(hipSetDevice(ctx->device));
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
-    // Region r of the pool holds member r of every scene, scene after scene.
+    // Region r of the pool holds member r of every distinct scene, scene
+    // after scene; a scene listed again (several views of one scene) shares
+    // the slices of its first listing.
     DevMem crh_scene::*members[] = {
         &crh_scene::nodes,    &crh_scene::tris,    &crh_scene::pos,     &crh_scene::nrm,
         &crh_scene::uv,       &crh_scene::idx,     &crh_scene::primMat, &crh_scene::mats,
         &crh_scene::emissive, &crh_scene::spheres, &crh_scene::sphereMat,
         &crh_scene::texels,   &crh_scene::texInfo};
     constexpr int kR = 13;
+    std::vector<uint32_t> first(n);
+    std::unordered_map<const crh_scene*, uint32_t> seen;
+    for (uint32_t i = 0; i < n; ++i) first[i] = seen.emplace(scenes[i], i).first->second;
     std::vector<size_t> off(size_t(n) * kR);
     size_t total = 0;
     for (int r = 0; r < kR; ++r)
         for (uint32_t i = 0; i < n; ++i) {
+            if (first[i] != i) continue;
             off[size_t(i) * kR + r] = total;
             total += align((scenes[i]->*members[r]).bytes);
         }
+    for (uint32_t i = 0; i < n; ++i)
+        for (int r = 0; r < kR; ++r) off[size_t(i) * kR + r] = off[size_t(first[i]) * kR + r];
     auto* b = new crh_batch();
     b->ctx.set(ctx);
     hipError_t e = b->pool.alloc(std::max<size_t>(total, 256));
     if (e == hipSuccess) e = b->next.alloc(crh::kBatchMaxChunks * sizeof(uint32_t));
     char* base = b->pool.as<char>();
     for (uint32_t i = 0; i < n && e == hipSuccess; ++i)
-        for (int r = 0; r < kR && e == hipSuccess; ++r) {
+        for (int r = 0; r < kR && e == hipSuccess && first[i] == i; ++r) {
             const DevMem& src = scenes[i]->*members[r];
             if (src.bytes)
                 e = hipMemcpyAsync(base + off[size_t(i) * kR + r], src.p, src.bytes,
@@ -1544,8 +1554,10 @@ int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh
         v.texInfo = reinterpret_cast<const uint32_t*>(at(12));
         v.spec = nullptr;  // batched rendering transports RGB
         b->views[i] = v;
-        b->nodes += sc->nodes.bytes / sizeof(crh::GNode);
-        b->tris += sc->tris.bytes / sizeof(crh::GTri);
+        if (first[i] == i) {
+            b->nodes += sc->nodes.bytes / sizeof(crh::GNode);
+            b->tris += sc->tris.bytes / sizeof(crh::GTri);
+        }
         b->maxStack = std::max(b->maxStack, v.stackCap);
     }
     *out = b;

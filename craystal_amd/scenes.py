@@ -139,36 +139,60 @@ def icosphere(center, radius, subdiv, material) -> MeshData:
     return MeshData(P, V.astype(np.float32), F, material)
 
 
-def heightfield(cells: int, extent: float, amplitude: float, seed: int, material,
-                octaves: int = 4) -> MeshData:
-    """A (cells x cells)-quad height field over [-extent/2, extent/2]^2 in xz
-    (2 cells^2 triangles): a sum of seeded sinusoid octaves, vertex normals
-    from central differences."""
-    rng = Stream(seed)
-    n = cells + 1
-    t = np.linspace(-extent / 2, extent / 2, n)
-    X, Z = np.meshgrid(t, t, indexing="xy")
-    Y = np.zeros_like(X)
-    dX, dZ = np.zeros_like(X), np.zeros_like(X)
+def _octave_params(rng: "Stream", octaves: int, extent: float, amplitude: float):
+    """Per octave (frequency, amplitude, two phases, direction angle): the
+    frequency doubles and the amplitude halves from octave to octave."""
+    out = []
     for o in range(octaves):
         f = (2.0 ** o) * 2.0 * math.pi / extent * rng.uniform(0.8, 1.6)
         a = amplitude / (2.0 ** o)
         ph1, ph2, ang = rng.uniform(0, 6.283), rng.uniform(0, 6.283), rng.uniform(0, 3.1416)
+        out.append((f, a, ph1, ph2, ang))
+    return out
+
+
+def _height(params, X, Z):
+    """Height and its x / z derivatives of the octave sum at (X, Z) (float64)."""
+    Y = np.zeros_like(X)
+    dX, dZ = np.zeros_like(X), np.zeros_like(X)
+    for f, a, ph1, ph2, ang in params:
         u = math.cos(ang) * X + math.sin(ang) * Z
         v = -math.sin(ang) * X + math.cos(ang) * Z
-        Y += a * np.sin(f * u + ph1) * np.cos(f * v + ph2)
-        du = a * f * np.cos(f * u + ph1) * np.cos(f * v + ph2)
-        dv = -a * f * np.sin(f * u + ph1) * np.sin(f * v + ph2)
+        su, cu = np.sin(f * u + ph1), np.cos(f * u + ph1)
+        sv, cv = np.sin(f * v + ph2), np.cos(f * v + ph2)
+        Y += a * su * cv
+        du = a * f * cu * cv
+        dv = -a * f * su * sv
         dX += du * math.cos(ang) - dv * math.sin(ang)
         dZ += du * math.sin(ang) + dv * math.cos(ang)
+    return Y, dX, dZ
+
+
+def heightfield(cells: int, extent: float, amplitude: float, seed: int, material,
+                octaves: int = 4, params=None) -> MeshData:
+    """A (cells x cells)-quad height field over [-extent/2, extent/2]^2 in xz
+    (2 cells^2 triangles): a sum of seeded sinusoid octaves (fBm-style:
+    frequency x2, amplitude /2 per octave), vertex normals from the analytic
+    derivatives.  params: octave parameters from _octave_params (drawn from
+    `seed` when None)."""
+    if params is None:
+        params = _octave_params(Stream(seed), octaves, extent, amplitude)
+    n = cells + 1
+    t = np.linspace(-extent / 2, extent / 2, n)
+    X, Z = np.meshgrid(t, t, indexing="xy")
+    Y, dX, dZ = _height(params, X, Z)
     P = np.stack([X, Y, Z], -1).reshape(-1, 3).astype(np.float32)
-    N = np.stack([-dX, np.ones_like(X), -dZ], -1).reshape(-1, 3)
-    N = (N / np.linalg.norm(N, axis=1, keepdims=True)).astype(np.float32)
-    i, j = np.meshgrid(np.arange(cells), np.arange(cells), indexing="xy")
-    a = (j * n + i).reshape(-1)
-    b, c, d = a + 1, a + n + 1, a + n
+    del X, Z, Y
+    N = np.stack([-dX, np.ones_like(dX), -dZ], -1).reshape(-1, 3)
+    del dX, dZ
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    N = N.astype(np.float32)
+    i, j = np.meshgrid(np.arange(cells, dtype=np.uint32), np.arange(cells, dtype=np.uint32),
+                       indexing="xy")
+    a = (j * np.uint32(n) + i).reshape(-1)
+    b, c, d = a + 1, a + np.uint32(n + 1), a + np.uint32(n)
     # counter-clockwise seen from +y: (a, d, c), (a, c, b)
-    I = np.stack([a, d, c, a, c, b], -1).reshape(-1).astype(np.uint32)
+    I = np.stack([a, d, c, a, c, b], -1).reshape(-1)
     return MeshData(P, N, I, material)
 
 
@@ -443,6 +467,65 @@ def batch_scene(index: int, seed: int = SCENE_SEED, tri_range=(1_000, 500_000),
                up=(0.0, 1.0, 0.0), fov=50.0, size=(res, res), spp=spp, max_depth=max_depth,
                rr_start_depth=3)
     return data, cam
+
+
+def disk_emitter(center, normal, radius: float, segments: int, material) -> MeshData:
+    """A flat disk as a triangle fan around its centre, facing `normal`
+    (counter-clockwise seen from that side; every vertex normal = normal)."""
+    nrm = _normalize(np.asarray(normal, np.float64))
+    u = _normalize(np.cross(nrm, (0.0, 1.0, 0.0) if abs(nrm[1]) < 0.9 else (1.0, 0.0, 0.0)))
+    v = np.cross(nrm, u)
+    ang = np.arange(segments) * (2.0 * math.pi / segments)
+    rim = (np.asarray(center, np.float64)
+           + radius * (np.cos(ang)[:, None] * u + np.sin(ang)[:, None] * v))
+    P = np.vstack([np.asarray(center, np.float64)[None], rim]).astype(np.float32)
+    N = np.repeat(nrm[None].astype(np.float32), segments + 1, axis=0)
+    k = np.arange(segments, dtype=np.uint32)
+    I = np.stack([np.zeros_like(k), 1 + k, 1 + (k + 1) % segments], -1).reshape(-1)
+    return MeshData(P, N, I, material)
+
+
+def terrain_scene(cells: int = 4095, extent: float = 1000.0, amplitude: float = 50.0,
+                  octaves: int = 8, n_spheres: int = 2000, subdiv: int = 4,
+                  size=(3840, 2160), seed: int = SCENE_SEED):
+    """The large-scale workload: an fBm height-field terrain ((cells + 1)^2
+    vertices, 2 cells^2 triangles; 4095 cells = 33,538,050 triangles, 8
+    octaves, amplitude 50 m over 1 km) with n_spheres icospheres (subdivision
+    4: 5,120 triangles each) resting on it, glossy Principled materials
+    (roughness U[0.02, 0.5]), a small sun-like disk emitter and a constant
+    sky.  Four camera views around the centre, 3840 x 2160, 256 spp, max
+    depth 24.  Returns (SceneData, [camera kwargs] x 4).  Smaller `cells`,
+    `n_spheres`, `subdiv` and `size` give the same layout at test sizes."""
+    rng = Stream(seed * 131 + 77)
+
+    def glossy(base, metal=0.0):
+        mats.append(principled(base, rng.uniform(0.02, 0.5), metal))
+        return len(mats) - 1
+
+    mats = [emitter((2400.0, 2250.0, 2000.0))]
+    params = _octave_params(rng, octaves, extent, amplitude)
+    meshes = [heightfield(cells, extent, amplitude, 0, glossy((0.32, 0.28, 0.2)), params=params)]
+    sun_dir = _normalize(np.array([0.45, 0.62, -0.64]))
+    meshes.append(disk_emitter(sun_dir * 0.7 * extent, -sun_dir, 0.02 * extent, 16, 0))
+    for _ in range(n_spheres):
+        r = rng.uniform(0.002, 0.008) * extent
+        x, z = rng.uniform(-0.45, 0.45) * extent, rng.uniform(-0.45, 0.45) * extent
+        y = float(_height(params, np.array([x]), np.array([z]))[0][0]) + 0.7 * r
+        base = (rng.uniform(), rng.uniform(), rng.uniform())
+        metal = 1.0 if rng.uniform() < 0.3 else 0.0
+        meshes.append(icosphere((x, y, z), r, subdiv, glossy(base, metal)))
+    data = SceneData(meshes=meshes, materials=mats, envRadiance=(0.35, 0.45, 0.65))
+    data.emissiveIndex = _emissive_prims(data)
+    cams = []
+    for k in range(4):
+        a = 0.6 + k * math.pi / 2
+        px, pz = 0.3 * extent * math.cos(a), 0.3 * extent * math.sin(a)
+        tx, tz = 0.05 * extent * math.cos(a + 2.0), 0.05 * extent * math.sin(a + 2.0)
+        py = float(_height(params, np.array([px]), np.array([pz]))[0][0]) + 0.06 * extent
+        ty = float(_height(params, np.array([tx]), np.array([tz]))[0][0])
+        cams.append(dict(pos=(px, py, pz), target=(tx, ty, tz), up=(0.0, 1.0, 0.0), fov=50.0,
+                         size=tuple(size), spp=256, max_depth=24, rr_start_depth=3))
+    return data, cams
 
 
 def lampshade_room(light_radiance: float = 40.0):

@@ -465,7 +465,9 @@ typedef struct crh_batch_info {
 } crh_batch_info;
 /* Concatenates the scenes' device data (BVH nodes, triangles, vertices,
  * materials, emitter lists) into one pool; each scene's BVH keeps its own
- * indices, relative to its slice.  The scenes may be destroyed afterwards. */
+ * indices, relative to its slice.  A scene listed more than once (several
+ * camera views of one scene) is stored once and its entries share it; the
+ * info counts it once.  The scenes may be destroyed afterwards. */
 int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh_batch** out);
 int crh_batch_get_info(const crh_batch* b, crh_batch_info* info);
 /* Renders spp[i] samples (indices first_sample, first_sample + 1, ...) of
