@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -345,7 +346,16 @@ GpuBvh toGpuLayout(const BvhResult& b, const float* pos, const uint32_t* idx) {
         }
         o.left = ref(r.a);
         o.right = ref(r.b);
-        o.pad0 = o.pad1 = 0;
+        // the same single-precision test the slab code makes per axis
+        auto flat = [](const crh_bvh_node_ref& c) {
+            for (int k = 0; k < 3; ++k) {
+                const float ext = c.pmax[k] - c.pmin[k];
+                if (std::fabs(ext) < FLT_EPSILON) return 1u;
+            }
+            return 0u;
+        };
+        o.flat = flat(L) | (flat(R) << 1);
+        o.pad1 = 0;
     }
     g.rootRef = ref(b.root);
     return g;

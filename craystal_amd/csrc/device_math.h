@@ -110,11 +110,16 @@ struct Ray {
 };
 
 // Per-ray constants for the slab test: invD = 1/d is what the reference
-// recomputes per box (Math/AABB.cu:23); dflat marks |d| < FLT_EPSILON.
+// recomputes per box (Math/AABB.cu:23).
 struct RayPre {
     F3 o, d, inv;
     float tMin, tMax;
 };
+// Boxes a node's box test must take through boxHit: the node's flat-box bits,
+// or both when the ray's tMin or tMax is NaN (see boxHitFast).
+CRH_D uint32_t slowBoxes(uint32_t flat, const RayPre& r) {
+    return flat | (__builtin_isunordered(r.tMin, r.tMax) ? 3u : 0u);
+}
 CRH_D RayPre prep(const Ray& r) {
     RayPre p;
     p.o = r.o;
@@ -157,6 +162,31 @@ CRH_D bool boxHit(const float* bmin, const float* bmax, const RayPre& r,
     ok &= slabAxis(bmin[1], bmax[1], r.o.y, r.d.y, r.inv.y, tMin, tMax);
     ok &= slabAxis(bmin[2], bmax[2], r.o.z, r.d.z, r.inv.z, tMin, tMax);
     if (!ok || tMin > tMax) return false;
+    if (tMin > r.tMax || tMax < r.tMin) return false;
+    hitT = tMin;
+    return true;
+}
+
+// boxHit for a box with no flat axis (no |max - min| < FLT_EPSILON; the
+// builder marks the boxes that have one) and a ray whose tMin and tMax are
+// not NaN.  fmaxf / fminf (v_max_f32 / v_min_f32) stand for the reference's
+// `t0 > tMin ? t0 : tMin` selects: while the accumulator is not NaN both
+// give the other operand for a NaN t0 / t1 (0 * inf on a slab plane) and the
+// larger / smaller one otherwise; they can differ only in the sign of a zero
+// tie, which none of the comparisons the traversal makes can see.
+CRH_D bool boxHitFast(const float* bmin, const float* bmax, const RayPre& r, float& hitT) {
+    const float o[3] = {r.o.x, r.o.y, r.o.z};
+    const float inv[3] = {r.inv.x, r.inv.y, r.inv.z};
+    float tMin = r.tMin, tMax = r.tMax;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float t0 = (bmin[k] - o[k]) * inv[k];
+        float t1 = (bmax[k] - o[k]) * inv[k];
+        const bool sw = inv[k] < 0.0f;
+        tMin = fmaxf(tMin, sw ? t1 : t0);
+        tMax = fminf(tMax, sw ? t0 : t1);
+    }
+    if (tMin > tMax) return false;
     if (tMin > r.tMax || tMax < r.tMin) return false;
     hitT = tMin;
     return true;

@@ -30,7 +30,9 @@ struct alignas(16) GNode {
     float rmax[3];
     uint32_t left;
     uint32_t right;
-    uint32_t pad0;
+    // bit 0: the left box has an axis with |max - min| < FLT_EPSILON (the
+    // reference's flat-box case, Math/AABB.cu:10-21); bit 1: the right box.
+    uint32_t flat;
     uint32_t pad1;
 };
 static_assert(sizeof(GNode) == 64, "GNode must be 64 B");

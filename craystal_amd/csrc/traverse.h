@@ -188,8 +188,9 @@ CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, Stack& stack,
             const float rmin[3] = {b.z, b.w, c.x};
             const float rmax[3] = {c.y, c.z, c.w};
             float lt = __builtin_inff(), rt = __builtin_inff();
-            bool hl = boxHit(lmin, lmax, r, lt);
-            bool hr = boxHit(rmin, rmax, r, rt);
+            const uint32_t fl = slowBoxes(d.z, r);
+            bool hl = (fl & 1u) ? boxHit(lmin, lmax, r, lt) : boxHitFast(lmin, lmax, r, lt);
+            bool hr = (fl & 2u) ? boxHit(rmin, rmax, r, rt) : boxHitFast(rmin, rmax, r, rt);
             if (hl && hr) {
                 // Reference pushes the far child first, then the near one
                 // (Core/BVH.cu:60-72); near = right iff lt > rt.
@@ -305,8 +306,9 @@ CRH_D bool travStep(const SceneView& s, TravState& t, Stack& stack, WorkCount& w
         const float rmin[3] = {b.z, b.w, c.x};
         const float rmax[3] = {c.y, c.z, c.w};
         float lt = __builtin_inff(), rt = __builtin_inff();
-        bool hl = boxHit(lmin, lmax, r, lt);
-        bool hr = boxHit(rmin, rmax, r, rt);
+        const uint32_t fl = slowBoxes(d.z, r);
+        bool hl = (fl & 1u) ? boxHit(lmin, lmax, r, lt) : boxHitFast(lmin, lmax, r, lt);
+        bool hr = (fl & 2u) ? boxHit(rmin, rmax, r, rt) : boxHitFast(rmin, rmax, r, rt);
         if (hl && hr) {
             bool rightNear = lt > rt;
             uint32_t nearRef = rightNear ? d.y : d.x;
