@@ -11,8 +11,10 @@ scene and film already resident in HBM.
 Also reported: Grays/s of the render (extension + shadow + MIS rays counted by
 the kernel), the closest-hit throughput on a 16,777,216-ray seeded batch
 (8M primary + 8M secondary, device resident), the roofline of the dominant
-kernel from its HIP-event-timed launches, and the CPU oracle timed on the
-host cores on a bounded sample (one spp pass of the same image).
+kernel from its HIP-event-timed launches, the wavefront integrator, the
+64-scene batch, the guided-vs-plain comparison, the 43.8M-triangle terrain
+(four 3840 x 2160 views in one batched launch), and the CPU oracle timed on
+the host cores on a bounded sample (spp passes of the same image).
 """
 from __future__ import annotations
 
@@ -55,9 +57,9 @@ def parse():
                     help="independent runs per method in the guided-vs-plain comparison")
     ap.add_argument("--batch-scenes", type=int, default=64,
                     help="scenes of the many-scene batch leg (0 = skip)")
-    ap.add_argument("--terrain", action="store_true",
-                    help="add the large-scale leg: the 43.8M-triangle terrain, four "
-                         "3840x2160 views in one batched launch, and its closest-hit batch")
+    ap.add_argument("--no-terrain", action="store_true",
+                    help="skip the large-scale leg (the 43.8M-triangle terrain, four "
+                         "3840x2160 views in one batched launch, and its closest-hit batch)")
     ap.add_argument("--terrain-spp", type=int, default=2,
                     help="timed spp per view of the terrain leg")
     ap.add_argument("--no-traffic", action="store_true",
@@ -71,6 +73,10 @@ def parse():
     ap.add_argument("--profile-wavefront", action="store_true",
                     help="counter-collection mode for the wavefront: the spheres scene at "
                          "1920x1080, 8 spp (with --profile-tuning applied first)")
+    ap.add_argument("--profile-terrain", action="store_true",
+                    help="counter-collection mode for the terrain: one spp of its four "
+                         "3840x2160 views (batched, no graph) and a 16,777,216-ray closest-hit "
+                         "batch (with --profile-tuning applied first)")
     ap.add_argument("--profile-guided", action="store_true",
                     help="counter-collection mode for the guiding kernels: the occluded room "
                          "at 256x256, 16 spp of the guided wavefront with training, 16 without, "
@@ -149,6 +155,37 @@ def profile_kernels(cr, L, scenes, tuning: str | None = None):
     print(json.dumps({"copy_bytes": nbytes, "trace_rays": n,
                       "trace_node_visits": cnt["node_visits"], "trace_tri_tests": cnt["tri_tests"],
                       "render_samples": W * H, "render_rays": st["rays"]}), flush=True)
+
+
+def profile_terrain(cr, L, scenes, tuning: str | None = None):
+    """Minimal launches on the 43.8M-triangle terrain for rocprofv3: one spp
+    of the four 3840 x 2160 views in one batched launch sequence (graph
+    capture off), then one 16,777,216-ray closest-hit batch from the first
+    view with work counting and one without."""
+    from craystal_amd.batch import Batch
+    ctx = cr.get_context(0)
+    if tuning:
+        ctx.set_tuning(**json.loads(tuning))
+    ctx.set_tuning(batch_graph=2)
+    data, cams = scenes.terrain_scene()
+    scene = cr.Scene(data)
+    batch = Batch([scene] * len(cams))
+    sensors = [cr.Sensor(c["size"]) for c in cams]
+    cam_objs = [scenes.make_camera(c, s) for c, s in zip(cams, sensors)]
+    cfg = cr.Configs(maxDepth=cams[0]["max_depth"], rrThreshold=0.05, useMIS=True,
+                     rrStartDepth=cams[0]["rr_start_depth"])
+    st = batch.render(cam_objs, [1] * len(cams), cfg, first_sample=1, readback=False)
+    n = 1 << 24
+    rays = scenes.mixed_batch(data, cams[0], n)
+    rb, hb = cr.DeviceBuffer(rays.nbytes), cr.DeviceBuffer(n * 16)
+    rb.copyFromHost(rays)
+    cnt = cr.trace_closest_device(scene, rb, n, hb, count_work=True)
+    tr = cr.trace_closest_device(scene, rb, n, hb)
+    print(json.dumps({"render_samples": 4 * cams[0]["size"][0] * cams[0]["size"][1],
+                      "render_ms": round(st["kernel_ms"], 3), "render_rays": st["rays"],
+                      "trace_rays": n, "trace_ms": round(tr["kernel_ms"], 3),
+                      "trace_node_visits": cnt["node_visits"],
+                      "trace_tri_tests": cnt["tri_tests"]}), flush=True)
 
 
 def batch_leg(cr, scenes, n: int) -> dict:
@@ -411,6 +448,9 @@ def main():
     if a.profile_wavefront:
         profile_wavefront(cr, L, scenes, a.profile_tuning)
         return
+    if a.profile_terrain:
+        profile_terrain(cr, L, scenes, a.profile_tuning)
+        return
 
     data, camd = scenes.spheres_scene()
     W, H = camd["size"]
@@ -544,6 +584,7 @@ def main():
                            persistent=ttuned["persistent"])
         if ttuned["persistent"] == 1:
             prof_tuning.update(refill=ttuned["refill"], pt_grid=ttuned["pt_grid"])
+        progress(f"profile tuning: {json.dumps(prof_tuning)}")
         if not a.no_traffic:
             # HBM traffic of both kernels at the launch shapes chosen, from two
             # rocprofv3 child runs while this process is idle
@@ -591,7 +632,7 @@ def main():
         progress(f"batch: {out['batch']['msamples_per_s']} Msamples/s")
     if a.guided_seconds > 0 and a.guided_runs > 0:
         out["guided"] = guided_leg(cr, scenes, a.guided_seconds, a.guided_runs)
-    if a.terrain:
+    if not a.no_terrain:
         out["terrain"] = terrain_leg(cr, scenes, a.terrain_spp, a.trace_rays)
 
     if not a.no_cpu:
