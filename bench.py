@@ -653,6 +653,28 @@ def main():
                                          f"oracle, OpenMP over rows",
                                "seconds": round(cs, 3),
                                "grays_per_s": round(crays / cs / 1e9, 5)}
+        # one core: one spp pass of the same image
+        t0 = time.perf_counter()
+        _, _, r1 = orc_scene.render_samples(cam, (W, H), cfg, passes, threads=1)
+        c1 = time.perf_counter() - t0
+        out["cpu_baseline"]["one_core"] = {
+            "value": round(px / c1 / 1e6, 4), "unit": "Msamples/s", "seconds": round(c1, 3),
+            "sample": "one spp pass of the same 1920x1080 image on one thread",
+            "grays_per_s": round(r1 / c1 / 1e9, 5)}
+        if rays is not None:
+            # the closest-hit batch in full on all cores, and its first 2^20 rays on one
+            t0 = time.perf_counter()
+            orc_scene.trace_closest(rays)
+            ct = time.perf_counter() - t0
+            m = min(len(rays), 1 << 20)
+            t0 = time.perf_counter()
+            orc_scene.trace_closest(rays[:m], threads=1)
+            ct1 = time.perf_counter() - t0
+            out["cpu_baseline"]["closest_hit"] = {
+                "grays_per_s": round(len(rays) / ct / 1e9, 5), "seconds": round(ct, 3),
+                "cores": threads, "rays": len(rays),
+                "one_core_grays_per_s": round(m / ct1 / 1e9, 5), "one_core_rays": m}
+        progress(f"cpu baseline: {out['cpu_baseline']['value']} Msamples/s")
     print(json.dumps(out), flush=True)
 
 
