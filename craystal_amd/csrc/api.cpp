@@ -415,12 +415,12 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(t->persistent <= 2 && t->wf_persistent <= 2,
               "persistent must be 0 (keep), 1 (on) or 2 (off)");
     ARG_CHECK(t->refill <= 64 && t->wf_refill <= 64, "refill must be in [1, 64]");
-    ARG_CHECK(t->render_waves == 0 || t->render_waves == 1 || t->render_waves == 4 ||
-                  t->render_waves == 5,
-              "render_waves must be 0 (keep), 1, 4 or 5");
-    ARG_CHECK(t->batch_waves == 0 || t->batch_waves == 1 || t->batch_waves == 4 ||
-                  t->batch_waves == 5,
-              "batch_waves must be 0 (keep), 1, 4 or 5");
+    ARG_CHECK(t->render_waves == 0 || t->render_waves == 1 ||
+                  (t->render_waves >= 4 && t->render_waves <= 6),
+              "render_waves must be 0 (keep), 1, 4, 5 or 6");
+    ARG_CHECK(t->batch_waves == 0 || t->batch_waves == 1 ||
+                  (t->batch_waves >= 4 && t->batch_waves <= 6),
+              "batch_waves must be 0 (keep), 1, 4, 5 or 6");
     ARG_CHECK(t->render_regen <= 2, "render_regen must be 0 (keep), 1 (on) or 2 (off)");
     ARG_CHECK(t->render_grid <= (1u << 20), "render_grid too large");
     ARG_CHECK(t->wf_finish <= 65, "wf_finish must be 0 (keep), 1..64 (bounces) or 65 (off)");

@@ -1422,8 +1422,10 @@ void orc_trace_closest(void* sp, const crh_ray* rays, uint64_t n,
                        crh_hit* hits, uint32_t* node_visits,
                        uint32_t* tri_tests, int nthreads) {
     const Scene& s = *(Scene*)sp;
-    if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for schedule(dynamic, 1024)
+    // The thread count applies to this call only (num_threads clause), not to
+    // later calls.
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nt)
     for (int64_t i = 0; i < int64_t(n); ++i) {
         RayHit rh;
         rh.ray = toRay(rays[i]);
@@ -1438,8 +1440,10 @@ void orc_trace_closest(void* sp, const crh_ray* rays, uint64_t n,
 void orc_trace_any(void* sp, const crh_ray* rays, uint64_t n, uint8_t* occ,
                    int nthreads) {
     const Scene& s = *(Scene*)sp;
-    if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for schedule(dynamic, 1024)
+    // The thread count applies to this call only (num_threads clause), not to
+    // later calls.
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nt)
     for (int64_t i = 0; i < int64_t(n); ++i) {
         RayHit rh;
         rh.ray = toRay(rays[i]);
@@ -1501,9 +1505,9 @@ uint64_t orc_render_samples(void* sp, const crh_camera* cam, uint32_t W,
                             uint32_t H, const crh_integrator_cfg* cfg,
                             uint32_t si, float* L, float* pix, int nthreads) {
     const Scene& s = *(Scene*)sp;
-    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();  // this call only
     uint64_t rays = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : rays)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : rays) num_threads(nt)
     for (int64_t y = 0; y < int64_t(H); ++y) {
         Tally tl;
         for (uint32_t x = 0; x < W; ++x) {

@@ -66,8 +66,11 @@ def test_batch_equals_solo_renders(gpu):
     try:
         runs = [dict(batch_graph=1), dict(batch_graph=1),        # capture, then replay
                 dict(batch_graph=2, batch_tile_w=32, batch_chunks=1),
-                dict(batch_graph=1, batch_tile_w=8, batch_grid=7, batch_chunks=8, batch_stack=2),
-                dict(batch_graph=1, batch_tile_w=64, batch_grid=4096, batch_chunks=3)]
+                dict(batch_graph=1, batch_tile_w=8, batch_grid=7, batch_chunks=8, batch_stack=2,
+                     batch_waves=6),
+                dict(batch_graph=1, batch_tile_w=64, batch_grid=4096, batch_chunks=3,
+                     batch_waves=5),
+                dict(batch_graph=2, batch_waves=1)]
         for kw in runs:
             ctx.set_tuning(**kw)
             for s in sensors:

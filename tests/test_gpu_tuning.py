@@ -44,9 +44,11 @@ def test_render_tuning_invariance(gpu):
     ctx = cr.get_context(0)
     saved = ctx.get_tuning()
     try:
+        # every candidate (register budgets included) must give the same film, bit for bit
         res = autotune.tune_render(scene, cfg, spp=1, wf_grids=(64, 512), wf_blocks=(64, 256),
-                                   render_bands=(1, 2))
-        assert len(res["megakernel_ms"]) == 2 * 4   # stacks x (2 bands + 2 regeneration grids)
+                                   render_bands=(1, 2), render_waves=(1, 4, 5, 6))
+        # stacks x budgets x (2 bands + 2 regeneration grids)
+        assert len(res["megakernel_ms"]) == 2 * 4 * 4
         assert res["mode"] in (L.MODE_MEGAKERNEL, L.MODE_WAVEFRONT)
         assert len(res["wavefront_ms"]) == 16
         t = ctx.get_tuning()
@@ -63,3 +65,8 @@ def test_tuning_rejects_bad_values(gpu):
         ctx.set_tuning(stack=7)
     with pytest.raises(cr.RuntimeError):
         ctx.set_tuning(wf_refill=65)
+    for w in (2, 3, 7):
+        with pytest.raises(cr.RuntimeError):
+            ctx.set_tuning(render_waves=w)
+        with pytest.raises(cr.RuntimeError):
+            ctx.set_tuning(batch_waves=w)
