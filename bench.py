@@ -324,7 +324,7 @@ def guided_leg(cr, scenes, seconds: float, repeats: int) -> dict:
     sensor = cr.Sensor(size)
     scene.setCamera(scenes.make_camera(camd, sensor))
     px = size[0] * size[1]
-    tuned = autotune.tune_render(scene, cfg, spp=16, stacks=(2, 3), render_waves=(4,),
+    tuned = autotune.tune_render(scene, cfg, spp=16, stacks=(2, 3), render_waves=(4, 5, 6),
                                  render_regen=((2, 0), (1, 1024), (1, 2048)),
                                  wf_grids=(16384,), wf_blocks=(128,), wf_persistent=((2, 0),),
                                  wf_finishes=(65,))
@@ -344,7 +344,8 @@ def guided_leg(cr, scenes, seconds: float, repeats: int) -> dict:
         progress(f"guided leg run {r}: plain {runs['plain'][-1]['final_relmse']:.4f} "
                  f"guided {runs['guided'][-1]['final_relmse']:.4f}")
     out = {"scene": "occluded room, 171,388 tris, 256x256, max depth 16", "target_relmse": 0.01,
-           "plain_tuning": {k: tuned[k] for k in ("render_stack", "render_regen", "render_grid")},
+           "plain_tuning": {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen",
+                                                        "render_grid")},
            "seconds_per_run": seconds, "runs": repeats,
            "reference": "craystal_amd/data/room256_plain65536.npy (plain megakernel, 65,536 spp)"}
     for k, v in runs.items():
