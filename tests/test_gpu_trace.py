@@ -128,3 +128,43 @@ def test_persistent_refill_bit_identical(gpu, orc):
         _assert_hits_equal(pt, os_.trace_closest(rays))
     finally:
         ctx.set_tuning(**saved)
+
+
+def test_slab_fast_path_nan_and_plane_cases(gpu, orc):
+    """Boxes without a flat axis take the select-free slab test; it must give
+    the reference's hits and node/triangle counts when a slab value is NaN
+    (zero direction component with the origin on a box plane: 0 * inf), for
+    tMin = -0, and when tMin or tMax is NaN (those rays take the reference
+    path for every box)."""
+    mat = scenes.diffuse((0.5, 0.5, 0.5))
+    data = cr.SceneData(meshes=[scenes.icosphere((0, 0, 0), 1.0, 3, 0),
+                                scenes.icosphere((1.7, 0.3, -0.4), 0.6, 2, 0)], materials=[mat])
+    gs, os_ = _scene_pair(data, orc)
+    P = np.concatenate([m.position for m in data.meshes])
+    rng = np.random.default_rng(23)
+    n = 1 << 14
+    rays = np.zeros(n, L.RAY_DTYPE)
+    o = rng.uniform(-2.5, 3.0, (n, 3)).astype(np.float32)
+    pick = P[rng.integers(0, len(P), n)]
+    on_plane = rng.integers(0, 3, n)
+    o[np.arange(n), on_plane] = pick[np.arange(n), on_plane]     # origin on a vertex plane
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d[np.arange(n), on_plane] = 0.0                                # parallel to that plane
+    both = rng.uniform(size=n) < 0.3
+    d[both, (on_plane[both] + 1) % 3] = 0.0
+    rays["origin"] = o
+    rays["dir"] = d
+    kind = rng.integers(0, 6, n)
+    rays["tmin"] = np.select([kind == 1, kind == 2], [np.float32(-0.0), np.float32(np.nan)],
+                             np.float32(0.0))
+    rays["tmax"] = np.select([kind == 3, kind == 4], [np.float32(np.nan), np.float32(1.5)],
+                             np.float32(np.inf))
+    oh, nv, nt = os_.trace_closest(rays, counts=True)
+    _assert_hits_equal(gs.trace_closest(rays), oh)
+    assert np.array_equal(gs.trace_any(rays), os_.trace_any(rays))
+    rb, hb = cr.DeviceBuffer(rays.nbytes), cr.DeviceBuffer(n * 16)
+    rb.copyFromHost(rays)
+    cnt = cr.trace_closest_device(gs, rb, n, hb, count_work=True)
+    assert cnt["node_visits"] == int(nv.sum(dtype=np.uint64))
+    assert cnt["tri_tests"] == int(nt.sum(dtype=np.uint64))
+    assert (oh["prim"] != L.CRH_INVALID_PRIM).sum() > 1000
