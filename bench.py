@@ -128,7 +128,9 @@ def profile_guided(cr, L, scenes):
 
 
 def profile_kernels(cr, L, scenes, tuning: str | None = None):
-    """Minimal launches for rocprofv3 --pmc runs (FETCH_SIZE / WRITE_SIZE).
+    """Minimal launches for rocprofv3 --pmc runs (FETCH_SIZE / WRITE_SIZE): a
+    256 MiB copy, the closest-hit batch counted and at the tuned shape, one
+    megakernel launch (of as many sample indices as render_pool groups).
     tuning: JSON of crh_tuning fields to apply first (the launch shapes the
     timed run uses)."""
     if tuning:
@@ -151,10 +153,12 @@ def profile_kernels(cr, L, scenes, tuning: str | None = None):
     scene.setCamera(cam)
     cfg = cr.Configs(maxDepth=camd["max_depth"], rrThreshold=0.05, useMIS=True,
                      rrStartDepth=camd["rr_start_depth"])
-    st = cr.PathTraceIntegrator(cfg).dispatch(scene, 1, first_sample=0, readback=False)
+    # one megakernel launch of as many sample indices as the tuning renders together
+    spl = max(1, min(8, cr.get_context(0).get_tuning()["render_pool"] // (W * H)))
+    st = cr.PathTraceIntegrator(cfg).dispatch(scene, spl, first_sample=0, readback=False)
     print(json.dumps({"copy_bytes": nbytes, "trace_rays": n,
                       "trace_node_visits": cnt["node_visits"], "trace_tri_tests": cnt["tri_tests"],
-                      "render_samples": W * H, "render_rays": st["rays"]}), flush=True)
+                      "render_samples": spl * W * H, "render_rays": st["rays"]}), flush=True)
 
 
 def profile_terrain(cr, L, scenes, tuning: str | None = None):
@@ -474,13 +478,14 @@ def main():
                                  wf_persistent=((2, 0), (1, 16, 2048), (1, 16, 4096)),
                                  wf_pools=(1 << 23,), render_waves=(1, 4, 5, 6),
                                  render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
-                                 wf_finishes=(65, 1, 2, 3), render_bands=(1, 2))
+                                 wf_finishes=(65, 1, 2, 3), render_bands=(1, 2),
+                                 render_pools=(1 << 21, 1 << 22, 1 << 23))
     sensor.clear()
 
     progress(f"scene built, autotuned: {tuned['render_stack']=} {tuned['wf_pool']=}")
     # launch shapes the HBM-traffic child runs use (the timed ones)
     prof_tuning = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen",
-                                         "render_band")}
+                                         "render_band", "render_pool")}
     if tuned["render_regen"] == 1:
         prof_tuning["render_grid"] = tuned["render_grid"]
     # warmup (untimed)
@@ -511,11 +516,14 @@ def main():
     # dominant kernel: the megakernel launches, timed on their own stream
     st2 = L.crh_stats()
     cfg_c = cfg.to_c()
+    nsp = max(1, min(8, a.steps))
     L.call("crh_render", scene.handle, C.byref(cam.data), C.byref(cfg_c), a.warmup + a.steps,
-           max(1, min(8, a.steps)), sensor.handle, C.byref(st2))
-    launches = max(1, min(8, a.steps))
+           nsp, sensor.handle, C.byref(st2))
+    # megakernel launches = pass groups (stats count them plus one film update per spp)
+    launches = max(1, int(st2.launches) - nsp)
     avg_launch_s = st2.main_kernel_ms / launches / 1e3
-    achieved = b_sample * px / avg_launch_s / 1e9
+    samples_per_launch = nsp * px / launches
+    achieved = b_sample * samples_per_launch / avg_launch_s / 1e9
 
     out = {
         "metric": "Msamples/s + time-to-relMSE 0.01, guided vs plain, 1 MI355X; Grays/s",
@@ -547,6 +555,7 @@ def main():
                      "traffic": traffic["render"] if traffic else None,
                      "bytes_per_sample": round(b_sample, 1),
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                     "samples_per_launch": int(samples_per_launch),
                      "node_visits_per_sample": round(nodes_ps, 3),
                      "tri_tests_per_sample": round(tris_ps, 3)},
     }
@@ -560,12 +569,14 @@ def main():
     wst = wf.dispatch(scene, a.steps, first_sample=a.warmup, readback=False)
     wel = time.perf_counter() - t0
     out["tuning"] = {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen",
-                                           "render_grid", "render_band", "wf_stack", "wf_grid",
+                                           "render_grid", "render_band", "render_pool",
+                                           "wf_stack", "wf_grid",
                                            "wf_trace_block", "wf_trace_grid", "wf_persistent",
                                            "wf_refill",
                                            "wf_pool", "wf_finish")}
     out["tuning"]["wavefront_ms_per_spp"] = tuned["wavefront_ms"]
     out["tuning"]["megakernel_ms_per_spp"] = tuned["megakernel_ms"]
+    out["tuning"]["render_pool_ms_per_spp"] = tuned["render_pool_ms"]
     out["wavefront"] = {"msamples_per_s": round(samples / wel / 1e6, 3),
                         "ms_per_step": round(wel / a.steps * 1e3, 3),
                         "grays_per_s": round(wst["rays"] / wel / 1e9, 4)}

@@ -70,10 +70,13 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
                 stacks=(1, 2), wf_grids=(1024, 2048, 4096), wf_blocks=(128, 256),
                 wf_persistent=((2, 0), (1, 16)), wf_pools=(None,), render_waves=(1,),
                 render_regen=((2, 0), (1, 1024), (1, 2048)), wf_finishes=(65,),
-                render_bands=(1,), first_sample: int = 1 << 20) -> dict:
+                render_bands=(1,), render_pools=(None,),
+                first_sample: int = 1 << 20) -> dict:
     """Tune the megakernel stack placement, register budget and path
     regeneration ((mode, persistent grid) pairs, mode 2 = one path per
-    thread) and the wavefront stack placement,
+    thread), then, for the best of those, its paths per launch
+    (render_pools; None keeps the current value), and the wavefront stack
+    placement,
     grid/block sizes, persistence ((mode, refill) or (mode, refill, traversal
     grid) tuples; mode 1 = lanes refill, 2 = off), path pool (paths in
     flight; None keeps the current one) and finishing depth on the scene's current
@@ -107,6 +110,17 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
     best_mk = min(res_mk, key=res_mk.get)
     best_stack, best_waves, best_regen, best_grid, best_band = best_mk
     ctx.set_tuning(**mk_kw(*best_mk))
+    # paths per launch (sample indices rendered together) for the chosen shape
+    rpool0 = ctx.get_tuning()["render_pool"]
+    res_pool = {}
+    for p in [rpool0 if p is None else p for p in render_pools]:
+        ctx.set_tuning(render_pool=p)
+        run(cfg)
+        ms, img = run(cfg)
+        assert img.tobytes() == ref.tobytes(), "megakernel launch shape changed the image"
+        res_pool[p] = ms
+    best_pool = min(res_pool, key=res_pool.get)
+    ctx.set_tuning(render_pool=best_pool)
     wf_cfg = Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
     pool0 = ctx.get_tuning()["wf_pool"]
     pools = [pool0 if p is None else p for p in wf_pools]
@@ -126,10 +140,12 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
     best_wf = min(res_wf, key=res_wf.get)
     st, g, b, pm, rf, tg, pool, fin = best_wf
     ctx.set_tuning(**wf_kw(st, g, b, pm, rf, tg, pool, fin))
-    mk_ms, wf_ms = res_mk[best_mk], res_wf[best_wf]
+    mk_ms, wf_ms = min(res_mk[best_mk], res_pool[best_pool]), res_wf[best_wf]
     return {"mode": L.MODE_WAVEFRONT if wf_ms < mk_ms else L.MODE_MEGAKERNEL,
             "render_stack": best_stack, "render_waves": best_waves, "render_regen": best_regen,
-            "render_grid": best_grid, "render_band": best_band, "wf_stack": st, "wf_grid": g, "wf_trace_block": b,
+            "render_grid": best_grid, "render_band": best_band, "render_pool": best_pool,
+            "render_pool_ms": {str(k): round(v, 3) for k, v in res_pool.items()},
+            "wf_stack": st, "wf_grid": g, "wf_trace_block": b,
             "wf_persistent": pm, "wf_refill": rf, "wf_trace_grid": tg or g, "wf_pool": pool,
             "wf_finish": fin,
             "megakernel_ms": {f"stack={k[0]},waves={k[1]},regen={k[2]},grid={k[3]},band={k[4]}":
