@@ -473,13 +473,13 @@ def main():
     # launch shapes picked by the autotuner (outside the timed region; every
     # candidate must render the same image bit for bit)
     from craystal_amd import autotune
-    tuned = autotune.tune_render(scene, cfg, spp=8, stacks=(1, 2, 3), wf_grids=(16384,),
+    tuned = autotune.tune_render(scene, cfg, spp=16, stacks=(1, 2, 3), wf_grids=(16384,),
                                  wf_blocks=(128, 256),
                                  wf_persistent=((2, 0), (1, 16, 2048), (1, 16, 4096)),
                                  wf_pools=(1 << 23, 1 << 24), render_waves=(1, 4, 5, 6),
                                  render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
                                  wf_finishes=(65, 1, 2, 3), render_bands=(1, 2),
-                                 render_pools=(1 << 21, 1 << 22, 1 << 23, 1 << 24))
+                                 render_pools=(1 << 21, 1 << 23, 1 << 24, 1 << 25))
     sensor.clear()
 
     progress(f"scene built, autotuned: {tuned['render_stack']=} {tuned['wf_pool']=}")

@@ -33,7 +33,7 @@ struct Tuning {
     uint32_t wfPersistent = 0;
     uint32_t wfRefill = 16;
     uint32_t wfPool = 1u << 23;  // paths in flight (several samples per pixel for small images)
-    uint32_t renderPool = 1u << 21;  // megakernel: paths per launch, the same way
+    uint32_t renderPool = 1u << 23;  // megakernel: paths per launch, the same way
     uint32_t renderWaves = 4;        // megakernels: minimum waves per SIMD (1, 4, 5 or 6)
     uint32_t batchWaves = 4;         // batch megakernel: the same for kBatchRender
     uint32_t renderRegen = 1;        // megakernel: lanes take a new path when theirs ends
