@@ -212,7 +212,7 @@ def batch_leg(cr, scenes, n: int) -> dict:
     cfg = cr.Configs(maxDepth=12, rrThreshold=0.05, useMIS=True, rrStartDepth=3)
     tuned = autotune.tune_batch(batch, cams, [1] * n, cfg, tile_ws=(8, 16, 32),
                                 grids=(1024, 2048), chunks=(1, 8), stacks=(1, 3), waves=(4, 5, 6),
-                                reps=1)
+                                reps=1, slots=(1, 4, 16), slot_spp=spp)
     for s in sensors:
         s.clear()
     batch.render(cams, spp, cfg, first_sample=0, readback=False)     # captures the graph
@@ -231,7 +231,8 @@ def batch_leg(cr, scenes, n: int) -> dict:
             "grays_per_s": round(st["rays"] / (st["kernel_ms"] / 1e3) / 1e9, 4),
             "solo_sequential_ms": round(solo_ms, 3),
             "tuning": {k: tuned[k] for k in ("batch_tile_w", "batch_grid", "batch_chunks",
-                                              "batch_stack", "batch_waves")}}
+                                              "batch_stack", "batch_waves", "batch_slots")},
+            "slots_ms": tuned["slots_ms"]}
 
 
 def terrain_leg(cr, scenes, spp: int, trace_rays: int) -> dict:

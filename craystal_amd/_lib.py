@@ -108,7 +108,8 @@ class crh_tuning(C.Structure):
                                    "render_grid", "wf_finish", "wf_stack", "wf_trace_grid", "wf_trace_block",
                                    "wf_shade_grid", "wf_persistent", "wf_refill", "wf_pool",
                                    "batch_stack", "batch_tile_w", "batch_grid", "batch_chunks",
-                                   "batch_graph", "batch_waves", "render_band")]
+                                   "batch_graph", "batch_waves", "render_band",
+                                   "batch_slots")]
 
 
 class crh_batch_info(C.Structure):

@@ -155,11 +155,14 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
 
 def tune_batch(batch, cameras, spp, cfg: Configs, device: int = 0, tile_ws=(8, 16, 32, 64),
                grids=(512, 1024, 2048), chunks=(1, 8), stacks=(1, 2), waves=(4,),
-               reps: int = 2, first_sample: int = 1 << 20) -> dict:
+               reps: int = 2, first_sample: int = 1 << 20, slots=(None,),
+               slot_spp=None) -> dict:
     """Tune the batched renderer's tile shape, persistent grid, work ranges,
-    stack placement and register budget.  Every candidate renders the whole
-    batch; all films must come out bit-identical before the fastest choice
-    is applied."""
+    stack placement and register budget, then, for the best of those, the
+    sample indices per launch (slots; None keeps the current value), rendering
+    slot_spp samples per scene (default: spp).  Every candidate renders the
+    whole batch; all films of a stage must come out bit-identical before the
+    fastest choice is applied."""
     ctx = get_context(device)
     sensors = [c.getSensor() for c in cameras]
     ref, res = None, {}
@@ -184,7 +187,29 @@ def tune_batch(batch, cameras, spp, cfg: Configs, device: int = 0, tile_ws=(8, 1
     tw, g, ch, st, wv = best
     ctx.set_tuning(batch_tile_w=tw, batch_grid=g, batch_chunks=ch, batch_stack=st, batch_graph=1,
                    batch_waves=wv)
+    sl0 = ctx.get_tuning()["batch_slots"]
+    res_sl, ref2 = {}, None
+    for sl in [sl0 if x is None else x for x in slots]:
+        ctx.set_tuning(batch_slots=sl)
+        ms = []
+        for r in range(reps + 1):
+            for s in sensors:
+                s.clear()
+            out = batch.render(cameras, slot_spp or spp, cfg, first_sample=first_sample,
+                               readback=(r == 0))
+            if r == 0:
+                imgs = b"".join(s.data.tobytes() + s.weight.tobytes() for s in sensors)
+                if ref2 is None:
+                    ref2 = imgs
+                elif imgs != ref2:
+                    raise AssertionError(f"batch slots {sl} changed an image")
+            else:
+                ms.append(out["kernel_ms"])
+        res_sl[sl] = _median(ms)
+    best_sl = min(res_sl, key=res_sl.get)
+    ctx.set_tuning(batch_slots=best_sl)
     return {"batch_tile_w": tw, "batch_grid": g, "batch_chunks": ch, "batch_stack": st,
-            "batch_waves": wv,
+            "batch_waves": wv, "batch_slots": best_sl,
+            "slots_ms": {str(k): round(v, 3) for k, v in res_sl.items()},
             "timings_ms": {f"tile_w={k[0]},grid={k[1]},chunks={k[2]},stack={k[3]},waves={k[4]}":
                            round(v, 4) for k, v in res.items()}}

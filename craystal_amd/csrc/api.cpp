@@ -425,6 +425,7 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(t->render_grid <= (1u << 20), "render_grid too large");
     ARG_CHECK(t->wf_finish <= 65, "wf_finish must be 0 (keep), 1..64 (bounces) or 65 (off)");
     ARG_CHECK(t->render_band <= 2, "render_band must be 0 (keep), 1 (on) or 2 (off)");
+    ARG_CHECK(t->batch_slots <= 64, "batch_slots must be 0 (keep) or 1..64");
     ARG_CHECK(t->wf_pool <= (1u << 28) && t->render_pool <= (1u << 28),
               "wf_pool and render_pool must be at most 2^28 paths");
     ARG_CHECK(okStack(t->batch_stack),
@@ -461,6 +462,7 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     if (t->batch_grid) o.batchGrid = t->batch_grid;
     if (t->batch_chunks) o.batchChunks = t->batch_chunks;
     if (t->batch_graph) o.batchGraph = t->batch_graph == 1 ? 1u : 0u;
+    if (t->batch_slots) o.batchSlots = t->batch_slots;
     return CRH_OK;
 }
 
@@ -493,6 +495,7 @@ int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
     t->batch_grid = o.batchGrid;
     t->batch_chunks = o.batchChunks;
     t->batch_graph = o.batchGraph ? 1u : 2u;
+    t->batch_slots = o.batchSlots;
     return CRH_OK;
 }
 
@@ -1623,6 +1626,11 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
     HIP_TRY(hipSetDevice(ctx->device));
     crh::Tuning tun = ctx->tuning;
     if (b->maxStack > uint32_t(crh::kPrivStackLevels)) tun.batchPriv = 0;
+    const uint32_t maxSlots = std::max<uint32_t>(1u, tun.batchSlots);
+    // Per-sample buffers for up to maxSlots sample indices per launch (a
+    // scene takes part in launches of at most its own spp indices).
+    for (uint32_t i = 0; i < n; ++i)
+        HIP_TRY(ensureFilmSlots(films[i], std::max<uint32_t>(1u, std::min(maxSlots, spp[i]))));
     // Scenes by decreasing spp: those still sampling at pass k form a prefix.
     std::vector<uint32_t> order(n);
     for (uint32_t i = 0; i < n; ++i) order[i] = i;
@@ -1671,15 +1679,22 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
     const uint32_t* dTile = b->prefix.as<uint32_t>();
     const uint32_t* dPix = dTile + n + 1;
     auto* dCount = counter.as<unsigned long long>();
-    // The pass sequence; launched directly or recorded into a graph.
+    // The pass sequence; launched directly or recorded into a graph.  Each
+    // launch covers up to maxSlots sample indices, as long as the set of
+    // scenes still sampling stays the same.
     auto passes = [&](hipStream_t st, unsigned long long* cnt) -> hipError_t {
         uint32_t active = n;
-        for (uint32_t k = 0; k < maxSpp; ++k) {
+        for (uint32_t k = 0; k < maxSpp;) {
             while (active > 0 && spp[order[active - 1]] <= k) --active;
+            const uint32_t g = std::min(maxSlots, spp[order[active - 1]] - k);
+            if (uint64_t(tilePre[active]) * g >= (uint64_t(1) << 32))
+                return hipErrorInvalidValue;
             hipError_t e = crh::launchBatchPass(dTab, dTile, dPix, active, tilePre[active],
-                                                pixPre[active], iv, first_sample + k, b->maxStack,
-                                                tun, b->next.as<uint32_t>(), cnt, st);
+                                                pixPre[active], iv, first_sample + k, g,
+                                                b->maxStack, tun, b->next.as<uint32_t>(), cnt,
+                                                st);
             if (e != hipSuccess) return e;
+            k += g;
         }
         return hipSuccess;
     };
@@ -1690,7 +1705,7 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
         std::vector<uint64_t> sig = {n, first_sample, maxSpp, iv.maxDepth,
                                      uint64_t(__builtin_bit_cast(uint32_t, iv.rrThreshold)),
                                      iv.rrStartDepth, iv.useMIS, tun.batchPriv, tw,
-                                     tun.batchGrid, tun.batchChunks, tun.batchWaves};
+                                     tun.batchGrid, tun.batchChunks, tun.batchWaves, maxSlots};
         for (uint32_t j = 0; j < n; ++j) sig.push_back(spp[order[j]]);
         sig.insert(sig.end(), pre.begin(), pre.end());
         if (!b->exec || sig != b->sig) {
@@ -1728,7 +1743,13 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
         stats->main_kernel_ms = ms;
         stats->total_ms = nowMs() - t0;
         stats->rays = c[0];
-        stats->launches = tun.batchGraph ? 1 : uint64_t(maxSpp) * 3;
+        // per launch group: counter reset, megakernel, one gather per index
+        uint64_t groups = 0;
+        for (uint32_t k = 0, active = n; k < maxSpp; ++groups) {
+            while (active > 0 && spp[order[active - 1]] <= k) --active;
+            k += std::min(maxSlots, spp[order[active - 1]] - k);
+        }
+        stats->launches = tun.batchGraph ? 1 : 2 * groups + maxSpp;
     }
     return CRH_OK;
     GUARD_END

@@ -36,6 +36,7 @@ struct Tuning {
     uint32_t renderPool = 1u << 23;  // megakernel: paths per launch, the same way
     uint32_t renderWaves = 4;        // megakernels: minimum waves per SIMD (1, 4, 5 or 6)
     uint32_t batchWaves = 4;         // batch megakernel: the same for kBatchRender
+    uint32_t batchSlots = 4;         // batch: sample indices per launch
     uint32_t renderRegen = 1;        // megakernel: lanes take a new path when theirs ends
     uint32_t renderGrid = 2048;      // megakernel with regeneration: persistent blocks
     uint32_t xcdBand = 0;            // megakernel: each XCD takes a contiguous band of tiles
@@ -86,13 +87,16 @@ hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
                             float2* P, unsigned long long* counters,
                             bool countWork, const Tuning& t, hipStream_t st,
                             bool spectral = false, uint32_t* next = nullptr);
-// One sample index over the first nActive scenes of the table: persistent
-// megakernel over `items` tiles, then the film gather over `pixels` pixels.
-// next: kBatchMaxChunks device counters.
+// Sample indices si .. si + slots - 1 over the first nActive scenes of the
+// table: one persistent megakernel launch over `items` tiles of each of the
+// `slots` sample indices (slot k of a scene's film buffers receives sample
+// si + k), then one film gather per sample index, in sample order, over
+// `pixels` pixels.  items * slots must fit in 32 bits.  next:
+// kBatchMaxChunks device counters.
 hipError_t launchBatchPass(const BatchScene* tab, const uint32_t* tilePrefix,
                            const uint32_t* pixPrefix, uint32_t nActive, uint32_t items,
                            uint32_t pixels, const IntegratorView& cfg, uint32_t si,
-                           uint32_t stackCap, const Tuning& t, uint32_t* next,
+                           uint32_t slots, uint32_t stackCap, const Tuning& t, uint32_t* next,
                            unsigned long long* counters, hipStream_t st);
 hipError_t launchFilmGather(const float4* L, const float2* P, uint32_t W,
                             uint32_t H, float* sum, float* wsum,

@@ -335,11 +335,13 @@ __global__ void kBatchReset(uint32_t* __restrict__ next) {
 template <bool MIS, int SM, int W>
 __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W, 8))) void kBatchRender(
     const BatchScene* __restrict__ tab, const uint32_t* __restrict__ tilePrefix,
-    uint32_t nActive, IntegratorView cfg, uint32_t si, uint32_t tileW, uint32_t chunks,
-    uint32_t* __restrict__ next, unsigned long long* __restrict__ counters) {
+    uint32_t nActive, IntegratorView cfg, uint32_t si, uint32_t slots, uint32_t tileW,
+    uint32_t chunks, uint32_t* __restrict__ next, unsigned long long* __restrict__ counters) {
     extern __shared__ uint2 lds[];
     __shared__ uint32_t taken;
-    const uint32_t T = tilePrefix[nActive];
+    // items: slot-major, slot k's tiles of every active scene, then slot k + 1's
+    const uint32_t tilesPerSlot = tilePrefix[nActive];
+    const uint32_t T = tilesPerSlot * slots;
     const uint32_t tileH = kRenderBlock / tileW;
     const uint32_t home = blockIdx.x % chunks;
     uint32_t rays = 0;
@@ -355,30 +357,36 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W,
             const uint32_t item = __builtin_amdgcn_readfirstlane(taken);
             __syncthreads();
             if (item >= hi) break;
-            const uint32_t j = __builtin_amdgcn_readfirstlane(findScene(tilePrefix, nActive, item));
+            const uint32_t slot = item / tilesPerSlot;
+            const uint32_t rem = item - slot * tilesPerSlot;
+            const uint32_t j = __builtin_amdgcn_readfirstlane(findScene(tilePrefix, nActive, rem));
             const BatchScene& b = tab[j];
-            const uint32_t tile = item - tilePrefix[j];
+            const uint32_t tile = rem - tilePrefix[j];
             const uint32_t x = (tile % b.tilesX) * tileW + threadIdx.x % tileW;
             const uint32_t y = (tile / b.tilesX) * tileH + threadIdx.x / tileW;
+            const size_t off = size_t(slot) * b.cam.width * b.cam.height;
             if (x < b.cam.width && y < b.cam.height)
-                tracePixel<MIS, false, false>(b.view, stk, b.cam, cfg, x, y, si,
-                                       reinterpret_cast<float4*>(b.L),
-                                       reinterpret_cast<float2*>(b.P), rays, wc);
+                tracePixel<MIS, false, false>(b.view, stk, b.cam, cfg, x, y, si + slot,
+                                       reinterpret_cast<float4*>(b.L) + off,
+                                       reinterpret_cast<float2*>(b.P) + off, rays, wc);
         }
     }
     flushCounters<false>(counters, rays, 0, 0);
 }
 
-// Film update of every active scene: pixPrefix[j] is scene j's first pixel.
+// Film update of every active scene with the samples of slot `slot`:
+// pixPrefix[j] is scene j's first pixel.
 __global__ __launch_bounds__(256) void kBatchGather(const BatchScene* __restrict__ tab,
                                                     const uint32_t* __restrict__ pixPrefix,
-                                                    uint32_t nActive) {
+                                                    uint32_t nActive, uint32_t slot) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= pixPrefix[nActive]) return;
     const uint32_t j = findScene(pixPrefix, nActive, i);
     const BatchScene& b = tab[j];
-    gatherPixel(reinterpret_cast<const float4*>(b.L), reinterpret_cast<const float2*>(b.P),
-                b.cam.width, b.cam.height, b.sum, b.wsum, i - pixPrefix[j]);
+    const size_t off = size_t(slot) * b.cam.width * b.cam.height;
+    gatherPixel(reinterpret_cast<const float4*>(b.L) + off,
+                reinterpret_cast<const float2*>(b.P) + off, b.cam.width, b.cam.height, b.sum,
+                b.wsum, i - pixPrefix[j]);
 }
 
 // ---------------------------------------------------------------------------
@@ -510,12 +518,14 @@ hipError_t launchRenderPass(const SceneView& s, const CameraView& cam,
 hipError_t launchBatchPass(const BatchScene* tab, const uint32_t* tilePrefix,
                            const uint32_t* pixPrefix, uint32_t nActive, uint32_t items,
                            uint32_t pixels, const IntegratorView& cfg, uint32_t si,
-                           uint32_t stackCap, const Tuning& t, uint32_t* next,
+                           uint32_t slots, uint32_t stackCap, const Tuning& t, uint32_t* next,
                            unsigned long long* counters, hipStream_t st) {
-    if (nActive == 0 || items == 0) return hipSuccess;
+    if (nActive == 0 || items == 0 || slots == 0) return hipSuccess;
+    if (uint64_t(items) * slots >= (uint64_t(1) << 32)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(kBatchReset, dim3(1), dim3(64), 0, st, next);
     const size_t lds = stackLdsBytes(t.batchPriv, stackCap, kRenderBlock);
-    const dim3 grid(items < t.batchGrid ? items : t.batchGrid), block(kRenderBlock);
+    const uint32_t total = items * slots;
+    const dim3 grid(total < t.batchGrid ? total : t.batchGrid), block(kRenderBlock);
     const uint32_t tw = t.batchTileW, ch = t.batchChunks;
     withStack(t.batchPriv, [&](auto sm) {
         constexpr int SM = decltype(sm)::value;
@@ -523,14 +533,15 @@ hipError_t launchBatchPass(const BatchScene* tab, const uint32_t* tilePrefix,
             constexpr int W = decltype(wv)::value;
             if (cfg.useMIS)
                 hipLaunchKernelGGL((kBatchRender<true, SM, W>), grid, block, lds, st, tab,
-                                   tilePrefix, nActive, cfg, si, tw, ch, next, counters);
+                                   tilePrefix, nActive, cfg, si, slots, tw, ch, next, counters);
             else
                 hipLaunchKernelGGL((kBatchRender<false, SM, W>), grid, block, lds, st, tab,
-                                   tilePrefix, nActive, cfg, si, tw, ch, next, counters);
+                                   tilePrefix, nActive, cfg, si, slots, tw, ch, next, counters);
         });
     });
-    hipLaunchKernelGGL(kBatchGather, dim3((pixels + 255u) / 256u), dim3(256), 0, st, tab,
-                       pixPrefix, nActive);
+    for (uint32_t k = 0; k < slots; ++k)
+        hipLaunchKernelGGL(kBatchGather, dim3((pixels + 255u) / 256u), dim3(256), 0, st, tab,
+                           pixPrefix, nActive, k);
     return hipGetLastError();
 }
 

@@ -285,6 +285,8 @@ typedef struct crh_tuning {
     uint32_t render_band;    /* megakernel (one path per thread): 1 = the blocks of each XCD
                                 take a contiguous band of tiles (its own L2), 2 = tiles in
                                 block order */
+    uint32_t batch_slots;    /* batched rendering: sample indices per launch, 1..64 (while the
+                                set of scenes still sampling stays the same) */
 } crh_tuning;
 
 /* ---- errors / device ------------------------------------------------- */
@@ -473,9 +475,10 @@ int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh
 int crh_batch_get_info(const crh_batch* b, crh_batch_info* info);
 /* Renders spp[i] samples (indices first_sample, first_sample + 1, ...) of
  * scene i through cams[i] into films[i] (distinct films), for every i, with
- * the megakernel integrator: per sample index one persistent launch over the
- * 256-pixel tiles of every scene that still takes samples, then one film
- * update over all of them.  With crh_tuning.batch_graph the pass sequence is
+ * the megakernel integrator: one persistent launch over the 256-pixel tiles of
+ * every scene that still takes samples, for up to crh_tuning.batch_slots
+ * sample indices at a time (while that set of scenes stays the same), then
+ * one film update per sample index over all of them.  With crh_tuning.batch_graph the pass sequence is
  * captured once into a hipGraph and replayed while pass counts, image sizes
  * and tuning stay the same.  Every film ends bit-identical to crh_render of
  * its scene alone with the same arguments. */

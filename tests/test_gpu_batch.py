@@ -66,11 +66,14 @@ def test_batch_equals_solo_renders(gpu):
     try:
         runs = [dict(batch_graph=1), dict(batch_graph=1),        # capture, then replay
                 dict(batch_graph=2, batch_tile_w=32, batch_chunks=1),
+                # several sample indices per launch (graph and direct launches)
+                dict(batch_graph=1, batch_slots=2), dict(batch_graph=2, batch_slots=3),
+                dict(batch_graph=1, batch_slots=64, batch_grid=5),
                 dict(batch_graph=1, batch_tile_w=8, batch_grid=7, batch_chunks=8, batch_stack=2,
                      batch_waves=6),
                 dict(batch_graph=1, batch_tile_w=64, batch_grid=4096, batch_chunks=3,
                      batch_waves=5),
-                dict(batch_graph=2, batch_waves=1)]
+                dict(batch_graph=2, batch_waves=1, batch_slots=1)]
         for kw in runs:
             ctx.set_tuning(**kw)
             for s in sensors:
@@ -123,10 +126,12 @@ def test_batch_autotuner_keeps_images(gpu):
     saved = ctx.get_tuning()
     try:
         res = autotune.tune_batch(batch, cams, spp, CFG, tile_ws=(8, 32), grids=(64, 1024),
-                                  chunks=(1, 8), stacks=(1, 2), reps=1)
-        assert len(res["timings_ms"]) == 16
+                                  chunks=(1, 8), stacks=(1, 2), reps=1, slots=(1, 2, 8),
+                                  slot_spp=[k + 3 for k in spp])
+        assert len(res["timings_ms"]) == 16 and len(res["slots_ms"]) == 3
         t = ctx.get_tuning()
         assert t["batch_tile_w"] == res["batch_tile_w"] and t["batch_grid"] == res["batch_grid"]
+        assert t["batch_slots"] == res["batch_slots"]
     finally:
         ctx.set_tuning(**saved)
 

@@ -65,6 +65,8 @@ def test_tuning_rejects_bad_values(gpu):
         ctx.set_tuning(stack=7)
     with pytest.raises(cr.RuntimeError):
         ctx.set_tuning(wf_refill=65)
+    with pytest.raises(cr.RuntimeError):
+        ctx.set_tuning(batch_slots=65)
     for w in (2, 3, 7):
         with pytest.raises(cr.RuntimeError):
             ctx.set_tuning(render_waves=w)
