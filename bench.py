@@ -212,7 +212,7 @@ def batch_leg(cr, scenes, n: int) -> dict:
     cfg = cr.Configs(maxDepth=12, rrThreshold=0.05, useMIS=True, rrStartDepth=3)
     tuned = autotune.tune_batch(batch, cams, [1] * n, cfg, tile_ws=(8, 16, 32),
                                 grids=(1024, 2048), chunks=(1, 8), stacks=(1, 3), waves=(4, 5, 6),
-                                reps=1, slots=(1, 4, 16), slot_spp=spp)
+                                reps=1, slots=(1, 4, 16, 64), slot_spp=spp)
     for s in sensors:
         s.clear()
     batch.render(cams, spp, cfg, first_sample=0, readback=False)     # captures the graph
@@ -477,7 +477,7 @@ def main():
     tuned = autotune.tune_render(scene, cfg, spp=16, stacks=(1, 2, 3), wf_grids=(16384,),
                                  wf_blocks=(128, 256),
                                  wf_persistent=((2, 0), (1, 16, 2048), (1, 16, 4096)),
-                                 wf_pools=(1 << 23, 1 << 24), render_waves=(1, 4, 5, 6),
+                                 wf_pools=(1 << 23, 1 << 24, 1 << 25), render_waves=(1, 4, 5, 6),
                                  render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
                                  wf_finishes=(65, 1, 2, 3), render_bands=(1, 2),
                                  render_pools=(1 << 21, 1 << 23, 1 << 24, 1 << 25))
