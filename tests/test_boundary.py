@@ -148,3 +148,18 @@ def test_scene_generators_are_deterministic():
     r2 = scenes.mixed_batch(d, cam, 1000)
     assert r1.tobytes() == r2.tobytes()
     assert np.allclose(np.linalg.norm(r1["dir"], axis=1), 1, atol=1e-5)
+
+
+def test_oracle_reproduces_committed_hit_fixtures(orc):
+    """The oracle's closest hits against the committed fixtures: the Cornell
+    batch whole, and every 256th ray of the 16,777,216-ray batch of the
+    983k-triangle scene (its full SHA-256 is checked on the GPU)."""
+    g = np.load(GOLD / "hits_cornell.npz")
+    data, _ = scenes.cornell_box()
+    assert orc.Scene(data).trace_closest(g["rays"]).tobytes() == g["hits"].tobytes()
+    meta = json.loads((GOLD / "hits_spheres1m.json").read_text())
+    data, cam = scenes.spheres_scene()
+    rays = scenes.mixed_batch(data, cam, meta["rays"])[::meta["sample_stride"]]
+    sample = np.load(GOLD / "hits_spheres1m_sample.npz")["hits"]
+    assert orc.Scene(data).trace_closest(rays).tobytes() == sample.tobytes()
+    assert 0.3 < (sample["prim"] != 0xFFFFFFFF).mean() < 0.4
