@@ -34,9 +34,9 @@ struct Tuning {
     uint32_t wfRefill = 16;
     uint32_t wfPool = 1u << 23;  // paths in flight (several samples per pixel for small images)
     uint32_t renderPool = 1u << 23;  // megakernel: paths per launch, the same way
-    uint32_t renderWaves = 4;        // megakernels: minimum waves per SIMD (1, 4, 5 or 6)
-    uint32_t batchWaves = 4;         // batch megakernel: the same for kBatchRender
-    uint32_t batchSlots = 4;         // batch: sample indices per launch
+    uint32_t renderWaves = 6;        // megakernels: minimum waves per SIMD (1, 4, 5 or 6)
+    uint32_t batchWaves = 6;         // batch megakernel: the same for kBatchRender
+    uint32_t batchSlots = 16;        // batch: sample indices per launch
     uint32_t renderRegen = 1;        // megakernel: lanes take a new path when theirs ends
     uint32_t renderGrid = 2048;      // megakernel with regeneration: persistent blocks
     uint32_t xcdBand = 0;            // megakernel: each XCD takes a contiguous band of tiles
