@@ -211,7 +211,7 @@ def batch_leg(cr, scenes, n: int) -> dict:
     spp = [c["spp"] for _, c in items]
     cfg = cr.Configs(maxDepth=12, rrThreshold=0.05, useMIS=True, rrStartDepth=3)
     tuned = autotune.tune_batch(batch, cams, [1] * n, cfg, tile_ws=(8, 16, 32),
-                                grids=(1024, 2048), chunks=(1, 8), stacks=(1, 3), waves=(4, 5, 6),
+                                grids=(1024, 2048), chunks=(1, 8), stacks=(1, 3), waves=(4, 6, 8),
                                 reps=1, slots=(1, 4, 16, 64), slot_spp=spp)
     for s in sensors:
         s.clear()
@@ -329,7 +329,7 @@ def guided_leg(cr, scenes, seconds: float, repeats: int) -> dict:
     sensor = cr.Sensor(size)
     scene.setCamera(scenes.make_camera(camd, sensor))
     px = size[0] * size[1]
-    tuned = autotune.tune_render(scene, cfg, spp=16, stacks=(2, 3), render_waves=(4, 5, 6),
+    tuned = autotune.tune_render(scene, cfg, spp=16, stacks=(2, 3), render_waves=(4, 6, 8),
                                  render_regen=((2, 0), (1, 1024), (1, 2048)),
                                  wf_grids=(16384,), wf_blocks=(128,), wf_persistent=((2, 0),),
                                  wf_finishes=(65,))
@@ -477,7 +477,7 @@ def main():
     tuned = autotune.tune_render(scene, cfg, spp=16, stacks=(1, 2, 3), wf_grids=(16384,),
                                  wf_blocks=(128, 256),
                                  wf_persistent=((2, 0), (1, 16, 2048), (1, 16, 4096)),
-                                 wf_pools=(1 << 23, 1 << 24, 1 << 25), render_waves=(1, 4, 5, 6),
+                                 wf_pools=(1 << 23, 1 << 24, 1 << 25), render_waves=(1, 4, 6, 8),
                                  render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
                                  wf_finishes=(65, 1, 2, 3), render_bands=(1, 2),
                                  render_pools=(1 << 21, 1 << 23, 1 << 24, 1 << 25))

@@ -416,11 +416,11 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
               "persistent must be 0 (keep), 1 (on) or 2 (off)");
     ARG_CHECK(t->refill <= 64 && t->wf_refill <= 64, "refill must be in [1, 64]");
     ARG_CHECK(t->render_waves == 0 || t->render_waves == 1 ||
-                  (t->render_waves >= 4 && t->render_waves <= 6),
-              "render_waves must be 0 (keep), 1, 4, 5 or 6");
+                  (t->render_waves >= 4 && t->render_waves <= 6) || t->render_waves == 8,
+              "render_waves must be 0 (keep), 1, 4, 5, 6 or 8");
     ARG_CHECK(t->batch_waves == 0 || t->batch_waves == 1 ||
-                  (t->batch_waves >= 4 && t->batch_waves <= 6),
-              "batch_waves must be 0 (keep), 1, 4, 5 or 6");
+                  (t->batch_waves >= 4 && t->batch_waves <= 6) || t->batch_waves == 8,
+              "batch_waves must be 0 (keep), 1, 4, 5, 6 or 8");
     ARG_CHECK(t->render_regen <= 2, "render_regen must be 0 (keep), 1 (on) or 2 (off)");
     ARG_CHECK(t->render_grid <= (1u << 20), "render_grid too large");
     ARG_CHECK(t->wf_finish <= 65, "wf_finish must be 0 (keep), 1..64 (bounces) or 65 (off)");

@@ -34,8 +34,8 @@ struct Tuning {
     uint32_t wfRefill = 16;
     uint32_t wfPool = 1u << 23;  // paths in flight (several samples per pixel for small images)
     uint32_t renderPool = 1u << 23;  // megakernel: paths per launch, the same way
-    uint32_t renderWaves = 6;        // megakernels: minimum waves per SIMD (1, 4, 5 or 6)
-    uint32_t batchWaves = 6;         // batch megakernel: the same for kBatchRender
+    uint32_t renderWaves = 8;        // megakernels: minimum waves per SIMD (1, 4, 5, 6 or 8)
+    uint32_t batchWaves = 8;         // batch megakernel: the same for kBatchRender
     uint32_t batchSlots = 16;        // batch: sample indices per launch
     uint32_t renderRegen = 1;        // megakernel: lanes take a new path when theirs ends
     uint32_t renderGrid = 2048;      // megakernel with regeneration: persistent blocks
@@ -61,12 +61,14 @@ inline void withStack(uint32_t mode, F&& f) {
 }
 // Register budget of the megakernels: 1 = the compiler's choice, 4 = at most
 // 128 VGPRs (4 waves per SIMD), 5 = at most 96 VGPRs (5 waves, more spills),
-// 6 = at most 80 VGPRs.  withWaves calls f(integral_constant<int, w>).
+// 6 = at most 80 VGPRs, 8 = at most 64 (the most waves a SIMD holds).
+// withWaves calls f(integral_constant<int, w>).
 template <class F>
 inline void withWaves(uint32_t w, F&& f) {
     if (w == 4) f(std::integral_constant<int, 4>{});
     else if (w == 5) f(std::integral_constant<int, 5>{});
     else if (w == 6) f(std::integral_constant<int, 6>{});
+    else if (w == 8) f(std::integral_constant<int, 8>{});
     else f(std::integral_constant<int, 1>{});
 }
 // LDS bytes per block of a stack mode.

@@ -258,7 +258,7 @@ typedef struct crh_tuning {
                                 max(1, render_pool / pixels), capped by the spp asked */
     uint32_t render_waves;   /* megakernels: 1 = compiler's register budget, 4 = at most
                                 128 VGPRs so 4 waves fit per SIMD, 5 = at most 96 VGPRs,
-                                6 = at most 80 VGPRs (more spills) */
+                                6 = at most 80, 8 = at most 64 (more spills) */
     uint32_t render_regen;   /* megakernel: 1 = persistent blocks whose lanes start a new
                                 path as soon as theirs ends, 2 = one path per thread */
     uint32_t render_grid;    /* megakernel with regeneration: persistent blocks */
@@ -281,7 +281,7 @@ typedef struct crh_tuning {
     uint32_t batch_grid;     /* batched rendering: persistent blocks per pass */
     uint32_t batch_chunks;   /* batched rendering: work ranges with own counters, 1..8 */
     uint32_t batch_graph;    /* batched rendering: 1 = passes replayed from a hipGraph, 2 = off */
-    uint32_t batch_waves;    /* batch megakernel register budget: 1, 4, 5 or 6 waves per SIMD */
+    uint32_t batch_waves;    /* batch megakernel register budget: 1, 4, 5, 6 or 8 waves per SIMD */
     uint32_t render_band;    /* megakernel (one path per thread): 1 = the blocks of each XCD
                                 take a contiguous band of tiles (its own L2), 2 = tiles in
                                 block order */

@@ -68,7 +68,7 @@ def test_batch_equals_solo_renders(gpu):
                 dict(batch_graph=2, batch_tile_w=32, batch_chunks=1),
                 # several sample indices per launch (graph and direct launches)
                 dict(batch_graph=1, batch_slots=2), dict(batch_graph=2, batch_slots=3),
-                dict(batch_graph=1, batch_slots=64, batch_grid=5),
+                dict(batch_graph=1, batch_slots=64, batch_grid=5, batch_waves=8),
                 dict(batch_graph=1, batch_tile_w=8, batch_grid=7, batch_chunks=8, batch_stack=2,
                      batch_waves=6),
                 dict(batch_graph=1, batch_tile_w=64, batch_grid=4096, batch_chunks=3,

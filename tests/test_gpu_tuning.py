@@ -46,9 +46,9 @@ def test_render_tuning_invariance(gpu):
     try:
         # every candidate (register budgets included) must give the same film, bit for bit
         res = autotune.tune_render(scene, cfg, spp=1, wf_grids=(64, 512), wf_blocks=(64, 256),
-                                   render_bands=(1, 2), render_waves=(1, 4, 5, 6))
+                                   render_bands=(1, 2), render_waves=(1, 4, 5, 6, 8))
         # stacks x budgets x (2 bands + 2 regeneration grids)
-        assert len(res["megakernel_ms"]) == 2 * 4 * 4
+        assert len(res["megakernel_ms"]) == 2 * 5 * 4
         assert res["mode"] in (L.MODE_MEGAKERNEL, L.MODE_WAVEFRONT)
         assert len(res["wavefront_ms"]) == 16
         t = ctx.get_tuning()
