@@ -432,8 +432,8 @@ def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not "
-                                   "part of it (use --gpus 1)"}), flush=True)
+        print(json.dumps({"error": "this benchmark measures one GPU only; multi-GPU runs are "
+                                   "not part of it (use --gpus 1)"}), flush=True)
         sys.exit(2)
 
     traffic = None

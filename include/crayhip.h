@@ -148,8 +148,8 @@ typedef struct crh_mesh {
     uint32_t material_id;
 } crh_mesh;
 
-/* SceneData (Scene/Scene.h:40-47), plus the constant environment radiance
- * added by this build (0 = the reference's black miss). */
+/* SceneData (Scene/Scene.h:40-47), plus a constant environment radiance (an
+ * addition; 0 = the reference's black miss). */
 typedef struct crh_scene_desc {
     const crh_mesh* meshes;
     uint32_t nmesh;

@@ -181,7 +181,7 @@ struct Scene {
     std::vector<uint32_t> emissive;
     Spec env;
     BVHOut bvh;
-    // Spectral transport (this build's addition): per material 3 x 95 bins
+    // Spectral transport (an addition; the reference transports RGB): per material 3 x 95 bins
     // (diffuse, specular, emission), their bin averages, the environment
     // spectrum and the film's per-lane CIE weights.
     bool spectral = false;
@@ -1170,7 +1170,7 @@ Spec tracePath(const Scene& s, const crh_camera& cam, uint32_t W, uint32_t H,
             ray.o = offsetRayOrigin(it.posW, it.orientedFaceNormal());
             ray.d = wi;
         } else {
-            // Constant environment (this build's addition; zero = reference).
+            // Constant environment (an addition; zero = the reference, which returns 0).
             radiance = radiance + (wl.on ? lanesOf(s.envSpec.data(), wl) : s.env) * beta;
             terminate = true;
         }

@@ -518,7 +518,7 @@ def test_integrator_direct_light_double_count(orc, mis):
 
 
 # ---------------------------------------------------------------------------
-# Spectral transport (this build's addition): CIE fit, XYZ normalisation and
+# Spectral transport (an addition to the reference): CIE fit, XYZ normalisation and
 # the reduction of flat spectra to the (reference-pinned) RGB estimator.
 # ---------------------------------------------------------------------------
 def _oracle_cmf(orc):
