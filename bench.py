@@ -286,7 +286,7 @@ def terrain_leg(cr, scenes, spp: int, trace_rays: int) -> dict:
         rb, hb = cr.DeviceBuffer(rays.nbytes), cr.DeviceBuffer(n * 16)
         rb.copyFromHost(rays)
         tt = autotune.tune_trace(scene, rays, reps=1, stacks=(2, 3), blocks=(256,),
-                                 persistent=((2, 0), (1, 16)), pt_grids=(2048, 4096))
+                                 persistent=((2, 0), (1, 16), (1, 32)), pt_grids=(1024, 4096))
         cr.trace_closest_device(scene, rb, n, hb)
         cnt = cr.trace_closest_device(scene, rb, n, hb, count_work=True)
         kms = float(np.mean([cr.trace_closest_device(scene, rb, n, hb)["kernel_ms"]
