@@ -480,7 +480,8 @@ def main():
                                  wf_pools=(1 << 23, 1 << 24, 1 << 25), render_waves=(1, 4, 6, 8),
                                  render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
                                  wf_finishes=(65, 1, 2, 3), render_bands=(1, 2),
-                                 render_pools=(1 << 21, 1 << 23, 1 << 24, 1 << 25))
+                                 render_pools=(1 << 21, 1 << 23, 1 << 24, 1 << 25, 1 << 26),
+                                 pool_spp=32)
     sensor.clear()
 
     progress(f"scene built, autotuned: {tuned['render_stack']=} {tuned['wf_pool']=}")

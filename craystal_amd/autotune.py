@@ -70,12 +70,13 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
                 stacks=(1, 2), wf_grids=(1024, 2048, 4096), wf_blocks=(128, 256),
                 wf_persistent=((2, 0), (1, 16)), wf_pools=(None,), render_waves=(1,),
                 render_regen=((2, 0), (1, 1024), (1, 2048)), wf_finishes=(65,),
-                render_bands=(1,), render_pools=(None,),
+                render_bands=(1,), render_pools=(None,), pool_spp: int | None = None,
                 first_sample: int = 1 << 20) -> dict:
     """Tune the megakernel stack placement, register budget and path
     regeneration ((mode, persistent grid) pairs, mode 2 = one path per
     thread), then, for the best of those, its paths per launch
-    (render_pools; None keeps the current value), and the wavefront stack
+    (render_pools, timed over pool_spp samples, default spp; None keeps the
+    current value), and the wavefront stack
     placement,
     grid/block sizes, persistence ((mode, refill) or (mode, refill, traversal
     grid) tuples; mode 1 = lanes refill, 2 = off), path pool (paths in
@@ -88,10 +89,10 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
     res_mk, res_wf = {}, {}
     ref = None
 
-    def run(c):
+    def run(c, n=spp):
         sensor.clear()
-        st = PathTraceIntegrator(c).dispatch(scene, spp, first_sample=first_sample)
-        return st["kernel_ms"] / spp, sensor.data.copy()
+        st = PathTraceIntegrator(c).dispatch(scene, n, first_sample=first_sample)
+        return st["kernel_ms"] / n, sensor.data.copy()
 
     def mk_kw(st, wv, rg, grid, band):
         return dict(render_stack=st, render_waves=wv, render_regen=rg, render_band=band,
@@ -112,12 +113,13 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
     ctx.set_tuning(**mk_kw(*best_mk))
     # paths per launch (sample indices rendered together) for the chosen shape
     rpool0 = ctx.get_tuning()["render_pool"]
-    res_pool = {}
+    res_pool, ref_pool = {}, None
     for p in [rpool0 if p is None else p for p in render_pools]:
         ctx.set_tuning(render_pool=p)
-        run(cfg)
-        ms, img = run(cfg)
-        assert img.tobytes() == ref.tobytes(), "megakernel launch shape changed the image"
+        run(cfg, pool_spp or spp)
+        ms, img = run(cfg, pool_spp or spp)
+        ref_pool = img if ref_pool is None else ref_pool
+        assert img.tobytes() == ref_pool.tobytes(), "megakernel launch shape changed the image"
         res_pool[p] = ms
     best_pool = min(res_pool, key=res_pool.get)
     ctx.set_tuning(render_pool=best_pool)
@@ -140,7 +142,9 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
     best_wf = min(res_wf, key=res_wf.get)
     st, g, b, pm, rf, tg, pool, fin = best_wf
     ctx.set_tuning(**wf_kw(st, g, b, pm, rf, tg, pool, fin))
-    mk_ms, wf_ms = min(res_mk[best_mk], res_pool[best_pool]), res_wf[best_wf]
+    # the pool stage is comparable with the wavefront only when timed over the same spp
+    mk_ms = min(res_mk[best_mk], res_pool[best_pool]) if not pool_spp else res_mk[best_mk]
+    wf_ms = res_wf[best_wf]
     return {"mode": L.MODE_WAVEFRONT if wf_ms < mk_ms else L.MODE_MEGAKERNEL,
             "render_stack": best_stack, "render_waves": best_waves, "render_regen": best_regen,
             "render_grid": best_grid, "render_band": best_band, "render_pool": best_pool,
