@@ -286,7 +286,8 @@ def terrain_leg(cr, scenes, spp: int, trace_rays: int) -> dict:
         rb, hb = cr.DeviceBuffer(rays.nbytes), cr.DeviceBuffer(n * 16)
         rb.copyFromHost(rays)
         tt = autotune.tune_trace(scene, rays, reps=1, stacks=(2, 3), blocks=(256,),
-                                 persistent=((2, 0), (1, 16), (1, 32)), pt_grids=(1024, 4096))
+                                 persistent=((2, 0), (1, 16), (1, 32)), pt_grids=(1024, 4096),
+                                 tris=(1, 2))
         cr.trace_closest_device(scene, rb, n, hb)
         cnt = cr.trace_closest_device(scene, rb, n, hb, count_work=True)
         kms = float(np.mean([cr.trace_closest_device(scene, rb, n, hb)["kernel_ms"]
@@ -299,7 +300,7 @@ def terrain_leg(cr, scenes, spp: int, trace_rays: int) -> dict:
             "node_visits_per_ray": round(cnt["node_visits"] / n, 3),
             "tri_tests_per_ray": round(cnt["tri_tests"] / n, 3),
             "tuning": {k: tt[k] for k in ("stack", "trace_block", "persistent", "refill",
-                                          "pt_grid")},
+                                          "pt_grid", "trace_tris")},
             "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
                          "bytes_per_ray": round(b_ray, 1)}}
@@ -593,11 +594,12 @@ def main():
         rb.copyFromHost(rays)
         ttuned = autotune.tune_trace(scene, rays, reps=2, stacks=(2, 3), blocks=(64, 128, 256),
                                      persistent=((2, 0), (1, 16), (1, 32)),
-                                     pt_grids=(2048, 4096))
+                                     pt_grids=(2048, 4096), tris=(1, 2))
         prof_tuning.update(stack=ttuned["stack"], trace_block=ttuned["trace_block"],
                            persistent=ttuned["persistent"])
         if ttuned["persistent"] == 1:
-            prof_tuning.update(refill=ttuned["refill"], pt_grid=ttuned["pt_grid"])
+            prof_tuning.update(refill=ttuned["refill"], pt_grid=ttuned["pt_grid"],
+                               trace_tris=ttuned["trace_tris"])
         progress(f"profile tuning: {json.dumps(prof_tuning)}")
         if not a.no_traffic:
             # HBM traffic of both kernels at the launch shapes chosen, from two
@@ -621,7 +623,7 @@ def main():
             "node_visits_per_ray": round(cnt["node_visits"] / n, 3),
             "tri_tests_per_ray": round(cnt["tri_tests"] / n, 3),
             "tuning": {k: ttuned[k] for k in ("stack", "trace_block", "persistent", "refill",
-                                              "pt_grid")},
+                                              "pt_grid", "trace_tris")},
             "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
                          "traffic": traffic["kTraceClosest"] if traffic else None,

@@ -109,7 +109,7 @@ class crh_tuning(C.Structure):
                                    "wf_shade_grid", "wf_persistent", "wf_refill", "wf_pool",
                                    "batch_stack", "batch_tile_w", "batch_grid", "batch_chunks",
                                    "batch_graph", "batch_waves", "render_band",
-                                   "batch_slots")]
+                                   "batch_slots", "trace_tris")]
 
 
 class crh_batch_info(C.Structure):

@@ -24,9 +24,10 @@ def _median(xs):
 
 def tune_trace(scene: Scene, rays: np.ndarray, reps: int = 3, device: int = 0,
                stacks=(1, 2), blocks=(64, 128, 256), persistent=((2, 0),),
-               pt_grids=(1024,)) -> dict:
+               pt_grids=(1024,), tris=(1,)) -> dict:
     """Tune the closest-hit batch kernel over stack placement, block size and
-    persistence ((mode, refill) pairs; mode 1 = lanes refill, 2 = off);
+    persistence ((mode, refill) pairs; mode 1 = lanes refill, 2 = off; with
+    refill, also the persistent grid and the triangles per traversal step);
     returns the chosen tuning and the per-candidate timings (ms)."""
     ctx = get_context(device)
     saved = ctx.get_tuning()
@@ -39,11 +40,17 @@ def tune_trace(scene: Scene, rays: np.ndarray, reps: int = 3, device: int = 0,
     cands = []
     for st, bl, (pm, rf) in itertools.product(stacks, blocks, persistent):
         for pg in (pt_grids if pm == 1 else (0,)):
-            cands.append((st, bl, pm, rf, pg))
-    for st, bl, pm, rf, pg in cands:
+            for tr in (tris if pm == 1 else (1,)):
+                cands.append((st, bl, pm, rf, pg, tr))
+
+    def kw_of(st, bl, pm, rf, pg, tr):
         kw = dict(stack=st, trace_block=bl, persistent=pm)
         if pm == 1:
-            kw.update(refill=rf, pt_grid=pg)
+            kw.update(refill=rf, pt_grid=pg, trace_tris=tr)
+        return kw
+
+    for c in cands:
+        kw = kw_of(*c)
         ctx.set_tuning(**kw)
         trace_closest_device(scene, rb, n, hb)
         ms = [trace_closest_device(scene, rb, n, hb)["kernel_ms"] for _ in range(reps)]
@@ -54,16 +61,14 @@ def tune_trace(scene: Scene, rays: np.ndarray, reps: int = 3, device: int = 0,
         elif out.tobytes() != ref.tobytes():
             ctx.set_tuning(**saved)
             raise AssertionError(f"tuning {kw} changed the hits")
-        results[(st, bl, pm, rf, pg)] = _median(ms)
+        results[c] = _median(ms)
     best = min(results, key=results.get)
-    st, bl, pm, rf, pg = best
-    kw = dict(stack=st, trace_block=bl, persistent=pm)
-    if pm == 1:
-        kw.update(refill=rf, pt_grid=pg)
-    ctx.set_tuning(**kw)
+    st, bl, pm, rf, pg, tr = best
+    ctx.set_tuning(**kw_of(*best))
     return {"stack": st, "trace_block": bl, "persistent": pm, "refill": rf, "pt_grid": pg,
+            "trace_tris": tr,
             "timings_ms": {f"stack={k[0]},block={k[1]},persistent={k[2]},refill={k[3]},"
-                           f"grid={k[4]}": round(v, 4) for k, v in results.items()}}
+                           f"grid={k[4]},tris={k[5]}": round(v, 4) for k, v in results.items()}}
 
 
 def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,

@@ -426,6 +426,7 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(t->wf_finish <= 65, "wf_finish must be 0 (keep), 1..64 (bounces) or 65 (off)");
     ARG_CHECK(t->render_band <= 2, "render_band must be 0 (keep), 1 (on) or 2 (off)");
     ARG_CHECK(t->batch_slots <= 64, "batch_slots must be 0 (keep) or 1..64");
+    ARG_CHECK(t->trace_tris <= 4, "trace_tris must be 0 (keep) or 1..4");
     ARG_CHECK(t->wf_pool <= (1u << 28) && t->render_pool <= (1u << 28),
               "wf_pool and render_pool must be at most 2^28 paths");
     ARG_CHECK(okStack(t->batch_stack),
@@ -463,6 +464,7 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     if (t->batch_chunks) o.batchChunks = t->batch_chunks;
     if (t->batch_graph) o.batchGraph = t->batch_graph == 1 ? 1u : 0u;
     if (t->batch_slots) o.batchSlots = t->batch_slots;
+    if (t->trace_tris) o.traceTris = t->trace_tris;
     return CRH_OK;
 }
 
@@ -496,6 +498,7 @@ int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
     t->batch_chunks = o.batchChunks;
     t->batch_graph = o.batchGraph ? 1u : 2u;
     t->batch_slots = o.batchSlots;
+    t->trace_tris = o.traceTris;
     return CRH_OK;
 }
 

@@ -37,6 +37,7 @@ struct Tuning {
     uint32_t renderWaves = 8;        // megakernels: minimum waves per SIMD (1, 4, 5, 6 or 8)
     uint32_t batchWaves = 8;         // batch megakernel: the same for kBatchRender
     uint32_t batchSlots = 16;        // batch: sample indices per launch
+    uint32_t traceTris = 1;          // persistent ray batches: triangles per step
     uint32_t renderRegen = 1;        // megakernel: lanes take a new path when theirs ends
     uint32_t renderGrid = 2048;      // megakernel with regeneration: persistent blocks
     uint32_t xcdBand = 0;            // megakernel: each XCD takes a contiguous band of tiles

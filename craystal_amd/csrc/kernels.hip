@@ -76,11 +76,12 @@ __global__ __launch_bounds__(64) void kResetCounter(uint32_t* c) {
 
 template <bool ANY, int SM>
 __global__ __launch_bounds__(kTraceBlock) void kTracePersistent(
-    SceneView s, BatchSrc src, uint32_t n, uint32_t* __restrict__ counter, uint32_t refill) {
+    SceneView s, BatchSrc src, uint32_t n, uint32_t* __restrict__ counter, uint32_t refill,
+    uint32_t tris) {
     extern __shared__ uint2 lds[];
     auto stk = makeStack<SM>(lds);
     WorkCount wc{0, 0};
-    persistentTrace<ANY, false>(s, src, counter, n, refill, stk, wc);
+    persistentTrace<ANY, false>(s, src, counter, n, refill, stk, wc, tris);
 }
 
 // ---------------------------------------------------------------------------
@@ -406,7 +407,7 @@ static hipError_t launchPersistent(const SceneView& s, const BatchSrc& src, uint
     const dim3 grid(t.ptGrid), block(B);
     withStack(t.privStack, [&](auto sm) {
         hipLaunchKernelGGL((kTracePersistent<ANY, decltype(sm)::value>), grid, block, lds, st, s,
-                           src, n, counter, t.refill);
+                           src, n, counter, t.refill, t.traceTris);
     });
     return hipGetLastError();
 }

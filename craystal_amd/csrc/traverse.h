@@ -267,33 +267,41 @@ CRH_D bool travInit(const SceneView& s, const Ray& ray, TravState& t) {
     return !boxHit(s.rootMin, s.rootMax, t.r, dummy);
 }
 
-// One step of traverse()'s work (one internal node, or one triangle of a
-// leaf); returns true when the ray is done.
+// One step of traverse()'s work (one internal node, or up to `tris`
+// consecutive triangles of a leaf); returns true when the ray is done.
 template <bool ANY, bool COUNT, class Stack>
-CRH_D bool travStep(const SceneView& s, TravState& t, Stack& stack, WorkCount& wc) {
+CRH_D bool travStep(const SceneView& s, TravState& t, Stack& stack, WorkCount& wc,
+                    uint32_t tris = 1) {
     const RayPre& r = t.r;
-    if (t.cur & kLeafBit) {  // one triangle per step
+    if (t.cur & kLeafBit) {
         uint32_t i = t.cur & ~kLeafBit;
-        const float4* tp = reinterpret_cast<const float4*>(s.tris + i);
-        float4 a = tp[0], b = tp[1], c = tp[2];
-        uint32_t pw = __float_as_uint(a.w);
-        if (COUNT) wc.tris++;
-        float u, v, tt;
-        if (triHit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), r, u, v, tt)) {
-            if (!(tt < r.tMin || tt > r.tMax)) {
-                t.isHit = true;
-                if (tt < t.hit.t) {
-                    t.hit.t = tt;
-                    t.hit.u = u;
-                    t.hit.v = v;
-                    t.hit.prim = pw & ~kLeafBit;
+        bool last = false;
+        for (uint32_t k = 0; k < tris; ++k) {
+            const float4* tp = reinterpret_cast<const float4*>(s.tris + i);
+            float4 a = tp[0], b = tp[1], c = tp[2];
+            uint32_t pw = __float_as_uint(a.w);
+            if (COUNT) wc.tris++;
+            float u, v, tt;
+            if (triHit(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), r, u, v,
+                       tt)) {
+                if (!(tt < r.tMin || tt > r.tMax)) {
+                    t.isHit = true;
+                    if (tt < t.hit.t) {
+                        t.hit.t = tt;
+                        t.hit.u = u;
+                        t.hit.v = v;
+                        t.hit.prim = pw & ~kLeafBit;
+                    }
+                    t.closestT = smin(t.closestT, t.hit.t);
+                    if (ANY) return true;
                 }
-                t.closestT = smin(t.closestT, t.hit.t);
-                if (ANY) return true;
             }
+            last = (pw & kLeafBit) != 0;  // the leaf's last triangle
+            if (last) break;
+            ++i;
         }
-        if (!(pw & kLeafBit)) {
-            t.cur = (i + 1u) | kLeafBit;
+        if (!last) {
+            t.cur = i | kLeafBit;
             return false;
         }
     } else {
@@ -344,11 +352,11 @@ CRH_D bool travStep(const SceneView& s, TravState& t, Stack& stack, WorkCount& w
 // Persistent traversal with per-lane ray refill.  Rays [0, n) are handed
 // out by a WaveDispenser over *counter (zero at launch), one atomic per
 // chunk.  A wave keeps stepping its lanes and refills once `refill` or more
-// of them are idle while rays remain.  src.load(i) gives ray i,
-// src.store(i, isHit, hit) its result.
+// of them are idle while rays remain; a step tests up to `tris` triangles of
+// a leaf.  src.load(i) gives ray i, src.store(i, isHit, hit) its result.
 template <bool ANY, bool COUNT, class Stack, class Src>
 CRH_D void persistentTrace(const SceneView& s, Src& src, uint32_t* counter, uint32_t n,
-                           uint32_t refill, Stack& stack, WorkCount& wc) {
+                           uint32_t refill, Stack& stack, WorkCount& wc, uint32_t tris = 1) {
     const uint32_t kNone = 0xFFFFFFFFu;
     const uint32_t chunk = dispenserChunk(n);
     uint32_t mine = kNone;
@@ -369,7 +377,7 @@ CRH_D void persistentTrace(const SceneView& s, Src& src, uint32_t* counter, uint
         }
         while (true) {
             if (mine != kNone) {
-                if (travStep<ANY, COUNT>(s, t, stack, wc)) {
+                if (travStep<ANY, COUNT>(s, t, stack, wc, tris)) {
                     src.store(mine, t.isHit, t.hit);
                     mine = kNone;
                 }

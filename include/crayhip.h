@@ -287,6 +287,8 @@ typedef struct crh_tuning {
                                 block order */
     uint32_t batch_slots;    /* batched rendering: sample indices per launch, 1..64 (while the
                                 set of scenes still sampling stays the same) */
+    uint32_t trace_tris;     /* ray batches, persistent: triangles of a leaf tested per
+                                traversal step, 1..4 */
 } crh_tuning;
 
 /* ---- errors / device ------------------------------------------------- */

@@ -42,7 +42,9 @@ def test_long_leaves_hits_bit_exact(gpu, orc):
     saved = ctx.get_tuning()
     try:
         for kw in (dict(persistent=2), dict(persistent=1, refill=16, pt_grid=64),
-                   dict(persistent=1, refill=1, pt_grid=5)):
+                   dict(persistent=1, refill=1, pt_grid=5, trace_tris=1),
+                   dict(persistent=1, refill=16, pt_grid=64, trace_tris=2),
+                   dict(persistent=1, refill=8, pt_grid=7, trace_tris=4)):
             ctx.set_tuning(**kw)
             h = gs.trace_closest(rays)
             for k in ("prim", "t", "u", "v"):

@@ -17,10 +17,11 @@ def test_trace_tuning_invariance_and_autotune(gpu):
     ctx = cr.get_context(0)
     saved = ctx.get_tuning()
     try:
-        res = autotune.tune_trace(scene, rays, reps=2,
-                                  persistent=((2, 0), (1, 8), (1, 32)), pt_grids=(64, 1024))
+        # every candidate (triangles per step included) must give the same hits
+        res = autotune.tune_trace(scene, rays, reps=2, persistent=((2, 0), (1, 8), (1, 32)),
+                                  pt_grids=(64, 1024), tris=(1, 3))
         assert res["stack"] in (1, 2) and res["trace_block"] in (64, 128, 256)
-        assert len(res["timings_ms"]) == 6 * 5
+        assert len(res["timings_ms"]) == 6 * 9
         assert ctx.get_tuning()["trace_block"] == res["trace_block"]
         # any-hit: private stack and persistent refill equal the LDS stack
         ctx.set_tuning(stack=1, persistent=2)
@@ -29,7 +30,9 @@ def test_trace_tuning_invariance_and_autotune(gpu):
         b = scene.trace_any(rays)
         ctx.set_tuning(stack=1, persistent=1, refill=4, pt_grid=32)
         c = scene.trace_any(rays)
-        assert np.array_equal(a, b) and np.array_equal(a, c)
+        ctx.set_tuning(trace_tris=2)
+        d = scene.trace_any(rays)
+        assert np.array_equal(a, b) and np.array_equal(a, c) and np.array_equal(a, d)
     finally:
         ctx.set_tuning(**saved)
 
@@ -67,6 +70,8 @@ def test_tuning_rejects_bad_values(gpu):
         ctx.set_tuning(wf_refill=65)
     with pytest.raises(cr.RuntimeError):
         ctx.set_tuning(batch_slots=65)
+    with pytest.raises(cr.RuntimeError):
+        ctx.set_tuning(trace_tris=5)
     for w in (2, 3, 7):
         with pytest.raises(cr.RuntimeError):
             ctx.set_tuning(render_waves=w)
