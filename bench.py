@@ -663,7 +663,7 @@ def main():
             passes += 1
         cs = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(passes * px / cs / 1e6, 4), "unit": "Msamples/s",
-                               "cores": threads, "kind": "port",
+                               "cores": threads, "kind": "port", "cpu_model": _cpu_model(),
                                "sample": f"{passes} spp passes of the same 1920x1080 image "
                                          f"({passes * px} paths, {crays} rays) by the C++ "
                                          f"oracle, OpenMP over rows",
@@ -692,6 +692,17 @@ def main():
                 "one_core_grays_per_s": round(m / ct1 / 1e9, 5), "one_core_rays": m}
         progress(f"cpu baseline: {out['cpu_baseline']['value']} Msamples/s")
     print(json.dumps(out), flush=True)
+
+
+def _cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo, as lscpu reports it)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 if __name__ == "__main__":

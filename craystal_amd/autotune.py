@@ -118,7 +118,11 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
     ctx.set_tuning(**mk_kw(*best_mk))
     # paths per launch (sample indices rendered together) for the chosen shape
     rpool0 = ctx.get_tuning()["render_pool"]
-    res_pool, ref_pool = {}, None
+    # Over the same spp the first stage's image is the reference; over other
+    # spp the stage's own first image is (and the stage-1 image is checked
+    # again below, once the pool is chosen).
+    same_spp = pool_spp is None or pool_spp == spp
+    res_pool, ref_pool = {}, (ref if same_spp else None)
     for p in [rpool0 if p is None else p for p in render_pools]:
         ctx.set_tuning(render_pool=p)
         run(cfg, pool_spp or spp)
@@ -128,6 +132,14 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
         res_pool[p] = ms
     best_pool = min(res_pool, key=res_pool.get)
     ctx.set_tuning(render_pool=best_pool)
+    # The megakernel as it now stands (chosen shape and pool), timed over spp
+    # like the wavefront below, so the mode choice compares like with like.
+    if same_spp:
+        mk_ms = min(res_mk[best_mk], res_pool[best_pool])
+    else:
+        run(cfg)
+        mk_ms, img = run(cfg)
+        assert img.tobytes() == ref.tobytes(), "megakernel launch shape changed the image"
     wf_cfg = Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
     pool0 = ctx.get_tuning()["wf_pool"]
     pools = [pool0 if p is None else p for p in wf_pools]
@@ -147,8 +159,6 @@ def tune_render(scene: Scene, cfg: Configs, spp: int = 1, device: int = 0,
     best_wf = min(res_wf, key=res_wf.get)
     st, g, b, pm, rf, tg, pool, fin = best_wf
     ctx.set_tuning(**wf_kw(st, g, b, pm, rf, tg, pool, fin))
-    # the pool stage is comparable with the wavefront only when timed over the same spp
-    mk_ms = min(res_mk[best_mk], res_pool[best_pool]) if not pool_spp else res_mk[best_mk]
     wf_ms = res_wf[best_wf]
     return {"mode": L.MODE_WAVEFRONT if wf_ms < mk_ms else L.MODE_MEGAKERNEL,
             "render_stack": best_stack, "render_waves": best_waves, "render_regen": best_regen,

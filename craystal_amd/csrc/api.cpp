@@ -220,6 +220,11 @@ struct crh_film {
     CtxRef ctx;
     uint32_t w = 0, h = 0;
     DevMem sum, wsum, L, P;
+    // Work counters of crh_render (rays, node visits, triangle tests),
+    // allocated with the film so that a render allocates nothing itself.
+    DevMem work;
+    // Pass timing events of crh_render, created once and grown on demand.
+    std::vector<hipEvent_t> passEv;
     // Wavefront path state, allocated on first wavefront render.
     DevMem wfArena;
     crh::WfState wf{};
@@ -230,6 +235,9 @@ struct crh_film {
     // Training samples waiting in gSamples (accumulated over passes until a
     // batch is full) and the buffer's capacity in samples.
     size_t gPending = 0, gCap = 0;
+    ~crh_film() {
+        for (hipEvent_t e : passEv) (void)hipEventDestroy(e);
+    }
 };
 
 namespace {
@@ -1008,6 +1016,7 @@ int crh_film_create(crh_context* ctx, uint32_t w, uint32_t h, crh_film** out) {
     if (e == hipSuccess) e = f->wsum.alloc(n * sizeof(float));
     if (e == hipSuccess) e = f->L.alloc(n * sizeof(float4));
     if (e == hipSuccess) e = f->P.alloc(n * sizeof(float2));
+    if (e == hipSuccess) e = f->work.alloc(3 * sizeof(unsigned long long));
     if (e != hipSuccess) {
         delete f;
         return hipFail("film allocation", e, __FILE__, __LINE__);
@@ -1065,21 +1074,8 @@ int crh_render(crh_scene* s, const crh_camera* cam,
                            cfg->rr_start_depth, cfg->use_mis ? 1u : 0u};
     double t0 = nowMs();
     HIP_TRY(hipSetDevice(ctx->device));
-    DevMem counter;
-    HIP_TRY(counter.alloc(3 * sizeof(unsigned long long)));
+    DevMem& counter = film->work;
     HIP_TRY(hipMemsetAsync(counter.p, 0, counter.bytes, ctx->stream));
-    // One event pair per pass times the megakernel launches on their stream.
-    std::vector<hipEvent_t> evs;
-    struct EvGuard {
-        std::vector<hipEvent_t>& v;
-        ~EvGuard() {
-            for (hipEvent_t e : v) (void)hipEventDestroy(e);
-        }
-    } evGuard{evs};
-    if (stats) {
-        evs.resize(size_t(spp) * 2, nullptr);
-        for (auto& e : evs) HIP_TRY(hipEventCreate(&e));
-    }
     const bool wavefront = cfg->mode == CRH_MODE_WAVEFRONT;
     const crh::Tuning tun = effective(ctx->tuning, s->view);
     const crh::WfLaunch wl = wfLaunch(tun);
@@ -1092,12 +1088,22 @@ int crh_render(crh_scene* s, const crh_camera* cam,
         HIP_TRY(ensureFilmSlots(film, slots));
     }
     const size_t npix = size_t(film->w) * film->h;
+    // One event pair per pass group times the main launches on their stream;
+    // the film keeps the events for later calls.
+    std::vector<hipEvent_t>& evs = film->passEv;
+    const size_t nev = stats ? 2 * size_t((spp + slots - 1) / slots) : 0;
+    while (evs.size() < nev) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreate(&e));
+        evs.push_back(e);
+    }
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     // Pass groups: `slots` sample indices per megakernel launch or wavefront
     // pass, gathered into the film in sample order.
     for (uint32_t i = 0; i < spp; i += slots) {
         const uint32_t active = std::min(slots, spp - i);
-        if (stats) HIP_TRY(hipEventRecord(evs[2 * i], ctx->stream));
+        const size_t g = i / slots;
+        if (stats) HIP_TRY(hipEventRecord(evs[2 * g], ctx->stream));
         if (wavefront)
             HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, active, film->wf, wl,
                                        cfg->count_work != 0, ctx->stream));
@@ -1107,7 +1113,7 @@ int crh_render(crh_scene* s, const crh_camera* cam,
                                           counter.as<unsigned long long>(),
                                           cfg->count_work != 0, tun, ctx->stream,
                                           cfg->spectral != 0, ctx->counter));
-        if (stats) HIP_TRY(hipEventRecord(evs[2 * i + 1], ctx->stream));
+        if (stats) HIP_TRY(hipEventRecord(evs[2 * g + 1], ctx->stream));
         for (uint32_t k = 0; k < active; ++k) {
             const float4* Ls = (wavefront ? film->wf.rad : film->L.as<float4>()) + k * npix;
             const float2* Ps = (wavefront ? film->wf.P : film->P.as<float2>()) + k * npix;
@@ -1121,9 +1127,9 @@ int crh_render(crh_scene* s, const crh_camera* cam,
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         double mainMs = 0.0;
-        for (uint32_t i = 0; i < spp; i += slots) {
+        for (size_t g = 0; 2 * g < nev; ++g) {
             float m = 0.f;
-            HIP_TRY(hipEventElapsedTime(&m, evs[2 * i], evs[2 * i + 1]));
+            HIP_TRY(hipEventElapsedTime(&m, evs[2 * g], evs[2 * g + 1]));
             mainMs += m;
         }
         unsigned long long c[3] = {0, 0, 0};
@@ -1477,6 +1483,7 @@ struct crh_batch {
     uint64_t nodes = 0, tris = 0;
     uint32_t maxStack = 1;
     DevMem table, prefix, next, graphCounter;
+    DevMem counter;  // work counters of the direct (non-graph) launches
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<uint64_t> sig;  // what the captured graph was built for
@@ -1573,11 +1580,7 @@ int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh
 
 int crh_cie_cmf(float* xyz) {
     ARG_CHECK(xyz, "NULL argument");
-    double c[3];
-    for (int i = 0; i < crh::kCieBins; ++i) {
-        crh::cieFit(crh::kCieLambda0 + crh::kCieStep * i, c);
-        for (int j = 0; j < 3; ++j) xyz[3 * i + j] = float(c[j]);
-    }
+    for (int i = 0; i < crh::kCieBins; ++i) crh::cieTable(i, xyz + 3 * i);
     return CRH_OK;
 }
 
@@ -1669,8 +1672,8 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
         HIP_TRY(b->prefix.alloc(pre.size() * sizeof(uint32_t)));
         b->dropGraph();  // the captured graph points at the old buffers
     }
-    DevMem counter;
-    HIP_TRY(counter.alloc(3 * sizeof(unsigned long long)));
+    DevMem& counter = b->counter;
+    if (!counter.p) HIP_TRY(counter.alloc(3 * sizeof(unsigned long long)));
     HIP_TRY(hipMemsetAsync(counter.p, 0, counter.bytes, ctx->stream));
     HIP_TRY(hipMemcpyAsync(b->table.p, tab.data(), tab.size() * sizeof(crh::BatchScene),
                            hipMemcpyHostToDevice, ctx->stream));

@@ -416,7 +416,10 @@ hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
                               uint64_t n, crh_hit* hits, uint32_t* work,
                               const Tuning& t, uint32_t* counter, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    if (t.persistent && !work && counter && n < (uint64_t(1) << 32))
+    // The persistent kernels' dispenser counts in 32 bits and may run up to
+    // (waves of the launch) x (chunk <= 512) past n before every wave sees
+    // the end: n <= 2^31 keeps that far below 2^32.
+    if (t.persistent && !work && counter && n <= (uint64_t(1) << 31))
         return launchPersistent<false>(s, BatchSrc{rays, hits, nullptr}, uint32_t(n), counter, t,
                                        st);
     const uint32_t B = t.traceBlock;
@@ -437,7 +440,7 @@ hipError_t launchTraceClosest(const SceneView& s, const crh_ray* rays,
 hipError_t launchTraceAny(const SceneView& s, const crh_ray* rays, uint64_t n,
                           uint8_t* occ, const Tuning& t, uint32_t* counter, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    if (t.persistent && counter && n < (uint64_t(1) << 32))
+    if (t.persistent && counter && n <= (uint64_t(1) << 31))  // as launchTraceClosest
         return launchPersistent<true>(s, BatchSrc{rays, nullptr, occ}, uint32_t(n), counter, t, st);
     const uint32_t B = t.traceBlock;
     const size_t lds = stackLdsBytes(t.privStack, s.stackCap, B);

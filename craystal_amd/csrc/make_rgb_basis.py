@@ -12,8 +12,8 @@ colour is the target exactly:
     RGB_E(s) = diag(1 / w) * M * XYZ(s)
 
 M is the XYZ -> linear sRGB matrix (the inverse of the sRGB -> XYZ matrix
-the film uses), XYZ(s) the CIE fit of spectral.cpp with its Y
-normalisation, and w = M * XYZ(1) so that the flat spectrum 1 has
+the film uses), XYZ(s) the sum of the CIE table (cie_table.h, the
+reference's kXYZTable) times s, and w = M * XYZ(1) so that the flat spectrum 1 has
 RGB_E = (1, 1, 1) (white balanced to the equal-energy spectrum).  White
 itself is the flat spectrum 1; targets: cyan (0,1,1), magenta (1,0,1),
 yellow (1,1,0), red (1,0,0), green (0,1,0), blue (0,0,1).
@@ -22,13 +22,13 @@ Run: python craystal_amd/csrc/make_rgb_basis.py  (rewrites rgb_basis.h)
 """
 from __future__ import annotations
 
+import re
 from pathlib import Path
 
 import numpy as np
 from scipy.optimize import minimize
 
 BINS = 95
-LAMBDA = 360.0 + 5.0 * np.arange(BINS)
 SRGB_TO_XYZ = np.array([[0.412453, 0.357580, 0.180423],
                         [0.212671, 0.715160, 0.072169],
                         [0.019334, 0.119193, 0.950227]])
@@ -36,24 +36,18 @@ TARGETS = [("cyan", (0, 1, 1)), ("magenta", (1, 0, 1)), ("yellow", (1, 1, 0)),
            ("red", (1, 0, 0)), ("green", (0, 1, 0)), ("blue", (0, 0, 1))]
 
 
-def _lobe(x, mu, s1, s2):
-    t = (x - mu) / np.where(x < mu, s1, s2)
-    return np.exp(-0.5 * t * t)
-
-
-def cie_fit(lam: np.ndarray) -> np.ndarray:
-    """(3, n): Wyman, Sloan, Shirley 2013 multi-lobe fit (as spectral.cpp)."""
-    x = (1.056 * _lobe(lam, 599.8, 37.9, 31.0) + 0.362 * _lobe(lam, 442.0, 16.0, 26.7)
-         - 0.065 * _lobe(lam, 501.1, 20.4, 26.2))
-    y = 0.821 * _lobe(lam, 568.8, 46.9, 40.5) + 0.286 * _lobe(lam, 530.9, 16.3, 31.1)
-    z = 1.217 * _lobe(lam, 437.0, 11.8, 36.0) + 0.681 * _lobe(lam, 459.0, 26.0, 13.8)
-    return np.stack([x, y, z])
+def cie_table() -> np.ndarray:
+    """(3, 95): the colour matching functions of the bins, read from cie_table.h."""
+    text = Path(__file__).with_name("cie_table.h").read_text()
+    body = text[text.index("kCieXYZ[3][95] = {"):]
+    vals = [float(v) for v in re.findall(r"(-?[0-9]+\.[0-9]+)f", body)]
+    assert len(vals) == 3 * BINS
+    return np.array(vals).reshape(3, BINS)
 
 
 def rgb_e_matrix() -> np.ndarray:
     """(3, 95): spectrum -> white-balanced linear sRGB."""
-    cmf = cie_fit(LAMBDA)
-    to_xyz = cmf * 5.0 / (cmf[1].sum() * 5.0)          # Y(flat 1) = 1
+    to_xyz = cie_table()                                # the scale divides out below
     m = np.linalg.inv(SRGB_TO_XYZ) @ to_xyz
     w = m.sum(axis=1)                                   # RGB of the flat spectrum 1
     return m / w[:, None]
@@ -78,7 +72,7 @@ def main() -> None:
     A = rgb_e_matrix()
     rows = [(name, solve(t, A)) for name, t in TARGETS]
     out = ["// Generated by make_rgb_basis.py (Smits's RGB -> spectrum basis, recomputed for",
-           "// this library's CIE fit and sRGB matrix); do not edit.",
+           "// this library's CIE table and sRGB matrix); do not edit.",
            "#pragma once", "", "namespace crh {", "",
            "// Basis spectra, 95 bins from 360 nm in 5 nm steps; white is the flat 1.",
            "// Rows: cyan, magenta, yellow, red, green, blue.",

@@ -1,6 +1,6 @@
 // Host side of spectral transport: the CIE 1931 colour matching functions
 // and the per-scene spectral table the megakernel reads (layout: kSpec* in
-// shading.h).
+// spec_layout.h).
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -8,20 +8,18 @@
 
 namespace crh {
 
-constexpr int kCieBins = 95;            // 360, 365, ..., 830 nm
-constexpr double kCieLambda0 = 360.0;
-constexpr double kCieStep = 5.0;
+constexpr int kCieBins = 95;  // spectral bins; a spectrum's bin k pairs with table entry k
+constexpr double kCieLambda0 = 360.0;  // nominal bin wavelengths 360, 365, ..., 830 nm
+constexpr double kCieStep = 5.0;       // (used to sample smooth curves into bins)
 
-// x-bar, y-bar, z-bar at lambda (nm): the multi-lobe piecewise-Gaussian fit of
-// C. Wyman, P.-P. Sloan, P. Shirley, "Simple Analytic Approximations to the
-// CIE XYZ Color Matching Functions", JCGT 2(2), 2013.
-void cieFit(double lambda, double xyz[3]);
+// x-bar, y-bar, z-bar of bin k: the CIE 1931 2-degree table of the reference
+// (kXYZTable, Core/SpectrumTabData.h:7-55; generated into cie_table.h).
+void cieTable(int k, float xyz[3]);
 
-// Normalisation so that a constant spectrum 1 has Y = 1:
-// 1 / (sum_i y-bar(lambda_i) * step).
-double cieYNorm();
-
-// XYZ of a 95-bin spectrum: sum_i (x, y, z)-bar(lambda_i) * s_i * step * norm.
+// XYZ of a 95-bin spectrum as BroadSpectrum::toXYZ (Core/Spectrum.cpp:37-45)
+// computes it: a float sum over bins of (x, y, z)-bar * s_k, then times
+// kCIENormalizationY * kXYZStep (Core/Spectrum.h:286, SpectrumTabData.h:57).
+// The reference reads x-bar for Z (Core/Spectrum.cpp:40); here Z uses z-bar.
 void spectrumToXYZ(const float* s, float xyz[3]);
 
 // RGB -> 95-bin spectrum (Smits's decomposition into white + one secondary +

@@ -440,12 +440,14 @@ int crh_render_guided(crh_scene* scene, const crh_camera* cam,
                       const crh_guided_cfg* gcfg, uint32_t first_sample,
                       uint32_t spp, crh_film* film, crh_stats* stats);
 
-/* ---- spectra (no reference counterpart is used by its integrator;
- * BroadSpectrum::toXYZ, Core/Spectrum.cpp:36-44, is the analogue) -------- */
-/* The colour matching functions used: x, y, z-bar at 360 + 5 i nm, i < 95
- * (an analytic fit, see craystal_amd/csrc/spectral.h), 285 floats. */
+/* ---- spectra (the reference's integrator transports RGB; its 95-bin
+ * BroadSpectrum and CIE table, Core/Spectrum.h:278-324, are the analogue) -- */
+/* The colour matching functions of the 95 bins: the reference's kXYZTable
+ * (Core/SpectrumTabData.h:7-55) as x, y, z-bar per bin, 285 floats. */
 int crh_cie_cmf(float* xyz);
-/* XYZ of a 95-bin spectrum, normalised so that a constant 1 has Y = 1. */
+/* XYZ of a 95-bin spectrum: BroadSpectrum::toXYZ (Core/Spectrum.cpp:37-45),
+ * sum of table * s times kXYZStep * kCIENormalizationY, with z-bar for Z
+ * (the reference reads x-bar there, Core/Spectrum.cpp:40). */
 int crh_spectrum_to_xyz(const float* spectrum, float* xyz);
 /* RGB -> 95-bin spectrum, replacing BroadSpectrum::fromRGB
  * (Core/Spectrum.cpp:49, which returns zeros; its Smits tables are

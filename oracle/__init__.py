@@ -42,6 +42,7 @@ def lib() -> C.CDLL:
             "orc_bvh_free": (None, [P]),
             "orc_scene_create": (P, [P]),
             "orc_cie_cmf": (None, [P]),
+            "orc_spectrum_to_xyz": (None, [P, P]),
             "orc_sphere_hit": (C.c_int, [P, f32, P, P]),
             "orc_texture_sample": (None, [P, f32, f32, P]),
             "orc_scene_free": (None, [P]),

@@ -39,9 +39,14 @@
 #include <numeric>
 #include <vector>
 
+#include "../craystal_amd/csrc/cie_table.h"  // CIE 1931 table data (kXYZTable)
 #include "../craystal_amd/csrc/rgb_basis.h"  // basis data of the RGB uplift
 #include "../include/crayhip.h"
 #include "vecmath.h"
+
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
 
 namespace orc {
 
@@ -156,19 +161,24 @@ struct BVHOut {
     double buildMs = 0;
 };
 
-// CIE 1931 colour matching functions at lambda (nm): the piecewise-Gaussian
-// fit of Wyman, Sloan and Shirley, "Simple Analytic Approximations to the CIE
-// XYZ Color Matching Functions", JCGT 2(2), 2013 (coefficients of its
-// multi-lobe fit).
-inline void cieXYZ(double l, double out[3]) {
-    auto g = [](double x, double mu, double s1, double s2) {
-        double t = (x - mu) / (x < mu ? s1 : s2);
-        return std::exp(-0.5 * t * t);
-    };
-    out[0] = 1.056 * g(l, 599.8, 37.9, 31.0) + 0.362 * g(l, 442.0, 16.0, 26.7) -
-             0.065 * g(l, 501.1, 20.4, 26.2);
-    out[1] = 0.821 * g(l, 568.8, 46.9, 40.5) + 0.286 * g(l, 530.9, 16.3, 31.1);
-    out[2] = 1.217 * g(l, 437.0, 11.8, 36.0) + 0.681 * g(l, 459.0, 26.0, 13.8);
+// CIE 1931 colour matching functions of spectral bin k: the reference's
+// kXYZTable (Core/SpectrumTabData.h:7-55), with its step kXYZStep = 470/95
+// (:57) and normalisation kCIENormalizationY (Core/Spectrum.h:286).  The data
+// header is generated by craystal_amd/csrc/make_cie_table.py from the CIE's
+// 1 nm table as the reference's Tools/spectrum.py resampled it; the values
+// are pinned against the standard observer in tests/test_oracle_pins.py.
+inline void cieXYZ(int k, double out[3]) {
+    for (int j = 0; j < 3; ++j) out[j] = double(crh::kCieXYZ[j][k]);
+}
+// BroadSpectrum::toXYZ (Core/Spectrum.cpp:37-45): float accumulation in bin
+// order, then * (kCIENormalizationY * kXYZStep).  Z reads z-bar (the
+// reference reads x-bar, :40, a bug fixed here on purpose).
+inline void spectrumToXYZ(const float* s, float out[3]) {
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < 95; ++i)
+        for (int j = 0; j < 3; ++j) acc[j] += crh::kCieXYZ[j][i] * s[i];
+    const float k = crh::kCieNormY * crh::kCieXYZStep;
+    for (int j = 0; j < 3; ++j) out[j] = acc[j] * k;
 }
 
 struct Scene {
@@ -813,8 +823,16 @@ struct BSDF {
     }
 
     // --- Principled (PrincipledBRDF.cu:9-151)
+    // ORC_MUTANT (0 = the restatement) plants one known mistake for the
+    // tests that show the Principled closed forms catch it: 1 drops the
+    // second 4*ci*cv division, 2 drops the Disney G remap, 3 un-inverts
+    // the f0 lerp, 4 drops the 2 of fd90, 5 uses alpha = r (not r^2) in D.
     static float ggxD(float r, float ch) {  // :9-18
+#if ORC_MUTANT == 5
+        float a = r;
+#else
         float a = r * r;
+#endif
         float a2 = a * a;
         float c2 = ch * ch;
         float den = float(1.0 + (double(a2) - 1.0) * double(c2));
@@ -826,24 +844,40 @@ struct BSDF {
         return 1 / (cv + std::sqrt(a + b - a * b));
     }
     static float smithG(float r, float ci, float cv) {  // :26-33
+#if ORC_MUTANT == 2
+        float a = r;
+#else
         float a = float(0.5 + double(r) * 0.5);
+#endif
         a *= a;
         return smithG1(a, ci) * smithG1(a, cv);
     }
     Spec prDiffuse(float cv, float ci, float cd) const {  // :135-141
+#if ORC_MUTANT == 4
+        float fd90 = float(0.5 + 1.0 * double(rough) * double(cd) * double(cd));
+#else
         float fd90 = float(0.5 + 2.0 * double(rough) * double(cd) * double(cd));
+#endif
         float fl = float(1.0 + (double(fd90) - 1.0) * std::pow(1.0 - double(ci), 5.0));
         float fv = float(1.0 + (double(fd90) - 1.0) * std::pow(1.0 - double(cv), 5.0));
         return kd * kInvPi * fv * fl;
     }
     Spec prSpecular(float cv, float ci, float ch, float cd) const {  // :143-150
         float D = ggxD(rough, ch);
-        Spec f0 = kd * (1.0f - metal) + Spec(0.04f) * metal;
+#if ORC_MUTANT == 3
+        Spec f0 = Spec(0.04f) * (1.0f - metal) + kd * metal;
+#else
+        Spec f0 = kd * (1.0f - metal) + Spec(0.04f) * metal;  // lerp(base, 0.04, m)
+#endif
         float v0 = 1.f - cd;
         float v1 = v0 * v0;
         Spec F = f0 + (Spec(1.f) - f0) * v1 * v1 * v0;  // Fresnel.cu:3-8
         float G = smithG(rough, ci, cv);
+#if ORC_MUTANT == 1
+        float den = 1.0f;
+#else
         float den = float(4.0 * double(ci) * double(cv));
+#endif
         return D * F * G / den;
     }
     Spec prEval(V3 wo, V3 wi) const {  // :57-73
@@ -1278,14 +1312,12 @@ Scene* makeScene(const crh_scene_desc& d) {
             s->avgKd.push_back(a / 95.0f);
             s->avgKs.push_back(b / 95.0f);
         }
-        double ysum = 0.0, c[3];
+        // Hero-bin film weight of one of a path's three bins: the XYZ sum's
+        // factor (kCIENormalizationY * kXYZStep) times 95 / 3.
+        const double k = double(crh::kCieNormY * crh::kCieXYZStep) * 95.0 / 3.0;
+        double c[3];
         for (int i = 0; i < 95; ++i) {
-            cieXYZ(360.0 + 5.0 * i, c);
-            ysum += c[1] * 5.0;
-        }
-        const double k = (1.0 / ysum) * 5.0 * 95.0 / 3.0;
-        for (int i = 0; i < 95; ++i) {
-            cieXYZ(360.0 + 5.0 * i, c);
+            cieXYZ(i, c);
             for (int j = 0; j < 3; ++j) s->filmW[i][j] = float(c[j] * k);
         }
     }
@@ -1548,10 +1580,11 @@ int orc_max_threads(void) { return omp_get_max_threads(); }
 void orc_cie_cmf(float* xyz) {
     double c[3];
     for (int i = 0; i < 95; ++i) {
-        cieXYZ(360.0 + 5.0 * i, c);
+        cieXYZ(i, c);
         for (int j = 0; j < 3; ++j) xyz[3 * i + j] = float(c[j]);
     }
 }
+void orc_spectrum_to_xyz(const float* s, float* xyz) { spectrumToXYZ(s, xyz); }
 
 // One sphere test (SphereSOA::intersect + intersectShape's range test) on a
 // one-sphere scene; out = (t, u, v).  Returns 1 on a hit.

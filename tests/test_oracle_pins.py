@@ -518,8 +518,9 @@ def test_integrator_direct_light_double_count(orc, mis):
 
 
 # ---------------------------------------------------------------------------
-# Spectral transport (an addition to the reference): CIE fit, XYZ normalisation and
-# the reduction of flat spectra to the (reference-pinned) RGB estimator.
+# Spectral transport (an addition to the reference): the CIE table and XYZ sum
+# of the reference's BroadSpectrum (Core/Spectrum.cpp:37-45, SpectrumTabData.h),
+# and the reduction of flat spectra to the (reference-pinned) RGB estimator.
 # ---------------------------------------------------------------------------
 def _oracle_cmf(orc):
     out = np.zeros((95, 3), np.float32)
@@ -527,35 +528,57 @@ def _oracle_cmf(orc):
     return out
 
 
-def test_cie_fit_known_values_and_product_table(orc):
-    c = _oracle_cmf(orc)
-    lam = 360 + 5 * np.arange(95)
-    # CIE 1931 2-degree observer: y(555) = 1.0, x(600) = 1.0622, z(445) = 1.7826
-    assert abs(c[lam == 555][0, 1] - 1.0) < 0.01
-    assert abs(c[lam == 600][0, 0] - 1.0622) < 0.02
-    assert abs(c[lam == 445][0, 2] - 1.7826) < 0.06
-    # peaks where the standard observer has them
-    assert lam[np.argmax(c[:, 1])] in (555, 560)
-    assert lam[np.argmax(c[:, 0])] in (595, 600, 605)
-    assert lam[np.argmax(c[:, 2])] in (440, 445, 450)
-    # equal-energy white sits at x = y = 1/3 in CIE 1931: X, Y, Z integrals agree to 3 %
+def _oracle_xyz(orc, s):
+    s = np.ascontiguousarray(s, np.float32)
+    out = np.zeros(3, np.float32)
+    orc.lib().orc_spectrum_to_xyz(s.ctypes.data, out.ctypes.data)
+    return out
+
+
+# Table spacing: the reference's generator sampled 361..830 nm at 95 points.
+TABLE_LAMBDA = 361.0 + np.arange(95) * 469.0 / 94.0
+
+
+def test_cie_table_is_the_standard_observer(orc):
+    """The table against the CIE 1931 2-degree observer's published features:
+    y-bar peaks at 555 nm with 1.0, x-bar at 599 nm with 1.0622, z-bar at 446 nm
+    with 1.7830; integral of y-bar (1 nm sum) = 106.857, the reference's
+    normalisation; equal-energy white has X = Y = Z."""
+    c = _oracle_cmf(orc).astype(np.float64)
+    lam = TABLE_LAMBDA
+    assert abs(lam[np.argmax(c[:, 1])] - 555) < 3 and abs(c[:, 1].max() - 1.0) < 1e-3
+    assert abs(lam[np.argmax(c[:, 0])] - 599) < 3 and abs(c[:, 0].max() - 1.0622) < 2e-3
+    assert abs(lam[np.argmax(c[:, 2])] - 446) < 3 and abs(c[:, 2].max() - 1.7830) < 2e-3
+    assert c.min() >= 0.0 and c[-1].max() < 1e-5 and c[0].max() < 1e-3
     tot = c.sum(0)
-    assert np.all(np.abs(tot / tot[1] - 1.0) < 0.03)
-    # the product's table is the same fit
-    assert np.array_equal(cr.cie_cmf(), c)
+    assert np.all(np.abs(tot / tot[1] - 1.0) < 2e-3)
+    assert abs(tot[1] * 469.0 / 94.0 - 106.857) < 0.05
+    # the product's table is the same data
+    assert np.array_equal(cr.cie_cmf(), c.astype(np.float32))
 
 
-def test_spectrum_to_xyz_normalisation_and_linearity():
+def test_spectrum_to_xyz_reference_normalisation_and_z_fix(orc):
+    """XYZ = table sum * kXYZStep (470/95) * kCIENormalizationY (1/106.857).
+    The table is spaced 469/94 nm, so a flat spectrum 1 gets
+    Y = (470/95) / (469/94) = 0.99158 of the 1 nm integral's 1.  A spectrum
+    at bin 17 (446 nm) alone has Z/X = z-bar/x-bar = 5.14 there; the
+    reference's x-bar-for-Z bug would give 1."""
     one = np.ones(95, np.float32)
     xyz = cr.spectrum_to_xyz(one)
-    assert abs(xyz[1] - 1.0) < 1e-6
+    assert abs(xyz[1] - (470 / 95) / (469 / 94)) < 1e-4
+    assert np.allclose(xyz, xyz[1], rtol=2e-3)               # equal-energy white
+    assert np.array_equal(xyz, _oracle_xyz(orc, one))
+    d = np.zeros(95, np.float32)
+    d[17] = 2.0
+    xb = cr.spectrum_to_xyz(d)
+    assert abs(xb[2] / xb[0] - 1.782902 / 0.346678) < 1e-4
+    assert abs(xb[1] - 2.0 * 0.031038 * (470 / 95) / 106.8569131841719) < 1e-7
     rng = np.random.default_rng(4)
     a, b = rng.random(95).astype(np.float32), rng.random(95).astype(np.float32)
     lhs = cr.spectrum_to_xyz(2.0 * a + 3.0 * b)
     rhs = 2.0 * cr.spectrum_to_xyz(a) + 3.0 * cr.spectrum_to_xyz(b)
     assert np.allclose(lhs, rhs, rtol=1e-5)
-    c = cr.cie_cmf().astype(np.float64)
-    assert np.allclose(xyz, c.sum(0) / c[:, 1].sum(), rtol=1e-5)
+    assert np.array_equal(cr.spectrum_to_xyz(a), _oracle_xyz(orc, a))
 
 
 def test_flat_spectra_reduce_to_the_rgb_estimator(orc):

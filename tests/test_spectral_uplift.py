@@ -75,4 +75,6 @@ def test_homogeneous_and_continuous_across_orderings():
 def test_white_point_is_the_flat_spectrum():
     w = cr.spectrum_to_xyz(cr.rgb_to_spectrum((1.0, 1.0, 1.0)))
     assert np.allclose(w, cr.spectrum_to_xyz(np.ones(95, np.float32)))
-    assert abs(w[1] - 1.0) < 1e-6
+    # Y of the flat 1 under the reference's normalisation: kXYZStep (470/95)
+    # over the table's spacing (469/94)
+    assert abs(w[1] - (470 / 95) / (469 / 94)) < 1e-4
