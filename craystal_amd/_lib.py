@@ -172,6 +172,8 @@ SIGNATURES = {
     "crh_trace_any": [_P, _P, C.c_uint64, _P],
     "crh_trace_closest_device": [_P, _P, C.c_uint64, _P, C.c_int, C.POINTER(crh_stats)],
     "crh_trace_any_device": [_P, _P, C.c_uint64, _P, C.POINTER(crh_stats)],
+    "crh_trace_closest_device_noalloc": [_P, _P, C.c_uint64, _P, C.c_int,
+                                         C.POINTER(crh_stats)],
     "crh_film_create": [_P, u32, u32, C.POINTER(_P)],
     "crh_film_clear": [_P],
     "crh_film_readback": [_P, _P, _P],

@@ -26,6 +26,7 @@
 #include "../../include/crayhip.h"
 #include "bvh_build.h"
 #include "kernels.h"
+#include "noalloc.h"
 #include "spectral.h"
 #include "wavefront.h"
 #include "guiding_kernels.h"
@@ -394,6 +395,8 @@ int crh_context_create(int device, crh_context** out) {
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->counter), 256);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    // The allocation-free entry point's scratch: this device's address of it.
+    if (e == hipSuccess) e = crh::noallocScratchInit(device);
     if (e != hipSuccess) {
         crh_context_destroy(c);
         return hipFail("context setup", e, __FILE__, __LINE__);
@@ -928,6 +931,39 @@ int crh_trace_closest_device(crh_scene* s, const crh_buffer* rays, uint64_t n,
     if (stats) {
         stats->node_visits = nodes;
         stats->tri_tests = tris;
+    }
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_trace_closest_device_noalloc(crh_scene* s, const crh_buffer* rays, uint64_t n,
+                                     crh_buffer* hits, int count_work, crh_stats* stats) {
+    ARG_CHECK(s && rays && hits, "NULL argument");
+    ARG_CHECK(rays->mem.bytes >= n * sizeof(crh_ray), "ray buffer too small");
+    ARG_CHECK(hits->mem.bytes >= n * sizeof(crh_hit), "hit buffer too small");
+    ARG_CHECK(n < (uint64_t(1) << 40), "ray count too large");
+    GUARD_BEGIN
+    crh_context* ctx = s->ctx;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    double t0 = nowMs();
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    unsigned long long totals[2] = {0, 0};
+    HIP_TRY(crh::traceClosestNoAlloc(ctx->device, s->view, rays->mem.as<crh_ray>(), n,
+                                     hits->mem.as<crh_hit>(), count_work != 0,
+                                     effective(ctx->tuning, s->view), ctx->stream, totals));
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    if (stats) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        stats->kernel_ms = ms;
+        stats->main_kernel_ms = ms;
+        stats->total_ms = nowMs() - t0;
+        stats->rays = n;
+        stats->node_visits = totals[0];
+        stats->tri_tests = totals[1];
+        stats->launches = 1;
     }
     return CRH_OK;
     GUARD_END

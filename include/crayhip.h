@@ -361,6 +361,20 @@ int crh_trace_closest_device(crh_scene* scene, const crh_buffer* rays,
                              crh_stats* stats);
 int crh_trace_any_device(crh_scene* scene, const crh_buffer* rays, uint64_t n,
                          crh_buffer* occluded, crh_stats* stats);
+/* No counterpart in the reference.  crh_trace_closest_device for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments and results, but it calls
+ * no hipMalloc / hipFree and no allocating library routine.  Its scratch (the
+ * persistent kernel's counter, the per-ray work counts and their totals) is
+ * one fixed device array of 8 MiB + 256 B per GPU, allocated when the library
+ * loads and held until the process ends; the work counts of longer batches
+ * are taken in chunks of 2^20 rays, so n is limited only as in
+ * crh_trace_closest_device (n < 2^40).  Every call on a GPU shares that
+ * array: calls that overlap on one GPU (two host threads, or two contexts on
+ * one device) race. */
+int crh_trace_closest_device_noalloc(crh_scene* scene, const crh_buffer* rays,
+                                     uint64_t n, crh_buffer* hits, int count_work,
+                                     crh_stats* stats);
 
 /* ---- film (Sensor / SensorData, Core/Sensor.{h,cu}) -------------------- */
 int crh_film_create(crh_context* ctx, uint32_t width, uint32_t height,
