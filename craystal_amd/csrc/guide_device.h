@@ -70,7 +70,10 @@ CRH_D float actualPdfLocal(const BSDF& b, F3 wo, F3 wi) {
     if (b.type == 1u) {
         if (wo.z <= 0.0f) return 0.f;
         float spec = 0.f;
+        // The half vector the sampler drew is the one of +-(wo + wi) with
+        // positive z (the other sign maps to the same wi, below the horizon).
         F3 h = normalize(wo + wi);
+        if (h.z < 0.f) h = h * -1.f;
         float cd = dot(wo, h);
         if (h.z > 0.f && fabsf(cd) > 0.f) spec = ggxD(b.rough, h.z) * h.z / (4.f * fabsf(cd));
         return b.pS * spec + (1.f - b.pS) * cosHemiPdf(wi);

@@ -57,6 +57,17 @@ def lib() -> C.CDLL:
             "orc_render_samples": (u64, [P, P, u32, u32, P, u32, P, P, C.c_int]),
             "orc_render": (u64, [P, P, u32, u32, P, u32, u32, P, P, C.c_int]),
             "orc_max_threads": (C.c_int, []),
+            "orc_render_guided": (u64, [P, P, u32, u32, P, P, f32, u32, u32, C.c_int, u32, u32,
+                                        P, P, C.c_int]),
+            "orc_render_samples_guided": (u64, [P, P, u32, u32, P, P, f32, u32, u32, C.c_int,
+                                                u32, P, P, C.c_int]),
+            "orc_mixture": (None, [P, P]),
+            "orc_mixture_pdf": (f32, [P, P]),
+            "orc_mixture_sample": (None, [P, P, P]),
+            "orc_actual_pdf_local": (f32, [P, P, P]),
+            "orc_guided_sample_local": (None, [P, P, f32, P, P, P, P, P]),
+            "orc_bsdf_sample_dirs": (None, [P, P, P, u64, P]),
+            "orc_actual_pdf_batch": (None, [P, P, P, u64, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(_lib, name)
@@ -132,6 +143,33 @@ class Scene:
         c = cfg.to_c()
         rays = lib().orc_render_samples(self.h, _ref(cam.getData()), W, H, _ref(c),
                                         sample_index, _p(Lr), _p(P), threads)
+        return Lr, P, rays
+
+    def render_guided(self, cam, size, cfg, out48, first, spp, alpha=0.5, mask=3,
+                      max_depth=0, bsdf_as_guide=False, threads=0):
+        """Guided film: out48 = the guide's 48 outputs used at every vertex
+        (None with bsdf_as_guide: the proposal is the BSDF's own sampler)."""
+        W, H = size
+        s = np.zeros((H, W, 3), np.float32)
+        w = np.zeros((H, W), np.float32)
+        c = cfg.to_c()
+        o = None if out48 is None else np.ascontiguousarray(out48, np.float32)
+        rays = lib().orc_render_guided(self.h, _ref(cam.getData()), W, H, _ref(c), _p(o),
+                                       alpha, mask, max_depth, 1 if bsdf_as_guide else 0,
+                                       first, spp, _p(s), _p(w), threads)
+        return s, w, rays
+
+    def render_samples_guided(self, cam, size, cfg, out48, sample_index, alpha=0.5, mask=3,
+                              max_depth=0, bsdf_as_guide=False, threads=0):
+        W, H = size
+        Lr = np.zeros((H, W, 3), np.float32)
+        P = np.zeros((H, W, 2), np.float32)
+        c = cfg.to_c()
+        o = None if out48 is None else np.ascontiguousarray(out48, np.float32)
+        rays = lib().orc_render_samples_guided(self.h, _ref(cam.getData()), W, H, _ref(c), _p(o),
+                                               alpha, mask, max_depth,
+                                               1 if bsdf_as_guide else 0, sample_index, _p(Lr),
+                                               _p(P), threads)
         return Lr, P, rays
 
     def render(self, cam, size, cfg, first, spp, threads=0):
@@ -223,3 +261,65 @@ def film_splat(sum_, wsum, L3, px, py):
     H, W = wsum.shape
     L3 = np.ascontiguousarray(L3, np.float32)
     lib().orc_film_splat(_p(sum_), _p(wsum), W, H, _p(L3), px, py)
+
+
+def mixture(out48):
+    """(weights (8), means (8, 3), kappas (8)) of the guide head."""
+    o = np.ascontiguousarray(out48, np.float32)
+    m = np.zeros(40, np.float32)
+    lib().orc_mixture(_p(o), _p(m))
+    return m[:8], m[8:32].reshape(8, 3), m[32:40]
+
+
+def mixture_pdf(out48, d):
+    o = np.ascontiguousarray(out48, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    return float(lib().orc_mixture_pdf(_p(o), _p(d)))
+
+
+def mixture_sample(out48, u3):
+    o = np.ascontiguousarray(out48, np.float32)
+    u = np.ascontiguousarray(u3, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().orc_mixture_sample(_p(o), _p(u), _p(d))
+    return d
+
+
+def actual_pdf_local(mat, wo, wi):
+    m = mat.to_c()
+    wo = np.ascontiguousarray(wo, np.float32)
+    wi = np.ascontiguousarray(wi, np.float32)
+    return float(lib().orc_actual_pdf_local(_ref(m), _p(wo), _p(wi)))
+
+
+def guided_sample_local(mat, out48, alpha, wo, u4):
+    """One guided continuation (identity frame) from the uniforms u4; out48
+    None = the BSDF as the proposal.  Returns (wi, f * |cos|, pdf)."""
+    m = mat.to_c()
+    o = None if out48 is None else np.ascontiguousarray(out48, np.float32)
+    wo = np.ascontiguousarray(wo, np.float32)
+    u = np.ascontiguousarray(u4, np.float32)
+    wi = np.zeros(3, np.float32)
+    f = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    lib().orc_guided_sample_local(_ref(m), _p(o), alpha, _p(wo), _p(u), _p(wi), _p(f), _p(pdf))
+    return wi, f, float(pdf[0])
+
+
+def bsdf_sample_dirs(mat, wo, u):
+    """Directions of BSDF::sampleLocal for the uniforms u (n, 2)."""
+    m = mat.to_c()
+    wo = np.ascontiguousarray(wo, np.float32)
+    u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+    wi = np.zeros((len(u), 3), np.float32)
+    lib().orc_bsdf_sample_dirs(_ref(m), _p(wo), _p(u), len(u), _p(wi))
+    return wi
+
+
+def actual_pdf_batch(mat, wo, wi):
+    m = mat.to_c()
+    wo = np.ascontiguousarray(wo, np.float32)
+    wi = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    out = np.zeros(len(wi), np.float32)
+    lib().orc_actual_pdf_batch(_ref(m), _p(wo), _p(wi), len(wi), _p(out))
+    return out

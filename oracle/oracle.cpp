@@ -27,6 +27,14 @@
 //   Textures           bilinear closed form, channel fill, saturating casts
 //   RGB uplift         colour round trip through the CIE table, bounds,
 //                      homogeneity, continuity across channel orderings
+//   Principled eval    hand-derived closed forms (r = 1, normal incidence,
+//                      grazing views) that fail for each of five planted
+//                      mistakes (ORC_MUTANT)
+//   CIE table          the standard observer's peaks and integrals; Y of a
+//                      flat spectrum = (470/95) / (469/94)
+//   Guided estimator   the sampler's actual density vs histograms; mixture
+//                      vs the float64 model; a guide equal to the BSDF gives
+//                      the plain weight; closed-form and plain-image means
 // Parity unpinned at bit level: glm's exact operation order beyond the
 // camera KAT tolerance and CUDA libdevice transcendental results.
 #include <omp.h>
@@ -970,6 +978,160 @@ struct BSDF {
 };
 
 // ---------------------------------------------------------------------------
+// Guided continuation (an addition with no reference counterpart): one-sample
+// MIS of the reference's BSDF sampling (fraction alpha) and a proposal, the
+// guide's 8-lobe von Mises-Fisher mixture, as the product's guided path does
+// it (craystal_amd/csrc/wavefront.hip kWfShadeCont, guide_device.h).
+// Pinned by: the actual density of each BSDF sampling procedure against
+// histograms of its samples; the mixture against the float64 model of
+// oracle/guiding.py (itself pinned by quadrature and histograms); a guide
+// equal to the BSDF's own procedure giving the plain estimator's weight for
+// every direction; guided images against closed forms and plain images.
+// ---------------------------------------------------------------------------
+struct Mixture {
+    float w[8];
+    V3 mu[8];
+    float k[8];
+};
+
+// The network's head from its 48 outputs (5 per lobe: weight logit, mean,
+// kappa): softmax weights, unit means (length floored at 1e-6), kappa =
+// softplus + 0.01.
+Mixture mixtureFromOutputs(const float* o) {
+    Mixture m;
+    float amax = -INFINITY;
+    for (int j = 0; j < 8; ++j) amax = std::max(amax, o[5 * j]);
+    float sum = 0.f;
+    for (int j = 0; j < 8; ++j) {
+        m.w[j] = std::exp(o[5 * j] - amax);
+        sum += m.w[j];
+    }
+    for (int j = 0; j < 8; ++j) {
+        m.w[j] /= sum;
+        float mx = o[5 * j + 1], my = o[5 * j + 2], mz = o[5 * j + 3];
+        float len = std::max(std::sqrt(mx * mx + my * my + mz * mz), 1e-6f);
+        m.mu[j] = V3(mx / len, my / len, mz / len);
+        float x = o[5 * j + 4];
+        m.k[j] = (x > 20.f ? x : std::log1p(std::exp(std::min(x, 20.f)))) + 0.01f;
+    }
+    return m;
+}
+
+// vMF density k / (2 pi (1 - e^{-2k})) exp(k (mu.d - 1)) (stable form).
+float vmfDensity(float k, float c) {
+    return k / (2.f * kPi * (-std::expm1(-2.f * k))) * std::exp(k * (c - 1.f));
+}
+float mixturePdf(const Mixture& m, V3 d) {
+    float q = 0.f;
+    for (int j = 0; j < 8; ++j) q += m.w[j] * vmfDensity(m.k[j], dot(m.mu[j], d));
+    return q;
+}
+// Lobe by u0 against the weights' CDF (the last lobe if rounding leaves u0
+// above it); direction by W. Jakob's stable inversion (2012) from (u1, u2).
+V3 mixtureSample(const Mixture& m, float u0, float u1, float u2) {
+    int j = 7;
+    float c = 0.f;
+    for (int i = 0; i < 8; ++i) {
+        c += m.w[i];
+        if (u0 < c) {
+            j = i;
+            break;
+        }
+    }
+    float k = m.k[j];
+    V3 mu = m.mu[j];
+    float w = 1.f + std::log(u1 + (1.f - u1) * std::exp(-2.f * k)) / k;
+    w = std::min(std::max(w, -1.f), 1.f);
+    float sn = std::sqrt(std::max(0.f, 1.f - w * w));
+    V3 a = std::fabs(mu.x) > 0.9f ? V3(0, 1, 0) : V3(1, 0, 0);
+    V3 t = normalize(cross(a, mu));
+    V3 b = cross(mu, t);
+    float phi = 2.f * kPi * u2;
+    return normalize(mu * w + (t * std::cos(phi) + b * std::sin(phi)) * sn);
+}
+
+// Density of the directions BSDF::sampleLocal produces (which the reference
+// reports only for Lambertian): Principled picks the GGX half-vector lobe
+// with probability pS (PrincipledBRDF.cu:112), whose reflected direction has
+// density D(h) cos(theta_h) / (4 |wo.h|), else the cosine lobe.
+float actualPdfLocal(const BSDF& b, V3 wo, V3 wi) {
+    if (b.type == CRH_MAT_DIFFUSE) return cosineHemispherePdf(wi);
+    if (b.type == CRH_MAT_PRINCIPLED) {
+        if (wo.z <= 0.0f) return 0.f;
+        float spec = 0.f;
+        // The half vector the sampler drew is the one of +-(wo + wi) with
+        // positive z (the other sign maps to the same wi).
+        V3 h = normalize(wo + wi);
+        if (h.z < 0.f) h = -h;
+        float cd = dot(wo, h);
+        if (h.z > 0.f && std::fabs(cd) > 0.f)
+            spec = BSDF::ggxD(b.rough, h.z) * h.z / (4.f * std::fabs(cd));
+        return b.pS * spec + (1.f - b.pS) * cosineHemispherePdf(wi);
+    }
+    return 0.f;  // Phong is never guided
+}
+
+struct Guide {
+    Mixture mix;
+    float alpha = 0.5f;     // probability of the BSDF's own sample
+    uint32_t mask = 3u;     // bit t: guide BSDF type t (0 Lambertian, 1 Principled)
+    uint32_t maxDepth = 0;  // guide vertices at depth < maxDepth (0: all)
+    bool bsdfAsGuide = false;  // the proposal is the BSDF's own procedure
+    bool applies(const BSDF& b, uint32_t depth) const {
+        return b.type != CRH_MAT_PHONG && ((mask >> b.type) & 1u) &&
+               (maxDepth == 0u || depth < maxDepth);
+    }
+};
+
+// One guided continuation.  Draws, in order: the technique (1), then the
+// BSDF's 2D sample, or the lobe (1) and the direction (2) of the proposal
+// (the BSDF-as-guide proposal draws the same three and samples the BSDF
+// from the last two).  Returns f * |cos| at wiL (local; wi is the world
+// direction) and sets the pdf whose ratio f / pdf is the sample's weight:
+// q_mix * p_rep / p_act with q_mix = alpha p_act + (1 - alpha) q_guide, so
+// E[f / pdf] equals the plain estimator's E[f / p_rep] over wi ~ p_act.
+template <class S>
+Spec guidedSample(const BSDF& b, const Guide& g, S& smp, V3 wo, V3& wi, float& pdf) {
+    V3 woL = b.frame.toLocal(wo), wiL;
+    float dummy = 0.f;
+    if (smp.next1D() < g.alpha) {
+        V2 u = smp.next2D();
+        wiL = b.sampleLocal(woL, u, dummy);
+        wi = b.frame.toWorld(wiL);
+    } else {
+        float u0 = smp.next1D();
+        V2 u = smp.next2D();
+        if (g.bsdfAsGuide) {
+            (void)u0;
+            wiL = b.sampleLocal(woL, u, dummy);
+            wi = b.frame.toWorld(wiL);
+        } else {
+            wi = mixtureSample(g.mix, u0, u.x, u.y);
+            wiL = b.frame.toLocal(wi);
+        }
+    }
+    Spec f = b.evalLocal(woL, wiL);
+    float prep = b.pdfLocal(woL, wiL);
+    float pact = actualPdfLocal(b, woL, wiL);
+    float q = g.bsdfAsGuide ? pact : mixturePdf(g.mix, wi);
+    float qmix = g.alpha * pact + (1.f - g.alpha) * q;
+    pdf = (prep > 0.f && pact > 0.f && qmix > 0.f) ? qmix * prep / pact : 0.f;
+    return f;
+}
+
+// Uniforms given by the caller (for the unit pins), in the order drawn.
+struct FixedUniforms {
+    const float* u;
+    int i = 0;
+    float next1D() { return u[i++]; }
+    V2 next2D() {
+        V2 r(u[i], u[i + 1]);
+        i += 2;
+        return r;
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Camera (Core/Camera.cu:35-39, Core/CameraProxy.h:25-40)
 // ---------------------------------------------------------------------------
 void cameraUpdate(crh_camera& c) {
@@ -1139,9 +1301,10 @@ Spec evalMIS(const Scene& s, const Isect& it, const BSDF& bsdf, Sampler& smp,
 }
 
 // pathTraceKernel body (:142-227) for one (pixel, sample index).
+// guide (optional): the guided continuation at the vertices it applies to.
 Spec tracePath(const Scene& s, const crh_camera& cam, uint32_t W, uint32_t H,
                const crh_integrator_cfg& cfg, uint32_t x, uint32_t y,
-               uint32_t sampleIndex, V2& pixelOut, Tally& tl) {
+               uint32_t sampleIndex, V2& pixelOut, Tally& tl, const Guide* guide = nullptr) {
     Sampler smp(x, y, sampleIndex);
     V2 j = smp.next2D();
     V2 pixel(float(x) + j.x, float(y) + j.y);
@@ -1198,7 +1361,9 @@ Spec tracePath(const Scene& s, const crh_camera& cam, uint32_t W, uint32_t H,
             }
             V3 wi;
             float pdf = 0.0f;
-            Spec f = bsdf.sampleEvaluate(smp, it.viewW, wi, pdf);
+            Spec f = guide && guide->applies(bsdf, depth)
+                         ? guidedSample(bsdf, *guide, smp, it.viewW, wi, pdf)
+                         : bsdf.sampleEvaluate(smp, it.viewW, wi, pdf);
             if (pdf <= 0.0f) break;
             beta = beta * (f / pdf);
             ray.o = offsetRayOrigin(it.posW, it.orientedFaceNormal());
@@ -1533,9 +1698,19 @@ void orc_film_splat(float* sum, float* wsum, uint32_t W, uint32_t H,
 
 // Per-sample radiance of one pass (sample index `si`) for every pixel.
 // L: W*H*3 floats.  Returns rays traced.
+static uint64_t renderSamples(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
+                              const crh_integrator_cfg* cfg, uint32_t si, float* L, float* pix,
+                              int nthreads, const Guide* guide);
+
 uint64_t orc_render_samples(void* sp, const crh_camera* cam, uint32_t W,
                             uint32_t H, const crh_integrator_cfg* cfg,
                             uint32_t si, float* L, float* pix, int nthreads) {
+    return renderSamples(sp, cam, W, H, cfg, si, L, pix, nthreads, nullptr);
+}
+
+static uint64_t renderSamples(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
+                              const crh_integrator_cfg* cfg, uint32_t si, float* L, float* pix,
+                              int nthreads, const Guide* guide) {
     const Scene& s = *(Scene*)sp;
     const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();  // this call only
     uint64_t rays = 0;
@@ -1544,7 +1719,7 @@ uint64_t orc_render_samples(void* sp, const crh_camera* cam, uint32_t W,
         Tally tl;
         for (uint32_t x = 0; x < W; ++x) {
             V2 p;
-            Spec r = tracePath(s, *cam, W, H, *cfg, x, uint32_t(y), si, p, tl);
+            Spec r = tracePath(s, *cam, W, H, *cfg, x, uint32_t(y), si, p, tl, guide);
             size_t id = size_t(W) * y + x;
             for (int c = 0; c < 3; ++c) L[3 * id + c] = r.v[c];
             if (pix) {
@@ -1560,19 +1735,117 @@ uint64_t orc_render_samples(void* sp, const crh_camera* cam, uint32_t W,
 // Full render of sample indices [first, first + spp) into (sum, wsum), which
 // the caller zeroes.  Deterministic: per-pass radiance is computed in
 // parallel, then splatted in row-major pixel order.
-uint64_t orc_render(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
-                    const crh_integrator_cfg* cfg, uint32_t first, uint32_t spp,
-                    float* sum, float* wsum, int nthreads) {
+static uint64_t renderFilm(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
+                           const crh_integrator_cfg* cfg, uint32_t first, uint32_t spp,
+                           float* sum, float* wsum, int nthreads, const Guide* guide) {
     std::vector<float> L(size_t(W) * H * 3), P(size_t(W) * H * 2);
     uint64_t rays = 0;
     for (uint32_t i = 0; i < spp; ++i) {
-        rays += orc_render_samples(sp, cam, W, H, cfg, first + i, L.data(),
-                                   P.data(), nthreads);
+        rays += renderSamples(sp, cam, W, H, cfg, first + i, L.data(), P.data(), nthreads,
+                              guide);
         for (size_t id = 0; id < size_t(W) * H; ++id)
             splat(sum, wsum, W, H, Spec(L[3 * id], L[3 * id + 1], L[3 * id + 2]),
                   V2(P[2 * id], P[2 * id + 1]));
     }
     return rays;
+}
+
+uint64_t orc_render(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
+                    const crh_integrator_cfg* cfg, uint32_t first, uint32_t spp,
+                    float* sum, float* wsum, int nthreads) {
+    return renderFilm(sp, cam, W, H, cfg, first, spp, sum, wsum, nthreads, nullptr);
+}
+
+// ---- guided rendering and its pieces (an addition; see Guide) ---------------
+// out48: the guide network's 48 outputs, one mixture for every vertex (a
+// network whose weights are zero outputs its last bias for any input); NULL
+// with bsdf_as_guide != 0 makes the proposal the BSDF's own procedure.
+static Guide makeGuide(const float* out48, float alpha, uint32_t mask, uint32_t maxDepth,
+                       int bsdfAsGuide) {
+    Guide g;
+    if (out48) g.mix = mixtureFromOutputs(out48);
+    g.alpha = alpha;
+    g.mask = mask;
+    g.maxDepth = maxDepth;
+    g.bsdfAsGuide = bsdfAsGuide != 0 || !out48;
+    return g;
+}
+
+uint64_t orc_render_guided(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
+                           const crh_integrator_cfg* cfg, const float* out48, float alpha,
+                           uint32_t mask, uint32_t maxDepth, int bsdfAsGuide, uint32_t first,
+                           uint32_t spp, float* sum, float* wsum, int nthreads) {
+    const Guide g = makeGuide(out48, alpha, mask, maxDepth, bsdfAsGuide);
+    return renderFilm(sp, cam, W, H, cfg, first, spp, sum, wsum, nthreads, &g);
+}
+
+uint64_t orc_render_samples_guided(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
+                                   const crh_integrator_cfg* cfg, const float* out48,
+                                   float alpha, uint32_t mask, uint32_t maxDepth,
+                                   int bsdfAsGuide, uint32_t si, float* L, float* pix,
+                                   int nthreads) {
+    const Guide g = makeGuide(out48, alpha, mask, maxDepth, bsdfAsGuide);
+    return renderSamples(sp, cam, W, H, cfg, si, L, pix, nthreads, &g);
+}
+
+// Mixture of 48 outputs as 40 floats: weights (8), means (8 x 3), kappas (8).
+void orc_mixture(const float* out48, float* mix40) {
+    Mixture m = mixtureFromOutputs(out48);
+    for (int j = 0; j < 8; ++j) {
+        mix40[j] = m.w[j];
+        for (int c = 0; c < 3; ++c) mix40[8 + 3 * j + c] = m.mu[j][c];
+        mix40[32 + j] = m.k[j];
+    }
+}
+float orc_mixture_pdf(const float* out48, const float* d) {
+    return mixturePdf(mixtureFromOutputs(out48), V3(d[0], d[1], d[2]));
+}
+void orc_mixture_sample(const float* out48, const float* u3, float* d) {
+    V3 r = mixtureSample(mixtureFromOutputs(out48), u3[0], u3[1], u3[2]);
+    for (int c = 0; c < 3; ++c) d[c] = r[c];
+}
+float orc_actual_pdf_local(const crh_material* m, const float* wo, const float* wi) {
+    BSDF b;
+    b.init(*m, Frame());
+    return actualPdfLocal(b, V3(wo[0], wo[1], wo[2]), V3(wi[0], wi[1], wi[2]));
+}
+// Batches for the density pins: n BSDF sample directions from u (2 per
+// sample), and the actual density at n directions.
+void orc_bsdf_sample_dirs(const crh_material* m, const float* wo, const float* u, uint64_t n,
+                          float* wi) {
+    BSDF b;
+    b.init(*m, Frame());
+    const V3 o(wo[0], wo[1], wo[2]);
+    for (uint64_t i = 0; i < n; ++i) {
+        float pdf = 0.f;
+        V3 w = b.sampleLocal(o, V2(u[2 * i], u[2 * i + 1]), pdf);
+        for (int c = 0; c < 3; ++c) wi[3 * i + c] = w[c];
+    }
+}
+void orc_actual_pdf_batch(const crh_material* m, const float* wo, const float* wi, uint64_t n,
+                          float* out) {
+    BSDF b;
+    b.init(*m, Frame());
+    const V3 o(wo[0], wo[1], wo[2]);
+    for (uint64_t i = 0; i < n; ++i)
+        out[i] = actualPdfLocal(b, o, V3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]));
+}
+// One guided continuation in the local frame (identity frame) from the
+// uniforms u (4 floats, drawn in order); returns wi, f * |cos| and the pdf.
+void orc_guided_sample_local(const crh_material* m, const float* out48, float alpha,
+                             const float* wo, const float* u, float* wi, float* f, float* pdf) {
+    BSDF b;
+    b.init(*m, Frame());
+    const Guide g = makeGuide(out48, alpha, 3u, 0u, 0);
+    FixedUniforms fu{u};
+    V3 w;
+    float p = 0.f;
+    Spec r = guidedSample(b, g, fu, V3(wo[0], wo[1], wo[2]), w, p);
+    for (int c = 0; c < 3; ++c) {
+        wi[c] = w[c];
+        f[c] = r.v[c];
+    }
+    *pdf = p;
 }
 
 int orc_max_threads(void) { return omp_get_max_threads(); }

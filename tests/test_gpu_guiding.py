@@ -213,3 +213,61 @@ def test_trained_guide_lowers_variance(gpu):
     # unbiased: the image means agree within 3 sigma of their spread
     se = np.sqrt(m_plain.var(ddof=1) / len(m_plain) + m_guided.var(ddof=1) / len(m_guided))
     assert abs(m_plain.mean() - m_guided.mean()) <= 3 * se, (m_plain.mean(), m_guided.mean(), se)
+
+
+def _frozen_outputs(seed=4):
+    """48 outputs of a frozen guide: distinct lobes, kappa logits in [-1, 4]."""
+    rng = np.random.default_rng(seed)
+    o = rng.normal(size=48) * 1.5
+    for j in range(8):
+        o[5 * j + 4] = rng.uniform(-1.0, 4.0)
+    return o.astype(np.float32)
+
+
+@pytest.mark.parametrize("mis", [True, False])
+def test_guided_film_matches_oracle_with_a_frozen_guide(gpu, orc, mis):
+    """The guided estimator against the oracle's restatement: a frozen guide
+    (all weights zero, so every vertex gets the mixture of the last layer's
+    bias) on the Cornell box with Principled walls and boxes and Lambertian
+    side walls, one-sample MIS alpha 0.5, guiding every vertex.  Same bar as
+    the plain films: weights bit-equal, the large majority of pixels equal
+    to 1e-4 relative, relMSE small."""
+    import craystal_amd as cr
+    from craystal_amd import scenes
+    from craystal_amd.guiding import GuidedIntegrator
+    data, camd = scenes.cornell_box()
+    data.materials[0] = scenes.principled((0.7, 0.6, 0.5), 0.35, 0.6)
+    out48 = _frozen_outputs()
+    theta = np.zeros(L.GUIDE_PARAMS, np.float32)
+    theta[-L.GUIDE_OUTPUTS:] = out48                    # the output layer's bias
+    P = np.array(data.concatenated()[0])
+    guide = Guide(P.min(0), P.max(0))
+    guide.set_params(theta)
+    pos, nrm, wo, _, _ = _inputs(200)
+    raw, mix = guide.eval(pos, nrm, wo)
+    assert np.array_equal(raw, np.tile(out48, (200, 1)))   # frozen: the bias everywhere
+    w, mu, kap = orc.mixture(out48)
+    assert np.allclose(mix[:, :8], w, atol=1e-6) and np.allclose(mix[:, 32:], kap, rtol=1e-6)
+    assert np.allclose(mix[:, 8:32].reshape(-1, 8, 3), mu, atol=1e-6)
+    size = (48, 48)
+    cfg = cr.Configs(maxDepth=8, rrThreshold=0.05, useMIS=mis, rrStartDepth=3,
+                     mode=L.MODE_WAVEFRONT)
+    scene = cr.Scene(data)
+    sensor = cr.Sensor(size)
+    cam = scenes.make_camera(camd, sensor)
+    scene.setCamera(cam)
+    gi = GuidedIntegrator(cfg, guide, alpha=0.5)
+    gi.dispatch(scene, 8, first_sample=0, train=False)
+    osc = orc.Scene(data)
+    s, wsum, _ = osc.render_guided(cam, size, cfg, out48, 0, 8, alpha=0.5)
+    assert np.array_equal(sensor.weight.view(np.uint32), wsum.view(np.uint32))
+    g = sensor.createImage()
+    o = s / wsum[..., None]
+    assert np.isfinite(g).all()
+    close = np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1e-3)
+    assert close.mean() > 0.9, f"only {close.mean():.3f} of values match to 1e-4"
+    rel = float(np.mean((g - o) ** 2 / (o ** 2 + 1e-2)))
+    assert rel < 1e-3, rel
+    # the guided image differs from the plain one (the guide is in use)
+    s2, w2, _ = osc.render(cam, size, cfg, 0, 8)
+    assert not np.allclose(o, s2 / w2[..., None], rtol=1e-3)
