@@ -74,10 +74,10 @@ def build_cpp_tests() -> Path:
     root = PKG.parent
     outdir = root / "tests" / "cpp" / "_build"
     outdir.mkdir(parents=True, exist_ok=True)
-    for name in ("api_test", "import_test"):
+    for name in ("api_test", "import_test", "reference_shape_test"):
         _run(["g++", "-O1", "-std=c++17", f"-I{root / 'include'}",
               str(root / "tests" / "cpp" / f"{name}.cpp"), f"-L{PKG}", "-lcrayhip",
-              f"-Wl,-rpath,{PKG}", "-o", str(outdir / name)])
+              f"-Wl,-rpath,{PKG}", "-lz", "-o", str(outdir / name)])
     return outdir / "api_test"
 
 
@@ -85,7 +85,7 @@ def build_cli() -> Path:
     """The crayli command line renderer next to libcrayhip.so."""
     out = PKG / "crayli"
     _run(["g++", "-O2", "-std=c++17", f"-I{PKG.parent / 'include'}", str(CSRC / "crayli.cpp"),
-          f"-L{PKG}", "-lcrayhip", "-Wl,-rpath,$ORIGIN", "-o", str(out)])
+          f"-L{PKG}", "-lcrayhip", "-Wl,-rpath,$ORIGIN", "-lz", "-o", str(out)])
     return out
 
 

@@ -42,8 +42,8 @@ int main(int argc, char** argv) {
         cfg.useMIS = !noMIS;
         CRay::PathTraceIntegrator integrator(cfg);
         crh_stats st = integrator.dispatch(*scene, spp);
-        auto sensor = scene->getCamera()->getSensor();
-        CRay::writeEXR(out, sensor->getSize()[0], sensor->getSize()[1], sensor->createImage());
+        // crayRenderSample (CRayli.cpp:7-14): the sensor's image as EXR
+        scene->getCamera()->getSensor()->createImage()->writeEXR(out);
         std::printf("wrote %s: %d spp, %.1f ms, %llu rays\n", out.c_str(), spp, st.kernel_ms,
                     static_cast<unsigned long long>(st.rays));
     } catch (const std::exception& e) {

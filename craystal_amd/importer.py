@@ -21,7 +21,10 @@ Rules kept from the reference:
   GenNormals steps); identical (position, uv, normal) corners are joined.
 Meshes: a new mesh starts at every `o`/`g` statement and every material
 change, in file order (assimp splits an OBJ per object/group and material).
-Textures (map_Kd) are outside this path: a model that uses one is rejected.
+A map_Kd diffuse texture (its last token, relative to the scene file) is
+read by imageio.load_image (PNG or EXR, as Image::load) into
+SceneData.textureImages, and its material gets hasDiffuseTexture and the
+image's index (Importer.cpp:113-120).
 """
 from __future__ import annotations
 
@@ -31,7 +34,9 @@ from pathlib import Path
 
 import numpy as np
 
-from .core import Camera, MaterialData, MaterialType, MeshData, Scene, SceneData, Sensor
+from .core import (Camera, MaterialData, MaterialFlags, MaterialType, MeshData, Scene, SceneData,
+                   Sensor)
+from .imageio import load_image
 
 
 class ImportError_(ValueError):
@@ -86,8 +91,8 @@ def parse_mtl(text: str) -> dict[str, dict]:
             cur[key] = float(rest[0])
         elif key == "Tr":
             cur["d"] = 1.0 - float(rest[0])
-        elif key.startswith("map_Kd"):
-            cur["map_Kd"] = " ".join(rest)
+        elif key == "map_Kd" and rest:
+            cur["map_Kd"] = rest[-1]
     return mats
 
 
@@ -145,17 +150,17 @@ def parse_obj(text: str, mtl_loader=None):
             np.asarray(VN, np.float64).reshape(-1, 3), meshes, mtl)
 
 
-def build_scene_data(obj_text: str, lights: dict, mtl_loader=None) -> SceneData:
+def build_scene_data(obj_text: str, lights: dict, mtl_loader=None,
+                     image_loader=None) -> SceneData:
+    """image_loader(name) -> (H, W, C) texture image of a map_Kd name."""
     V, VT, VN, meshes, mtl = parse_obj(obj_text, mtl_loader)
     names: list[str] = []
     for m in meshes:
         if m["material"] not in names:
             names.append(m["material"])
-    materials = []
+    materials, textures = [], []
     for name in names:
         d = mtl.get(name, {}) if name is not None else {}
-        if "map_Kd" in d:
-            raise ImportError_(f"material {name!r}: textures are not supported")
         md = MaterialData(diffuseRefl=d.get("Kd", (0.0, 0.0, 0.0)),
                           specularRefl=d.get("Ks", (0.0, 0.0, 0.0)),
                           transmittance=((1.0 - d["d"],) * 3) if "d" in d else (0.0, 0.0, 0.0),
@@ -163,6 +168,12 @@ def build_scene_data(obj_text: str, lights: dict, mtl_loader=None) -> SceneData:
                           type=MaterialType.Phong,
                           emission=lights.get(name, (0.0, 0.0, 0.0)) if name is not None
                           else (0.0, 0.0, 0.0))
+        if "map_Kd" in d:      # Utils/Importer.cpp:113-120
+            if image_loader is None:
+                raise ImportError_(f"material {name!r}: no image loader for its texture")
+            md.flags |= int(MaterialFlags.hasDiffuseTexture)
+            md.diffuseHandle = len(textures)
+            textures.append(image_loader(d["map_Kd"]))
         materials.append(md.finalize())
     out_meshes, emissive, prim = [], [], 0
     for m in meshes:
@@ -197,7 +208,7 @@ def build_scene_data(obj_text: str, lights: dict, mtl_loader=None) -> SceneData:
         if materials[mid].isEmissive():
             emissive += list(range(prim, prim + nt))
         prim += nt
-    data = SceneData(meshes=out_meshes, materials=materials)
+    data = SceneData(meshes=out_meshes, materials=materials, textureImages=textures)
     data.emissiveIndex = emissive
     return data
 
@@ -229,7 +240,10 @@ class Importer:
             p = self.resolveResourcePath(rel)
             return parse_mtl(p.read_text()) if p.is_file() else {}
 
-        return build_scene_data(obj.read_text(), lights, mtl_loader), camd
+        def image_loader(rel):
+            return load_image(self.resolveResourcePath(rel))
+
+        return build_scene_data(obj.read_text(), lights, mtl_loader, image_loader), camd
 
     def load(self, path, device: int = 0) -> Scene:
         data, camd = self.load_data(path)

@@ -1,5 +1,8 @@
 // CRaystal.h - C++ host API over libcrayhip.so, mirroring the reference's
 // public classes so reference-style host code ports by changing includes:
+//   CRay::Float2/Float3/UInt2   Core/Vec.h:8-63 (Vec.h: .x/.y/.z, .r/.g/.b, [])
+//   CRay::Image                 Utils/Image.h:22-95 (Image.h)
+//   CRay::Spectrum              Core/Spectrum.h:237-276 (RGB, as the reference uses it)
 //   CRay::RuntimeError          Core/Error.h:59-66
 //   CRay::MaterialData          Core/Material.h:34-66 (+ finalize)
 //   CRay::MeshData / SceneData  Scene/Triangle.h:36-42, Scene/Scene.h:40-47
@@ -8,12 +11,13 @@
 //   CRay::Camera                Core/Camera.h:12-50
 //   CRay::Scene                 Scene/Scene.h:49-77
 //   CRay::PathTraceIntegrator   Core/Integrator.h:10-38
-// Header-only; link with -lcrayhip.  Every nonzero C-ABI status throws
+// Header-only; link with -lcrayhip -lz.  Every nonzero C-ABI status throws
 // CRay::RuntimeError carrying crh_last_error().  Ownership follows the
 // reference: host objects own device memory, copy is deleted, move is
 // allowed, Camera/Sensor/Scene are shared through ::Ref aliases.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
 #include <memory>
@@ -23,22 +27,31 @@
 #include <vector>
 
 #include "../crayhip.h"
+#include "Image.h"
+#include "Vec.h"
 
 namespace CRay {
 
-using Float = float;
-using Float2 = std::array<float, 2>;
-using Float3 = std::array<float, 3>;
-using UInt2 = std::array<uint32_t, 2>;
+// The reference transports RGB (using Spectrum = RGBSpectrum,
+// Core/Spectrum.h:326): three floats with the Spectrum helpers host code
+// calls.
+struct Spectrum : Float3 {
+    using Float3::Float3;
+    Spectrum() = default;
+    Spectrum(const Float3& v) : Float3(v) {}
+    static Spectrum fromRGB(Float3 rgb, bool isIllum = false) {
+        (void)isIllum;
+        return Spectrum(rgb);
+    }
+    static void initialize() {}
+    Float3 toRGB() const { return *this; }
+    float maxValue() const { return std::max(std::max(x, y), z); }
+    float averageValue() const { return (x + y + z) / 3.0f; }
+};
 
-class RuntimeError : public std::runtime_error {
-   public:
-    RuntimeError(int code, const std::string& what)
-        : std::runtime_error(what), mCode(code) {}
-    int code() const { return mCode; }
-
-   private:
-    int mCode;
+// Logger::init (Utils/Logger.cpp): this library logs nothing itself.
+struct Logger {
+    static void init() {}
 };
 
 namespace detail {
@@ -66,11 +79,11 @@ enum class MaterialType : uint32_t { FullDiffuse = 0, Principled = 1, Phong = 2 
 enum class MaterialFlags : uint32_t { IsEmissive = 1, IsDiffuse = 2, hasDiffuseTexture = 4 };
 
 struct MaterialData {
-    Float3 diffuseRefl{0, 0, 0};
+    Spectrum diffuseRefl{0, 0, 0};
     uint32_t diffuseHandle = 0;
-    Float3 specularRefl{0, 0, 0};
-    Float3 transmittance{0, 0, 0};
-    Float3 emission{0, 0, 0};
+    Spectrum specularRefl{0, 0, 0};
+    Spectrum transmittance{0, 0, 0};
+    Spectrum emission{0, 0, 0};
     Float shininess = 0;
     Float IoR = 1;
     MaterialType type = MaterialType::FullDiffuse;
@@ -115,18 +128,11 @@ struct SphereData {
     float radius = 1.f;
 };
 
-// An Image as Texture(const Image&) reads it (Core/Texture.cu:46-62):
-// pixel (x, y) at pixels[(y * width + x) * channels].
-struct TextureImage {
-    uint32_t width = 0, height = 0, channels = 4;
-    std::vector<float> pixels;
-};
-
 struct SceneData {
     std::vector<SphereData> spheres;
     std::vector<MeshData> meshes;
     std::vector<MaterialData> materials;
-    std::vector<TextureImage> textureImages;
+    std::vector<Image> textureImages;  // diffuse textures, by MaterialData::diffuseHandle
     std::vector<uint32_t> emissiveIndex;
     Float3 envRadiance{0, 0, 0};  // constant environment; zero = reference
     crh_bvh_params bvhParams{32, 4, 20, 0};
@@ -151,7 +157,7 @@ class BVH {
         std::vector<uint32_t> idx;
         for (const auto& m : data.meshes) {
             uint32_t off = uint32_t(pos.size() / 3);
-            for (const auto& p : m.position) pos.insert(pos.end(), p.begin(), p.end());
+            for (const auto& p : m.position) pos.insert(pos.end(), {p.x, p.y, p.z});
             for (uint32_t i : m.index) idx.push_back(i + off);
         }
         if (mBvh) crh_bvh_destroy(mBvh);
@@ -179,9 +185,9 @@ class Sensor {
    public:
     using Ref = std::shared_ptr<Sensor>;
     explicit Sensor(UInt2 size, uint32_t spp = 1u, int device = 0) : mSize(size), mSpp(spp) {
-        detail::check(crh_film_create(detail::context(device), size[0], size[1], &mFilm));
-        mData.resize(size_t(size[0]) * size[1] * 3);
-        mWeight.resize(size_t(size[0]) * size[1]);
+        detail::check(crh_film_create(detail::context(device), size.x, size.y, &mFilm));
+        mData.resize(size_t(size.x) * size.y * 3);
+        mWeight.resize(size_t(size.x) * size.y);
     }
     Sensor(const Sensor&) = delete;
     Sensor& operator=(const Sensor&) = delete;
@@ -199,13 +205,21 @@ class Sensor {
     void readbackDeviceData() {
         detail::check(crh_film_readback(mFilm, mData.data(), mWeight.data()));
     }
-    // rgb = sum / weight (Core/Sensor.cu:93-112), row-major W*H*3.
-    std::vector<float> createImage() const {
-        std::vector<float> img(mData.size());
-        for (size_t i = 0; i < mWeight.size(); ++i)
-            for (int c = 0; c < 3; ++c) img[3 * i + c] = mData[3 * i + c] / mWeight[i];
+    // Sensor::createImage (Core/Sensor.cu:93-112): a 3-channel linear image
+    // of rgb = sum / weight from the last readback.
+    Image::Ref createImage() const {
+        auto img = std::make_shared<Image>(mSize.x, mSize.y, 3, ColorSpace::Linear);
+        for (uint32_t y = 0; y < mSize.y; ++y)
+            for (uint32_t x = 0; x < mSize.x; ++x) {
+                const size_t i = size_t(mSize.x) * y + x;
+                for (int c = 0; c < 3; ++c)
+                    img->setPixel(int(x), int(y), c, mData[3 * i + c] / mWeight[i]);
+            }
         return img;
     }
+    // The film's sums (3 floats per pixel, row-major) and weights of the last readback.
+    const std::vector<float>& sums() const { return mData; }
+    const std::vector<float>& weights() const { return mWeight; }
     crh_film* handle() const { return mFilm; }
 
    private:
@@ -258,15 +272,15 @@ class Scene {
         for (size_t i = 0; i < data.meshes.size(); ++i) {
             const MeshData& m = data.meshes[i];
             crh_mesh& c = meshes[i];
-            c.position = m.position.empty() ? nullptr : m.position[0].data();
+            c.position = m.position.empty() ? nullptr : &m.position[0].x;
             if (m.normal.size() == m.position.size() && !m.normal.empty()) {
-                c.normal = m.normal[0].data();
+                c.normal = &m.normal[0].x;
             } else {
                 nrmFill[i].assign(m.position.size() * 3, 0.f);
                 c.normal = nrmFill[i].data();
             }
             c.texcrd = m.texCrd.size() == m.position.size() && !m.texCrd.empty()
-                           ? m.texCrd[0].data()
+                           ? &m.texCrd[0].x
                            : nullptr;
             c.nvert = uint32_t(m.position.size());
             c.index = m.index.data();
@@ -293,9 +307,19 @@ class Scene {
         d.sphere_radius = rad.data();
         d.nsphere = uint32_t(data.spheres.size());
         d.sphere_material = data.sphereMaterials.empty() ? nullptr : data.sphereMaterials.data();
+        // Texture(const Image&) (Core/Texture.cu:46-62): channel c of pixel
+        // (x, y) to texel y * width + x, interleaved.
         std::vector<crh_texture> tex;
-        for (const auto& t : data.textureImages)
-            tex.push_back(crh_texture{t.width, t.height, t.channels, t.pixels.data()});
+        std::vector<std::vector<float>> texels(data.textureImages.size());
+        for (size_t t = 0; t < data.textureImages.size(); ++t) {
+            const Image& im = data.textureImages[t];
+            const int ch = im.getChannels();
+            texels[t].resize(size_t(im.getArea()) * ch);
+            for (int c = 0; c < ch; ++c)
+                for (int i = 0; i < im.getArea(); ++i) texels[t][size_t(i) * ch + c] = im.getPixel(i, c);
+            tex.push_back(crh_texture{uint32_t(im.getWidth()), uint32_t(im.getHeight()),
+                                      uint32_t(ch), texels[t].data()});
+        }
         d.textures = tex.data();
         d.ntexture = uint32_t(tex.size());
         detail::check(crh_scene_create(detail::context(device), &d, &mScene));

@@ -12,10 +12,12 @@
 //     emissive iff its material is;
 //   * a mesh starts at every o / g statement and material change; polygons
 //     are fan-triangulated; faces without normals get their flat normal;
-//     identical (position, uv, normal) corners are joined; textured
-//     materials are rejected (textures are outside this path).
-// writeEXR: single-part scanline OpenEXR, no compression, HALF channels in
-// B, G, R order.
+//     identical (position, uv, normal) corners are joined;
+//   * a map_Kd diffuse texture (its last token, relative to the scene file)
+//     is loaded by Image::load (PNG or EXR) into SceneData::textureImages,
+//     and its material gets hasDiffuseTexture and the image's index
+//     (Utils/Importer.cpp:113-120).
+// Images are written with Image::writeEXR (Image.h).
 #pragma once
 
 #include <cmath>
@@ -30,6 +32,7 @@
 #include <vector>
 
 #include "CRaystal.h"
+#include "Image.h"
 
 namespace CRay {
 
@@ -115,7 +118,8 @@ inline std::string stripComment(const std::string& raw) {
 struct MtlEntry {
     Float3 kd{0, 0, 0}, ks{0, 0, 0};
     float ns = 0.f, ni = 1.f, d = 1.f;
-    bool hasD = false, textured = false;
+    bool hasD = false;
+    std::string diffuseMap;  // map_Kd file name, empty if none
 };
 
 inline std::map<std::string, MtlEntry> parseMtl(const std::string& text) {
@@ -148,115 +152,14 @@ inline std::map<std::string, MtlEntry> parseMtl(const std::string& text) {
         } else if (t[0] == "Tr" && t.size() >= 2) {
             cur->d = 1.f - std::stof(t[1]);
             cur->hasD = true;
-        } else if (t[0].rfind("map_Kd", 0) == 0) {
-            cur->textured = true;
+        } else if (t[0] == "map_Kd" && t.size() >= 2) {
+            cur->diffuseMap = t.back();
         }
     }
     return mats;
 }
 
-// float -> IEEE half, round to nearest even (NaN stays NaN).
-inline uint16_t floatToHalf(float f) {
-    uint32_t x;
-    std::memcpy(&x, &f, 4);
-    const uint32_t sign = (x >> 16) & 0x8000u;
-    const uint32_t exp = (x >> 23) & 0xFFu;
-    uint32_t mant = x & 0x007FFFFFu;
-    if (exp == 0xFFu) {
-        uint32_t m = mant >> 13;
-        return uint16_t(sign | 0x7C00u | (mant ? (m ? m : 1u) : 0u));
-    }
-    const int e = int(exp) - 127 + 15;
-    if (e >= 0x1F) return uint16_t(sign | 0x7C00u);
-    if (e <= 0) {  // subnormal half (or zero)
-        if (exp == 0) return uint16_t(sign);
-        const uint32_t m = mant | 0x00800000u;
-        const uint32_t s = uint32_t(126 - int(exp));  // 14 .. 126
-        if (s > 25) return uint16_t(sign);
-        uint32_t h = m >> s;
-        const uint32_t r = m & ((1u << s) - 1u), half = 1u << (s - 1);
-        if (r > half || (r == half && (h & 1u))) ++h;
-        return uint16_t(sign | h);
-    }
-    uint32_t h = sign | (uint32_t(e) << 10) | (mant >> 13);
-    const uint32_t r = mant & 0x1FFFu;
-    if (r > 0x1000u || (r == 0x1000u && (h & 1u))) ++h;
-    return uint16_t(h);
-}
-
-inline void put32(std::string& b, int32_t v) { b.append(reinterpret_cast<const char*>(&v), 4); }
-inline void putF(std::string& b, float v) { b.append(reinterpret_cast<const char*>(&v), 4); }
-inline void putAttr(std::string& b, const char* name, const char* type, const std::string& v) {
-    b += name;
-    b += '\0';
-    b += type;
-    b += '\0';
-    put32(b, int32_t(v.size()));
-    b += v;
-}
-
 }  // namespace detail
-
-// rgb: height rows of width pixels, 3 floats each (Sensor::createImage()).
-inline void writeEXR(const std::filesystem::path& path, uint32_t width, uint32_t height,
-                     const std::vector<float>& rgb) {
-    if (path.extension() != ".exr")
-        throw RuntimeError(CRH_ERR_INVALID_ARG, "Output file must be .exr");
-    if (rgb.size() < size_t(width) * height * 3)
-        throw RuntimeError(CRH_ERR_INVALID_ARG, "image smaller than width x height x 3");
-    using namespace detail;
-    const char* names[3] = {"B", "G", "R"};
-    const int src[3] = {2, 1, 0};
-    std::string chl;
-    for (const char* n : names) {
-        chl += n;
-        chl += '\0';
-        put32(chl, 1);  // HALF
-        chl += std::string(4, '\0');  // pLinear + reserved
-        put32(chl, 1);
-        put32(chl, 1);
-    }
-    chl += '\0';
-    std::string box;
-    put32(box, 0);
-    put32(box, 0);
-    put32(box, int32_t(width) - 1);
-    put32(box, int32_t(height) - 1);
-    std::string f1, v2;
-    putF(f1, 1.0f);
-    putF(v2, 0.0f);
-    putF(v2, 0.0f);
-    std::string out;
-    put32(out, 20000630);
-    put32(out, 2);
-    putAttr(out, "channels", "chlist", chl);
-    putAttr(out, "compression", "compression", std::string(1, '\0'));
-    putAttr(out, "dataWindow", "box2i", box);
-    putAttr(out, "displayWindow", "box2i", box);
-    putAttr(out, "lineOrder", "lineOrder", std::string(1, '\0'));
-    putAttr(out, "pixelAspectRatio", "float", f1);
-    putAttr(out, "screenWindowCenter", "v2f", v2);
-    putAttr(out, "screenWindowWidth", "float", f1);
-    out += '\0';
-    const uint64_t lineBytes = uint64_t(width) * 2 * 3;
-    const uint64_t first = out.size() + 8 * uint64_t(height);
-    for (uint32_t y = 0; y < height; ++y) {
-        uint64_t off = first + y * (8 + lineBytes);
-        out.append(reinterpret_cast<const char*>(&off), 8);
-    }
-    for (uint32_t y = 0; y < height; ++y) {
-        put32(out, int32_t(y));
-        put32(out, int32_t(lineBytes));
-        for (int c = 0; c < 3; ++c)
-            for (uint32_t x = 0; x < width; ++x) {
-                uint16_t h = floatToHalf(rgb[(size_t(y) * width + x) * 3 + src[c]]);
-                out.append(reinterpret_cast<const char*>(&h), 2);
-            }
-    }
-    std::ofstream f(path, std::ios::binary);
-    f.write(out.data(), std::streamsize(out.size()));
-    if (!f) throw RuntimeError(CRH_ERR_INVALID_ARG, "cannot write " + path.string());
-}
 
 struct CameraDesc {
     float fovY = 60.f;
@@ -403,10 +306,12 @@ class Importer {
                 auto it = mtl.find(n.second);
                 if (it != mtl.end()) e = it->second;
             }
-            if (e.textured)
-                throw RuntimeError(CRH_ERR_INVALID_ARG,
-                                   "material " + n.second + ": textures are not supported");
             MaterialData md;
+            if (!e.diffuseMap.empty()) {
+                md.flags |= uint32_t(MaterialFlags::hasDiffuseTexture);
+                md.diffuseHandle = uint32_t(data.textureImages.size());
+                data.textureImages.push_back(Image::load(resolveResourcePath(e.diffuseMap)));
+            }
             md.diffuseRefl = e.kd;
             md.specularRefl = e.ks;
             if (e.hasD) md.transmittance = {1.f - e.d, 1.f - e.d, 1.f - e.d};

@@ -20,9 +20,7 @@ using namespace CRay;
 
 static MeshData quad(Float3 c, Float3 a, Float3 b, Float3 n, uint32_t mat) {
     MeshData m;
-    m.position = {c, {c[0] + a[0], c[1] + a[1], c[2] + a[2]},
-                  {c[0] + a[0] + b[0], c[1] + a[1] + b[1], c[2] + a[2] + b[2]},
-                  {c[0] + b[0], c[1] + b[1], c[2] + b[2]}};
+    m.position = {c, c + a, c + a + b, c + b};
     m.normal = {n, n, n, n};
     m.index = {0, 1, 2, 0, 2, 3};
     m.materialID = mat;
@@ -90,8 +88,12 @@ int main(int argc, char** argv) {
     sdata.materials.push_back(tex);
     sdata.meshes[0].texCrd = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
     sdata.meshes[0].materialID = 2;
-    sdata.textureImages.push_back(TextureImage{2, 2, 3, {0.9f, 0.1f, 0.1f, 0.1f, 0.9f, 0.1f,
-                                                         0.1f, 0.1f, 0.9f, 0.9f, 0.9f, 0.9f}});
+    const float texel[12] = {0.9f, 0.1f, 0.1f, 0.1f, 0.9f, 0.1f,
+                             0.1f, 0.1f, 0.9f, 0.9f, 0.9f, 0.9f};
+    Image texImg(2, 2, 3);
+    for (int i = 0; i < 4; ++i)
+        for (int c = 0; c < 3; ++c) texImg.setPixel(i % 2, i / 2, c, texel[3 * i + c]);
+    sdata.textureImages.push_back(std::move(texImg));
     sdata.spheres = {SphereData{{0.f, 0.5f, 0.f}, 0.4f}, SphereData{{1.f, 0.3f, 1.f}, 0.3f}};
     sdata.sphereMaterials = {0, 2};
 
@@ -105,9 +107,11 @@ int main(int argc, char** argv) {
     PathTraceIntegrator integ(cfg);
     crh_stats rs = integ.dispatch(*scene, 2);
     auto img = sensor->createImage();
+    EXPECT(img->getWidth() == 32 && img->getHeight() == 24 && img->getChannels() == 3);
     double mean = 0;
-    for (float v : img) mean += v;
-    mean /= img.size();
+    for (const auto& layer : img->getRawData())
+        for (float v : layer) mean += v;
+    mean /= 3.0 * img->getArea();
     EXPECT(rs.rays > 0 && std::isfinite(mean) && mean > 0);
     std::vector<crh_ray> rays(2);
     rays[0] = crh_ray{{0, 1, 0}, 0.f, {0, -1, 0}, INFINITY};
@@ -147,8 +151,9 @@ int main(int argc, char** argv) {
     crh_stats ss = PathTraceIntegrator(scfg).dispatch(*sscene, 2);
     auto simg = ssensor->createImage();
     double smean = 0;
-    for (float v : simg) smean += v;
-    smean /= simg.size();
+    for (const auto& layer : simg->getRawData())
+        for (float v : layer) smean += v;
+    smean /= 3.0 * simg->getArea();
     EXPECT(ss.rays > 0 && std::isfinite(smean) && smean > 0 && std::fabs(smean - mean) > 1e-6);
     std::printf("cpp gpu api ok: mean %.5f rays %llu\n", mean, (unsigned long long)rs.rays);
     return 0;

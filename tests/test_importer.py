@@ -89,12 +89,169 @@ f 3 4 5
     assert data.emissiveIndex == [3, 4]          # primitive IDs in mesh and face order
 
 
-def test_texture_is_rejected(tmp_path):
-    (tmp_path / "t.mtl").write_text("newmtl Tex\nKd 1 1 1\nmap_Kd wood.png\n")
-    (tmp_path / "t.obj").write_text("mtllib t.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl Tex\nf 1 2 3\n")
+def _png_bytes(pix: np.ndarray, filters) -> bytes:
+    """An 8-bit PNG of pix (H, W, C uint8, row 0 first in the file), encoded
+    here from the PNG specification with the given filter type per row."""
+    import zlib
+    h, w, ch = pix.shape
+    rows = pix.reshape(h, w * ch).astype(np.int64)
+    raw = b""
+    for y in range(h):
+        f = filters[y % len(filters)]
+        prev = rows[y - 1] if y > 0 else np.zeros(w * ch, np.int64)
+        cur = rows[y]
+        a = np.concatenate([np.zeros(ch, np.int64), cur[:-ch]])
+        c = np.concatenate([np.zeros(ch, np.int64), prev[:-ch]])
+        if f == 0:
+            pred = np.zeros_like(cur)
+        elif f == 1:
+            pred = a
+        elif f == 2:
+            pred = prev
+        elif f == 3:
+            pred = (a + prev) // 2
+        else:
+            pa, pb, pc = np.abs(prev - c), np.abs(a - c), np.abs(a + prev - 2 * c)
+            pred = np.where((pa <= pb) & (pa <= pc), a, np.where(pb <= pc, prev, c))
+        raw += bytes([f]) + ((cur - pred) & 0xFF).astype(np.uint8).tobytes()
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+    ct = {1: 0, 3: 2, 4: 6}[ch]
+    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, ct, 0, 0, 0))
+            + chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b""))
+
+
+def _exr_zip_bytes(chans, h, w, comp=3) -> bytes:
+    """A scanline OpenEXR with ZIP (3, 16 lines per block) or ZIPS (2)
+    compression, encoded here from the specification: per block the lines'
+    channel samples, bytes split into even / odd halves, then the delta
+    predictor (+128), then zlib.  chans: list of (name, type, (H, W) array)."""
+    import zlib
+    chans = sorted(chans, key=lambda c: c[0])
+    chlist = b"".join(n.encode() + b"\0" + struct.pack("<iB3xii", t, 0, 1, 1)
+                      for n, t, _ in chans) + b"\0"
+    box = struct.pack("<iiii", 0, 0, w - 1, h - 1)
+
+    def attr(n, t, v):
+        return n.encode() + b"\0" + t.encode() + b"\0" + struct.pack("<i", len(v)) + v
+    head = struct.pack("<ii", 20000630, 2) + attr("channels", "chlist", chlist) + \
+        attr("compression", "compression", bytes([comp])) + attr("dataWindow", "box2i", box) + \
+        attr("displayWindow", "box2i", box) + attr("lineOrder", "lineOrder", b"\0") + b"\0"
+    lines = 16 if comp == 3 else 1
+    blocks = []
+    dt = {0: "<u4", 1: "<f2", 2: "<f4"}
+    for y0 in range(0, h, lines):
+        raw = b"".join(np.asarray(a[y], dt[t]).tobytes()
+                       for y in range(y0, min(h, y0 + lines)) for _, t, a in chans)
+        b = np.frombuffer(raw, np.uint8)
+        split = np.concatenate([b[0::2], b[1::2]]).astype(np.int64)
+        delta = np.concatenate([split[:1], (split[1:] - split[:-1] + 128) & 0xFF])
+        z = zlib.compress(delta.astype(np.uint8).tobytes())
+        blocks.append(struct.pack("<ii", y0, len(z)) + z)
+    table_at = len(head)
+    offs, pos = [], table_at + 8 * len(blocks)
+    for blk in blocks:
+        offs.append(pos)
+        pos += len(blk)
+    return head + struct.pack(f"<{len(blocks)}Q", *offs) + b"".join(blocks)
+
+
+def test_png_reader_known_answers(tmp_path):
+    """Every PNG filter type, grey / RGB / RGBA: values /255 (and ^2.2 to
+    linear), rows bottom-up as the reference's stb load flips them."""
+    from craystal_amd.imageio import read_png, write_png
+    rng = np.random.default_rng(1)
+    for ch in (1, 3, 4):
+        pix = rng.integers(0, 256, (9, 6, ch)).astype(np.uint8)
+        (tmp_path / "t.png").write_bytes(_png_bytes(pix, [0, 1, 2, 3, 4]))
+        got = read_png(tmp_path / "t.png", to_linear=False)
+        assert np.array_equal(got, pix[::-1].astype(np.float32) / np.float32(255))
+        lin = read_png(tmp_path / "t.png", to_linear=True)
+        assert np.allclose(lin, (pix[::-1] / 255.0) ** 2.2, rtol=1e-6, atol=1e-7)
+    img = rng.random((5, 4, 3)).astype(np.float32)
+    write_png(tmp_path / "w.png", img, gamma_correct=True)
+    back = read_png(tmp_path / "w.png", to_linear=False)[::-1]
+    assert np.array_equal(back * 255, np.floor(np.clip(img ** (1 / 2.2), 0, 1) * 255))
+    (tmp_path / "ga.png").write_bytes(_png_bytes(pix[..., :1], [0]).replace(
+        b"IHDR" + struct.pack(">IIBB", 6, 9, 8, 0), b"IHDR" + struct.pack(">IIBB", 6, 9, 8, 4)))
+    with pytest.raises(ValueError):
+        read_png(tmp_path / "ga.png")
+
+
+def test_exr_reader_zip_and_channel_mapping(tmp_path):
+    """ZIP and ZIPS compressed EXR with FLOAT, HALF and UINT channels and an
+    extra alpha: load_exr_image puts R, G, B in channels 0-2 and leaves the
+    fourth channel zero (Image::readEXR copies R, G, B only)."""
+    from craystal_amd.imageio import load_exr_image, read_exr
+    rng = np.random.default_rng(2)
+    h, w = 37, 11                                        # blocks of 16 with a ragged tail
+    R = rng.random((h, w)).astype(np.float32) * 5
+    G = rng.random((h, w)).astype(np.float16)
+    B = rng.integers(0, 1000, (h, w)).astype(np.uint32)
+    A = np.ones((h, w), np.float32)
+    for comp in (3, 2):
+        (tmp_path / "z.exr").write_bytes(_exr_zip_bytes(
+            [("R", 2, R), ("G", 1, G), ("B", 0, B), ("A", 2, A)], h, w, comp))
+        data, hdr = read_exr(tmp_path / "z.exr")
+        assert [n for n, _ in hdr["channels"]] == ["A", "B", "G", "R"]
+        img = load_exr_image(tmp_path / "z.exr")
+        assert img.shape == (h, w, 4)
+        assert np.array_equal(img[..., 0], R) and np.array_equal(img[..., 1], G.astype(np.float32))
+        assert np.array_equal(img[..., 2], B.astype(np.float32)) and not img[..., 3].any()
+
+
+def _cpp_image_dump(path):
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent / "cpp" / "_build" / "import_test"
+    d = path.parent
+    if not (d / "dump.xml").exists():
+        write_obj(scenes.cornell_box()[0], ["W", "R", "G", "L"], d, stem="dump")
+    r = subprocess.run([str(exe), str(d / "dump.xml"), str(d / "dump_out.exr"), str(path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    k = [i for i, ln in enumerate(lines) if ln.startswith("image ")][0]
+    w, h, c = (int(v) for v in lines[k].split()[1:])
+    vals = np.array([float.fromhex(v) for v in lines[k + 1:]], np.float32)
+    return vals.reshape(c, h, w).transpose(1, 2, 0)
+
+
+def test_cpp_image_readers_match_python(tmp_path):
+    """Image::load (include/craystal/Image.h) reads PNG and ZIP-compressed EXR
+    to the same floats as the Python readers."""
+    from craystal_amd.imageio import load_image
+    rng = np.random.default_rng(4)
+    pix = rng.integers(0, 256, (7, 5, 3)).astype(np.uint8)
+    (tmp_path / "a.png").write_bytes(_png_bytes(pix, [4, 3, 1, 2, 0]))
+    assert np.array_equal(_cpp_image_dump(tmp_path / "a.png"), load_image(tmp_path / "a.png"))
+    R = rng.random((20, 3)).astype(np.float32)
+    (tmp_path / "b.exr").write_bytes(_exr_zip_bytes(
+        [("R", 2, R), ("G", 1, R.astype(np.float16)), ("B", 2, 2 * R)], 20, 3))
+    assert np.array_equal(_cpp_image_dump(tmp_path / "b.exr"), load_image(tmp_path / "b.exr"))
+
+
+def test_map_kd_texture_is_loaded(tmp_path):
+    """A map_Kd texture becomes a texture image of its material (flag and
+    handle set, Utils/Importer.cpp:113-120), in both importers."""
+    from craystal_amd.imageio import load_image
+    pix = np.arange(4 * 3 * 3, dtype=np.uint8).reshape(4, 3, 3) * 7
+    (tmp_path / "wood.png").write_bytes(_png_bytes(pix, [1]))
+    (tmp_path / "t.mtl").write_text("newmtl Tex\nKd 1 1 1\nmap_Kd -s 1 1 1 wood.png\n"
+                                    "newmtl Light\nKd 0 0 0\n")
+    (tmp_path / "t.obj").write_text(
+        "mtllib t.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0 1\n"
+        "usemtl Tex\nf 1/1 2/2 3/3\nusemtl Light\nf 1 3 2\n")
     (tmp_path / "t.xml").write_text(XML)
-    with pytest.raises(ImportError_, match="textures"):
-        Importer().load_data(tmp_path / "t.xml")
+    data, _ = Importer().load_data(tmp_path / "t.xml")
+    tex = data.materials[0]
+    assert tex.flags & int(cr.MaterialFlags.hasDiffuseTexture) and tex.diffuseHandle == 0
+    assert not data.materials[1].flags & int(cr.MaterialFlags.hasDiffuseTexture)
+    assert len(data.textureImages) == 1
+    assert np.array_equal(data.textureImages[0], load_image(tmp_path / "wood.png"))
+    lines = _cpp_import(tmp_path / "t.xml", tmp_path / "c.exr")
+    assert "texture 0 3x4x3" in lines
 
 
 def test_round_trip_of_the_cornell_box_matches_its_bvh(tmp_path):
