@@ -50,7 +50,7 @@ class crh_hit(C.Structure):
 
 
 class crh_bvh_params(C.Structure):
-    _fields_ = [("bins", u32), ("max_leaf", u32), ("max_depth", u32), ("reserved", u32)]
+    _fields_ = [("bins", u32), ("max_leaf", u32), ("max_depth", u32), ("device", u32)]
 
 
 class crh_bvh_node_ref(C.Structure):
@@ -172,6 +172,7 @@ SIGNATURES = {
     "crh_trace_any": [_P, _P, C.c_uint64, _P],
     "crh_trace_closest_device": [_P, _P, C.c_uint64, _P, C.c_int, C.POINTER(crh_stats)],
     "crh_trace_any_device": [_P, _P, C.c_uint64, _P, C.POINTER(crh_stats)],
+    "crh_bvh_build_device": [_P, _P, u32, _P, u32, C.POINTER(crh_bvh_params), C.POINTER(_P)],
     "crh_trace_closest_device_noalloc": [_P, _P, C.c_uint64, _P, C.c_int,
                                          C.POINTER(crh_stats)],
     "crh_film_create": [_P, u32, u32, C.POINTER(_P)],

@@ -188,6 +188,7 @@ class SceneData:
     envRadiance: tuple = (0.0, 0.0, 0.0)
     # BVH parameters (bins, max leaf, max depth); zeros take the reference's.
     bvhParams: tuple = (32, 4, 20)
+    bvhDevice: bool = False  # build the BVH on the GPU (bit-identical to the host build)
     # Spectral transport: (nmaterial, 3, 95) diffuse, specular and emission
     # spectra, 5 nm bins from 360 nm; the environment's 95 bins.  None: the
     # RGB values are uplifted (rgb_to_spectrum) when the scene is created.
@@ -237,7 +238,8 @@ class SceneData:
         d.emissive_index = em.ctypes.data_as(C.POINTER(C.c_uint32))
         d.nemissive = len(em)
         d.env_radiance[:] = [float(x) for x in self.envRadiance]
-        d.bvh.bins, d.bvh.max_leaf, d.bvh.max_depth = self.bvhParams
+        d.bvh.bins, d.bvh.max_leaf, d.bvh.max_depth = self.bvhParams[:3]
+        d.bvh.device = 1 if self.bvhDevice else 0
         if self.spectra is not None:
             sp = np.ascontiguousarray(self.spectra, dtype=np.float32)
             if sp.shape != (len(self.materials), 3, SPECTRUM_BINS):
@@ -287,16 +289,22 @@ def _stats_dict(st: L.crh_bvh_stats) -> dict:
 
 
 class BVH:
-    """BVH::build (Core/BVH.cu:88-338) on a triangle soup, host side."""
+    """BVH::build (Core/BVH.cu:88-338) on a triangle soup: on the host, or
+    with device=True on the GPU (crh_bvh_build_device, the same result bit
+    for bit)."""
 
     def __init__(self, positions: np.ndarray, indices: np.ndarray, bins=32, max_leaf=4,
-                 max_depth=20):
+                 max_depth=20, device: bool = False, gpu: int = 0):
         pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
         idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
         p = L.crh_bvh_params(bins, max_leaf, max_depth, 0)
         h = C.c_void_p()
-        L.call("crh_bvh_build", L.ptr(pos), len(pos), L.ptr(idx), len(idx) // 3,
-               C.byref(p), C.byref(h))
+        if device:
+            L.call("crh_bvh_build_device", get_context(gpu).handle, L.ptr(pos), len(pos),
+                   L.ptr(idx), len(idx) // 3, C.byref(p), C.byref(h))
+        else:
+            L.call("crh_bvh_build", L.ptr(pos), len(pos), L.ptr(idx), len(idx) // 3,
+                   C.byref(p), C.byref(h))
         self.handle = h
 
     def stats(self) -> dict:

@@ -25,6 +25,7 @@
 
 #include "../../include/crayhip.h"
 #include "bvh_build.h"
+#include "bvh_device.h"
 #include "kernels.h"
 #include "noalloc.h"
 #include "spectral.h"
@@ -627,6 +628,50 @@ int crh_bvh_build(const float* pos, uint32_t nvert, const uint32_t* idx,
     GUARD_END
 }
 
+namespace {
+// The GPU builder from host arrays: uploads the geometry, builds, returns.
+hipError_t buildOnDevice(crh_context* ctx, const float* pos, uint32_t nvert, const uint32_t* idx,
+                         uint32_t ntri, const crh::BuildParams& bp, crh::BvhResult& out) {
+    const double t0 = nowMs();
+    DevMem dp, di;
+    hipError_t e = dp.alloc(size_t(nvert) * 3 * sizeof(float));
+    if (e == hipSuccess) e = di.alloc(size_t(ntri) * 3 * sizeof(uint32_t));
+    if (e == hipSuccess && nvert)
+        e = hipMemcpyAsync(dp.p, pos, dp.bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && ntri)
+        e = hipMemcpyAsync(di.p, idx, di.bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = crh::buildBvhDevice(dp.as<float>(), di.as<uint32_t>(), ntri, bp, ctx->stream, out);
+    out.buildMs = nowMs() - t0;
+    return e;
+}
+}  // namespace
+
+int crh_bvh_build_device(crh_context* ctx, const float* pos, uint32_t nvert,
+                         const uint32_t* idx, uint32_t ntri, const crh_bvh_params* params,
+                         crh_bvh** out) {
+    ARG_CHECK(ctx && out, "NULL argument");
+    *out = nullptr;
+    ARG_CHECK((pos || nvert == 0) && (idx || ntri == 0), "NULL geometry");
+    GUARD_BEGIN
+    for (size_t i = 0; i < size_t(ntri) * 3; ++i)
+        ARG_CHECK(idx[i] < nvert, "vertex index out of range");
+    crh::BuildParams bp = crh::resolveParams(params);
+    ARG_CHECK(bp.bins >= 2 && bp.bins <= crh::kDeviceMaxBins,
+              "the GPU builder takes bins in [2, 64]");
+    ARG_CHECK(bp.maxDepth <= crh::kDeviceMaxDepth, "the GPU builder takes max_depth <= 64");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto* b = new crh_bvh();
+    hipError_t e = buildOnDevice(ctx, pos, nvert, idx, ntri, bp, b->r);
+    if (e != hipSuccess) {
+        delete b;
+        return hipFail("GPU BVH build", e, __FILE__, __LINE__);
+    }
+    *out = b;
+    return CRH_OK;
+    GUARD_END
+}
+
 int crh_bvh_get_stats(const crh_bvh* bvh, crh_bvh_stats* stats) {
     ARG_CHECK(bvh && stats, "NULL argument");
     fillStats(bvh->r, stats);
@@ -736,11 +781,23 @@ int crh_scene_create(crh_context* ctx, const crh_scene_desc* d,
                   "diffuse_tex out of range");
     crh::BuildParams bp = crh::resolveParams(&d->bvh);
     ARG_CHECK(bp.bins >= 2 && bp.bins <= 1024, "bins must be in [2, 1024]");
+    ARG_CHECK(d->bvh.device <= 1u, "bvh.device must be 0 (host) or 1 (GPU)");
+    ARG_CHECK(!d->bvh.device || (bp.bins <= crh::kDeviceMaxBins && bp.maxDepth <= crh::kDeviceMaxDepth),
+              "the GPU builder takes bins in [2, 64] and max_depth <= 64");
 
     auto* s = new crh_scene();
     s->ctx.set(ctx);
-    s->bvh = crh::buildBvh(pos.data(), uint32_t(pos.size() / 3), idx.data(),
-                           ntri, bp);
+    if (d->bvh.device) {
+        HIP_TRY(hipSetDevice(ctx->device));
+        hipError_t e = buildOnDevice(ctx, pos.data(), uint32_t(pos.size() / 3), idx.data(), ntri,
+                                     bp, s->bvh);
+        if (e != hipSuccess) {
+            delete s;
+            return hipFail("GPU BVH build", e, __FILE__, __LINE__);
+        }
+    } else {
+        s->bvh = crh::buildBvh(pos.data(), uint32_t(pos.size() / 3), idx.data(), ntri, bp);
+    }
     crh::GpuBvh g = crh::toGpuLayout(s->bvh, pos.data(), idx.data());
     std::vector<crh::GMaterial> gm(d->nmaterial);
     for (uint32_t i = 0; i < d->nmaterial; ++i) {

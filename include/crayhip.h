@@ -72,7 +72,9 @@ typedef struct crh_bvh_params {
     uint32_t bins;
     uint32_t max_leaf;
     uint32_t max_depth;
-    uint32_t reserved;
+    /* 1: build on the GPU (crh_scene_create; the same nodes and primitive
+     * order bit for bit, for bins <= 64 and max_depth <= 64); 0: on the host. */
+    uint32_t device;
 } crh_bvh_params;
 
 /* Canonical export of a reference BVHNode (Core/BVH.h:17-30): tag 0 = leaf
@@ -323,6 +325,13 @@ int crh_buffer_destroy(crh_buffer* buf);
  * reference's node array and primitive order for the same parameters. */
 int crh_bvh_build(const float* pos, uint32_t nvert, const uint32_t* idx,
                   uint32_t ntri, const crh_bvh_params* params, crh_bvh** out);
+/* BVH::build on the GPU of ctx's device (no counterpart in the reference,
+ * whose builder is host code): the result of crh_bvh_build bit for bit (node
+ * array, primitive order, statistics) for bins <= 64 and max_depth <= 64.
+ * Host arrays in; build_ms is the call's wall time (uploads included). */
+int crh_bvh_build_device(crh_context* ctx, const float* pos, uint32_t nvert,
+                         const uint32_t* idx, uint32_t ntri,
+                         const crh_bvh_params* params, crh_bvh** out);
 int crh_bvh_get_stats(const crh_bvh* bvh, crh_bvh_stats* stats);
 /* nodes: node_count entries; prim_order: prim_count entries
  * (bvhToMeshPrimitiveIndex, Core/BVH.cu:322-338). */
