@@ -79,7 +79,8 @@ def parse():
                          "batch (with --profile-tuning applied first)")
     ap.add_argument("--profile-guided", action="store_true",
                     help="counter-collection mode for the guiding kernels: the occluded room "
-                         "at 256x256, 16 spp of the guided wavefront with training, 16 without, "
+                         "at 256x256, 16 spp of the fused guided kernel with training, 16 "
+                         "without, 16 of the guided wavefront, "
                          "then 16 spp of the plain megakernel and of the plain wavefront")
     return ap.parse_args()
 
@@ -102,29 +103,34 @@ def profile_wavefront(cr, L, scenes, tuning: str | None = None):
 
 
 def profile_guided(cr, L, scenes):
-    """Launches of the guided wavefront (training and inference), the plain
-    megakernel and the plain wavefront on the occluded room, 256 x 256, max
-    depth 16, for rocprofv3 kernel traces and counter runs."""
+    """Launches of the fused guided megakernel (training and inference), the
+    guided wavefront, the plain megakernel and the plain wavefront on the
+    occluded room, 256 x 256, max depth 16, for rocprofv3 kernel traces and
+    counter runs (MfmaUtil of the fused kernel)."""
     from craystal_amd.guiding import Guide, GuidedIntegrator
     data, camd = scenes.occluded_room()
     cfg = cr.Configs(maxDepth=16, rrThreshold=0.05, useMIS=True, rrStartDepth=3)
     scene = cr.Scene(data)
     sensor = cr.Sensor((256, 256))
     scene.setCamera(scenes.make_camera(camd, sensor))
+    ctx = cr.get_context(0)
     P = data.concatenated()[0]
     guide = Guide(P.min(0), P.max(0), seed=11)
-    gi = GuidedIntegrator(cr.Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT}), guide,
-                          alpha=0.5, lr=1e-2, train_batch=4096, train_steps=4, materials=1,
-                          target=0)
+    gi = GuidedIntegrator(cfg, guide, alpha=0.5, lr=1e-2, train_batch=4096, train_steps=4,
+                          materials=1, target=0)
+    ctx.set_tuning(guided_fused=1)
     tr = gi.dispatch(scene, 16, first_sample=0, train=True, readback=False)
     inf = gi.dispatch(scene, 16, first_sample=16, train=False, readback=False)
+    ctx.set_tuning(guided_fused=2)
+    wfg = gi.dispatch(scene, 16, first_sample=16, train=False, readback=False)
+    ctx.set_tuning(guided_fused=1)
     mk = cr.PathTraceIntegrator(cfg).dispatch(scene, 16, first_sample=32, readback=False)
     wf = cr.PathTraceIntegrator(cr.Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})).dispatch(
         scene, 16, first_sample=48, readback=False)
-    print(json.dumps({"pixels": 65536, "spp_each": 16, "guided_train_ms": tr["kernel_ms"],
-                      "guided_ms": inf["kernel_ms"], "megakernel_ms": mk["kernel_ms"],
-                      "wavefront_ms": wf["kernel_ms"], "guide_steps": guide.info()["steps"]}),
-          flush=True)
+    print(json.dumps({"pixels": 65536, "spp_each": 16, "fused_guided_train_ms": tr["kernel_ms"],
+                      "fused_guided_ms": inf["kernel_ms"], "wavefront_guided_ms": wfg["kernel_ms"],
+                      "megakernel_ms": mk["kernel_ms"], "wavefront_ms": wf["kernel_ms"],
+                      "guide_steps": guide.info()["steps"]}), flush=True)
 
 
 def profile_kernels(cr, L, scenes, tuning: str | None = None):

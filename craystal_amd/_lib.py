@@ -109,7 +109,8 @@ class crh_tuning(C.Structure):
                                    "wf_shade_grid", "wf_persistent", "wf_refill", "wf_pool",
                                    "batch_stack", "batch_tile_w", "batch_grid", "batch_chunks",
                                    "batch_graph", "batch_waves", "render_band",
-                                   "batch_slots", "trace_tris")]
+                                   "batch_slots", "trace_tris", "guided_fused", "guided_stack",
+                                   "guided_waves", "guided_grid", "guided_pool")]
 
 
 class crh_batch_info(C.Structure):

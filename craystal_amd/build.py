@@ -55,7 +55,8 @@ def build(verbose: bool = True) -> Path:
         o = OBJ / (src + ".o")
         cmds.append(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
         objs.append(str(o))
-    for src in ("kernels.hip", "wavefront.hip", "guiding.hip", "noalloc.hip", "bvh_device.hip"):
+    for src in ("kernels.hip", "wavefront.hip", "guiding.hip", "noalloc.hip", "bvh_device.hip",
+                "guided_mk.hip"):
         o = OBJ / (src + ".o")
         cmds.append([str(ROCM / "bin" / "hipcc"), *HIPCC_FLAGS, "-c", str(CSRC / src), "-o",
                      str(o)])

@@ -30,6 +30,7 @@
 #include "noalloc.h"
 #include "spectral.h"
 #include "wavefront.h"
+#include "guided_mk.h"
 #include "guiding_kernels.h"
 
 static_assert(sizeof(crh_ray) == 32, "crh_ray layout");
@@ -167,7 +168,7 @@ struct CtxRef {
 crh::Tuning effective(const crh::Tuning& t, const crh::SceneView& v) {
     crh::Tuning e = t;
     if (v.stackCap > uint32_t(crh::kPrivStackLevels))
-        e.privStack = e.renderPriv = e.wfPriv = e.batchPriv = 0;
+        e.privStack = e.renderPriv = e.wfPriv = e.batchPriv = e.guidedPriv = 0;
     return e;
 }
 // Core/CameraProxy.h:25-40 view of a camera for a w x h film: aspect in
@@ -237,6 +238,10 @@ struct crh_film {
     // Training samples waiting in gSamples (accumulated over passes until a
     // batch is full) and the buffer's capacity in samples.
     size_t gPending = 0, gCap = 0;
+    // Fused guided rendering: per-wave network outputs, training records and
+    // their counters (records written, samples produced).
+    DevMem gScratch, gRec, gCnt;
+    uint32_t gRecCap = 0;
     ~crh_film() {
         for (hipEvent_t e : passEv) (void)hipEventDestroy(e);
     }
@@ -335,6 +340,20 @@ hipError_t ensureGuided(crh_film* f, uint32_t slots) {
     w.recCap = uint32_t(std::min<size_t>(cap, 0xFFFFFFF0u));
     f->guidedReady = true;
     return hipSuccess;
+}
+
+// Buffers of the fused guided megakernel: `scratchFloats` network outputs
+// and `recCap` training records (none when not training).
+hipError_t ensureGuidedFused(crh_film* f, size_t scratchFloats, uint32_t recCap) {
+    hipError_t e = hipSuccess;
+    if (f->gScratch.bytes < scratchFloats * sizeof(float))
+        e = f->gScratch.alloc(scratchFloats * sizeof(float));
+    if (e == hipSuccess && f->gCnt.bytes == 0) e = f->gCnt.alloc(256);
+    if (e == hipSuccess && f->gRecCap < recCap) {
+        e = f->gRec.alloc(size_t(recCap) * crh::kRecF4 * sizeof(float4));
+        f->gRecCap = e == hipSuccess ? recCap : 0u;
+    }
+    return e;
 }
 
 // Samples per pixel rendered together: enough paths to fill `pool`, no
@@ -449,6 +468,14 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     ARG_CHECK(t->batch_grid <= (1u << 20), "grid too large");
     ARG_CHECK(t->batch_chunks <= crh::kBatchMaxChunks, "batch_chunks must be in [1, 8]");
     ARG_CHECK(t->batch_graph <= 2, "batch_graph must be 0 (keep), 1 (on) or 2 (off)");
+    ARG_CHECK(t->guided_fused <= 2, "guided_fused must be 0 (keep), 1 (on) or 2 (off)");
+    ARG_CHECK(okStack(t->guided_stack),
+              "guided_stack must be 0 (keep), 1 (LDS), 2 (private) or 3 (short LDS + private)");
+    ARG_CHECK(t->guided_waves == 0 || t->guided_waves == 1 ||
+                  (t->guided_waves >= 4 && t->guided_waves <= 6) || t->guided_waves == 8,
+              "guided_waves must be 0 (keep), 1, 4, 5, 6 or 8");
+    ARG_CHECK(t->guided_grid <= (1u << 16), "guided_grid must be at most 65536");
+    ARG_CHECK(t->guided_pool <= (1u << 28), "guided_pool must be at most 2^28 paths");
     crh::Tuning& o = ctx->tuning;
     if (t->stack) o.privStack = t->stack - 1u;
     if (t->trace_block) o.traceBlock = t->trace_block;
@@ -477,6 +504,11 @@ int crh_context_set_tuning(crh_context* ctx, const crh_tuning* t) {
     if (t->batch_graph) o.batchGraph = t->batch_graph == 1 ? 1u : 0u;
     if (t->batch_slots) o.batchSlots = t->batch_slots;
     if (t->trace_tris) o.traceTris = t->trace_tris;
+    if (t->guided_fused) o.guidedFused = t->guided_fused == 1 ? 1u : 0u;
+    if (t->guided_stack) o.guidedPriv = t->guided_stack - 1u;
+    if (t->guided_waves) o.guidedWaves = t->guided_waves;
+    if (t->guided_grid) o.guidedGrid = t->guided_grid;
+    if (t->guided_pool) o.guidedPool = t->guided_pool;
     return CRH_OK;
 }
 
@@ -511,6 +543,11 @@ int crh_context_get_tuning(const crh_context* ctx, crh_tuning* t) {
     t->batch_graph = o.batchGraph ? 1u : 2u;
     t->batch_slots = o.batchSlots;
     t->trace_tris = o.traceTris;
+    t->guided_fused = o.guidedFused ? 1u : 2u;
+    t->guided_stack = o.guidedPriv + 1u;
+    t->guided_waves = o.guidedWaves;
+    t->guided_grid = o.guidedGrid;
+    t->guided_pool = o.guidedPool;
     return CRH_OK;
 }
 
@@ -1479,17 +1516,36 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
     double t0 = nowMs();
     HIP_TRY(hipSetDevice(ctx->device));
     const crh::Tuning tun = effective(ctx->tuning, s->view);
+    const bool fused = tun.guidedFused != 0u;
+    const uint32_t gmask = gc->materials ? gc->materials : 3u;
     crh::WfLaunch wl = wfLaunch(tun);
     // The per-path finish never takes guided vertices away: with guiding at
     // every depth it is off, else it starts no earlier than max_depth.
     wl.finishDepth = (gc->max_depth == 0u || wl.finishDepth == 0u)
                          ? 0u
                          : std::max(wl.finishDepth, gc->max_depth);
-    const uint32_t slots = wfSlots(tun, film, spp);
-    HIP_TRY(ensureGuided(film, slots));
-    HIP_TRY(hipMemsetAsync(film->wf.work, 0, 64, ctx->stream));
+    const uint32_t slots = fused ? poolSlots(tun.guidedPool, film, spp) : wfSlots(tun, film, spp);
     const size_t npix = size_t(film->w) * film->h;
     const uint32_t batch = gc->train_batch;
+    uint32_t recCap = 0;
+    if (fused) {
+        HIP_TRY(ensureFilmSlots(film, slots));
+        const uint64_t tiles = uint64_t((film->w + 15u) / 16u) * ((film->h + 15u) / 16u);
+        const uint32_t grid = uint32_t(std::min<uint64_t>(tiles * slots, tun.guidedGrid));
+        // Records for the training the pass group feeds (twice the samples
+        // its steps consume, at least 2^20), at most four per path.
+        if (gc->train)
+            recCap = uint32_t(std::min<uint64_t>(
+                {uint64_t(4) * npix * slots,
+                 std::max<uint64_t>(uint64_t(2) * gc->train_steps * slots * batch, 1u << 20),
+                 0xFFFFFFF0ull}));
+        HIP_TRY(ensureGuidedFused(film, crh::guidedScratchFloats(grid), recCap));
+        HIP_TRY(hipMemsetAsync(film->work.p, 0, film->work.bytes, ctx->stream));
+    } else {
+        HIP_TRY(ensureGuided(film, slots));
+        HIP_TRY(hipMemsetAsync(film->wf.work, 0, 64, ctx->stream));
+        recCap = film->wf.recCap;
+    }
     const size_t padB = (size_t(batch) + 63) & ~size_t(63);
     if (gc->train && padB > g->capSamples) {
         HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * padB * 64 * 2));
@@ -1497,32 +1553,61 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
         g->capSamples = padB;
     }
     // Sample buffer: one pass group's records on top of a batch still filling.
-    const size_t needCap = size_t(film->wf.recCap) + batch;
+    const size_t needCap = size_t(recCap) + batch;
     if (gc->train && film->gCap < needCap) {
         HIP_TRY(film->gSamples.alloc(needCap * sizeof(crh::GuideSample)));
         film->gCap = needCap;
         film->gPending = 0;
     }
-    double mainMs = 0.0;
+    std::vector<hipEvent_t>& evs = film->passEv;
+    const size_t nev = stats ? 2 * size_t((spp + slots - 1) / slots) : 0;
+    while (evs.size() < nev) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreate(&e));
+        evs.push_back(e);
+    }
     uint32_t lastSamples = 0;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     for (uint32_t i = 0; i < spp; i += slots) {
         const uint32_t active = std::min(slots, spp - i);
+        const size_t pg = i / slots;
         crh::GuideSample* smpBuf = film->gSamples.as<crh::GuideSample>();
-        HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, active, film->wf, wl,
-                                   false, ctx->stream, &g->view, gc->alpha,
-                                   gc->train ? smpBuf + film->gPending : nullptr,
-                                   uint32_t(film->gCap - film->gPending),
-                                   gc->materials ? gc->materials : 3u, gc->max_depth,
-                                   gc->target));
+        crh::GuideSample* smpOut = gc->train ? smpBuf + film->gPending : nullptr;
+        const uint32_t smpRoom = uint32_t(film->gCap - film->gPending);
+        if (stats) HIP_TRY(hipEventRecord(evs[2 * pg], ctx->stream));
+        if (fused) {
+            uint32_t* cnt = film->gCnt.as<uint32_t>();
+            const crh::GuidedPassArgs ga{&g->view, gc->alpha, gmask, gc->max_depth,
+                                         film->gScratch.as<float>(),
+                                         gc->train ? film->gRec.as<float4>() : nullptr, cnt,
+                                         recCap};
+            HIP_TRY(crh::launchGuidedPass(s->view, cv, iv, first_sample + i, active,
+                                          film->L.as<float4>(), film->P.as<float2>(),
+                                          film->work.as<unsigned long long>(), tun, ctx->counter,
+                                          ga, ctx->stream));
+        } else {
+            HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, active, film->wf, wl,
+                                       false, ctx->stream, &g->view, gc->alpha, smpOut, smpRoom,
+                                       gmask, gc->max_depth, gc->target));
+        }
+        if (stats) HIP_TRY(hipEventRecord(evs[2 * pg + 1], ctx->stream));
+        const float4* Lp = fused ? film->L.as<float4>() : film->wf.rad;
+        const float2* Pp = fused ? film->P.as<float2>() : film->wf.P;
         for (uint32_t k = 0; k < active; ++k)
-            HIP_TRY(crh::launchFilmGather(film->wf.rad + k * npix, film->wf.P + k * npix,
-                                          film->w, film->h, film->sum.as<float>(),
-                                          film->wsum.as<float>(), ctx->stream));
+            HIP_TRY(crh::launchFilmGather(Lp + k * npix, Pp + k * npix, film->w, film->h,
+                                          film->sum.as<float>(), film->wsum.as<float>(),
+                                          ctx->stream));
         if (gc->train) {
+            uint32_t* trainCounter = fused ? film->gCnt.as<uint32_t>() + 1
+                                           : film->wf.counters + crh::kCTrain;
+            if (fused)
+                HIP_TRY(crh::launchGuidedRecordsToSamples(
+                    film->gRec.as<float4>(), film->gCnt.as<uint32_t>(), recCap,
+                    film->L.as<float4>(), smpOut, smpRoom, trainCounter, gc->target,
+                    ctx->stream));
             uint32_t produced = 0;
-            HIP_TRY(hipMemcpyAsync(&produced, film->wf.counters + crh::kCTrain, 4,
-                                   hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(&produced, trainCounter, 4, hipMemcpyDeviceToHost,
+                                   ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
             produced = uint32_t(std::min<size_t>(produced, film->gCap - film->gPending));
             lastSamples = produced;
@@ -1555,8 +1640,15 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
     if (stats) {
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        double mainMs = 0.0;
+        for (size_t pg = 0; 2 * pg < nev; ++pg) {
+            float m = 0.f;
+            HIP_TRY(hipEventElapsedTime(&m, evs[2 * pg], evs[2 * pg + 1]));
+            mainMs += m;
+        }
         unsigned long long c[3] = {0, 0, 0};
-        HIP_TRY(hipMemcpy(c, film->wf.work, sizeof(c), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(c, fused ? film->work.p : (void*)film->wf.work, sizeof(c),
+                          hipMemcpyDeviceToHost));
         stats->kernel_ms = ms;
         stats->main_kernel_ms = mainMs;
         stats->total_ms = nowMs() - t0;

@@ -1,0 +1,47 @@
+// The fused guided path tracer (guided_mk.hip): persistent path-regenerating
+// megakernel whose waves evaluate the guide on MFMA between the light sample
+// and the continuation of each bounce.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "guiding.h"
+#include "guiding_kernels.h"
+#include "kernels.h"
+
+namespace crh {
+
+// Per-wave output buffer of the network for one pass: 64 lanes x kGuideRaw
+// floats, for every wave of the persistent grid.
+constexpr uint32_t kGuideRaw = 40;
+size_t guidedScratchFloats(uint32_t grid);
+// Dynamic LDS of one block: the traversal stack, then a 3 KB tile per wave.
+size_t guidedLdsBytes(uint32_t stackMode, uint32_t stackCap);
+
+struct GuidedPassArgs {
+    const GuideView* guide;
+    float alpha;
+    uint32_t mask;        // bit t: guide BSDF type t (0 Lambertian, 1 Principled)
+    uint32_t maxDepth;    // guide vertices at depth < maxDepth (0: all)
+    float* scratch;       // guidedScratchFloats(grid) floats
+    float4* rec;          // training records (kRecF4 float4 each), may be null
+    uint32_t* recCounter; // records written this pass (zeroed by the launcher)
+    uint32_t recCap;
+};
+
+// Sample indices si .. si + active - 1 into slots of L and P (as
+// launchRenderPass) with guided continuations; grid = min(tiles * active,
+// t.renderGrid) persistent blocks of 256 threads; next: a device counter.
+hipError_t launchGuidedPass(const SceneView& s, const CameraView& cam, const IntegratorView& cfg,
+                            uint32_t si, uint32_t active, float4* L, float2* P,
+                            unsigned long long* counters, const Tuning& t, uint32_t* next,
+                            const GuidedPassArgs& g, hipStream_t st);
+
+// Training samples from the pass's records (as the wavefront's): incident
+// radiance along the guided direction from the final radiance L[out] of the
+// record's path.  trainCounter receives the samples produced.
+hipError_t launchGuidedRecordsToSamples(const float4* rec, const uint32_t* recCounter,
+                                        uint32_t recCap, const float4* L, GuideSample* out,
+                                        uint32_t cap, uint32_t* trainCounter,
+                                        uint32_t radianceTarget, hipStream_t st);
+
+}  // namespace crh

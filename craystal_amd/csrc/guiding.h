@@ -7,9 +7,9 @@
 
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
-#define CRH_GHD __host__ __device__ __forceinline__
+#define CRH_GHD constexpr __host__ __device__ __forceinline__
 #else
-#define CRH_GHD inline
+#define CRH_GHD constexpr inline
 #endif
 
 namespace crh {

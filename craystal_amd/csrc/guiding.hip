@@ -10,117 +10,21 @@
 // B-operand order, so each operand is one 16-byte LDS read per lane.
 #include <hip/hip_runtime.h>
 
+#include "guide_mlp.h"
 #include "guiding.h"
 #include "guiding_kernels.h"
 
 namespace crh {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-constexpr int kRow = 72;           // LDS row stride of an activation tile
-constexpr int kTile = 16 * kRow;   // elements per activation tile
 constexpr int kWavesPerBlock = 4;
 
 namespace {
-
-__device__ __forceinline__ __bf16 tobf(float v) { return static_cast<__bf16>(v); }
-__device__ __forceinline__ float frombf(__bf16 v) { return static_cast<float>(v); }
-
-__device__ __forceinline__ float softplusf(float x) {
-    return x > 20.f ? x : log1pf(expf(fminf(x, 20.f)));
-}
 
 // Stage the forward weights (5 layers x 4096 bf16) in LDS.
 __device__ void stageWeights(const __bf16* __restrict__ wq, __bf16* ldsW) {
     const uint4* src = reinterpret_cast<const uint4*>(wq);
     uint4* dst = reinterpret_cast<uint4*>(ldsW);
     for (int i = threadIdx.x; i < kGQElems / 8; i += blockDim.x) dst[i] = src[i];
-}
-
-// Input features of sample m of the tile into row m (lanes cover 16
-// features each).  valid = false writes a zero row.
-__device__ void featureRow(__bf16* H, int m, int chunk, bool valid, const float* pos,
-                           const float* nrm, const float* wo, const float* bmin,
-                           const float* bext) {
-    for (int j = 0; j < 16; ++j) {
-        int f = chunk * 16 + j;
-        float v = 0.f;
-        if (valid) {
-            if (f < 30) {
-                int a = f / 10, r = f % 10, lvl = r >> 1;
-                float e = bext[a] > 0.f ? bext[a] : 1.f;
-                float p = (pos[a] - bmin[a]) / e;
-                float arg = float(1 << lvl) * 3.14159265358979323846f * p;
-                v = (r & 1) ? cosf(arg) : sinf(arg);
-            } else if (f >= 32 && f < 35) {
-                v = nrm[f - 32];
-            } else if (f >= 35 && f < 38) {
-                v = wo[f - 35];
-            }
-        }
-        H[m * kRow + f] = tobf(v);
-    }
-}
-
-// One layer: out = act(in * W_l + b_l) for the wave's 16-row tile.
-// For the last layer the fp32 result goes to outF ([16][48]).
-__device__ void layerForward(int l, const __bf16* in, __bf16* out, float* outF,
-                             const __bf16* ldsW, const float* __restrict__ theta) {
-    const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
-    const int N = gLayerN(l), NT = N >> 4;
-    const float* bias = theta + gBOff(l);
-    for (int t = 0; t < NT; ++t) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            bf16x8 a = *reinterpret_cast<const bf16x8*>(in + r16 * kRow + 32 * s + 8 * g);
-            bf16x8 b = *reinterpret_cast<const bf16x8*>(ldsW + gQOff(l) +
-                                                         ((s * NT + t) * 64 + lane) * 8);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-        }
-        const int col = 16 * t + r16;
-        const float bc = bias[col];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = 4 * g + i;
-            float v = acc[i] + bc;
-            if (l < kGLayers - 1)
-                out[row * kRow + col] = tobf(fmaxf(v, 0.f));
-            else
-                outF[row * kGOut + col] = v;
-        }
-    }
-}
-
-struct Head {
-    float w[kGLobes], mu[kGLobes][3], k[kGLobes], kraw[kGLobes], mlen[kGLobes];
-};
-
-__device__ void computeHead(const float* o, Head& h) {
-    float amax = -INFINITY;
-    for (int j = 0; j < kGLobes; ++j) amax = fmaxf(amax, o[5 * j]);
-    float sum = 0.f;
-    for (int j = 0; j < kGLobes; ++j) {
-        h.w[j] = expf(o[5 * j] - amax);
-        sum += h.w[j];
-    }
-    for (int j = 0; j < kGLobes; ++j) {
-        h.w[j] /= sum;
-        float mx = o[5 * j + 1], my = o[5 * j + 2], mz = o[5 * j + 3];
-        float len = fmaxf(sqrtf(mx * mx + my * my + mz * mz), 1e-6f);
-        h.mlen[j] = len;
-        h.mu[j][0] = mx / len;
-        h.mu[j][1] = my / len;
-        h.mu[j][2] = mz / len;
-        h.kraw[j] = o[5 * j + 4];
-        h.k[j] = softplusf(h.kraw[j]) + kGKappaMin;
-    }
-}
-
-__device__ __forceinline__ float vmfPdf(float k, float c) {
-    // k / (2 pi (1 - e^{-2k})) * exp(k (c - 1))
-    return k / (2.f * 3.14159265358979323846f * (-expm1f(-2.f * k))) * expf(k * (c - 1.f));
 }
 
 }  // namespace

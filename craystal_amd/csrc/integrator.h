@@ -131,6 +131,16 @@ CRH_D void misBsdfPart(const SceneView& s, F3 pos, const Ray& r, bool isHit,
 }
 
 // CameraProxy::generateRay (Core/CameraProxy.h:25-40).
+// Pixel `inTile` (0..255) of a 16x16 tile; each 64-pixel quarter is an 8x8
+// block (one wave of a 256-thread block).
+CRH_D void tileCoordOf(uint32_t tile, uint32_t inTile, uint32_t tilesX, uint32_t& x,
+                       uint32_t& y) {
+    uint32_t tx = tile % tilesX, ty = tile / tilesX;
+    uint32_t wave = inTile >> 6, lane = inTile & 63u;
+    x = tx * 16u + (wave & 1u) * 8u + (lane & 7u);
+    y = ty * 16u + (wave >> 1) * 8u + (lane >> 3);
+}
+
 CRH_D Ray cameraRay(const CameraView& cam, F2 pix) {
     float u = (2.0f * (pix.x / float(cam.width)) - 1.0f);
     float v = (1.0f - 2.0f * (pix.y / float(cam.height)));

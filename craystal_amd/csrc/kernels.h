@@ -48,6 +48,12 @@ struct Tuning {
     uint32_t batchGrid = 1024;  // persistent blocks
     uint32_t batchChunks = 8;   // item ranges with their own counters (1..8)
     uint32_t batchGraph = 1;    // 1: passes replayed from a captured hipGraph
+    // fused guided megakernel
+    uint32_t guidedFused = 1;         // 1: guided rendering in the fused megakernel
+    uint32_t guidedPriv = 2;          // stack mode
+    uint32_t guidedWaves = 6;         // register budget (waves per SIMD)
+    uint32_t guidedGrid = 4096;       // persistent blocks
+    uint32_t guidedPool = 1u << 23;   // paths per launch
 };
 
 constexpr uint32_t kBatchMaxChunks = 8;

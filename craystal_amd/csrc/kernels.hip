@@ -98,15 +98,6 @@ CRH_D uint32_t xcdBand(uint32_t b, uint32_t n) {
     return (b % 8u) * per + b / 8u;
 }
 
-// Pixel `inTile` (0..255) of a 16x16 tile; each 64-pixel quarter is an 8x8
-// block (one wave of a 256-thread block).
-CRH_D void tileCoordOf(uint32_t tile, uint32_t inTile, uint32_t tilesX, uint32_t& x,
-                       uint32_t& y) {
-    uint32_t tx = tile % tilesX, ty = tile / tilesX;
-    uint32_t wave = inTile >> 6, lane = inTile & 63u;
-    x = tx * 16u + (wave & 1u) * 8u + (lane & 7u);
-    y = ty * 16u + (wave >> 1) * 8u + (lane >> 3);
-}
 CRH_D void tileCoord(uint32_t tile, uint32_t tilesX, uint32_t& x, uint32_t& y) {
     tileCoordOf(tile, threadIdx.x, tilesX, x, y);
 }

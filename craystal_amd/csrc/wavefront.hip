@@ -376,31 +376,8 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
         if (!guided) {
             f = bsdfSampleEval(bsdf, smp, it.view, wi, pdf);
         } else {
-            // One-sample MIS of BSDF sampling (fraction alpha) and the
-            // guide's mixture.  The weight f * (p_act / p_rep) / q_mix has
-            // the same expectation as the plain estimator's f / p_rep with
-            // wi ~ p_act, so guided and plain images converge to the same
-            // (reference) result; p_rep is the pdf the reference reports,
-            // p_act the density its sampler actually has.
             Mixture mix = loadMixture(w.gMix + size_t(p) * 40);
-            F3 woL = toLocal(bsdf.frame, it.view);
-            F3 wiL;
-            if (smp.next1D() < alpha) {
-                F2 u = smp.next2D();
-                float dummy;
-                wiL = sampleLocal(bsdf, woL, u, dummy);
-                wi = toWorld(bsdf.frame, wiL);
-            } else {
-                float u0 = smp.next1D();
-                F2 u = smp.next2D();
-                wi = mixtureSample(mix, u0, u.x, u.y);
-                wiL = toLocal(bsdf.frame, wi);
-            }
-            f = evalLocal(bsdf, woL, wiL);
-            float prep = pdfLocal(bsdf, woL, wiL);
-            float pact = actualPdfLocal(bsdf, woL, wiL);
-            qmix = alpha * pact + (1.f - alpha) * mixturePdf(mix, wi);
-            pdf = (prep > 0.f && pact > 0.f && qmix > 0.f) ? qmix * prep / pact : 0.f;
+            f = guidedSampleEval(bsdf, mix, alpha, smp, it.view, wi, pdf, qmix);
         }
         bool record = false;
         Spec betaW = beta;

@@ -83,9 +83,12 @@ class Guide:
 
 
 class GuidedIntegrator:
-    """Wavefront path tracer with one-sample MIS between BSDF sampling
-    (fraction alpha) and the guide; optionally trains the guide online on
-    every pass's vertices (Adam, KL divergence)."""
+    """Path tracer with one-sample MIS between BSDF sampling (fraction alpha)
+    and the guide; optionally trains the guide online on every pass's
+    vertices (Adam, KL divergence).  It runs on the fused megakernel (the
+    guide evaluated inside the path kernel) or the wavefront, as the
+    context's tuning field guided_fused says; both give the same film for
+    the same guide."""
 
     LAMBERTIAN, PRINCIPLED = 1, 2
 
@@ -121,7 +124,8 @@ class GuidedIntegrator:
                self.guide.handle, C.byref(g), first_sample, spp, sensor.handle, C.byref(st))
         if readback:
             sensor.readbackDeviceData()
-        self.last_stats = {"kernel_ms": st.kernel_ms, "total_ms": st.total_ms, "rays": st.rays,
+        self.last_stats = {"kernel_ms": st.kernel_ms, "main_kernel_ms": st.main_kernel_ms,
+                           "total_ms": st.total_ms, "rays": st.rays,
                            "train_samples": st.node_visits}
         return self.last_stats
 

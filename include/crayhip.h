@@ -291,6 +291,13 @@ typedef struct crh_tuning {
                                 set of scenes still sampling stays the same) */
     uint32_t trace_tris;     /* ray batches, persistent: triangles of a leaf tested per
                                 traversal step, 1..4 */
+    uint32_t guided_fused;   /* guided rendering: 1 = the fused megakernel (the guide's
+                                inference inside the path kernel), 2 = the wavefront.  Same
+                                film for the same guide */
+    uint32_t guided_stack;   /* fused guided megakernel: stack as above */
+    uint32_t guided_waves;   /* fused guided megakernel: register budget as render_waves */
+    uint32_t guided_grid;    /* fused guided megakernel: persistent blocks */
+    uint32_t guided_pool;    /* fused guided megakernel: paths per launch, as render_pool */
 } crh_tuning;
 
 /* ---- errors / device ------------------------------------------------- */
