@@ -37,29 +37,64 @@ def _run(cmd: list[str]) -> None:
     subprocess.run(cmd, check=True)
 
 
-def _run_all(cmds: list[list[str]]) -> None:
-    """The compile commands at once (independent translation units)."""
+def _run_all(cmds: list[list[str]], jobs: int | None = None) -> None:
+    """The compile commands (independent translation units), at most `jobs`
+    at a time (default: the CPU count, at most 8)."""
+    jobs = jobs or max(1, min(8, os.cpu_count() or 1))
+    running: list[tuple[subprocess.Popen, list[str]]] = []
+    failed: list[tuple[int, list[str]]] = []
+
+    def reap(block: bool) -> None:
+        for p, c in list(running):
+            rc = p.wait() if block else p.poll()
+            if rc is not None:
+                running.remove((p, c))
+                if rc != 0:
+                    failed.append((rc, c))
+                if block:
+                    return
+
     for c in cmds:
+        while len(running) >= jobs:
+            reap(block=False)
+            if len(running) >= jobs:
+                running[0][0].wait()
         print(" ".join(c), flush=True)
-    procs = [subprocess.Popen(c) for c in cmds]
-    codes = [p.wait() for p in procs]
-    for c, rc in zip(cmds, codes):
-        if rc != 0:
-            raise subprocess.CalledProcessError(rc, c)
+        running.append((subprocess.Popen(c), c))
+    while running:
+        reap(block=True)
+    if failed:
+        raise subprocess.CalledProcessError(failed[0][0], failed[0][1])
 
 
 def build(verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
     objs, cmds = [], []
+    hipcc = str(ROCM / "bin" / "hipcc")
+
+    def hip(src: str, obj: str, *defs: str) -> None:
+        o = OBJ / obj
+        cmds.append([hipcc, *HIPCC_FLAGS, *defs, "-c", str(CSRC / src), "-o", str(o)])
+        objs.append(str(o))
+
+    # Longest compiles first.
+    # The fused guided kernel: one object per stack mode, then the rest.
+    for part in range(4):
+        hip("guided_mk.hip", f"guided_mk_{part}.hip.o", f"-DCRH_GMK_PART={part}")
+    hip("wavefront.hip", "wavefront.hip.o")
+    for sm in range(3):
+        hip("wf_finish.hip", f"wf_finish_{sm}.hip.o", f"-DCRH_WF_SM={sm}")
+    # The megakernel families, one object per (family, stack mode), so that
+    # their many register-budget variants compile in parallel.
+    for fam in range(3):
+        for sm in range(3):
+            hip("mk_inst.hip", f"mk_inst_{fam}_{sm}.hip.o", f"-DCRH_MK_FAMILY={fam}",
+                f"-DCRH_MK_SM={sm}")
+    for src in ("kernels.hip", "guiding.hip", "noalloc.hip", "bvh_device.hip"):
+        hip(src, src + ".o")
     for src in ("api.cpp", "bvh_build.cpp", "spectral.cpp"):
         o = OBJ / (src + ".o")
         cmds.append(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
-        objs.append(str(o))
-    for src in ("kernels.hip", "wavefront.hip", "guiding.hip", "noalloc.hip", "bvh_device.hip",
-                "guided_mk.hip"):
-        o = OBJ / (src + ".o")
-        cmds.append([str(ROCM / "bin" / "hipcc"), *HIPCC_FLAGS, "-c", str(CSRC / src), "-o",
-                     str(o)])
         objs.append(str(o))
     _run_all(cmds)
     tmp = OUT.with_suffix(".so.tmp")

@@ -1193,8 +1193,6 @@ int crh_render(crh_scene* s, const crh_camera* cam,
               "unknown integrator mode");
     ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
     ARG_CHECK(!cfg->spectral || s->view.spec, "spectral transport needs a scene with spectra");
-    ARG_CHECK(!cfg->spectral || cfg->mode == CRH_MODE_MEGAKERNEL,
-              "spectral transport runs on the megakernel");
     ARG_CHECK(!cfg->spectral || !cfg->count_work, "spectral transport has no counting variant");
     GUARD_BEGIN
     crh_context* ctx = s->ctx;
@@ -1236,7 +1234,8 @@ int crh_render(crh_scene* s, const crh_camera* cam,
         if (stats) HIP_TRY(hipEventRecord(evs[2 * g], ctx->stream));
         if (wavefront)
             HIP_TRY(crh::wavefrontPass(s->view, cv, iv, first_sample + i, active, film->wf, wl,
-                                       cfg->count_work != 0, ctx->stream));
+                                       cfg->count_work != 0, ctx->stream, nullptr, 0.5f, nullptr,
+                                       0u, 3u, 0u, 0u, cfg->spectral != 0));
         else
             HIP_TRY(crh::launchRenderPass(s->view, cv, iv, first_sample + i, active,
                                           film->L.as<float4>(), film->P.as<float2>(),
@@ -1700,8 +1699,8 @@ int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh
         &crh_scene::nodes,    &crh_scene::tris,    &crh_scene::pos,     &crh_scene::nrm,
         &crh_scene::uv,       &crh_scene::idx,     &crh_scene::primMat, &crh_scene::mats,
         &crh_scene::emissive, &crh_scene::spheres, &crh_scene::sphereMat,
-        &crh_scene::texels,   &crh_scene::texInfo};
-    constexpr int kR = 13;
+        &crh_scene::texels,   &crh_scene::texInfo, &crh_scene::spec};
+    constexpr int kR = 14;
     std::vector<uint32_t> first(n);
     std::unordered_map<const crh_scene*, uint32_t> seen;
     for (uint32_t i = 0; i < n; ++i) first[i] = seen.emplace(scenes[i], i).first->second;
@@ -1750,7 +1749,7 @@ int crh_batch_create(crh_context* ctx, crh_scene* const* scenes, uint32_t n, crh
         v.sphereMat = sc->view.sphereMat ? reinterpret_cast<const uint32_t*>(at(10)) : nullptr;
         v.texels = reinterpret_cast<const float*>(at(11));
         v.texInfo = reinterpret_cast<const uint32_t*>(at(12));
-        v.spec = nullptr;  // batched rendering transports RGB
+        v.spec = reinterpret_cast<const float*>(at(13));
         b->views[i] = v;
         if (first[i] == i) {
             b->nodes += sc->nodes.bytes / sizeof(crh::GNode);
@@ -1799,7 +1798,6 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
                      crh_stats* stats) {
     ARG_CHECK(b && cams && films && spp && cfg, "NULL argument");
     ARG_CHECK(cfg->mode == CRH_MODE_MEGAKERNEL, "batched rendering uses the megakernel mode");
-    ARG_CHECK(!cfg->spectral, "batched rendering transports RGB");
     ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
     const uint32_t n = uint32_t(b->views.size());
     uint32_t maxSpp = 0;
@@ -1883,7 +1881,7 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
             hipError_t e = crh::launchBatchPass(dTab, dTile, dPix, active, tilePre[active],
                                                 pixPre[active], iv, first_sample + k, g,
                                                 b->maxStack, tun, b->next.as<uint32_t>(), cnt,
-                                                st);
+                                                st, cfg->spectral != 0);
             if (e != hipSuccess) return e;
             k += g;
         }
@@ -1896,7 +1894,8 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
         std::vector<uint64_t> sig = {n, first_sample, maxSpp, iv.maxDepth,
                                      uint64_t(__builtin_bit_cast(uint32_t, iv.rrThreshold)),
                                      iv.rrStartDepth, iv.useMIS, tun.batchPriv, tw,
-                                     tun.batchGrid, tun.batchChunks, tun.batchWaves, maxSlots};
+                                     tun.batchGrid, tun.batchChunks, tun.batchWaves, maxSlots,
+                                     cfg->spectral != 0 ? 1u : 0u};
         for (uint32_t j = 0; j < n; ++j) sig.push_back(spp[order[j]]);
         sig.insert(sig.end(), pre.begin(), pre.end());
         if (!b->exec || sig != b->sig) {

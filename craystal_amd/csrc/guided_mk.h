@@ -36,6 +36,29 @@ hipError_t launchGuidedPass(const SceneView& s, const CameraView& cam, const Int
                             unsigned long long* counters, const Tuning& t, uint32_t* next,
                             const GuidedPassArgs& g, hipStream_t st);
 
+// One launch of the fused kernel; launchGuidedSM<SM> (defined once per
+// stack mode, guided_mk.hip built with CRH_GMK_PART = SM) picks the register
+// budget and MIS variant.
+struct GuidedLaunch {
+    SceneView s;
+    CameraView cam;
+    IntegratorView cfg;
+    uint32_t si, tilesX, tiles, total;
+    uint32_t* next;
+    float4* L;
+    float2* P;
+    unsigned long long* counters;
+    const GuidedPassArgs* g;
+    uint32_t grid, tileOff;
+    size_t lds;
+    uint32_t waves;
+    hipStream_t st;
+};
+template <int SM> hipError_t launchGuidedSM(const GuidedLaunch& a);
+template <> hipError_t launchGuidedSM<0>(const GuidedLaunch&);
+template <> hipError_t launchGuidedSM<1>(const GuidedLaunch&);
+template <> hipError_t launchGuidedSM<2>(const GuidedLaunch&);
+
 // Training samples from the pass's records (as the wavefront's): incident
 // radiance along the guided direction from the final radiance L[out] of the
 // record's path.  trainCounter receives the samples produced.

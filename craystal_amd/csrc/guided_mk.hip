@@ -28,6 +28,10 @@
 #include "wave.h"
 #include "wavefront.h"
 
+#ifndef CRH_GMK_PART
+#error "compile with -DCRH_GMK_PART=<0|1|2> (kernels of one stack mode) or 3 (the rest)"
+#endif
+
 namespace crh {
 
 namespace {
@@ -291,6 +295,7 @@ kRenderGuided(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint
     if (lane == 0 && r64) atomicAdd(counters, r64);
 }
 
+#if CRH_GMK_PART == 3
 // Records -> training samples, as kWfRecordsToSamples with the final
 // radiance from L.
 __global__ __launch_bounds__(256) void kGuidedRecordsToSamples(const float4* __restrict__ rec,
@@ -342,8 +347,11 @@ __global__ void kGuidedReset(uint32_t* __restrict__ a, uint32_t* __restrict__ b)
     }
 }
 
+#endif  // CRH_GMK_PART == 3
+
 }  // namespace
 
+#if CRH_GMK_PART == 3
 size_t guidedScratchFloats(uint32_t grid) {
     return size_t(grid) * (kRenderBlock / 64u) * 64u * kGuideRaw;
 }
@@ -367,22 +375,11 @@ hipError_t launchGuidedPass(const SceneView& s, const CameraView& cam, const Int
         uint32_t((stackLdsBytes(t.guidedPriv, s.stackCap, kRenderBlock) + 15) & ~size_t(15));
     const size_t lds = guidedLdsBytes(t.guidedPriv, s.stackCap);
     hipLaunchKernelGGL(kGuidedReset, dim3(1), dim3(64), 0, st, next, g.recCounter);
-    withStack(t.guidedPriv, [&](auto sm) {
-        constexpr int SM = decltype(sm)::value;
-        withWaves(t.guidedWaves, [&](auto wv) {
-            constexpr int Wv = decltype(wv)::value;
-            if (cfg.useMIS)
-                hipLaunchKernelGGL((kRenderGuided<true, SM, Wv>), dim3(grid), dim3(kRenderBlock),
-                                   lds, st, s, cam, cfg, si, tilesX, tiles, total, next, L, P,
-                                   counters, *g.guide, g.alpha, g.mask, g.maxDepth, g.scratch,
-                                   tileOff, g.rec, g.recCounter, g.recCap);
-            else
-                hipLaunchKernelGGL((kRenderGuided<false, SM, Wv>), dim3(grid), dim3(kRenderBlock),
-                                   lds, st, s, cam, cfg, si, tilesX, tiles, total, next, L, P,
-                                   counters, *g.guide, g.alpha, g.mask, g.maxDepth, g.scratch,
-                                   tileOff, g.rec, g.recCounter, g.recCap);
-        });
-    });
+    const GuidedLaunch gl{s, cam, cfg, si, tilesX, tiles, total, next, L, P, counters, &g,
+                          grid, tileOff, lds, t.guidedWaves, st};
+    hipError_t e = hipSuccess;
+    withStack(t.guidedPriv, [&](auto sm) { e = launchGuidedSM<decltype(sm)::value>(gl); });
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
@@ -395,5 +392,31 @@ hipError_t launchGuidedRecordsToSamples(const float4* rec, const uint32_t* recCo
                        recCap, L, out, cap, trainCounter, radianceTarget);
     return hipGetLastError();
 }
+
+#endif  // CRH_GMK_PART == 3
+
+#if CRH_GMK_PART < 3
+// kRenderGuided for one stack mode (this object's CRH_GMK_PART), every
+// register budget and MIS on / off.
+template <>
+hipError_t launchGuidedSM<CRH_GMK_PART>(const GuidedLaunch& a) {
+    constexpr int SM = CRH_GMK_PART;
+    const GuidedPassArgs& g = *a.g;
+    withWaves(a.waves, [&](auto wv) {
+        constexpr int Wv = decltype(wv)::value;
+        if (a.cfg.useMIS)
+            hipLaunchKernelGGL((kRenderGuided<true, SM, Wv>), dim3(a.grid), dim3(kRenderBlock),
+                               a.lds, a.st, a.s, a.cam, a.cfg, a.si, a.tilesX, a.tiles, a.total,
+                               a.next, a.L, a.P, a.counters, *g.guide, g.alpha, g.mask,
+                               g.maxDepth, g.scratch, a.tileOff, g.rec, g.recCounter, g.recCap);
+        else
+            hipLaunchKernelGGL((kRenderGuided<false, SM, Wv>), dim3(a.grid), dim3(kRenderBlock),
+                               a.lds, a.st, a.s, a.cam, a.cfg, a.si, a.tilesX, a.tiles, a.total,
+                               a.next, a.L, a.P, a.counters, *g.guide, g.alpha, g.mask,
+                               g.maxDepth, g.scratch, a.tileOff, g.rec, g.recCounter, g.recCap);
+    });
+    return hipGetLastError();
+}
+#endif
 
 }  // namespace crh

@@ -106,7 +106,52 @@ hipError_t launchBatchPass(const BatchScene* tab, const uint32_t* tilePrefix,
                            const uint32_t* pixPrefix, uint32_t nActive, uint32_t items,
                            uint32_t pixels, const IntegratorView& cfg, uint32_t si,
                            uint32_t slots, uint32_t stackCap, const Tuning& t, uint32_t* next,
-                           unsigned long long* counters, hipStream_t st);
+                           unsigned long long* counters, hipStream_t st, bool spectral = false);
+// Arguments of one megakernel launch (kRenderPass or kRenderRegen) and the
+// per-stack-mode launchers: mk_inst.hip is compiled once per kernel family
+// and stack mode, each defining one of the specialisations below.
+struct MkArgs {
+    SceneView s;
+    CameraView cam;
+    IntegratorView cfg;
+    uint32_t si, tilesX, tiles, band, total;  // total: paths of a regenerating launch
+    uint32_t* next;
+    float4* L;
+    float2* P;
+    unsigned long long* counters;
+    dim3 grid;
+    size_t lds;
+    uint32_t waves;
+    bool mis, count, spectral;
+    hipStream_t st;
+};
+struct BatchArgs {
+    const BatchScene* tab;
+    const uint32_t* tilePrefix;
+    uint32_t nActive;
+    IntegratorView cfg;
+    uint32_t si, slots, tileW, chunks;
+    uint32_t* next;
+    unsigned long long* counters;
+    dim3 grid;
+    size_t lds;
+    uint32_t waves;
+    bool mis, spectral;
+    hipStream_t st;
+};
+template <int SM> hipError_t launchRenderSM(const MkArgs& a);
+template <int SM> hipError_t launchRegenSM(const MkArgs& a);
+template <int SM> hipError_t launchBatchSM(const BatchArgs& a);
+template <> hipError_t launchRenderSM<0>(const MkArgs&);
+template <> hipError_t launchRenderSM<1>(const MkArgs&);
+template <> hipError_t launchRenderSM<2>(const MkArgs&);
+template <> hipError_t launchRegenSM<0>(const MkArgs&);
+template <> hipError_t launchRegenSM<1>(const MkArgs&);
+template <> hipError_t launchRegenSM<2>(const MkArgs&);
+template <> hipError_t launchBatchSM<0>(const BatchArgs&);
+template <> hipError_t launchBatchSM<1>(const BatchArgs&);
+template <> hipError_t launchBatchSM<2>(const BatchArgs&);
+
 hipError_t launchFilmGather(const float4* L, const float2* P, uint32_t W,
                             uint32_t H, float* sum, float* wsum,
                             hipStream_t st);

@@ -80,6 +80,8 @@ struct WfLaunch {
 struct GuideView;
 struct GuideSample;
 
+// spectral: hero-wavelength transport of the scene's spectra (not with a
+// guide or counted work); the pass's radiance buffer then holds XYZ.
 // guide != null: guided continuation sampling (one-sample MIS with the
 // guide's mixture, fraction alpha for BSDF sampling).  samples != null:
 // also convert this pass's vertex records into training samples.
@@ -92,6 +94,23 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                          const GuideView* guide = nullptr, float alpha = 0.5f,
                          GuideSample* samples = nullptr, uint32_t sampleCap = 0,
                          uint32_t guideMask = 3u, uint32_t guideDepth = 0u,
-                         uint32_t guideTarget = 0u);
+                         uint32_t guideTarget = 0u, bool spectral = false);
+
+// The finishing launch (kWfFinish) of one pass; launchWfFinishSM<SM> is
+// defined once per stack mode (wf_finish.hip built with CRH_WF_SM = SM).
+struct WfFinishArgs {
+    SceneView s;
+    WfState w;
+    IntegratorView cfg;
+    uint32_t cur, d0, grid;
+    size_t lds;
+    uint32_t waves;
+    bool spectral;
+    hipStream_t st;
+};
+template <int SM> hipError_t launchWfFinishSM(const WfFinishArgs& a);
+template <> hipError_t launchWfFinishSM<0>(const WfFinishArgs&);
+template <> hipError_t launchWfFinishSM<1>(const WfFinishArgs&);
+template <> hipError_t launchWfFinishSM<2>(const WfFinishArgs&);
 
 }  // namespace crh

@@ -28,62 +28,13 @@
 
 #include "guide_device.h"
 #include "guiding_kernels.h"
-#include "integrator.h"
-#include "path.h"
-#include "wavefront.h"
 #include "kernels.h"
 #include "wave.h"
+#include "wf_device.h"
 
 namespace crh {
 
-namespace {
-
-enum : uint32_t {
-    F_MIS = 1u,      // MIS ray pending (traced by the next extend)
-    F_LIGHT = 2u,    // light geometry valid, shadow ray traced
-    F_MISHIT = 4u,   // MIS ray query returned a hit (intersect() == true)
-    F_EXTHIT = 8u,   // extension query returned a hit
-    F_ALIVE = 16u,   // the path continues (it is in the next extension queue)
-};
-
-CRH_D uint32_t laneId() { return __lane_id(); }
-
-CRH_D Ray loadRay(const float4* o, const float4* d, uint32_t p) {
-    float4 a = o[p], b = d[p];
-    return Ray{mk3(a.x, a.y, a.z), a.w, mk3(b.x, b.y, b.z), b.w};
-}
-CRH_D void storeRay(float4* o, float4* d, uint32_t p, const Ray& r) {
-    o[p] = make_float4(r.o.x, r.o.y, r.o.z, r.tMin);
-    d[p] = make_float4(r.d.x, r.d.y, r.d.z, r.tMax);
-}
-CRH_D Hit loadHit(const uint4* h, uint32_t p) {
-    uint4 v = h[p];
-    return Hit{v.x, __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w)};
-}
-CRH_D void storeHit(uint4* h, uint32_t p, const Hit& x) {
-    h[p] = make_uint4(x.prim, __float_as_uint(x.t), __float_as_uint(x.u), __float_as_uint(x.v));
-}
-CRH_D Spec ldS(const float4* a, uint32_t p) {
-    float4 v = a[p];
-    return Spec{v.x, v.y, v.z};
-}
-CRH_D void stS(float4* a, uint32_t p, Spec s, float w = 0.f) {
-    a[p] = make_float4(s.r, s.g, s.b, w);
-}
-CRH_D Sampler loadSampler(const WfState& w, uint32_t p) {
-    uint32_t pix = p % w.npix;
-    uint32_t x = pix % w.width, y = pix / w.width;
-    Sampler s(0u, 0u, 0u);
-    s.state = w.rng[p];
-    s.inc = x * 1234567891u + y * 987654321u + 1u;
-    return s;
-}
-
-#define PERSISTENT_LOOP(i, count)                                            \
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (count);    \
-         i += gridDim.x * blockDim.x)
-
-}  // namespace
+using namespace wf;
 
 // Counter layout (WfCounters): ext[2], mis[2], shadow, cont.
 __global__ __launch_bounds__(64) void kWfReset(uint32_t* c, uint32_t mask) {
@@ -91,8 +42,11 @@ __global__ __launch_bounds__(64) void kWfReset(uint32_t* c, uint32_t mask) {
     if (i < kWfCounters && (mask >> i) & 1u) c[i] = 0u;
 }
 
-__global__ __launch_bounds__(256) void kWfGenerate(WfState w, CameraView cam,
-                                                   uint32_t si) {
+// SPEC: the path draws its hero bin right after the pixel jitter (as the
+// megakernel's tracePixel) and keeps it in its flags.
+template <bool SPEC>
+__global__ __launch_bounds__(256) void kWfGenerate(WfState w, CameraView cam, uint32_t si,
+                                                   const float* spec) {
     uint32_t n = w.active * w.npix;
     PERSISTENT_LOOP(p, n) {
         uint32_t px = p % w.npix;
@@ -100,12 +54,13 @@ __global__ __launch_bounds__(256) void kWfGenerate(WfState w, CameraView cam,
         Sampler smp(x, y, si + p / w.npix);
         F2 j = smp.next2D();
         F2 pix{float(x) + j.x, float(y) + j.y};
+        const uint32_t hero = SPEC ? heroLanes(spec, smp.next1D()).b0 : 0u;
         Ray r = cameraRay(cam, pix);
         storeRay(w.rayO, w.rayD, p, r);
         w.rng[p] = smp.state;
         stS(w.beta, p, sp(1.f));
         stS(w.rad, p, sp(0.f));
-        w.flags[p] = F_ALIVE;
+        w.flags[p] = F_ALIVE | (hero << kHeroShift);
         if (w.recId) w.recId[p] = kNoRecord;
         w.P[p] = make_float2(pix.x, pix.y);
         w.qExt[0][p] = p;
@@ -210,6 +165,7 @@ __global__ __launch_bounds__(256) void kWfShadowPT(SceneView s, WfState w, uint3
 
 // Resolve the MIS estimate of the previous bounce:
 // radiance += (value / selPdf) * beta_k  (Core/Integrator.cu:139, :186).
+template <bool SPEC>
 __global__ __launch_bounds__(256) void kWfMisResolve(SceneView s, WfState w,
                                                      uint32_t cur) {
     const uint32_t n = w.counters[kCMis0 + cur];
@@ -223,7 +179,8 @@ __global__ __launch_bounds__(256) void kWfMisResolve(SceneView s, WfState w,
         float4 bw4 = w.misBw[p];
         float4 pe = w.misPos[p];
         misBsdfPart(s, mk3(pe.x, pe.y, pe.z), r, (fl & F_MISHIT) != 0u, h,
-                    __float_as_uint(pe.w), Spec{bw4.x, bw4.y, bw4.z}, bw4.w, value);
+                    __float_as_uint(pe.w), Spec{bw4.x, bw4.y, bw4.z}, bw4.w, value,
+                    SPEC ? pathLanes(s, fl) : rgbLanes());
         Spec rad = ldS(w.rad, p);
         rad = rad + (value / vs.w) * ldS(w.misBeta, p);
         stS(w.rad, p, rad);
@@ -233,7 +190,7 @@ __global__ __launch_bounds__(256) void kWfMisResolve(SceneView s, WfState w,
 
 // Process the extension hit: environment / emitter terms, or light
 // selection producing a shadow ray (Core/Integrator.cu:165-190, :25-53).
-template <bool GUIDED>
+template <bool GUIDED, bool SPEC>
 __global__ __launch_bounds__(256) void kWfShadeHit(SceneView s, WfState w,
                                                    uint32_t cur) {
     const uint32_t n = w.counters[kCExt0 + cur];
@@ -252,14 +209,17 @@ __global__ __launch_bounds__(256) void kWfShadeHit(SceneView s, WfState w,
             }
         }
         bool shade = false, shadow = false;
+        const Lanes lw = SPEC ? pathLanes(s, fl) : rgbLanes();
         if (!(fl & F_EXTHIT)) {
-            stS(w.rad, p, ldS(w.rad, p) + sp3(s.env) * beta);
+            const Spec env = SPEC ? laneSpec(s.spec + kSpecEnv, lw) : sp3(s.env);
+            stS(w.rad, p, ldS(w.rad, p) + env * beta);
         } else {
             Ray r = loadRay(w.rayO, w.rayD, p);
             Hit h = loadHit(w.hit, p);
-            const GMaterial& m = s.mats[matIdOf(s, h.prim)];
+            const uint32_t mid = matIdOf(s, h.prim);
+            const GMaterial& m = s.mats[mid];
             if (m.flags & 1u) {
-                stS(w.rad, p, ldS(w.rad, p) + sp3(m.emission) * beta);
+                stS(w.rad, p, ldS(w.rad, p) + emissionOf(m, mid, lw) * beta);
             } else {
                 shade = true;
                 Isect it = makeIsect(s, r, h);
@@ -309,7 +269,7 @@ __global__ __launch_bounds__(256) void kWfShadow(SceneView s, WfState w) {
 
 // Light weight, MIS BSDF ray and the continuation sample
 // (Core/Integrator.cu:57-99, :186-223).
-template <bool MIS, bool GUIDED>
+template <bool MIS, bool GUIDED, bool SPEC>
 __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
                                                     IntegratorView cfg,
                                                     uint32_t cur, uint32_t depth,
@@ -324,7 +284,8 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
         Isect it = makeIsect(s, r, h);
         const uint32_t mid = matIdOf(s, h.prim);
         const GMaterial& m = s.mats[mid];
-        BSDF bsdf = makeBSDF(s, m, mid, it, h, rgbLanes());
+        const Lanes lw = SPEC ? pathLanes(s, fl) : rgbLanes();
+        BSDF bsdf = makeBSDF(s, m, mid, it, h, lw);
         Sampler smp = loadSampler(w, p);
         Spec beta = ldS(w.beta, p);
         Spec rad = ldS(w.rad, p);
@@ -338,7 +299,7 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
             g.area = b.w;
             g.eid = w.lgId[p];
             g.valid = true;
-            ls = lightFinish(s, g, bsdf, it);
+            ls = lightFinish(s, g, bsdf, it, lw);
         }
         bool misRay = false;
         if (MIS) {
@@ -420,91 +381,13 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
     }
 }
 
-// Finishes every path still in flight after `d0` wavefront bounces in one
-// persistent launch: a lane takes the next entry of the extension and MIS
-// queues (a WaveDispenser: one atomic per chunk), traces and resolves the pending MIS ray of
-// the path's last vertex (as kWfMisResolve), gives the record of its last
-// guided vertex its snapshot (as kWfShadeHit), and, if the path continues,
-// runs pathTraceKernel's loop from depth d0 one bounce per iteration (as
-// the megakernel, without guiding); a lane whose path ends takes the next
-// entry.  Each path performs the operations of the pure wavefront in the
-// same order, so the film is the same; the launch replaces the tail of
-// nearly empty bounces, each of which costs a full traversal launch.
-template <bool MIS, int SM, int W>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void kWfFinish(
-    SceneView s, WfState w, IntegratorView cfg, uint32_t cur, uint32_t d0) {
-    extern __shared__ uint2 lds[];
-    auto stk = makeStack<SM>(lds);
-    PathCtx<false, typename StackOf<SM>::T> c{s, stk, 0, {0, 0}};
-    const uint32_t nExt = w.counters[kCExt0 + cur];
-    const uint32_t total = nExt + w.counters[kCMis0 + cur];
-    bool live = false;
-    WaveDispenser disp;
-    const uint32_t chunk = dispenserChunk(total);
-    uint32_t p = 0, depth = 0;
-    Ray ray{};
-    Spec rad = sp(0.f), beta = sp(1.f);
-    Sampler smp(0u, 0u, 0u);
-    while (true) {
-        {
-            const uint32_t i = disp.take(&w.counters[kCPtExt], total, chunk, !live);
-            if (i < total) {
-                const bool alive = i < nExt;
-                p = alive ? w.qExt[cur][i] : w.qMis[cur][i - nExt];
-                const uint32_t fl = w.flags[p];
-                // the MIS entry of a path that continues is handled with its
-                // extension entry (flags are not written here, so this test
-                // holds whichever entry a lane takes first)
-                if (alive || !(fl & F_ALIVE)) {
-                    rad = ldS(w.rad, p);
-                    if (fl & F_MIS) {
-                        Ray r = loadRay(w.misO, w.misD, p);
-                        Hit h;
-                        bool hit = closest(c, r, h);
-                        float4 vs = w.misVal[p];
-                        Spec value{vs.x, vs.y, vs.z};
-                        float4 bw4 = w.misBw[p];
-                        float4 pe = w.misPos[p];
-                        misBsdfPart(s, mk3(pe.x, pe.y, pe.z), r, hit, h, __float_as_uint(pe.w),
-                                    Spec{bw4.x, bw4.y, bw4.z}, bw4.w, value);
-                        rad = rad + (value / vs.w) * ldS(w.misBeta, p);
-                    }
-                    if (alive) {
-                        if (w.recId) {
-                            const uint32_t rid = w.recId[p];
-                            if (rid != kNoRecord) {
-                                w.rec[size_t(rid) * kRecF4 + 6] =
-                                    make_float4(rad.r, rad.g, rad.b, 1.f);
-                                w.recId[p] = kNoRecord;
-                            }
-                        }
-                        ray = loadRay(w.rayO, w.rayD, p);
-                        beta = ldS(w.beta, p);
-                        smp = loadSampler(w, p);
-                        depth = d0;
-                        live = true;
-                    } else {
-                        stS(w.rad, p, rad);
-                    }
-                }
-            }
-        }
-        if (!__any(live)) {
-            if (disp.exhausted) break;
-            continue;
-        }
-        if (live) {
-            bool done = !pathBounce<MIS, false>(c, cfg, depth, ray, rad, beta, smp, rgbLanes());
-            if (!done && ++depth >= cfg.maxDepth) done = true;
-            if (done) {
-                stS(w.rad, p, rad);
-                live = false;
-            }
-        }
+// Spectral transport: every path's three-lane radiance to CIE XYZ once its
+// pass has ended (what tracePixel does when its path ends).
+__global__ __launch_bounds__(256) void kWfToXYZ(SceneView s, WfState w) {
+    const uint32_t n = w.active * w.npix;
+    PERSISTENT_LOOP(p, n) {
+        stS(w.rad, p, laneXYZ(ldS(w.rad, p), pathLanes(s, w.flags[p])));
     }
-    unsigned long long r64 = c.rays;
-    for (int off = 32; off > 0; off >>= 1) r64 += __shfl_down(r64, off, 64);
-    if (laneId() == 0 && r64) atomicAdd(w.work, r64);
 }
 
 // Training samples from this pass's records: incident radiance along the
@@ -564,8 +447,9 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                          const WfLaunch& L, bool countWork, hipStream_t st,
                          const GuideView* guide, float alpha, GuideSample* samples,
                          uint32_t sampleCap, uint32_t guideMask, uint32_t guideDepth,
-                         uint32_t guideTarget) {
+                         uint32_t guideTarget, bool spectral) {
     if (active == 0 || active > w.slots) return hipErrorInvalidValue;
+    if (spectral && (!s.spec || guide || countWork)) return hipErrorInvalidValue;
     w.active = active;
     const uint32_t TB = L.traceBlock;
     const size_t lds = stackLdsBytes(L.privStack, s.stackCap, TB);
@@ -600,7 +484,16 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                        (1u << kCExt1) | (1u << kCMis0) | (1u << kCMis1) |
                            (1u << kCShadow) | (1u << kCCont) | (1u << kCRec) |
                            (1u << kCTrain));
-    hipLaunchKernelGGL(kWfGenerate, dim3(L.shadeGrid), b256, 0, st, w, cam, si);
+    if (spectral)
+        hipLaunchKernelGGL(kWfGenerate<true>, dim3(L.shadeGrid), b256, 0, st, w, cam, si, s.spec);
+    else
+        hipLaunchKernelGGL(kWfGenerate<false>, dim3(L.shadeGrid), b256, 0, st, w, cam, si, s.spec);
+    auto misResolve = [&](uint32_t c) {
+        if (spectral)
+            hipLaunchKernelGGL(kWfMisResolve<true>, dim3(L.shadeGrid), b256, 0, st, s, w, c);
+        else
+            hipLaunchKernelGGL(kWfMisResolve<false>, dim3(L.shadeGrid), b256, 0, st, s, w, c);
+    };
     uint32_t cur = 0;
     // Bounces run as wavefront launches up to finishDepth; the paths still
     // in flight then finish in one per-path launch.
@@ -609,14 +502,19 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
     for (uint32_t d = 0; d < dEnd; ++d) {
         const uint32_t nxt = cur ^ 1u;
         extend(cur, countWork);
-        hipLaunchKernelGGL(kWfMisResolve, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
+        misResolve(cur);
         hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
                            (1u << (kCExt0 + nxt)) | (1u << (kCMis0 + nxt)) |
                                (1u << kCShadow) | (1u << kCCont));
         if (guided)
-            hipLaunchKernelGGL((kWfShadeHit<true>), dim3(L.shadeGrid), b256, 0, st, s, w, cur);
+            hipLaunchKernelGGL((kWfShadeHit<true, false>), dim3(L.shadeGrid), b256, 0, st, s, w,
+                               cur);
+        else if (spectral)
+            hipLaunchKernelGGL((kWfShadeHit<false, true>), dim3(L.shadeGrid), b256, 0, st, s, w,
+                               cur);
         else
-            hipLaunchKernelGGL((kWfShadeHit<false>), dim3(L.shadeGrid), b256, 0, st, s, w, cur);
+            hipLaunchKernelGGL((kWfShadeHit<false, false>), dim3(L.shadeGrid), b256, 0, st, s, w,
+                               cur);
         shadow();
         // Past guideDepth the guided render continues like the plain one (no
         // inference); records of earlier vertices still get their snapshots.
@@ -626,43 +524,38 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
             hipError_t e = launchGuideInfer(*guide, w.qCont, w.counters + kCCont, 0u, w.gPos,
                                             w.gNrm, w.gWo, w.gMix, nullptr, L.shadeGrid, st);
             if (e != hipSuccess) return e;
-            if (cfg.useMIS)
-                hipLaunchKernelGGL((kWfShadeCont<true, true>), dim3(L.shadeGrid), b256, 0, st, s,
-                                   w, cfg, cur, d, alpha, gm);
-            else
-                hipLaunchKernelGGL((kWfShadeCont<false, true>), dim3(L.shadeGrid), b256, 0, st, s,
-                                   w, cfg, cur, d, alpha, gm);
-        } else if (cfg.useMIS) {
-            hipLaunchKernelGGL((kWfShadeCont<true, false>), dim3(L.shadeGrid), b256, 0, st, s, w,
-                               cfg, cur, d, alpha, 0u);
+#define CRH_WFC(MIS, GUIDED, SPEC, GM)                                                      \
+    hipLaunchKernelGGL((kWfShadeCont<MIS, GUIDED, SPEC>), dim3(L.shadeGrid), b256, 0, st, s, w, \
+                       cfg, cur, d, alpha, GM)
+            if (cfg.useMIS) CRH_WFC(true, true, false, gm);
+            else CRH_WFC(false, true, false, gm);
+        } else if (spectral) {
+            if (cfg.useMIS) CRH_WFC(true, false, true, 0u);
+            else CRH_WFC(false, false, true, 0u);
         } else {
-            hipLaunchKernelGGL((kWfShadeCont<false, false>), dim3(L.shadeGrid), b256, 0, st, s,
-                               w, cfg, cur, d, alpha, 0u);
+            if (cfg.useMIS) CRH_WFC(true, false, false, 0u);
+            else CRH_WFC(false, false, false, 0u);
+#undef CRH_WFC
         }
         cur = nxt;
     }
     if (finish) {
-        const size_t flds = stackLdsBytes(L.privStack, s.stackCap, 256);
         hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters, 1u << kCPtExt);
-        withStack(L.privStack, [&](auto sm) {
-            constexpr int SM = decltype(sm)::value;
-            withWaves(L.finishWaves, [&](auto wv) {
-                constexpr int W = decltype(wv)::value;
-                if (cfg.useMIS)
-                    hipLaunchKernelGGL((kWfFinish<true, SM, W>), dim3(L.finishGrid), b256, flds,
-                                       st, s, w, cfg, cur, dEnd);
-                else
-                    hipLaunchKernelGGL((kWfFinish<false, SM, W>), dim3(L.finishGrid), b256, flds,
-                                       st, s, w, cfg, cur, dEnd);
-            });
-        });
+        const WfFinishArgs fa{s, w, cfg, cur, dEnd, L.finishGrid,
+                              stackLdsBytes(L.privStack, s.stackCap, 256), L.finishWaves,
+                              spectral, st};
+        hipError_t e = hipSuccess;
+        withStack(L.privStack, [&](auto sm) { e = launchWfFinishSM<decltype(sm)::value>(fa); });
+        if (e != hipSuccess) return e;
     } else {
         // MIS rays spawned at the last bounce: trace and resolve them.
         hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
                            1u << (kCExt0 + cur));
         extend(cur, false);
-        hipLaunchKernelGGL(kWfMisResolve, dim3(L.shadeGrid), b256, 0, st, s, w, cur);
+        misResolve(cur);
     }
+    if (spectral)
+        hipLaunchKernelGGL(kWfToXYZ, dim3(L.shadeGrid), b256, 0, st, s, w);
     if (guided && samples)
         hipLaunchKernelGGL(kWfRecordsToSamples, dim3(L.shadeGrid), b256, 0, st, w, samples,
                            sampleCap, guideTarget);

@@ -485,6 +485,16 @@ def disk_emitter(center, normal, radius: float, segments: int, material) -> Mesh
     return MeshData(P, N, I, material)
 
 
+# BVH parameters (bins, max leaf, max depth) of the terrain's fast build:
+# the reference's depth cap of 20 (Core/BVH.cu:150-212) leaves the
+# 43.8M-triangle terrain with leaves of up to 599 triangles (101.8 triangle
+# tests per ray); without it (64 is the GPU builder's limit, the SAH tree
+# stops at 29 levels) leaves keep to the 4-triangle rule.  A different tree
+# from the reference's, so its hits are checked against the oracle built
+# with the same parameters, not against the reference-parameter tree.
+TERRAIN_FAST_BVH = (32, 4, 64)
+
+
 def terrain_scene(cells: int = 4095, extent: float = 1000.0, amplitude: float = 50.0,
                   octaves: int = 8, n_spheres: int = 2000, subdiv: int = 4,
                   size=(3840, 2160), seed: int = SCENE_SEED):

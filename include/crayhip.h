@@ -228,9 +228,10 @@ typedef struct crh_integrator_cfg {
     uint32_t mode;
     uint32_t count_work; /* != 0: run the counting variant (fills
                             node_visits / tri_tests; slower) */
-    uint32_t spectral;   /* != 0: spectral transport (megakernel, scenes created
-                            with spectra): each path carries three of the 95
-                            wavelength bins and the film receives CIE XYZ */
+    uint32_t spectral;   /* != 0: spectral transport (megakernel or wavefront;
+                            scenes with spectra, no counting variant): each
+                            path carries three of the 95 wavelength bins and
+                            the film receives CIE XYZ */
     uint32_t reserved;
 } crh_integrator_cfg;
 

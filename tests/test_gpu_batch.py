@@ -151,3 +151,40 @@ def test_batch_argument_errors(gpu):
         batch.render(cams, [1, 1], cr.Configs(mode=L.MODE_WAVEFRONT))
     with pytest.raises(cr.RuntimeError):
         cr.get_context(0).set_tuning(batch_tile_w=12)
+
+
+def test_batch_spectral_equals_solo_renders(gpu):
+    """Spectral transport in the batched launch (each scene's spectral table
+    is part of the batch pool): every XYZ film is bit-identical to the scene
+    rendered alone with the spectral megakernel, with one and several sample
+    indices per launch, directly and from the captured graph."""
+    items = _small_batch(n=4, seed=9)
+    scene_list = [cr.Scene(d) for d, _ in items]
+    camds = [c for _, c in items]
+    spp = [c["spp"] for c in camds]
+    cfg = cr.Configs(maxDepth=12, rrThreshold=0.05, useMIS=True, rrStartDepth=3, spectral=True)
+    first = 31
+    solo = []
+    for sc, camd, k in zip(scene_list, camds, spp):
+        sensor = cr.Sensor(camd["size"])
+        sc.setCamera(scenes.make_camera(camd, sensor))
+        cr.PathTraceIntegrator(cfg).dispatch(sc, k, first_sample=first)
+        solo.append((sensor.data.copy(), sensor.weight.copy()))
+    # the spectral films differ from the RGB ones (XYZ of three hero lanes)
+    rgb = _solo(scene_list, camds, spp, first)
+    assert not _same(solo, rgb)
+    batch = Batch(scene_list)
+    sensors = [cr.Sensor(c["size"]) for c in camds]
+    cams = [scenes.make_camera(c, s) for c, s in zip(camds, sensors)]
+    ctx = cr.get_context(0)
+    saved = ctx.get_tuning()
+    try:
+        for kw in (dict(batch_graph=1), dict(batch_graph=1), dict(batch_graph=2, batch_slots=2),
+                   dict(batch_graph=1, batch_slots=64, batch_stack=2, batch_waves=6)):
+            ctx.set_tuning(**kw)
+            for s in sensors:
+                s.clear()
+            batch.render(cams, spp, cfg, first_sample=first)
+            assert _same(_films(sensors), solo), kw
+    finally:
+        ctx.set_tuning(**saved)

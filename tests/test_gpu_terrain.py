@@ -102,3 +102,43 @@ def test_view_film_matches_oracle(gpu, orc):
     close = np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1e-3)
     assert close.mean() > 0.9, close.mean()
     assert float(np.mean((g - o) ** 2 / (o ** 2 + 1e-2))) < 1e-3
+
+
+def test_fast_bvh_parameters_bit_exact(gpu, orc):
+    """The terrain's fast parameter set (no effective depth cap), built on
+    the GPU, on a 256k-triangle terrain: node arrays and primitive order
+    equal the oracle's build with the same parameters, and hit IDs, t and
+    the node / triangle counts equal the oracle's traversal of that tree;
+    leaves keep to the 4-triangle rule where the reference cap gives long
+    ones."""
+    data, cams = scenes.terrain_scene(cells=320, n_spheres=40, subdiv=3, size=(48, 27))
+    data.bvhParams = scenes.TERRAIN_FAST_BVH
+    data.bvhDevice = True
+    gs, os_ = cr.Scene(data), orc.Scene(data)
+    gst, ost = gs.bvh_stats(), os_.bvh_stats()
+    for k in ("node_count", "max_depth", "max_leaf", "root"):
+        assert gst[k] == ost[k], k
+    assert gst["max_leaf"] <= 12 and gst["max_depth"] < 60
+    gn, gp = gs.export_bvh()
+    on, op = os_.export_bvh()
+    assert gn.tobytes() == on.tobytes() and np.array_equal(gp, op)
+    n = 1 << 16
+    rays = scenes.mixed_batch(data, cams[0], n)
+    oh, nv, nt = os_.trace_closest(rays, counts=True)
+    assert nt.sum() / n < 10
+    ctx = cr.get_context(0)
+    saved = ctx.get_tuning()
+    try:
+        for kw in (dict(persistent=2, stack=3), dict(persistent=1, refill=16, pt_grid=64),
+                   dict(persistent=1, refill=32, pt_grid=4096, trace_tris=2, stack=2)):
+            ctx.set_tuning(**kw)
+            h = gs.trace_closest(rays)
+            for k in ("prim", "t", "u", "v"):
+                assert np.array_equal(h[k].view(np.uint32), oh[k].view(np.uint32)), (kw, k)
+    finally:
+        ctx.set_tuning(**saved)
+    rb, hb = cr.DeviceBuffer(rays.nbytes), cr.DeviceBuffer(n * 16)
+    rb.copyFromHost(rays)
+    cnt = cr.trace_closest_device(gs, rb, n, hb, count_work=True)
+    assert cnt["node_visits"] == int(nv.sum(dtype=np.uint64))
+    assert cnt["tri_tests"] == int(nt.sum(dtype=np.uint64))
