@@ -51,16 +51,24 @@ def parse():
     ap.add_argument("--trace-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-trace", action="store_true", help="skip the closest-hit batch")
-    ap.add_argument("--guided-seconds", type=float, default=8.0,
+    ap.add_argument("--guided-seconds", type=float, default=6.0,
                     help="render time of each run of the guided-vs-plain comparison (0 = skip)")
-    ap.add_argument("--guided-runs", type=int, default=5,
+    ap.add_argument("--guided-runs", type=int, default=9,
                     help="independent runs per method in the guided-vs-plain comparison")
+    ap.add_argument("--guided-size", type=int, choices=(256, 1024), default=1024,
+                    help="image size of the guided-vs-plain comparison (its 65,536-spp "
+                         "reference is committed for both)")
     ap.add_argument("--batch-scenes", type=int, default=64,
                     help="scenes of the many-scene batch leg (0 = skip)")
     ap.add_argument("--no-terrain", action="store_true",
                     help="skip the large-scale leg (the 43.8M-triangle terrain, four "
                          "3840x2160 views in one batched launch, and its closest-hit batch)")
-    ap.add_argument("--terrain-spp", type=int, default=2,
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the small spectral Cornell-box configuration")
+    ap.add_argument("--terrain-bvh", choices=("fast", "reference"), default="fast",
+                    help="terrain BVH: built on the GPU without the depth cap (fast) or "
+                         "with the reference's parameters")
+    ap.add_argument("--terrain-spp", type=int, default=4,
                     help="timed spp per view of the terrain leg")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc runs that measure HBM traffic")
@@ -168,8 +176,8 @@ def profile_kernels(cr, L, scenes, tuning: str | None = None):
 
 
 def profile_terrain(cr, L, scenes, tuning: str | None = None):
-    """Minimal launches on the 43.8M-triangle terrain for rocprofv3: one spp
-    of the four 3840 x 2160 views in one batched launch sequence (graph
+    """Minimal launches on the 43.8M-triangle terrain (the fast GPU-built
+    BVH of the timed leg) for rocprofv3: one spp of the four 3840 x 2160 views in one batched launch sequence (graph
     capture off), then one 16,777,216-ray closest-hit batch from the first
     view with work counting and one without."""
     from craystal_amd.batch import Batch
@@ -178,6 +186,8 @@ def profile_terrain(cr, L, scenes, tuning: str | None = None):
         ctx.set_tuning(**json.loads(tuning))
     ctx.set_tuning(batch_graph=2)
     data, cams = scenes.terrain_scene()
+    data.bvhParams = scenes.TERRAIN_FAST_BVH   # the BVH the timed terrain leg uses
+    data.bvhDevice = True
     scene = cr.Scene(data)
     batch = Batch([scene] * len(cams))
     sensors = [cr.Sensor(c["size"]) for c in cams]
@@ -226,6 +236,13 @@ def batch_leg(cr, scenes, n: int) -> dict:
         s.clear()
     st = batch.render(cams, spp, cfg, first_sample=0, readback=False)
     samples = sum(c["size"][0] * c["size"][1] * k for (_, c), k in zip(items, spp))
+    # the batched wavefront (one launch per stage and bounce over every
+    # scene's paths), its queue capacity and launch shapes tuned on 1 spp
+    wtuned = autotune.tune_batch_wavefront(batch, cams, [1] * n, cfg)
+    from craystal_amd import _lib as L
+    wcfg = cr.Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
+    batch.render(cams, spp, wcfg, first_sample=0, readback=False)    # captures the graph
+    wst = batch.render(cams, spp, wcfg, first_sample=0, readback=False)
     solo_ms = 0.0
     for sc, cam, k in zip(scene_list, cams, spp):
         sc.setCamera(cam)
@@ -236,25 +253,35 @@ def batch_leg(cr, scenes, n: int) -> dict:
             "msamples_per_s": round(samples / (st["kernel_ms"] / 1e3) / 1e6, 2),
             "grays_per_s": round(st["rays"] / (st["kernel_ms"] / 1e3) / 1e9, 4),
             "solo_sequential_ms": round(solo_ms, 3),
+            "wavefront": {"kernel_ms": round(wst["kernel_ms"], 3),
+                          "msamples_per_s": round(samples / (wst["kernel_ms"] / 1e3) / 1e6, 2),
+                          "tuning": {k: wtuned[k] for k in ("wf_pool", "wf_finish", "wf_stack",
+                                                             "wf_trace_block", "wf_grid")}},
             "tuning": {k: tuned[k] for k in ("batch_tile_w", "batch_grid", "batch_chunks",
                                               "batch_stack", "batch_waves", "batch_slots")},
             "slots_ms": tuned["slots_ms"]}
 
 
-def terrain_leg(cr, scenes, spp: int, trace_rays: int) -> dict:
+def terrain_leg(cr, scenes, spp: int, trace_rays: int, bvh: str = "fast") -> dict:
     """The large-scale workload: the fBm terrain (4096 x 4096 vertices,
     33,538,050 triangles) with 2,000 subdivision-4 icospheres (43,778,066
     triangles with the sun disk), four 3840 x 2160 camera views rendered in
     one persistent launch per sample index (the scene held once in the pool),
     max depth 24, MIS.  One warm-up spp, then `spp` timed spp of every view.
     Also the closest-hit batch of `trace_rays` mixed rays from the first
-    view, at the shape the autotuner picks for it."""
+    view, at the shape the autotuner picks for it.  bvh: "fast" builds the
+    BVH on the GPU without the reference's depth cap of 20
+    (scenes.TERRAIN_FAST_BVH), "reference" with the reference's parameters
+    (leaves of up to 599 triangles)."""
     from craystal_amd import autotune
     from craystal_amd.batch import Batch
     t0 = time.perf_counter()
     data, cams = scenes.terrain_scene()
     gen_s = time.perf_counter() - t0
     progress(f"terrain generated: {data.triangle_count} triangles in {gen_s:.1f} s")
+    if bvh == "fast":
+        data.bvhParams = scenes.TERRAIN_FAST_BVH
+        data.bvhDevice = True
     t0 = time.perf_counter()
     scene = cr.Scene(data)
     build_s = time.perf_counter() - t0
@@ -277,7 +304,9 @@ def terrain_leg(cr, scenes, spp: int, trace_rays: int) -> dict:
            "triangles": int(data.triangle_count), "views": len(cams), "width": W, "height": H,
            "max_depth": cams[0]["max_depth"], "spp_timed": spp,
            "generate_s": round(gen_s, 2), "scene_create_s": round(build_s, 2),
-           "bvh": {"nodes": bst["node_count"], "max_depth": bst["max_depth"],
+           "bvh": {"params": "bins, max leaf, max depth = " + ", ".join(map(str, data.bvhParams)),
+                   "built_on": "gpu" if data.bvhDevice else "host",
+                   "nodes": bst["node_count"], "max_depth": bst["max_depth"],
                    "max_leaf": bst["max_leaf"], "build_s": round(bst["build_ms"] / 1e3, 3)},
            "pool_mb": round(info["pool_bytes"] / 2**20, 1),
            "kernel_ms": round(st["kernel_ms"], 3), "wall_ms": round(el * 1e3, 3),
@@ -314,52 +343,125 @@ def terrain_leg(cr, scenes, spp: int, trace_rays: int) -> dict:
     return out
 
 
-def guided_leg(cr, scenes, seconds: float, repeats: int) -> dict:
-    """Time-to-relMSE 0.01 on the occluded-light room at 256 x 256 (max depth
-    16) against the committed 65,536-spp plain image.  The reference's
-    estimator is heavy-tailed (rare, very bright samples from its Principled
-    BSDF and the emitter double count), so one run's relMSE at a given time
-    varies by more than 10x between independent runs; each method therefore
-    renders `repeats` independent runs (disjoint sample indices, fresh guide)
-    of `seconds` render time, and the medians are reported.  Guided: the
-    wavefront with one-sample MIS (alpha 0.5) at Lambertian vertices, the
-    guide training during the first 25% of each run.  The plain megakernel's
-    launch shape is tuned on this scene first (path regeneration pays off
-    here), so both methods run at their best."""
+def c1_leg(cr, L, scenes, with_cpu: bool, reps: int = 20) -> dict:
+    """The small plain configuration: the spectral Cornell box (36 triangles,
+    smooth 95-bin reflectance spectra, constant-spectrum emitter), 256 x 256,
+    16 spp, max depth 8, roulette from depth 3, MIS, hero-wavelength transport
+    into a CIE XYZ film.  The megakernel and the wavefront are tuned on it
+    (every candidate gives the same film), then each renders the 16 spp
+    `reps` times; the mean rate is reported.  The CPU oracle renders the same
+    16 spp on all host cores and on one."""
+    from craystal_amd import autotune
+    data, camd = scenes.cornell_box_spectral()
+    W, H = 256, 256
+    scene = cr.Scene(data)
+    sensor = cr.Sensor((W, H))
+    cam = scenes.make_camera(camd, sensor)
+    scene.setCamera(cam)
+    cfg = cr.Configs(maxDepth=8, rrThreshold=0.05, useMIS=True, rrStartDepth=3, spectral=True)
+    spp = 16
+    tuned = autotune.tune_render(scene, cfg, spp=spp, stacks=(1, 2, 3), render_waves=(4, 6, 8),
+                                 render_regen=((2, 0), (1, 1024), (1, 2048)),
+                                 render_pools=(1 << 20, 1 << 21), wf_grids=(4096, 16384),
+                                 wf_blocks=(128, 256), wf_persistent=((2, 0),),
+                                 wf_pools=(1 << 20, 1 << 21), wf_finishes=(65, 1, 2))
+    samples = W * H * spp
+    out = {"scene": "spectral Cornell box, 36 tris, 256x256, 16 spp, max depth 8, RR from "
+                    "depth 3, MIS, 95-bin spectra (hero wavelength), CIE XYZ film",
+           "samples_per_render": samples}
+    for name, mode in (("megakernel", L.MODE_MEGAKERNEL), ("wavefront", L.MODE_WAVEFRONT)):
+        integ = cr.PathTraceIntegrator(cr.Configs(**{**cfg.__dict__, "mode": mode}))
+        integ.dispatch(scene, spp, first_sample=0, readback=False)
+        t0 = time.perf_counter()
+        for r in range(reps):
+            integ.dispatch(scene, spp, first_sample=(r + 1) * spp, readback=False)
+        el = time.perf_counter() - t0
+        out[name] = {"msamples_per_s": round(samples * reps / el / 1e6, 2),
+                     "ms_per_render": round(el / reps * 1e3, 3)}
+    out["tuning"] = {k: tuned[k] for k in ("mode", "render_stack", "render_waves",
+                                            "render_regen", "render_pool", "wf_pool",
+                                            "wf_finish")}
+    if with_cpu:
+        import oracle
+        orc = oracle.Scene(data)
+        threads = int(oracle.lib().orc_max_threads())
+        t0 = time.perf_counter()
+        orc.render(cam, (W, H), cfg, 0, spp)
+        c = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.render(cam, (W, H), cfg, 0, 2, threads=1)
+        c1 = time.perf_counter() - t0
+        out["cpu_oracle"] = {"msamples_per_s": round(samples / c / 1e6, 3), "cores": threads,
+                             "seconds": round(c, 3), "cpu_model": _cpu_model(),
+                             "one_core_msamples_per_s": round(W * H * 2 / c1 / 1e6, 3),
+                             "one_core_sample": "2 of the 16 spp on one thread"}
+    return out
+
+
+def guided_leg(cr, scenes, seconds: float, repeats: int, size: int = 1024) -> dict:
+    """Guided vs plain on the occluded-light room (200 objects, 171,388
+    triangles, max depth 16) at size x size against the committed
+    65,536-spp plain image of the same size: equal-time relMSE and the time
+    to relMSE 0.01.  The reference's estimator is heavy-tailed (rare, very
+    bright samples from its Principled BSDF and the emitter double count),
+    so one run's relMSE at a given time varies by several times between
+    independent runs; each method therefore renders `repeats` independent
+    runs (disjoint sample indices, a fresh guide per run) of `seconds`
+    render time and the medians are reported.  Where a run does not reach
+    0.01 within its time, its time to 0.01 is extrapolated from relMSE ~ 1/t
+    fitted on the second half of its curve.  Guided: the fused guided
+    megakernel (guide inference on MFMA inside the path kernel), one-sample
+    MIS with alpha 0.5 at Lambertian vertices, the guide training during the
+    first 25 % of each run.  The plain megakernel's launch shape is tuned on
+    this scene first, so both methods run at their best."""
     from craystal_amd import autotune
     from craystal_amd import experiments as X
-    ref = np.load(ROOT / "craystal_amd" / "data" / "room256_plain65536.npy")
+    ref_path = ROOT / "craystal_amd" / "data" / f"room{size}_plain65536.npy"
+    ref = np.load(ref_path).astype(np.float32)
     data, camd = scenes.occluded_room()
     cfg = cr.Configs(maxDepth=16, rrThreshold=0.05, useMIS=True, rrStartDepth=3)
-    size = (256, 256)
     scene = cr.Scene(data)
-    sensor = cr.Sensor(size)
+    sensor = cr.Sensor((size, size))
     scene.setCamera(scenes.make_camera(camd, sensor))
-    px = size[0] * size[1]
-    tuned = autotune.tune_render(scene, cfg, spp=16, stacks=(2, 3), render_waves=(4, 6, 8),
-                                 render_regen=((2, 0), (1, 1024), (1, 2048)),
+    px = size * size
+    tuned = autotune.tune_render(scene, cfg, spp=4 if size > 256 else 16, stacks=(2, 3),
+                                 render_waves=(6, 8),
+                                 render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
                                  wf_grids=(16384,), wf_blocks=(128,), wf_persistent=((2, 0),),
                                  wf_finishes=(65,))
-    # the guided wavefront keeps 256 samples per pixel in flight (16M paths):
-    # its per-bounce launches then cost less per sample than at 64
-    cr.get_context(0).set_tuning(wf_pool=1 << 24)
+    # the fused guided kernel's launch shape, tuned with a guide trained for
+    # a few passes (training changes which vertices are guided and how long
+    # paths get; the tuning does not change any image)
+    from craystal_amd.guiding import Guide, GuidedIntegrator
+    P = data.concatenated()[0]
+    gi = GuidedIntegrator(cfg, Guide(P.min(0), P.max(0), seed=5), alpha=0.5, lr=1e-2,
+                          train_batch=4096, train_steps=4, materials=1, target=0)
+    for k in range(4):
+        gi.dispatch(scene, 4, first_sample=(1 << 26) + 4 * k, train=True, readback=False)
+    gtuned = autotune.tune_guided(gi, scene, spp=4 if size > 256 else 16)
+    del gi
     sensor.clear()
+    chunk = 16 if size > 256 else 64
     runs = {"plain": [], "guided": []}
     for r in range(repeats):
         first = (r + 1) << 21
-        pc = X.plain_curve(scene, sensor, cfg, ref, seconds, 64, first=first)
+        pc = X.plain_curve(scene, sensor, cfg, ref, seconds, chunk, first=first)
         runs["plain"].append(X.summarize(pc, 0.01, px))
-        gc, info = X.guided_curve(scene, sensor, data, cfg, ref, seconds, 256, first=first,
+        gc, info = X.guided_curve(scene, sensor, data, cfg, ref, seconds, chunk, first=first,
                                   seed=11 + r, materials=1, target=0, train_batch=4096,
                                   train_steps=4, lr=1e-2, alpha=0.5)
         runs["guided"].append({**X.summarize(gc, 0.01, px), **info})
-        progress(f"guided leg run {r}: plain {runs['plain'][-1]['final_relmse']:.4f} "
+        progress(f"guided leg run {r}: relMSE plain {runs['plain'][-1]['final_relmse']:.4f} "
                  f"guided {runs['guided'][-1]['final_relmse']:.4f}")
-    out = {"scene": "occluded room, 171,388 tris, 256x256, max depth 16", "target_relmse": 0.01,
+    out = {"scene": f"occluded room, 200 objects, 171,388 tris, {size}x{size}, max depth 16",
+           "target_relmse": 0.01,
            "plain_tuning": {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen",
                                                         "render_grid")},
+           "guided_kernel": "fused guided megakernel (kRenderGuided)",
+           "guided_tuning": {k: gtuned[k] for k in ("guided_stack", "guided_waves",
+                                                     "guided_grid", "guided_pool")},
            "seconds_per_run": seconds, "runs": repeats,
-           "reference": "craystal_amd/data/room256_plain65536.npy (plain megakernel, 65,536 spp)"}
+           "reference": f"craystal_amd/data/{ref_path.name} (plain megakernel, 65,536 spp)"}
     for k, v in runs.items():
         out[k] = {"relmse_at_end": [round(x["final_relmse"], 5) for x in v],
                   "median_relmse_at_end": float(np.median([x["final_relmse"] for x in v])),
@@ -367,6 +469,7 @@ def guided_leg(cr, scenes, seconds: float, repeats: int) -> dict:
                   "median_extrapolated_seconds": float(np.median([x["extrapolated_seconds"]
                                                                  for x in v])),
                   "msamples_per_s": round(float(np.mean([x["msamples_per_s"] for x in v])), 2),
+                  "spp_at_end": [x["final_spp"] for x in v],
                   "reached_target_s": [x["seconds_to_target"] for x in v],
                   "relmse_trimmed_at_end": [round(x["final_relmse_trimmed"], 5) for x in v],
                   "median_relmse_trimmed_at_end": float(np.median(
@@ -374,11 +477,15 @@ def guided_leg(cr, scenes, seconds: float, repeats: int) -> dict:
     out["speedup_median_time_to_target"] = round(
         out["plain"]["median_extrapolated_seconds"] / out["guided"]["median_extrapolated_seconds"],
         3)
+    out["equal_time_relmse_ratio"] = round(
+        out["plain"]["median_relmse_at_end"] / out["guided"]["median_relmse_at_end"], 3)
     # diagnostic: the same ratio on relMSE without the 0.1 % of pixels with the
     # largest error (equal render times, relMSE ~ 1/t)
     out["speedup_trimmed_diagnostic"] = round(
         out["plain"]["median_relmse_trimmed_at_end"] / out["guided"]["median_relmse_trimmed_at_end"],
         3)
+    out["guided_vs_plain_sample_rate"] = round(
+        out["guided"]["msamples_per_s"] / out["plain"]["msamples_per_s"], 3)
     return out
 
 
@@ -649,13 +756,25 @@ def main():
             f"(fetch x {traffic['fetch_scale']}, write x {traffic['write_scale']})")
     if "closest_hit" in out:
         progress(f"closest hit: {out['closest_hit']['grays_per_s']} Grays/s")
+    # Each further leg tunes launch shapes for its own workload; every one
+    # starts from the tuning the main workload ran with.
+    ctx = cr.get_context(0)
+    main_tuning = ctx.get_tuning()
+    if not a.no_c1:
+        ctx.set_tuning(**main_tuning)
+        out["c1"] = c1_leg(cr, L, scenes, not a.no_cpu)
+        progress(f"C1: megakernel {out['c1']['megakernel']['msamples_per_s']}, wavefront "
+                 f"{out['c1']['wavefront']['msamples_per_s']} Msamples/s")
     if a.batch_scenes > 0:
+        ctx.set_tuning(**main_tuning)
         out["batch"] = batch_leg(cr, scenes, a.batch_scenes)
         progress(f"batch: {out['batch']['msamples_per_s']} Msamples/s")
     if a.guided_seconds > 0 and a.guided_runs > 0:
-        out["guided"] = guided_leg(cr, scenes, a.guided_seconds, a.guided_runs)
+        ctx.set_tuning(**main_tuning)
+        out["guided"] = guided_leg(cr, scenes, a.guided_seconds, a.guided_runs, a.guided_size)
     if not a.no_terrain:
-        out["terrain"] = terrain_leg(cr, scenes, a.terrain_spp, a.trace_rays)
+        ctx.set_tuning(**main_tuning)
+        out["terrain"] = terrain_leg(cr, scenes, a.terrain_spp, a.trace_rays, a.terrain_bvh)
 
     if not a.no_cpu:
         import oracle

@@ -232,3 +232,77 @@ def tune_batch(batch, cameras, spp, cfg: Configs, device: int = 0, tile_ws=(8, 1
             "slots_ms": {str(k): round(v, 3) for k, v in res_sl.items()},
             "timings_ms": {f"tile_w={k[0]},grid={k[1]},chunks={k[2]},stack={k[3]},waves={k[4]}":
                            round(v, 4) for k, v in res.items()}}
+
+
+def tune_guided(integrator, scene: Scene, spp: int = 4, device: int = 0, stacks=(2, 3),
+                waves=(4, 6, 8), grids=(2048, 4096), pools=(None,),
+                first_sample: int = 1 << 20) -> dict:
+    """Tune the fused guided megakernel (stack placement, register budget,
+    persistent grid, paths per launch) on the scene's current camera with
+    the integrator's guide as it stands (no training, so every candidate
+    must render the same image bit for bit); applies the fastest."""
+    ctx = get_context(device)
+    sensor = scene.getCamera().getSensor()
+    pool0 = ctx.get_tuning()["guided_pool"]
+    res, ref = {}, None
+    for st, wv, g, p in itertools.product(stacks, waves, grids,
+                                          [pool0 if p is None else p for p in pools]):
+        ctx.set_tuning(guided_fused=1, guided_stack=st, guided_waves=wv, guided_grid=g,
+                       guided_pool=p)
+        sensor.clear()
+        integrator.dispatch(scene, spp, first_sample=first_sample, train=False, readback=False)
+        sensor.clear()
+        ms = integrator.dispatch(scene, spp, first_sample=first_sample, train=False)["kernel_ms"]
+        img = sensor.data.tobytes() + sensor.weight.tobytes()
+        if ref is None:
+            ref = img
+        elif img != ref:
+            raise AssertionError(f"guided tuning {(st, wv, g, p)} changed the image")
+        res[(st, wv, g, p)] = ms / spp
+    best = min(res, key=res.get)
+    st, wv, g, p = best
+    ctx.set_tuning(guided_fused=1, guided_stack=st, guided_waves=wv, guided_grid=g, guided_pool=p)
+    return {"guided_stack": st, "guided_waves": wv, "guided_grid": g, "guided_pool": p,
+            "ms_per_spp": {str(k): round(v, 4) for k, v in res.items()}}
+
+
+def tune_batch_wavefront(batch, cameras, spp, cfg: Configs, device: int = 0,
+                         pools=(1 << 22, 1 << 23, 1 << 24), finishes=(1, 2, 65),
+                         stacks=(1, 3), blocks=(128, 256), grids=(8192, 16384), reps: int = 1,
+                         first_sample: int = 1 << 20) -> dict:
+    """Tune the batched wavefront: paths in flight per pass (wf_pool, the
+    queue capacity: sample indices of every pixel of the batch per pass),
+    the bounce after which paths finish in one per-path launch, the stack
+    placement, the traversal block size and the grids.  Every candidate
+    renders the whole batch (graph replay), all films must come out
+    bit-identical, and the fastest is applied."""
+    ctx = get_context(device)
+    sensors = [c.getSensor() for c in cameras]
+    wcfg = Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
+    ref, res = None, {}
+    for pool, fin, st, tb, g in itertools.product(pools, finishes, stacks, blocks, grids):
+        ctx.set_tuning(wf_pool=pool, wf_finish=fin, wf_stack=st, wf_trace_block=tb,
+                       wf_trace_grid=g, wf_shade_grid=g, batch_graph=1)
+        ms = []
+        for r in range(reps + 1):
+            for s in sensors:
+                s.clear()
+            out = batch.render(cameras, spp, wcfg, first_sample=first_sample, readback=(r == 0))
+            if r == 0:
+                imgs = b"".join(s.data.tobytes() + s.weight.tobytes() for s in sensors)
+                if ref is None:
+                    ref = imgs
+                elif imgs != ref:
+                    raise AssertionError(f"batch wavefront tuning {(pool, fin, st, tb, g)} "
+                                         "changed an image")
+            else:
+                ms.append(out["kernel_ms"])
+        res[(pool, fin, st, tb, g)] = _median(ms)
+    best = min(res, key=res.get)
+    pool, fin, st, tb, g = best
+    ctx.set_tuning(wf_pool=pool, wf_finish=fin, wf_stack=st, wf_trace_block=tb, wf_trace_grid=g,
+                   wf_shade_grid=g, batch_graph=1)
+    return {"wf_pool": pool, "wf_finish": fin, "wf_stack": st, "wf_trace_block": tb,
+            "wf_grid": g, "ms": res[best],
+            "timings_ms": {f"pool={k[0]},finish={k[1]},stack={k[2]},block={k[3]},grid={k[4]}":
+                           round(v, 3) for k, v in res.items()}}

@@ -251,29 +251,21 @@ namespace {
 
 // Carves the wavefront structure-of-arrays state for `slots` samples per
 // pixel out of one allocation (reallocated when more slots are needed).
-hipError_t ensureWavefront(crh_film* f, uint32_t slots) {
-    if (f->wfReady && f->wf.slots >= slots) return hipSuccess;
-    f->wfReady = f->guidedReady = false;
-    f->gArena.reset();
-    const size_t n = size_t(f->w) * f->h * slots;
+// Path state of n paths of the wavefront in one arena (SoA arrays, queues,
+// counters); sizes and guided buffers are the caller's.
+hipError_t layoutWavefront(DevMem& arena, crh::WfState& w, size_t n) {
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t v4 = align(n * 16), u1 = align(n * 4), u2 = align(n * 8);
     const size_t total = 17 * v4 + u2 + 9 * u1 + align(64) + align(64);
-    hipError_t e = f->wfArena.alloc(total);
+    hipError_t e = arena.alloc(total);
     if (e != hipSuccess) return e;
-    char* base = f->wfArena.as<char>();
+    char* base = arena.as<char>();
     size_t off = 0;
     auto take = [&](size_t bytes) {
         char* p = base + off;
         off += bytes;
         return p;
     };
-    crh::WfState& w = f->wf;
-    w.width = f->w;
-    w.height = f->h;
-    w.npix = f->w * f->h;
-    w.slots = slots;
-    w.active = slots;
     w.rayO = (float4*)take(v4);
     w.rayD = (float4*)take(v4);
     w.hit = (uint4*)take(v4);
@@ -307,6 +299,21 @@ hipError_t ensureWavefront(crh_film* f, uint32_t slots) {
     w.gMix = nullptr;
     w.recId = nullptr;
     w.recCap = 0;
+    return hipSuccess;
+}
+
+hipError_t ensureWavefront(crh_film* f, uint32_t slots) {
+    if (f->wfReady && f->wf.slots >= slots) return hipSuccess;
+    f->wfReady = f->guidedReady = false;
+    f->gArena.reset();
+    crh::WfState& w = f->wf;
+    hipError_t e = layoutWavefront(f->wfArena, w, size_t(f->w) * f->h * slots);
+    if (e != hipSuccess) return e;
+    w.width = f->w;
+    w.height = f->h;
+    w.npix = f->w * f->h;
+    w.slots = slots;
+    w.active = slots;
     f->wfReady = true;
     return hipSuccess;
 }
@@ -1668,6 +1675,11 @@ struct crh_batch {
     uint32_t maxStack = 1;
     DevMem table, prefix, next, graphCounter;
     DevMem counter;  // work counters of the direct (non-graph) launches
+    // Wavefront mode: path state of up to wfPaths paths (several sample
+    // indices of every pixel of the batch per pass).
+    DevMem wfArena;
+    crh::WfState wf{};
+    size_t wfPaths = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<uint64_t> sig;  // what the captured graph was built for
@@ -1797,7 +1809,9 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
                      const uint32_t* spp, const crh_integrator_cfg* cfg, uint32_t first_sample,
                      crh_stats* stats) {
     ARG_CHECK(b && cams && films && spp && cfg, "NULL argument");
-    ARG_CHECK(cfg->mode == CRH_MODE_MEGAKERNEL, "batched rendering uses the megakernel mode");
+    ARG_CHECK(cfg->mode == CRH_MODE_MEGAKERNEL || cfg->mode == CRH_MODE_WAVEFRONT,
+              "unknown integrator mode");
+    ARG_CHECK(!cfg->count_work, "batched rendering has no counting variant");
     ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
     const uint32_t n = uint32_t(b->views.size());
     uint32_t maxSpp = 0;
@@ -1814,11 +1828,12 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
     double t0 = nowMs();
     HIP_TRY(hipSetDevice(ctx->device));
     crh::Tuning tun = ctx->tuning;
-    if (b->maxStack > uint32_t(crh::kPrivStackLevels)) tun.batchPriv = 0;
+    if (b->maxStack > uint32_t(crh::kPrivStackLevels)) tun.batchPriv = tun.wfPriv = 0;
+    const bool wavefront = cfg->mode == CRH_MODE_WAVEFRONT;
     const uint32_t maxSlots = std::max<uint32_t>(1u, tun.batchSlots);
     // Per-sample buffers for up to maxSlots sample indices per launch (a
     // scene takes part in launches of at most its own spp indices).
-    for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t i = 0; i < n && !wavefront; ++i)
         HIP_TRY(ensureFilmSlots(films[i], std::max<uint32_t>(1u, std::min(maxSlots, spp[i]))));
     // Scenes by decreasing spp: those still sampling at pass k form a prefix.
     std::vector<uint32_t> order(n);
@@ -1868,12 +1883,44 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
     const uint32_t* dTile = b->prefix.as<uint32_t>();
     const uint32_t* dPix = dTile + n + 1;
     auto* dCount = counter.as<unsigned long long>();
+    // Wavefront mode: paths in flight per pass (wf_pool, at least every
+    // pixel of the batch once), the same sample index grouping rule.
+    const crh::WfLaunch wl = wfLaunch(tun);
+    if (wavefront) {
+        const size_t allPix = pixPre[n];
+        const size_t want = std::max<size_t>(allPix, size_t(tun.wfPool) / allPix * allPix);
+        const size_t paths = std::min<size_t>(want, allPix * std::max<uint32_t>(1u, maxSpp));
+        if (b->wfPaths < paths) {
+            b->dropGraph();  // the captured graph points at the old state
+            HIP_TRY(layoutWavefront(b->wfArena, b->wf, paths));
+            b->wfPaths = paths;
+        }
+    }
     // The pass sequence; launched directly or recorded into a graph.  Each
-    // launch covers up to maxSlots sample indices, as long as the set of
-    // scenes still sampling stays the same.
+    // launch covers up to maxSlots sample indices (wavefront: as many as its
+    // path state holds), as long as the set of scenes still sampling stays
+    // the same.
     auto passes = [&](hipStream_t st, unsigned long long* cnt) -> hipError_t {
         uint32_t active = n;
         for (uint32_t k = 0; k < maxSpp;) {
+            if (wavefront) {
+                while (active > 0 && spp[order[active - 1]] <= k) --active;
+                crh::WfState w = b->wf;
+                w.npix = pixPre[active];
+                w.width = w.height = 0;  // per scene, from the table
+                w.slots = uint32_t(std::min<size_t>(b->wfPaths / w.npix, 0xFFFFFFFFu));
+                w.work = cnt;
+                const uint32_t g = std::min(w.slots, spp[order[active - 1]] - k);
+                if (uint64_t(w.npix) * g >= (uint64_t(1) << 32)) return hipErrorInvalidValue;
+                hipError_t e = crh::wavefrontBatchPass(dTab, dPix, active, b->maxStack, iv,
+                                                       first_sample + k, g, w, wl, st,
+                                                       cfg->spectral != 0);
+                if (e == hipSuccess)
+                    e = crh::launchBatchGatherPass(dTab, dPix, active, w.npix, g, w.rad, w.P, st);
+                if (e != hipSuccess) return e;
+                k += g;
+                continue;
+            }
             while (active > 0 && spp[order[active - 1]] <= k) --active;
             const uint32_t g = std::min(maxSlots, spp[order[active - 1]] - k);
             if (uint64_t(tilePre[active]) * g >= (uint64_t(1) << 32))
@@ -1895,7 +1942,9 @@ int crh_batch_render(crh_batch* b, const crh_camera* cams, crh_film* const* film
                                      uint64_t(__builtin_bit_cast(uint32_t, iv.rrThreshold)),
                                      iv.rrStartDepth, iv.useMIS, tun.batchPriv, tw,
                                      tun.batchGrid, tun.batchChunks, tun.batchWaves, maxSlots,
-                                     cfg->spectral != 0 ? 1u : 0u};
+                                     cfg->spectral != 0 ? 1u : 0u, wavefront ? 1u : 0u,
+                                     wl.traceGrid, wl.traceBlock, wl.shadeGrid, wl.privStack,
+                                     wl.finishDepth, wl.finishWaves, wl.finishGrid, b->wfPaths};
         for (uint32_t j = 0; j < n; ++j) sig.push_back(spp[order[j]]);
         sig.insert(sig.end(), pre.begin(), pre.end());
         if (!b->exec || sig != b->sig) {

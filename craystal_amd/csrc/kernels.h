@@ -152,6 +152,13 @@ template <> hipError_t launchBatchSM<0>(const BatchArgs&);
 template <> hipError_t launchBatchSM<1>(const BatchArgs&);
 template <> hipError_t launchBatchSM<2>(const BatchArgs&);
 
+// Film updates of the first nActive scenes of the table from a pass buffer
+// holding all of them (the batched wavefront's): slot k of scene j at
+// L + k * pixels + pixPrefix[j], pixels = pixPrefix[nActive]; one launch per
+// slot, in sample order.
+hipError_t launchBatchGatherPass(const BatchScene* tab, const uint32_t* pixPrefix,
+                                 uint32_t nActive, uint32_t pixels, uint32_t slots,
+                                 const float4* L, const float2* P, hipStream_t st);
 hipError_t launchFilmGather(const float4* L, const float2* P, uint32_t W,
                             uint32_t H, float* sum, float* wsum,
                             hipStream_t st);

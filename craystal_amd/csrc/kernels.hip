@@ -138,18 +138,32 @@ __global__ void kBatchReset(uint32_t* __restrict__ next) {
 }
 
 // Film update of every active scene with the samples of slot `slot`:
-// pixPrefix[j] is scene j's first pixel.
+// pixPrefix[j] is scene j's first pixel.  The samples are the scenes' own
+// buffers (slot k of scene j at b.L + k * w * h), or with Ls set a pass
+// buffer holding every active scene (slot k of scene j at
+// Ls + k * stride + pixPrefix[j], as the batched wavefront lays them out).
 __global__ __launch_bounds__(256) void kBatchGather(const BatchScene* __restrict__ tab,
                                                     const uint32_t* __restrict__ pixPrefix,
-                                                    uint32_t nActive, uint32_t slot) {
+                                                    uint32_t nActive, uint32_t slot,
+                                                    const float4* __restrict__ Ls,
+                                                    const float2* __restrict__ Ps,
+                                                    uint32_t stride) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= pixPrefix[nActive]) return;
     const uint32_t j = findScene(pixPrefix, nActive, i);
     const BatchScene& b = tab[j];
-    const size_t off = size_t(slot) * b.cam.width * b.cam.height;
-    gatherPixel(reinterpret_cast<const float4*>(b.L) + off,
-                reinterpret_cast<const float2*>(b.P) + off, b.cam.width, b.cam.height, b.sum,
-                b.wsum, i - pixPrefix[j]);
+    const float4* L;
+    const float2* P;
+    if (Ls) {
+        const size_t off = size_t(slot) * stride + pixPrefix[j];
+        L = Ls + off;
+        P = Ps + off;
+    } else {
+        const size_t off = size_t(slot) * b.cam.width * b.cam.height;
+        L = reinterpret_cast<const float4*>(b.L) + off;
+        P = reinterpret_cast<const float2*>(b.P) + off;
+    }
+    gatherPixel(L, P, b.cam.width, b.cam.height, b.sum, b.wsum, i - pixPrefix[j]);
 }
 
 // ---------------------------------------------------------------------------
@@ -257,7 +271,16 @@ hipError_t launchBatchPass(const BatchScene* tab, const uint32_t* tilePrefix,
     if (e != hipSuccess) return e;
     for (uint32_t k = 0; k < slots; ++k)
         hipLaunchKernelGGL(kBatchGather, dim3((pixels + 255u) / 256u), dim3(256), 0, st, tab,
-                           pixPrefix, nActive, k);
+                           pixPrefix, nActive, k, nullptr, nullptr, 0u);
+    return hipGetLastError();
+}
+
+hipError_t launchBatchGatherPass(const BatchScene* tab, const uint32_t* pixPrefix,
+                                 uint32_t nActive, uint32_t pixels, uint32_t slots,
+                                 const float4* L, const float2* P, hipStream_t st) {
+    for (uint32_t k = 0; k < slots; ++k)
+        hipLaunchKernelGGL(kBatchGather, dim3((pixels + 255u) / 256u), dim3(256), 0, st, tab,
+                           pixPrefix, nActive, k, L, P, pixels);
     return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 #include "integrator.h"
 #include "path.h"
 #include "wave.h"
+#include "batch_device.h"
 
 namespace crh {
 
@@ -180,18 +181,6 @@ kRenderRegen(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint3
 // The table holds the active scenes first (sorted by decreasing spp, so the
 // scenes that still render at pass k are a prefix of it).  tilePrefix[j] is
 // the first work item (tile) of scene j; tilePrefix[nActive] = item count.
-
-// Index j with prefix[j] <= item < prefix[j + 1] (prefix[0] = 0).
-CRH_D uint32_t findScene(const uint32_t* __restrict__ prefix, uint32_t n, uint32_t item) {
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1u) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (prefix[mid] <= item) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 
 // Persistent blocks take tiles from `chunks` contiguous ranges of the item
 // list, each with its own counter; block b starts on range b % chunks (the

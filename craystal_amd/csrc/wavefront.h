@@ -96,10 +96,21 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                          uint32_t guideMask = 3u, uint32_t guideDepth = 0u,
                          uint32_t guideTarget = 0u, bool spectral = false);
 
+// One pass of a batch: sample indices si .. si + active - 1 of every pixel
+// of the first nActive scenes of the table (pass pixel q is pixel
+// q - pixPrefix[j] of scene j, pixPrefix[nActive] = w.npix), their films'
+// samples in w.rad / w.P at slot * w.npix + q.  Scene views and cameras come
+// from the table; stackCap: the deepest of their trees.  No guide, no
+// counted work.
+hipError_t wavefrontBatchPass(const BatchScene* tab, const uint32_t* pixPrefix, uint32_t nActive,
+                              uint32_t stackCap, const IntegratorView& cfg, uint32_t si,
+                              uint32_t active, WfState& w, const WfLaunch& L, hipStream_t st,
+                              bool spectral);
+
 // The finishing launch (kWfFinish) of one pass; launchWfFinishSM<SM> is
 // defined once per stack mode (wf_finish.hip built with CRH_WF_SM = SM).
 struct WfFinishArgs {
-    SceneView s;
+    SceneView s;  // one scene, or (batch) the table below
     WfState w;
     IntegratorView cfg;
     uint32_t cur, d0, grid;
@@ -107,6 +118,10 @@ struct WfFinishArgs {
     uint32_t waves;
     bool spectral;
     hipStream_t st;
+    bool batch;
+    const BatchScene* tab;
+    const uint32_t* pixPrefix;
+    uint32_t nActive;
 };
 template <int SM> hipError_t launchWfFinishSM(const WfFinishArgs& a);
 template <> hipError_t launchWfFinishSM<0>(const WfFinishArgs&);

@@ -43,24 +43,33 @@ __global__ __launch_bounds__(64) void kWfReset(uint32_t* c, uint32_t mask) {
 }
 
 // SPEC: the path draws its hero bin right after the pixel jitter (as the
-// megakernel's tracePixel) and keeps it in its flags.
-template <bool SPEC>
+// megakernel's tracePixel) and keeps it in its flags.  A batched pass's
+// pixel p % npix is pixel p % npix - pixPrefix[j] of active scene j, seen
+// through that scene's camera; the path keeps j in its flags.
+template <bool SPEC, class Scenes>
 __global__ __launch_bounds__(256) void kWfGenerate(WfState w, CameraView cam, uint32_t si,
-                                                   const float* spec) {
+                                                   Scenes S) {
     uint32_t n = w.active * w.npix;
     PERSISTENT_LOOP(p, n) {
-        uint32_t px = p % w.npix;
-        uint32_t x = px % w.width, y = px / w.width;
+        uint32_t px = p % w.npix, width = w.width, sc = 0u;
+        const CameraView* cv = &cam;
+        if constexpr (Scenes::kBatch) {
+            sc = findScene(S.pixPrefix, S.nActive, px);
+            px -= S.pixPrefix[sc];
+            cv = &S.tab[sc].cam;
+            width = cv->width;
+        }
+        uint32_t x = px % width, y = px / width;
         Sampler smp(x, y, si + p / w.npix);
         F2 j = smp.next2D();
         F2 pix{float(x) + j.x, float(y) + j.y};
-        const uint32_t hero = SPEC ? heroLanes(spec, smp.next1D()).b0 : 0u;
-        Ray r = cameraRay(cam, pix);
+        const uint32_t hero = SPEC ? heroLanes(S.view(sc).spec, smp.next1D()).b0 : 0u;
+        Ray r = cameraRay(*cv, pix);
         storeRay(w.rayO, w.rayD, p, r);
         w.rng[p] = smp.state;
         stS(w.beta, p, sp(1.f));
         stS(w.rad, p, sp(0.f));
-        w.flags[p] = F_ALIVE | (hero << kHeroShift);
+        w.flags[p] = F_ALIVE | (hero << kHeroShift) | (sc << kSceneShift);
         if (w.recId) w.recId[p] = kNoRecord;
         w.P[p] = make_float2(pix.x, pix.y);
         w.qExt[0][p] = p;
@@ -69,8 +78,8 @@ __global__ __launch_bounds__(256) void kWfGenerate(WfState w, CameraView cam, ui
 }
 
 // Closest hit for the extension queue followed by the MIS queue.
-template <bool COUNT, int SM>
-__global__ __launch_bounds__(256) void kWfExtend(SceneView s, WfState w,
+template <bool COUNT, int SM, class Scenes>
+__global__ __launch_bounds__(256) void kWfExtend(Scenes S, WfState w,
                                                  uint32_t cur) {
     extern __shared__ uint2 lds[];
     auto stk = makeStack<SM>(lds);
@@ -84,13 +93,13 @@ __global__ __launch_bounds__(256) void kWfExtend(SceneView s, WfState w,
         if (i < nExt) {
             uint32_t p = w.qExt[cur][i];
             Ray r = loadRay(w.rayO, w.rayD, p);
-            bool hit = closestHit<COUNT>(s, r, h, stk, wc);
+            bool hit = closestHit<COUNT>(S.view(sceneIdx<Scenes>(w, p)), r, h, stk, wc);
             storeHit(w.hit, p, h);
             if (hit) atomicOr(&w.flags[p], F_EXTHIT);
         } else {
             uint32_t p = w.qMis[cur][i - nExt];
             Ray r = loadRay(w.misO, w.misD, p);
-            bool hit = closestHit<COUNT>(s, r, h, stk, wc);
+            bool hit = closestHit<COUNT>(S.view(sceneIdx<Scenes>(w, p)), r, h, stk, wc);
             storeHit(w.misHit, p, h);
             if (hit) atomicOr(&w.flags[p], F_MISHIT);
         }
@@ -165,13 +174,14 @@ __global__ __launch_bounds__(256) void kWfShadowPT(SceneView s, WfState w, uint3
 
 // Resolve the MIS estimate of the previous bounce:
 // radiance += (value / selPdf) * beta_k  (Core/Integrator.cu:139, :186).
-template <bool SPEC>
-__global__ __launch_bounds__(256) void kWfMisResolve(SceneView s, WfState w,
+template <bool SPEC, class Scenes>
+__global__ __launch_bounds__(256) void kWfMisResolve(Scenes S, WfState w,
                                                      uint32_t cur) {
     const uint32_t n = w.counters[kCMis0 + cur];
     PERSISTENT_LOOP(i, n) {
         uint32_t p = w.qMis[cur][i];
         uint32_t fl = w.flags[p];
+        const auto& s = S.view(sceneOf(fl));
         Ray r = loadRay(w.misO, w.misD, p);
         Hit h = loadHit(w.misHit, p);
         float4 vs = w.misVal[p];
@@ -190,13 +200,14 @@ __global__ __launch_bounds__(256) void kWfMisResolve(SceneView s, WfState w,
 
 // Process the extension hit: environment / emitter terms, or light
 // selection producing a shadow ray (Core/Integrator.cu:165-190, :25-53).
-template <bool GUIDED, bool SPEC>
-__global__ __launch_bounds__(256) void kWfShadeHit(SceneView s, WfState w,
+template <bool GUIDED, bool SPEC, class Scenes>
+__global__ __launch_bounds__(256) void kWfShadeHit(Scenes S, WfState w,
                                                    uint32_t cur) {
     const uint32_t n = w.counters[kCExt0 + cur];
     PERSISTENT_LOOP(i, n) {
         uint32_t p = w.qExt[cur][i];
         uint32_t fl = w.flags[p];
+        const auto& s = S.view(sceneOf(fl));
         Spec beta = ldS(w.beta, p);
         if (GUIDED) {
             // Radiance before anything found along the guided direction of
@@ -228,7 +239,7 @@ __global__ __launch_bounds__(256) void kWfShadeHit(SceneView s, WfState w,
                     w.gNrm[p] = make_float4(it.frame.N.x, it.frame.N.y, it.frame.N.z, 0.f);
                     w.gWo[p] = make_float4(it.view.x, it.view.y, it.view.z, 0.f);
                 }
-                Sampler smp = loadSampler(w, p);
+                Sampler smp = loadSampler(S, w, p, fl);
                 LightGeom g = lightGeom(s, it, smp);
                 w.rng[p] = smp.state;
                 if (g.valid) {
@@ -251,8 +262,8 @@ __global__ __launch_bounds__(256) void kWfShadeHit(SceneView s, WfState w,
     }
 }
 
-template <int SM>
-__global__ __launch_bounds__(256) void kWfShadow(SceneView s, WfState w) {
+template <int SM, class Scenes>
+__global__ __launch_bounds__(256) void kWfShadow(Scenes S, WfState w) {
     extern __shared__ uint2 lds[];
     auto stk = makeStack<SM>(lds);
     const uint32_t n = w.counters[kCShadow];
@@ -262,15 +273,15 @@ __global__ __launch_bounds__(256) void kWfShadow(SceneView s, WfState w) {
         uint32_t p = w.qShadow[i];
         Ray r = loadRay(w.shO, w.shD, p);
         Hit h;
-        bool occ = traverse<true, false>(s, r, h, stk, wc);
+        bool occ = traverse<true, false>(S.view(sceneIdx<Scenes>(w, p)), r, h, stk, wc);
         if (occ) w.flags[p] &= ~F_LIGHT;  // only this thread touches p here
     }
 }
 
 // Light weight, MIS BSDF ray and the continuation sample
 // (Core/Integrator.cu:57-99, :186-223).
-template <bool MIS, bool GUIDED, bool SPEC>
-__global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
+template <bool MIS, bool GUIDED, bool SPEC, class Scenes>
+__global__ __launch_bounds__(256) void kWfShadeCont(Scenes S, WfState w,
                                                     IntegratorView cfg,
                                                     uint32_t cur, uint32_t depth,
                                                     float alpha, uint32_t gmask) {
@@ -279,6 +290,7 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
     PERSISTENT_LOOP(i, n) {
         uint32_t p = w.qCont[i];
         uint32_t fl = w.flags[p];
+        const auto& s = S.view(sceneOf(fl));
         Ray r = loadRay(w.rayO, w.rayD, p);
         Hit h = loadHit(w.hit, p);
         Isect it = makeIsect(s, r, h);
@@ -286,7 +298,7 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
         const GMaterial& m = s.mats[mid];
         const Lanes lw = SPEC ? pathLanes(s, fl) : rgbLanes();
         BSDF bsdf = makeBSDF(s, m, mid, it, h, lw);
-        Sampler smp = loadSampler(w, p);
+        Sampler smp = loadSampler(S, w, p, fl);
         Spec beta = ldS(w.beta, p);
         Spec rad = ldS(w.rad, p);
         LightSample ls = noLight();
@@ -383,10 +395,12 @@ __global__ __launch_bounds__(256) void kWfShadeCont(SceneView s, WfState w,
 
 // Spectral transport: every path's three-lane radiance to CIE XYZ once its
 // pass has ended (what tracePixel does when its path ends).
-__global__ __launch_bounds__(256) void kWfToXYZ(SceneView s, WfState w) {
+template <class Scenes>
+__global__ __launch_bounds__(256) void kWfToXYZ(Scenes S, WfState w) {
     const uint32_t n = w.active * w.npix;
     PERSISTENT_LOOP(p, n) {
-        stS(w.rad, p, laneXYZ(ldS(w.rad, p), pathLanes(s, w.flags[p])));
+        const uint32_t fl = w.flags[p];
+        stS(w.rad, p, laneXYZ(ldS(w.rad, p), pathLanes(S.view(sceneOf(fl)), fl)));
     }
 }
 
@@ -433,39 +447,48 @@ __global__ __launch_bounds__(256) void kWfRecordsToSamples(WfState w, GuideSampl
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-template <int SM>
+template <int SM, class Scenes>
 static void launchExtend(bool count, dim3 g, dim3 b, size_t lds, hipStream_t st,
-                         const SceneView& s, const WfState& w, uint32_t cur) {
+                         const Scenes& S, const WfState& w, uint32_t cur) {
     if (count)
-        hipLaunchKernelGGL((kWfExtend<true, SM>), g, b, lds, st, s, w, cur);
+        hipLaunchKernelGGL((kWfExtend<true, SM, Scenes>), g, b, lds, st, S, w, cur);
     else
-        hipLaunchKernelGGL((kWfExtend<false, SM>), g, b, lds, st, s, w, cur);
+        hipLaunchKernelGGL((kWfExtend<false, SM, Scenes>), g, b, lds, st, S, w, cur);
 }
 
-hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
-                         const IntegratorView& cfg, uint32_t si, uint32_t active, WfState& w,
-                         const WfLaunch& L, bool countWork, hipStream_t st,
-                         const GuideView* guide, float alpha, GuideSample* samples,
-                         uint32_t sampleCap, uint32_t guideMask, uint32_t guideDepth,
-                         uint32_t guideTarget, bool spectral) {
+// One wavefront pass over the scene(s) of S.  stackCap: the deepest tree of
+// them; cam: the camera of a single scene (a batch's come from its table).
+// Batched passes trace one ray per thread (no lane refill: the persistent
+// traversal kernels hold one scene).
+template <class Scenes>
+static hipError_t wfPassImpl(const Scenes& S, uint32_t stackCap, const CameraView& cam,
+                             const IntegratorView& cfg, uint32_t si, uint32_t active, WfState& w,
+                             const WfLaunch& L, bool countWork, hipStream_t st,
+                             const GuideView* guide, float alpha, GuideSample* samples,
+                             uint32_t sampleCap, uint32_t guideMask, uint32_t guideDepth,
+                             uint32_t guideTarget, bool spectral) {
     if (active == 0 || active > w.slots) return hipErrorInvalidValue;
-    if (spectral && (!s.spec || guide || countWork)) return hipErrorInvalidValue;
+    if (spectral && (guide || countWork)) return hipErrorInvalidValue;
+    if (Scenes::kBatch && (guide || countWork)) return hipErrorInvalidValue;
     w.active = active;
     const uint32_t TB = L.traceBlock;
-    const size_t lds = stackLdsBytes(L.privStack, s.stackCap, TB);
+    const size_t lds = stackLdsBytes(L.privStack, stackCap, TB);
     const dim3 b256(256), bT(TB);
     const bool guided = guide != nullptr;
-    const bool pt = L.persistent != 0;
+    const bool pt = L.persistent != 0 && !Scenes::kBatch;
     auto extend = [&](uint32_t cur, bool count) {
         if (pt && !count)
             hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters, 1u << kCPtExt);
         withStack(L.privStack, [&](auto sm) {
             constexpr int SM = decltype(sm)::value;
-            if (pt && !count)
-                hipLaunchKernelGGL((kWfExtendPT<SM>), dim3(L.traceGrid), bT, lds, st, s, w, cur,
-                                   L.refill);
-            else
-                launchExtend<SM>(count, dim3(L.traceGrid), bT, lds, st, s, w, cur);
+            if constexpr (!Scenes::kBatch) {
+                if (pt && !count) {
+                    hipLaunchKernelGGL((kWfExtendPT<SM>), dim3(L.traceGrid), bT, lds, st,
+                                       S.s, w, cur, L.refill);
+                    return;
+                }
+            }
+            launchExtend<SM>(count, dim3(L.traceGrid), bT, lds, st, S, w, cur);
         });
     };
     auto shadow = [&]() {
@@ -473,11 +496,14 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
             hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters, 1u << kCPtShadow);
         withStack(L.privStack, [&](auto sm) {
             constexpr int SM = decltype(sm)::value;
-            if (pt)
-                hipLaunchKernelGGL((kWfShadowPT<SM>), dim3(L.traceGrid), bT, lds, st, s, w,
-                                   L.refill);
-            else
-                hipLaunchKernelGGL((kWfShadow<SM>), dim3(L.traceGrid), bT, lds, st, s, w);
+            if constexpr (!Scenes::kBatch) {
+                if (pt) {
+                    hipLaunchKernelGGL((kWfShadowPT<SM>), dim3(L.traceGrid), bT, lds, st,
+                                       S.s, w, L.refill);
+                    return;
+                }
+            }
+            hipLaunchKernelGGL((kWfShadow<SM, Scenes>), dim3(L.traceGrid), bT, lds, st, S, w);
         });
     };
     hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
@@ -485,14 +511,18 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
                            (1u << kCShadow) | (1u << kCCont) | (1u << kCRec) |
                            (1u << kCTrain));
     if (spectral)
-        hipLaunchKernelGGL(kWfGenerate<true>, dim3(L.shadeGrid), b256, 0, st, w, cam, si, s.spec);
+        hipLaunchKernelGGL((kWfGenerate<true, Scenes>), dim3(L.shadeGrid), b256, 0, st, w, cam,
+                           si, S);
     else
-        hipLaunchKernelGGL(kWfGenerate<false>, dim3(L.shadeGrid), b256, 0, st, w, cam, si, s.spec);
+        hipLaunchKernelGGL((kWfGenerate<false, Scenes>), dim3(L.shadeGrid), b256, 0, st, w, cam,
+                           si, S);
     auto misResolve = [&](uint32_t c) {
         if (spectral)
-            hipLaunchKernelGGL(kWfMisResolve<true>, dim3(L.shadeGrid), b256, 0, st, s, w, c);
+            hipLaunchKernelGGL((kWfMisResolve<true, Scenes>), dim3(L.shadeGrid), b256, 0, st, S,
+                               w, c);
         else
-            hipLaunchKernelGGL(kWfMisResolve<false>, dim3(L.shadeGrid), b256, 0, st, s, w, c);
+            hipLaunchKernelGGL((kWfMisResolve<false, Scenes>), dim3(L.shadeGrid), b256, 0, st, S,
+                               w, c);
     };
     uint32_t cur = 0;
     // Bounces run as wavefront launches up to finishDepth; the paths still
@@ -506,44 +536,63 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
         hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters,
                            (1u << (kCExt0 + nxt)) | (1u << (kCMis0 + nxt)) |
                                (1u << kCShadow) | (1u << kCCont));
-        if (guided)
-            hipLaunchKernelGGL((kWfShadeHit<true, false>), dim3(L.shadeGrid), b256, 0, st, s, w,
-                               cur);
-        else if (spectral)
-            hipLaunchKernelGGL((kWfShadeHit<false, true>), dim3(L.shadeGrid), b256, 0, st, s, w,
-                               cur);
-        else
-            hipLaunchKernelGGL((kWfShadeHit<false, false>), dim3(L.shadeGrid), b256, 0, st, s, w,
-                               cur);
+        if constexpr (!Scenes::kBatch) {
+            if (guided)
+                hipLaunchKernelGGL((kWfShadeHit<true, false, Scenes>), dim3(L.shadeGrid), b256, 0,
+                                   st, S, w, cur);
+        }
+        if (!guided && spectral)
+            hipLaunchKernelGGL((kWfShadeHit<false, true, Scenes>), dim3(L.shadeGrid), b256, 0, st,
+                               S, w, cur);
+        else if (!guided)
+            hipLaunchKernelGGL((kWfShadeHit<false, false, Scenes>), dim3(L.shadeGrid), b256, 0,
+                               st, S, w, cur);
         shadow();
         // Past guideDepth the guided render continues like the plain one (no
         // inference); records of earlier vertices still get their snapshots.
         const bool guideHere = guided && guideMask != 0u && (guideDepth == 0u || d < guideDepth);
         const uint32_t gm = guideMask;
-        if (guideHere) {
-            hipError_t e = launchGuideInfer(*guide, w.qCont, w.counters + kCCont, 0u, w.gPos,
-                                            w.gNrm, w.gWo, w.gMix, nullptr, L.shadeGrid, st);
-            if (e != hipSuccess) return e;
 #define CRH_WFC(MIS, GUIDED, SPEC, GM)                                                      \
-    hipLaunchKernelGGL((kWfShadeCont<MIS, GUIDED, SPEC>), dim3(L.shadeGrid), b256, 0, st, s, w, \
-                       cfg, cur, d, alpha, GM)
-            if (cfg.useMIS) CRH_WFC(true, true, false, gm);
-            else CRH_WFC(false, true, false, gm);
+    hipLaunchKernelGGL((kWfShadeCont<MIS, GUIDED, SPEC, Scenes>), dim3(L.shadeGrid), b256, 0, st, \
+                       S, w, cfg, cur, d, alpha, GM)
+        if (guideHere) {
+            if constexpr (!Scenes::kBatch) {
+                hipError_t e = launchGuideInfer(*guide, w.qCont, w.counters + kCCont, 0u, w.gPos,
+                                                w.gNrm, w.gWo, w.gMix, nullptr, L.shadeGrid, st);
+                if (e != hipSuccess) return e;
+                if (cfg.useMIS) CRH_WFC(true, true, false, gm);
+                else CRH_WFC(false, true, false, gm);
+            }
         } else if (spectral) {
             if (cfg.useMIS) CRH_WFC(true, false, true, 0u);
             else CRH_WFC(false, false, true, 0u);
         } else {
             if (cfg.useMIS) CRH_WFC(true, false, false, 0u);
             else CRH_WFC(false, false, false, 0u);
-#undef CRH_WFC
         }
+#undef CRH_WFC
         cur = nxt;
     }
     if (finish) {
         hipLaunchKernelGGL(kWfReset, dim3(1), dim3(64), 0, st, w.counters, 1u << kCPtExt);
-        const WfFinishArgs fa{s, w, cfg, cur, dEnd, L.finishGrid,
-                              stackLdsBytes(L.privStack, s.stackCap, 256), L.finishWaves,
-                              spectral, st};
+        WfFinishArgs fa{};
+        fa.w = w;
+        fa.cfg = cfg;
+        fa.cur = cur;
+        fa.d0 = dEnd;
+        fa.grid = L.finishGrid;
+        fa.lds = stackLdsBytes(L.privStack, stackCap, 256);
+        fa.waves = L.finishWaves;
+        fa.spectral = spectral;
+        fa.st = st;
+        if constexpr (Scenes::kBatch) {
+            fa.batch = true;
+            fa.tab = S.tab;
+            fa.pixPrefix = S.pixPrefix;
+            fa.nActive = S.nActive;
+        } else {
+            fa.s = S.s;
+        }
         hipError_t e = hipSuccess;
         withStack(L.privStack, [&](auto sm) { e = launchWfFinishSM<decltype(sm)::value>(fa); });
         if (e != hipSuccess) return e;
@@ -555,11 +604,31 @@ hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
         misResolve(cur);
     }
     if (spectral)
-        hipLaunchKernelGGL(kWfToXYZ, dim3(L.shadeGrid), b256, 0, st, s, w);
+        hipLaunchKernelGGL((kWfToXYZ<Scenes>), dim3(L.shadeGrid), b256, 0, st, S, w);
     if (guided && samples)
         hipLaunchKernelGGL(kWfRecordsToSamples, dim3(L.shadeGrid), b256, 0, st, w, samples,
                            sampleCap, guideTarget);
     return hipGetLastError();
+}
+
+hipError_t wavefrontPass(const SceneView& s, const CameraView& cam,
+                         const IntegratorView& cfg, uint32_t si, uint32_t active, WfState& w,
+                         const WfLaunch& L, bool countWork, hipStream_t st,
+                         const GuideView* guide, float alpha, GuideSample* samples,
+                         uint32_t sampleCap, uint32_t guideMask, uint32_t guideDepth,
+                         uint32_t guideTarget, bool spectral) {
+    if (spectral && !s.spec) return hipErrorInvalidValue;
+    return wfPassImpl(OneScene{s}, s.stackCap, cam, cfg, si, active, w, L, countWork, st, guide,
+                      alpha, samples, sampleCap, guideMask, guideDepth, guideTarget, spectral);
+}
+
+hipError_t wavefrontBatchPass(const BatchScene* tab, const uint32_t* pixPrefix, uint32_t nActive,
+                              uint32_t stackCap, const IntegratorView& cfg, uint32_t si,
+                              uint32_t active, WfState& w, const WfLaunch& L, hipStream_t st,
+                              bool spectral) {
+    if (nActive == 0 || nActive > kSceneMask + 1u) return hipErrorInvalidValue;
+    return wfPassImpl(SceneTable{tab, pixPrefix, nActive}, stackCap, CameraView{}, cfg, si,
+                      active, w, L, false, st, nullptr, 0.5f, nullptr, 0u, 3u, 0u, 0u, spectral);
 }
 
 }  // namespace crh

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "batch_device.h"
 #include "integrator.h"
 #include "path.h"
 #include "wavefront.h"
@@ -43,6 +44,32 @@ CRH_D Spec ldS(const float4* a, uint32_t p) {
 CRH_D void stS(float4* a, uint32_t p, Spec s, float w = 0.f) {
     a[p] = make_float4(s.r, s.g, s.b, w);
 }
+// Where a path's scene comes from: the one scene of a render (OneScene, the
+// view passed by value) or the batch table (SceneTable), in which case the
+// path carries its scene's index in bits kSceneShift.. of its flags (set by
+// the generate kernel) and pixel p of a pass is pixel p - pixPrefix[j] of
+// scene j.
+constexpr uint32_t kSceneShift = 8, kSceneMask = 0x1FFFu;  // up to 8192 scenes
+CRH_D uint32_t sceneOf(uint32_t fl) { return (fl >> kSceneShift) & kSceneMask; }
+struct OneScene {
+    SceneView s;
+    static constexpr bool kBatch = false;
+    CRH_D const SceneView& view(uint32_t) const { return s; }
+};
+struct SceneTable {
+    const BatchScene* tab;
+    const uint32_t* pixPrefix;  // first pixel of each active scene, then the total
+    uint32_t nActive;
+    static constexpr bool kBatch = true;
+    CRH_D SceneView view(uint32_t j) const { return tab[j].view; }
+};
+// The scene index of path p (0 for one scene).
+template <class Scenes>
+CRH_D uint32_t sceneIdx(const WfState& w, uint32_t p) {
+    if constexpr (Scenes::kBatch) return sceneOf(w.flags[p]);
+    else return 0u;
+}
+
 CRH_D Sampler loadSampler(const WfState& w, uint32_t p) {
     uint32_t pix = p % w.npix;
     uint32_t x = pix % w.width, y = pix / w.width;
@@ -50,6 +77,23 @@ CRH_D Sampler loadSampler(const WfState& w, uint32_t p) {
     s.state = w.rng[p];
     s.inc = x * 1234567891u + y * 987654321u + 1u;
     return s;
+}
+
+// The sampler of path p of a batched pass: the pixel coordinates are its
+// scene's (flags fl carry the scene index).
+template <class Scenes>
+CRH_D Sampler loadSampler(const Scenes& S, const WfState& w, uint32_t p, uint32_t fl) {
+    if constexpr (!Scenes::kBatch) {
+        return loadSampler(w, p);
+    } else {
+        const uint32_t j = sceneOf(fl);
+        const uint32_t pix = p % w.npix - S.pixPrefix[j], width = S.tab[j].cam.width;
+        const uint32_t x = pix % width, y = pix / width;
+        Sampler s(0u, 0u, 0u);
+        s.state = w.rng[p];
+        s.inc = x * 1234567891u + y * 987654321u + 1u;
+        return s;
+    }
 }
 
 #define PERSISTENT_LOOP(i, count)                                            \

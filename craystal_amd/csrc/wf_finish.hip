@@ -4,6 +4,8 @@
 // parallel with the rest of the wavefront.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 #include "wave.h"
 #include "wf_device.h"
@@ -18,6 +20,12 @@ using namespace wf;
 
 namespace {
 
+template <class Scenes>
+CRH_D const SceneView& laneScene(const SceneView& sv, const Scenes& S) {
+    if constexpr (Scenes::kBatch) return sv;
+    else return S.view(0);
+}
+
 // Finishes every path still in flight after `d0` wavefront bounces in one
 // persistent launch: a lane takes the next entry of the extension and MIS
 // queues (a WaveDispenser: one atomic per chunk), traces and resolves the pending MIS ray of
@@ -28,11 +36,15 @@ namespace {
 // entry.  Each path performs the operations of the pure wavefront in the
 // same order, so the film is the same; the launch replaces the tail of
 // nearly empty bounces, each of which costs a full traversal launch.
-template <bool MIS, bool SPEC, int SM, int W>
+template <bool MIS, bool SPEC, int SM, int W, class Scenes>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void kWfFinish(
-    SceneView s, WfState w, IntegratorView cfg, uint32_t cur, uint32_t d0) {
+    Scenes S, WfState w, IntegratorView cfg, uint32_t cur, uint32_t d0) {
     extern __shared__ uint2 lds[];
     auto stk = makeStack<SM>(lds);
+    // A batch's lanes hold paths of different scenes: sv is the scene of the
+    // lane's current path (s refers to it); one scene's view is the argument.
+    SceneView sv = S.view(0);
+    const SceneView& s = laneScene(sv, S);
     PathCtx<false, typename StackOf<SM>::T> c{s, stk, 0, {0, 0}};
     const uint32_t nExt = w.counters[kCExt0 + cur];
     const uint32_t total = nExt + w.counters[kCMis0 + cur];
@@ -55,6 +67,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                 // extension entry (flags are not written here, so this test
                 // holds whichever entry a lane takes first)
                 if (alive || !(fl & F_ALIVE)) {
+                    if constexpr (Scenes::kBatch) sv = S.view(sceneOf(fl));
                     rad = ldS(w.rad, p);
                     if (SPEC) lw = pathLanes(s, fl);
                     if (fl & F_MIS) {
@@ -80,7 +93,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                         }
                         ray = loadRay(w.rayO, w.rayD, p);
                         beta = ldS(w.beta, p);
-                        smp = loadSampler(w, p);
+                        smp = loadSampler(S, w, p, fl);
                         depth = d0;
                         live = true;
                     } else {
@@ -114,14 +127,19 @@ hipError_t launchWfFinishSM<CRH_WF_SM>(const WfFinishArgs& a) {
     constexpr int SM = CRH_WF_SM;
     withWaves(a.waves, [&](auto wv) {
         constexpr int W = decltype(wv)::value;
-#define CRH_WFF(MIS, SPEC)                                                                       \
-    hipLaunchKernelGGL((kWfFinish<MIS, SPEC, SM, W>), dim3(a.grid), dim3(256), a.lds, a.st, a.s, \
-                       a.w, a.cfg, a.cur, a.d0)
-        if (a.spectral && a.cfg.useMIS) CRH_WFF(true, true);
-        else if (a.spectral) CRH_WFF(false, true);
-        else if (a.cfg.useMIS) CRH_WFF(true, false);
-        else CRH_WFF(false, false);
+        auto go = [&](const auto& S) {
+            using Scenes = std::decay_t<decltype(S)>;
+#define CRH_WFF(MIS, SPEC)                                                                  \
+    hipLaunchKernelGGL((kWfFinish<MIS, SPEC, SM, W, Scenes>), dim3(a.grid), dim3(256), a.lds, \
+                       a.st, S, a.w, a.cfg, a.cur, a.d0)
+            if (a.spectral && a.cfg.useMIS) CRH_WFF(true, true);
+            else if (a.spectral) CRH_WFF(false, true);
+            else if (a.cfg.useMIS) CRH_WFF(true, false);
+            else CRH_WFF(false, false);
 #undef CRH_WFF
+        };
+        if (a.batch) go(SceneTable{a.tab, a.pixPrefix, a.nActive});
+        else go(OneScene{a.s});
     });
     return hipGetLastError();
 }

@@ -147,8 +147,8 @@ def test_batch_argument_errors(gpu):
         batch.render(cams, [1, 1], CFG)
     s1 = cr.Sensor(camds[1]["size"])
     cams[1] = scenes.make_camera(camds[1], s1)
-    with pytest.raises(cr.RuntimeError):          # wavefront mode is not batched
-        batch.render(cams, [1, 1], cr.Configs(mode=L.MODE_WAVEFRONT))
+    with pytest.raises(cr.RuntimeError):          # an unknown integrator mode
+        batch.render(cams, [1, 1], cr.Configs(mode=7))
     with pytest.raises(cr.RuntimeError):
         cr.get_context(0).set_tuning(batch_tile_w=12)
 
@@ -185,6 +185,56 @@ def test_batch_spectral_equals_solo_renders(gpu):
             for s in sensors:
                 s.clear()
             batch.render(cams, spp, cfg, first_sample=first)
+            assert _same(_films(sensors), solo), kw
+    finally:
+        ctx.set_tuning(**saved)
+
+
+@pytest.mark.parametrize("spectral", [False, True])
+def test_batch_wavefront_equals_solo_renders(gpu, spectral):
+    """The batched wavefront: one launch per stage and bounce covers the
+    paths of every active scene (each path keeps its scene's index in its
+    flags; views and cameras come from the batch table), several sample
+    indices per pass as the path state allows, scenes with fewer samples
+    dropping out; each film is bit-identical to the scene rendered alone by
+    the megakernel, directly and from the captured graph, for several path
+    pools (queue capacities) and finishing depths."""
+    items = _small_batch(n=5, seed=13)
+    scene_list = [cr.Scene(d) for d, _ in items]
+    camds = [c for _, c in items]
+    spp = [c["spp"] for c in camds]
+    spp[2] = 0                          # a scene with nothing to render this call
+    cfg = cr.Configs(maxDepth=12, rrThreshold=0.05, useMIS=True, rrStartDepth=3,
+                     spectral=spectral)
+    first = 41
+    solo = []
+    for sc, camd, k in zip(scene_list, camds, spp):
+        sensor = cr.Sensor(camd["size"])
+        sc.setCamera(scenes.make_camera(camd, sensor))
+        if k:
+            cr.PathTraceIntegrator(cfg).dispatch(sc, k, first_sample=first)
+        else:
+            sensor.readbackDeviceData()
+        solo.append((sensor.data.copy(), sensor.weight.copy()))
+    batch = Batch(scene_list)
+    sensors = [cr.Sensor(c["size"]) for c in camds]
+    cams = [scenes.make_camera(c, s) for c, s in zip(camds, sensors)]
+    npix = sum(c["size"][0] * c["size"][1] for c in camds)
+    wcfg = cr.Configs(**{**cfg.__dict__, "mode": L.MODE_WAVEFRONT})
+    ctx = cr.get_context(0)
+    saved = ctx.get_tuning()
+    try:
+        for kw in (dict(batch_graph=1, wf_pool=npix, wf_finish=65),
+                   dict(batch_graph=1, wf_pool=npix, wf_finish=65),     # graph replay
+                   dict(batch_graph=2, wf_pool=3 * npix, wf_finish=2),
+                   dict(batch_graph=1, wf_pool=2 * npix, wf_finish=1, wf_stack=1),
+                   dict(batch_graph=2, wf_pool=1 << 22, wf_finish=3, wf_stack=3,
+                        wf_trace_block=64, wf_trace_grid=7, wf_shade_grid=5)):
+            ctx.set_tuning(**kw)
+            for s in sensors:
+                s.clear()
+            st = batch.render(cams, spp, wcfg, first_sample=first)
+            assert st["rays"] > 0
             assert _same(_films(sensors), solo), kw
     finally:
         ctx.set_tuning(**saved)
