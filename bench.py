@@ -63,6 +63,11 @@ def parse():
     ap.add_argument("--no-terrain", action="store_true",
                     help="skip the large-scale leg (the 43.8M-triangle terrain, four "
                          "3840x2160 views in one batched launch, and its closest-hit batch)")
+    ap.add_argument("--tuning", type=str, default=None,
+                    help="JSON of launch shapes (crh_tuning fields) to apply instead of "
+                         "running the autotuner on the main workload")
+    ap.add_argument("--only-main", action="store_true",
+                    help="only the main workload (no other legs, CPU baseline or traffic runs)")
     ap.add_argument("--no-c1", action="store_true",
                     help="skip the small spectral Cornell-box configuration")
     ap.add_argument("--terrain-bvh", choices=("fast", "reference"), default="fast",
@@ -90,7 +95,12 @@ def parse():
                          "at 256x256, 16 spp of the fused guided kernel with training, 16 "
                          "without, 16 of the guided wavefront, "
                          "then 16 spp of the plain megakernel and of the plain wavefront")
-    return ap.parse_args()
+    a = ap.parse_args()
+    if a.only_main:
+        a.no_cpu = a.no_trace = a.no_traffic = a.no_terrain = a.no_c1 = True
+        a.batch_scenes = 0
+        a.guided_seconds = 0.0
+    return a
 
 
 def profile_wavefront(cr, L, scenes, tuning: str | None = None):
@@ -168,7 +178,7 @@ def profile_kernels(cr, L, scenes, tuning: str | None = None):
     cfg = cr.Configs(maxDepth=camd["max_depth"], rrThreshold=0.05, useMIS=True,
                      rrStartDepth=camd["rr_start_depth"])
     # one megakernel launch of as many sample indices as the tuning renders together
-    spl = max(1, min(8, cr.get_context(0).get_tuning()["render_pool"] // (W * H)))
+    spl = max(1, cr.get_context(0).get_tuning()["render_pool"] // (W * H))
     st = cr.PathTraceIntegrator(cfg).dispatch(scene, spl, first_sample=0, readback=False)
     print(json.dumps({"copy_bytes": nbytes, "trace_rays": n,
                       "trace_node_visits": cnt["node_visits"], "trace_tri_tests": cnt["tri_tests"],
@@ -588,14 +598,23 @@ def main():
     # launch shapes picked by the autotuner (outside the timed region; every
     # candidate must render the same image bit for bit)
     from craystal_amd import autotune
-    tuned = autotune.tune_render(scene, cfg, spp=16, stacks=(1, 2, 3), wf_grids=(16384,),
-                                 wf_blocks=(128, 256),
-                                 wf_persistent=((2, 0), (1, 16, 2048), (1, 16, 4096)),
-                                 wf_pools=(1 << 23, 1 << 24, 1 << 25), render_waves=(1, 4, 6, 8),
-                                 render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
-                                 wf_finishes=(65, 1, 2, 3), render_bands=(1, 2),
-                                 render_pools=(1 << 21, 1 << 23, 1 << 24, 1 << 25, 1 << 26),
-                                 pool_spp=32)
+    if a.tuning:
+        # launch shapes given (a previous run's "tuning"): no tuner launches,
+        # so a profiler sees only the timed run's kernels
+        ctx0 = cr.get_context(0)
+        ctx0.set_tuning(**{k: v for k, v in json.loads(a.tuning).items()
+                           if k in ctx0.TUNING_FIELDS})
+        t0_ = ctx0.get_tuning()
+        tuned = {**t0_, "wf_grid": t0_["wf_shade_grid"], "megakernel_ms": {},
+                 "wavefront_ms": {}, "render_pool_ms": {}}
+    else:
+        tuned = autotune.tune_render(
+            scene, cfg, spp=16, stacks=(1, 2, 3), wf_grids=(16384,), wf_blocks=(128, 256),
+            wf_persistent=((2, 0), (1, 16, 2048), (1, 16, 4096)),
+            wf_pools=(1 << 23, 1 << 24, 1 << 25), render_waves=(1, 4, 6, 8),
+            render_regen=((2, 0), (1, 1024), (1, 2048), (1, 4096)),
+            wf_finishes=(65, 1, 2, 3), render_bands=(1, 2),
+            render_pools=(1 << 21, 1 << 23, 1 << 24, 1 << 25, 1 << 26), pool_spp=32)
     sensor.clear()
 
     progress(f"scene built, autotuned: {tuned['render_stack']=} {tuned['wf_pool']=}")
@@ -629,16 +648,12 @@ def main():
     samples = px * a.steps
     msps = samples / elapsed / 1e6
     main_ms = stats["kernel_ms"]
-    # dominant kernel: the megakernel launches, timed on their own stream
-    st2 = L.crh_stats()
-    cfg_c = cfg.to_c()
-    nsp = max(1, min(8, a.steps))
-    L.call("crh_render", scene.handle, C.byref(cam.data), C.byref(cfg_c), a.warmup + a.steps,
-           nsp, sensor.handle, C.byref(st2))
-    # megakernel launches = pass groups (stats count them plus one film update per spp)
-    launches = max(1, int(st2.launches) - nsp)
-    avg_launch_s = st2.main_kernel_ms / launches / 1e3
-    samples_per_launch = nsp * px / launches
+    # dominant kernel: the megakernel launches of the timed run, timed with
+    # HIP events on their stream (launches = pass groups; the stats count
+    # them plus one film update per spp)
+    launches = max(1, int(stats["launches"]) - a.steps)
+    avg_launch_s = stats["main_kernel_ms"] / launches / 1e3
+    samples_per_launch = a.steps * px / launches
     achieved = b_sample * samples_per_launch / avg_launch_s / 1e9
 
     out = {
@@ -705,9 +720,13 @@ def main():
         rb = cr.DeviceBuffer(rays.nbytes)
         hb = cr.DeviceBuffer(n * 16)
         rb.copyFromHost(rays)
-        ttuned = autotune.tune_trace(scene, rays, reps=2, stacks=(2, 3), blocks=(64, 128, 256),
-                                     persistent=((2, 0), (1, 16), (1, 32)),
-                                     pt_grids=(2048, 4096), tris=(1, 2))
+        if a.tuning:
+            ttuned = cr.get_context(0).get_tuning()
+        else:
+            ttuned = autotune.tune_trace(scene, rays, reps=2, stacks=(2, 3),
+                                         blocks=(64, 128, 256),
+                                         persistent=((2, 0), (1, 16), (1, 32)),
+                                         pt_grids=(2048, 4096), tris=(1, 2))
         prof_tuning.update(stack=ttuned["stack"], trace_block=ttuned["trace_block"],
                            persistent=ttuned["persistent"])
         if ttuned["persistent"] == 1:
@@ -765,15 +784,19 @@ def main():
         out["c1"] = c1_leg(cr, L, scenes, not a.no_cpu)
         progress(f"C1: megakernel {out['c1']['megakernel']['msamples_per_s']}, wavefront "
                  f"{out['c1']['wavefront']['msamples_per_s']} Msamples/s")
+    batch_tuning = {}
     if a.batch_scenes > 0:
         ctx.set_tuning(**main_tuning)
         out["batch"] = batch_leg(cr, scenes, a.batch_scenes)
+        batch_tuning = out["batch"]["tuning"]
         progress(f"batch: {out['batch']['msamples_per_s']} Msamples/s")
     if a.guided_seconds > 0 and a.guided_runs > 0:
         ctx.set_tuning(**main_tuning)
         out["guided"] = guided_leg(cr, scenes, a.guided_seconds, a.guided_runs, a.guided_size)
     if not a.no_terrain:
+        # the four views render as a batch: at the batch leg's tuned shape
         ctx.set_tuning(**main_tuning)
+        ctx.set_tuning(**batch_tuning)
         out["terrain"] = terrain_leg(cr, scenes, a.terrain_spp, a.trace_rays, a.terrain_bvh)
 
     if not a.no_cpu:

@@ -200,6 +200,8 @@ SIGNATURES = {
     "crh_batch_destroy": [_P],
     "crh_render_guided": [_P, C.POINTER(crh_camera), C.POINTER(crh_integrator_cfg), _P,
                           C.POINTER(crh_guided_cfg), u32, u32, _P, C.POINTER(crh_stats)],
+    "crh_render_guided_views": [_P, _P, _P, u32, C.POINTER(crh_integrator_cfg), _P,
+                                C.POINTER(crh_guided_cfg), u32, u32, C.POINTER(crh_stats)],
 }
 GUIDE_PARAMS = 19760
 GUIDE_OUTPUTS = 48

@@ -544,8 +544,8 @@ class PathTraceIntegrator:
                sensor.handle, C.byref(st))
         if readback:
             sensor.readbackDeviceData()
-        self.last_stats = {"kernel_ms": st.kernel_ms, "total_ms": st.total_ms,
-                           "rays": st.rays, "launches": st.launches}
+        self.last_stats = {"kernel_ms": st.kernel_ms, "main_kernel_ms": st.main_kernel_ms,
+                           "total_ms": st.total_ms, "rays": st.rays, "launches": st.launches}
         return self.last_stats
 
 

@@ -242,6 +242,10 @@ struct crh_film {
     // their counters (records written, samples produced).
     DevMem gScratch, gRec, gCnt;
     uint32_t gRecCap = 0;
+    // Guided rendering of several views (crh_render_guided_views, kept by
+    // the first view's film): the pass's sample buffers of all views and the
+    // device view table.
+    DevMem vL, vP, vTab;
     ~crh_film() {
         for (hipEvent_t e : passEv) (void)hipEventDestroy(e);
     }
@@ -1305,6 +1309,51 @@ constexpr uint32_t kLossHist = 1024;
 
 float* lossSlot(crh_guide* g) { return g->loss.as<float>() + (g->step % kLossHist); }
 
+// After a guided pass group: its records become training samples (fused
+// kernel: from the final radiance in recL), which join the film's pending
+// samples; then up to active * train_steps Adam steps, one per full batch.
+int guidedTrainPass(crh_guide* g, crh_film* film, const crh_guided_cfg* gc, uint32_t active,
+                    bool fused, uint32_t recCap, const float4* recL, crh::GuideSample* smpOut,
+                    uint32_t smpRoom, uint32_t& lastSamples) {
+    crh_context* ctx = film->ctx;
+    const uint32_t batch = gc->train_batch;
+    crh::GuideSample* smpBuf = film->gSamples.as<crh::GuideSample>();
+    uint32_t* trainCounter = fused ? film->gCnt.as<uint32_t>() + 1
+                                   : film->wf.counters + crh::kCTrain;
+    if (fused)
+        HIP_TRY(crh::launchGuidedRecordsToSamples(film->gRec.as<float4>(), film->gCnt.as<uint32_t>(),
+                                                  recCap, recL, smpOut, smpRoom, trainCounter,
+                                                  gc->target, ctx->stream));
+    uint32_t produced = 0;
+    HIP_TRY(hipMemcpyAsync(&produced, trainCounter, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    produced = uint32_t(std::min<size_t>(produced, film->gCap - film->gPending));
+    lastSamples = produced;
+    g->lastSamples = produced;
+    const size_t pending = film->gPending + produced;
+    uint32_t k = 0;
+    for (; k < gc->train_steps * active && size_t(k + 1) * batch <= pending; ++k) {
+        g->step++;
+        float* slot = lossSlot(g);
+        HIP_TRY(hipMemsetAsync(slot, 0, 4, ctx->stream));
+        HIP_TRY(crh::launchGuideTrainStep(g->view, smpBuf + size_t(k) * batch, batch,
+                                          g->hsave.as<uint16_t>(), g->dsave.as<uint16_t>(), slot,
+                                          gc->lr, g->step, true, ctx->stream));
+    }
+    // Keep the newest unused samples (at most k batches' worth, so the move
+    // never overlaps) at the front for the next pass.
+    size_t rem = pending - size_t(k) * batch;
+    if (k > 0) {
+        rem = std::min(rem, size_t(k) * batch);
+        if (rem)
+            HIP_TRY(hipMemcpyAsync(smpBuf, smpBuf + (pending - rem),
+                                   rem * sizeof(crh::GuideSample), hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+    }
+    film->gPending = rem;
+    return CRH_OK;
+}
+
 // He-uniform init (fan-in 64) from SplitMix64; biases zero.
 void heInit(std::vector<float>& th, uint32_t seed) {
     uint64_t x = 0x9E3779B97F4A7C15ull * (uint64_t(seed) + 1);
@@ -1604,41 +1653,9 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
                                           film->sum.as<float>(), film->wsum.as<float>(),
                                           ctx->stream));
         if (gc->train) {
-            uint32_t* trainCounter = fused ? film->gCnt.as<uint32_t>() + 1
-                                           : film->wf.counters + crh::kCTrain;
-            if (fused)
-                HIP_TRY(crh::launchGuidedRecordsToSamples(
-                    film->gRec.as<float4>(), film->gCnt.as<uint32_t>(), recCap,
-                    film->L.as<float4>(), smpOut, smpRoom, trainCounter, gc->target,
-                    ctx->stream));
-            uint32_t produced = 0;
-            HIP_TRY(hipMemcpyAsync(&produced, trainCounter, 4, hipMemcpyDeviceToHost,
-                                   ctx->stream));
-            HIP_TRY(hipStreamSynchronize(ctx->stream));
-            produced = uint32_t(std::min<size_t>(produced, film->gCap - film->gPending));
-            lastSamples = produced;
-            g->lastSamples = produced;
-            const size_t pending = film->gPending + produced;
-            uint32_t k = 0;
-            for (; k < gc->train_steps * active && size_t(k + 1) * batch <= pending; ++k) {
-                g->step++;
-                float* slot = lossSlot(g);
-                HIP_TRY(hipMemsetAsync(slot, 0, 4, ctx->stream));
-                HIP_TRY(crh::launchGuideTrainStep(g->view, smpBuf + size_t(k) * batch, batch,
-                                                  g->hsave.as<uint16_t>(), g->dsave.as<uint16_t>(),
-                                                  slot, gc->lr, g->step, true, ctx->stream));
-            }
-            // Keep the newest unused samples (at most k batches' worth, so the
-            // move never overlaps) at the front for the next pass.
-            size_t rem = pending - size_t(k) * batch;
-            if (k > 0) {
-                rem = std::min(rem, size_t(k) * batch);
-                if (rem)
-                    HIP_TRY(hipMemcpyAsync(smpBuf, smpBuf + (pending - rem),
-                                           rem * sizeof(crh::GuideSample),
-                                           hipMemcpyDeviceToDevice, ctx->stream));
-            }
-            film->gPending = rem;
+            const int rc = guidedTrainPass(g, film, gc, active, fused, recCap,
+                                           film->L.as<float4>(), smpOut, smpRoom, lastSamples);
+            if (rc != CRH_OK) return rc;
         }
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1657,6 +1674,143 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
                           hipMemcpyDeviceToHost));
         stats->kernel_ms = ms;
         stats->main_kernel_ms = mainMs;
+        stats->total_ms = nowMs() - t0;
+        stats->rays = c[0];
+        stats->node_visits = lastSamples;
+        stats->launches = (spp + slots - 1) / slots;
+    }
+    return CRH_OK;
+    GUARD_END
+}
+
+int crh_render_guided_views(crh_scene* s, const crh_camera* cams, crh_film* const* films,
+                            uint32_t nviews, const crh_integrator_cfg* cfg, crh_guide* g,
+                            const crh_guided_cfg* gc, uint32_t first_sample, uint32_t spp,
+                            crh_stats* stats) {
+    ARG_CHECK(s && cams && films && cfg && g && gc, "NULL argument");
+    ARG_CHECK(nviews >= 1 && nviews <= 4096, "view count must be in [1, 4096]");
+    ARG_CHECK(g->ctx == s->ctx, "objects belong to different contexts");
+    ARG_CHECK(gc->alpha >= 0.f && gc->alpha <= 1.f, "alpha must be in [0, 1]");
+    ARG_CHECK(!cfg->spectral, "guided rendering transports RGB");
+    ARG_CHECK(gc->materials <= 3u, "materials must be a mask of bits 0 and 1");
+    ARG_CHECK(gc->target <= 1u, "target must be 0 (BSDF x radiance) or 1 (radiance)");
+    ARG_CHECK(cfg->max_depth <= 1024, "max_depth too large");
+    ARG_CHECK(!gc->train || (gc->train_batch >= 64 && gc->train_batch <= (1u << 22)),
+              "train_batch must be in [64, 2^22]");
+    for (uint32_t i = 0; i < nviews; ++i) {
+        ARG_CHECK(films[i], "NULL film");
+        ARG_CHECK(films[i]->ctx == s->ctx, "film belongs to a different context");
+        for (uint32_t j = 0; j < i; ++j) ARG_CHECK(films[j] != films[i], "films must differ");
+    }
+    GUARD_BEGIN
+    crh_context* ctx = s->ctx;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    const crh::IntegratorView iv{cfg->max_depth, cfg->rr_threshold, cfg->rr_start_depth,
+                                 cfg->use_mis ? 1u : 0u};
+    double t0 = nowMs();
+    HIP_TRY(hipSetDevice(ctx->device));
+    const crh::Tuning tun = effective(ctx->tuning, s->view);
+    const uint32_t gmask = gc->materials ? gc->materials : 3u;
+    // The view table: cameras, 16 x 16 tile and pixel prefixes, and the
+    // films the gather updates.
+    std::vector<crh::CameraView> cv(nviews);
+    std::vector<uint32_t> pre(2 * (size_t(nviews) + 1), 0);
+    uint32_t* tilePre = pre.data();
+    uint32_t* pixPre = pre.data() + nviews + 1;
+    std::vector<crh::BatchScene> tab(nviews);
+    for (uint32_t i = 0; i < nviews; ++i) {
+        crh_film* f = films[i];
+        cv[i] = makeCameraView(&cams[i], f->w, f->h);
+        const uint64_t tiles = uint64_t((f->w + 15u) / 16u) * ((f->h + 15u) / 16u);
+        ARG_CHECK(uint64_t(tilePre[i]) + tiles < (uint64_t(1) << 23) &&
+                      uint64_t(pixPre[i]) + uint64_t(f->w) * f->h < (uint64_t(1) << 31),
+                  "views too large for one launch");
+        tilePre[i + 1] = tilePre[i] + uint32_t(tiles);
+        pixPre[i + 1] = pixPre[i] + f->w * f->h;
+        std::memset(&tab[i], 0, sizeof(tab[i]));
+        tab[i].cam = cv[i];
+        tab[i].sum = f->sum.as<float>();
+        tab[i].wsum = f->wsum.as<float>();
+    }
+    const uint32_t tiles = tilePre[nviews], totalPix = pixPre[nviews];
+    uint64_t k = std::max<uint64_t>(1, tun.guidedPool / totalPix);
+    k = std::min<uint64_t>(k, std::max<uint32_t>(spp, 1u));
+    k = std::min<uint64_t>(k, std::max<uint64_t>(1, ((uint64_t(1) << 23) - 1) / tiles));
+    k = std::min<uint64_t>(k, std::max<uint64_t>(1, (uint64_t(1) << 31) / totalPix / 4));
+    const uint32_t slots = uint32_t(k);
+    crh_film* f0 = films[0];
+    const size_t np = size_t(totalPix) * slots;
+    if (f0->vL.bytes < np * sizeof(float4)) HIP_TRY(f0->vL.alloc(np * sizeof(float4)));
+    if (f0->vP.bytes < np * sizeof(float2)) HIP_TRY(f0->vP.alloc(np * sizeof(float2)));
+    const size_t camB = nviews * sizeof(crh::CameraView), preB = pre.size() * sizeof(uint32_t),
+                 tabB = nviews * sizeof(crh::BatchScene);
+    const size_t preOff = (camB + 255) & ~size_t(255), tabOff = (preOff + preB + 255) & ~size_t(255);
+    if (f0->vTab.bytes < tabOff + tabB) HIP_TRY(f0->vTab.alloc(tabOff + tabB));
+    char* vt = f0->vTab.as<char>();
+    HIP_TRY(hipMemcpyAsync(vt, cv.data(), camB, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(vt + preOff, pre.data(), preB, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(vt + tabOff, tab.data(), tabB, hipMemcpyHostToDevice, ctx->stream));
+    const auto* dTilePre = reinterpret_cast<const uint32_t*>(vt + preOff);
+    const auto* dPixPre = dTilePre + nviews + 1;
+    const auto* dTab = reinterpret_cast<const crh::BatchScene*>(vt + tabOff);
+    const crh::GuidedViews views{reinterpret_cast<const crh::CameraView*>(vt), dTilePre, dPixPre,
+                                 nviews, tiles, totalPix};
+    const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(tiles) * slots, tun.guidedGrid));
+    const uint32_t batch = gc->train_batch;
+    uint32_t recCap = 0;
+    if (gc->train)
+        recCap = uint32_t(std::min<uint64_t>(
+            {uint64_t(4) * np, std::max<uint64_t>(uint64_t(2) * gc->train_steps * slots * batch,
+                                                   1u << 20),
+             0xFFFFFFF0ull}));
+    HIP_TRY(ensureGuidedFused(f0, crh::guidedScratchFloats(grid), recCap));
+    HIP_TRY(hipMemsetAsync(f0->work.p, 0, f0->work.bytes, ctx->stream));
+    const size_t padB = (size_t(batch) + 63) & ~size_t(63);
+    if (gc->train && padB > g->capSamples) {
+        HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * padB * 64 * 2));
+        HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * padB * 64 * 2));
+        g->capSamples = padB;
+    }
+    const size_t needCap = size_t(recCap) + batch;
+    if (gc->train && f0->gCap < needCap) {
+        HIP_TRY(f0->gSamples.alloc(needCap * sizeof(crh::GuideSample)));
+        f0->gCap = needCap;
+        f0->gPending = 0;
+    }
+    uint32_t lastSamples = 0;
+    double mainMs = 0.0;
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    for (uint32_t i = 0; i < spp; i += slots) {
+        const uint32_t active = std::min(slots, spp - i);
+        crh::GuideSample* smpOut =
+            gc->train ? f0->gSamples.as<crh::GuideSample>() + f0->gPending : nullptr;
+        const uint32_t smpRoom = uint32_t(f0->gCap - f0->gPending);
+        const crh::GuidedPassArgs ga{&g->view, gc->alpha, gmask, gc->max_depth,
+                                     f0->gScratch.as<float>(),
+                                     gc->train ? f0->gRec.as<float4>() : nullptr,
+                                     f0->gCnt.as<uint32_t>(), recCap};
+        HIP_TRY(crh::launchGuidedPass(s->view, crh::CameraView{}, iv, first_sample + i, active,
+                                      f0->vL.as<float4>(), f0->vP.as<float2>(),
+                                      f0->work.as<unsigned long long>(), tun, ctx->counter, ga,
+                                      ctx->stream, &views));
+        HIP_TRY(crh::launchBatchGatherPass(dTab, dPixPre, nviews, totalPix, active,
+                                           f0->vL.as<float4>(), f0->vP.as<float2>(),
+                                           ctx->stream));
+        if (gc->train) {
+            const int rc = guidedTrainPass(g, f0, gc, active, true, recCap, f0->vL.as<float4>(),
+                                           smpOut, smpRoom, lastSamples);
+            if (rc != CRH_OK) return rc;
+        }
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    if (stats) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        unsigned long long c[3] = {0, 0, 0};
+        HIP_TRY(hipMemcpy(c, f0->work.p, sizeof(c), hipMemcpyDeviceToHost));
+        stats->kernel_ms = ms;
+        stats->main_kernel_ms = mainMs > 0.0 ? mainMs : ms;
         stats->total_ms = nowMs() - t0;
         stats->rays = c[0];
         stats->node_visits = lastSamples;

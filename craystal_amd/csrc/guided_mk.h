@@ -28,13 +28,27 @@ struct GuidedPassArgs {
     uint32_t recCap;
 };
 
+// Several camera views of one scene in one launch: cams, tilePrefix (16 x 16
+// tiles of the views before each, then the total) and pixPrefix (the same
+// for pixels) are device arrays; tiles = tilePrefix[n], totalPix =
+// pixPrefix[n].
+struct GuidedViews {
+    const CameraView* cams;
+    const uint32_t* tilePrefix;
+    const uint32_t* pixPrefix;
+    uint32_t n, tiles, totalPix;
+};
+
 // Sample indices si .. si + active - 1 into slots of L and P (as
 // launchRenderPass) with guided continuations; grid = min(tiles * active,
 // t.renderGrid) persistent blocks of 256 threads; next: a device counter.
+// views: several views (cam unused), sample slot k of view j's pixel i at
+// k * totalPix + pixPrefix[j] + i of L and P.
 hipError_t launchGuidedPass(const SceneView& s, const CameraView& cam, const IntegratorView& cfg,
                             uint32_t si, uint32_t active, float4* L, float2* P,
                             unsigned long long* counters, const Tuning& t, uint32_t* next,
-                            const GuidedPassArgs& g, hipStream_t st);
+                            const GuidedPassArgs& g, hipStream_t st,
+                            const GuidedViews* views = nullptr);
 
 // One launch of the fused kernel; launchGuidedSM<SM> (defined once per
 // stack mode, guided_mk.hip built with CRH_GMK_PART = SM) picks the register
@@ -53,6 +67,8 @@ struct GuidedLaunch {
     size_t lds;
     uint32_t waves;
     hipStream_t st;
+    GuidedViews views;
+    bool multi;
 };
 template <int SM> hipError_t launchGuidedSM(const GuidedLaunch& a);
 template <> hipError_t launchGuidedSM<0>(const GuidedLaunch&);

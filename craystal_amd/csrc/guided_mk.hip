@@ -27,6 +27,7 @@
 #include "path.h"
 #include "wave.h"
 #include "wavefront.h"
+#include "batch_device.h"
 
 #ifndef CRH_GMK_PART
 #error "compile with -DCRH_GMK_PART=<0|1|2> (kernels of one stack mode) or 3 (the rest)"
@@ -148,14 +149,18 @@ __device__ __forceinline__ void guideWave(const GuideView& gv, bool guided, F3 p
     waveSync();  // the outputs, written by other lanes, are read next
 }
 
-template <bool MIS, int SM, int W>
+// MULTI: several camera views of the scene in one launch (GuidedViews):
+// path p is tile (p % perSlot) >> 8 of all views' tiles, view after view,
+// of sample slot p / perSlot, and writes slot * totalPix + pixPrefix[view] +
+// its pixel of L and P; otherwise one camera and slot * npix + pixel.
+template <bool MIS, int SM, int W, bool MULTI>
 __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W, 8))) void
 kRenderGuided(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint32_t tilesX,
               uint32_t tiles, uint32_t total, uint32_t* __restrict__ next, float4* __restrict__ L,
               float2* __restrict__ P, unsigned long long* __restrict__ counters, GuideView gv,
               float alpha, uint32_t gmask, uint32_t guideDepth, float* __restrict__ scratch,
               uint32_t tileOff, float4* __restrict__ rec, uint32_t* __restrict__ recCounter,
-              uint32_t recCap) {
+              uint32_t recCap, GuidedViews vs) {
     extern __shared__ uint2 lds[];
     auto stk = makeStack<SM>(lds);
     PathCtx<false, typename StackOf<SM>::T> c{s, stk, 0, {0, 0}};
@@ -186,18 +191,28 @@ kRenderGuided(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint
             const uint32_t p = disp.take(next, total, chunk, !live);
             if (p < total) {
                 const uint32_t slot = p / perSlot, q = p - slot * perSlot;
-                uint32_t x, y;
-                tileCoordOf(q >> 8, q & 255u, tilesX, x, y);
-                if (x < cam.width && y < cam.height) {
+                uint32_t x, y, view = 0u;
+                const CameraView* cv = &cam;
+                if constexpr (MULTI) {
+                    view = findScene(vs.tilePrefix, vs.n, q >> 8);
+                    cv = &vs.cams[view];
+                    tileCoordOf((q >> 8) - vs.tilePrefix[view], q & 255u,
+                                (cv->width + 15u) / 16u, x, y);
+                } else {
+                    tileCoordOf(q >> 8, q & 255u, tilesX, x, y);
+                }
+                if (x < cv->width && y < cv->height) {
                     smp = Sampler(x, y, si + slot);
                     F2 j = smp.next2D();
                     pix = F2{float(x) + j.x, float(y) + j.y};
-                    ray = cameraRay(cam, pix);
+                    ray = cameraRay(*cv, pix);
                     rad = sp(0.f);
                     beta = sp(1.f);
                     depth = 0;
                     recId = kNoRecord;
-                    out = slot * npix + size_t(y) * cam.width + x;
+                    out = MULTI ? size_t(slot) * vs.totalPix + vs.pixPrefix[view] +
+                                      size_t(y) * cv->width + x
+                                : slot * npix + size_t(y) * cam.width + x;
                     live = true;
                     if (cfg.maxDepth == 0u) finish();
                 }
@@ -364,9 +379,9 @@ size_t guidedLdsBytes(uint32_t stackMode, uint32_t stackCap) {
 hipError_t launchGuidedPass(const SceneView& s, const CameraView& cam, const IntegratorView& cfg,
                             uint32_t si, uint32_t active, float4* L, float2* P,
                             unsigned long long* counters, const Tuning& t, uint32_t* next,
-                            const GuidedPassArgs& g, hipStream_t st) {
+                            const GuidedPassArgs& g, hipStream_t st, const GuidedViews* views) {
     const uint32_t tilesX = (cam.width + 15u) / 16u, tilesY = (cam.height + 15u) / 16u;
-    const uint32_t tiles = tilesX * tilesY;
+    const uint32_t tiles = views ? views->tiles : tilesX * tilesY;
     if (uint64_t(tiles) * active >= (uint64_t(1) << 23)) return hipErrorInvalidValue;
     if (!g.guide || !g.scratch || !next) return hipErrorInvalidValue;
     const uint32_t total = tiles * active * 256u;
@@ -376,7 +391,8 @@ hipError_t launchGuidedPass(const SceneView& s, const CameraView& cam, const Int
     const size_t lds = guidedLdsBytes(t.guidedPriv, s.stackCap);
     hipLaunchKernelGGL(kGuidedReset, dim3(1), dim3(64), 0, st, next, g.recCounter);
     const GuidedLaunch gl{s, cam, cfg, si, tilesX, tiles, total, next, L, P, counters, &g,
-                          grid, tileOff, lds, t.guidedWaves, st};
+                          grid, tileOff, lds, t.guidedWaves, st,
+                          views ? *views : GuidedViews{}, views != nullptr};
     hipError_t e = hipSuccess;
     withStack(t.guidedPriv, [&](auto sm) { e = launchGuidedSM<decltype(sm)::value>(gl); });
     if (e != hipSuccess) return e;
@@ -404,16 +420,16 @@ hipError_t launchGuidedSM<CRH_GMK_PART>(const GuidedLaunch& a) {
     const GuidedPassArgs& g = *a.g;
     withWaves(a.waves, [&](auto wv) {
         constexpr int Wv = decltype(wv)::value;
-        if (a.cfg.useMIS)
-            hipLaunchKernelGGL((kRenderGuided<true, SM, Wv>), dim3(a.grid), dim3(kRenderBlock),
-                               a.lds, a.st, a.s, a.cam, a.cfg, a.si, a.tilesX, a.tiles, a.total,
-                               a.next, a.L, a.P, a.counters, *g.guide, g.alpha, g.mask,
-                               g.maxDepth, g.scratch, a.tileOff, g.rec, g.recCounter, g.recCap);
-        else
-            hipLaunchKernelGGL((kRenderGuided<false, SM, Wv>), dim3(a.grid), dim3(kRenderBlock),
-                               a.lds, a.st, a.s, a.cam, a.cfg, a.si, a.tilesX, a.tiles, a.total,
-                               a.next, a.L, a.P, a.counters, *g.guide, g.alpha, g.mask,
-                               g.maxDepth, g.scratch, a.tileOff, g.rec, g.recCounter, g.recCap);
+#define CRH_RG(MIS, MULTI)                                                                     \
+    hipLaunchKernelGGL((kRenderGuided<MIS, SM, Wv, MULTI>), dim3(a.grid), dim3(kRenderBlock),  \
+                       a.lds, a.st, a.s, a.cam, a.cfg, a.si, a.tilesX, a.tiles, a.total, a.next, \
+                       a.L, a.P, a.counters, *g.guide, g.alpha, g.mask, g.maxDepth, g.scratch,   \
+                       a.tileOff, g.rec, g.recCounter, g.recCap, a.views)
+        if (a.cfg.useMIS && a.multi) CRH_RG(true, true);
+        else if (a.multi) CRH_RG(false, true);
+        else if (a.cfg.useMIS) CRH_RG(true, false);
+        else CRH_RG(false, false);
+#undef CRH_RG
     });
     return hipGetLastError();
 }

@@ -105,10 +105,7 @@ class GuidedIntegrator:
         self.max_depth, self.materials, self.target = max_depth, materials, target
         self.last_stats: dict = {}
 
-    def dispatch(self, scene, spp: int, first_sample: int = 0, train: bool = False,
-                 readback: bool = True) -> dict:
-        cam = scene.getCamera()
-        sensor = cam.getSensor()
+    def _cfg(self, train: bool):
         g = L.crh_guided_cfg()
         g.alpha = self.alpha
         g.train = 1 if train else 0
@@ -118,6 +115,13 @@ class GuidedIntegrator:
         g.max_depth = self.max_depth
         g.materials = self.materials
         g.target = self.target
+        return g
+
+    def dispatch(self, scene, spp: int, first_sample: int = 0, train: bool = False,
+                 readback: bool = True) -> dict:
+        cam = scene.getCamera()
+        sensor = cam.getSensor()
+        g = self._cfg(train)
         st = L.crh_stats()
         cfg = self.configs.to_c()
         L.call("crh_render_guided", scene.handle, C.byref(cam.data), C.byref(cfg),
@@ -126,6 +130,29 @@ class GuidedIntegrator:
             sensor.readbackDeviceData()
         self.last_stats = {"kernel_ms": st.kernel_ms, "main_kernel_ms": st.main_kernel_ms,
                            "total_ms": st.total_ms, "rays": st.rays,
+                           "train_samples": st.node_visits}
+        return self.last_stats
+
+
+    def dispatch_views(self, scene, cameras, spp: int, first_sample: int = 0,
+                       train: bool = False, readback: bool = True) -> dict:
+        """Guided rendering of several camera views of the scene (each with
+        its sensor), all of them in each launch, with this guide."""
+        sensors = [c.getSensor() for c in cameras]
+        if not cameras or any(s is None for s in sensors):
+            raise ValueError("every camera needs a sensor")
+        cams = (L.crh_camera * len(cameras))(*[c.data for c in cameras])
+        films = (C.c_void_p * len(cameras))(*[s.handle.value for s in sensors])
+        g = self._cfg(train)
+        st = L.crh_stats()
+        cfg = self.configs.to_c()
+        L.call("crh_render_guided_views", scene.handle, cams, films, len(cameras), C.byref(cfg),
+               self.guide.handle, C.byref(g), first_sample, spp, C.byref(st))
+        for s in sensors:
+            s.setSPP(spp)
+            if readback:
+                s.readbackDeviceData()
+        self.last_stats = {"kernel_ms": st.kernel_ms, "total_ms": st.total_ms, "rays": st.rays,
                            "train_samples": st.node_visits}
         return self.last_stats
 

@@ -444,11 +444,14 @@ int crh_guide_get_info(const crh_guide* g, crh_guide_info* info);
 int crh_guide_get_losses(const crh_guide* g, float* losses, uint32_t n);
 int crh_guide_destroy(crh_guide* g);
 
-/* Guided rendering: the wavefront integrator with one-sample MIS between
- * BSDF sampling (fraction alpha) and the guide's mixture at every
- * Lambertian / Principled vertex; the estimator's expectation equals the
- * plain integrator's.  Passes run in groups of samples per pixel kept in
- * flight together (crh_tuning.wf_pool).  With train != 0, the vertices with
+/* Guided rendering: the path tracer with one-sample MIS between BSDF
+ * sampling (fraction alpha) and the guide's mixture at every Lambertian /
+ * Principled vertex; the estimator's expectation equals the plain
+ * integrator's.  It runs on the fused guided megakernel (the guide evaluated
+ * on MFMA inside the path kernel; crh_tuning.guided_*) or, with
+ * guided_fused = 2, on the wavefront (crh_tuning.wf_*); both give the same
+ * film for the same guide.  Passes run in groups of samples per pixel kept
+ * in flight together (guided_pool or wf_pool).  With train != 0, the vertices with
  * non-zero radiance become training samples; they accumulate on the film over
  * groups (and calls) and each group of k passes is followed by up to
  * k * train_steps Adam steps, one per full batch of train_batch samples. */
@@ -470,6 +473,17 @@ int crh_render_guided(crh_scene* scene, const crh_camera* cam,
                       const crh_integrator_cfg* cfg, crh_guide* guide,
                       const crh_guided_cfg* gcfg, uint32_t first_sample,
                       uint32_t spp, crh_film* film, crh_stats* stats);
+/* Guided rendering of nviews camera views of one scene with one guide, all
+ * views in each launch of the fused guided megakernel (views[i] into
+ * films[i], spp samples each); training, when on, uses every view's
+ * vertices.  Each film is bit-identical to crh_render_guided of that view
+ * alone with the same guide when the guide does not train.  No counterpart
+ * in the reference, which renders one camera per dispatch
+ * (Core/Integrator.cu:233-262). */
+int crh_render_guided_views(crh_scene* scene, const crh_camera* cams, crh_film* const* films,
+                            uint32_t nviews, const crh_integrator_cfg* cfg, crh_guide* guide,
+                            const crh_guided_cfg* gcfg, uint32_t first_sample, uint32_t spp,
+                            crh_stats* stats);
 
 /* ---- spectra (the reference's integrator transports RGB; its 95-bin
  * BroadSpectrum and CIE table, Core/Spectrum.h:278-324, are the analogue) -- */
