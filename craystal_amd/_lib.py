@@ -124,7 +124,7 @@ class crh_guided_cfg(C.Structure):
 
 
 class crh_guide_info(C.Structure):
-    _fields_ = [("steps", u32), ("last_samples", u32), ("last_loss", f32), ("reserved", u32)]
+    _fields_ = [("steps", u32), ("last_samples", u32), ("last_loss", f32), ("width", u32)]
 
 
 class crh_stats(C.Structure):
@@ -183,6 +183,7 @@ SIGNATURES = {
     "crh_render": [_P, C.POINTER(crh_camera), C.POINTER(crh_integrator_cfg), u32, u32, _P,
                    C.POINTER(crh_stats)],
     "crh_guide_create": [_P, _P, _P, u32, C.POINTER(_P)],
+    "crh_guide_create_width": [_P, _P, _P, u32, u32, C.POINTER(_P)],
     "crh_guide_set_params": [_P, _P, u32],
     "crh_guide_get_params": [_P, _P, u32],
     "crh_guide_eval": [_P, _P, _P, _P, u32, _P, _P],
@@ -204,6 +205,7 @@ SIGNATURES = {
                                 C.POINTER(crh_guided_cfg), u32, u32, C.POINTER(crh_stats)],
 }
 GUIDE_PARAMS = 19760
+GUIDE_PARAMS_128 = 64048
 GUIDE_OUTPUTS = 48
 
 _lib = None

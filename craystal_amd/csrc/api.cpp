@@ -1301,6 +1301,8 @@ struct crh_guide {
     uint32_t step = 0;
     uint32_t lastSamples = 0;
     size_t capSamples = 0;
+    uint32_t width = 64;  // hidden width of the network
+    uint32_t params() const { return uint32_t(crh::gParamsOf(int(width))); }
 };
 
 namespace {
@@ -1354,8 +1356,11 @@ int guidedTrainPass(crh_guide* g, crh_film* film, const crh_guided_cfg* gc, uint
     return CRH_OK;
 }
 
-// He-uniform init (fan-in 64) from SplitMix64; biases zero.
-void heInit(std::vector<float>& th, uint32_t seed) {
+// He-uniform init (limit sqrt(6 / fan-in), fan-in = the layer's inputs) from
+// SplitMix64, layer by layer in the parameter order; biases zero.
+template <int WD>
+void heInitW(std::vector<float>& th, uint32_t seed) {
+    using G = crh::GNet<WD>;
     uint64_t x = 0x9E3779B97F4A7C15ull * (uint64_t(seed) + 1);
     auto next = [&]() {
         x += 0x9E3779B97F4A7C15ull;
@@ -1365,17 +1370,21 @@ void heInit(std::vector<float>& th, uint32_t seed) {
         z ^= z >> 31;
         return double(z >> 11) * (1.0 / 9007199254740992.0);
     };
-    th.assign(crh::kGParams, 0.f);
-    const double lim = std::sqrt(6.0 / 64.0);
+    th.assign(crh::kGParamsOf<WD>, 0.f);
     for (int l = 0; l < crh::kGLayers; ++l) {
-        int w0 = crh::gWOff(l), N = crh::gLayerN(l);
-        for (int i = 0; i < 64 * N; ++i) th[w0 + i] = float((2.0 * next() - 1.0) * lim);
+        const double lim = std::sqrt(6.0 / double(G::K(l)));
+        const int w0 = G::WOff(l), n = G::K(l) * G::N(l);
+        for (int i = 0; i < n; ++i) th[w0 + i] = float((2.0 * next() - 1.0) * lim);
     }
+}
+void heInit(std::vector<float>& th, uint32_t seed, uint32_t width) {
+    if (width == 128) heInitW<128>(th, seed);
+    else heInitW<64>(th, seed);
 }
 
 int guideUpload(crh_guide* g, const float* theta) {
     crh_context* ctx = g->ctx;
-    HIP_TRY(hipMemcpyAsync(g->theta.p, theta, crh::kGParams * 4, hipMemcpyHostToDevice,
+    HIP_TRY(hipMemcpyAsync(g->theta.p, theta, size_t(g->params()) * 4, hipMemcpyHostToDevice,
                            ctx->stream));
     HIP_TRY(hipMemsetAsync(g->m.p, 0, g->m.bytes, ctx->stream));
     HIP_TRY(hipMemsetAsync(g->v.p, 0, g->v.bytes, ctx->stream));
@@ -1392,18 +1401,26 @@ extern "C" {
 
 int crh_guide_create(crh_context* ctx, const float* bmin, const float* bmax, uint32_t seed,
                      crh_guide** out) {
+    return crh_guide_create_width(ctx, bmin, bmax, seed, 64, out);
+}
+
+int crh_guide_create_width(crh_context* ctx, const float* bmin, const float* bmax,
+                           uint32_t seed, uint32_t width, crh_guide** out) {
     ARG_CHECK(ctx && bmin && bmax && out, "NULL argument");
+    ARG_CHECK(width == 64 || width == 128, "guide width must be 64 or 128");
     *out = nullptr;
     GUARD_BEGIN
     HIP_TRY(hipSetDevice(ctx->device));
     auto* g = new crh_guide();
     g->ctx.set(ctx);
-    hipError_t e = g->theta.alloc(crh::kGParams * 4);
-    if (e == hipSuccess) e = g->m.alloc(crh::kGParams * 4);
-    if (e == hipSuccess) e = g->v.alloc(crh::kGParams * 4);
-    if (e == hipSuccess) e = g->grad.alloc(crh::kGParams * 4);
-    if (e == hipSuccess) e = g->wq.alloc(crh::kGQElems * 2);
-    if (e == hipSuccess) e = g->wqT.alloc(crh::kGQElems * 2);
+    g->width = width;
+    const size_t np = g->params(), nq = size_t(crh::gQElemsOf(int(width)));
+    hipError_t e = g->theta.alloc(np * 4);
+    if (e == hipSuccess) e = g->m.alloc(np * 4);
+    if (e == hipSuccess) e = g->v.alloc(np * 4);
+    if (e == hipSuccess) e = g->grad.alloc(np * 4);
+    if (e == hipSuccess) e = g->wq.alloc(nq * 2);
+    if (e == hipSuccess) e = g->wqT.alloc(nq * 2);
     if (e == hipSuccess) e = g->loss.alloc((kLossHist + 1) * sizeof(float));
     if (e != hipSuccess) {
         delete g;
@@ -1416,12 +1433,13 @@ int crh_guide_create(crh_context* ctx, const float* bmin, const float* bmax, uin
     v.grad = g->grad.as<float>();
     v.wq = g->wq.as<uint16_t>();
     v.wqT = g->wqT.as<uint16_t>();
+    v.width = width;
     for (int k = 0; k < 3; ++k) {
         v.bmin[k] = bmin[k];
         v.bext[k] = bmax[k] - bmin[k];
     }
     std::vector<float> th;
-    heInit(th, seed);
+    heInit(th, seed, width);
     int rc = guideUpload(g, th.data());
     if (rc != CRH_OK) {
         delete g;
@@ -1434,16 +1452,16 @@ int crh_guide_create(crh_context* ctx, const float* bmin, const float* bmax, uin
 
 int crh_guide_set_params(crh_guide* g, const float* theta, uint32_t n) {
     ARG_CHECK(g && theta, "NULL argument");
-    ARG_CHECK(n == crh::kGParams, "parameter count must be CRH_GUIDE_PARAMS");
+    ARG_CHECK(n == g->params(), "parameter count must be the guide's (CRH_GUIDE_PARAMS at width 64)");
     HIP_TRY(hipSetDevice(g->ctx->device));
     return guideUpload(g, theta);
 }
 
 int crh_guide_get_params(const crh_guide* g, float* theta, uint32_t n) {
     ARG_CHECK(g && theta, "NULL argument");
-    ARG_CHECK(n == crh::kGParams, "parameter count must be CRH_GUIDE_PARAMS");
+    ARG_CHECK(n == g->params(), "parameter count must be the guide's (CRH_GUIDE_PARAMS at width 64)");
     HIP_TRY(hipSetDevice(g->ctx->device));
-    HIP_TRY(hipMemcpy(theta, g->theta.p, crh::kGParams * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(theta, g->theta.p, size_t(n) * 4, hipMemcpyDeviceToHost));
     return CRH_OK;
 }
 
@@ -1490,8 +1508,8 @@ int crh_guide_train(crh_guide* g, const float* samples, uint32_t n, float lr, in
     const size_t padN = (size_t(n) + 63) & ~size_t(63);
     if (padN > g->capSamples || padN * sizeof(crh::GuideSample) > g->samples.bytes) {
         HIP_TRY(g->samples.alloc(padN * sizeof(crh::GuideSample)));
-        HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * padN * 64 * 2));
-        HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * padN * 64 * 2));
+        HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * padN * g->width * 2));
+        HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * padN * g->width * 2));
         g->capSamples = padN;
     }
     HIP_TRY(hipMemcpyAsync(g->samples.p, samples, size_t(n) * sizeof(crh::GuideSample),
@@ -1505,7 +1523,7 @@ int crh_guide_train(crh_guide* g, const float* samples, uint32_t n, float lr, in
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (loss) HIP_TRY(hipMemcpy(loss, slot, 4, hipMemcpyDeviceToHost));
     if (!apply_adam) {
-        if (grad) HIP_TRY(hipMemcpy(grad, g->grad.p, crh::kGParams * 4, hipMemcpyDeviceToHost));
+        if (grad) HIP_TRY(hipMemcpy(grad, g->grad.p, size_t(g->params()) * 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemset(g->grad.p, 0, g->grad.bytes));
     }
     return CRH_OK;
@@ -1518,6 +1536,7 @@ int crh_guide_get_info(const crh_guide* g, crh_guide_info* info) {
     std::memset(info, 0, sizeof(*info));
     info->steps = g->step;
     info->last_samples = g->lastSamples;
+    info->width = g->width;
     if (g->step) {
         HIP_TRY(hipSetDevice(g->ctx->device));
         HIP_TRY(hipStreamSynchronize(g->ctx->stream));
@@ -1603,8 +1622,8 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
     }
     const size_t padB = (size_t(batch) + 63) & ~size_t(63);
     if (gc->train && padB > g->capSamples) {
-        HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * padB * 64 * 2));
-        HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * padB * 64 * 2));
+        HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * padB * g->width * 2));
+        HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * padB * g->width * 2));
         g->capSamples = padB;
     }
     // Sample buffer: one pass group's records on top of a batch still filling.
@@ -1767,8 +1786,8 @@ int crh_render_guided_views(crh_scene* s, const crh_camera* cams, crh_film* cons
     HIP_TRY(hipMemsetAsync(f0->work.p, 0, f0->work.bytes, ctx->stream));
     const size_t padB = (size_t(batch) + 63) & ~size_t(63);
     if (gc->train && padB > g->capSamples) {
-        HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * padB * 64 * 2));
-        HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * padB * 64 * 2));
+        HIP_TRY(g->hsave.alloc(size_t(crh::kGLayers) * padB * g->width * 2));
+        HIP_TRY(g->dsave.alloc(size_t(crh::kGLayers) * padB * g->width * 2));
         g->capSamples = padB;
     }
     const size_t needCap = size_t(recCap) + batch;

@@ -13,8 +13,8 @@ namespace crh {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kRow = 72;           // LDS row stride of an activation tile
-constexpr int kTile = 16 * kRow;   // elements per activation tile
+constexpr int kRow = GNet<64>::kRow;    // LDS row stride of a 64-wide network's tile
+constexpr int kTile = GNet<64>::kTile;  // elements per activation tile
 
 __device__ __forceinline__ __bf16 tobf(float v) { return static_cast<__bf16>(v); }
 __device__ __forceinline__ float frombf(__bf16 v) { return static_cast<float>(v); }
@@ -24,7 +24,8 @@ __device__ __forceinline__ float softplusf(float x) {
 }
 
 // Input features of sample m of the tile into row m (lanes cover 16
-// features each).  valid = false writes a zero row.
+// features each; rows ROW elements apart).  valid = false writes a zero row.
+template <int ROW = kRow>
 __device__ __forceinline__ void featureRow(__bf16* H, int m, int chunk, bool valid, const float* pos,
                            const float* nrm, const float* wo, const float* bmin,
                            const float* bext) {
@@ -49,23 +50,27 @@ __device__ __forceinline__ void featureRow(__bf16* H, int m, int chunk, bool val
                 v = pick(wo, f - 35);
             }
         }
-        H[m * kRow + f] = tobf(v);
+        H[m * ROW + f] = tobf(v);
     }
 }
 
-// One layer: out = act(in * W_l + b_l) for the wave's 16-row tile.
-// For the last layer the fp32 result goes to outF ([16][48]).
+// One layer of the width-WD network: out = act(in * W_l + b_l) for the
+// wave's 16-row tile (rows GNet<WD>::kRow apart), K(l) / 32 MFMA steps per
+// 16 output columns, in k order; wq: the forward operand copy (LDS or
+// global memory).  For the last layer the fp32 result goes to outF
+// ([16][48]).
+template <int WD>
 __device__ __forceinline__ void layerForward(int l, const __bf16* in, __bf16* out, float* outF,
-                             const __bf16* ldsW, const float* __restrict__ theta) {
+                                             const __bf16* wq, const float* __restrict__ theta) {
+    using G = GNet<WD>;
     const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
-    const int N = gLayerN(l), NT = N >> 4;
-    const float* bias = theta + gBOff(l);
+    const int N = G::N(l), NT = N >> 4, KS = G::K(l) >> 5;
+    const float* bias = theta + G::BOff(l);
     for (int t = 0; t < NT; ++t) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            bf16x8 a = *reinterpret_cast<const bf16x8*>(in + r16 * kRow + 32 * s + 8 * g);
-            bf16x8 b = *reinterpret_cast<const bf16x8*>(ldsW + gQOff(l) +
+        for (int s = 0; s < KS; ++s) {
+            bf16x8 a = *reinterpret_cast<const bf16x8*>(in + r16 * G::kRow + 32 * s + 8 * g);
+            bf16x8 b = *reinterpret_cast<const bf16x8*>(wq + G::QOff(l) +
                                                          ((s * NT + t) * 64 + lane) * 8);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
         }
@@ -76,7 +81,7 @@ __device__ __forceinline__ void layerForward(int l, const __bf16* in, __bf16* ou
             const int row = 4 * g + i;
             float v = acc[i] + bc;
             if (l < kGLayers - 1)
-                out[row * kRow + col] = tobf(fmaxf(v, 0.f));
+                out[row * G::kRow + col] = tobf(fmaxf(v, 0.f));
             else
                 outF[row * kGOut + col] = v;
         }

@@ -14,8 +14,9 @@ namespace crh {
 // floats, for every wave of the persistent grid.
 constexpr uint32_t kGuideRaw = 40;
 size_t guidedScratchFloats(uint32_t grid);
-// Dynamic LDS of one block: the traversal stack, then a 3 KB tile per wave.
-size_t guidedLdsBytes(uint32_t stackMode, uint32_t stackCap);
+// Dynamic LDS of one block: the traversal stack, then one activation tile
+// per wave (16 rows of width + 8 bf16).
+size_t guidedLdsBytes(uint32_t stackMode, uint32_t stackCap, uint32_t width);
 
 struct GuidedPassArgs {
     const GuideView* guide;

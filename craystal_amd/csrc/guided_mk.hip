@@ -20,6 +20,8 @@
 // final radiance read from L at conversion.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "guide_device.h"
 #include "guide_mlp.h"
 #include "guided_mk.h"
@@ -37,7 +39,9 @@ namespace crh {
 
 namespace {
 
-constexpr uint32_t kTileBytes = kTile * 2;  // a 16 x 72 bf16 activation tile
+// A wave's 16-row bf16 activation tile of the width-WD network.
+template <int WD>
+constexpr uint32_t kTileBytesOf = GNet<WD>::kTile * 2;
 
 // Orders the LDS accesses of the lanes of a wave across this point (LDS
 // only: the wave's global loads stay in flight).
@@ -62,49 +66,69 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t bufferOf(const void* p, uint32
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(bytes), 0x00020000);
 }
 
-// Layer L of the MLP on the wave's 16-row tile H, in place.  The layer's
-// weights (in MFMA operand order) and biases are loaded in one batch before
-// the lane's A operands; hidden layers write bf16 ReLU outputs back into H
-// after every lane has read its operands, the output layer writes row r's
-// first kGuideRaw outputs to floats outOff / 4 + r * kGuideRaw .. of the
-// buffer `out`.  The arithmetic is layerForward's.
-template <int L>
+// Layer L of the width-WD MLP on the wave's 16-row tile H, in place:
+// hidden layers write bf16 ReLU outputs back into H after every lane has
+// read its A operands, the output layer writes row r's first kGuideRaw
+// outputs to floats outOff / 4 + r * kGuideRaw .. of the buffer `out`.  The
+// 64-wide layers load all their weights (in MFMA operand order) and biases
+// in one batch before the A operands; the 128-wide ones (4 x 8 operand
+// blocks) load each 16-column block's as they go.  The arithmetic, and its
+// order, is layerForward's.
+template <int WD, int L>
 __device__ __forceinline__ void layerInPlace(__bf16* H, __amdgpu_buffer_rsrc_t wq,
                                              __amdgpu_buffer_rsrc_t theta,
                                              __amdgpu_buffer_rsrc_t out, uint32_t outOff) {
-    constexpr int NT = gLayerN(L) >> 4;
+    using G = GNet<WD>;
+    constexpr int NT = G::N(L) >> 4, KS = G::K(L) >> 5, R = G::kRow;
+    constexpr bool kBatch = WD == 64;
     const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
-    bf16x8 b[2][NT];
-    float bc[NT];
+    auto ldB = [&](int s, int t) {
+        return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
+                                              wq, lane * 16, (G::QOff(L) + (s * NT + t) * 64 * 8) * 2, 0));
+    };
+    auto ldBias = [&](int t) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                             theta, r16 * 4, (G::BOff(L) + 16 * t) * 4, 0));
+    };
+    bf16x8 bAll[kBatch ? KS : 1][kBatch ? NT : 1];
+    float bcAll[kBatch ? NT : 1];
+    if constexpr (kBatch) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
+        for (int t = 0; t < NT; ++t) {
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
-            b[s][t] = __builtin_bit_cast(
-                bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
-                            wq, lane * 16, (gQOff(L) + (s * NT + t) * 64 * 8) * 2, 0));
-        bc[t] = __builtin_bit_cast(
-            float, __builtin_amdgcn_raw_buffer_load_b32(theta, r16 * 4,
-                                                        (gBOff(L) + 16 * t) * 4, 0));
+            for (int s = 0; s < KS; ++s) bAll[s][t] = ldB(s, t);
+            bcAll[t] = ldBias(t);
+        }
     }
-    bf16x8 a[2];
+    bf16x8 a[KS];
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
-        a[s] = *reinterpret_cast<const bf16x8*>(H + r16 * kRow + 32 * s + 8 * g);
+    for (int s = 0; s < KS; ++s)
+        a[s] = *reinterpret_cast<const bf16x8*>(H + r16 * R + 32 * s + 8 * g);
     if (L < kGLayers - 1) ldsSync();
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
+        bf16x8 b[KS];
+        float bc;
+        if constexpr (kBatch) {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) b[s] = bAll[s][t];
+            bc = bcAll[t];
+        } else {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) b[s] = ldB(s, t);
+            bc = ldBias(t);
+        }
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], b[s][t], acc, 0, 0, 0);
+        for (int s = 0; s < KS; ++s)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], b[s], acc, 0, 0, 0);
         const int col = 16 * t + r16;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int row = 4 * g + i;
-            const float v = acc[i] + bc[t];
+            const float v = acc[i] + bc;
             if (L < kGLayers - 1)
-                H[row * kRow + col] = tobf(fmaxf(v, 0.f));
+                H[row * R + col] = tobf(fmaxf(v, 0.f));
             else if (col < int(kGuideRaw))
                 __builtin_amdgcn_raw_buffer_store_b32(
                     __builtin_bit_cast(uint32_t, v), out, (4 * g * int(kGuideRaw) + r16) * 4,
@@ -114,26 +138,27 @@ __device__ __forceinline__ void layerInPlace(__bf16* H, __amdgpu_buffer_rsrc_t w
     if (L < kGLayers - 1) ldsSync();
 }
 
-template <int L>
+template <int WD, int L>
 __device__ __forceinline__ void layersFrom(__bf16* H, __amdgpu_buffer_rsrc_t wq,
                                            __amdgpu_buffer_rsrc_t theta,
                                            __amdgpu_buffer_rsrc_t out, uint32_t outOff) {
-    layerInPlace<L>(H, wq, theta, out, outOff);
-    if constexpr (L + 1 < kGLayers) layersFrom<L + 1>(H, wq, theta, out, outOff);
+    layerInPlace<WD, L>(H, wq, theta, out, outOff);
+    if constexpr (L + 1 < kGLayers) layersFrom<WD, L + 1>(H, wq, theta, out, outOff);
 }
 
 // The guide for the wave's lanes with `guided` set: the 64 lanes' queries
 // are four 16-row tiles (a tile with no guided lane is skipped); lane i's
 // kGuideRaw outputs go to raw[i * kGuideRaw ..] (its own slot of the wave's
 // 64 x kGuideRaw floats).  Wave-uniform control flow.
+template <int WD>
 __device__ __forceinline__ void guideWave(const GuideView& gv, bool guided, F3 pos, F3 nrm,
                                           F3 wo, unsigned char* tile, float* __restrict__ raw) {
     const uint64_t gm = __ballot(guided);
     if (gm == 0) return;
     const int lane = threadIdx.x & 63, m = lane & 15, chunk = lane >> 4;
     __bf16* H = reinterpret_cast<__bf16*>(tile);
-    const __amdgpu_buffer_rsrc_t wq = bufferOf(gv.wq, uint32_t(kGQElems) * 2u);
-    const __amdgpu_buffer_rsrc_t theta = bufferOf(gv.theta, uint32_t(kGParams) * 4u);
+    const __amdgpu_buffer_rsrc_t wq = bufferOf(gv.wq, uint32_t(kGQElemsOf<WD>) * 2u);
+    const __amdgpu_buffer_rsrc_t theta = bufferOf(gv.theta, uint32_t(kGParamsOf<WD>) * 4u);
     const __amdgpu_buffer_rsrc_t out = bufferOf(raw, 64u * kGuideRaw * 4u);
     for (int t = 0; t < 4; ++t) {
         if (((gm >> (16 * t)) & 0xFFFFull) == 0) continue;
@@ -142,9 +167,9 @@ __device__ __forceinline__ void guideWave(const GuideView& gv, bool guided, F3 p
         const float pa[3] = {__shfl(pos.x, src, 64), __shfl(pos.y, src, 64), __shfl(pos.z, src, 64)};
         const float na[3] = {__shfl(nrm.x, src, 64), __shfl(nrm.y, src, 64), __shfl(nrm.z, src, 64)};
         const float wa[3] = {__shfl(wo.x, src, 64), __shfl(wo.y, src, 64), __shfl(wo.z, src, 64)};
-        featureRow(H, m, chunk, valid, pa, na, wa, gv.bmin, gv.bext);
+        featureRow<GNet<WD>::kRow>(H, m, chunk, valid, pa, na, wa, gv.bmin, gv.bext);
         ldsSync();
-        layersFrom<0>(H, wq, theta, out, uint32_t(16 * t) * kGuideRaw * 4u);
+        layersFrom<WD, 0>(H, wq, theta, out, uint32_t(16 * t) * kGuideRaw * 4u);
     }
     waveSync();  // the outputs, written by other lanes, are read next
 }
@@ -153,7 +178,7 @@ __device__ __forceinline__ void guideWave(const GuideView& gv, bool guided, F3 p
 // path p is tile (p % perSlot) >> 8 of all views' tiles, view after view,
 // of sample slot p / perSlot, and writes slot * totalPix + pixPrefix[view] +
 // its pixel of L and P; otherwise one camera and slot * npix + pixel.
-template <bool MIS, int SM, int W, bool MULTI>
+template <bool MIS, int SM, int W, bool MULTI, int WD>
 __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(W, 8))) void
 kRenderGuided(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint32_t tilesX,
               uint32_t tiles, uint32_t total, uint32_t* __restrict__ next, float4* __restrict__ L,
@@ -165,7 +190,8 @@ kRenderGuided(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint
     auto stk = makeStack<SM>(lds);
     PathCtx<false, typename StackOf<SM>::T> c{s, stk, 0, {0, 0}};
     const uint32_t waveInBlock = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    unsigned char* tile = reinterpret_cast<unsigned char*>(lds) + tileOff + waveInBlock * kTileBytes;
+    unsigned char* tile =
+        reinterpret_cast<unsigned char*>(lds) + tileOff + waveInBlock * kTileBytesOf<WD>;
     float* raw = scratch + size_t(blockIdx.x * (kRenderBlock / 64u) +
                                   __builtin_amdgcn_readfirstlane(waveInBlock)) *
                                64u * kGuideRaw;
@@ -260,7 +286,7 @@ kRenderGuided(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint
             }
             if (done) finish();
         }
-        guideWave(gv, guided, qPos, qNrm, qWo, tile, raw);
+        guideWave<WD>(gv, guided, qPos, qNrm, qWo, tile, raw);
         // The continuation (kWfShadeCont's, with the guide's mixture).
         if (cont) {
             Isect it = makeIsect(s, ray, h);
@@ -371,9 +397,9 @@ size_t guidedScratchFloats(uint32_t grid) {
     return size_t(grid) * (kRenderBlock / 64u) * 64u * kGuideRaw;
 }
 
-size_t guidedLdsBytes(uint32_t stackMode, uint32_t stackCap) {
+size_t guidedLdsBytes(uint32_t stackMode, uint32_t stackCap, uint32_t width) {
     const size_t st = (stackLdsBytes(stackMode, stackCap, kRenderBlock) + 15) & ~size_t(15);
-    return st + (kRenderBlock / 64u) * kTileBytes;
+    return st + (kRenderBlock / 64u) * (width == 128 ? kTileBytesOf<128> : kTileBytesOf<64>);
 }
 
 hipError_t launchGuidedPass(const SceneView& s, const CameraView& cam, const IntegratorView& cfg,
@@ -388,7 +414,7 @@ hipError_t launchGuidedPass(const SceneView& s, const CameraView& cam, const Int
     const uint32_t grid = std::min<uint32_t>(tiles * active, t.guidedGrid);
     const uint32_t tileOff =
         uint32_t((stackLdsBytes(t.guidedPriv, s.stackCap, kRenderBlock) + 15) & ~size_t(15));
-    const size_t lds = guidedLdsBytes(t.guidedPriv, s.stackCap);
+    const size_t lds = guidedLdsBytes(t.guidedPriv, s.stackCap, g.guide->width);
     hipLaunchKernelGGL(kGuidedReset, dim3(1), dim3(64), 0, st, next, g.recCounter);
     const GuidedLaunch gl{s, cam, cfg, si, tilesX, tiles, total, next, L, P, counters, &g,
                           grid, tileOff, lds, t.guidedWaves, st,
@@ -420,16 +446,22 @@ hipError_t launchGuidedSM<CRH_GMK_PART>(const GuidedLaunch& a) {
     const GuidedPassArgs& g = *a.g;
     withWaves(a.waves, [&](auto wv) {
         constexpr int Wv = decltype(wv)::value;
+        auto go = [&](auto wd) {
+            constexpr int WD = decltype(wd)::value;
 #define CRH_RG(MIS, MULTI)                                                                     \
-    hipLaunchKernelGGL((kRenderGuided<MIS, SM, Wv, MULTI>), dim3(a.grid), dim3(kRenderBlock),  \
-                       a.lds, a.st, a.s, a.cam, a.cfg, a.si, a.tilesX, a.tiles, a.total, a.next, \
-                       a.L, a.P, a.counters, *g.guide, g.alpha, g.mask, g.maxDepth, g.scratch,   \
-                       a.tileOff, g.rec, g.recCounter, g.recCap, a.views)
-        if (a.cfg.useMIS && a.multi) CRH_RG(true, true);
-        else if (a.multi) CRH_RG(false, true);
-        else if (a.cfg.useMIS) CRH_RG(true, false);
-        else CRH_RG(false, false);
+    hipLaunchKernelGGL((kRenderGuided<MIS, SM, Wv, MULTI, WD>), dim3(a.grid),                  \
+                       dim3(kRenderBlock), a.lds, a.st, a.s, a.cam, a.cfg, a.si, a.tilesX,       \
+                       a.tiles, a.total, a.next, a.L, a.P, a.counters, *g.guide, g.alpha,        \
+                       g.mask, g.maxDepth, g.scratch, a.tileOff, g.rec, g.recCounter, g.recCap, \
+                       a.views)
+            if (a.cfg.useMIS && a.multi) CRH_RG(true, true);
+            else if (a.multi) CRH_RG(false, true);
+            else if (a.cfg.useMIS) CRH_RG(true, false);
+            else CRH_RG(false, false);
 #undef CRH_RG
+        };
+        if (g.guide->width == 128) go(std::integral_constant<int, 128>{});
+        else go(std::integral_constant<int, 64>{});
     });
     return hipGetLastError();
 }

@@ -1,14 +1,19 @@
 // Path-guiding network kernels for gfx950: inference and training of the
-// 64-wide MLP with v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32
-// accumulation), the von Mises-Fisher mixture head and Adam.
+// MLP (hidden width 64 or 128, template WD) with v_mfma_f32_16x16x32_bf16
+// (BF16 operands, FP32 accumulation), the von Mises-Fisher mixture head and
+// Adam.
 //
 // Tiles: one wave computes 16 samples; a block of 4 waves moves through the
 // batch in lockstep (64 samples per iteration) so block barriers stay in
-// uniform control flow.  Activations are row-major [16][72] bf16 tiles in
-// LDS (72 = 64 + 8 pad: row starts 36 dwords apart, 2-way bank aliasing at
-// most).  Forward weights are staged once per block in LDS, already in MFMA
-// B-operand order, so each operand is one 16-byte LDS read per lane.
+// uniform control flow.  Activations are row-major [16][WD + 8] bf16 tiles
+// in LDS (8 bf16 of padding: row starts 4 banks apart).  The 64-wide
+// network's forward weights (40 KB) are staged once per block in LDS,
+// already in MFMA B-operand order, so each operand is one 16-byte LDS read
+// per lane; the 128-wide network's (128 KB) are read from global memory
+// (L2) in the same order.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "guide_mlp.h"
 #include "guiding.h"
@@ -20,11 +25,21 @@ constexpr int kWavesPerBlock = 4;
 
 namespace {
 
-// Stage the forward weights (5 layers x 4096 bf16) in LDS.
-__device__ void stageWeights(const __bf16* __restrict__ wq, __bf16* ldsW) {
+// Whether the width-WD network's forward weights are staged in LDS.
+template <int WD>
+constexpr bool kStageW = WD == 64;
+template <int WD>
+constexpr int kStagedElems = kStageW<WD> ? kGQElemsOf<WD> : 0;
+
+// Stage the forward weights in LDS (64-wide network), or use them in
+// global memory; returns where the layers read them.
+template <int WD>
+__device__ const __bf16* stageWeights(const __bf16* __restrict__ wq, __bf16* ldsW) {
+    if constexpr (!kStageW<WD>) return wq;
     const uint4* src = reinterpret_cast<const uint4*>(wq);
     uint4* dst = reinterpret_cast<uint4*>(ldsW);
-    for (int i = threadIdx.x; i < kGQElems / 8; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < kGQElemsOf<WD> / 8; i += blockDim.x) dst[i] = src[i];
+    return ldsW;
 }
 
 }  // namespace
@@ -33,6 +48,7 @@ __device__ void stageWeights(const __bf16* __restrict__ wq, __bf16* ldsW) {
 // Inference: per queued path, mixture parameters
 //   out[p * 40 + ...] = w[8], mu[8][3], kappa[8]
 // ---------------------------------------------------------------------------
+template <int WD>
 __global__ __launch_bounds__(256) void kGuideInfer(GuideView gv, const uint32_t* __restrict__ queue,
                                                    const uint32_t* __restrict__ count,
                                                    uint32_t countDirect,
@@ -41,14 +57,16 @@ __global__ __launch_bounds__(256) void kGuideInfer(GuideView gv, const uint32_t*
                                                    const float4* __restrict__ wo,
                                                    float* __restrict__ params,
                                                    float* __restrict__ rawOut) {
+    using G = GNet<WD>;
     extern __shared__ __align__(16) unsigned char smem[];
     __bf16* ldsW = reinterpret_cast<__bf16*>(smem);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __bf16* H0 = ldsW + kGQElems + wave * (2 * kTile);
-    __bf16* H1 = H0 + kTile;
-    float* outF = reinterpret_cast<float*>(ldsW + kGQElems + kWavesPerBlock * 2 * kTile) +
+    __bf16* H0 = ldsW + kStagedElems<WD> + wave * (2 * G::kTile);
+    __bf16* H1 = H0 + G::kTile;
+    float* outF = reinterpret_cast<float*>(ldsW + kStagedElems<WD> +
+                                           kWavesPerBlock * 2 * G::kTile) +
                   wave * 16 * kGOut;
-    stageWeights(reinterpret_cast<const __bf16*>(gv.wq), ldsW);
+    const __bf16* W = stageWeights<WD>(reinterpret_cast<const __bf16*>(gv.wq), ldsW);
     const uint32_t n = count ? *count : countDirect;
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         __syncthreads();
@@ -63,13 +81,13 @@ __global__ __launch_bounds__(256) void kGuideInfer(GuideView gv, const uint32_t*
             float4 W = valid ? wo[p] : make_float4(0, 0, 0, 0);
             const float pa[3] = {P.x, P.y, P.z}, na[3] = {Nn.x, Nn.y, Nn.z},
                         wa[3] = {W.x, W.y, W.z};
-            featureRow(H0, m, chunk, valid, pa, na, wa, gv.bmin, gv.bext);
+            featureRow<G::kRow>(H0, m, chunk, valid, pa, na, wa, gv.bmin, gv.bext);
         }
         __syncthreads();
         __bf16* in = H0;
         __bf16* out = H1;
         for (int l = 0; l < kGLayers; ++l) {
-            layerForward(l, in, out, outF, ldsW, gv.theta);
+            layerForward<WD>(l, in, out, outF, W, gv.theta);
             __syncthreads();
             __bf16* t = in;
             in = out;
@@ -100,25 +118,29 @@ __global__ __launch_bounds__(256) void kGuideInfer(GuideView gv, const uint32_t*
 // ---------------------------------------------------------------------------
 // Training, pass 1: forward, KL-loss gradient at the outputs, backward
 // through the layers; saves every layer's input H_l and output gradient
-// delta_l (bf16 rows of 64) for the weight-gradient GEMM.
+// delta_l (bf16 rows of WD; H_0 and delta_4 use their first 64 and 48
+// columns) for the weight-gradient GEMM.
 // ---------------------------------------------------------------------------
+template <int WD>
 __global__ __launch_bounds__(256) void kGuideFwdBwd(GuideView gv,
                                                     const GuideSample* __restrict__ samples,
                                                     uint32_t n, float invN,
                                                     __bf16* __restrict__ Hsave,
                                                     __bf16* __restrict__ Dsave,
                                                     float* __restrict__ lossSum) {
+    using G = GNet<WD>;
+    constexpr int kT = G::kTile, kR = G::kRow;
     extern __shared__ __align__(16) unsigned char smem[];
     __bf16* ldsW = reinterpret_cast<__bf16*>(smem);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4,
               r16 = lane & 15;
     // per wave: H_0..H_4 (5 tiles) + two delta tiles
-    __bf16* Hw = ldsW + kGQElems + wave * (7 * kTile);
-    __bf16* Da = Hw + 5 * kTile;
-    __bf16* Db = Da + kTile;
-    float* outF = reinterpret_cast<float*>(ldsW + kGQElems + kWavesPerBlock * 7 * kTile) +
+    __bf16* Hw = ldsW + kStagedElems<WD> + wave * (7 * kT);
+    __bf16* Da = Hw + 5 * kT;
+    __bf16* Db = Da + kT;
+    float* outF = reinterpret_cast<float*>(ldsW + kStagedElems<WD> + kWavesPerBlock * 7 * kT) +
                   wave * 16 * kGOut;
-    stageWeights(reinterpret_cast<const __bf16*>(gv.wq), ldsW);
+    const __bf16* W = stageWeights<WD>(reinterpret_cast<const __bf16*>(gv.wq), ldsW);
     const size_t padN = (size_t(n) + 63) & ~size_t(63);
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         __syncthreads();
@@ -128,16 +150,16 @@ __global__ __launch_bounds__(256) void kGuideFwdBwd(GuideView gv,
             uint32_t idx = tile0 + m;
             bool valid = idx < n;
             const GuideSample& smp = samples[valid ? idx : 0];
-            featureRow(Hw, m, chunk, valid, smp.pos, smp.nrm, smp.wo, gv.bmin, gv.bext);
+            featureRow<kR>(Hw, m, chunk, valid, smp.pos, smp.nrm, smp.wo, gv.bmin, gv.bext);
         }
         __syncthreads();
         for (int l = 0; l < kGLayers; ++l) {
-            layerForward(l, Hw + l * kTile, l + 1 < kGLayers ? Hw + (l + 1) * kTile : nullptr,
-                         outF, ldsW, gv.theta);
+            layerForward<WD>(l, Hw + l * kT, l + 1 < kGLayers ? Hw + (l + 1) * kT : nullptr,
+                             outF, W, gv.theta);
             __syncthreads();
         }
         // Output gradient (oracle/guiding.py loss_and_grad_out), lanes 0..15.
-        for (int c = lane; c < 16 * kRow; c += 64) Da[c] = tobf(0.f);
+        for (int c = lane; c < 16 * kR; c += 64) Da[c] = tobf(0.f);
         __syncthreads();
         if (lane < 16) {
             uint32_t idx = tile0 + lane;
@@ -165,7 +187,7 @@ __global__ __launch_bounds__(256) void kGuideFwdBwd(GuideView gv,
                     float sig = 1.f / (1.f + expf(-h.kraw[j]));
                     float gk = gsc * resp * dlv * sig;
                     float s = gsc * resp * h.k[j] / h.mlen[j];
-                    __bf16* row = Da + lane * kRow;
+                    __bf16* row = Da + lane * kR;
                     row[5 * j] = tobf(ga);
                     for (int c = 0; c < 3; ++c)
                         row[5 * j + 1 + c] = tobf(s * (d[c] - cmu[j] * h.mu[j][c]));
@@ -174,40 +196,42 @@ __global__ __launch_bounds__(256) void kGuideFwdBwd(GuideView gv,
             }
         }
         __syncthreads();
-        // Save H_l rows and delta_4; then backward: delta_{l-1} =
-        // (delta_l W_l^T) masked by H_l > 0.
-        for (int c = lane; c < 16 * 64; c += 64) {
-            int m = c >> 6, f = c & 63;
+        // Save H_l rows (K(l) columns) and delta_4 (64 columns, 48 used);
+        // then backward: delta_{l-1} = (delta_l W_l^T) masked by H_l > 0.
+        for (int c = lane; c < 16 * WD; c += 64) {
+            int m = c / WD, f = c % WD;
             uint32_t idx = tile0 + m;
             if (idx < padN) {
                 for (int l = 0; l < kGLayers; ++l)
-                    Hsave[(size_t(l) * padN + idx) * 64 + f] = Hw[l * kTile + m * kRow + f];
-                Dsave[(size_t(4) * padN + idx) * 64 + f] = Da[m * kRow + f];
+                    if (f < G::K(l))
+                        Hsave[(size_t(l) * padN + idx) * WD + f] = Hw[l * kT + m * kR + f];
+                if (f < G::NP(kGLayers - 1))
+                    Dsave[(size_t(4) * padN + idx) * WD + f] = Da[m * kR + f];
             }
         }
         __bf16* dIn = Da;
         __bf16* dOut = Db;
         for (int l = kGLayers - 1; l >= 1; --l) {
-            const __bf16* H = Hw + l * kTile;
-            for (int t = 0; t < 4; ++t) {
+            const __bf16* H = Hw + l * kT;
+            const int KT = G::K(l) >> 4, NS = G::NP(l) >> 5;
+            for (int t = 0; t < KT; ++t) {
                 f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    bf16x8 a = *reinterpret_cast<const bf16x8*>(dIn + r16 * kRow + 32 * s + 8 * g);
+                for (int s = 0; s < NS; ++s) {
+                    bf16x8 a = *reinterpret_cast<const bf16x8*>(dIn + r16 * kR + 32 * s + 8 * g);
                     bf16x8 b = *reinterpret_cast<const bf16x8*>(
-                        reinterpret_cast<const __bf16*>(gv.wqT) + gQOff(l) +
-                                                                 ((s * 4 + t) * 64 + lane) * 8);
+                        reinterpret_cast<const __bf16*>(gv.wqT) + G::QOff(l) +
+                        ((s * KT + t) * 64 + lane) * 8);
                     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
                 }
                 const int col = 16 * t + r16;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int row = 4 * g + i;
-                    float hv = frombf(H[row * kRow + col]);
+                    float hv = frombf(H[row * kR + col]);
                     __bf16 dv = tobf(hv > 0.f ? acc[i] : 0.f);
-                    dOut[row * kRow + col] = dv;
+                    dOut[row * kR + col] = dv;
                     uint32_t idx = tile0 + row;
-                    if (idx < padN) Dsave[(size_t(l - 1) * padN + idx) * 64 + col] = dv;
+                    if (idx < padN) Dsave[(size_t(l - 1) * padN + idx) * WD + col] = dv;
                 }
             }
             __syncthreads();
@@ -221,19 +245,24 @@ __global__ __launch_bounds__(256) void kGuideFwdBwd(GuideView gv,
 // ---------------------------------------------------------------------------
 // Training, pass 2: dW_l = H_l^T delta_l over the batch (K = samples, 32 per
 // MFMA step) and db_l = sum delta_l, accumulated into grad with atomics.
-// grid.x = 16 output tiles per layer x 5 layers, grid.y = batch chunks.
+// grid.x = (WD / 16)^2 output tiles per layer x 5 layers (tiles outside a
+// layer's K x N leave at once), grid.y = batch chunks.
 // ---------------------------------------------------------------------------
+template <int WD>
 __global__ __launch_bounds__(64) void kGuideWeightGrad(GuideView gv, const __bf16* __restrict__ Hsave,
                                                        const __bf16* __restrict__ Dsave, uint32_t n,
                                                        uint32_t chunk) {
+    using G = GNet<WD>;
+    constexpr int TS = WD / 16;  // tiles per side
     const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
-    const int l = blockIdx.x / 16, tile = blockIdx.x % 16, a = tile / 4, b = tile % 4;
-    const int N = gLayerN(l);
-    if (16 * b >= N) return;
+    const int l = blockIdx.x / (TS * TS), tile = blockIdx.x % (TS * TS), a = tile / TS,
+              b = tile % TS;
+    const int N = G::N(l);
+    if (16 * a >= G::K(l) || 16 * b >= N) return;
     const size_t padN = (size_t(n) + 63) & ~size_t(63);
     const uint32_t s0 = blockIdx.y * chunk, s1 = min(n, s0 + chunk);
-    const __bf16* H = Hsave + size_t(l) * padN * 64;
-    const __bf16* D = Dsave + size_t(l) * padN * 64;
+    const __bf16* H = Hsave + size_t(l) * padN * WD;
+    const __bf16* D = Dsave + size_t(l) * padN * WD;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     float bsum = 0.f;
     for (uint32_t s = s0; s < s1; s += 32) {
@@ -242,8 +271,8 @@ __global__ __launch_bounds__(64) void kGuideWeightGrad(GuideView gv, const __bf1
         for (int i = 0; i < 8; ++i) {
             uint32_t smp = s + 8 * g + i;
             bool ok = smp < s1;
-            av[i] = ok ? H[size_t(smp) * 64 + 16 * a + r16] : tobf(0.f);
-            bv[i] = ok ? D[size_t(smp) * 64 + 16 * b + r16] : tobf(0.f);
+            av[i] = ok ? H[size_t(smp) * WD + 16 * a + r16] : tobf(0.f);
+            bv[i] = ok ? D[size_t(smp) * WD + 16 * b + r16] : tobf(0.f);
             if (a == 0) bsum += frombf(bv[i]);
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
@@ -252,13 +281,13 @@ __global__ __launch_bounds__(64) void kGuideWeightGrad(GuideView gv, const __bf1
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         int row = 16 * a + 4 * g + i, col = 16 * b + r16;
-        atomicAdd(&grad[gWOff(l) + row * N + col], acc[i]);
+        atomicAdd(&grad[G::WOff(l) + row * N + col], acc[i]);
     }
     if (a == 0) {
         // lanes with the same column (r16) hold partial sums for g = 0..3
         bsum += __shfl_xor(bsum, 16, 64);
         bsum += __shfl_xor(bsum, 32, 64);
-        if (g == 0) atomicAdd(&grad[gBOff(l) + 16 * b + r16], bsum);
+        if (g == 0) atomicAdd(&grad[G::BOff(l) + 16 * b + r16], bsum);
     }
 }
 
@@ -266,10 +295,12 @@ __global__ __launch_bounds__(64) void kGuideWeightGrad(GuideView gv, const __bf1
 // Adam on the fp32 master parameters; refreshes both bf16 operand copies
 // and clears the gradient for the next step.
 // ---------------------------------------------------------------------------
+template <int WD>
 __global__ __launch_bounds__(256) void kGuideAdam(GuideView gv, float lr, float b1, float b2,
                                                   float eps, float c1, float c2) {
+    using G = GNet<WD>;
     int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= kGParams) return;
+    if (i >= kGParamsOf<WD>) return;
     float gr = gv.grad[i];
     float m = b1 * gv.m[i] + (1.f - b1) * gr;
     float v = b2 * gv.v[i] + (1.f - b2) * gr * gr;
@@ -280,38 +311,44 @@ __global__ __launch_bounds__(256) void kGuideAdam(GuideView gv, float lr, float 
     gv.grad[i] = 0.f;
     // weight element?  find its layer
     for (int l = 0; l < kGLayers; ++l) {
-        int w0 = gWOff(l), N = gLayerN(l);
-        if (i >= w0 && i < w0 + 64 * N) {
+        int w0 = G::WOff(l), K = G::K(l), N = G::N(l);
+        if (i >= w0 && i < w0 + K * N) {
             int k = (i - w0) / N, c = (i - w0) % N;
-            reinterpret_cast<__bf16*>(gv.wq)[gQOff(l) + gSwz(k, c, N)] = tobf(th);
-            reinterpret_cast<__bf16*>(gv.wqT)[gQOff(l) + gSwz(c, k, 64)] = tobf(th);
+            reinterpret_cast<__bf16*>(gv.wq)[G::QOff(l) + gSwz(k, c, N)] = tobf(th);
+            reinterpret_cast<__bf16*>(gv.wqT)[G::QOff(l) + gSwz(c, k, K)] = tobf(th);
         }
     }
 }
 
-// Rebuild the bf16 operand copies from theta (after a host upload).
+// Rebuild the bf16 operand copies from theta (after a host upload): one
+// thread per element of the copies; W^T's padding rows stay zero.
+template <int WD>
 __global__ __launch_bounds__(256) void kGuideQuantize(GuideView gv) {
+    using G = GNet<WD>;
     int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= kGQElems) return;
-    reinterpret_cast<__bf16*>(gv.wqT)[i] = tobf(0.f);  // padding rows stay zero
+    if (i >= kGQElemsOf<WD>) return;
+    reinterpret_cast<__bf16*>(gv.wqT)[i] = tobf(0.f);
     for (int l = 0; l < kGLayers; ++l) {
-        int w0 = gWOff(l), N = gLayerN(l);
-        int j = i - l * 4096;
-        if (j >= 0 && j < 64 * N) {
+        int w0 = G::WOff(l), N = G::N(l);
+        int j = i - G::QOff(l);
+        if (j >= 0 && j < G::K(l) * N) {
             int k = j / N, c = j % N;
             float th = gv.theta[w0 + j];
-            reinterpret_cast<__bf16*>(gv.wq)[gQOff(l) + gSwz(k, c, N)] = tobf(th);
+            reinterpret_cast<__bf16*>(gv.wq)[G::QOff(l) + gSwz(k, c, N)] = tobf(th);
         }
     }
 }
+template <int WD>
 __global__ __launch_bounds__(256) void kGuideQuantizeT(GuideView gv) {
+    using G = GNet<WD>;
     int i = blockIdx.x * 256 + threadIdx.x;
     for (int l = 0; l < kGLayers; ++l) {
-        int w0 = gWOff(l), N = gLayerN(l);
-        int j = i - l * 4096;
-        if (j >= 0 && j < 64 * N) {
+        int w0 = G::WOff(l), K = G::K(l), N = G::N(l);
+        int j = i - G::QOff(l);
+        if (j >= 0 && j < K * N) {
             int k = j / N, c = j % N;
-            reinterpret_cast<__bf16*>(gv.wqT)[gQOff(l) + gSwz(c, k, 64)] = tobf(gv.theta[w0 + j]);
+            reinterpret_cast<__bf16*>(gv.wqT)[G::QOff(l) + gSwz(c, k, K)] =
+                tobf(gv.theta[w0 + j]);
         }
     }
 }
@@ -319,37 +356,60 @@ __global__ __launch_bounds__(256) void kGuideQuantizeT(GuideView gv) {
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-size_t guideInferLds() {
-    return size_t(kGQElems) * 2 + size_t(kWavesPerBlock) * 2 * kTile * 2 +
+namespace {
+
+template <int WD>
+size_t inferLds() {
+    return size_t(kStagedElems<WD>) * 2 + size_t(kWavesPerBlock) * 2 * GNet<WD>::kTile * 2 +
            size_t(kWavesPerBlock) * 16 * kGOut * 4;
 }
-size_t guideTrainLds() {
-    return size_t(kGQElems) * 2 + size_t(kWavesPerBlock) * 7 * kTile * 2 +
+template <int WD>
+size_t trainLds() {
+    return size_t(kStagedElems<WD>) * 2 + size_t(kWavesPerBlock) * 7 * GNet<WD>::kTile * 2 +
            size_t(kWavesPerBlock) * 16 * kGOut * 4;
 }
 
+// f(integral_constant<int, WD>) for the view's width.
+template <class F>
+void withWidth(uint32_t width, F&& f) {
+    if (width == 128) f(std::integral_constant<int, 128>{});
+    else f(std::integral_constant<int, 64>{});
+}
+
+}  // namespace
+
 hipError_t guideSetLdsLimits() {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideInfer),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       int(guideInferLds()));
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&kGuideFwdBwd),
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(guideTrainLds()));
+    hipError_t e = hipSuccess;
+    auto set = [&](const void* k, size_t bytes) {
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+    };
+    set(reinterpret_cast<const void*>(&kGuideInfer<64>), inferLds<64>());
+    set(reinterpret_cast<const void*>(&kGuideInfer<128>), inferLds<128>());
+    set(reinterpret_cast<const void*>(&kGuideFwdBwd<64>), trainLds<64>());
+    set(reinterpret_cast<const void*>(&kGuideFwdBwd<128>), trainLds<128>());
+    return e;
 }
 
 hipError_t launchGuideInfer(const GuideView& gv, const uint32_t* queue, const uint32_t* count,
                             uint32_t countDirect, const float4* pos, const float4* nrm,
                             const float4* wo, float* params, float* rawOut, uint32_t grid,
                             hipStream_t st) {
-    hipLaunchKernelGGL(kGuideInfer, dim3(grid), dim3(256), guideInferLds(), st, gv, queue, count,
-                       countDirect, pos, nrm, wo, params, rawOut);
+    withWidth(gv.width, [&](auto w) {
+        constexpr int WD = decltype(w)::value;
+        hipLaunchKernelGGL(kGuideInfer<WD>, dim3(grid), dim3(256), inferLds<WD>(), st, gv, queue,
+                           count, countDirect, pos, nrm, wo, params, rawOut);
+    });
     return hipGetLastError();
 }
 
 hipError_t launchGuideQuantize(const GuideView& gv, hipStream_t st) {
-    hipLaunchKernelGGL(kGuideQuantize, dim3(kGQElems / 256), dim3(256), 0, st, gv);
-    hipLaunchKernelGGL(kGuideQuantizeT, dim3(kGQElems / 256), dim3(256), 0, st, gv);
+    withWidth(gv.width, [&](auto w) {
+        constexpr int WD = decltype(w)::value;
+        const dim3 grid((kGQElemsOf<WD> + 255) / 256);
+        hipLaunchKernelGGL(kGuideQuantize<WD>, grid, dim3(256), 0, st, gv);
+        hipLaunchKernelGGL(kGuideQuantizeT<WD>, grid, dim3(256), 0, st, gv);
+    });
     return hipGetLastError();
 }
 
@@ -361,17 +421,20 @@ hipError_t launchGuideTrainStep(const GuideView& gv, const GuideSample* samples,
     if (n == 0) return hipSuccess;
     uint32_t blocks = (n + 63) / 64;
     if (blocks > 512) blocks = 512;
-    hipLaunchKernelGGL(kGuideFwdBwd, dim3(blocks), dim3(256), guideTrainLds(), st, gv, samples, n,
-                       1.0f / float(n), Hsave, Dsave, lossSum);
-    const uint32_t chunk = 2048;
-    dim3 wg(16 * kGLayers, (n + chunk - 1) / chunk);
-    hipLaunchKernelGGL(kGuideWeightGrad, wg, dim3(64), 0, st, gv, Hsave, Dsave, n, chunk);
-    if (applyAdam) {
-        const float b1 = 0.9f, b2 = 0.999f;
-        float c1 = 1.f - powf(b1, float(step)), c2 = 1.f - powf(b2, float(step));
-        hipLaunchKernelGGL(kGuideAdam, dim3((kGParams + 255) / 256), dim3(256), 0, st, gv, lr, b1,
-                           b2, 1e-8f, c1, c2);
-    }
+    withWidth(gv.width, [&](auto w) {
+        constexpr int WD = decltype(w)::value;
+        hipLaunchKernelGGL(kGuideFwdBwd<WD>, dim3(blocks), dim3(256), trainLds<WD>(), st, gv,
+                           samples, n, 1.0f / float(n), Hsave, Dsave, lossSum);
+        const uint32_t chunk = 2048;
+        dim3 wg((WD / 16) * (WD / 16) * kGLayers, (n + chunk - 1) / chunk);
+        hipLaunchKernelGGL(kGuideWeightGrad<WD>, wg, dim3(64), 0, st, gv, Hsave, Dsave, n, chunk);
+        if (applyAdam) {
+            const float b1 = 0.9f, b2 = 0.999f;
+            float c1 = 1.f - powf(b1, float(step)), c2 = 1.f - powf(b2, float(step));
+            hipLaunchKernelGGL(kGuideAdam<WD>, dim3((kGParamsOf<WD> + 255) / 256), dim3(256), 0,
+                               st, gv, lr, b1, b2, 1e-8f, c1, c2);
+        }
+    });
     return hipGetLastError();
 }
 

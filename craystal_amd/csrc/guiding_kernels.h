@@ -9,18 +9,17 @@
 namespace crh {
 
 struct GuideView {
-    float* theta;  // fp32 master parameters (kGParams)
+    float* theta;  // fp32 master parameters (gParamsOf(width))
     float* m;      // Adam first moment
     float* v;      // Adam second moment
     float* grad;   // gradient accumulator (cleared by the Adam kernel)
-    uint16_t* wq;  // bf16 bits: W_l in MFMA B-operand order (kGQElems)
-    uint16_t* wqT; // bf16 bits: W_l^T, zero-padded to 64 x 64, same order
+    uint16_t* wq;  // bf16 bits: W_l in MFMA B-operand order (gQElemsOf(width))
+    uint16_t* wqT; // bf16 bits: W_l^T, rows padded to a multiple of 64, same order
     float bmin[3];
     float bext[3];
+    uint32_t width;  // hidden width: 64 or 128
 };
 
-size_t guideInferLds();
-size_t guideTrainLds();
 hipError_t guideSetLdsLimits();
 hipError_t launchGuideInfer(const GuideView& gv, const uint32_t* queue, const uint32_t* count,
                             uint32_t countDirect, const float4* pos, const float4* nrm,

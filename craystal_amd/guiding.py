@@ -15,21 +15,25 @@ from .core import get_context
 
 
 class Guide:
-    def __init__(self, bmin, bmax, seed: int = 1234, device: int = 0):
+    def __init__(self, bmin, bmax, seed: int = 1234, device: int = 0, width: int = 64):
+        """width: the hidden width of the network, 64 or 128."""
         self.ctx = get_context(device)
         lo = np.ascontiguousarray(bmin, np.float32)
         hi = np.ascontiguousarray(bmax, np.float32)
         h = C.c_void_p()
-        L.call("crh_guide_create", self.ctx.handle, L.ptr(lo), L.ptr(hi), seed, C.byref(h))
+        L.call("crh_guide_create_width", self.ctx.handle, L.ptr(lo), L.ptr(hi), seed, width,
+               C.byref(h))
         self.handle = h
         self.bmin, self.bmax = lo, hi
+        self.width = width
+        self.n_params = L.GUIDE_PARAMS_128 if width == 128 else L.GUIDE_PARAMS
 
     def set_params(self, theta: np.ndarray):
         t = np.ascontiguousarray(theta, np.float32).reshape(-1)
         L.call("crh_guide_set_params", self.handle, L.ptr(t), len(t))
 
     def get_params(self) -> np.ndarray:
-        t = np.zeros(L.GUIDE_PARAMS, np.float32)
+        t = np.zeros(self.n_params, np.float32)
         L.call("crh_guide_get_params", self.handle, L.ptr(t), len(t))
         return t
 
@@ -57,16 +61,17 @@ class Guide:
         """One step; returns (loss, gradient or None)."""
         s = np.ascontiguousarray(samples, np.float32).reshape(-1, 16)
         loss = np.zeros(1, np.float32)
-        grad = None if apply_adam else np.zeros(L.GUIDE_PARAMS, np.float32)
+        grad = None if apply_adam else np.zeros(self.n_params, np.float32)
         L.call("crh_guide_train", self.handle, L.ptr(s), len(s), lr, 1 if apply_adam else 0,
                L.ptr(loss), None if grad is None else L.ptr(grad))
         return float(loss[0]), grad
 
     def info(self) -> dict:
-        """Adam steps taken, samples of the last trained pass, last loss."""
+        """Adam steps taken, samples of the last trained pass, last loss, width."""
         i = L.crh_guide_info()
         L.call("crh_guide_get_info", self.handle, C.byref(i))
-        return {"steps": i.steps, "last_samples": i.last_samples, "last_loss": i.last_loss}
+        return {"steps": i.steps, "last_samples": i.last_samples, "last_loss": i.last_loss,
+                "width": i.width}
 
     def losses(self, n: int | None = None) -> np.ndarray:
         """Losses of the last n Adam steps (at most 1024), oldest first."""
@@ -162,8 +167,8 @@ def flatten_params(P) -> np.ndarray:
     return np.concatenate([np.concatenate([W.reshape(-1), b]) for W, b in P]).astype(np.float32)
 
 
-def unflatten_params(theta: np.ndarray):
-    shapes = [(64, 64), (64, 64), (64, 64), (64, 64), (64, 48)]
+def unflatten_params(theta: np.ndarray, width: int = 64):
+    shapes = [(64, width), (width, width), (width, width), (width, width), (width, 48)]
     P, off = [], 0
     for k, nn in shapes:
         W = theta[off:off + k * nn].reshape(k, nn).astype(np.float64)

@@ -409,17 +409,23 @@ int crh_render(crh_scene* scene, const crh_camera* cam,
                uint32_t spp, crh_film* film, crh_stats* stats);
 
 /* ---- learned path guiding (no counterpart in the reference) ------------
- * A 64-wide MLP (4 hidden ReLU layers, BF16 weights on MFMA, FP32
- * accumulation) maps (position, shading normal, outgoing direction) to an
- * 8-lobe von Mises-Fisher mixture; trained online with Adam on the KL
- * divergence to the renderer's radiance samples.  Parameter layout: for each
- * of the 5 layers, W (64 x N row-major, N = 64, 64, 64, 64, 48) then b (N);
- * CRH_GUIDE_PARAMS floats in total. */
+ * An MLP of hidden width 64 (or 128; 4 hidden ReLU layers, BF16 weights on
+ * MFMA, FP32 accumulation) maps (position, shading normal, outgoing
+ * direction) to an 8-lobe von Mises-Fisher mixture; trained online with
+ * Adam on the KL divergence to the renderer's radiance samples.  Parameter
+ * layout: for each of the 5 layers, W (K x N row-major, K = 64, w, w, w, w,
+ * N = w, w, w, w, 48 for width w) then b (N); CRH_GUIDE_PARAMS floats at
+ * width 64, CRH_GUIDE_PARAMS_128 at width 128. */
 #define CRH_GUIDE_PARAMS 19760u
+#define CRH_GUIDE_PARAMS_128 64048u
 #define CRH_GUIDE_OUTPUTS 48u
-/* scene bounds normalise the position encoding; seed picks the He init */
+/* scene bounds normalise the position encoding; seed picks the He init;
+ * width 64 */
 int crh_guide_create(crh_context* ctx, const float* bmin, const float* bmax,
                      uint32_t seed, crh_guide** out);
+/* the same with hidden width 64 or 128 */
+int crh_guide_create_width(crh_context* ctx, const float* bmin, const float* bmax,
+                           uint32_t seed, uint32_t width, crh_guide** out);
 int crh_guide_set_params(crh_guide* g, const float* theta, uint32_t n);
 int crh_guide_get_params(const crh_guide* g, float* theta, uint32_t n);
 /* n queries (pos, nrm, wo: 3 floats each).  raw: n*48 network outputs;
@@ -437,7 +443,7 @@ typedef struct crh_guide_info {
     uint32_t steps;
     uint32_t last_samples;
     float last_loss;
-    uint32_t reserved;
+    uint32_t width;  /* hidden width of the network: 64 or 128 */
 } crh_guide_info;
 int crh_guide_get_info(const crh_guide* g, crh_guide_info* info);
 /* Losses of the last n Adam steps, oldest first; n <= min(steps, 1024). */

@@ -10,8 +10,8 @@ Model
   features  p = (x - bmin) / (bmax - bmin); for axis a, level l < 5:
             sin(2^l pi p_a), cos(2^l pi p_a)  -> 30; then 2 zeros (32);
             shading normal n (3); outgoing direction wo (3); zero pad to 64.
-  MLP       64 -> 64 -> 64 -> 64 -> 64 -> 48 (ReLU after the first four),
-            the last layer's outputs 40..47 unused.
+  MLP       64 -> w -> w -> w -> w -> 48 (ReLU after the first four; hidden
+            width w = 64, or 128), the last layer's outputs 40..47 unused.
   head      for lobe j < 8: a = o[5j], m = o[5j+1:5j+4], k = o[5j+4];
             weights softmax(a), mean m/|m|, kappa softplus(k) + 0.01.
   vMF       v(w; mu, k) = k / (2 pi (1 - e^{-2k})) * exp(k (mu.w - 1))
@@ -30,6 +30,11 @@ N_OUT = 48
 N_LOBES = 8
 N_FREQ = 5
 LAYERS = [(N_IN, WIDTH), (WIDTH, WIDTH), (WIDTH, WIDTH), (WIDTH, WIDTH), (WIDTH, N_OUT)]
+
+
+def layers(width=WIDTH):
+    """(fan-in, fan-out) of the 5 layers of a network of hidden width `width`."""
+    return [(N_IN, width), (width, width), (width, width), (width, width), (width, N_OUT)]
 KAPPA_MIN = 0.01
 
 
@@ -57,11 +62,11 @@ def features(pos, nrm, wo, bmin, bmax):
     return X
 
 
-def init_params(seed=1234):
+def init_params(seed=1234, width=WIDTH):
     """He-uniform weights (fan-in), zero biases; float64 master copy."""
     rng = np.random.default_rng(seed)
     P = []
-    for fi, fo in LAYERS:
+    for fi, fo in layers(width):
         lim = np.sqrt(6.0 / fi)
         P.append([rng.uniform(-lim, lim, (fi, fo)), np.zeros(fo)])
     return P

@@ -126,6 +126,32 @@ def test_fused_launch_variants_give_one_film(gpu, tuning_restored):
                 assert got == ref, (stack, waves, grid, pool)
 
 
+@pytest.mark.parametrize("mis", [True, False])
+def test_fused_film_equals_wavefront_width128(gpu, tuning_restored, mis):
+    """The 128-wide network (hidden layers of 4 x 8 MFMA operand blocks,
+    weights read per 16-column block) in the fused kernel and in the guided
+    wavefront's inference kernel: bit-identical films for a randomly
+    initialised guide, at every stack placement and register budget tried."""
+    ctx = tuning_restored
+    data, camd = scenes.occluded_room(n_objects=12, subdiv=1)
+    P = np.array(data.concatenated()[0])
+    guide = Guide(P.min(0), P.max(0), seed=13, width=128)
+    assert guide.info()["width"] == 128 and len(guide.get_params()) == L.GUIDE_PARAMS_128
+    scene = cr.Scene(data)
+    sensor = cr.Sensor((56, 48))
+    scene.setCamera(scenes.make_camera(camd, sensor))
+    cfg = cr.Configs(maxDepth=10, rrThreshold=0.05, useMIS=mis, rrStartDepth=3)
+    gi = GuidedIntegrator(cfg, guide, alpha=0.5)
+    ref, _ = _render(ctx, scene, sensor, gi, False, 6, 41)
+    for stack, waves in ((2, 6), (3, 8), (1, 4)):
+        got, st = _render(ctx, scene, sensor, gi, True, 6, 41, guided_stack=stack,
+                          guided_waves=waves)
+        _say(f"width 128, stack {stack} waves {waves}: {st['kernel_ms']:.2f} ms")
+        assert got == ref, (stack, waves)
+    img = sensor.createImage()
+    assert np.isfinite(img).all() and img.mean() > 0
+
+
 def test_fused_training_runs_and_is_unbiased(gpu, tuning_restored):
     """Training from the fused kernel's records: steps run with finite
     losses, and trained / untrained guided image means agree with the plain

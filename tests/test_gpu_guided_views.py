@@ -5,7 +5,8 @@ sun disk, sky).  With a guide that does not train, every view's film is bit
 for bit the film crh_render_guided makes of that view alone: the random
 streams are keyed by pixel and sample index and a row of the network's
 16-row tiles does not depend on the other rows.  With training on, the
-guide learns from every view's vertices."""
+guide learns from every view's vertices.  The terrain configuration's guide
+is 128 wide; both widths are covered."""
 import numpy as np
 import pytest
 
@@ -25,17 +26,18 @@ def _terrain():
     return data, cams
 
 
-def _gi(data, seed, materials=0):
+def _gi(data, seed, materials=0, width=64):
     P = data.concatenated()[0]
-    return GuidedIntegrator(CFG, Guide(P.min(0), P.max(0), seed=seed), alpha=0.5, lr=1e-2,
-                            train_batch=1024, train_steps=4, materials=materials, target=0)
+    return GuidedIntegrator(CFG, Guide(P.min(0), P.max(0), seed=seed, width=width), alpha=0.5,
+                            lr=1e-2, train_batch=1024, train_steps=4, materials=materials,
+                            target=0)
 
 
-@pytest.mark.parametrize("materials", [0, 1])
-def test_views_equal_solo_guided_renders(gpu, materials):
+@pytest.mark.parametrize("materials,width", [(0, 64), (1, 64), (0, 128)])
+def test_views_equal_solo_guided_renders(gpu, materials, width):
     data, cams = _terrain()
     scene = cr.Scene(data)
-    gi = _gi(data, 7, materials)
+    gi = _gi(data, 7, materials, width)
     spp, first = 3, 11
     solo = []
     for camd in cams:
@@ -64,10 +66,11 @@ def test_views_equal_solo_guided_renders(gpu, materials):
     assert np.isfinite(img).all() and img.mean() > 0.01
 
 
-def test_views_train_the_guide(gpu):
+@pytest.mark.parametrize("width", [64, 128])
+def test_views_train_the_guide(gpu, width):
     data, cams = _terrain()
     scene = cr.Scene(data)
-    gi = _gi(data, 9)
+    gi = _gi(data, 9, width=width)
     sensors = [cr.Sensor(c["size"]) for c in cams]
     cam_objs = [scenes.make_camera(c, s) for c, s in zip(cams, sensors)]
     theta0 = gi.guide.get_params()

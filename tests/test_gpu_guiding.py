@@ -1,5 +1,6 @@
 """GPU guiding network vs the float64 oracle (oracle/guiding.py): forward
-outputs, mixture head, gradients (BF16-rounding-aware tolerances) and Adam."""
+outputs, mixture head, gradients (BF16-rounding-aware tolerances) and Adam,
+for the 64-wide network and the 128-wide one."""
 import numpy as np
 import pytest
 
@@ -29,9 +30,10 @@ def _bf16_tol(ref, rel=0.02):
     return rel * (np.abs(ref) + 0.05 * np.abs(ref).max() + 1e-6)
 
 
-def test_mfma_forward_matches_oracle(gpu):
-    P = G.init_params(3)
-    g = Guide(BMIN, BMAX)
+@pytest.mark.parametrize("width", [64, 128])
+def test_mfma_forward_matches_oracle(gpu, width):
+    P = G.init_params(3, width=width)
+    g = Guide(BMIN, BMAX, width=width)
     g.set_params(flatten_params(P))
     assert np.array_equal(g.get_params(), flatten_params(P))
     pos, nrm, wo, _, _ = _inputs(1000)        # not a multiple of 64: ragged tail
@@ -53,9 +55,10 @@ def test_mfma_forward_matches_oracle(gpu):
     assert np.allclose(mix[:, :8].sum(1), 1, atol=1e-5)
 
 
-def test_gradients_match_oracle(gpu):
-    P = G.init_params(5)
-    g = Guide(BMIN, BMAX)
+@pytest.mark.parametrize("width", [64, 128])
+def test_gradients_match_oracle(gpu, width):
+    P = G.init_params(5, width=width)
+    g = Guide(BMIN, BMAX, width=width)
     g.set_params(flatten_params(P))
     pos, nrm, wo, wi, y = _inputs(2048, 1)
     q = np.full(len(pos), 1 / (4 * np.pi), np.float32)
@@ -72,9 +75,10 @@ def test_gradients_match_oracle(gpu):
     assert np.array_equal(g.get_params(), flatten_params(P))
 
 
-def test_adam_step_matches_oracle(gpu):
-    P = G.init_params(9)
-    g = Guide(BMIN, BMAX)
+@pytest.mark.parametrize("width", [64, 128])
+def test_adam_step_matches_oracle(gpu, width):
+    P = G.init_params(9, width=width)
+    g = Guide(BMIN, BMAX, width=width)
     g.set_params(flatten_params(P))
     pos, nrm, wo, wi, y = _inputs(1024, 2)
     q = np.full(len(pos), 1 / (4 * np.pi), np.float32)
@@ -84,7 +88,7 @@ def test_adam_step_matches_oracle(gpu):
     theta = g.get_params()
     opt = G.Adam([[W.copy(), b.copy()] for W, b in P], lr=1e-2)
     expect = flatten_params(opt.step([[W.copy(), b.copy()] for W, b in P],
-                                     [tuple(x) for x in unflatten_params(grad)]))
+                                     [tuple(x) for x in unflatten_params(grad, width)]))
     moved = theta - flatten_params(P)
     # first Adam step: |delta| = lr where the gradient is not ~0
     # (|m_hat / (sqrt(v_hat) + eps)| = |g| / (|g| + 1e-8), so 1 within 1e-4
@@ -95,10 +99,12 @@ def test_adam_step_matches_oracle(gpu):
     assert np.mean(np.sign(moved[big]) == np.sign(expect[big] - flatten_params(P)[big])) > 0.99
 
 
-def test_training_fits_a_target_lobe(gpu):
+@pytest.mark.parametrize("width", [64, 128])
+def test_training_fits_a_target_lobe(gpu, width):
     """On-GPU training against a fixed target distribution lowers the KL loss
     and concentrates the mixture toward the target mean."""
-    g = Guide(BMIN, BMAX, seed=21)
+    g = Guide(BMIN, BMAX, seed=21, width=width)
+    assert g.info()["width"] == width
     rng = np.random.default_rng(7)
     pos, nrm, wo, _, _ = _inputs(4096, 3)
     mu = np.array([0.0, 0.0, 1.0])

@@ -1,6 +1,7 @@
 """Pins of the path-guiding oracle (oracle/guiding.py): finite differences,
 quadrature, sampling histograms and a closed-form Adam step.  CPU only."""
 import numpy as np
+import pytest
 
 from oracle import guiding as G
 
@@ -30,8 +31,11 @@ def test_features_layout():
     assert not X[0, 38:].any() and not X[0, 30:32].any()
 
 
-def test_backward_matches_finite_differences():
-    P = G.init_params(7)
+@pytest.mark.parametrize("width", [64, 128])
+def test_backward_matches_finite_differences(width):
+    P = G.init_params(7, width=width)
+    assert [W.shape for W, _ in P] == [(64, width), (width, width), (width, width),
+                                       (width, width), (width, 48)]
     X, d, y, q = _batch()
     out, acts = G.forward(P, X)
     _, Gout = G.loss_and_grad_out(out, d, y, q)
