@@ -73,6 +73,9 @@ def parse():
     ap.add_argument("--terrain-bvh", choices=("fast", "reference"), default="fast",
                     help="terrain BVH: built on the GPU without the depth cap (fast) or "
                          "with the reference's parameters")
+    ap.add_argument("--terrain-guided-spp", type=int, default=2,
+                    help="spp of every terrain view trained, then timed, with the 128-wide "
+                         "guide (0 = skip)")
     ap.add_argument("--terrain-spp", type=int, default=4,
                     help="timed spp per view of the terrain leg")
     ap.add_argument("--no-traffic", action="store_true",
@@ -272,7 +275,39 @@ def batch_leg(cr, scenes, n: int) -> dict:
             "slots_ms": tuned["slots_ms"]}
 
 
-def terrain_leg(cr, scenes, spp: int, trace_rays: int, bvh: str = "fast") -> dict:
+def terrain_guided(cr, scene, data, cams, cfg, spp: int, px: float) -> dict:
+    """Guided rendering of the terrain's four views in one fused launch per
+    pass group (crh_render_guided_views) with the 128-wide guide: `spp`
+    spp of every view with training (the guide learns from all views),
+    then `spp` timed spp without.  One-sample MIS with alpha 0.5 at every
+    Lambertian and Principled vertex."""
+    from craystal_amd import autotune
+    from craystal_amd.guiding import Guide, GuidedIntegrator
+    P = data.concatenated()[0]
+    guide = Guide(P.min(0), P.max(0), seed=17, width=128)
+    gi = GuidedIntegrator(cfg, guide, alpha=0.5, lr=1e-2, train_batch=16384, train_steps=4,
+                          materials=0, target=0)
+    t0 = time.perf_counter()
+    tr = gi.dispatch_views(scene, cams, spp, first_sample=1 << 20, train=True, readback=False)
+    train_s = time.perf_counter() - t0
+    for s in (c.getSensor() for c in cams):
+        s.clear()
+    t0 = time.perf_counter()
+    st = gi.dispatch_views(scene, cams, spp, first_sample=1 << 21, train=False, readback=False)
+    el = time.perf_counter() - t0
+    n = px * spp
+    return {"guide_width": 128, "spp_trained": spp, "spp_timed": spp,
+            "adam_steps": guide.info()["steps"], "train_s": round(train_s, 3),
+            "train_msamples_per_s": round(n / train_s / 1e6, 2),
+            "kernel_ms": round(st["kernel_ms"], 3), "wall_ms": round(el * 1e3, 3),
+            "msamples_per_s": round(n / el / 1e6, 2),
+            "grays_per_s": round(st["rays"] / el / 1e9, 4),
+            "tuning": {k: cr.get_context(0).get_tuning()[k]
+                       for k in ("guided_stack", "guided_waves", "guided_grid", "guided_pool")}}
+
+
+def terrain_leg(cr, scenes, spp: int, trace_rays: int, bvh: str = "fast",
+                guided_spp: int = 2) -> dict:
     """The large-scale workload: the fBm terrain (4096 x 4096 vertices,
     33,538,050 triangles) with 2,000 subdivision-4 icospheres (43,778,066
     triangles with the sun disk), four 3840 x 2160 camera views rendered in
@@ -324,6 +359,9 @@ def terrain_leg(cr, scenes, spp: int, trace_rays: int, bvh: str = "fast") -> dic
            "grays_per_s": round(st["rays"] / el / 1e9, 4),
            "rays_per_sample": round(st["rays"] / samples, 4)}
     progress(f"terrain render: {out['msamples_per_s']} Msamples/s")
+    if guided_spp > 0:
+        out["guided"] = terrain_guided(cr, scene, data, cam_objs, cfg, guided_spp, samples / spp)
+        progress(f"terrain guided: {out['guided']['msamples_per_s']} Msamples/s")
     del sensors, cam_objs, batch
     if trace_rays > 0:
         n = trace_rays
@@ -797,7 +835,8 @@ def main():
         # the four views render as a batch: at the batch leg's tuned shape
         ctx.set_tuning(**main_tuning)
         ctx.set_tuning(**batch_tuning)
-        out["terrain"] = terrain_leg(cr, scenes, a.terrain_spp, a.trace_rays, a.terrain_bvh)
+        out["terrain"] = terrain_leg(cr, scenes, a.terrain_spp, a.trace_rays, a.terrain_bvh,
+                                     a.terrain_guided_spp)
 
     if not a.no_cpu:
         import oracle
