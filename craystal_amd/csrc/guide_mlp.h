@@ -25,6 +25,14 @@ __device__ __forceinline__ float softplusf(float x) {
 
 // Input features of sample m of the tile into row m (lanes cover 16
 // features each; rows ROW elements apart).  valid = false writes a zero row.
+// Feature 10a + 2l (+ 1) is sin (cos) of 2^l pi p_a, p_a the position's
+// offset along axis a over the box extent, l < 5; 32-34 the normal, 35-37
+// the outgoing direction, the rest zero.  A lane of chunk 0 holds features
+// 0-15 (axes 0 and 1), of chunk 1 features 16-31 (axes 1 and 2): it takes
+// sincos of pi p for its two axes once and the higher levels by angle
+// doubling (sin 2x = 2 sin x cos x, cos 2x = (cos x - sin x)(cos x + sin x)),
+// within a few single-precision ulps of the direct values, far below the
+// bf16 rounding of the features.
 template <int ROW = kRow>
 __device__ __forceinline__ void featureRow(__bf16* H, int m, int chunk, bool valid, const float* pos,
                            const float* nrm, const float* wo, const float* bmin,
@@ -32,25 +40,33 @@ __device__ __forceinline__ void featureRow(__bf16* H, int m, int chunk, bool val
     // chunk varies per lane: the three-element arrays are read with selects,
     // not indexed, so they stay in registers
     auto pick = [](const float* v, int i) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); };
+    float sn[2][5], cs[2][5];  // the lane's axes chunk and chunk + 1 (chunks 2, 3: unused)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int a = min(chunk + q, 2);
+        const float ea = pick(bext, a);
+        const float e = ea > 0.f ? ea : 1.f;
+        const float p = (pick(pos, a) - pick(bmin, a)) / e;
+        sincosf(3.14159265358979323846f * p, &sn[q][0], &cs[q][0]);
+#pragma unroll
+        for (int l = 1; l < 5; ++l) {
+            sn[q][l] = 2.f * sn[q][l - 1] * cs[q][l - 1];
+            cs[q][l] = (cs[q][l - 1] - sn[q][l - 1]) * (cs[q][l - 1] + sn[q][l - 1]);
+        }
+    }
+    // feature f (< 30) of a lane whose first axis is a0: axis f / 10
+    auto freq = [&](int f, int a0) {
+        const int q = f / 10 - a0, l = (f % 10) >> 1;
+        return (f & 1) ? cs[q][l] : sn[q][l];
+    };
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        int f = chunk * 16 + j;
-        float v = 0.f;
-        if (valid) {
-            if (f < 30) {
-                int a = f / 10, r = f % 10, lvl = r >> 1;
-                float ea = pick(bext, a);
-                float e = ea > 0.f ? ea : 1.f;
-                float p = (pick(pos, a) - pick(bmin, a)) / e;
-                float arg = float(1 << lvl) * 3.14159265358979323846f * p;
-                v = (r & 1) ? cosf(arg) : sinf(arg);
-            } else if (f >= 32 && f < 35) {
-                v = pick(nrm, f - 32);
-            } else if (f >= 35 && f < 38) {
-                v = pick(wo, f - 35);
-            }
-        }
-        H[m * ROW + f] = tobf(v);
+        // the candidates of chunks 0 .. 3 for this j (indices known at compile time)
+        const float v0 = freq(j, 0);
+        const float v1 = 16 + j < 30 ? freq(16 + j, 1) : 0.f;
+        const float v2 = j < 3 ? pick(nrm, j) : (j < 6 ? pick(wo, j - 3) : 0.f);
+        float v = chunk == 0 ? v0 : (chunk == 1 ? v1 : (chunk == 2 ? v2 : 0.f));
+        H[m * ROW + chunk * 16 + j] = tobf(valid ? v : 0.f);
     }
 }
 
