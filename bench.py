@@ -486,7 +486,8 @@ def guided_leg(cr, scenes, seconds: float, repeats: int, size: int = 1024) -> di
                           train_batch=4096, train_steps=4, materials=1, target=0)
     for k in range(4):
         gi.dispatch(scene, 4, first_sample=(1 << 26) + 4 * k, train=True, readback=False)
-    gtuned = autotune.tune_guided(gi, scene, spp=4 if size > 256 else 16)
+    gtuned = autotune.tune_guided(gi, scene, spp=4 if size > 256 else 16, waves=(6, 8),
+                                  grids=(2048, 4096, 8192), pools=(None, 1 << 24))
     del gi
     sensor.clear()
     chunk = 16 if size > 256 else 64

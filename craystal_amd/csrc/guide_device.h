@@ -26,9 +26,7 @@ CRH_D Mixture loadMixture(const float* p) {
     return m;
 }
 
-CRH_D float vmfDensity(float k, float c) {
-    return k / (2.f * kPi * (-expm1f(-2.f * k))) * expf(k * (c - 1.f));
-}
+CRH_D float vmfDensity(float k, float c) { return vmfPdf(k, c); }
 
 CRH_D float mixturePdf(const Mixture& m, F3 d) {
     float q = 0.f;
@@ -81,8 +79,8 @@ CRH_D RawMixture rawMixture(const float* o) {
     return RawMixture{o, amax, headExpSum(o, amax)};
 }
 CRH_D F3 rawMean(const RawMixture& m, int j) {
-    const float len = headMeanLength(m.o, j);
-    return mk3(m.o[5 * j + 1] / len, m.o[5 * j + 2] / len, m.o[5 * j + 3] / len);
+    const float il = gRcp(headMeanLength(m.o, j));
+    return mk3(m.o[5 * j + 1] * il, m.o[5 * j + 2] * il, m.o[5 * j + 3] * il);
 }
 CRH_D float mixturePdf(const RawMixture& m, F3 d) {
     float q = 0.f;

@@ -19,8 +19,34 @@ constexpr int kTile = GNet<64>::kTile;  // elements per activation tile
 __device__ __forceinline__ __bf16 tobf(float v) { return static_cast<__bf16>(v); }
 __device__ __forceinline__ float frombf(__bf16 v) { return static_cast<float>(v); }
 
+// The head's single-precision building blocks on the hardware's
+// transcendental and reciprocal instructions (v_exp_f32, v_log_f32,
+// v_rcp_f32, v_sqrt_f32, about 1 ulp each): a head value is within a few
+// 1e-7 (relative) of the correctly rounded one, far below what the bf16
+// network resolves, at a fraction of the library functions' cost.
+__device__ __forceinline__ float gExp(float x) {
+    return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+}
+__device__ __forceinline__ float gLog(float x) {
+    return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+}
+__device__ __forceinline__ float gRcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float gSqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// log(1 + e), e >= 0, and exp(x) - 1, x <= 0, with Kahan's correction for
+// the rounding of 1 + e (of exp(x)).
+__device__ __forceinline__ float gLog1p(float e) {
+    const float y = 1.f + e;
+    return y == 1.f ? e : gLog(y) * (e * gRcp(y - 1.f));
+}
+__device__ __forceinline__ float gExpm1(float x) {
+    const float u = gExp(x);
+    if (u == 1.f) return x;
+    const float y = u - 1.f;
+    return y == -1.f ? -1.f : y * (x * gRcp(gLog(u)));
+}
+
 __device__ __forceinline__ float softplusf(float x) {
-    return x > 20.f ? x : log1pf(expf(fminf(x, 20.f)));
+    return x > 20.f ? x : gLog1p(gExp(fminf(x, 20.f)));
 }
 
 // Input features of sample m of the tile into row m (lanes cover 16
@@ -110,7 +136,8 @@ struct Head {
 
 // The head lobe by lobe (every user computes a lobe's values with these):
 // softmax weight (largest logit subtracted, the exponentials summed in lobe
-// order), unit mean (length floored at 1e-6) and kappa = softplus + 0.01.
+// order), unit mean (length floored at 1e-6, the components scaled by its
+// reciprocal) and kappa = softplus + 0.01.
 __device__ __forceinline__ float headLogitMax(const float* o) {
     float amax = -INFINITY;
 #pragma unroll
@@ -120,15 +147,15 @@ __device__ __forceinline__ float headLogitMax(const float* o) {
 __device__ __forceinline__ float headExpSum(const float* o, float amax) {
     float sum = 0.f;
 #pragma unroll
-    for (int j = 0; j < kGLobes; ++j) sum += expf(o[5 * j] - amax);
+    for (int j = 0; j < kGLobes; ++j) sum += gExp(o[5 * j] - amax);
     return sum;
 }
 __device__ __forceinline__ float headWeight(const float* o, int j, float amax, float sum) {
-    return expf(o[5 * j] - amax) / sum;
+    return gExp(o[5 * j] - amax) * gRcp(sum);
 }
 __device__ __forceinline__ float headMeanLength(const float* o, int j) {
     const float mx = o[5 * j + 1], my = o[5 * j + 2], mz = o[5 * j + 3];
-    return fmaxf(sqrtf(mx * mx + my * my + mz * mz), 1e-6f);
+    return fmaxf(gSqrt(mx * mx + my * my + mz * mz), 1e-6f);
 }
 __device__ __forceinline__ float headKappa(const float* o, int j) {
     return softplusf(o[5 * j + 4]) + kGKappaMin;
@@ -139,11 +166,11 @@ __device__ __forceinline__ void computeHead(const float* o, Head& h) {
 #pragma unroll
     for (int j = 0; j < kGLobes; ++j) {
         h.w[j] = headWeight(o, j, amax, sum);
-        const float len = headMeanLength(o, j);
+        const float len = headMeanLength(o, j), il = gRcp(len);
         h.mlen[j] = len;
-        h.mu[j][0] = o[5 * j + 1] / len;
-        h.mu[j][1] = o[5 * j + 2] / len;
-        h.mu[j][2] = o[5 * j + 3] / len;
+        h.mu[j][0] = o[5 * j + 1] * il;
+        h.mu[j][1] = o[5 * j + 2] * il;
+        h.mu[j][2] = o[5 * j + 3] * il;
         h.kraw[j] = o[5 * j + 4];
         h.k[j] = headKappa(o, j);
     }
@@ -151,7 +178,7 @@ __device__ __forceinline__ void computeHead(const float* o, Head& h) {
 
 __device__ __forceinline__ float vmfPdf(float k, float c) {
     // k / (2 pi (1 - e^{-2k})) * exp(k (c - 1))
-    return k / (2.f * 3.14159265358979323846f * (-expm1f(-2.f * k))) * expf(k * (c - 1.f));
+    return k * gRcp(2.f * 3.14159265358979323846f * (-gExpm1(-2.f * k))) * gExp(k * (c - 1.f));
 }
 
 }  // namespace crh
