@@ -125,8 +125,10 @@ CRH_D RayPre prep(const Ray& r) {
     p.o = r.o;
     p.d = r.d;
     p.inv = {1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
-    p.tMin = r.tMin;
-    p.tMax = r.tMax;
+    // Canonical (quiet) once here, so that the fmaxf / fminf of every box
+    // test need not quiet them again; values and comparisons are unchanged.
+    p.tMin = __builtin_canonicalizef(r.tMin);
+    p.tMax = __builtin_canonicalizef(r.tMax);
     return p;
 }
 
@@ -173,7 +175,11 @@ CRH_D bool boxHit(const float* bmin, const float* bmax, const RayPre& r,
 // `t0 > tMin ? t0 : tMin` selects: while the accumulator is not NaN both
 // give the other operand for a NaN t0 / t1 (0 * inf on a slab plane) and the
 // larger / smaller one otherwise; they can differ only in the sign of a zero
-// tie, which none of the comparisons the traversal makes can see.
+// tie, which none of the comparisons the traversal makes can see.  The
+// reference's final range test (tMin > ray tMax or tMax < ray tMin) cannot
+// reject here: tMin starts at the ray's tMin and only grows, tMax starts at
+// its tMax and only shrinks, none of them NaN, so tMin <= tMax implies
+// ray tMin <= tMin <= tMax <= ray tMax.
 CRH_D bool boxHitFast(const float* bmin, const float* bmax, const RayPre& r, float& hitT) {
     const float o[3] = {r.o.x, r.o.y, r.o.z};
     const float inv[3] = {r.inv.x, r.inv.y, r.inv.z};
@@ -187,7 +193,6 @@ CRH_D bool boxHitFast(const float* bmin, const float* bmax, const RayPre& r, flo
         tMax = fminf(tMax, sw ? t0 : t1);
     }
     if (tMin > tMax) return false;
-    if (tMin > r.tMax || tMax < r.tMin) return false;
     hitT = tMin;
     return true;
 }

@@ -32,13 +32,25 @@ struct WorkCount {
     uint32_t nodes, tris;
 };
 
-// Traversal stack in LDS: entry k of a thread at base[k * stride], stride =
-// threads per block, so a wave's lanes touch consecutive 8-byte slots.
+// Stack entries in LDS: entry k of a thread at slot k * (threads per block)
+// + thread of the kernel's dynamic LDS (which starts with the stack), so a
+// wave's lanes touch consecutive 8-byte slots.  The slots are addressed
+// through an LDS (address space 3) pointer computed from the thread and
+// block size at each access: the accesses compile to ds_read / ds_write,
+// and no stack object holds a base or stride that would have to live (and
+// be reloaded) in private memory next to a private array.
+typedef __attribute__((address_space(3))) unsigned long long LdsU64;
+extern __shared__ uint2 crhStackLds[];
+CRH_D LdsU64* ldsEntry(int k) {
+    return (LdsU64*)crhStackLds + (threadIdx.x + uint32_t(k) * blockDim.x);
+}
+CRH_D void ldsPut(int k, uint2 v) { *ldsEntry(k) = __builtin_bit_cast(unsigned long long, v); }
+CRH_D uint2 ldsGet(int k) { return __builtin_bit_cast(uint2, (unsigned long long)*ldsEntry(k)); }
+
+// Traversal stack in LDS.
 struct LdsStack {
-    uint2* base;
-    uint32_t stride;
-    CRH_D void put(int k, uint2 v) { base[k * stride] = v; }
-    CRH_D uint2 get(int k) const { return base[k * stride]; }
+    CRH_D void put(int k, uint2 v) { ldsPut(k, v); }
+    CRH_D uint2 get(int k) const { return ldsGet(k); }
 };
 
 // Traversal stack in private (scratch) memory: no LDS, so occupancy is set
@@ -57,14 +69,12 @@ struct PrivStack {
 // limit occupancy.
 constexpr int kShortStack = kShortStackLds;
 struct ShortStack {
-    uint2* base;
-    uint32_t stride;
     uint2 a[kPrivStack - kShortStack];
     CRH_D void put(int k, uint2 v) {
-        if (k < kShortStack) base[k * stride] = v;
+        if (k < kShortStack) ldsPut(k, v);
         else a[k - kShortStack] = v;
     }
-    CRH_D uint2 get(int k) const { return k < kShortStack ? base[k * stride] : a[k - kShortStack]; }
+    CRH_D uint2 get(int k) const { return k < kShortStack ? ldsGet(k) : a[k - kShortStack]; }
 };
 
 // SphereSOA::intersect (Scene/Sphere.cu:6-40) with intersectShape's range
@@ -418,18 +428,13 @@ struct StackOf<2> {
     using T = ShortStack;
 };
 
+// The stack of the calling thread; `lds` is the kernel's dynamic LDS, whose
+// start the LDS stacks occupy (crhStackLds is the same memory).
 template <int SM>
 CRH_D typename StackOf<SM>::T makeStack(uint2* lds) {
-    if constexpr (SM == 1) {
-        return PrivStack{};
-    } else if constexpr (SM == 2) {
-        ShortStack s;
-        s.base = lds + threadIdx.x;
-        s.stride = blockDim.x;
-        return s;
-    } else {
-        return LdsStack{lds + threadIdx.x, blockDim.x};
-    }
+    (void)lds;
+    typename StackOf<SM>::T s;  // default-initialised: the private entries are not cleared
+    return s;
 }
 
 }  // namespace crh
