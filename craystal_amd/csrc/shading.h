@@ -378,6 +378,14 @@ CRH_D float smithG(float r, float ci, float cv) {
     a *= a;
     return smithG1(a, ci) * smithG1(a, cv);
 }
+// The reference's pow(1.0 - c, 5.0) in double (PrincipledBRDF.cu:138-139)
+// as ((x x)^2) x: within 2 double ulps of pow's value, far below the float
+// rounding the result goes through next, at a few multiplications instead
+// of a double-precision pow.
+CRH_D double pow5d(double x) {
+    const double x2 = x * x;
+    return x2 * x2 * x;
+}
 CRH_D Spec prEval(const BSDF& b, F3 wo, F3 wi) {
     if (wo.z <= 0.0f || wi.z <= 0.0f) return sp(0.f);
     F3 wh = normalize(wo + wi);
@@ -385,8 +393,8 @@ CRH_D Spec prEval(const BSDF& b, F3 wo, F3 wi) {
     float cd = dot(wo, wh);
     // evaluateDiffuse (:135-141), double subexpressions as in the reference
     float fd90 = float(0.5 + 2.0 * double(b.rough) * double(cd) * double(cd));
-    float fl = float(1.0 + (double(fd90) - 1.0) * pow(1.0 - double(ci), 5.0));
-    float fv = float(1.0 + (double(fd90) - 1.0) * pow(1.0 - double(cv), 5.0));
+    float fl = float(1.0 + (double(fd90) - 1.0) * pow5d(1.0 - double(ci)));
+    float fv = float(1.0 + (double(fd90) - 1.0) * pow5d(1.0 - double(cv)));
     Spec diff = b.kd * kInvPi * fv * fl * (1.0f - b.metal);
     // evaluateSpecular (:143-150)
     float D = ggxD(b.rough, ch);
