@@ -3,12 +3,12 @@
 // live lane runs one bounce of its path: the closest hit, the emitter or
 // environment term and the light sample with MIS, as pathBounce does.  Then
 // the wave evaluates the guide for its lanes whose vertex is guided, on
-// v_mfma_f32_16x16x32_bf16: the 64 lanes' queries are four 16-row tiles,
-// each row's features built in the wave's LDS tile, five layers computed in
-// place (A operands loaded before the tile is overwritten), the weights
-// read in MFMA operand order from L2.  Each lane keeps its row's 40 outputs
-// in its own slot of a per-wave scratch buffer, computes the mixture head,
-// and makes the continuation: one-sample MIS of BSDF sampling and the
+// v_mfma_f32_16x16x32_bf16: the guided lanes' queries are packed into as
+// few 16-row tiles as they fill, each row's features built in the wave's
+// LDS tile, five layers computed in place (A operands loaded before the
+// tile is overwritten), the weights read in MFMA operand order from L2.
+// Each guided lane finds its row's 40 outputs in a per-wave scratch buffer,
+// computes the mixture head, and makes the continuation: one-sample MIS of BSDF sampling and the
 // mixture (guidedSampleEval), or the plain BSDF sample at other vertices,
 // then the kill and roulette.  A row's outputs do not depend on the other
 // rows of its tile, and every step and draw is the wavefront guided path's
@@ -146,24 +146,44 @@ __device__ __forceinline__ void layersFrom(__bf16* H, __amdgpu_buffer_rsrc_t wq,
     if constexpr (L + 1 < kGLayers) layersFrom<WD, L + 1>(H, wq, theta, out, outOff);
 }
 
-// The guide for the wave's lanes with `guided` set: the 64 lanes' queries
-// are four 16-row tiles (a tile with no guided lane is skipped); lane i's
-// kGuideRaw outputs go to raw[i * kGuideRaw ..] (its own slot of the wave's
-// 64 x kGuideRaw floats).  Wave-uniform control flow.
+// Position of set bit k (0-based, in ascending order) of m, for k below
+// popcount(m): the largest p with fewer than k + 1 set bits below p.
+__device__ __forceinline__ int kthSetBit(uint64_t m, uint32_t k) {
+    int pos = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+        const uint64_t below = m & ((uint64_t(1) << (pos + step)) - 1u);
+        if (uint32_t(__popcll(below)) <= k) pos += step;
+    }
+    return pos;
+}
+
+// The guide for the wave's lanes with `guided` set, packed into as few
+// 16-row tiles as they fill: the guided lane of rank r (r-th guided lane
+// of the wave in lane order) is row r % 16 of tile r / 16 and gets its
+// kGuideRaw outputs at raw[r * kGuideRaw ..] (the wave's 64 x kGuideRaw
+// floats); returns this lane's rank.  A row's outputs do not depend on the
+// other rows of its tile, so the packing changes no value.  Wave-uniform
+// control flow.
 template <int WD>
-__device__ __forceinline__ void guideWave(const GuideView& gv, bool guided, F3 pos, F3 nrm,
-                                          F3 wo, unsigned char* tile, float* __restrict__ raw) {
+__device__ __forceinline__ uint32_t guideWave(const GuideView& gv, bool guided, F3 pos, F3 nrm,
+                                              F3 wo, unsigned char* tile,
+                                              float* __restrict__ raw) {
     const uint64_t gm = __ballot(guided);
-    if (gm == 0) return;
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(gm >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo(uint32_t(gm), 0u));
+    if (gm == 0) return rank;
     const int lane = threadIdx.x & 63, m = lane & 15, chunk = lane >> 4;
+    const int n = __popcll(gm);
+    const int srcOfRow = kthSetBit(gm, uint32_t(lane));  // the lane of rank `lane`
     __bf16* H = reinterpret_cast<__bf16*>(tile);
     const __amdgpu_buffer_rsrc_t wq = bufferOf(gv.wq, uint32_t(kGQElemsOf<WD>) * 2u);
     const __amdgpu_buffer_rsrc_t theta = bufferOf(gv.theta, uint32_t(kGParamsOf<WD>) * 4u);
     const __amdgpu_buffer_rsrc_t out = bufferOf(raw, 64u * kGuideRaw * 4u);
-    for (int t = 0; t < 4; ++t) {
-        if (((gm >> (16 * t)) & 0xFFFFull) == 0) continue;
-        const int src = 16 * t + m;
-        const bool valid = __shfl(guided ? 1 : 0, src, 64) != 0;
+    for (int t = 0; 16 * t < n; ++t) {
+        const int row = 16 * t + m;
+        const bool valid = row < n;
+        const int src = __shfl(srcOfRow, row, 64);
         const float pa[3] = {__shfl(pos.x, src, 64), __shfl(pos.y, src, 64), __shfl(pos.z, src, 64)};
         const float na[3] = {__shfl(nrm.x, src, 64), __shfl(nrm.y, src, 64), __shfl(nrm.z, src, 64)};
         const float wa[3] = {__shfl(wo.x, src, 64), __shfl(wo.y, src, 64), __shfl(wo.z, src, 64)};
@@ -172,6 +192,7 @@ __device__ __forceinline__ void guideWave(const GuideView& gv, bool guided, F3 p
         layersFrom<WD, 0>(H, wq, theta, out, uint32_t(16 * t) * kGuideRaw * 4u);
     }
     waveSync();  // the outputs, written by other lanes, are read next
+    return rank;
 }
 
 // MULTI: several camera views of the scene in one launch (GuidedViews):
@@ -286,7 +307,7 @@ kRenderGuided(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint
             }
             if (done) finish();
         }
-        guideWave<WD>(gv, guided, qPos, qNrm, qWo, tile, raw);
+        const uint32_t gRank = guideWave<WD>(gv, guided, qPos, qNrm, qWo, tile, raw);
         // The continuation (kWfShadeCont's, with the guide's mixture).
         if (cont) {
             Isect it = makeIsect(s, ray, h);
@@ -296,7 +317,7 @@ kRenderGuided(SceneView s, CameraView cam, IntegratorView cfg, uint32_t si, uint
             float pdf = 0.0f, qmix = 0.f;
             Spec f;
             if (guided) {
-                const RawMixture mix = rawMixture(raw + lane * kGuideRaw);
+                const RawMixture mix = rawMixture(raw + gRank * kGuideRaw);
                 f = guidedSampleEval(bsdf, mix, alpha, smp, it.view, wi, pdf, qmix);
             } else {
                 f = bsdfSampleEval(bsdf, smp, it.view, wi, pdf);
