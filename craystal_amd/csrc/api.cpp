@@ -1226,7 +1226,6 @@ int crh_render(crh_scene* s, const crh_camera* cam,
     } else {
         HIP_TRY(ensureFilmSlots(film, slots));
     }
-    const size_t npix = size_t(film->w) * film->h;
     // One event pair per pass group times the main launches on their stream;
     // the film keeps the events for later calls.
     std::vector<hipEvent_t>& evs = film->passEv;
@@ -1254,12 +1253,10 @@ int crh_render(crh_scene* s, const crh_camera* cam,
                                           cfg->count_work != 0, tun, ctx->stream,
                                           cfg->spectral != 0, ctx->counter));
         if (stats) HIP_TRY(hipEventRecord(evs[2 * g + 1], ctx->stream));
-        for (uint32_t k = 0; k < active; ++k) {
-            const float4* Ls = (wavefront ? film->wf.rad : film->L.as<float4>()) + k * npix;
-            const float2* Ps = (wavefront ? film->wf.P : film->P.as<float2>()) + k * npix;
-            HIP_TRY(crh::launchFilmGather(Ls, Ps, film->w, film->h, film->sum.as<float>(),
-                                          film->wsum.as<float>(), ctx->stream));
-        }
+        HIP_TRY(crh::launchFilmGather(wavefront ? film->wf.rad : film->L.as<float4>(),
+                                      wavefront ? film->wf.P : film->P.as<float2>(), film->w,
+                                      film->h, film->sum.as<float>(), film->wsum.as<float>(),
+                                      ctx->stream, active));
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev1));
@@ -1667,10 +1664,8 @@ int crh_render_guided(crh_scene* s, const crh_camera* cam, const crh_integrator_
         if (stats) HIP_TRY(hipEventRecord(evs[2 * pg + 1], ctx->stream));
         const float4* Lp = fused ? film->L.as<float4>() : film->wf.rad;
         const float2* Pp = fused ? film->P.as<float2>() : film->wf.P;
-        for (uint32_t k = 0; k < active; ++k)
-            HIP_TRY(crh::launchFilmGather(Lp + k * npix, Pp + k * npix, film->w, film->h,
-                                          film->sum.as<float>(), film->wsum.as<float>(),
-                                          ctx->stream));
+        HIP_TRY(crh::launchFilmGather(Lp, Pp, film->w, film->h, film->sum.as<float>(),
+                                      film->wsum.as<float>(), ctx->stream, active));
         if (gc->train) {
             const int rc = guidedTrainPass(g, film, gc, active, fused, recCap,
                                            film->L.as<float4>(), smpOut, smpRoom, lastSamples);

@@ -159,8 +159,10 @@ template <> hipError_t launchBatchSM<2>(const BatchArgs&);
 hipError_t launchBatchGatherPass(const BatchScene* tab, const uint32_t* pixPrefix,
                                  uint32_t nActive, uint32_t pixels, uint32_t slots,
                                  const float4* L, const float2* P, hipStream_t st);
+// Film update with `slots` consecutive sample slots (slot k at L + k * W * H),
+// in slot order.
 hipError_t launchFilmGather(const float4* L, const float2* P, uint32_t W,
                             uint32_t H, float* sum, float* wsum,
-                            hipStream_t st);
+                            hipStream_t st, uint32_t slots = 1);
 
 }  // namespace crh

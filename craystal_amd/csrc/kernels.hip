@@ -91,12 +91,8 @@ __global__ void kResetNext(uint32_t* __restrict__ next) {
 // which is the order a sequential SensorData::addSample loop over the pass
 // (Core/Sensor.cu:19-43) delivers them; samples with NaN/Inf are dropped and
 // zero weights skipped (they add exactly nothing).
-CRH_D void gatherPixel(const float4* __restrict__ L, const float2* __restrict__ P,
-                       uint32_t W, uint32_t H, float* __restrict__ sum,
-                       float* __restrict__ wsum, uint32_t i) {
-    int x = int(i % W), y = int(i / W);
-    float s0 = sum[3 * size_t(i)], s1 = sum[3 * size_t(i) + 1],
-          s2 = sum[3 * size_t(i) + 2], ws = wsum[i];
+CRH_D void gatherAdd(const float4* __restrict__ L, const float2* __restrict__ P, uint32_t W,
+                     uint32_t H, int x, int y, float& s0, float& s1, float& s2, float& ws) {
     for (int dy = -1; dy <= 1; ++dy) {
         int sy = y + dy;
         if (sy < 0 || sy >= int(H)) continue;
@@ -119,18 +115,36 @@ CRH_D void gatherPixel(const float4* __restrict__ L, const float2* __restrict__ 
             ws += w;
         }
     }
+}
+CRH_D void gatherPixel(const float4* __restrict__ L, const float2* __restrict__ P,
+                       uint32_t W, uint32_t H, float* __restrict__ sum,
+                       float* __restrict__ wsum, uint32_t i) {
+    float s0 = sum[3 * size_t(i)], s1 = sum[3 * size_t(i) + 1],
+          s2 = sum[3 * size_t(i) + 2], ws = wsum[i];
+    gatherAdd(L, P, W, H, int(i % W), int(i / W), s0, s1, s2, ws);
     sum[3 * size_t(i)] = s0;
     sum[3 * size_t(i) + 1] = s1;
     sum[3 * size_t(i) + 2] = s2;
     wsum[i] = ws;
 }
 
+// `slots` consecutive sample slots (slot k at L + k * W * H), one after the
+// other, with the pixel's sums kept in registers in between: the same
+// additions in the same order as one gather per slot.
 __global__ __launch_bounds__(256) void kFilmGather(
     const float4* __restrict__ L, const float2* __restrict__ P, uint32_t W,
-    uint32_t H, float* __restrict__ sum, float* __restrict__ wsum) {
-    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    uint32_t H, uint32_t slots, float* __restrict__ sum, float* __restrict__ wsum) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= W * H) return;
-    gatherPixel(L, P, W, H, sum, wsum, i);
+    const size_t npix = size_t(W) * H;
+    float s0 = sum[3 * size_t(i)], s1 = sum[3 * size_t(i) + 1],
+          s2 = sum[3 * size_t(i) + 2], ws = wsum[i];
+    for (uint32_t k = 0; k < slots; ++k)
+        gatherAdd(L + k * npix, P + k * npix, W, H, int(i % W), int(i / W), s0, s1, s2, ws);
+    sum[3 * size_t(i)] = s0;
+    sum[3 * size_t(i) + 1] = s1;
+    sum[3 * size_t(i) + 2] = s2;
+    wsum[i] = ws;
 }
 
 __global__ void kBatchReset(uint32_t* __restrict__ next) {
@@ -286,10 +300,11 @@ hipError_t launchBatchGatherPass(const BatchScene* tab, const uint32_t* pixPrefi
 
 hipError_t launchFilmGather(const float4* L, const float2* P, uint32_t W,
                             uint32_t H, float* sum, float* wsum,
-                            hipStream_t st) {
+                            hipStream_t st, uint32_t slots) {
     uint32_t n = W * H;
+    if (n == 0 || slots == 0) return hipSuccess;
     hipLaunchKernelGGL(kFilmGather, dim3((n + 255u) / 256u), dim3(256), 0, st,
-                       L, P, W, H, sum, wsum);
+                       L, P, W, H, slots, sum, wsum);
     return hipGetLastError();
 }
 
