@@ -53,6 +53,12 @@ def parse():
     ap.add_argument("--no-trace", action="store_true", help="skip the closest-hit batch")
     ap.add_argument("--guided-seconds", type=float, default=6.0,
                     help="render time of each run of the guided-vs-plain comparison (0 = skip)")
+    ap.add_argument("--guided-materials", type=int, choices=(1, 2, 3), default=1,
+                    help="guided leg: BSDFs guided (1 Lambertian, 2 Principled, 3 both)")
+    ap.add_argument("--guided-train-steps", type=int, default=32,
+                    help="guided leg: Adam steps per training pass")
+    ap.add_argument("--guided-train-batch", type=int, default=16384,
+                    help="guided leg: training samples per Adam step")
     ap.add_argument("--guided-runs", type=int, default=9,
                     help="independent runs per method in the guided-vs-plain comparison")
     ap.add_argument("--guided-size", type=int, choices=(256, 1024), default=1024,
@@ -446,7 +452,8 @@ def c1_leg(cr, L, scenes, with_cpu: bool, reps: int = 20) -> dict:
     return out
 
 
-def guided_leg(cr, scenes, seconds: float, repeats: int, size: int = 1024) -> dict:
+def guided_leg(cr, scenes, seconds: float, repeats: int, size: int = 1024,
+               materials: int = 1, train_steps: int = 32, train_batch: int = 16384) -> dict:
     """Guided vs plain on the occluded-light room (200 objects, 171,388
     triangles, max depth 16) at size x size against the committed
     65,536-spp plain image of the same size: equal-time relMSE and the time
@@ -459,9 +466,10 @@ def guided_leg(cr, scenes, seconds: float, repeats: int, size: int = 1024) -> di
     0.01 within its time, its time to 0.01 is extrapolated from relMSE ~ 1/t
     fitted on the second half of its curve.  Guided: the fused guided
     megakernel (guide inference on MFMA inside the path kernel), one-sample
-    MIS with alpha 0.5 at Lambertian vertices, the guide training during the
-    first 25 % of each run.  The plain megakernel's launch shape is tuned on
-    this scene first, so both methods run at their best."""
+    MIS with alpha 0.5 at the vertices whose BSDF `materials` selects (1:
+    Lambertian, 2: Principled, 3: both), the guide training during the first
+    25 % of each run.  The plain megakernel's launch shape is tuned on this
+    scene first, so both methods run at their best."""
     from craystal_amd import autotune
     from craystal_amd import experiments as X
     ref_path = ROOT / "craystal_amd" / "data" / f"room{size}_plain65536.npy"
@@ -483,7 +491,7 @@ def guided_leg(cr, scenes, seconds: float, repeats: int, size: int = 1024) -> di
     from craystal_amd.guiding import Guide, GuidedIntegrator
     P = data.concatenated()[0]
     gi = GuidedIntegrator(cfg, Guide(P.min(0), P.max(0), seed=5), alpha=0.5, lr=1e-2,
-                          train_batch=4096, train_steps=4, materials=1, target=0)
+                          train_batch=4096, train_steps=4, materials=materials, target=0)
     for k in range(4):
         gi.dispatch(scene, 4, first_sample=(1 << 26) + 4 * k, train=True, readback=False)
     gtuned = autotune.tune_guided(gi, scene, spp=4 if size > 256 else 16, waves=(6, 8),
@@ -497,8 +505,9 @@ def guided_leg(cr, scenes, seconds: float, repeats: int, size: int = 1024) -> di
         pc = X.plain_curve(scene, sensor, cfg, ref, seconds, chunk, first=first)
         runs["plain"].append(X.summarize(pc, 0.01, px))
         gc, info = X.guided_curve(scene, sensor, data, cfg, ref, seconds, chunk, first=first,
-                                  seed=11 + r, materials=1, target=0, train_batch=4096,
-                                  train_steps=4, lr=1e-2, alpha=0.5)
+                                  seed=11 + r, materials=materials, target=0,
+                                  train_batch=train_batch, train_steps=train_steps, lr=1e-2,
+                                  alpha=0.5)
         runs["guided"].append({**X.summarize(gc, 0.01, px), **info})
         progress(f"guided leg run {r}: relMSE plain {runs['plain'][-1]['final_relmse']:.4f} "
                  f"guided {runs['guided'][-1]['final_relmse']:.4f}")
@@ -507,6 +516,8 @@ def guided_leg(cr, scenes, seconds: float, repeats: int, size: int = 1024) -> di
            "plain_tuning": {k: tuned[k] for k in ("render_stack", "render_waves", "render_regen",
                                                         "render_grid")},
            "guided_kernel": "fused guided megakernel (kRenderGuided)",
+           "guided_materials": {1: "Lambertian", 2: "Principled", 3: "Lambertian and Principled"}[materials],
+           "train_steps_per_pass": train_steps, "train_batch": train_batch,
            "guided_tuning": {k: gtuned[k] for k in ("guided_stack", "guided_waves",
                                                      "guided_grid", "guided_pool")},
            "seconds_per_run": seconds, "runs": repeats,
@@ -523,6 +534,8 @@ def guided_leg(cr, scenes, seconds: float, repeats: int, size: int = 1024) -> di
                   "relmse_trimmed_at_end": [round(x["final_relmse_trimmed"], 5) for x in v],
                   "median_relmse_trimmed_at_end": float(np.median(
                       [x["final_relmse_trimmed"] for x in v]))}
+    out["guided"]["adam_steps_per_run"] = [x["train_steps"] for x in runs["guided"]]
+    out["guided"]["train_seconds_per_run"] = [x["train_seconds"] for x in runs["guided"]]
     out["speedup_median_time_to_target"] = round(
         out["plain"]["median_extrapolated_seconds"] / out["guided"]["median_extrapolated_seconds"],
         3)
@@ -831,7 +844,9 @@ def main():
         progress(f"batch: {out['batch']['msamples_per_s']} Msamples/s")
     if a.guided_seconds > 0 and a.guided_runs > 0:
         ctx.set_tuning(**main_tuning)
-        out["guided"] = guided_leg(cr, scenes, a.guided_seconds, a.guided_runs, a.guided_size)
+        out["guided"] = guided_leg(cr, scenes, a.guided_seconds, a.guided_runs, a.guided_size,
+                                   a.guided_materials, a.guided_train_steps,
+                                   a.guided_train_batch)
     if not a.no_terrain:
         # the four views render as a batch: at the batch leg's tuned shape
         ctx.set_tuning(**main_tuning)
