@@ -5,6 +5,8 @@
 //   sampleLight (Core/Integrator.cu:21-72)  = lightGeom -> [shadow query] -> lightFinish
 //   evalMIS     (Core/Integrator.cu:74-140) = misLightPart -> misBsdfRay ->
 //                                              [closest query] -> misBsdfPart
+//                (the query skipped where mayHitEmitter shows it cannot
+//                 reach the sampled emitter: misBsdfPart would add nothing)
 //
 // Random numbers are drawn only inside lightGeom (3: select + barycentric),
 // misBsdfRay (2) and the continuation sample, in the reference's order.
@@ -107,6 +109,28 @@ CRH_D bool misBsdfRay(const BSDF& bsdf, Sampler& smp, const Isect& it, Ray& r,
     r.d = dir;
     r.tMax = __builtin_inff();
     return true;
+}
+
+// Whether the MIS BSDF ray r can have the sampled emitter triangle eid as its
+// closest hit.  The traversal records a triangle only where the
+// Moller-Trumbore test on it (triHit, on the same vertex differences the
+// BVH's triangle records hold) gives a t inside the ray's range and below
+// the closest hit so far (at most +inf), so where that test fails on
+// triangle eid the closest hit is another primitive or none, the emitter
+// half of evalMIS adds nothing, and the ray need not be traced.  With
+// analytic spheres in the scene a sphere whose index equals eid matches too
+// (hitsLight), so there every ray is traced.
+CRH_D bool mayHitEmitter(const SceneView& s, const Ray& r, uint32_t eid) {
+    if (s.nSphere) return true;
+    const TriData tri = getTri(s, eid);
+    RayPre p;
+    p.o = r.o;
+    p.d = r.d;
+    p.tMin = r.tMin;
+    p.tMax = r.tMax;
+    float u, v, t;
+    return triHit(tri.p[0], tri.p[1] - tri.p[0], tri.p[2] - tri.p[0], p, u, v, t) &&
+           !(t < r.tMin || t > r.tMax) && t < __builtin_inff();
 }
 
 // Emitter half of evalMIS (:100-135) given the MIS ray's closest-hit query

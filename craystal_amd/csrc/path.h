@@ -47,7 +47,7 @@ CRH_D Spec evalMIS(PathCtx<COUNT, Stack>& c, const Isect& it, const BSDF& bsdf,
     Ray r;
     Spec bw;
     float bp;
-    if (misBsdfRay(bsdf, smp, it, r, bw, bp)) {
+    if (misBsdfRay(bsdf, smp, it, r, bw, bp) && mayHitEmitter(c.s, r, ls.id)) {
         Hit h;
         bool isHit = closest(c, r, h);
         misBsdfPart(c.s, it.pos, r, isHit, h, ls.id, bw, bp, value, w);

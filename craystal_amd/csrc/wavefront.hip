@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) void kWfShadeCont(Scenes S, WfState w,
                 Ray mr;
                 Spec bw;
                 float bp;
-                if (misBsdfRay(bsdf, smp, it, mr, bw, bp)) {
+                if (misBsdfRay(bsdf, smp, it, mr, bw, bp) && mayHitEmitter(s, mr, ls.id)) {
                     misRay = true;
                     storeRay(w.misO, w.misD, p, mr);
                     stS(w.misVal, p, value, ls.selPdf);

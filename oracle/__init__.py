@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
             "orc_bsdf_sample_local": (None, [P, P, P, P, P, P]),
             "orc_film_splat": (None, [P, P, u32, u32, P, f32, f32]),
             "orc_render_samples": (u64, [P, P, u32, u32, P, u32, P, P, C.c_int]),
+            "orc_render_samples_counted": (u64, [P, P, u32, u32, P, u32, P, P, C.c_int, P]),
             "orc_render": (u64, [P, P, u32, u32, P, u32, u32, P, P, C.c_int]),
             "orc_max_threads": (C.c_int, []),
             "orc_render_guided": (u64, [P, P, u32, u32, P, P, f32, u32, u32, C.c_int, u32, u32,
@@ -136,11 +137,20 @@ class Scene:
         lib().orc_trace_brute(self.h, _p(rays), len(rays), _p(hits))
         return hits
 
-    def render_samples(self, cam, size, cfg, sample_index, threads=0):
+    def render_samples(self, cam, size, cfg, sample_index, threads=0, counts=False):
+        """One sample index of every pixel: (radiance, film positions, rays
+        traced); counts=True adds the MIS BSDF rays among them that cannot hit
+        their sampled emitter triangle (a renderer may skip those)."""
         W, H = size
         Lr = np.zeros((H, W, 3), np.float32)
         P = np.zeros((H, W, 2), np.float32)
         c = cfg.to_c()
+        if counts:
+            un = C.c_uint64(0)
+            rays = lib().orc_render_samples_counted(self.h, _ref(cam.getData()), W, H, _ref(c),
+                                                    sample_index, _p(Lr), _p(P), threads,
+                                                    C.byref(un))
+            return Lr, P, rays, un.value
         rays = lib().orc_render_samples(self.h, _ref(cam.getData()), W, H, _ref(c),
                                         sample_index, _p(Lr), _p(P), threads)
         return Lr, P, rays

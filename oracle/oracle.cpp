@@ -1178,6 +1178,11 @@ struct LightSample {
 
 struct Tally {
     uint64_t rays = 0;
+    // MIS BSDF rays that cannot have the sampled emitter triangle as their
+    // closest hit (its Moller-Trumbore test fails, no spheres): a renderer
+    // may skip tracing them, since evalMIS adds nothing for them.  Counted
+    // only; every ray is still traced here.
+    uint64_t misUnreachable = 0;
 };
 
 inline const crh_material& matOf(const Scene& s, uint32_t prim) {
@@ -1274,6 +1279,12 @@ Spec evalMIS(const Scene& s, const Isect& it, const BSDF& bsdf, Sampler& smp,
             RayHit rh;
             rh.ray.o = offsetRayOrigin(it.posW, it.orientedFaceNormal());
             rh.ray.d = dir;
+            if (s.sphC.empty()) {
+                float eu = 0, ev = 0, et = kFltInf;
+                if (!(triIntersect(s, ls.emissiveID, rh.ray, eu, ev, et) &&
+                      !(et < rh.ray.tMin || et > rh.ray.tMax) && et < kFltInf))
+                    tl.misUnreachable++;
+            }
             // :108 compares primitive indices only: with the reference's
             // sphere materials a sphere of the light's index matches too.
             if (intersect(s, rh, tl) && rh.prim == ls.emissiveID &&
@@ -1702,19 +1713,40 @@ static uint64_t renderSamples(void* sp, const crh_camera* cam, uint32_t W, uint3
                               const crh_integrator_cfg* cfg, uint32_t si, float* L, float* pix,
                               int nthreads, const Guide* guide);
 
+static uint64_t renderSamplesCounted(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
+                                     const crh_integrator_cfg* cfg, uint32_t si, float* L,
+                                     float* pix, int nthreads, const Guide* guide,
+                                     uint64_t* misUnreachable);
+
 uint64_t orc_render_samples(void* sp, const crh_camera* cam, uint32_t W,
                             uint32_t H, const crh_integrator_cfg* cfg,
                             uint32_t si, float* L, float* pix, int nthreads) {
     return renderSamples(sp, cam, W, H, cfg, si, L, pix, nthreads, nullptr);
 }
 
+// As orc_render_samples; also returns in *misUnreachable the MIS BSDF rays
+// whose sampled emitter triangle they cannot hit (see Tally).
+uint64_t orc_render_samples_counted(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
+                                    const crh_integrator_cfg* cfg, uint32_t si, float* L,
+                                    float* pix, int nthreads, uint64_t* misUnreachable) {
+    return renderSamplesCounted(sp, cam, W, H, cfg, si, L, pix, nthreads, nullptr,
+                                misUnreachable);
+}
+
 static uint64_t renderSamples(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
                               const crh_integrator_cfg* cfg, uint32_t si, float* L, float* pix,
                               int nthreads, const Guide* guide) {
+    return renderSamplesCounted(sp, cam, W, H, cfg, si, L, pix, nthreads, guide, nullptr);
+}
+
+static uint64_t renderSamplesCounted(void* sp, const crh_camera* cam, uint32_t W, uint32_t H,
+                                     const crh_integrator_cfg* cfg, uint32_t si, float* L,
+                                     float* pix, int nthreads, const Guide* guide,
+                                     uint64_t* misUnreachable) {
     const Scene& s = *(Scene*)sp;
     const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();  // this call only
-    uint64_t rays = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : rays) num_threads(nt)
+    uint64_t rays = 0, unreach = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : rays, unreach) num_threads(nt)
     for (int64_t y = 0; y < int64_t(H); ++y) {
         Tally tl;
         for (uint32_t x = 0; x < W; ++x) {
@@ -1728,7 +1760,9 @@ static uint64_t renderSamples(void* sp, const crh_camera* cam, uint32_t W, uint3
             }
         }
         rays += tl.rays;
+        unreach += tl.misUnreachable;
     }
+    if (misUnreachable) *misUnreachable = unreach;
     return rays;
 }
 
