@@ -11,8 +11,8 @@ What fixes the answers independently of the code under test:
   oracle/guiding.py (pinned by quadrature and histograms in
   tests/test_guiding_oracle.py);
 * a proposal equal to the BSDF's own procedure gives, for every direction,
-  the plain estimator's weight f / p_rep (the survey's "guide == BSDF reduces
-  to the plain path");
+  the plain estimator's weight f / p_rep (with the guide equal to the BSDF,
+  the guided estimator reduces to the plain path);
 * guided images: the diffuse plane under a constant environment equals the
   closed form 0.95 rho E, and on a Principled scene the guided mean equals
   the plain oracle's within 3 sigma.
