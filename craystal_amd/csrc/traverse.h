@@ -138,6 +138,22 @@ CRH_D void loadNode(const GNode* nodes, uint32_t i, float4& a, float4& b,
 
 // Returns true if any primitive was hit inside [tMin, tMax] (the reference's
 // return value).  ANY: stop at the first such primitive.
+// Both child boxes of a node.  One branch per node: where either box is flat
+// (or the ray's range is NaN) both take the reference's slab test boxHit,
+// which for a box without flat axes gives boxHitFast's result and t bit for
+// bit, so a node's outcome is the same whichever branch it takes.
+CRH_D void nodeBoxes(uint32_t fl, const float* lmin, const float* lmax, const float* rmin,
+                     const float* rmax, const RayPre& r, float& lt, float& rt, bool& hl,
+                     bool& hr) {
+    if (fl & 3u) {
+        hl = boxHit(lmin, lmax, r, lt);
+        hr = boxHit(rmin, rmax, r, rt);
+    } else {
+        hl = boxHitFast(lmin, lmax, r, lt);
+        hr = boxHitFast(rmin, rmax, r, rt);
+    }
+}
+
 template <bool ANY, bool COUNT, class Stack>
 CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, Stack& stack,
                     WorkCount& wc) {
@@ -198,9 +214,8 @@ CRH_D bool traverse(const SceneView& s, const Ray& ray, Hit& hit, Stack& stack,
             const float rmin[3] = {b.z, b.w, c.x};
             const float rmax[3] = {c.y, c.z, c.w};
             float lt = __builtin_inff(), rt = __builtin_inff();
-            const uint32_t fl = slowBoxes(d.z, r);
-            bool hl = (fl & 1u) ? boxHit(lmin, lmax, r, lt) : boxHitFast(lmin, lmax, r, lt);
-            bool hr = (fl & 2u) ? boxHit(rmin, rmax, r, rt) : boxHitFast(rmin, rmax, r, rt);
+            bool hl, hr;
+            nodeBoxes(slowBoxes(d.z, r), lmin, lmax, rmin, rmax, r, lt, rt, hl, hr);
             if (hl && hr) {
                 // Reference pushes the far child first, then the near one
                 // (Core/BVH.cu:60-72); near = right iff lt > rt.
@@ -324,9 +339,8 @@ CRH_D bool travStep(const SceneView& s, TravState& t, Stack& stack, WorkCount& w
         const float rmin[3] = {b.z, b.w, c.x};
         const float rmax[3] = {c.y, c.z, c.w};
         float lt = __builtin_inff(), rt = __builtin_inff();
-        const uint32_t fl = slowBoxes(d.z, r);
-        bool hl = (fl & 1u) ? boxHit(lmin, lmax, r, lt) : boxHitFast(lmin, lmax, r, lt);
-        bool hr = (fl & 2u) ? boxHit(rmin, rmax, r, rt) : boxHitFast(rmin, rmax, r, rt);
+        bool hl, hr;
+        nodeBoxes(slowBoxes(d.z, r), lmin, lmax, rmin, rmax, r, lt, rt, hl, hr);
         if (hl && hr) {
             bool rightNear = lt > rt;
             uint32_t nearRef = rightNear ? d.y : d.x;
