@@ -339,8 +339,11 @@ CRH_D bool travStep(const SceneView& s, TravState& t, Stack& stack, WorkCount& w
         const float rmin[3] = {b.z, b.w, c.x};
         const float rmax[3] = {c.y, c.z, c.w};
         float lt = __builtin_inff(), rt = __builtin_inff();
-        bool hl, hr;
-        nodeBoxes(slowBoxes(d.z, r), lmin, lmax, rmin, rmax, r, lt, rt, hl, hr);
+        // Per box here: one branch per node measured 5 % slower in the
+        // persistent trace kernel (it is faster in traverse()).
+        const uint32_t fl = slowBoxes(d.z, r);
+        bool hl = (fl & 1u) ? boxHit(lmin, lmax, r, lt) : boxHitFast(lmin, lmax, r, lt);
+        bool hr = (fl & 2u) ? boxHit(rmin, rmax, r, rt) : boxHitFast(rmin, rmax, r, rt);
         if (hl && hr) {
             bool rightNear = lt > rt;
             uint32_t nearRef = rightNear ? d.y : d.x;
